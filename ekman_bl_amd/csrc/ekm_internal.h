@@ -1,0 +1,127 @@
+// ekm_internal.h -- shared types of the gfx950 duct-step library.
+//
+// Device layout (the reference's NP=1 layout, src/modules.F90:265-279):
+//   physical field U(0:NX+1, 0:NZ+1, 0:NY+1) double, x fastest;
+//   Fourier field CU(0:NX/2, 0:NZ+1, 0:NY+1) double2, kx fastest, in place.
+//   metrics (0:NZ+2, 0:NY+2, face 1:2), K fastest (src/ALLOCATION.F90:166-170).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/ekman_bl_amd.h"
+
+#define EKM_CHECK(call)                                  \
+  do {                                                   \
+    hipError_t e__ = (call);                             \
+    if (e__ != hipSuccess) {                             \
+      ekm_note_hip_error(e__, #call, __FILE__, __LINE__); \
+      return EKM_ERR_HIP;                                \
+    }                                                    \
+  } while (0)
+
+void ekm_note_hip_error(hipError_t e, const char* what, const char* file, int line);
+
+struct Dims {
+  int NX, NY, NZ;       // grid_def sizes
+  int NXC, NKX;         // NX/2+1 complex modes, NX/3 kept by the 2/3 rule
+  int NX2, NY2, NZ2;    // padded extents
+  int NZ3, NY3;         // metric extents
+  size_t npts;          // doubles per 3D field
+  size_t plane;         // NZ2*NY2
+  size_t mplane;        // NZ3*NY3
+};
+
+// Device metrics, each (NZ3*NY3*2) except jac (NZ3*NY3).
+struct Metrics {
+  const double *g11, *g12, *g22, *c11, *c12, *c21, *c22, *jac;
+};
+
+__host__ __device__ inline size_t mf(const Dims& d, int k, int j, int f) {
+  return (size_t)k + (size_t)d.NZ3 * ((size_t)j + (size_t)d.NY3 * (size_t)(f - 1));
+}
+__host__ __device__ inline size_t mi(const Dims& d, int k, int j) {
+  return (size_t)k + (size_t)d.NZ3 * (size_t)j;
+}
+
+// Index ranges of INIT_DUCT (src/duct.F90:7-232).
+struct Ranges {
+  int js, je, zs, ze;
+};
+
+// One boundary row of an ADI system, the net effect of the APPLY_BC_*
+// routines of src/boundary.F90 (IC_TYPE = 7) after all their assignments.
+enum BcKind : int {
+  BC_CONST = 0,    // rhs = val
+  BC_LAGGED = 1,   // rhs = field on the same line at index src (type 6)
+  BC_THFLUX_LO = 2,  // rhs = val / (0.5*(C22(K,1,2)+C22(K+1,1,1))) * J(K,1)
+  BC_THFLUX_NY = 3,  // rhs = 0.5*val*(J(K,NY)+J(K,NY-1)) / C22(K,NY-1,1)
+  BC_THFLUX_NY1 = 4, // rhs = 0.5*val*(J(K,NY+1)+J(K,NY)) / C22(K,NY,1)
+};
+struct BcRow {
+  int row;  // -1 = unused
+  int kind;
+  int src;
+  int pad;
+  double a, b, c, val;
+};
+struct BcTable {
+  BcRow r[4];
+  int sp;          // 1: THOMAS_REAL_SP elimination of row NZ+1 by row NZ (type 5)
+  int sp_metric;   // D = GMAT_11(NZ,J,2) (U1) instead of 1.0
+  int pad[2];
+};
+
+// ---- FFT ------------------------------------------------------------------
+struct FftPlan {
+  int nx = 0;
+  double* tw = nullptr;   // exp(-2 pi i k / (NX/2)), k < NX/4 (re,im)
+  double* tw2 = nullptr;  // exp(-2 pi i k / NX), k <= NX/2 (re,im)
+};
+int fft_plan_init(FftPlan& p, int nx);
+void fft_plan_free(FftPlan& p);
+// in-place transforms of npencil pencils of stride NX+2
+int fft_forward(const FftPlan& p, double* v, long npencil, hipStream_t st);
+int fft_inverse(const FftPlan& p, double* v, long npencil, hipStream_t st);
+
+// ---- multigrid --------------------------------------------------------------
+struct HostMetrics;
+struct MgLevel {
+  int nx = 0, ny = 0;
+  size_t phi_off = 0, rhs_off = 0;  // offsets inside a solve's workspace (doubles)
+  const double* coef = nullptr;     // c1..c8 [8][ny][nx]
+  const double* s9 = nullptr;       // c1+c3+c5+c7 before boundary folding
+  const double* corr9 = nullptr;    // additive corner correction of c9
+  const double* jl = nullptr;       // INT_JACOB at the level's points (0 at specified points)
+  int in_lds = 0;                   // level lives in LDS inside the solve kernel
+};
+struct MgPlan {
+  int ngrid = 0;
+  int nxa = 0, nxb = 0, nyc = 0, nyd = 0;
+  int maxcy = 0;
+  double tolmax = 0.0;
+  int nsolve_max = 0;
+  MgLevel lev[17];
+  size_t ws_global = 0;      // doubles per solve in global memory
+  size_t ws_lds = 0;         // doubles per solve in LDS
+  size_t resf_g = 0, phcor_g = 0, resf_l = 0, phcor_l = 0, red_l = 0;
+  int lc = 0;                // finest level held in LDS
+  double* ws = nullptr;      // global workspace [nsolve_max][ws_global]
+  double* coefmem = nullptr; // all level coefficient arrays
+  int* cycles = nullptr;     // [nsolve_max]
+  double* kx2 = nullptr;     // [nsolve_max]
+  double* rhs = nullptr;     // [nsolve_max][plane] RHS_TMP
+  double* phi = nullptr;     // [nsolve_max][plane]
+  int block = 512;
+};
+int mg_plan_build(MgPlan& mg, const ekm_params& p, const Dims& d, const HostMetrics& hm, int nsolve_max,
+                  int block, size_t lds_budget_doubles);
+void mg_plan_free(MgPlan& mg);
+// solves nsolve planes of mg.rhs (RHS_TMP) into mg.phi (iguess: mg.phi holds
+// the initial guesses); cycle counts in mg.cycles
+int mg_solve_ex(MgPlan& mg, const Dims& d, int nsolve, int iguess, hipStream_t st);
+
+// ---- solver -------------------------------------------------------------------
+struct ekm_solver_impl;

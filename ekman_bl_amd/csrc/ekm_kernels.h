@@ -1,0 +1,46 @@
+// ekm_kernels.h -- launchers of the substep kernels (ekm_kernels.hip).
+#pragma once
+#include "ekm_internal.h"
+
+struct RhsArgs {
+  const double2* cu[3];
+  const double2* cp;
+  const double2* cth;
+  double2* cr[3];
+  double2* cf[3];
+  double2* crth;
+  double2* cfth;
+  const double* kx;
+  const double* kx2;
+  const double* thbar;
+  Ranges rv, rt;
+  int rk, th, cont_strat, cor_mean_only, px_on;
+  double temp3, temp4, nu, f0, ug, wg, ri_tau, nupr, px;
+};
+
+int launch_rhs_fourier(const Dims& d, const Metrics& M, const RhsArgs& a, hipStream_t st);
+int launch_contravariant(const Dims& d, const Metrics& M, const Ranges& r, const double* U2,
+                         const double* U3, double* U2b, double* U3b, hipStream_t st);
+int launch_flux(const Dims& d, const Metrics& M, const Ranges& r, const double* U1, const double* Q,
+                const double* U2b, const double* U3b, double nu, double* Sx, double* Syz,
+                hipStream_t st);
+int launch_accum(const Dims& d, const Ranges& r, const double* kx, const double* Sx,
+                 const double* Syz, double* CF, double* CR, double temp5, hipStream_t st);
+// lu: scratch of 7 * NZ2*NY2 doubles
+int launch_adi(const Dims& d, const Metrics& M, const Ranges& r, const BcTable& by,
+               const BcTable& bz, double t1, double* U, double* R, double* S, double* lu,
+               int th_corners, hipStream_t st);
+int launch_cfl(const Dims& d, const Metrics& M, const Ranges& r, const double* U1, const double* U2,
+               const double* U3, double cfl, double dx, unsigned long long* out, hipStream_t st);
+int launch_faces(const Dims& d, const Metrics& M, const Ranges& r, const double* CU2, const double* CU3,
+                 double* F2, double* F3, hipStream_t st);
+int launch_mg_gather(const Dims& d, const Metrics& M, const Ranges& r, const double* kx, const double* CU1,
+                     const double* F2, const double* F3, double* rhs, int nmode, hipStream_t st);
+int launch_mg_scatter(const Dims& d, const double* phi, int nmode, int fp32, double* PHI, hipStream_t st);
+int launch_project(const Dims& d, const Metrics& M, const Ranges& r, const double* kx, const double* PHI,
+                   const double* F2, const double* F3, double* CU1, double* CU2, double* CU3, double* DIV,
+                   hipStream_t st);
+int launch_pupdate(const Dims& d, const Ranges& r, const double* PHI, double* CP, double temp4, hipStream_t st);
+int launch_div_reduce(const Dims& d, const double* S, double* out, hipStream_t st);
+int launch_thomas_generic(const double* A, const double* B, const double* C, double* G, double* W, int ny,
+                          int nx, hipStream_t st);

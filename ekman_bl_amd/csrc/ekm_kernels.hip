@@ -1,0 +1,766 @@
+// ekm_kernels.hip -- explicit RHS, nonlinear fluxes, ADI sweeps and the
+// pressure projection of one RK3 substep (reference: src/duct.F90 RK_DUCT
+// :237-2469 and REM_DIV_CURV :2472-2818).
+//
+// All kernels are memory bound FP64 passes over the device-resident fields.
+// Physical kernels map consecutive threads to consecutive x (i) so every
+// load of a (k,j) row is one coalesced run; Fourier kernels map threads to
+// consecutive kx modes the same way.  The ADI sweeps give every thread one
+// (i,k) or (i,j) line: for a fixed row the threads of a wavefront touch
+// consecutive i, so each step of the sequential recurrence is a coalesced
+// row load.  The tridiagonal factors do not depend on i (no eddy viscosity in
+// the plain step), so they are computed once per line by k_lu and shared.
+#include "ekm_kernels.h"
+
+namespace {
+
+__device__ inline size_t ph(const Dims& d, int i, int k, int j) {
+  return (size_t)i + (size_t)d.NX2 * ((size_t)k + (size_t)d.NZ2 * (size_t)j);
+}
+__device__ inline size_t fo(const Dims& d, int m, int k, int j) {
+  return (size_t)m + (size_t)d.NXC * ((size_t)k + (size_t)d.NZ2 * (size_t)j);
+}
+
+#define G11(k, j, f) M.g11[mf(d, k, j, f)]
+#define G12(k, j, f) M.g12[mf(d, k, j, f)]
+#define G22(k, j, f) M.g22[mf(d, k, j, f)]
+#define C11(k, j, f) M.c11[mf(d, k, j, f)]
+#define C12(k, j, f) M.c12[mf(d, k, j, f)]
+#define C21(k, j, f) M.c21[mf(d, k, j, f)]
+#define C22(k, j, f) M.c22[mf(d, k, j, f)]
+#define JAC(k, j) M.jac[mi(d, k, j)]
+
+__device__ inline bool in(int v, int lo, int hi) { return v >= lo && v <= hi; }
+
+// ---------------------------------------------------------------------------
+// explicit Fourier-space RHS (src/duct.F90:298-632)
+// ---------------------------------------------------------------------------
+__global__ void k_rhs_fourier(Dims d, Metrics M, RhsArgs a) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t n = (size_t)d.NXC * d.NZ2 * d.NY2;
+  if (t >= n) return;
+  int m = (int)(t % d.NXC);
+  size_t r = t / d.NXC;
+  int k = (int)(r % d.NZ2), j = (int)(r / d.NZ2);
+  size_t q = t;
+  const size_t dk = d.NXC, dj = (size_t)d.NXC * d.NZ2;
+  if (in(j, a.rv.js, a.rv.je) && in(k, a.rv.zs, a.rv.ze)) {
+    double Jc = JAC(k, j);
+    const double2* CP = a.cp;
+    double2 p = CP[q], pkp = CP[q + dk], pkm = CP[q - dk], pjp = CP[q + dj], pjm = CP[q - dj];
+    double2 cr[3], cf[3], cu[3];
+    for (int f = 0; f < 3; ++f) {
+      cu[f] = a.cu[f][q];
+      cr[f] = make_double2(Jc * cu[f].x, Jc * cu[f].y);
+      if (a.rk > 1) {
+        double2 o = a.cf[f][q];
+        cr[f].x = cr[f].x + a.temp3 * o.x;
+        cr[f].y = cr[f].y + a.temp3 * o.y;
+      }
+    }
+    // pressure gradient as explicit Euler (:366-409)
+    double a11p = 0.5 * C11(k + 1, j, 2), a11 = 0.5 * C11(k, j, 2);
+    double a21p = 0.5 * C21(k, j + 1, 1), a21 = 0.5 * C21(k, j, 1);
+    double a12p = 0.5 * C12(k + 1, j, 2), a12 = 0.5 * C12(k, j, 2);
+    double a22p = 0.5 * C22(k, j + 1, 1), a22 = 0.5 * C22(k, j, 1);
+    double2 cs1, cs2;
+    cs1.x = a11p * (p.x + pkp.x) - a11 * (p.x + pkm.x) + a21p * (p.x + pjp.x) - a21 * (p.x + pjm.x);
+    cs1.y = a11p * (p.y + pkp.y) - a11 * (p.y + pkm.y) + a21p * (p.y + pjp.y) - a21 * (p.y + pjm.y);
+    cs2.x = a12p * (p.x + pkp.x) - a12 * (p.x + pkm.x) + a22p * (p.x + pjp.x) - a22 * (p.x + pjm.x);
+    cs2.y = a12p * (p.y + pkp.y) - a12 * (p.y + pkm.y) + a22p * (p.y + pjp.y) - a22 * (p.y + pjm.y);
+    double b = a.temp4 * a.kx[m] * Jc;
+    cr[0].x = cr[0].x - (0.0 * p.x - b * p.y);
+    cr[0].y = cr[0].y - (0.0 * p.y + b * p.x);
+    cr[1].x = cr[1].x - a.temp4 * cs2.x;
+    cr[1].y = cr[1].y - a.temp4 * cs2.y;
+    cr[2].x = cr[2].x - a.temp4 * cs1.x;
+    cr[2].y = cr[2].y - a.temp4 * cs1.y;
+    if (m == 0 && a.px_on) cr[2].x = cr[2].x - a.temp4 * a.px * Jc;  // (:431-453)
+    // horizontal viscous terms and rotation (:461-475)
+    double fv = a.nu * a.kx2[m] * Jc;
+    for (int f = 0; f < 3; ++f) cf[f] = make_double2(-(fv * cu[f].x), -(fv * cu[f].y));
+    double og = (a.cor_mean_only && m > 0) ? 0.0 : a.ug;
+    double ow = (a.cor_mean_only && m > 0) ? 0.0 : a.wg;
+    cf[2].x = cf[2].x + a.f0 * (cu[0].x - og) * Jc;
+    cf[2].y = cf[2].y + a.f0 * cu[0].y * Jc;
+    cf[0].x = cf[0].x - a.f0 * (cu[2].x - ow) * Jc;
+    cf[0].y = cf[0].y - a.f0 * cu[2].y * Jc;
+    if (a.th) {  // buoyancy (:517-523)
+      double2 c = a.cth[q];
+      double bj = Jc * a.ri_tau;
+      cf[1].x = cf[1].x - bj * c.x;
+      cf[1].y = cf[1].y - bj * c.y;
+    }
+    for (int f = 0; f < 3; ++f) {
+      a.cr[f][q] = cr[f];
+      a.cf[f][q] = cf[f];
+    }
+  } else {
+    for (int f = 0; f < 3; ++f) a.cr[f][q] = make_double2(0.0, 0.0);
+  }
+  if (!a.th) return;
+  // scalar (:532-632)
+  if (in(j, a.rt.js, a.rt.je) && in(k, a.rt.zs, a.rt.ze)) {
+    double Jc = JAC(k, j);
+    double2 c = a.cth[q];
+    double2 crth = make_double2(Jc * c.x, Jc * c.y);
+    if (a.rk > 1) {
+      double2 o = a.cfth[q];
+      crth.x = crth.x + a.temp3 * o.x;
+      crth.y = crth.y + a.temp3 * o.y;
+    }
+    double fth = -a.nupr * Jc * a.kx2[m];
+    double2 cfth = make_double2(fth * c.x, fth * c.y);
+    double2 u2 = a.cu[1][q];
+    if (a.cont_strat) {
+      cfth.x = cfth.x + Jc * u2.x;
+      cfth.y = cfth.y + Jc * u2.y;
+    } else {
+      const double* TB = a.thbar;
+      size_t tt = (size_t)k + (size_t)d.NZ2 * j, tk = 1, tj = d.NZ2;
+      double gb = 0.5 * C12(k + 1, j, 2) * (TB[tt] + TB[tt + tk]) - 0.5 * C12(k, j, 2) * (TB[tt] + TB[tt - tk]) +
+                  0.5 * C22(k, j + 1, 1) * (TB[tt] + TB[tt + tj]) - 0.5 * C22(k, j, 1) * (TB[tt] + TB[tt - tj]);
+      cfth.x = cfth.x - u2.x * gb;
+      cfth.y = cfth.y - u2.y * gb;
+      double dif = 0.25 * a.nupr *
+                   (+G12(k + 1, j, 2) * (TB[tt + tk + tj] + TB[tt + tj] - TB[tt - tj] - TB[tt + tk - tj]) -
+                    G12(k, j, 2) * (TB[tt - tk + tj] + TB[tt + tj] - TB[tt - tj] - TB[tt - tk - tj]) +
+                    G12(k, j + 1, 1) * (TB[tt + tk + tj] + TB[tt + tk] - TB[tt - tk + tj] - TB[tt - tk]) -
+                    G12(k, j, 1) * (TB[tt + tk] + TB[tt + tk - tj] - TB[tt - tk] - TB[tt - tk - tj]) +
+                    4.0 * (G11(k + 1, j, 2) * (TB[tt + tk] - TB[tt]) - G11(k, j, 2) * (TB[tt] - TB[tt - tk])) +
+                    4.0 * (G22(k, j + 1, 1) * (TB[tt + tj] - TB[tt]) - G22(k, j, 1) * (TB[tt] - TB[tt - tj])));
+      cfth.x = cfth.x + dif;
+    }
+    a.crth[q] = crth;
+    a.cfth[q] = cfth;
+  } else {
+    a.crth[q] = make_double2(0.0, 0.0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// contravariant face velocities (:721-732)
+// ---------------------------------------------------------------------------
+__global__ void k_contravariant(Dims d, Metrics M, Ranges r, const double* __restrict__ U2,
+                                const double* __restrict__ U3, double* __restrict__ U2b,
+                                double* __restrict__ U3b) {
+  int nk = r.ze + 1 - r.zs + 1, nj = r.je + 1 - r.js + 1;
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t n = (size_t)d.NX2 * nk * nj;
+  if (t >= n) return;
+  int i = (int)(t % d.NX2);
+  size_t rr = t / d.NX2;
+  int k = r.zs + (int)(rr % nk), j = r.js + (int)(rr / nk);
+  size_t q = ph(d, i, k, j), dj = (size_t)d.NX2 * d.NZ2, dk = d.NX2;
+  U2b[q] = 0.5 * C22(k, j, 1) * (U2[q] + U2[q - dj]) + 0.5 * C21(k, j, 1) * (U3[q] + U3[q - dj]);
+  U3b[q] = 0.5 * C11(k, j, 2) * (U3[q] + U3[q - dk]) + 0.5 * C12(k, j, 2) * (U2[q] + U2[q - dk]);
+}
+
+// ---------------------------------------------------------------------------
+// nonlinear fluxes of one transported field Q (:683-1120, :1175-1237):
+// Sx = J U1 Q; Syz = -(U2b flux)_eta + nu (G12 Q_eta)_zeta - (U3b flux)_zeta + nu (G12 Q_zeta)_eta
+// ---------------------------------------------------------------------------
+__global__ void k_flux(Dims d, Metrics M, Ranges r, const double* __restrict__ U1,
+                       const double* __restrict__ Q, const double* __restrict__ U2b,
+                       const double* __restrict__ U3b, double nu, double* __restrict__ Sx,
+                       double* __restrict__ Syz) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.npts) return;
+  int i = (int)(t % d.NX2);
+  size_t rr = t / d.NX2;
+  int k = (int)(rr % d.NZ2), j = (int)(rr / d.NZ2);
+  if (!(in(j, r.js, r.je) && in(k, r.zs, r.ze))) {
+    Sx[t] = 0.0;
+    Syz[t] = 0.0;
+    return;
+  }
+  (void)i;
+  size_t q = t, dj = (size_t)d.NX2 * d.NZ2, dk = d.NX2;
+  double Jc = JAC(k, j);
+  double qc = Q[q];
+  Sx[q] = Jc * U1[q] * qc;
+  double s1 = 0.5 * U2b[q + dj] * (qc + Q[q + dj]) - 0.5 * U2b[q] * (qc + Q[q - dj]);
+  double s2 = 0.5 * U3b[q + dk] * (qc + Q[q + dk]) - 0.5 * U3b[q] * (qc + Q[q - dk]);
+  double qpp = Q[q + dk + dj], qp0 = Q[q + dk], qpm = Q[q + dk - dj];
+  double q0p = Q[q + dj], q0m = Q[q - dj];
+  double qmp = Q[q - dk + dj], qm0 = Q[q - dk], qmm = Q[q - dk - dj];
+  s1 = -s1 + 0.25 * nu * (G12(k + 1, j, 2) * (qpp + q0p - q0m - qpm) - G12(k, j, 2) * (qmp + q0p - q0m - qmm));
+  s2 = -s2 + 0.25 * nu * (G12(k, j + 1, 1) * (qpp + qp0 - qmp - qm0) - G12(k, j, 1) * (qp0 + qpm - qm0 - qmm));
+  Syz[q] = s1 + s2;
+}
+
+// CF = (CF - i kx Sx) + Syz; CR = CR + TEMP5 CF  (:714, :772-777, :1127-1149)
+__global__ void k_accum(Dims d, Ranges r, const double* __restrict__ kx,
+                        const double2* __restrict__ Sx, const double2* __restrict__ Syz,
+                        double2* __restrict__ CF, double2* __restrict__ CR, double temp5) {
+  int nk = r.ze - r.zs + 1, nj = r.je - r.js + 1;
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t n = (size_t)d.NXC * nk * nj;
+  if (t >= n) return;
+  int m = (int)(t % d.NXC);
+  size_t rr = t / d.NXC;
+  int k = r.zs + (int)(rr % nk), j = r.js + (int)(rr / nk);
+  size_t q = fo(d, m, k, j);
+  double2 f = CF[q], s = Sx[q], y = Syz[q];
+  double w = kx[m];
+  f.x = f.x - (0.0 * s.x - w * s.y);
+  f.y = f.y - (0.0 * s.y + w * s.x);
+  f.x = f.x + y.x;
+  f.y = f.y + y.y;
+  CF[q] = f;
+  double2 c = CR[q];
+  c.x = c.x + temp5 * f.x;
+  c.y = c.y + temp5 * f.y;
+  CR[q] = c;
+}
+
+// ---------------------------------------------------------------------------
+// ADI: explicit half before the y sweep (:1334-1371, :1640-1665, ...)
+// S = R - J U;  R = S/2 + J U + t1 [G11(K+1)(U(K+1)-U) - G11(K)(U-U(K-1))]
+// ---------------------------------------------------------------------------
+__global__ void k_adi_pre(Dims d, Metrics M, Ranges r, const double* __restrict__ U,
+                          double* __restrict__ R, double* __restrict__ S, double t1) {
+  int nk = r.ze - r.zs + 1, nj = r.je - r.js + 1;
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t n = (size_t)d.NX2 * nk * nj;
+  if (t >= n) return;
+  int i = (int)(t % d.NX2);
+  size_t rr = t / d.NX2;
+  int k = r.zs + (int)(rr % nk), j = r.js + (int)(rr / nk);
+  size_t q = ph(d, i, k, j), dk = d.NX2;
+  double Jc = JAC(k, j), u = U[q];
+  double s = R[q] - Jc * u;
+  S[q] = s;
+  R[q] = 0.5 * s + Jc * u + t1 * (G11(k + 1, j, 2) * (U[q + dk] - u) - G11(k, j, 2) * (u - U[q - dk]));
+}
+
+// Per-line factorisation of one sweep (THOMAS_REAL forward elimination,
+// src/boundary.F90:53-84, with the rows of the APPLY_BC_* routines).  dir 0:
+// y sweep, lines K = zs..ze, rows J = 0..NY+1; dir 1: z sweep, lines J, rows K.
+// Output per (line, row): mult (the reference's modified A), piv (modified B),
+// up (C).  Rows that neither the interior loop nor a BC routine sets keep the
+// [0 1 0 | 0] the reference initialises them to (:1399-1406, :1690-1697).
+__global__ void k_lu(Dims d, Metrics M, Ranges r, BcTable bc, double t1, int dir,
+                     double* __restrict__ mult, double* __restrict__ piv,
+                     double* __restrict__ up, double* __restrict__ spa) {
+  int line0 = dir == 0 ? r.zs : r.js, line1 = dir == 0 ? r.ze : r.je;
+  int L = line0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (L > line1) return;
+  int N = dir == 0 ? d.NY + 1 : d.NZ + 1;  // last row index
+  int lo = dir == 0 ? r.js : r.zs, hi = dir == 0 ? r.je : r.ze;
+  size_t base = (size_t)L * (N + 1);
+  double bprev = 0.0, cprev = 0.0;
+  double a_nz = 0.0;
+  for (int row = 0; row <= N; ++row) {
+    double a = 0.0, b = 1.0, c = 0.0;
+    if (row >= lo && row <= hi) {
+      int k = dir == 0 ? L : row, j = dir == 0 ? row : L;
+      if (dir == 0) {
+        a = -(t1 * G22(k, j, 1));
+        b = JAC(k, j) + t1 * (G22(k, j + 1, 1) + G22(k, j, 1));
+        c = -(t1 * G22(k, j + 1, 1));
+      } else {
+        a = -(t1 * G11(k, j, 2));
+        b = JAC(k, j) + t1 * (G11(k + 1, j, 2) + G11(k, j, 2));
+        c = -(t1 * G11(k + 1, j, 2));
+      }
+    }
+    for (int e = 0; e < 4; ++e)
+      if (bc.r[e].row == row) {
+        a = bc.r[e].a;
+        b = bc.r[e].b;
+        c = bc.r[e].c;
+      }
+    if (dir == 1 && bc.sp && row == N) {
+      // THOMAS_REAL_SP (src/boundary.F90:17-24): row NZ+1 -= row NZ * D/A(NZ)
+      double D = bc.sp_metric ? G11(d.NZ, L, 2) : 1.0;
+      a = a - bprev * D / a_nz;
+      b = b - cprev * D / a_nz;
+      spa[2 * L] = D;
+      spa[2 * L + 1] = a_nz;
+    }
+    if (row == N - 1) a_nz = a;
+    double mlt = 0.0, pv = b;
+    if (row > 0) {
+      mlt = -a / piv[base + row - 1];
+      pv = b + mlt * up[base + row - 1];
+    }
+    mult[base + row] = mlt;
+    piv[base + row] = pv;
+    up[base + row] = c;
+    bprev = b;
+    cprev = c;
+  }
+}
+
+__device__ inline double bc_rhs(const Dims& d, const Metrics& M, const BcRow& e, int K, double lag) {
+  switch (e.kind) {
+    case BC_LAGGED: return lag;
+    case BC_THFLUX_LO: return e.val / (0.5 * (C22(K, 1, 2) + C22(K + 1, 1, 1))) * JAC(K, 1);
+    case BC_THFLUX_NY: return 0.5 * e.val * (JAC(K, d.NY) + JAC(K, d.NY - 1)) / C22(K, d.NY - 1, 1);
+    case BC_THFLUX_NY1: return 0.5 * e.val * (JAC(K, d.NY + 1) + JAC(K, d.NY)) / C22(K, d.NY, 1);
+    default: return e.val;
+  }
+}
+
+// y sweep for lines (i,K): solves in place into U, then emits the z-sweep
+// right-hand side R = S/2 + J U* + t1 [G22(J+1)(U*(J+1)-U*) - G22(J)(U*-U*(J-1))]
+// (:1460-1478 for theta, :1744-1757 for U1, ...).
+__global__ void k_ysweep(Dims d, Metrics M, Ranges r, BcTable bc, double t1,
+                         const double* __restrict__ mult, const double* __restrict__ piv,
+                         const double* __restrict__ up, double* __restrict__ U,
+                         double* __restrict__ R, const double* __restrict__ S) {
+  int nk = r.ze - r.zs + 1;
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)d.NX2 * nk) return;
+  int i = (int)(t % d.NX2), K = r.zs + (int)(t / d.NX2);
+  const int N = d.NY + 1;
+  const size_t dj = (size_t)d.NX2 * d.NZ2;
+  const size_t q0 = ph(d, i, K, 0);
+  const size_t base = (size_t)K * (N + 1);
+  double lag[4];
+  for (int e = 0; e < 4; ++e) lag[e] = (bc.r[e].kind == BC_LAGGED) ? U[q0 + dj * bc.r[e].src] : 0.0;
+  double g = 0.0;
+  for (int row = 0; row <= N; ++row) {
+    double rhs = (row >= r.js && row <= r.je) ? R[q0 + dj * row] : 0.0;
+    for (int e = 0; e < 4; ++e)
+      if (bc.r[e].row == row) rhs = bc_rhs(d, M, bc.r[e], K, lag[e]);
+    g = (row == 0) ? rhs : rhs + mult[base + row] * g;
+    U[q0 + dj * row] = g;
+  }
+  double x2 = 0.0, x1 = g / piv[base + N];  // x1 = x(row+1), x2 = x(row+2)
+  U[q0 + dj * N] = x1;
+  for (int row = N - 1; row >= 0; --row) {
+    double x0 = (U[q0 + dj * row] - up[base + row] * x1) / piv[base + row];
+    U[q0 + dj * row] = x0;
+    int jr = row + 1;  // R at row+1 now has x(row), x(row+1), x(row+2)
+    if (jr >= r.js && jr <= r.je) {
+      size_t q = q0 + dj * jr;
+      R[q] = 0.5 * S[q] + JAC(K, jr) * x1 + t1 * (G22(K, jr + 1, 1) * (x2 - x1) - G22(K, jr, 1) * (x1 - x0));
+    }
+    x2 = x1;
+    x1 = x0;
+  }
+}
+
+// z sweep for lines (i,J) (:1503-1596 and analogues)
+__global__ void k_zsweep(Dims d, Metrics M, Ranges r, BcTable bc, const double* __restrict__ mult,
+                         const double* __restrict__ piv, const double* __restrict__ up,
+                         const double* __restrict__ spa, double* __restrict__ U,
+                         const double* __restrict__ R) {
+  int nj = r.je - r.js + 1;
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)d.NX2 * nj) return;
+  int i = (int)(t % d.NX2), J = r.js + (int)(t / d.NX2);
+  const int N = d.NZ + 1;
+  const size_t dk = d.NX2;
+  const size_t q0 = ph(d, i, 0, J);
+  const size_t base = (size_t)J * (N + 1);
+  double lag[4];
+  for (int e = 0; e < 4; ++e) lag[e] = (bc.r[e].kind == BC_LAGGED) ? U[q0 + dk * bc.r[e].src] : 0.0;
+  double g = 0.0, rprev = 0.0;
+  for (int row = 0; row <= N; ++row) {
+    double rhs = (row >= r.zs && row <= r.ze) ? R[q0 + dk * row] : 0.0;
+    for (int e = 0; e < 4; ++e)
+      if (bc.r[e].row == row) rhs = bc_rhs(d, M, bc.r[e], J, lag[e]);
+    if (bc.sp && row == N) rhs = rhs - rprev * spa[2 * J] / spa[2 * J + 1];
+    g = (row == 0) ? rhs : rhs + mult[base + row] * g;
+    U[q0 + dk * row] = g;
+    rprev = rhs;
+  }
+  double x1 = g / piv[base + N];
+  U[q0 + dk * N] = x1;
+  for (int row = N - 1; row >= 0; --row) {
+    double x0 = (U[q0 + dk * row] - up[base + row] * x1) / piv[base + row];
+    U[q0 + dk * row] = x0;
+    x1 = x0;
+  }
+}
+
+// scalar corner updates after the z sweep (:1606-1622)
+__global__ void k_th_corners(Dims d, double* __restrict__ U, int mode) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.NX2) return;
+  int NZ = d.NZ, NY = d.NY;
+  auto P = [&](int k, int j) -> double& { return U[ph(d, i, k, j)]; };
+  if (mode & 1) {
+    P(NZ + 1, NY + 1) = P(NZ, NY + 1);
+    P(NZ + 1, 0) = P(NZ, 0);
+    P(0, NY + 1) = P(1, NY + 1);
+    P(0, 0) = P(1, 0);
+  }
+  if (mode & 2) {
+    P(NZ + 1, NY + 1) = P(1, NY + 1);
+    P(NZ + 1, 0) = P(1, 0);
+    P(0, NY + 1) = P(NZ, NY + 1);
+    P(0, 0) = P(NZ, 0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// CFL time step (courant_curv_mpi, src/duct.F90:3605-3674): min over the
+// interior of cfl dx/|U1|, cfl J/|U2b|, cfl J/|U3b|.  Positive doubles order
+// like their bit patterns, so the minimum is an atomicMin on uint64.
+// ---------------------------------------------------------------------------
+__global__ void k_cfl(Dims d, Metrics M, Ranges r, const double* __restrict__ U1,
+                      const double* __restrict__ U2, const double* __restrict__ U3, double cfl,
+                      double dx, unsigned long long* __restrict__ out) {
+  int nk = r.ze - r.zs + 1, nj = r.je - r.js + 1;
+  size_t n = (size_t)d.NX2 * nk * nj;
+  double dt = 999.0;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    int i = (int)(t % d.NX2);
+    size_t rr = t / d.NX2;
+    int k = r.zs + (int)(rr % nk), j = r.js + (int)(rr / nk);
+    size_t q = ph(d, i, k, j), dj = (size_t)d.NX2 * d.NZ2, dk = d.NX2;
+    double u2b = 0.5 * C22(k, j, 1) * (U2[q] + U2[q - dj]) + 0.5 * C21(k, j, 1) * (U3[q] + U3[q - dj]);
+    double u3b = 0.5 * C11(k, j, 2) * (U3[q] + U3[q - dk]) + 0.5 * C12(k, j, 2) * (U2[q] + U2[q - dk]);
+    double Jc = JAC(k, j);
+    double dtx = cfl * dx / fabs(U1[q]);
+    double dty = cfl * Jc / fabs(u2b);
+    double dtz = cfl * Jc / fabs(u3b);
+    dt = fmin(dt, fmin(dtx, fmin(dty, dtz)));
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = dt;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMin(out, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+// ---------------------------------------------------------------------------
+// projection (REM_DIV_CURV)
+// ---------------------------------------------------------------------------
+// face fluxes of the Fourier velocity (:2511-2522), zero outside their range
+__global__ void k_faces(Dims d, Metrics M, Ranges r, const double2* __restrict__ CU2,
+                        const double2* __restrict__ CU3, double2* __restrict__ F2,
+                        double2* __restrict__ F3) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t n = (size_t)d.NXC * d.NZ2 * d.NY2;
+  if (t >= n) return;
+  size_t rr = t / d.NXC;
+  int k = (int)(rr % d.NZ2), j = (int)(rr / d.NZ2);
+  if (!(in(j, r.js, r.je + 1) && in(k, r.zs, r.ze + 1))) {
+    F2[t] = make_double2(0.0, 0.0);
+    F3[t] = make_double2(0.0, 0.0);
+    return;
+  }
+  size_t dk = d.NXC, dj = (size_t)d.NXC * d.NZ2;
+  double a22 = 0.5 * C22(k, j, 1), a21 = 0.5 * C21(k, j, 1);
+  double a11 = 0.5 * C11(k, j, 2), a12 = 0.5 * C12(k, j, 2);
+  double2 u2 = CU2[t], u3 = CU3[t], u2j = CU2[t - dj], u3j = CU3[t - dj], u2k = CU2[t - dk], u3k = CU3[t - dk];
+  F2[t] = make_double2(a22 * (u2.x + u2j.x) + a21 * (u3.x + u3j.x), a22 * (u2.y + u2j.y) + a21 * (u3.y + u3j.y));
+  F3[t] = make_double2(a11 * (u3.x + u3k.x) + a12 * (u2.x + u2k.x), a11 * (u3.y + u3k.y) + a12 * (u2.y + u2k.y));
+}
+
+// RHS_TMP planes of the per-mode solves (:2527-2535 and :2620-2656): solve
+// s = 2m + part, part 0 the imaginary and part 1 the real component; the
+// divergence is formed on the fly from the face fluxes.
+__global__ void k_mg_gather(Dims d, Metrics M, Ranges r, const double* __restrict__ kx,
+                            const double2* __restrict__ CU1, const double2* __restrict__ F2,
+                            const double2* __restrict__ F3, double* __restrict__ rhs, int nmode) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t n = (size_t)nmode * d.NZ2 * d.NY2;
+  if (t >= n) return;
+  int m = (int)(t % nmode);
+  size_t rr = t / nmode;
+  int k = (int)(rr % d.NZ2), j = (int)(rr / d.NZ2);
+  double vi = 0.0, vr = 0.0;
+  if (in(j, r.js, r.je) && in(k, r.zs, r.ze)) {
+    size_t q = fo(d, m, k, j), dk = d.NXC, dj = (size_t)d.NXC * d.NZ2;
+    double kj = kx[m] * JAC(k, j);
+    double2 u = CU1[q], f2 = F2[q], f2p = F2[q + dj], f3 = F3[q], f3p = F3[q + dk];
+    double re = (0.0 * u.x - kj * u.y) + (f2p.x - f2.x) + (f3p.x - f3.x);
+    double im = (0.0 * u.y + kj * u.x) + (f2p.y - f2.y) + (f3p.y - f3.y);
+    vi = -im;
+    vr = -re;
+  }
+  size_t p = (size_t)k + (size_t)d.NZ2 * j;
+  rhs[(size_t)(2 * m) * d.plane + p] = vi;
+  rhs[(size_t)(2 * m + 1) * d.plane + p] = vr;
+}
+
+// phi = CMPLX(P_TMP2, P_TMP) (:2662-2666), modes above NKX are zero
+__global__ void k_mg_scatter(Dims d, const double* __restrict__ phi, int nmode, int fp32,
+                             double2* __restrict__ PHI) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t n = (size_t)d.NXC * d.NZ2 * d.NY2;
+  if (t >= n) return;
+  int m = (int)(t % d.NXC);
+  size_t p = t / d.NXC;  // k + NZ2*j
+  double2 v = make_double2(0.0, 0.0);
+  if (m < nmode) {
+    v.x = phi[(size_t)(2 * m + 1) * d.plane + p];
+    v.y = phi[(size_t)(2 * m) * d.plane + p];
+    if (fp32) {
+      v.x = (double)(float)v.x;
+      v.y = (double)(float)v.y;
+    }
+  }
+  PHI[t] = v;
+}
+
+__device__ inline double2 face2(const Dims& d, const Metrics& M, const Ranges& r, const double2* F2,
+                                const double2* P, int m, int k, int j) {
+  size_t q = fo(d, m, k, j);
+  double2 f = F2[q];
+  if (in(j, r.js, r.je) && in(k, r.zs, r.ze)) {
+    size_t dk = d.NXC, dj = (size_t)d.NXC * d.NZ2;
+    double g = G22(k, j, 1), h = 0.25 * G12(k, j, 1);
+    double2 p0 = P[q], pj = P[q - dj], pk = P[q + dk], pkj = P[q + dk - dj], qk = P[q - dk], qkj = P[q - dk - dj];
+    f.x = f.x - g * (p0.x - pj.x) - h * (pk.x + pkj.x - qk.x - qkj.x);
+    f.y = f.y - g * (p0.y - pj.y) - h * (pk.y + pkj.y - qk.y - qkj.y);
+  }
+  return f;
+}
+__device__ inline double2 face3(const Dims& d, const Metrics& M, const Ranges& r, const double2* F3,
+                                const double2* P, int m, int k, int j) {
+  size_t q = fo(d, m, k, j);
+  double2 f = F3[q];
+  if (in(j, r.js, r.je) && in(k, r.zs, r.ze)) {
+    size_t dk = d.NXC, dj = (size_t)d.NXC * d.NZ2;
+    double g = G11(k, j, 2), h = 0.25 * G12(k, j, 2);
+    double2 p0 = P[q], pk = P[q - dk], pj = P[q + dj], pkj = P[q - dk + dj], qj = P[q - dj], qkj = P[q - dk - dj];
+    f.x = f.x - g * (p0.x - pk.x) - h * (pj.x + pkj.x - qj.x - qkj.x);
+    f.y = f.y - g * (p0.y - pk.y) - h * (pj.y + pkj.y - qj.y - qkj.y);
+  }
+  return f;
+}
+
+// velocity correction (:2713-2750) and the divergence of the corrected face
+// fluxes (:2754-2761); the corrected faces are formed on the fly and not kept.
+__global__ void k_project(Dims d, Metrics M, Ranges r, const double* __restrict__ kx,
+                          const double2* __restrict__ P, const double2* __restrict__ F2,
+                          const double2* __restrict__ F3, double2* __restrict__ CU1,
+                          double2* __restrict__ CU2, double2* __restrict__ CU3,
+                          double2* __restrict__ DIV) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t n = (size_t)d.NXC * d.NZ2 * d.NY2;
+  if (t >= n) return;
+  int m = (int)(t % d.NXC);
+  size_t rr = t / d.NXC;
+  int k = (int)(rr % d.NZ2), j = (int)(rr / d.NZ2);
+  if (!(in(j, r.js, r.je) && in(k, r.zs, r.ze))) {
+    DIV[t] = make_double2(0.0, 0.0);
+    return;
+  }
+  size_t q = t, dk = d.NXC, dj = (size_t)d.NXC * d.NZ2;
+  double w = kx[m];
+  double2 p0 = P[q], pjp = P[q + dj], pjm = P[q - dj], pkp = P[q + dk], pkm = P[q - dk];
+  double2 u1 = CU1[q], u2 = CU2[q], u3 = CU3[q];
+  double Jc = JAC(k, j);
+  u1.x = u1.x - (0.0 * p0.x - w * p0.y);
+  u1.y = u1.y - (0.0 * p0.y + w * p0.x);
+  double b22p = C22(k, j + 1, 1), b22 = C22(k, j, 1), b12p = C12(k + 1, j, 2), b12 = C12(k, j, 2);
+  double b11p = C11(k + 1, j, 2), b11 = C11(k, j, 2), b21p = C21(k, j + 1, 1), b21 = C21(k, j, 1);
+  u2.x = u2.x - 0.5 * (b22p * (p0.x + pjp.x) - b22 * (p0.x + pjm.x) + b12p * (pkp.x + p0.x) - b12 * (p0.x + pkm.x)) / Jc;
+  u2.y = u2.y - 0.5 * (b22p * (p0.y + pjp.y) - b22 * (p0.y + pjm.y) + b12p * (pkp.y + p0.y) - b12 * (p0.y + pkm.y)) / Jc;
+  u3.x = u3.x - 0.5 * (b11p * (p0.x + pkp.x) - b11 * (p0.x + pkm.x) + b21p * (pjp.x + p0.x) - b21 * (p0.x + pjm.x)) / Jc;
+  u3.y = u3.y - 0.5 * (b11p * (p0.y + pkp.y) - b11 * (p0.y + pkm.y) + b21p * (pjp.y + p0.y) - b21 * (p0.y + pjm.y)) / Jc;
+  CU1[q] = u1;
+  CU2[q] = u2;
+  CU3[q] = u3;
+  double2 f2 = face2(d, M, r, F2, P, m, k, j), f2p = face2(d, M, r, F2, P, m, k, j + 1);
+  double2 f3 = face3(d, M, r, F3, P, m, k, j), f3p = face3(d, M, r, F3, P, m, k + 1, j);
+  double kj = w * Jc;
+  DIV[q] = make_double2((0.0 * u1.x - kj * u1.y) + (f2p.x - f2.x) + (f3p.x - f3.x),
+                        (0.0 * u1.y + kj * u1.x) + (f2p.y - f2.y) + (f3p.y - f3.y));
+}
+
+// CPX += phi / TEMP4 over J = js-1..je+1, K = zs-1..ze+1 (:2460-2466)
+__global__ void k_pupdate(Dims d, Ranges r, const double2* __restrict__ P, double2* __restrict__ CP,
+                          double temp4) {
+  int nk = r.ze - r.zs + 3, nj = r.je - r.js + 3;
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t n = (size_t)d.NXC * nk * nj;
+  if (t >= n) return;
+  int m = (int)(t % d.NXC);
+  size_t rr = t / d.NXC;
+  int k = r.zs - 1 + (int)(rr % nk), j = r.js - 1 + (int)(rr / nk);
+  size_t q = fo(d, m, k, j);
+  double2 c = CP[q], p = P[q];
+  c.x = c.x + p.x / temp4;
+  c.y = c.y + p.y / temp4;
+  CP[q] = c;
+}
+
+// divergence diagnostic (:2790-2800): sum and max of |div| over
+// J = 2..NY-1, K = 2..NZ-1, I = 1..NX+1 of the physical field
+__global__ void k_div_reduce(Dims d, const double* __restrict__ S, double* __restrict__ out) {
+  int nk = d.NZ - 2, nj = d.NY - 2, ni = d.NX + 1;
+  size_t n = (size_t)ni * nk * nj;
+  double s = 0.0, mx = 0.0;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    int i = 1 + (int)(t % ni);
+    size_t rr = t / ni;
+    int k = 2 + (int)(rr % nk), j = 2 + (int)(rr / nk);
+    double v = fabs(S[ph(d, i, k, j)]);
+    s += v;
+    mx = fmax(mx, v);
+  }
+  __shared__ double rs[256], rm[256];
+  rs[threadIdx.x] = s;
+  rm[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      rs[threadIdx.x] += rs[threadIdx.x + w];
+      rm[threadIdx.x] = fmax(rm[threadIdx.x], rm[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&out[0], rs[0]);
+    atomicMax(reinterpret_cast<unsigned long long*>(&out[1]), (unsigned long long)__double_as_longlong(rm[0]));
+  }
+}
+
+// generic THOMAS_REAL with per-system coefficients (the drop-in entry point):
+// systems i = 0..nx, unknowns j = 0..ny, element (i,j) at i + (nx+1) j.
+__global__ void k_thomas_generic(const double* __restrict__ A, const double* __restrict__ B,
+                                 const double* __restrict__ C, double* __restrict__ G,
+                                 double* __restrict__ W, int ny, int nx) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > nx) return;
+  size_t ld = (size_t)nx + 1;
+  double bp = B[i];
+  W[i] = bp;
+  double g = G[i];
+  for (int j = 0; j < ny; ++j) {
+    size_t qn = i + ld * (j + 1);
+    double mlt = -A[qn] / bp;
+    bp = B[qn] + mlt * C[qn - ld];
+    g = G[qn] + mlt * g;
+    W[qn] = bp;
+    G[qn] = g;
+  }
+  double x = g / bp;
+  G[i + ld * ny] = x;
+  for (int j = ny - 1; j >= 0; --j) {
+    size_t q = i + ld * j;
+    x = (G[q] - C[q] * x) / W[q];
+    G[q] = x;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned nblk(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+int launch_rhs_fourier(const Dims& d, const Metrics& M, const RhsArgs& a, hipStream_t st) {
+  size_t n = (size_t)d.NXC * d.NZ2 * d.NY2;
+  hipLaunchKernelGGL(k_rhs_fourier, dim3(nblk(n, 256)), dim3(256), 0, st, d, M, a);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_contravariant(const Dims& d, const Metrics& M, const Ranges& r, const double* U2,
+                         const double* U3, double* U2b, double* U3b, hipStream_t st) {
+  size_t n = (size_t)d.NX2 * (r.ze - r.zs + 2) * (r.je - r.js + 2);
+  hipLaunchKernelGGL(k_contravariant, dim3(nblk(n, 256)), dim3(256), 0, st, d, M, r, U2, U3, U2b, U3b);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_flux(const Dims& d, const Metrics& M, const Ranges& r, const double* U1, const double* Q,
+                const double* U2b, const double* U3b, double nu, double* Sx, double* Syz,
+                hipStream_t st) {
+  hipLaunchKernelGGL(k_flux, dim3(nblk(d.npts, 256)), dim3(256), 0, st, d, M, r, U1, Q, U2b, U3b, nu, Sx, Syz);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_accum(const Dims& d, const Ranges& r, const double* kx, const double* Sx,
+                 const double* Syz, double* CF, double* CR, double temp5, hipStream_t st) {
+  size_t n = (size_t)d.NXC * (r.ze - r.zs + 1) * (r.je - r.js + 1);
+  hipLaunchKernelGGL(k_accum, dim3(nblk(n, 256)), dim3(256), 0, st, d, r, kx, (const double2*)Sx,
+                     (const double2*)Syz, (double2*)CF, (double2*)CR, temp5);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_adi(const Dims& d, const Metrics& M, const Ranges& r, const BcTable& by,
+               const BcTable& bz, double t1, double* U, double* R, double* S, double* lu,
+               int th_corners, hipStream_t st) {
+  size_t n = (size_t)d.NX2 * (r.ze - r.zs + 1) * (r.je - r.js + 1);
+  hipLaunchKernelGGL(k_adi_pre, dim3(nblk(n, 256)), dim3(256), 0, st, d, M, r, U, R, S, t1);
+  EKM_CHECK(hipGetLastError());
+  size_t ly = (size_t)d.NZ2 * d.NY2;  // (line, row) entries
+  double *my = lu, *py = lu + ly, *uy = lu + 2 * ly;
+  double *mz = lu + 3 * ly, *pz = lu + 4 * ly, *uz = lu + 5 * ly, *spa = lu + 6 * ly;
+  hipLaunchKernelGGL(k_lu, dim3(nblk(r.ze - r.zs + 1, 64)), dim3(64), 0, st, d, M, r, by, t1, 0, my, py, uy, spa);
+  EKM_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_lu, dim3(nblk(r.je - r.js + 1, 64)), dim3(64), 0, st, d, M, r, bz, t1, 1, mz, pz, uz, spa);
+  EKM_CHECK(hipGetLastError());
+  size_t ny_ = (size_t)d.NX2 * (r.ze - r.zs + 1);
+  hipLaunchKernelGGL(k_ysweep, dim3(nblk(ny_, 256)), dim3(256), 0, st, d, M, r, by, t1, my, py, uy, U, R,
+                     (const double*)S);
+  EKM_CHECK(hipGetLastError());
+  size_t nz_ = (size_t)d.NX2 * (r.je - r.js + 1);
+  hipLaunchKernelGGL(k_zsweep, dim3(nblk(nz_, 256)), dim3(256), 0, st, d, M, r, bz, mz, pz, uz,
+                     (const double*)spa, U, (const double*)R);
+  EKM_CHECK(hipGetLastError());
+  if (th_corners) {
+    hipLaunchKernelGGL(k_th_corners, dim3(nblk(d.NX2, 256)), dim3(256), 0, st, d, U, th_corners);
+    EKM_CHECK(hipGetLastError());
+  }
+  return EKM_OK;
+}
+int launch_cfl(const Dims& d, const Metrics& M, const Ranges& r, const double* U1, const double* U2,
+               const double* U3, double cfl, double dx, unsigned long long* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_cfl, dim3(1024), dim3(256), 0, st, d, M, r, U1, U2, U3, cfl, dx, out);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_faces(const Dims& d, const Metrics& M, const Ranges& r, const double* CU2, const double* CU3,
+                 double* F2, double* F3, hipStream_t st) {
+  size_t n = (size_t)d.NXC * d.NZ2 * d.NY2;
+  hipLaunchKernelGGL(k_faces, dim3(nblk(n, 256)), dim3(256), 0, st, d, M, r, (const double2*)CU2,
+                     (const double2*)CU3, (double2*)F2, (double2*)F3);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_mg_gather(const Dims& d, const Metrics& M, const Ranges& r, const double* kx, const double* CU1,
+                     const double* F2, const double* F3, double* rhs, int nmode, hipStream_t st) {
+  size_t n = (size_t)nmode * d.NZ2 * d.NY2;
+  hipLaunchKernelGGL(k_mg_gather, dim3(nblk(n, 256)), dim3(256), 0, st, d, M, r, kx, (const double2*)CU1,
+                     (const double2*)F2, (const double2*)F3, rhs, nmode);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_mg_scatter(const Dims& d, const double* phi, int nmode, int fp32, double* PHI, hipStream_t st) {
+  size_t n = (size_t)d.NXC * d.NZ2 * d.NY2;
+  hipLaunchKernelGGL(k_mg_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, d, phi, nmode, fp32, (double2*)PHI);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_project(const Dims& d, const Metrics& M, const Ranges& r, const double* kx, const double* PHI,
+                   const double* F2, const double* F3, double* CU1, double* CU2, double* CU3, double* DIV,
+                   hipStream_t st) {
+  size_t n = (size_t)d.NXC * d.NZ2 * d.NY2;
+  hipLaunchKernelGGL(k_project, dim3(nblk(n, 256)), dim3(256), 0, st, d, M, r, kx, (const double2*)PHI,
+                     (const double2*)F2, (const double2*)F3, (double2*)CU1, (double2*)CU2, (double2*)CU3,
+                     (double2*)DIV);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_pupdate(const Dims& d, const Ranges& r, const double* PHI, double* CP, double temp4, hipStream_t st) {
+  size_t n = (size_t)d.NXC * (r.ze - r.zs + 3) * (r.je - r.js + 3);
+  hipLaunchKernelGGL(k_pupdate, dim3(nblk(n, 256)), dim3(256), 0, st, d, r, (const double2*)PHI, (double2*)CP,
+                     temp4);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_div_reduce(const Dims& d, const double* S, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_div_reduce, dim3(512), dim3(256), 0, st, d, S, out);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_thomas_generic(const double* A, const double* B, const double* C, double* G, double* W, int ny,
+                          int nx, hipStream_t st) {
+  hipLaunchKernelGGL(k_thomas_generic, dim3(nblk((size_t)nx + 1, 256)), dim3(256), 0, st, A, B, C, G, W, ny, nx);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}

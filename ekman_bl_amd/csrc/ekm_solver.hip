@@ -1,0 +1,528 @@
+// ekm_solver.hip -- device-resident duct solver and the C ABI
+// (include/ekman_bl_amd.h).  One solver owns one HIP stream and all of its
+// fields; host<->device copies happen only in set/get calls.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ekm_host.h"
+#include "ekm_internal.h"
+#include "ekm_kernels.h"
+
+static thread_local std::string g_last_error;
+
+void ekm_note_hip_error(hipError_t e, const char* what, const char* file, int line) {
+  char buf[512];
+  std::snprintf(buf, sizeof(buf), "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+  g_last_error = buf;
+  std::fprintf(stderr, "ekman_bl_amd: %s\n", buf);
+}
+
+struct ekm_solver {
+  ekm_params p;
+  Dims d;
+  Ranges rv, rt;
+  hipStream_t st = nullptr;
+  FftPlan fft;
+  MgPlan mg;
+  HostMetrics hm;
+  double* met = nullptr;
+  Metrics M;
+  double *kx = nullptr, *kx2 = nullptr, *kx2s = nullptr, *thbar = nullptr;
+  double* U[3] = {nullptr, nullptr, nullptr};
+  double *P = nullptr, *TH = nullptr;
+  double* CF[3] = {nullptr, nullptr, nullptr};
+  double* CFTH = nullptr;
+  double* R[3] = {nullptr, nullptr, nullptr};
+  double* RTH = nullptr;
+  double *S1 = nullptr, *S2 = nullptr, *U2b = nullptr, *U3b = nullptr, *PHI = nullptr;
+  double* lu = nullptr;
+  double* red = nullptr;  // [0] div sum, [1] div max (bits), [2] cfl min (bits)
+  BcTable bcy[4], bcz[4];
+  double time = 0.0, delta_t = 0.0, delta_t_set = 0.0;
+  double h_bar[3], beta_bar[3], zeta_bar[3];
+  int rk_step = 1, time_step = 0;
+  ekm_diag diag;
+  std::vector<int> cyc_host;
+};
+
+extern "C" {
+
+const char* ekm_version(void) { return "ekman_bl_amd 0.1.0 gfx950"; }
+const char* ekm_last_error(void) { return g_last_error.c_str(); }
+
+void ekm_params_default(ekm_params* p) {
+  std::memset(p, 0, sizeof(*p));
+  p->nu = 1.0e-3;
+  p->lx = 4.0 * M_PI;
+  p->ly = 20.0;
+  p->lz = 2.0 * M_PI;
+  p->delta_t = 0.1;
+  p->variable_dt = 1;
+  p->cfl = 0.5;
+  p->update_dt = 1;
+  p->f_type = 3;
+  p->pr = 1.0;
+  p->mg_maxcy = 6;
+  p->mg_tolmax = 1.0e-6;
+  for (int s = 0; s < 4; ++s) p->mudpack_bc[s] = 2;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+static int check_params(const ekm_params& p) {
+  if (p.nx < 8 || p.nx > 1024 || (p.nx & (p.nx - 1))) return EKM_ERR_UNSUPPORTED;
+  if (p.ny < 4 || p.nz < 4) return EKM_ERR_ARG;
+  if (p.n_th < 0 || p.n_th > 1) return EKM_ERR_UNSUPPORTED;
+  if (p.nz + 2 != p.mg_ixp * (1 << (p.mg_iex - 1)) + 1) return EKM_ERR_ARG;   // src/mg_solver2d.F90:138-140
+  if (p.ny + 2 != p.mg_jyq * (1 << (p.mg_jey - 1)) + 1) return EKM_ERR_ARG;
+  if (p.mg_ixp < 2 || p.mg_jyq < 2 || p.mg_maxcy < 1) return EKM_ERR_ARG;
+  if (p.f_type != 1 && p.f_type != 2 && p.f_type != 3) return EKM_ERR_UNSUPPORTED;
+  for (int s = 0; s < 4; ++s)
+    if (p.mudpack_bc[s] < 0 || p.mudpack_bc[s] > 2) return EKM_ERR_UNSUPPORTED;
+  if ((p.mudpack_bc[0] == 0) != (p.mudpack_bc[1] == 0)) return EKM_ERR_ARG;
+  if ((p.mudpack_bc[2] == 0) != (p.mudpack_bc[3] == 0)) return EKM_ERR_ARG;
+  return EKM_OK;
+}
+
+// INIT_DUCT (src/duct.F90:7-232) for IC_TYPE = 7
+static void init_ranges(const ekm_params& p, Ranges& rv, Ranges& rt) {
+  int NY = p.ny, NZ = p.nz;
+  int wy0 = p.u_bc[2][EKM_YMIN], wy1 = p.u_bc[2][EKM_YMAX];
+  int uz0 = p.u_bc[0][EKM_ZMIN], uz1 = p.u_bc[0][EKM_ZMAX];
+  rv.js = (wy0 == 0 || wy0 == 1 || wy0 == 6 || wy0 == 7 || wy0 == 3) ? 1 : 2;
+  rv.zs = (uz0 == 1 || uz0 == 6) ? 1 : 2;
+  rv.je = (wy1 == 0 || wy1 == 1 || wy1 == 6) ? NY : NY - 1;
+  rv.ze = (uz1 == 0 || uz1 == 1 || uz1 == 6) ? NZ : NZ - 1;
+  int ty0 = p.th_bc[EKM_YMIN], ty1 = p.th_bc[EKM_YMAX];
+  int tz0 = p.th_bc[EKM_ZMIN], tz1 = p.th_bc[EKM_ZMAX];
+  rt.js = (ty0 == 0 || ty0 == 1 || ty0 == 5 || ty0 == 2 || ty0 == 6 || ty0 == 7 || ty0 == 8) ? 1 : 2;
+  rt.zs = (tz0 == 0 || tz0 == 1 || tz0 == 6) ? 1 : 2;
+  rt.je = (ty1 == 0 || ty1 == 1 || ty1 == 6 || ty1 == 5) ? NY : NY - 1;
+  rt.ze = (tz1 == 0 || tz1 == 1 || tz1 == 5 || tz1 == 6) ? NZ : NZ - 1;
+}
+
+static void set_hbar(ekm_solver* s) {
+  s->h_bar[0] = s->delta_t * (8.0 / 15.0);
+  s->h_bar[1] = s->delta_t * (2.0 / 15.0);
+  s->h_bar[2] = s->delta_t * (5.0 / 15.0);
+}
+
+static int free_all(ekm_solver* s) {
+  double* f[] = {s->U[0], s->U[1], s->U[2], s->P, s->TH, s->CF[0], s->CF[1], s->CF[2], s->CFTH,
+                 s->R[0], s->R[1], s->R[2], s->RTH, s->S1, s->S2, s->U2b, s->U3b, s->PHI, s->lu,
+                 s->red, s->met, s->kx, s->kx2, s->kx2s, s->thbar};
+  for (double* q : f)
+    if (q) (void)hipFree(q);
+  fft_plan_free(s->fft);
+  mg_plan_free(s->mg);
+  if (s->st) (void)hipStreamDestroy(s->st);
+  return EKM_OK;
+}
+
+static int fwd(ekm_solver* s, double* f) { return fft_forward(s->fft, f, (long)s->d.plane, s->st); }
+static int inv(ekm_solver* s, double* f) { return fft_inverse(s->fft, f, (long)s->d.plane, s->st); }
+
+#define TRY(x)                \
+  do {                        \
+    int e__ = (x);            \
+    if (e__ != EKM_OK) return e__; \
+  } while (0)
+
+extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const double* ypoint, const double* th_bar,
+                          ekm_solver** out) {
+  if (!pp || !xpoint || !ypoint || !out) return EKM_ERR_ARG;
+  *out = nullptr;
+  TRY(check_params(*pp));
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return EKM_ERR_NODEV;
+  ekm_solver* s = new ekm_solver();
+  s->p = *pp;
+  const ekm_params& p = s->p;
+  Dims& d = s->d;
+  d.NX = p.nx; d.NY = p.ny; d.NZ = p.nz;
+  d.NXC = p.nx / 2 + 1; d.NKX = p.nx / 3;
+  d.NX2 = p.nx + 2; d.NY2 = p.ny + 2; d.NZ2 = p.nz + 2;
+  d.NZ3 = p.nz + 3; d.NY3 = p.ny + 3;
+  d.npts = (size_t)d.NX2 * d.NZ2 * d.NY2;
+  d.plane = (size_t)d.NZ2 * d.NY2;
+  d.mplane = (size_t)d.NZ3 * d.NY3;
+  init_ranges(p, s->rv, s->rt);
+  int st = EKM_OK;
+  for (int c = 0; c < 4 && st == EKM_OK; ++c) {
+    if (c == 3 && !p.n_th) break;
+    st = host_bc_table(p, c, 0, s->bcy[c]);
+    if (st == EKM_OK) st = host_bc_table(p, c, 1, s->bcz[c]);
+  }
+  if (st != EKM_OK) { delete s; return st; }
+  auto fail = [&](int e) { free_all(s); delete s; return e; };
+  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) return fail(EKM_ERR_HIP);
+  // metrics (JACOBI_TRANS) on the host, then uploaded once
+  host_metrics(d.NZ, d.NY, xpoint, ypoint, s->hm);
+  size_t mn = d.mplane;
+  if (hipMalloc(&s->met, 15 * mn * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
+  for (int w = 0; w < 7; ++w)
+    if (hipMemcpy(s->met + (size_t)w * 2 * mn, s->hm.m[w].data(), 2 * mn * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(EKM_ERR_HIP);
+  if (hipMemcpy(s->met + 14 * mn, s->hm.jac.data(), mn * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(EKM_ERR_HIP);
+  s->M = Metrics{s->met, s->met + 2 * mn, s->met + 4 * mn, s->met + 6 * mn, s->met + 8 * mn,
+                 s->met + 10 * mn, s->met + 12 * mn, s->met + 14 * mn};
+  // wavenumbers (INIT_FFT: KXP, KX2P; src/ALLOCATION.F90:63-65)
+  std::vector<double> kx(d.NXC), kx2(d.NXC), kx2s(2 * (d.NKX + 1));
+  for (int m = 0; m < d.NXC; ++m) {
+    kx[m] = 2.0 * M_PI * m / p.lx;
+    kx2[m] = kx[m] * kx[m];
+  }
+  for (int m = 0; m <= d.NKX; ++m) kx2s[2 * m] = kx2s[2 * m + 1] = kx2[m];
+  if (hipMalloc(&s->kx, d.NXC * sizeof(double)) != hipSuccess || hipMalloc(&s->kx2, d.NXC * sizeof(double)) != hipSuccess ||
+      hipMalloc(&s->kx2s, kx2s.size() * sizeof(double)) != hipSuccess)
+    return fail(EKM_ERR_HIP);
+  (void)hipMemcpy(s->kx, kx.data(), d.NXC * sizeof(double), hipMemcpyHostToDevice);
+  (void)hipMemcpy(s->kx2, kx2.data(), d.NXC * sizeof(double), hipMemcpyHostToDevice);
+  (void)hipMemcpy(s->kx2s, kx2s.data(), kx2s.size() * sizeof(double), hipMemcpyHostToDevice);
+  // background scalar (src/diablo.F90:1152-1160)
+  std::vector<double> tb(d.plane, 0.0);
+  for (int j = 0; j < d.NY2; ++j)
+    for (int k = 0; k < d.NZ2; ++k) {
+      size_t q = (size_t)k + (size_t)d.NZ2 * j;
+      tb[q] = p.cont_strat ? (-ypoint[q] + ypoint[(size_t)d.NZ2 * (d.NY + 1)]) : (th_bar ? th_bar[q] : 0.0);
+    }
+  if (hipMalloc(&s->thbar, d.plane * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
+  (void)hipMemcpy(s->thbar, tb.data(), d.plane * sizeof(double), hipMemcpyHostToDevice);
+  // fields
+  double** f[] = {&s->U[0], &s->U[1], &s->U[2], &s->P, &s->TH, &s->CF[0], &s->CF[1], &s->CF[2], &s->CFTH,
+                  &s->R[0], &s->R[1], &s->R[2], &s->RTH, &s->S1, &s->S2, &s->U2b, &s->U3b, &s->PHI};
+  for (double** q : f) {
+    if (hipMalloc(q, d.npts * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
+    if (hipMemset(*q, 0, d.npts * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
+  }
+  if (hipMalloc(&s->lu, 7 * d.plane * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
+  if (hipMalloc(&s->red, 4 * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
+  if (fft_plan_init(s->fft, p.nx) != EKM_OK) return fail(EKM_ERR_HIP);
+  int nsolve = 2 * (d.NKX + 1);
+  st = mg_plan_build(s->mg, p, d, s->hm, nsolve, 512, 8192);
+  if (st != EKM_OK) return fail(st);
+  s->cyc_host.resize(nsolve);
+  s->delta_t = p.delta_t;
+  s->delta_t_set = p.delta_t;
+  set_hbar(s);
+  s->beta_bar[0] = 1.0; s->beta_bar[1] = 25.0 / 8.0; s->beta_bar[2] = 9.0 / 4.0;
+  s->zeta_bar[0] = 0.0; s->zeta_bar[1] = -17.0 / 8.0; s->zeta_bar[2] = -5.0 / 4.0;
+  std::memset(&s->diag, 0, sizeof(s->diag));
+  if (hipStreamSynchronize(s->st) != hipSuccess) return fail(EKM_ERR_HIP);
+  *out = s;
+  return EKM_OK;
+}
+
+extern "C" int ekm_destroy(ekm_solver* s) {
+  if (!s) return EKM_OK;
+  (void)hipStreamSynchronize(s->st);
+  free_all(s);
+  delete s;
+  return EKM_OK;
+}
+
+extern "C" int ekm_sync(ekm_solver* s) {
+  EKM_CHECK(hipStreamSynchronize(s->st));
+  return EKM_OK;
+}
+
+extern "C" int ekm_field_device_ptr(ekm_solver* s, int f, void** dptr) {
+  double* t[5] = {s->U[0], s->U[1], s->U[2], s->P, s->TH};
+  if (f < 0 || f > 4) return EKM_ERR_ARG;
+  *dptr = t[f];
+  return EKM_OK;
+}
+
+extern "C" int ekm_set_state(ekm_solver* s, const double* cu1, const double* cu2, const double* cu3,
+                             const double* cp, const double* cth, double time, int time_step) {
+  size_t n = s->d.npts * sizeof(double);
+  const double* src[4] = {cu1, cu2, cu3, cp};
+  double* dst[4] = {s->U[0], s->U[1], s->U[2], s->P};
+  for (int q = 0; q < 4; ++q) EKM_CHECK(hipMemcpyAsync(dst[q], src[q], n, hipMemcpyHostToDevice, s->st));
+  if (s->p.n_th && cth) EKM_CHECK(hipMemcpyAsync(s->TH, cth, n, hipMemcpyHostToDevice, s->st));
+  EKM_CHECK(hipStreamSynchronize(s->st));
+  s->time = time;
+  s->time_step = time_step;
+  s->rk_step = 1;
+  return EKM_OK;
+}
+
+extern "C" int ekm_get_state(ekm_solver* s, double* cu1, double* cu2, double* cu3, double* cp, double* cth) {
+  size_t n = s->d.npts * sizeof(double);
+  double* dst[4] = {cu1, cu2, cu3, cp};
+  double* src[4] = {s->U[0], s->U[1], s->U[2], s->P};
+  for (int q = 0; q < 4; ++q)
+    if (dst[q]) EKM_CHECK(hipMemcpyAsync(dst[q], src[q], n, hipMemcpyDeviceToHost, s->st));
+  if (cth && s->p.n_th) EKM_CHECK(hipMemcpyAsync(cth, s->TH, n, hipMemcpyDeviceToHost, s->st));
+  EKM_CHECK(hipStreamSynchronize(s->st));
+  return EKM_OK;
+}
+
+// REM_DIV_CURV (src/duct.F90:2472-2818); phi is left in s->PHI
+static int rem_div(ekm_solver* s) {
+  const Dims& d = s->d;
+  hipStream_t st = s->st;
+  int nmode = d.NKX + 1, nsolve = 2 * nmode;
+  TRY(launch_faces(d, s->M, s->rv, s->U[1], s->U[2], s->U2b, s->U3b, st));
+  TRY(launch_mg_gather(d, s->M, s->rv, s->kx, s->U[0], s->U2b, s->U3b, s->mg.rhs, nmode, st));
+  EKM_CHECK(hipMemcpyAsync(s->mg.kx2, s->kx2s, nsolve * sizeof(double), hipMemcpyDeviceToDevice, st));
+  TRY(mg_solve_ex(s->mg, d, nsolve, 0, st));
+  TRY(launch_mg_scatter(d, s->mg.phi, nmode, s->p.cmplx_fp32, s->PHI, st));
+  TRY(launch_project(d, s->M, s->rv, s->kx, s->PHI, s->U2b, s->U3b, s->U[0], s->U[1], s->U[2], s->S1, st));
+  TRY(inv(s, s->S1));
+  EKM_CHECK(hipMemsetAsync(s->red, 0, 2 * sizeof(double), st));
+  TRY(launch_div_reduce(d, s->S1, s->red, st));
+  double h[2];
+  EKM_CHECK(hipMemcpyAsync(h, s->red, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  EKM_CHECK(hipMemcpyAsync(s->cyc_host.data(), s->mg.cycles, nsolve * sizeof(int), hipMemcpyDeviceToHost, st));
+  EKM_CHECK(hipStreamSynchronize(st));
+  s->diag.div_ref = h[0] / ((double)(d.NX + 1) * (double)d.NY * (double)d.NZ);
+  s->diag.div_max = h[1];
+  int sum = 0, mx = 0;
+  for (int c : s->cyc_host) { sum += c; mx = c > mx ? c : mx; }
+  s->diag.mg_solves = nsolve;
+  s->diag.mg_cycles = sum;
+  s->diag.mg_max_cycles = mx;
+  if (std::isnan(s->diag.div_ref)) return EKM_ERR_DIV_NAN;
+  if (s->delta_t < 1.0e-4) return EKM_ERR_DT_SMALL;
+  return EKM_OK;
+}
+
+extern "C" int ekm_rem_div_curv(ekm_solver* s) { return rem_div(s); }
+
+extern "C" int ekm_get_phi(ekm_solver* s, double* phi) {
+  EKM_CHECK(hipMemcpyAsync(phi, s->PHI, s->d.npts * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  EKM_CHECK(hipStreamSynchronize(s->st));
+  return EKM_OK;
+}
+
+extern "C" int ekm_set_initial_physical(ekm_solver* s, const double* u1, const double* u2, const double* u3,
+                                        const double* th) {
+  size_t n = s->d.npts * sizeof(double);
+  const double* src[3] = {u1, u2, u3};
+  for (int q = 0; q < 3; ++q) {
+    EKM_CHECK(hipMemcpyAsync(s->U[q], src[q], n, hipMemcpyHostToDevice, s->st));
+    TRY(fwd(s, s->U[q]));
+  }
+  if (s->p.n_th && th) {
+    EKM_CHECK(hipMemcpyAsync(s->TH, th, n, hipMemcpyHostToDevice, s->st));
+    TRY(fwd(s, s->TH));
+  }
+  double* z[] = {s->P, s->CF[0], s->CF[1], s->CF[2], s->CFTH};
+  for (double* q : z) EKM_CHECK(hipMemsetAsync(q, 0, n, s->st));
+  s->time = 0.0;
+  s->time_step = 0;
+  s->rk_step = 1;
+  return rem_div(s);
+}
+
+// RK_DUCT (src/duct.F90:237-2469)
+extern "C" int ekm_rk_duct(ekm_solver* s) {
+  const ekm_params& p = s->p;
+  const Dims& d = s->d;
+  hipStream_t st = s->st;
+  const int rk = s->rk_step, cur_step = s->time_step + 1;
+  const double hb = s->h_bar[rk - 1];
+  const double TEMP1 = p.nu * hb / 2.0, TEMP3 = s->zeta_bar[rk - 1] * hb, TEMP4 = hb;
+  const double TEMP5 = s->beta_bar[rk - 1] * hb;
+  const int th = p.n_th > 0;
+  RhsArgs a;
+  for (int f = 0; f < 3; ++f) {
+    a.cu[f] = (const double2*)s->U[f];
+    a.cr[f] = (double2*)s->R[f];
+    a.cf[f] = (double2*)s->CF[f];
+  }
+  a.cp = (const double2*)s->P;
+  a.cth = (const double2*)s->TH;
+  a.crth = (double2*)s->RTH;
+  a.cfth = (double2*)s->CFTH;
+  a.kx = s->kx;
+  a.kx2 = s->kx2;
+  a.thbar = s->thbar;
+  a.rv = s->rv;
+  a.rt = s->rt;
+  a.rk = rk;
+  a.th = th;
+  a.cont_strat = p.cont_strat;
+  a.cor_mean_only = p.coriolis_mean_only;
+  a.px_on = (p.f_type == 1 || p.f_type == 2);
+  a.temp3 = TEMP3;
+  a.temp4 = TEMP4;
+  a.nu = p.nu;
+  a.f0 = p.f0;
+  a.ug = p.u_bc_c1[0][EKM_YMAX];
+  a.wg = p.wbulk0;
+  a.ri_tau = p.ri_tau;
+  a.nupr = p.nu / p.pr;
+  a.px = p.f_type == 1 ? p.px0 : p.px0 + p.amp_omega0 * std::cos(p.omega0 * s->time);
+  TRY(launch_rhs_fourier(d, s->M, a, st));
+  // velocities and scalar to physical space (:655-671)
+  for (int f = 0; f < 3; ++f) TRY(inv(s, s->U[f]));
+  if (th) TRY(inv(s, s->TH));
+  // momentum fluxes (:683-1161)
+  TRY(launch_contravariant(d, s->M, s->rv, s->U[1], s->U[2], s->U2b, s->U3b, st));
+  for (int f = 0; f < 3; ++f) {
+    TRY(launch_flux(d, s->M, s->rv, s->U[0], s->U[f], s->U2b, s->U3b, p.nu, s->S1, s->S2, st));
+    TRY(fwd(s, s->S1));
+    TRY(fwd(s, s->S2));
+    TRY(launch_accum(d, s->rv, s->kx, s->S1, s->S2, s->CF[f], s->R[f], TEMP5, st));
+  }
+  for (int f = 0; f < 3; ++f) TRY(inv(s, s->R[f]));
+  if (th) {  // (:1169-1625)
+    TRY(launch_flux(d, s->M, s->rt, s->U[0], s->TH, s->U2b, s->U3b, p.nu / p.pr, s->S1, s->S2, st));
+    TRY(fwd(s, s->S1));
+    TRY(fwd(s, s->S2));
+    TRY(launch_accum(d, s->rt, s->kx, s->S1, s->S2, s->CFTH, s->RTH, TEMP5, st));
+    TRY(inv(s, s->RTH));
+    int corners = (p.th_bc[EKM_ZMAX] == 1 ? 1 : 0) | (p.th_bc[EKM_ZMIN] == 6 ? 2 : 0);
+    TRY(launch_adi(d, s->M, s->rt, s->bcy[3], s->bcz[3], TEMP1 / p.pr, s->TH, s->RTH, s->S1, s->lu, corners, st));
+  }
+  // velocity ADI (:1640-2391)
+  for (int f = 0; f < 3; ++f)
+    TRY(launch_adi(d, s->M, s->rv, s->bcy[f], s->bcz[f], TEMP1, s->U[f], s->R[f], s->S1, s->lu, 0, st));
+  // CFL time step at the third substep (:2417-2419)
+  if (p.variable_dt && rk == 3 && (cur_step % p.update_dt) == 0) {
+    double init = 999.0;
+    unsigned long long bits;
+    std::memcpy(&bits, &init, sizeof(bits));
+    EKM_CHECK(hipMemcpyAsync(s->red + 2, &bits, sizeof(bits), hipMemcpyHostToDevice, st));
+    TRY(launch_cfl(d, s->M, s->rv, s->U[0], s->U[1], s->U[2], p.cfl, p.lx / (double)d.NX,
+                   reinterpret_cast<unsigned long long*>(s->red + 2), st));
+    double dt;
+    EKM_CHECK(hipMemcpyAsync(&dt, s->red + 2, sizeof(double), hipMemcpyDeviceToHost, st));
+    EKM_CHECK(hipStreamSynchronize(st));
+    if (dt > 0) {
+      s->delta_t = dt >= s->delta_t_set ? s->delta_t_set : dt;
+      set_hbar(s);
+    }
+  }
+  // back to Fourier space and project (:2421-2455)
+  for (int f = 0; f < 3; ++f) TRY(fwd(s, s->U[f]));
+  if (th) TRY(fwd(s, s->TH));
+  int stat = rem_div(s);
+  // pressure update (:2460-2466)
+  TRY(launch_pupdate(d, s->rv, s->PHI, s->P, TEMP4, st));
+  s->diag.status = stat;
+  if (rk == 3) {
+    s->time = s->time + s->delta_t;
+    s->time_step = cur_step;
+    s->rk_step = 1;
+  } else {
+    s->rk_step = rk + 1;
+  }
+  return stat;
+}
+
+extern "C" int ekm_time_step(ekm_solver* s, int nsteps) {
+  for (int n = 0; n < nsteps; ++n)
+    for (int q = 0; q < 3; ++q) TRY(ekm_rk_duct(s));
+  EKM_CHECK(hipStreamSynchronize(s->st));
+  return EKM_OK;
+}
+
+extern "C" int ekm_get_diag(ekm_solver* s, ekm_diag* dg) {
+  *dg = s->diag;
+  dg->delta_t = s->delta_t;
+  dg->time = s->time;
+  dg->time_step = s->time_step;
+  dg->rk_step = s->rk_step;
+  return EKM_OK;
+}
+
+extern "C" int ekm_multigrid_mudpack(ekm_solver* s, int nsolve, const double* kx2, const double* rhs_tmp, double* phi,
+                                     int* cycles) {
+  if (nsolve < 1 || nsolve > s->mg.nsolve_max) return EKM_ERR_ARG;
+  size_t pl = s->d.plane;
+  EKM_CHECK(hipMemcpyAsync(s->mg.kx2, kx2, nsolve * sizeof(double), hipMemcpyHostToDevice, s->st));
+  EKM_CHECK(hipMemcpyAsync(s->mg.rhs, rhs_tmp, nsolve * pl * sizeof(double), hipMemcpyHostToDevice, s->st));
+  TRY(mg_solve_ex(s->mg, s->d, nsolve, 0, s->st));
+  EKM_CHECK(hipMemcpyAsync(phi, s->mg.phi, nsolve * pl * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  if (cycles) EKM_CHECK(hipMemcpyAsync(cycles, s->mg.cycles, nsolve * sizeof(int), hipMemcpyDeviceToHost, s->st));
+  EKM_CHECK(hipStreamSynchronize(s->st));
+  return EKM_OK;
+}
+
+// ---- component drop-ins -----------------------------------------------------
+extern "C" int ekm_fft_x_to_fourier_dev(double* varp, int nx, int npencil, void* stream) {
+  FftPlan p;
+  TRY(fft_plan_init(p, nx));
+  int st = fft_forward(p, varp, npencil, (hipStream_t)stream);
+  EKM_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  fft_plan_free(p);
+  return st;
+}
+extern "C" int ekm_fft_x_to_physical_dev(double* varp, int nx, int npencil, void* stream) {
+  FftPlan p;
+  TRY(fft_plan_init(p, nx));
+  int st = fft_inverse(p, varp, npencil, (hipStream_t)stream);
+  EKM_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  fft_plan_free(p);
+  return st;
+}
+static int fft_host(double* varp, int nx, int npencil, int inverse) {
+  if (nx < 8 || npencil < 0) return EKM_ERR_ARG;
+  size_t n = (size_t)(nx + 2) * npencil * sizeof(double);
+  double* dv = nullptr;
+  EKM_CHECK(hipMalloc(&dv, n));
+  int st = EKM_OK;
+  if (hipMemcpy(dv, varp, n, hipMemcpyHostToDevice) != hipSuccess) st = EKM_ERR_HIP;
+  if (st == EKM_OK) st = inverse ? ekm_fft_x_to_physical_dev(dv, nx, npencil, nullptr) : ekm_fft_x_to_fourier_dev(dv, nx, npencil, nullptr);
+  if (st == EKM_OK && hipMemcpy(varp, dv, n, hipMemcpyDeviceToHost) != hipSuccess) st = EKM_ERR_HIP;
+  (void)hipFree(dv);
+  return st;
+}
+extern "C" int ekm_fft_x_to_fourier(double* varp, int nx, int npencil) { return fft_host(varp, nx, npencil, 0); }
+extern "C" int ekm_fft_x_to_physical(double* varp, int nx, int npencil) { return fft_host(varp, nx, npencil, 1); }
+
+extern "C" int ekm_thomas_real_dev(const double* a, const double* b, const double* c, double* g, int ny, int nx,
+                                   void* stream) {
+  double* w = nullptr;
+  size_t n = (size_t)(nx + 1) * (ny + 1);
+  EKM_CHECK(hipMalloc(&w, n * sizeof(double)));
+  int st = launch_thomas_generic(a, b, c, g, w, ny, nx, (hipStream_t)stream);
+  if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) st = EKM_ERR_HIP;
+  (void)hipFree(w);
+  return st;
+}
+
+static int thomas_host(double* a, double* b, double* c, double* g, int ny, int nx) {
+  if (nx < 0 || ny < 1) return EKM_ERR_ARG;
+  size_t n = (size_t)(nx + 1) * (ny + 1) * sizeof(double);
+  double* dv = nullptr;
+  EKM_CHECK(hipMalloc(&dv, 4 * n));
+  double *da = dv, *db = (double*)((char*)dv + n), *dc = (double*)((char*)dv + 2 * n), *dg = (double*)((char*)dv + 3 * n);
+  int st = EKM_OK;
+  if (hipMemcpy(da, a, n, hipMemcpyHostToDevice) != hipSuccess || hipMemcpy(db, b, n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dc, c, n, hipMemcpyHostToDevice) != hipSuccess || hipMemcpy(dg, g, n, hipMemcpyHostToDevice) != hipSuccess)
+    st = EKM_ERR_HIP;
+  if (st == EKM_OK) st = ekm_thomas_real_dev(da, db, dc, dg, ny, nx, nullptr);
+  if (st == EKM_OK && hipMemcpy(g, dg, n, hipMemcpyDeviceToHost) != hipSuccess) st = EKM_ERR_HIP;
+  (void)hipFree(dv);
+  return st;
+}
+
+extern "C" int ekm_thomas_real(double* a, double* b, double* c, double* g, int ny, int nx) {
+  return thomas_host(a, b, c, g, ny, nx);
+}
+
+// THOMAS_REAL_SP (src/boundary.F90:4-47): the extra elimination of row K+1
+// by row K is applied on the host copy, then the standard sweep runs on the GPU.
+extern "C" int ekm_thomas_real_sp(double* a, double* b, double* c, double* g, double dd, int k, int ny, int nx) {
+  if (k < 0 || k >= ny) return EKM_ERR_ARG;
+  size_t ld = (size_t)nx + 1;
+  std::vector<double> a2(a, a + ld * (ny + 1)), b2(b, b + ld * (ny + 1)), g2(g, g + ld * (ny + 1));
+  for (int i = 0; i <= nx; ++i) {
+    size_t p = i + ld * k, q = p + ld;
+    a2[q] = a2[q] - b2[p] * dd / a2[p];
+    b2[q] = b2[q] - c[p] * dd / a2[p];
+    g2[q] = g2[q] - g2[p] * dd / a2[p];
+  }
+  int st = thomas_host(a2.data(), b2.data(), c, g2.data(), ny, nx);
+  if (st == EKM_OK) std::memcpy(g, g2.data(), g2.size() * sizeof(double));
+  return st;
+}

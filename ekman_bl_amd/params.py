@@ -1,0 +1,264 @@
+"""Run parameters of the duct time step.
+
+`EkmParams` mirrors ``ekm_params`` in include/ekman_bl_amd.h field for field.
+It holds the part of the reference's compile-time ``grid_def`` and runtime
+``input.dat`` that the RK3 substep reads (reference: src/diablo.F90:242-337,
+rundir_unst/grid_def:6-44).  `read_input_dat` / `read_grid_def` parse those
+two files with the reference's fixed line order, so a run directory of the
+reference can be pointed at this package unchanged.
+
+`config(n)` builds the five benchmark configurations of this build (see
+DESIGN.md for how their physical description maps onto reference knobs).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import re
+from dataclasses import dataclass
+
+YMIN, YMAX, ZMIN, ZMAX = 0, 1, 2, 3
+
+
+class EkmParams(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("n_th", ctypes.c_int),
+        ("mg_ixp", ctypes.c_int), ("mg_jyq", ctypes.c_int),
+        ("mg_iex", ctypes.c_int), ("mg_jey", ctypes.c_int),
+        ("nu", ctypes.c_double), ("lx", ctypes.c_double),
+        ("ly", ctypes.c_double), ("lz", ctypes.c_double),
+        ("delta_t", ctypes.c_double),
+        ("variable_dt", ctypes.c_int),
+        ("cfl", ctypes.c_double),
+        ("update_dt", ctypes.c_int),
+        ("f_type", ctypes.c_int),
+        ("wbulk0", ctypes.c_double), ("px0", ctypes.c_double),
+        ("omega0", ctypes.c_double), ("amp_omega0", ctypes.c_double),
+        ("f0", ctypes.c_double),
+        ("u_bc", (ctypes.c_int * 4) * 3),
+        ("u_bc_c1", (ctypes.c_double * 4) * 3),
+        ("ri_tau", ctypes.c_double), ("pr", ctypes.c_double),
+        ("cont_strat", ctypes.c_int),
+        ("th_bc", ctypes.c_int * 4),
+        ("th_bc_c1", ctypes.c_double * 4),
+        ("mudpack_bc", ctypes.c_int * 4),
+        ("mg_maxcy", ctypes.c_int),
+        ("mg_tolmax", ctypes.c_double),
+        ("cmplx_fp32", ctypes.c_int),
+        ("coriolis_mean_only", ctypes.c_int),
+        ("reserved", ctypes.c_int * 8),
+    ]
+
+    # -- convenience ---------------------------------------------------------
+    @property
+    def shape_phys(self):
+        """(NY+2, NZ+2, NX+2): numpy shape of a physical field (x fastest)."""
+        return (self.ny + 2, self.nz + 2, self.nx + 2)
+
+    @property
+    def shape_fourier(self):
+        """(NY+2, NZ+2, NX/2+1): numpy shape of a complex Fourier field."""
+        return (self.ny + 2, self.nz + 2, self.nx // 2 + 1)
+
+    @property
+    def npts(self):
+        """Grid points as the metric counts them: NX * (NY+2) * (NZ+2)."""
+        return self.nx * (self.ny + 2) * (self.nz + 2)
+
+    def copy(self) -> "EkmParams":
+        p = EkmParams()
+        ctypes.memmove(ctypes.byref(p), ctypes.byref(self), ctypes.sizeof(EkmParams))
+        return p
+
+
+def mg_factors(n: int) -> tuple[int, int]:
+    """(ixp, iex) with n = ixp * 2**(iex-1) + 1 and the smallest ixp >= 2
+    (MUDPACK's size rule, src/mud2cr.f:148-150)."""
+    m = n - 1
+    iex = 1
+    while m % 2 == 0 and m // 2 >= 2:
+        m //= 2
+        iex += 1
+    if m < 2:
+        raise ValueError(f"{n} points do not fit MUDPACK's ixp*2**(iex-1)+1 rule")
+    return m, iex
+
+
+def base_params(nx: int, ny: int, nz: int, n_th: int = 0) -> EkmParams:
+    p = EkmParams()
+    p.nx, p.ny, p.nz, p.n_th = nx, ny, nz, n_th
+    p.mg_ixp, p.mg_iex = mg_factors(nz + 2)
+    p.mg_jyq, p.mg_jey = mg_factors(ny + 2)
+    p.nu, p.lx, p.ly, p.lz = 1.0e-3, 4 * math.pi, 20.0, 2 * math.pi
+    p.delta_t, p.variable_dt, p.cfl, p.update_dt = 0.1, 1, 0.5, 1
+    p.f_type = 3          # no mean-flow pressure forcing
+    p.pr = 1.0
+    p.mg_maxcy, p.mg_tolmax = 6, 1.0e-6
+    for s in range(4):
+        p.mudpack_bc[s] = 2
+    return p
+
+
+@dataclass
+class Config:
+    name: str
+    params: EkmParams
+    gamma: float          # tanh stretching factor of the y grid
+    steps: int            # step count the configuration is quoted on
+    seed: int
+    noise: float          # uniform noise amplitude, fraction of G
+    th_noise: float
+
+
+def _ekman_bcs(p: EkmParams, ug: float) -> None:
+    """Neutral/stratified Ekman layer on the reference's BC codes (IC_TYPE 7):
+    no-slip bottom, geostrophic/free-slip top, free-slip z ends."""
+    U1, U2, U3 = 0, 1, 2
+    p.u_bc[U1][YMIN] = 0; p.u_bc_c1[U1][YMIN] = 0.0
+    p.u_bc[U2][YMIN] = 0; p.u_bc_c1[U2][YMIN] = 0.0
+    p.u_bc[U3][YMIN] = 0; p.u_bc_c1[U3][YMIN] = 0.0
+    # top: U1 carries the geostrophic wind (its C1 doubles as the Coriolis
+    # offset, src/duct.F90:471), V = 0, W stress-free
+    p.u_bc[U1][YMAX] = 0; p.u_bc_c1[U1][YMAX] = ug
+    p.u_bc[U2][YMAX] = 0; p.u_bc_c1[U2][YMAX] = 0.0
+    p.u_bc[U3][YMAX] = 1; p.u_bc_c1[U3][YMAX] = 0.0
+    # free-slip z ends: U1, V Neumann, W = 0
+    for s in (ZMIN, ZMAX):
+        p.u_bc[U1][s] = 1; p.u_bc[U2][s] = 1; p.u_bc[U3][s] = 0
+    # pressure: Neumann at the wall and the z ends (z ends use the generalised
+    # bndcr), Dirichlet at the top so that the kx = 0 problem is not singular
+    p.mudpack_bc[0] = 2
+    p.mudpack_bc[1] = 2
+    p.mudpack_bc[2] = 2
+    p.mudpack_bc[3] = 1
+    # geostrophic offsets on the mean mode only (the reference subtracts them
+    # from every kx mode, src/duct.F90:471-473; both forms are supported)
+    p.coriolis_mean_only = 1
+
+
+def config(n: int) -> Config:
+    """The benchmark configurations 1..5."""
+    if n == 1:
+        p = base_params(64, 63, 63, n_th=0)
+        re_d = 500.0
+        p.nu = 1.0 / re_d
+        p.f0 = 2.0 / re_d
+        p.lx, p.ly, p.lz = 4 * math.pi, 20.0, 2 * math.pi
+        _ekman_bcs(p, 1.0)
+        p.delta_t, p.cfl = 0.2, 0.5
+        return Config("config1_neutral_64x65x65", p, 2.0, 100, 1, 0.05, 0.0)
+    if n in (2, 3, 5):
+        dims = {2: (256, 191, 255), 3: (512, 255, 511), 5: (1024, 383, 767)}[n]
+        re_d = {2: 1000.0, 3: 1000.0, 5: 5000.0}[n]
+        nf = {2: 30.0, 3: 50.0, 5: 50.0}[n]
+        p = base_params(*dims, n_th=1)
+        p.nu = 1.0 / re_d
+        p.f0 = 2.0 / re_d
+        p.lx, p.ly, p.lz = 4 * math.pi, 20.0, 2 * math.pi
+        _ekman_bcs(p, 1.0)
+        p.pr = 0.7
+        p.cont_strat = 1
+        p.ri_tau = (nf * p.f0) ** 2           # N^2 = RI_TAU with TH_BAR = -y
+        p.th_bc[YMIN] = 1; p.th_bc_c1[YMIN] = -1.0e-3   # surface cooling flux
+        p.th_bc[YMAX] = 1; p.th_bc_c1[YMAX] = 0.0
+        p.th_bc[ZMIN] = 1; p.th_bc[ZMAX] = 1
+        p.delta_t, p.cfl = 0.1, 0.5
+        steps = {2: 1000, 3: 500, 5: 2000}[n]
+        return Config(f"config{n}_stratified_{dims[0]}x{dims[1]+2}x{dims[2]+2}", p, 2.0, steps, 2, 0.05, 0.01)
+    raise ValueError(f"no configuration {n}")
+
+
+# ---------------------------------------------------------------------------
+# reference input files
+# ---------------------------------------------------------------------------
+def read_grid_def(path: str) -> dict:
+    """PARAMETER values of a grid_def file (NX, NY, NZ, N_TH, iixp, ...).
+    Integer expressions such as ``129-2`` are evaluated."""
+    out = {}
+    for line in open(path):
+        line = line.split("!")[0]
+        for name, expr in re.findall(r"PARAMETER\s*\(\s*(\w+)\s*=\s*([^,\)]+)", line, re.I):
+            expr = expr.strip()
+            if re.fullmatch(r"[0-9+\-*/ ()]+", expr):
+                out[name.lower()] = int(eval(expr))  # noqa: S307 - digits and operators only
+        m = re.search(r"PARAMETER\s*\((.*)\)", line, re.I)
+        if m and "," in m.group(1):
+            for part in m.group(1).split(","):
+                if "=" in part:
+                    k, v = part.split("=")
+                    v = v.strip()
+                    if re.fullmatch(r"[0-9+\-*/ ()]+", v):
+                        out[k.strip().lower()] = int(eval(v))  # noqa: S307
+    return out
+
+
+def _fortran_tokens(line: str) -> list[str]:
+    return line.replace(",", " ").split()
+
+
+def _fval(tok: str):
+    t = tok.strip().lower()
+    if t.startswith(".t") or t == "true":
+        return True
+    if t.startswith(".f") or t == "false":
+        return False
+    t = t.replace("d", "e")
+    if re.fullmatch(r"[+-]?\d+", t):
+        return int(t)
+    return float(t)
+
+
+def read_input_dat(path: str, p: EkmParams) -> dict:
+    """Fill `p` from an input.dat with the reference's line layout
+    (src/diablo.F90:242-337, version 1.21).  Returns every value read."""
+    lines = [ln for ln in open(path).read().splitlines()]
+    data = [ln for ln in lines if ln.strip() and not ln.lstrip().upper().startswith("C ")
+            and not ln.lstrip().startswith("#") and not ln.lstrip().upper() == "C"]
+    vals = [[_fval(t) for t in _fortran_tokens(ln)] for ln in data]
+    it = iter(vals)
+    out = {}
+    out["version"] = next(it)[0]
+    nu, lx, ly, lz = next(it)[:4]
+    p.nu, p.lx, p.ly, p.lz = nu, lx, ly, lz
+    nts, dt, _reset, vdt, cfl, _dfn, upd = next(it)[:7]
+    p.delta_t, p.variable_dt, p.cfl, p.update_dt = dt, int(vdt), cfl, upd
+    out["n_time_steps"] = nts
+    les, *_ = next(it)
+    out["les"] = les
+    mud_use, a, b, c, d = next(it)[:5]
+    out["mudpack_use"] = mud_use
+    p.mudpack_bc[0], p.mudpack_bc[1], p.mudpack_bc[2], p.mudpack_bc[3] = a, b, c, d
+    next(it)  # I/O
+    ics = next(it)
+    out["ic_type"] = ics[1]
+    ft, wb, px0, om, amp, _beta, f0, sponge = next(it)[:8]
+    p.f_type, p.wbulk0, p.px0, p.omega0, p.amp_omega0, p.f0 = ft, wb, px0, om, amp, f0
+    out["sponge_type"] = sponge
+    # BCs: U,V,W lower; U,V,W upper; U,V,W left; U,V,W right (:284-306)
+    for slot in (YMIN, YMAX, ZMIN, ZMAX):
+        for comp in range(3):
+            row = next(it)
+            p.u_bc[comp][slot] = row[0]
+            p.u_bc_c1[comp][slot] = row[1]
+    next(it)  # stochastic forcing
+    next(it)  # filter
+    if p.n_th:
+        next(it)  # CREATE_FLOW_TH
+        next(it)  # FILTER_TH
+        ri, pr, _bg, cs = next(it)[:4]
+        p.ri_tau, p.pr, p.cont_strat = ri, pr, int(cs)
+        for slot in (YMIN, YMAX, ZMIN, ZMAX):
+            row = next(it)
+            p.th_bc[slot] = row[0]
+            p.th_bc_c1[slot] = row[1]
+    return out
+
+
+def params_from_run_dir(grid_def: str, input_dat: str) -> EkmParams:
+    g = read_grid_def(grid_def)
+    p = base_params(g["nx"], g["ny"], g["nz"], n_th=g.get("n_th", 0))
+    if "iixp" in g:
+        p.mg_ixp, p.mg_jyq, p.mg_iex, p.mg_jey = g["iixp"], g["jjyq"], g["iiex"], g["jjey"]
+    read_input_dat(input_dat, p)
+    return p

@@ -1,0 +1,170 @@
+/*
+ * ekman_bl_amd.h -- C ABI of the MI355X (gfx950) DIABLO duct time step.
+ *
+ * Every entry point takes plain pointers and sizes; no torch or HIP types
+ * appear in the signatures (streams are passed as `void*`, a hipStream_t).
+ * A Fortran driver binds these through ISO_C_BINDING (see the interfaces in
+ * ekman_bl_amd/fortran/ekman_bl_amd_mod.f90); Python binds them with ctypes.
+ *
+ * Array layout (the reference's NP=1 layout, src/modules.F90:265-279):
+ *   physical field  U(0:NX+1, 0:NZ+1, 0:NY+1)  double, x fastest
+ *   Fourier field   CU(0:NX/2, 0:NZ+1, 0:NY+1) complex double (re,im
+ *                   interleaved), kx fastest; same bytes as the physical
+ *                   pencil, so transforms run in place
+ *   element (i,k,j) sits at i + (NX+2)*(k + (NZ+2)*j) (physical) and
+ *   m + (NX/2+1)*(k + (NZ+2)*j) complex words (Fourier).
+ *
+ * Status codes replace the reference's STOP statements.
+ */
+#ifndef EKMAN_BL_AMD_H
+#define EKMAN_BL_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EKM_OK               0
+#define EKM_ERR_ARG         -1  /* bad argument / size mismatch (src/mg_solver2d.F90:138-140) */
+#define EKM_ERR_HIP         -2  /* a HIP runtime call failed */
+#define EKM_ERR_UNSUPPORTED -3  /* boundary-condition type or option outside the supported set */
+#define EKM_ERR_DIV_NAN     -4  /* "DIV is a NaN" (src/duct.F90:2811) */
+#define EKM_ERR_DT_SMALL    -5  /* "DELTA_T is zero": DELTA_T < 1e-4 (src/duct.F90:2812) */
+#define EKM_ERR_NODEV       -6  /* no HIP device visible */
+
+/* Boundary-condition slots, in the order the reference reads them from
+ * input.dat (src/diablo.F90:284-326): lower (y=0), upper (y=top),
+ * left (z=0), right (z=end). */
+#define EKM_YMIN 0
+#define EKM_YMAX 1
+#define EKM_ZMIN 2
+#define EKM_ZMAX 3
+
+/* Run parameters: the subset of grid_def + input.dat that the RK3 substep
+ * reads (src/diablo.F90:242-337; rundir_unst/grid_def:6-44). */
+typedef struct ekm_params {
+  /* grid_def */
+  int nx, ny, nz;          /* NX (Fourier, periodic), NY, NZ interior counts */
+  int n_th;                /* N_TH: 0 or 1 scalar */
+  int mg_ixp, mg_jyq, mg_iex, mg_jey; /* iixp, jjyq, iiex, jjey: NZ+2 = ixp*2^(iex-1)+1, NY+2 = jyq*2^(jey-1)+1 */
+  /* input.dat: parameters and time march */
+  double nu, lx, ly, lz;
+  double delta_t;          /* DELTA_T as read, also DELTA_T_SET (the CFL cap) */
+  int variable_dt;         /* VARIABLE_DT */
+  double cfl;              /* CFL */
+  int update_dt;           /* UPDATE_DT */
+  /* forcing */
+  int f_type;              /* F_TYPE: 0,1,2 (mean-flow forcing on mode 0) */
+  double wbulk0, px0, omega0, amp_omega0, f0;
+  /* velocity BCs: [component 0=U1,1=U2(V),2=U3(W)][slot] type and C1 */
+  int u_bc[3][4];
+  double u_bc_c1[3][4];
+  /* scalar (N_TH = 1) */
+  double ri_tau, pr;
+  int cont_strat;          /* CONSTANT_STRATIFICATION */
+  int th_bc[4];
+  double th_bc_c1[4];
+  /* pressure solver: MUDPACK_BC(1..4) = nxa, nxb, nyc, nyd */
+  int mudpack_bc[4];
+  int mg_maxcy;            /* maxcy, 6 in the reference (src/mg_solver2d.F90:158) */
+  double mg_tolmax;        /* tolmax, 1e-6 in the reference (src/mg_solver2d.F90:188) */
+  /* options that reproduce or remove reference quirks */
+  int cmplx_fp32;          /* 1: round phi through default-kind CMPLX as the reference does (src/duct.F90:2664) */
+  int coriolis_mean_only;  /* 1: subtract the geostrophic offsets from mode 0 only; 0 = reference (all modes, src/duct.F90:471-473) */
+  int reserved[8];
+} ekm_params;
+
+/* Fill with the reference defaults that are not grid-size dependent. */
+void ekm_params_default(ekm_params* p);
+
+/* Diagnostics of the last substep. */
+typedef struct ekm_diag {
+  double div_ref;       /* reference metric: sum |J div u| / ((NX+1)*NY*NZ) over J=2..NY-1, K=2..NZ-1, I=1..NX+1 (src/duct.F90:2790-2800) */
+  double div_max;       /* max |J div u| over the same points */
+  double delta_t;       /* DELTA_T after the last substep */
+  double time;          /* TIME */
+  int time_step;        /* TIME_STEP */
+  int rk_step;          /* next RK_STEP to run (1..3) */
+  int mg_solves;        /* real Poisson solves in the last projection */
+  int mg_cycles;        /* W(2,1) cycles summed over those solves (reference iparm(17)) */
+  int mg_max_cycles;    /* largest cycle count of a single solve */
+  int status;           /* status of the last substep */
+} ekm_diag;
+
+typedef struct ekm_solver ekm_solver;  /* opaque: device-resident duct state */
+
+/* ---- whole-substep drop-in (replaces RK_DUCT, src/duct.F90:237) ---------- */
+
+/* Create a solver on the current HIP device. xpoint/ypoint are GRID_YZ.dat
+ * (z and y of every (K,J), K fastest, (NZ+2)*(NY+2) values; read at
+ * src/diablo.F90:1120-1128). th_bar: (NZ+2)*(NY+2) background scalar when
+ * cont_strat == 0, else NULL (src/diablo.F90:1152-1160). */
+int ekm_create(const ekm_params* p, const double* xpoint, const double* ypoint,
+               const double* th_bar, ekm_solver** out);
+int ekm_destroy(ekm_solver* s);
+
+/* Upload / download the Fourier state CU1, CU2, CU3, CP, CTH (host arrays,
+ * complex interleaved, (NX/2+1)*(NZ+2)*(NY+2) complex each; cth ignored when
+ * N_TH = 0). Replaces READ_FLOW/SAVE_FLOW's record 2 (src/diablo.F90:818-831). */
+int ekm_set_state(ekm_solver* s, const double* cu1, const double* cu2, const double* cu3,
+                  const double* cp, const double* cth, double time, int time_step);
+int ekm_get_state(ekm_solver* s, double* cu1, double* cu2, double* cu3, double* cp, double* cth);
+
+/* Physical-space initial fields (U1,U2,U3,TH on (NX+2)*(NZ+2)*(NY+2) doubles):
+ * forward x-FFT with 2/3 truncation, CP = 0, then the initial projection
+ * (CREATE_FLOW_DUCT, src/duct.F90:3089-3189). */
+int ekm_set_initial_physical(ekm_solver* s, const double* u1, const double* u2,
+                             const double* u3, const double* th);
+
+/* One RKW3+CN substep (RK_DUCT, src/duct.F90:237-2469); advances RK_STEP and,
+ * after the third substep, TIME and TIME_STEP (src/diablo.F90:152-157). */
+int ekm_rk_duct(ekm_solver* s);
+/* n full time steps (three substeps each). */
+int ekm_time_step(ekm_solver* s, int nsteps);
+/* Pressure projection alone (REM_DIV_CURV, src/duct.F90:2472-2818): the
+ * correction phi is left in the solver for ekm_get_phi. */
+int ekm_rem_div_curv(ekm_solver* s);
+int ekm_get_phi(ekm_solver* s, double* phi);
+int ekm_get_diag(ekm_solver* s, ekm_diag* d);
+/* Blocks until queued work on the solver's stream is done. */
+int ekm_sync(ekm_solver* s);
+/* Device pointer of state field f (0=U1,1=U2,2=U3,3=P,4=TH); the field's
+ * representation is Fourier between substeps. */
+int ekm_field_device_ptr(ekm_solver* s, int f, void** dptr);
+
+/* ---- component drop-ins (host pointers: copy in, compute on the GPU, copy out) */
+
+/* FFT_X_TO_FOURIER_OP(varp,cvarp,0,NY+1,0,NZP) (call sites src/mpi_duct.F90:290;
+ * routine in the missing fft.F90): in-place batched real->complex transform
+ * along x of `npencil` pencils of NX reals laid out with stride NX+2, scaled
+ * by 1/NX, modes NX/3+1..NX/2 zeroed (src/mpi_duct.F90:291-294). */
+int ekm_fft_x_to_fourier(double* varp, int nx, int npencil);
+/* FFT_X_TO_PHYSICAL_OP (call sites src/mpi_duct.F90:308): modes 0..NX/3 are
+ * used, the rest ignored; padding entries NX, NX+1 set to 0. */
+int ekm_fft_x_to_physical(double* varp, int nx, int npencil);
+/* THOMAS_REAL(A,B,C,G,NY,NX) (src/boundary.F90:53-84): NX+1 systems of NY+1
+ * unknowns, batch index fastest, A(0:NX,0:NY); solution returned in G; A and B
+ * are left unchanged (the reference clobbers them). */
+int ekm_thomas_real(double* a, double* b, double* c, double* g, int ny, int nx);
+/* THOMAS_REAL_SP(A,B,C,G,D,K,NY,NX) (src/boundary.F90:4-47). */
+int ekm_thomas_real_sp(double* a, double* b, double* c, double* g, double d, int k, int ny, int nx);
+/* MULTIGRID_MUDPACK for `nsolve` right-hand sides at once (src/mg_solver2d.F90:75-201):
+ * rhs_tmp[s] and phi[s] are (NZ+2)*(NY+2) planes (K fastest) of RHS_TMP and
+ * phi, kx2[s] the squared wavenumber each solve uses in cofcr
+ * (src/duct.F90:3900-3934). cycles[s] receives iparm(17). The metrics and BC
+ * flags are the solver's. */
+int ekm_multigrid_mudpack(ekm_solver* s, int nsolve, const double* kx2,
+                          const double* rhs_tmp, double* phi, int* cycles);
+
+/* ---- device-pointer variants (stream = hipStream_t or NULL) -------------- */
+int ekm_fft_x_to_fourier_dev(double* varp, int nx, int npencil, void* stream);
+int ekm_fft_x_to_physical_dev(double* varp, int nx, int npencil, void* stream);
+int ekm_thomas_real_dev(const double* a, const double* b, const double* c, double* g,
+                        int ny, int nx, void* stream);
+
+/* Library version string. */
+const char* ekm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EKMAN_BL_AMD_H */

@@ -1,0 +1,1909 @@
+/*
+ * ekman_oracle.c -- CPU FP64 oracle of the DIABLO duct RK3 substep.
+ *
+ * TEST INFRASTRUCTURE ONLY (see ekman_oracle.h for the pinning list).  A
+ * plain-C restatement of the reference algorithm; each function cites the
+ * reference file:line it follows.  Loops keep the reference's index ranges
+ * and operation order so that the GPU path can be compared against it at
+ * round-off level.  Built with -ffp-contract=off (no fused multiply-add).
+ */
+#include "ekman_oracle.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* context                                                                   */
+/* ------------------------------------------------------------------------ */
+
+typedef struct mg_level {
+  int nx, ny;
+  double* phi; /* (0:nx+1, 0:ny+1) */
+  double* cf;  /* cf(nx,ny,10), Fortran order */
+} mg_level;
+
+typedef struct mg_ctx {
+  int nxa, nxb, nyc, nyd;
+  int ixp, jyq, iex, jey, ngrid, nfx, nfy, maxcy;
+  double xa, xb, yc, yd, tolmax, relmax;
+  int kcycle, iprer, ipost, intpol, itero;
+  mg_level lev[51]; /* 1..ngrid */
+  double* resf;     /* nfx*nfy scratch (wk(kps) in the reference) */
+  double* phcor;    /* (nfx+2)*(nfy+2) scratch */
+  const ora* o;
+  double kx2;
+} mg_ctx;
+
+struct ora {
+  ekm_params p;
+  int NX, NY, NZ, NXC, NKX, NX2, NY2, NZ2, NZ3, NY3;
+  size_t npts; /* doubles per 3D field */
+  /* grid and metrics (JACOBI_TRANS) */
+  double *xp, *yp;               /* (0:NZ+1,0:NY+1) */
+  double* met[7];                /* (0:NZ+2,0:NY+2,1:2): G11 G12 G22 C11 C12 C21 C22 */
+  double* jac;                   /* INT_JACOB (0:NZ+2,0:NY+2) */
+  double* thbar;                 /* TH_BAR (0:NZ+1,0:NY+1) */
+  double *kx, *kx2;              /* KXP, KX2P (0:NX/2) */
+  /* fields: Fourier between substeps, in place with the physical pencil */
+  double *U[3], *P, *TH;
+  double *CF[3], *CFTH, *R[3], *RTH;
+  double *S1, *S2, *U2b, *U3b, *CS1, *PHI;
+  int jstart, jend, zstart, zend, jstart_th, jend_th, zstart_th, zend_th;
+  double time, delta_t, delta_t_set;
+  double h_bar[3], beta_bar[3], zeta_bar[3];
+  int rk_step, time_step;
+  ekm_diag diag;
+};
+
+#define IDX(o, i, k, j) ((size_t)(i) + (size_t)(o)->NX2 * ((size_t)(k) + (size_t)(o)->NZ2 * (size_t)(j)))
+#define CRE(o, m, k, j) (2 * (size_t)(m) + (size_t)(o)->NX2 * ((size_t)(k) + (size_t)(o)->NZ2 * (size_t)(j)))
+#define MI(o, k, j) ((size_t)(k) + (size_t)(o)->NZ3 * (size_t)(j))
+#define MF(o, k, j, f) (MI(o, k, j) + (size_t)(o)->NZ3 * (size_t)(o)->NY3 * (size_t)((f) - 1))
+#define PL(o, k, j) ((size_t)(k) + (size_t)(o)->NZ2 * (size_t)(j))
+
+#define G11(o, k, j, f) ((o)->met[0][MF(o, k, j, f)])
+#define G12(o, k, j, f) ((o)->met[1][MF(o, k, j, f)])
+#define G22(o, k, j, f) ((o)->met[2][MF(o, k, j, f)])
+#define C11(o, k, j, f) ((o)->met[3][MF(o, k, j, f)])
+#define C12(o, k, j, f) ((o)->met[4][MF(o, k, j, f)])
+#define C21(o, k, j, f) ((o)->met[5][MF(o, k, j, f)])
+#define C22(o, k, j, f) ((o)->met[6][MF(o, k, j, f)])
+#define JAC(o, k, j) ((o)->jac[MI(o, k, j)])
+
+static double* dalloc(size_t n) {
+  double* a = (double*)malloc(n * sizeof(double));
+  if (a) memset(a, 0, n * sizeof(double));
+  return a;
+}
+
+/* ------------------------------------------------------------------------ */
+/* x-FFT (the missing fft.F90; conventions from the call sites)              */
+/* ------------------------------------------------------------------------ */
+/*
+ * Forward: c_m = (1/NX) sum_i u_i exp(-2 pi i m i / NX), m = 0..NX/2, modes
+ * NKX+1..NX/2 set to zero (src/mpi_duct.F90:290-294).  Inverse: u_i =
+ * Re c_0 + 2 Re sum_{m=1}^{NKX} c_m exp(+2 pi i m i / NX), padding I = NX,
+ * NX+1 zeroed (src/mpi_duct.F90:303-313).  A length-NX/2 complex radix-2 FFT
+ * plus the usual even/odd split.
+ */
+typedef struct fft_plan {
+  int n, m, nkx;
+  double *wr, *wi;   /* exp(-2 pi i k / m), k < m/2 */
+  double *tr, *ti;   /* exp(-2 pi i k / n), k <= m */
+  int* rev;
+} fft_plan;
+
+static fft_plan* plan_make(int n) {
+  fft_plan* pl = (fft_plan*)calloc(1, sizeof(fft_plan));
+  int m = n / 2, lg = 0;
+  pl->n = n;
+  pl->m = m;
+  pl->nkx = n / 3;
+  while ((1 << lg) < m) lg++;
+  pl->wr = dalloc(m);
+  pl->wi = dalloc(m);
+  pl->tr = dalloc(m + 1);
+  pl->ti = dalloc(m + 1);
+  pl->rev = (int*)malloc(sizeof(int) * m);
+  for (int k = 0; k < m; ++k) {
+    pl->wr[k] = cos(2.0 * M_PI * k / m);
+    pl->wi[k] = -sin(2.0 * M_PI * k / m);
+    int r = 0;
+    for (int b = 0; b < lg; ++b)
+      if (k & (1 << b)) r |= 1 << (lg - 1 - b);
+    pl->rev[k] = r;
+  }
+  for (int k = 0; k <= m; ++k) {
+    pl->tr[k] = cos(2.0 * M_PI * k / n);
+    pl->ti[k] = -sin(2.0 * M_PI * k / n);
+  }
+  return pl;
+}
+
+static void plan_free(fft_plan* pl) {
+  free(pl->wr); free(pl->wi); free(pl->tr); free(pl->ti); free(pl->rev); free(pl);
+}
+
+/* in-place complex FFT of length m (sign -1 if inv == 0, +1 otherwise), unscaled */
+static void cfft(const fft_plan* pl, double* re, double* im, int inv) {
+  int m = pl->m;
+  for (int k = 0; k < m; ++k) {
+    int r = pl->rev[k];
+    if (r > k) {
+      double t = re[k]; re[k] = re[r]; re[r] = t;
+      t = im[k]; im[k] = im[r]; im[r] = t;
+    }
+  }
+  for (int len = 2; len <= m; len <<= 1) {
+    int half = len >> 1, step = m / len;
+    for (int s = 0; s < m; s += len)
+      for (int q = 0; q < half; ++q) {
+        double wr = pl->wr[q * step], wi = inv ? -pl->wi[q * step] : pl->wi[q * step];
+        int a = s + q, b = a + half;
+        double xr = re[b] * wr - im[b] * wi;
+        double xi = re[b] * wi + im[b] * wr;
+        re[b] = re[a] - xr; im[b] = im[a] - xi;
+        re[a] = re[a] + xr; im[a] = im[a] + xi;
+      }
+  }
+}
+
+static void fft_fwd_pencil(const fft_plan* pl, double* v, double* re, double* im) {
+  int n = pl->n, m = pl->m;
+  for (int q = 0; q < m; ++q) { re[q] = v[2 * q]; im[q] = v[2 * q + 1]; }
+  cfft(pl, re, im, 0);
+  double sc = 1.0 / (double)n;
+  for (int k = 0; k <= m; ++k) {
+    int a = k % m, b = (m - k) % m;
+    double zr = re[a], zi = im[a], yr = re[b], yi = -im[b]; /* conj(Z_{m-k}) */
+    double er = 0.5 * (zr + yr), ei = 0.5 * (zi + yi);
+    double orr = 0.5 * (zi - yi), oi = -0.5 * (zr - yr); /* (Z - conj)/(2i) */
+    double xr = er + (pl->tr[k] * orr - pl->ti[k] * oi);
+    double xi = ei + (pl->tr[k] * oi + pl->ti[k] * orr);
+    if (k <= pl->nkx) { v[2 * k] = xr * sc; v[2 * k + 1] = xi * sc; }
+    else { v[2 * k] = 0.0; v[2 * k + 1] = 0.0; }
+  }
+  (void)n;
+}
+
+static void fft_inv_pencil(const fft_plan* pl, double* v, double* re, double* im) {
+  int m = pl->m, nkx = pl->nkx;
+  /* X_k for k=0..m with X_k = 0 above NKX and Im X_0 ignored */
+  for (int k = 0; k < m; ++k) {
+    int b = m - k;
+    double xr = (k <= nkx) ? v[2 * k] : 0.0, xi = (k <= nkx && k > 0) ? v[2 * k + 1] : 0.0;
+    double yr = (b <= nkx) ? v[2 * b] : 0.0, yi = (b <= nkx && b > 0) ? -v[2 * b + 1] : 0.0;
+    if (b == m) { yr = 0.0; yi = 0.0; } /* Nyquist: zero after the 2/3 rule */
+    double er = xr + yr, ei = xi + yi;           /* X_k + conj(X_{m-k}) */
+    double dr = xr - yr, di = xi - yi;           /* X_k - conj(X_{m-k}) */
+    double cr = pl->tr[k], ci = -pl->ti[k];      /* exp(+2 pi i k / n) */
+    double orr = dr * cr - di * ci, oi = dr * ci + di * cr;
+    re[k] = er - oi; /* E + i O */
+    im[k] = ei + orr;
+  }
+  cfft(pl, re, im, 1);
+  for (int q = 0; q < m; ++q) { v[2 * q] = re[q]; v[2 * q + 1] = im[q]; }
+  v[2 * m] = 0.0;
+  v[2 * m + 1] = 0.0;
+}
+
+static void fft_many(double* varp, int nx, long npencil, int inv) {
+  fft_plan* pl = plan_make(nx);
+#pragma omp parallel
+  {
+    double* re = dalloc(nx / 2 + 1);
+    double* im = dalloc(nx / 2 + 1);
+#pragma omp for schedule(static)
+    for (long p = 0; p < npencil; ++p) {
+      double* v = varp + (size_t)p * (size_t)(nx + 2);
+      if (inv) fft_inv_pencil(pl, v, re, im);
+      else fft_fwd_pencil(pl, v, re, im);
+    }
+    free(re);
+    free(im);
+  }
+  plan_free(pl);
+}
+
+void ora_fft_fwd(double* varp, int nx, long npencil) { fft_many(varp, nx, npencil, 0); }
+void ora_fft_inv(double* varp, int nx, long npencil) { fft_many(varp, nx, npencil, 1); }
+
+static void fwd(ora* o, double* f) { ora_fft_fwd(f, o->NX, (long)o->NZ2 * o->NY2); }
+static void inv(ora* o, double* f) { ora_fft_inv(f, o->NX, (long)o->NZ2 * o->NY2); }
+
+/* ------------------------------------------------------------------------ */
+/* Thomas solvers (src/boundary.F90:4-84)                                    */
+/* ------------------------------------------------------------------------ */
+/* A,B,C,G(0:NX,0:NY), batch index first.  A and B are clobbered as in the
+ * reference. */
+void ora_thomas_real(double* a, double* b, double* c, double* g, int ny, int nx) {
+  size_t ld = (size_t)nx + 1;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i <= nx; ++i) {
+      size_t p = i + ld * j, q = p + ld;
+      a[q] = -a[q] / b[p];
+      b[q] = b[q] + a[q] * c[p];
+      g[q] = g[q] + a[q] * g[p];
+    }
+  for (int i = 0; i <= nx; ++i) g[i + ld * ny] = g[i + ld * ny] / b[i + ld * ny];
+  for (int j = ny - 1; j >= 0; --j)
+    for (int i = 0; i <= nx; ++i) {
+      size_t p = i + ld * j;
+      g[p] = (g[p] - c[p] * g[p + ld]) / b[p];
+    }
+}
+
+/* src/boundary.F90:4-47: one extra elimination of row K+1 by row K with
+ * factor D/A(K) before the standard sweep. */
+void ora_thomas_real_sp(double* a, double* b, double* c, double* g, double d, int k,
+                        int ny, int nx) {
+  size_t ld = (size_t)nx + 1;
+  for (int i = 0; i <= nx; ++i) {
+    size_t p = i + ld * k, q = p + ld;
+    a[q] = a[q] - b[p] * d / a[p];
+    b[q] = b[q] - c[p] * d / a[p];
+    g[q] = g[q] - g[p] * d / a[p];
+  }
+  ora_thomas_real(a, b, c, g, ny, nx);
+}
+
+/* ------------------------------------------------------------------------ */
+/* metrics: JACOBI_TRANS (src/diablo.F90:1088-1376)                          */
+/* ------------------------------------------------------------------------ */
+static double trunc8(double v) { return trunc(v * 100000000.0) * 1e-8; } /* :1220-1230 */
+
+static void metrics(ora* o) {
+  int NZ = o->NZ, NY = o->NY;
+  size_t n = (size_t)(NZ + 2) * (NY + 2) * 2; /* (1:NZ+1,1:NY+1,2) stored with 0-based pad */
+  double *xz = dalloc(n), *yz = dalloc(n), *xe = dalloc(n), *ye = dalloc(n);
+  double* ij = dalloc(n);
+#define L(k, j, f) ((size_t)(k) + (size_t)(NZ + 2) * ((size_t)(j) + (size_t)(NY + 2) * (size_t)((f) - 1)))
+#define XP(k, j) (o->xp[PL(o, k, j)])
+#define YP(k, j) (o->yp[PL(o, k, j)])
+  for (size_t q = 0; q < n; ++q) xz[q] = yz[q] = xe[q] = ye[q] = NAN;
+  /* :1166-1190 */
+  for (int j = 1; j <= NY; ++j)
+    for (int k = 1; k <= NZ; ++k) {
+      xz[L(k, j, 1)] = XP(k + 1, j) - XP(k, j);
+      xz[L(k, j, 2)] = 0.250 * (XP(k + 1, j + 1) + XP(k + 1, j) - XP(k - 1, j + 1) - XP(k - 1, j));
+      yz[L(k, j, 1)] = YP(k + 1, j) - YP(k, j);
+      yz[L(k, j, 2)] = 0.250 * (YP(k + 1, j + 1) + YP(k + 1, j) - YP(k - 1, j + 1) - YP(k - 1, j));
+      xe[L(k, j, 2)] = XP(k, j + 1) - XP(k, j);
+      xe[L(k, j, 1)] = 0.250 * (XP(k, j + 1) + XP(k + 1, j + 1) - XP(k, j - 1) - XP(k + 1, j - 1));
+      ye[L(k, j, 1)] = 0.250 * (YP(k, j + 1) + YP(k + 1, j + 1) - YP(k, j - 1) - YP(k + 1, j - 1));
+      ye[L(k, j, 2)] = YP(k, j + 1) - YP(k, j);
+    }
+  /* :1193-1217 linear extrapolation, same statement order */
+  for (int k = 1; k <= NZ + 1; ++k) {
+    xe[L(k, 1, 1)] = 2.0 * xe[L(k, 2, 1)] - xe[L(k, 3, 1)];
+    for (int f = 1; f <= 2; ++f) {
+      xe[L(k, NY + 1, f)] = 2.0 * xe[L(k, NY, f)] - xe[L(k, NY - 1, f)];
+      xz[L(k, NY + 1, f)] = 2.0 * xz[L(k, NY, f)] - xz[L(k, NY - 1, f)];
+    }
+    ye[L(k, 1, 1)] = 2.0 * ye[L(k, 2, 1)] - ye[L(k, 3, 1)];
+    for (int f = 1; f <= 2; ++f) {
+      ye[L(k, NY + 1, f)] = 2.0 * ye[L(k, NY, f)] - ye[L(k, NY - 1, f)];
+      yz[L(k, NY + 1, f)] = 2.0 * yz[L(k, NY, f)] - yz[L(k, NY - 1, f)];
+    }
+  }
+  for (int j = 1; j <= NY + 1; ++j) {
+    xz[L(1, j, 2)] = 2.0 * xz[L(2, j, 2)] - xz[L(3, j, 2)];
+    for (int f = 1; f <= 2; ++f) {
+      xz[L(NZ + 1, j, f)] = 2.0 * xz[L(NZ, j, f)] - xz[L(NZ - 1, j, f)];
+      xe[L(NZ + 1, j, f)] = 2.0 * xe[L(NZ, j, f)] - xe[L(NZ - 1, j, f)];
+    }
+    yz[L(1, j, 2)] = 2.0 * yz[L(2, j, 2)] - yz[L(3, j, 2)];
+    for (int f = 1; f <= 2; ++f) {
+      yz[L(NZ + 1, j, f)] = 2.0 * yz[L(NZ, j, f)] - yz[L(NZ - 1, j, f)];
+      ye[L(NZ + 1, j, f)] = 2.0 * ye[L(NZ, j, f)] - ye[L(NZ - 1, j, f)];
+    }
+  }
+  /* :1220-1230 truncation to 1e-8 */
+  for (int j = 1; j <= NY + 1; ++j)
+    for (int k = 1; k <= NZ + 1; ++k)
+      for (int f = 1; f <= 2; ++f) {
+        xz[L(k, j, f)] = trunc8(xz[L(k, j, f)]);
+        xe[L(k, j, f)] = trunc8(xe[L(k, j, f)]);
+        yz[L(k, j, f)] = trunc8(yz[L(k, j, f)]);
+        ye[L(k, j, f)] = trunc8(ye[L(k, j, f)]);
+      }
+  /* :1233-1264 */
+  for (int q = 0; q < 7; ++q)
+    for (size_t t = 0; t < (size_t)o->NZ3 * o->NY3 * 2; ++t) o->met[q][t] = NAN;
+  for (size_t t = 0; t < (size_t)o->NZ3 * o->NY3; ++t) o->jac[t] = NAN;
+  for (int j = 1; j <= NY + 1; ++j)
+    for (int k = 1; k <= NZ + 1; ++k)
+      for (int f = 1; f <= 2; ++f) {
+        size_t q = L(k, j, f);
+        double J = xz[q] * ye[q] - xe[q] * yz[q];
+        ij[q] = J;
+        double zx = ye[q] / J, ex = -yz[q] / J, zy = -xe[q] / J, ey = xz[q] / J;
+        G11(o, k, j, f) = (zx * zx + zy * zy) * J;
+        G12(o, k, j, f) = (zx * ex + zy * ey) * J;
+        G22(o, k, j, f) = (ex * ex + ey * ey) * J;
+        C11(o, k, j, f) = zx * J;
+        C12(o, k, j, f) = zy * J;
+        C21(o, k, j, f) = ex * J;
+        C22(o, k, j, f) = ey * J;
+      }
+  /* :1268-1292 cell Jacobian */
+  for (int j = 1; j <= NY; ++j)
+    for (int k = 1; k <= NZ; ++k)
+      JAC(o, k, j) = 0.250 * (ij[L(k, j, 1)] + ij[L(k, j, 2)] + ij[L(k, j + 1, 1)] + ij[L(k + 1, j, 2)]);
+  for (int j = 0; j <= NY + 1; ++j) {
+    JAC(o, NZ + 1, j) = 2.0 * JAC(o, NZ, j) - JAC(o, NZ - 1, j);
+    JAC(o, 0, j) = 2.0 * JAC(o, 1, j) - JAC(o, 2, j);
+  }
+  for (int k = 0; k <= NZ + 1; ++k) {
+    JAC(o, k, NY + 1) = 2.0 * JAC(o, k, NY) - JAC(o, k, NY - 1);
+    JAC(o, k, 0) = 2.0 * JAC(o, k, 1) - JAC(o, k, 2);
+  }
+  JAC(o, 0, 0) = 0.5 * (2.0 * JAC(o, 1, 0) - JAC(o, 2, 0) + 2.0 * JAC(o, 0, 1) - JAC(o, 0, 2));
+  JAC(o, NZ + 1, NY + 1) = 0.5 * (2.0 * JAC(o, NZ, NY + 1) - JAC(o, NZ - 1, NY + 1) +
+                                  2.0 * JAC(o, NZ + 1, NY) - JAC(o, NZ + 1, NY - 1));
+  /* :1294-1340 ghost metrics, same statement order (GMAT_12 corner is not
+   * re-set: the reference assigns GMAT_22 twice at :1322-1326) */
+  for (int j = 0; j <= NY + 1; ++j)
+    for (int f = 1; f <= 2; ++f)
+      for (int t = 0; t < 7; ++t) {
+        double* a = o->met[t];
+        a[MF(o, 0, j, f)] = 2.0 * a[MF(o, 1, j, f)] - a[MF(o, 2, j, f)];
+      }
+  for (int k = 0; k <= NZ + 1; ++k)
+    for (int f = 1; f <= 2; ++f)
+      for (int t = 0; t < 7; ++t) {
+        double* a = o->met[t];
+        a[MF(o, k, 0, f)] = 2.0 * a[MF(o, k, 1, f)] - a[MF(o, k, 2, f)];
+      }
+  for (int f = 1; f <= 2; ++f) {
+    /* GMAT_11, GMAT_22, CJOB_11, CJOB_12, CJOB_21, CJOB_22 */
+    int list[6] = {0, 2, 3, 4, 5, 6};
+    for (int t = 0; t < 6; ++t) {
+      double* a = o->met[list[t]];
+      a[MF(o, 0, 0, f)] = 0.5 * (2.0 * a[MF(o, 0, 1, f)] - a[MF(o, 0, 2, f)] +
+                                 2.0 * a[MF(o, 1, 0, f)] - a[MF(o, 2, 0, f)]);
+    }
+  }
+  /* :1343-1376 copies for the multigrid coefficient callback */
+  for (int f = 1; f <= 2; ++f) {
+    for (int q = 0; q < 7; ++q) {
+      double* a = o->met[q];
+      for (int k = 0; k <= NZ + 1; ++k) a[MF(o, k, NY + 2, f)] = a[MF(o, k, NY + 1, f)];
+      for (int j = 0; j <= NY + 1; ++j) a[MF(o, NZ + 2, j, f)] = a[MF(o, NZ + 1, j, f)];
+      a[MF(o, NZ + 2, NY + 2, f)] = a[MF(o, NZ + 1, NY + 1, f)];
+    }
+  }
+#undef L
+#undef XP
+#undef YP
+  free(xz); free(yz); free(xe); free(ye); free(ij);
+}
+
+int ora_get_metric(ora* o, int which, double* out) {
+  if (which < 0 || which > 7) return EKM_ERR_ARG;
+  size_t n = (size_t)o->NZ3 * o->NY3 * (which < 7 ? 2 : 1);
+  memcpy(out, which < 7 ? o->met[which] : o->jac, n * sizeof(double));
+  return EKM_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* MULTIGRID_MUDPACK (src/mg_solver2d.F90:75-255) and the mud2cr solver     */
+/* (src/mud2cr.f, src/mudcom.f), restated for method=0, kcycle=2, iprer=2,  */
+/* ipost=1, intpol=3 (src/mg_solver2d.F90:146-165).                          */
+/* ------------------------------------------------------------------------ */
+
+/* Boundary callback.  The reference bndcr (src/mg_solver2d.F90:206-255)
+ * implements kbdy=3,4 (y boundaries) as dphi/dy = 0: alfa=0, beta=1, gama=0,
+ * gbdy=0.  kbdy=1,2 (mixed at the z ends) is this build's generalisation for
+ * free-slip z ends: dphi/dz = 0, alfa=1, beta=0, gama=0, gbdy=0. */
+static void bndyc(int kbdy, double xory, double* alfa, double* beta, double* gama,
+                  double* gbdy) {
+  (void)xory;
+  if (kbdy == 1 || kbdy == 2) { *alfa = 1.0; *beta = 0.0; }
+  else { *alfa = 0.0; *beta = 1.0; }
+  *gama = 0.0;
+  *gbdy = 0.0;
+}
+
+/* cofcr (src/duct.F90:3900-3934) */
+static void cofcr(const ora* o, double kx2, double x, double y, double* cxx, double* cxy,
+                  double* cyy, double* cx, double* cy, double* ce) {
+  int k = (int)x, j = (int)y;
+  *cxx = 0.25 * (G11(o, k, j, 2) + G11(o, k + 1, j, 2) + G11(o, k, j, 1) + G11(o, k, j + 1, 1));
+  *cxy = 0.5 * (G12(o, k, j, 2) + G12(o, k + 1, j, 2) + G12(o, k, j, 1) + G12(o, k, j + 1, 1));
+  *cyy = 0.25 * (G22(o, k, j, 2) + G22(o, k + 1, j, 2) + G22(o, k, j, 1) + G22(o, k, j + 1, 1));
+  *cx = G11(o, k + 1, j, 2) - G11(o, k, j, 2) + G12(o, k, j + 1, 1) - G12(o, k, j, 1);
+  *cy = G22(o, k, j + 1, 1) - G22(o, k, j, 1) + G12(o, k + 1, j, 2) - G12(o, k, j, 2);
+  *ce = -(kx2 * JAC(o, k, j));
+}
+
+#define COF(x, y) cofcr(c->o, c->kx2, (x), (y), &cxx, &cxy, &cyy, &cx, &cy, &ce)
+#define CF(i, j, l) cf[((i)-1) + (size_t)nx * (((j)-1) + (size_t)ny * ((l)-1))]
+#define PH(i, j) phi[(i) + (size_t)(nx + 2) * (j)]
+
+/* dismd2cr (src/mud2cr.f:501-1270), method 0: no line-relaxation factors */
+static void mg_discretize(mg_ctx* c, int lev) {
+  mg_level* L = &c->lev[lev];
+  int nx = L->nx, ny = L->ny;
+  double* cf = L->cf;
+  double dlx = (c->xb - c->xa) / (nx - 1), dlx2 = dlx + dlx, dlxx = dlx * dlx;
+  double dly = (c->yd - c->yc) / (ny - 1), dly2 = dly + dly, dlyy = dly * dly;
+  double dlxy = dlx * dly, dlxy2 = dlxy + dlxy, dlxy4 = dlxy2 + dlxy2;
+  double dyox = dly / dlx, dxoy = dlx / dly;
+  double cxx, cxy, cyy, cx, cy, ce, c1, c2, c3, c4, c5, c6, c7, c8;
+  double x, y;
+  /* interior and nonspecified boundaries (:560-590) */
+  for (int j = 1; j <= ny; ++j) {
+    y = c->yc + (j - 1) * dly;
+    for (int i = 1; i <= nx; ++i) {
+      x = c->xa + (i - 1) * dlx;
+      COF(x, y);
+      cxx = fmax(cxx, fabs(cx) * dlx * 0.5);
+      cyy = fmax(cyy, fabs(cy) * dly * 0.5);
+      c1 = cxx / dlxx + cx / dlx2;
+      c2 = cxy / dlxy4;
+      c3 = cyy / dlyy + cy / dly2;
+      c4 = -c2;
+      c5 = cxx / dlxx - cx / dlx2;
+      c6 = c2;
+      c7 = cyy / dlyy - cy / dly2;
+      c8 = -c2;
+      CF(i, j, 1) = c1; CF(i, j, 2) = c2; CF(i, j, 3) = c3; CF(i, j, 4) = c4;
+      CF(i, j, 5) = c5; CF(i, j, 6) = c6; CF(i, j, 7) = c7; CF(i, j, 8) = c8;
+      CF(i, j, 9) = ce - (c1 + c3 + c5 + c7);
+    }
+  }
+  /* corners: virtual-virtual points not used (:594-637) */
+  COF(c->xa, c->yc);
+  c4 = -cxy / dlxy2;
+  CF(1, 1, 1) -= c4; CF(1, 1, 2) = 0.0; CF(1, 1, 3) -= c4; CF(1, 1, 4) = c4;
+  CF(1, 1, 5) -= c4; CF(1, 1, 6) = 0.0; CF(1, 1, 7) -= c4; CF(1, 1, 8) = c4;
+  CF(1, 1, 9) += 2.0 * c4;
+  COF(c->xb, c->yc);
+  c2 = cxy / dlxy2;
+  CF(nx, 1, 1) -= c2; CF(nx, 1, 2) = c2; CF(nx, 1, 3) -= c2; CF(nx, 1, 4) = 0.0;
+  CF(nx, 1, 5) -= c2; CF(nx, 1, 6) = c2; CF(nx, 1, 7) -= c2; CF(nx, 1, 8) = 0.0;
+  CF(nx, 1, 9) += 2.0 * c2;
+  COF(c->xa, c->yd);
+  c2 = cxy / dlxy2;
+  CF(1, ny, 1) -= c2; CF(1, ny, 2) = c2; CF(1, ny, 3) -= c2; CF(1, ny, 4) = 0.0;
+  CF(1, ny, 5) -= c2; CF(1, ny, 6) = c2; CF(1, ny, 7) -= c2; CF(1, ny, 8) = 0.0;
+  CF(1, ny, 9) += 2.0 * c2;
+  COF(c->xb, c->yd);
+  c4 = -cxy / dlxy2;
+  CF(nx, ny, 1) -= c4; CF(nx, ny, 2) = 0.0; CF(nx, ny, 3) -= c4; CF(nx, ny, 4) = c4;
+  CF(nx, ny, 5) -= c4; CF(nx, ny, 6) = 0.0; CF(nx, ny, 7) -= c4; CF(nx, ny, 8) = c4;
+  CF(nx, ny, 9) += 2.0 * c4;
+
+  double am1, bm1, gm1, dm1, a0, b0, g0, d0, ap1, bp1, gp1, dp1;
+  /* mixed derivative b.c. along x=xa (:639-679) */
+  if (c->nxa == 2) {
+    bndyc(1, c->yc, &am1, &bm1, &gm1, &dm1);
+    bndyc(1, c->yc + dly, &a0, &b0, &g0, &d0);
+    for (int j = 2; j <= ny - 1; ++j) {
+      y = c->yc + j * dly;
+      bndyc(1, y, &ap1, &bp1, &gp1, &dp1);
+      c4 = CF(1, j, 4); c5 = CF(1, j, 5); c6 = CF(1, j, 6);
+      CF(1, j, 1) = CF(1, j, 1) + c5;
+      CF(1, j, 2) = CF(1, j, 2) + c4;
+      CF(1, j, 3) = CF(1, j, 3) + c6 * (-bm1 / am1 * dxoy) + c5 * (b0 / a0 * dxoy) +
+                    c4 * (3. * bp1 / ap1 * dxoy + dlx2 * gp1 / ap1);
+      CF(1, j, 4) = 0.0; CF(1, j, 5) = 0.0; CF(1, j, 6) = 0.0;
+      CF(1, j, 7) = CF(1, j, 7) + c6 * (-3. * bm1 / am1 * dxoy + dlx2 * gm1 / am1) +
+                    c5 * (-b0 / a0 * dxoy) + c4 * (bp1 / ap1 * dxoy);
+      CF(1, j, 8) = CF(1, j, 8) + c6;
+      CF(1, j, 9) = CF(1, j, 9) + c6 * (4. * bm1 / am1 * dxoy) + c5 * (dlx2 * g0 / a0) +
+                    c4 * (-4. * bp1 / ap1 * dxoy);
+      am1 = a0; bm1 = b0; gm1 = g0; dm1 = d0;
+      a0 = ap1; b0 = bp1; g0 = gp1; d0 = dp1;
+    }
+  }
+  /* along x=xb (:681-720) */
+  if (c->nxb == 2) {
+    bndyc(2, c->yc, &am1, &bm1, &gm1, &dm1);
+    bndyc(2, c->yc + dly, &a0, &b0, &g0, &d0);
+    for (int j = 2; j <= ny - 1; ++j) {
+      y = c->yc + j * dly;
+      bndyc(2, y, &ap1, &bp1, &gp1, &dp1);
+      c1 = CF(nx, j, 1); c2 = CF(nx, j, 2); c8 = CF(nx, j, 8);
+      CF(nx, j, 1) = 0.0; CF(nx, j, 2) = 0.0;
+      CF(nx, j, 3) = CF(nx, j, 3) - c8 * (-bm1 / am1 * dxoy) - c1 * (b0 / a0 * dxoy) -
+                     c2 * (3.0 * bp1 / ap1 * dxoy + dlx2 * gp1 / ap1);
+      CF(nx, j, 4) = CF(nx, j, 4) + c2;
+      CF(nx, j, 5) = CF(nx, j, 5) + c1;
+      CF(nx, j, 6) = CF(nx, j, 6) + c8;
+      CF(nx, j, 7) = CF(nx, j, 7) + c8 * (3.0 * bm1 / am1 * dxoy - dlx2 * gm1 / am1) +
+                     c1 * (b0 / a0 * dxoy) + c2 * (-bp1 / ap1 * dxoy);
+      CF(nx, j, 8) = 0.0;
+      CF(nx, j, 9) = CF(nx, j, 9) - c8 * (4.0 * bm1 / am1 * dxoy) - c1 * (dlx2 * g0 / a0) -
+                     c2 * (-4.0 * bp1 / ap1 * dxoy);
+      am1 = a0; bm1 = b0; gm1 = g0; dm1 = d0;
+      a0 = ap1; b0 = bp1; g0 = gp1; d0 = dp1;
+    }
+  }
+  /* along y=yc (:721-759) */
+  if (c->nyc == 2) {
+    bndyc(3, c->xa, &am1, &bm1, &gm1, &dm1);
+    bndyc(3, c->xa + dlx, &a0, &b0, &g0, &d0);
+    for (int i = 2; i <= nx - 1; ++i) {
+      x = c->xa + i * dlx;
+      bndyc(3, x, &ap1, &bp1, &gp1, &dp1);
+      c6 = CF(i, 1, 6); c7 = CF(i, 1, 7); c8 = CF(i, 1, 8);
+      CF(i, 1, 1) = CF(i, 1, 1) + c6 * (-am1 / bm1 * dyox) + c7 * (a0 / b0 * dyox) +
+                    c8 * (3.0 * ap1 / bp1 * dyox + dly2 * gp1 / bp1);
+      CF(i, 1, 2) = CF(i, 1, 2) + c8;
+      CF(i, 1, 3) = CF(i, 1, 3) + c7;
+      CF(i, 1, 4) = CF(i, 1, 4) + c6;
+      CF(i, 1, 5) = CF(i, 1, 5) + c6 * (-3.0 * am1 / bm1 * dyox + dly2 * gm1 / bm1) +
+                    c7 * (-a0 / b0 * dyox) + c8 * (ap1 / bp1 * dyox);
+      CF(i, 1, 6) = 0.0; CF(i, 1, 7) = 0.0; CF(i, 1, 8) = 0.0;
+      CF(i, 1, 9) = CF(i, 1, 9) + c6 * (4.0 * am1 / bm1 * dyox) + c7 * dly2 * g0 / b0 +
+                    c8 * (-4.0 * ap1 / bp1 * dyox);
+      am1 = a0; bm1 = b0; gm1 = g0; dm1 = d0;
+      a0 = ap1; b0 = bp1; g0 = gp1; d0 = dp1;
+    }
+  }
+  /* along y=yd (:761-801) */
+  if (c->nyd == 2) {
+    bndyc(4, c->xa, &am1, &bm1, &gm1, &dm1);
+    bndyc(4, c->xa + dlx, &a0, &b0, &g0, &d0);
+    for (int i = 2; i <= nx - 1; ++i) {
+      x = c->xa + i * dlx;
+      bndyc(4, x, &ap1, &bp1, &gp1, &dp1);
+      c2 = CF(i, ny, 2); c3 = CF(i, ny, 3); c4 = CF(i, ny, 4);
+      CF(i, ny, 1) = CF(i, ny, 1) + c4 * (am1 / bm1 * dyox) + c3 * (-a0 / b0 * dyox) +
+                     c2 * (-3. * ap1 / bp1 * dyox - dly2 * gp1 / bp1);
+      CF(i, ny, 2) = 0.0; CF(i, ny, 3) = 0.0; CF(i, ny, 4) = 0.0;
+      CF(i, ny, 5) = CF(i, ny, 5) + c4 * (3.0 * am1 / bm1 * dyox - dly2 * gm1 / bm1) +
+                     c3 * (a0 / b0 * dyox) + c2 * (-ap1 / bp1 * dyox);
+      CF(i, ny, 6) = CF(i, ny, 6) + c4;
+      CF(i, ny, 7) = CF(i, ny, 7) + c3;
+      CF(i, ny, 8) = CF(i, ny, 8) + c2;
+      CF(i, ny, 9) = CF(i, ny, 9) + c4 * (-4. * am1 / bm1 * dyox) + c3 * (-dly2 * g0 / b0) +
+                     c2 * (4. * ap1 / bp1 * dyox);
+      am1 = a0; bm1 = b0; gm1 = g0; dm1 = d0;
+      a0 = ap1; b0 = bp1; g0 = gp1; d0 = dp1;
+    }
+  }
+  (void)dm1; (void)d0; (void)dp1;
+  double aa, ba, ga, da, ab, bb, gb, db, ac, bc, gc, dc, ad, bd, gd, dd, det;
+  /* corner adjustments at y=yc (:817-960) */
+  if (c->nyc == 2) {
+    if (c->nxa == 0) {
+      bndyc(3, c->xa, &a0, &b0, &g0, &d0);
+      bndyc(3, c->xa + dlx, &ap1, &bp1, &gp1, &dp1);
+      c7 = CF(1, 1, 7); c8 = CF(1, 1, 8);
+      CF(1, 1, 1) = CF(1, 1, 1) + c7 * (a0 / b0 * dyox) +
+                    c8 * (3.0 * ap1 / bp1 * dyox + dly2 * gp1 / bp1);
+      CF(1, 1, 2) = CF(1, 1, 2) + c8;
+      CF(1, 1, 3) = CF(1, 1, 3) + c7;
+      CF(1, 1, 5) = CF(1, 1, 5) + c7 * (-a0 / b0 * dyox) + c8 * (ap1 / bp1 * dyox);
+      CF(1, 1, 6) = 0.0; CF(1, 1, 7) = 0.0; CF(1, 1, 8) = 0.0;
+      CF(1, 1, 9) = CF(1, 1, 9) + c8 * (-4.0 * ap1 / bp1 * dyox) + c7 * (dly2 * g0 / b0);
+      bndyc(3, c->xb - dlx, &am1, &bm1, &gm1, &dm1);
+      bndyc(3, c->xb, &a0, &b0, &g0, &d0);
+      c6 = CF(nx, 1, 6); c7 = CF(nx, 1, 7);
+      CF(nx, 1, 1) = CF(nx, 1, 1) + c6 * (-am1 / bm1 * dyox) + c7 * (a0 / b0 * dyox);
+      CF(nx, 1, 3) = CF(nx, 1, 3) + c7;
+      CF(nx, 1, 4) = CF(nx, 1, 4) + c6;
+      CF(nx, 1, 5) = CF(nx, 1, 5) + c6 * (-3.0 * am1 / bm1 * dyox + dly2 * gm1 / bm1) +
+                     c7 * (-a0 / b0 * dyox);
+      CF(nx, 1, 6) = 0.0; CF(nx, 1, 7) = 0.0; CF(nx, 1, 8) = 0.0;
+      CF(nx, 1, 9) = CF(nx, 1, 9) + c6 * (4.0 * am1 / bm1 * dyox) + c7 * (dly2 * g0 / b0);
+    } else if (c->nxa == 2) {
+      bndyc(3, c->xa + dlx, &ac, &bc, &gc, &dc);
+      bndyc(1, c->yc + dly, &aa, &ba, &ga, &da);
+      c4 = CF(1, 1, 4); c8 = CF(1, 1, 8);
+      CF(1, 1, 1) = CF(1, 1, 1) + c8 * (3.0 * ac / bc * dyox + dly2 * gc / bc);
+      CF(1, 1, 2) = CF(1, 1, 2) + c8 + c4;
+      CF(1, 1, 3) = CF(1, 1, 3) + c4 * (3.0 * ba / aa * dxoy + dlx2 * ga / aa);
+      CF(1, 1, 4) = 0.0;
+      CF(1, 1, 5) = CF(1, 1, 5) + c8 * (ac / bc * dyox);
+      CF(1, 1, 6) = 0.0;
+      CF(1, 1, 7) = CF(1, 1, 7) + c4 * (ba / aa * dxoy);
+      CF(1, 1, 8) = 0.0;
+      CF(1, 1, 9) = CF(1, 1, 9) + c8 * (-4.0 * ac / bc * dyox) + c4 * (-4.0 * ba / aa * dxoy);
+      c5 = CF(1, 1, 5); c7 = CF(1, 1, 7);
+      bndyc(3, c->xa, &ac, &bc, &gc, &dc);
+      bndyc(1, c->yc, &aa, &ba, &ga, &da);
+      det = aa * bc - ba * ac;
+      CF(1, 1, 1) = CF(1, 1, 1) + c5;
+      CF(1, 1, 3) = CF(1, 1, 3) + c7;
+      CF(1, 1, 5) = 0.0; CF(1, 1, 7) = 0.0;
+      CF(1, 1, 9) = CF(1, 1, 9) + c5 * (dlx2 * (ga * bc - ba * gc) / det) +
+                    c7 * (dly2 * (aa * gc - ga * ac) / det);
+    }
+    if (c->nxb == 2) {
+      bndyc(2, c->yc + dly, &ab, &bb, &gb, &db);
+      bndyc(3, c->xb - dlx, &ac, &bc, &gc, &dc);
+      c2 = CF(nx, 1, 2); c6 = CF(nx, 1, 6);
+      CF(nx, 1, 1) = CF(nx, 1, 1) + c6 * (-ac / bc * dyox);
+      CF(nx, 1, 2) = 0.0;
+      CF(nx, 1, 3) = CF(nx, 1, 3) + c2 * (-3.0 * bb / ab * dxoy - dlx2 * gb / ab);
+      CF(nx, 1, 4) = CF(nx, 1, 4) + c6 + c2;
+      CF(nx, 1, 5) = CF(nx, 1, 5) + c6 * (-3.0 * ac / bc * dyox + dly2 * gc / bc);
+      CF(nx, 1, 6) = 0.0;
+      CF(nx, 1, 7) = CF(nx, 1, 7) - c2 * (bb / ab * dxoy);
+      CF(nx, 1, 8) = 0.0;
+      CF(nx, 1, 9) = CF(nx, 1, 9) + c6 * (4.0 * ac / bc * dyox) + c2 * (4.0 * bb / ab * dxoy);
+      bndyc(3, c->xb, &ac, &bc, &gc, &dc);
+      bndyc(2, c->yc, &ab, &bb, &gb, &db);
+      det = bc * ab - ac * bb;
+      c1 = CF(nx, 1, 1); c7 = CF(nx, 1, 7);
+      CF(nx, 1, 1) = 0.0;
+      CF(nx, 1, 3) = CF(nx, 1, 3) + c7;
+      CF(nx, 1, 5) = CF(nx, 1, 5) + c1;
+      CF(nx, 1, 7) = 0.0;
+      CF(nx, 1, 9) = CF(nx, 1, 9) + c1 * (dlx2 * (bb * gc - gb * bc) / det) +
+                     c7 * (dly2 * (ab * gc - gb * ac) / det);
+    }
+  }
+  /* corner adjustments at y=yd (:963-1087) */
+  if (c->nyd == 2) {
+    if (c->nxa == 0) {
+      bndyc(4, c->xa, &a0, &b0, &g0, &d0);
+      bndyc(4, c->xa + dlx, &ap1, &bp1, &gp1, &dp1);
+      c2 = CF(1, ny, 2); c3 = CF(1, ny, 3);
+      CF(1, ny, 1) = CF(1, ny, 1) + c3 * (-a0 / b0 * dyox) +
+                     c2 * (-3.0 * ap1 / bp1 * dyox - dly2 * gp1 / bp1);
+      CF(1, ny, 2) = 0.0; CF(1, ny, 3) = 0.0; CF(1, ny, 4) = 0.0;
+      CF(1, ny, 5) = CF(1, ny, 5) + c3 * (a0 / b0 * dyox) + c2 * (-ap1 / bp1 * dyox);
+      CF(1, ny, 7) = CF(1, ny, 7) + c3;
+      CF(1, ny, 8) = CF(1, ny, 8) + c2;
+      CF(1, ny, 9) = CF(1, ny, 9) + c3 * (-dly2 * g0 / b0) + c2 * (4.0 * ap1 / bp1 * dyox);
+      bndyc(4, c->xb - dlx, &am1, &bm1, &gm1, &dm1);
+      bndyc(4, c->xb, &a0, &b0, &g0, &d0);
+      c3 = CF(nx, ny, 3); c4 = CF(nx, ny, 4);
+      CF(nx, ny, 1) = CF(nx, ny, 1) + c4 * (am1 / bm1 * dyox) + c3 * (-a0 / b0 * dyox);
+      CF(nx, ny, 2) = 0.0; CF(nx, ny, 3) = 0.0; CF(nx, ny, 4) = 0.0;
+      CF(nx, ny, 5) = CF(nx, ny, 5) + c4 * (3.0 * am1 / bm1 * dyox - dly2 * gm1 / bm1) +
+                      c3 * (a0 / b0 * dyox);
+      CF(nx, ny, 6) = CF(nx, ny, 6) + c4;
+      CF(nx, ny, 7) = CF(nx, ny, 7) + c3;
+      CF(nx, ny, 9) = CF(nx, ny, 9) + c4 * (-4.0 * am1 / bm1 * dyox) + c3 * (-dly2 * g0 / b0);
+    } else if (c->nxa == 2) {
+      bndyc(4, c->xa + dlx, &ad, &bd, &gd, &dd);
+      bndyc(1, c->yd - dly, &aa, &ba, &ga, &da);
+      c2 = CF(1, ny, 2); c6 = CF(1, ny, 6);
+      CF(1, ny, 1) = CF(1, ny, 1) + c2 * (-3.0 * ad / bd * dyox - dly2 * gd / bd);
+      CF(1, ny, 2) = 0.0;
+      CF(1, ny, 3) = CF(1, ny, 3) + c6 * (-ba / aa * dxoy);
+      CF(1, ny, 4) = 0.0;
+      CF(1, ny, 5) = CF(1, ny, 5) + c2 * (-ad / bd * dyox);
+      CF(1, ny, 6) = 0.0;
+      CF(1, ny, 7) = CF(1, ny, 7) + c6 * (-3.0 * ba / aa * dxoy + dlx2 * ga / aa);
+      CF(1, ny, 8) = CF(1, ny, 8) + c6 + c2;
+      CF(1, ny, 9) = CF(1, ny, 9) + c2 * (4.0 * ad / bd * dyox) + c6 * (4.0 * ba / aa * dxoy);
+      bndyc(1, c->yd, &aa, &ba, &ga, &da);
+      bndyc(4, c->xa, &ad, &bd, &gd, &dd);
+      det = ad * ba - bd * aa;
+      c3 = CF(1, ny, 3); c5 = CF(1, ny, 5);
+      CF(1, ny, 1) = CF(1, ny, 1) + c5;
+      CF(1, ny, 3) = 0.0; CF(1, ny, 5) = 0.0;
+      CF(1, ny, 7) = CF(1, ny, 7) + c3;
+      CF(1, ny, 9) = CF(1, ny, 9) + c5 * (dlx2 * (ba * gd - ga * bd) / det) +
+                     c3 * (dly2 * (aa * gd - ga * ad) / det);
+    }
+    if (c->nxb == 2) {
+      bndyc(4, c->xb - dlx, &ad, &bd, &gd, &dd);
+      bndyc(2, c->yd - dly, &ab, &bb, &gb, &db);
+      c4 = CF(nx, ny, 4); c8 = CF(nx, ny, 8);
+      CF(nx, ny, 1) = CF(nx, ny, 1) + c4 * (ad / bd * dyox);
+      CF(nx, ny, 2) = 0.0;
+      CF(nx, ny, 3) = CF(nx, ny, 3) + c8 * (bb / ab * dxoy);
+      CF(nx, ny, 4) = 0.0;
+      CF(nx, ny, 5) = CF(nx, ny, 5) + c4 * (3.0 * ad / bd * dyox - dly2 * gd / bd);
+      CF(nx, ny, 6) = CF(nx, ny, 6) + c4 + c8;
+      CF(nx, ny, 7) = CF(nx, ny, 7) + c8 * (3.0 * bb / ab * dxoy - dlx2 * gb / ab);
+      CF(nx, ny, 8) = 0.0;
+      CF(nx, ny, 9) = CF(nx, ny, 9) + c4 * (-4.0 * ad / bd * dyox) + c8 * (-4.0 * bb / ab * dxoy);
+      bndyc(4, c->xb, &ad, &bd, &gd, &dd);
+      bndyc(2, c->yd, &ab, &bb, &gb, &db);
+      det = ad * bb - bd * ab;
+      c1 = CF(nx, ny, 1); c3 = CF(nx, ny, 3);
+      CF(nx, ny, 1) = 0.0; CF(nx, ny, 3) = 0.0;
+      CF(nx, ny, 5) = CF(nx, ny, 5) + c1;
+      CF(nx, ny, 7) = CF(nx, ny, 7) + c3;
+      CF(nx, ny, 9) = CF(nx, ny, 9) + c1 * (dlx2 * (bd * gb - gd * bb) / det) +
+                      c3 * (dly2 * (ab * gd - gb * ad) / det);
+    }
+  }
+  /* mixed-periodic corners (:1089-1166) */
+  if (c->nxa == 2 && c->nyc == 0) {
+    bndyc(1, c->yc, &a0, &b0, &g0, &d0);
+    bndyc(1, c->yc + dly, &ap1, &bp1, &gp1, &dp1);
+    c5 = CF(1, 1, 5); c4 = CF(1, 1, 4);
+    CF(1, 1, 3) = CF(1, 1, 3) + c5 * (b0 / a0 * dxoy) +
+                  c4 * (3. * bp1 / ap1 * dxoy + dlx2 * gp1 / ap1);
+    CF(1, 1, 2) = CF(1, 1, 2) + c4;
+    CF(1, 1, 1) = CF(1, 1, 1) + c5;
+    CF(1, 1, 7) = CF(1, 1, 7) + c5 * (-b0 / a0 * dxoy) + c4 * (bp1 / ap1 * dxoy);
+    CF(1, 1, 6) = 0.0; CF(1, 1, 5) = 0.0; CF(1, 1, 4) = 0.0;
+    CF(1, 1, 9) = CF(1, 1, 9) + c4 * (-4. * bp1 / ap1 * dxoy) + c5 * (dlx2 * g0 / a0);
+    bndyc(1, c->yd - dly, &am1, &bm1, &gm1, &dm1);
+    bndyc(1, c->yd, &a0, &b0, &g0, &d0);
+    c6 = CF(1, ny, 6); c5 = CF(1, ny, 5);
+    CF(1, ny, 3) = CF(1, ny, 3) + c6 * (-bm1 / am1 * dxoy) + c5 * (b0 / a0 * dxoy);
+    CF(1, ny, 1) = CF(1, ny, 1) + c5;
+    CF(1, ny, 8) = CF(1, ny, 8) + c6;
+    CF(1, ny, 7) = CF(1, ny, 7) + c6 * (-3. * bm1 / am1 * dxoy + dlx2 * gm1 / am1) +
+                   c5 * (-b0 / a0 * dxoy);
+    CF(1, ny, 6) = 0.0; CF(1, ny, 5) = 0.0; CF(1, ny, 4) = 0.0;
+    CF(1, ny, 9) = CF(1, ny, 9) + c6 * (4. * bm1 / am1 * dxoy) + c5 * (dlx2 * g0 / a0);
+  }
+  if (c->nxb == 2 && c->nyc == 0) {
+    bndyc(2, c->yc, &a0, &b0, &g0, &d0);
+    bndyc(2, c->yc + dly, &ap1, &bp1, &gp1, &dp1);
+    c2 = CF(nx, 1, 2); c1 = CF(nx, 1, 1);
+    CF(nx, 1, 3) = CF(nx, 1, 3) + c1 * (-b0 / a0 * dxoy) +
+                   c2 * (-3. * bp1 / ap1 * dxoy - dlx2 * gp1 / ap1);
+    CF(nx, 1, 2) = 0.0; CF(nx, 1, 1) = 0.0; CF(nx, 1, 8) = 0.0;
+    CF(nx, 1, 7) = CF(nx, 1, 7) + c1 * (b0 / a0 * dxoy) + c2 * (-bp1 / ap1 * dxoy);
+    CF(nx, 1, 5) = CF(nx, 1, 5) + c1;
+    CF(nx, 1, 4) = CF(nx, 1, 4) + c2;
+    CF(nx, 1, 9) = CF(nx, 1, 9) + c1 * (-dlx2 * g0 / a0) + c2 * (4. * bp1 / ap1 * dxoy);
+    bndyc(2, c->yd - dly, &am1, &bm1, &gm1, &dm1);
+    bndyc(2, c->yd, &a0, &b0, &g0, &d0);
+    c1 = CF(nx, ny, 1); c8 = CF(nx, ny, 8);
+    CF(nx, ny, 3) = CF(nx, ny, 3) + c8 * (bm1 / am1 * dxoy) + c1 * (-b0 / a0 * dxoy);
+    CF(nx, ny, 2) = 0.0; CF(nx, ny, 1) = 0.0; CF(nx, ny, 8) = 0.0;
+    CF(nx, ny, 7) = CF(nx, ny, 7) + c8 * (3. * bm1 / am1 * dxoy - dlx2 * gm1 / am1) +
+                    c1 * (b0 / a0 * dxoy);
+    CF(nx, ny, 6) = CF(nx, ny, 6) + c8;
+    CF(nx, ny, 5) = CF(nx, ny, 5) + c1;
+    CF(nx, ny, 9) = CF(nx, ny, 9) + c8 * (-4. * bm1 / am1 * dxoy) + c1 * (-dlx2 * g0 / a0);
+  }
+  /* specified (Dirichlet) boundaries (:1167-1205) */
+  if (c->nxa == 1)
+    for (int j = 1; j <= ny; ++j) { for (int l = 1; l <= 9; ++l) CF(1, j, l) = 0.0; CF(1, j, 9) = 1.0; }
+  if (c->nxb == 1)
+    for (int j = 1; j <= ny; ++j) { for (int l = 1; l <= 9; ++l) CF(nx, j, l) = 0.0; CF(nx, j, 9) = 1.0; }
+  if (c->nyc == 1)
+    for (int i = 1; i <= nx; ++i) { for (int l = 1; l <= 9; ++l) CF(i, 1, l) = 0.0; CF(i, 1, 9) = 1.0; }
+  if (c->nyd == 1)
+    for (int i = 1; i <= nx; ++i) { for (int l = 1; l <= 9; ++l) CF(i, ny, l) = 0.0; CF(i, ny, 9) = 1.0; }
+}
+
+/* adjmd2cr (src/mud2cr.f:1271-1616).  Every derivative-boundary correction
+ * there is a multiple of gbdy, which bndcr returns as 0 (src/mg_solver2d.F90:
+ * 230,248), so those corrections add exact zeros; what remains is setting the
+ * right-hand side at specified boundaries from phi (:1580-1615). */
+static void mg_adjust(mg_ctx* c, int lev) {
+  mg_level* L = &c->lev[lev];
+  int nx = L->nx, ny = L->ny;
+  double *cf = L->cf, *phi = L->phi;
+  if (c->nxa == 1) for (int j = 1; j <= ny; ++j) CF(1, j, 10) = PH(1, j);
+  if (c->nxb == 1) for (int j = 1; j <= ny; ++j) CF(nx, j, 10) = PH(nx, j);
+  if (c->nyc == 1) for (int i = 1; i <= nx; ++i) CF(i, 1, 10) = PH(i, 1);
+  if (c->nyd == 1) for (int i = 1; i <= nx; ++i) CF(i, ny, 10) = PH(i, ny);
+}
+
+/* relmd2crp (src/mud2cr.f:1693-1795): four colours, each swept as four
+ * row-phase loops (j = 1,2,3,4 mod 4), periodic virtual points refreshed
+ * after every colour. */
+static void mg_relax(mg_ctx* c, int lev) {
+  mg_level* L = &c->lev[lev];
+  int nx = L->nx, ny = L->ny;
+  double *cf = L->cf, *phi = L->phi;
+  int ist[4] = {1, 4, 3, 2};
+  for (int color = 0; color < 4; ++color) {
+    for (int ph = 0; ph < 4; ++ph) {
+      for (int j = 1 + ph; j <= ny; j += 4)
+        for (int i = ist[ph]; i <= nx; i += 4)
+          PH(i, j) = (CF(i, j, 10) - (CF(i, j, 1) * PH(i + 1, j) + CF(i, j, 2) * PH(i + 1, j + 1) +
+                                      CF(i, j, 3) * PH(i, j + 1) + CF(i, j, 4) * PH(i - 1, j + 1) +
+                                      CF(i, j, 5) * PH(i - 1, j) + CF(i, j, 6) * PH(i - 1, j - 1) +
+                                      CF(i, j, 7) * PH(i, j - 1) + CF(i, j, 8) * PH(i + 1, j - 1))) /
+                     CF(i, j, 9);
+    }
+    if (c->nxa == 0)
+      for (int j = 1; j <= ny; ++j) { PH(0, j) = PH(nx - 1, j); PH(nx + 1, j) = PH(2, j); }
+    if (c->nyc == 0)
+      for (int i = 1; i <= nx; ++i) { PH(i, 0) = PH(i, ny - 1); PH(i, ny + 1) = PH(i, 2); }
+    int t = ist[3];
+    ist[3] = ist[2]; ist[2] = ist[1]; ist[1] = ist[0]; ist[0] = t;
+  }
+}
+
+/* res2 (src/mudcom.f:120-258): full weighting of resf into rhsc */
+static void mg_res2(const mg_ctx* c, int nx, int ny, const double* resf, int ncx, int ncy,
+                    double* rhsc) {
+#define RF(i, j) resf[((i)-1) + (size_t)nx * ((j)-1)]
+#define RC(i, j) rhsc[((i)-1) + (size_t)ncx * ((j)-1)]
+#define FW(im, ip, jm, jp, i, j) \
+  ((RF(im, jm) + RF(ip, jm) + RF(im, jp) + RF(ip, jp) + 2. * (RF(im, j) + RF(ip, j) + RF(i, jm) + RF(i, jp)) + 4. * RF(i, j)) * .0625)
+  int ix = (ncx == nx) ? 0 : 1, jy = (ncy == ny) ? 0 : 1;
+  for (int jc = 2; jc <= ncy - 1; ++jc) {
+    int j = jy ? jc + jc - 1 : jc;
+    for (int ic = 2; ic <= ncx - 1; ++ic) {
+      int i = ix ? ic + ic - 1 : ic;
+      RC(ic, jc) = FW(i - 1, i + 1, j - 1, j + 1, i, j);
+    }
+  }
+  for (int jc = 1; jc <= ncy; jc += ncy - 1) {
+    int j = jc + jy * (jc - 1);
+    int jm1 = j - 1 > 2 ? j - 1 : 2, jp1 = j + 1 < ny - 1 ? j + 1 : ny - 1;
+    if (j == 1 && c->nyc == 0) jm1 = ny - 1;
+    if (j == ny && c->nyc == 0) jp1 = 2;
+    for (int ic = 1; ic <= ncx; ic += ncx - 1) {
+      int i = ic + ix * (ic - 1);
+      int im1 = i - 1 > 2 ? i - 1 : 2, ip1 = i + 1 < nx - 1 ? i + 1 : nx - 1;
+      if (i == 1 && c->nxa == 0) im1 = nx - 1;
+      if (i == nx && c->nxa == 0) ip1 = 2;
+      RC(ic, jc) = FW(im1, ip1, jm1, jp1, i, j);
+    }
+    for (int ic = 2; ic <= ncx - 1; ++ic) {
+      int i = ic + ix * (ic - 1);
+      RC(ic, jc) = FW(i - 1, i + 1, jm1, jp1, i, j);
+    }
+  }
+  for (int ic = 1; ic <= ncx; ic += ncx - 1) {
+    int i = ic + ix * (ic - 1);
+    int im1 = i - 1 > 2 ? i - 1 : 2, ip1 = i + 1 < nx - 1 ? i + 1 : nx - 1;
+    if (i == 1 && c->nxa == 0) im1 = nx - 1;
+    if (i == nx && c->nxa == 0) ip1 = 2;
+    for (int jc = 2; jc <= ncy - 1; ++jc) {
+      int j = jc + jy * (jc - 1);
+      RC(ic, jc) = FW(im1, ip1, j - 1, j + 1, i, j);
+    }
+  }
+  if (c->nxa == 1) for (int jc = 1; jc <= ncy; ++jc) RC(1, jc) = 0.0;
+  if (c->nxb == 1) for (int jc = 1; jc <= ncy; ++jc) RC(ncx, jc) = 0.0;
+  if (c->nyc == 1) for (int ic = 1; ic <= ncx; ++ic) RC(ic, 1) = 0.0;
+  if (c->nyd == 1) for (int ic = 1; ic <= ncx; ++ic) RC(ic, ncy) = 0.0;
+#undef RF
+#undef RC
+#undef FW
+}
+
+/* resmd2cr (src/mud2cr.f:1618-1665) */
+static void mg_restrict(mg_ctx* c, int lev) {
+  mg_level *F = &c->lev[lev], *C = &c->lev[lev - 1];
+  int nx = F->nx, ny = F->ny;
+  double *cf = F->cf, *phi = F->phi;
+  for (size_t t = 0; t < (size_t)(C->nx + 2) * (C->ny + 2); ++t) C->phi[t] = 0.0;
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i)
+      c->resf[(i - 1) + (size_t)nx * (j - 1)] =
+          CF(i, j, 10) - (CF(i, j, 1) * PH(i + 1, j) + CF(i, j, 2) * PH(i + 1, j + 1) +
+                          CF(i, j, 3) * PH(i, j + 1) + CF(i, j, 4) * PH(i - 1, j + 1) +
+                          CF(i, j, 5) * PH(i - 1, j) + CF(i, j, 6) * PH(i - 1, j - 1) +
+                          CF(i, j, 7) * PH(i, j - 1) + CF(i, j, 8) * PH(i + 1, j - 1) +
+                          CF(i, j, 9) * PH(i, j));
+  mg_res2(c, nx, ny, c->resf, C->nx, C->ny, C->cf + (size_t)9 * C->nx * C->ny);
+}
+
+/* prolon1 (src/mudcom.f:404-500), p(0:ncx+1) -> q(0:nx+1) */
+static void mg_prolon1(int ncx, const double* p, int nx, double* q, int nxa, int nxb, int intpol) {
+  int ist = 1, ioddst = 1, ifn = nx, ioddfn = nx;
+  if (nxa == 1) { ist = 2; ioddst = 3; }
+  if (nxb == 1) { ifn = nx - 1; ioddfn = nx - 2; }
+  if (intpol == 1 || ncx < 4) {
+    if (ncx < nx) {
+      for (int i = ioddst; i <= ioddfn; i += 2) q[i] = p[(i + 1) / 2];
+      for (int i = 2; i <= ifn; i += 2) q[i] = 0.5 * (q[i - 1] + q[i + 1]);
+    } else {
+      for (int i = ist; i <= ifn; ++i) q[i] = p[i];
+    }
+    if (nxa == 0) { q[0] = q[nx - 1]; q[nx + 1] = q[2]; }
+    return;
+  }
+  if (ncx < nx) {
+    for (int i = ioddst; i <= ioddfn; i += 2) q[i] = p[(i + 1) / 2];
+    for (int i = 4; i <= nx - 3; i += 2) q[i] = (-q[i - 3] + 9. * (q[i - 1] + q[i + 1]) - q[i + 3]) * .0625;
+    if (nxa != 0) {
+      q[2] = (5. * q[1] + 15. * q[3] - 5. * q[5] + q[7]) * .0625;
+      q[nx - 1] = (5. * q[nx] + 15. * q[nx - 2] - 5. * q[nx - 4] + q[nx - 6]) * .0625;
+    } else {
+      q[2] = (-q[nx - 2] + 9. * (q[1] + q[3]) - q[5]) * .0625;
+      q[nx - 1] = (-q[nx - 4] + 9. * (q[nx - 2] + q[nx]) - q[3]) * .0625;
+      q[nx + 1] = q[2];
+      q[0] = q[nx - 1];
+    }
+  } else {
+    for (int i = ist; i <= ifn; ++i) q[i] = p[i];
+    if (nxa == 0) { q[0] = q[nx - 1]; q[nx + 1] = q[2]; }
+  }
+}
+
+/* prolon2 (src/mudcom.f:261-402), p(0:ncx+1,0:ncy+1) -> q(0:nx+1,0:ny+1) */
+static void mg_prolon2(const mg_ctx* c, int ncx, int ncy, const double* p, int nx, int ny,
+                       double* q) {
+  int nxa = c->nxa, nxb = c->nxb, nyc = c->nyc, nyd = c->nyd, intpol = c->intpol;
+  int ist = 1, ifn = nx, jst = 1, jfn = ny, joddst = 1, joddfn = ny;
+  size_t lq = (size_t)nx + 2, lp = (size_t)ncx + 2;
+#define Q(i, j) q[(i) + lq * (j)]
+  if (nxa == 1) ist = 2;
+  if (nxb == 1) ifn = nx - 1;
+  if (nyc == 1) { jst = 2; joddst = 3; }
+  if (nyd == 1) { jfn = ny - 1; joddfn = ny - 2; }
+  if (intpol == 1 || ncy < 4) {
+    if (ncy < ny) {
+      for (int j = joddst; j <= joddfn; j += 2)
+        mg_prolon1(ncx, p + lp * (j / 2 + 1), nx, q + lq * j, nxa, nxb, intpol);
+      for (int j = 2; j <= jfn; j += 2)
+        for (int i = ist; i <= ifn; ++i) Q(i, j) = 0.5 * (Q(i, j - 1) + Q(i, j + 1));
+    } else {
+      for (int j = jst; j <= jfn; ++j) mg_prolon1(ncx, p + lp * j, nx, q + lq * j, nxa, nxb, intpol);
+    }
+    if (nyc == 0)
+      for (int i = ist; i <= ifn; ++i) { Q(i, 0) = Q(i, ny - 1); Q(i, ny + 1) = Q(i, 2); }
+    return;
+  }
+  if (ncy < ny) {
+    for (int j = joddst; j <= joddfn; j += 2)
+      mg_prolon1(ncx, p + lp * (j / 2 + 1), nx, q + lq * j, nxa, nxb, intpol);
+    for (int j = 4; j <= ny - 3; j += 2)
+      for (int i = ist; i <= ifn; ++i)
+        Q(i, j) = (-Q(i, j - 3) + 9. * (Q(i, j - 1) + Q(i, j + 1)) - Q(i, j + 3)) * .0625;
+    if (nyc != 0) {
+      for (int i = ist; i <= ifn; ++i) {
+        Q(i, 2) = (5. * Q(i, 1) + 15. * Q(i, 3) - 5. * Q(i, 5) + Q(i, 7)) * .0625;
+        Q(i, ny - 1) = (5. * Q(i, ny) + 15. * Q(i, ny - 2) - 5. * Q(i, ny - 4) + Q(i, ny - 6)) * .0625;
+      }
+    } else {
+      for (int i = ist; i <= ifn; ++i) {
+        Q(i, 2) = (-Q(i, ny - 2) + 9. * (Q(i, 1) + Q(i, 3)) - Q(i, 5)) * .0625;
+        Q(i, ny - 1) = (-Q(i, ny - 4) + 9. * (Q(i, ny - 2) + Q(i, ny)) - Q(i, 3)) * .0625;
+        Q(i, ny + 1) = Q(i, 2);
+        Q(i, 0) = Q(i, ny - 1);
+      }
+    }
+  } else {
+    for (int j = jst; j <= jfn; ++j) mg_prolon1(ncx, p + lp * j, nx, q + lq * j, nxa, nxb, intpol);
+    if (nyc == 0)
+      for (int i = ist; i <= ifn; ++i) { Q(i, 0) = Q(i, ny - 1); Q(i, ny + 1) = Q(i, 2); }
+  }
+#undef Q
+}
+
+/* cor2 (src/mudcom.f:502-553) */
+static void mg_correct(mg_ctx* c, int lev) {
+  mg_level *F = &c->lev[lev + 1], *C = &c->lev[lev];
+  int nx = F->nx, ny = F->ny;
+  double* phi = F->phi;
+  size_t n = (size_t)(nx + 2) * (ny + 2);
+  for (size_t t = 0; t < n; ++t) c->phcor[t] = 0.0;
+  mg_prolon2(c, C->nx, C->ny, C->phi, nx, ny, c->phcor);
+  int ist = 1, ifn = nx, jst = 1, jfn = ny;
+  if (c->nxa == 1) ist = 2;
+  if (c->nxb == 1) ifn = nx - 1;
+  if (c->nyc == 1) jst = 2;
+  if (c->nyd == 1) jfn = ny - 1;
+  for (int j = jst; j <= jfn; ++j)
+    for (int i = ist; i <= ifn; ++i) PH(i, j) = PH(i, j) + c->phcor[i + (size_t)(nx + 2) * j];
+  if (c->nyc == 0)
+    for (int i = ist; i <= ifn; ++i) { PH(i, 0) = PH(i, ny - 1); PH(i, ny + 1) = PH(i, 2); }
+  if (c->nxa == 0)
+    for (int j = jst; j <= jfn; ++j) { PH(0, j) = PH(nx - 1, j); PH(nx + 1, j) = PH(2, j); }
+}
+
+/* kcymd2cr (src/mud2cr.f:340-499): one k-cycle from level kcur */
+static void mg_kcycle(mg_ctx* c, int kcur) {
+  int kount[51];
+  for (int l = 1; l <= kcur; ++l) kount[l] = 0;
+  for (int l = 0; l < c->iprer; ++l) mg_relax(c, kcur);
+  if (kcur != 1) {
+    mg_restrict(c, kcur);
+    int klevel = kcur - 1, nrel = c->iprer;
+    for (;;) {
+      if (klevel == kcur) break;
+      kount[klevel]++;
+      for (int l = 0; l < nrel; ++l) mg_relax(c, klevel);
+      if (kount[klevel] == c->kcycle + 1) {
+        mg_correct(c, klevel);
+        kount[klevel] = 0;
+        klevel++;
+        nrel = c->ipost;
+      } else if (klevel > 1) {
+        mg_restrict(c, klevel);
+        klevel--;
+        nrel = c->iprer;
+      } else {
+        for (int l = 0; l < c->ipost; ++l) mg_relax(c, klevel);
+        mg_correct(c, 1);
+        nrel = c->ipost;
+        klevel = 2;
+      }
+    }
+  }
+  for (int l = 0; l < c->ipost; ++l) mg_relax(c, kcur);
+}
+
+/* trsfc2 (src/mudcom.f:61-118): inject phi and rhs from level lev+1 to lev */
+static void mg_transfer(mg_ctx* c, int lev) {
+  mg_level *F = &c->lev[lev + 1], *C = &c->lev[lev];
+  int nx = F->nx, ny = F->ny, ncx = C->nx, ncy = C->ny;
+  const double *phi = F->phi, *rhs = F->cf + (size_t)9 * nx * ny;
+  double *phic = C->phi, *rhsc = C->cf + (size_t)9 * ncx * ncy;
+  size_t lc = (size_t)ncx + 2;
+  for (int jc = 0; jc <= ncy + 1; ++jc) { phic[0 + lc * jc] = 0.0; phic[ncx + 1 + lc * jc] = 0.0; }
+  for (int ic = 0; ic <= ncx + 1; ++ic) { phic[ic] = 0.0; phic[ic + lc * (ncy + 1)] = 0.0; }
+  for (int jc = 1; jc <= ncy; ++jc) {
+    int j = (ncy < ny) ? jc + jc - 1 : jc;
+    for (int ic = 1; ic <= ncx; ++ic) {
+      int i = (ncx < nx) ? ic + ic - 1 : ic;
+      phic[ic + lc * jc] = PH(i, j);
+      rhsc[(ic - 1) + (size_t)ncx * (jc - 1)] = rhs[(i - 1) + (size_t)nx * (j - 1)];
+    }
+  }
+}
+
+static void mg_setup(mg_ctx* c, const ora* o) {
+  memset(c, 0, sizeof(*c));
+  const ekm_params* p = &o->p;
+  c->o = o;
+  c->nxa = p->mudpack_bc[0]; c->nxb = p->mudpack_bc[1];
+  c->nyc = p->mudpack_bc[2]; c->nyd = p->mudpack_bc[3];
+  c->ixp = p->mg_ixp; c->jyq = p->mg_jyq; c->iex = p->mg_iex; c->jey = p->mg_jey;
+  c->ngrid = c->iex > c->jey ? c->iex : c->jey;
+  c->nfx = o->NZ2; c->nfy = o->NY2;
+  c->maxcy = p->mg_maxcy; c->tolmax = p->mg_tolmax;
+  c->kcycle = 2; c->iprer = 2; c->ipost = 1; c->intpol = 3;
+  c->xa = 0.0; c->xb = (double)(c->nfx - 1); c->yc = 0.0; c->yd = (double)(c->nfy - 1);
+  for (int k = 1; k <= c->ngrid; ++k) {
+    int ex = k + c->iex - c->ngrid, ey = k + c->jey - c->ngrid;
+    mg_level* L = &c->lev[k];
+    L->nx = c->ixp * (1 << ((ex > 1 ? ex : 1) - 1)) + 1;
+    L->ny = c->jyq * (1 << ((ey > 1 ? ey : 1) - 1)) + 1;
+    L->phi = dalloc((size_t)(L->nx + 2) * (L->ny + 2));
+    L->cf = dalloc((size_t)L->nx * L->ny * 10);
+  }
+  c->resf = dalloc((size_t)c->nfx * c->nfy);
+  c->phcor = dalloc((size_t)(c->nfx + 2) * (c->nfy + 2));
+}
+
+static void mg_free(mg_ctx* c) {
+  for (int k = 1; k <= c->ngrid; ++k) { free(c->lev[k].phi); free(c->lev[k].cf); }
+  free(c->resf);
+  free(c->phcor);
+}
+
+/* MULTIGRID_MUDPACK (src/mg_solver2d.F90:75-201): phi = 0, rhs = -RHS_TMP
+ * (phi_bc, src/duct.F90:3867-3895), discretise every level (intl=0), then
+ * mud2cr1 (src/mud2cr.f:211-338) with iguess = 0. */
+static int mg_solve(mg_ctx* c, double kx2, const double* rhs_tmp, double* phiout) {
+  int nfx = c->nfx, nfy = c->nfy;
+  c->kx2 = kx2;
+  for (int k = 1; k <= c->ngrid; ++k) mg_discretize(c, k);
+  mg_level* T = &c->lev[c->ngrid];
+  /* swk2: phif = 0, rhs = -RHS_TMP, virtual boundaries zero */
+  for (size_t t = 0; t < (size_t)(nfx + 2) * (nfy + 2); ++t) T->phi[t] = 0.0;
+  double* rhs = T->cf + (size_t)9 * nfx * nfy;
+  for (int j = 1; j <= nfy; ++j)
+    for (int i = 1; i <= nfx; ++i)
+      rhs[(i - 1) + (size_t)nfx * (j - 1)] = -rhs_tmp[(i - 1) + (size_t)nfx * (j - 1)];
+  for (int k = c->ngrid - 1; k >= 1; --k) mg_transfer(c, k);
+  for (int k = 1; k <= c->ngrid; ++k) mg_adjust(c, k);
+  for (int k = 1; k <= c->ngrid - 1; ++k) {
+    mg_kcycle(c, k);
+    mg_level *C = &c->lev[k], *F = &c->lev[k + 1];
+    mg_prolon2(c, C->nx, C->ny, C->phi, F->nx, F->ny, F->phi);
+  }
+  /* phif holds the previous iterate for the relmax test (zero on entry) */
+  for (size_t t = 0; t < (size_t)nfx * nfy; ++t) phiout[t] = 0.0;
+  double* phi = T->phi;
+  int nx = nfx;
+  c->itero = 0;
+  for (int iter = 1; iter <= c->maxcy; ++iter) {
+    c->itero = iter;
+    mg_kcycle(c, c->ngrid);
+    if (c->tolmax > 0.0) {
+      double relmax = 0.0, phmax = 0.0;
+      for (int j = 1; j <= nfy; ++j)
+        for (int i = 1; i <= nfx; ++i) {
+          double w = PH(i, j), *pf = &phiout[(i - 1) + (size_t)nfx * (j - 1)];
+          phmax = fmax(phmax, fabs(w));
+          relmax = fmax(relmax, fabs(w - *pf));
+          *pf = w;
+        }
+      if (phmax > 0.0) relmax = relmax / phmax;
+      c->relmax = relmax;
+      if (relmax <= c->tolmax) return c->itero;
+    }
+  }
+  for (int j = 1; j <= nfy; ++j)
+    for (int i = 1; i <= nfx; ++i) phiout[(i - 1) + (size_t)nfx * (j - 1)] = PH(i, j);
+  return c->itero;
+}
+#undef COF
+#undef CF
+#undef PH
+
+int ora_mg_ngrid(ora* o) {
+  int a = o->p.mg_iex, b = o->p.mg_jey;
+  return a > b ? a : b;
+}
+
+int ora_mudpack(ora* o, double kx2, const double* rhs_tmp, double* phi) {
+  mg_ctx c;
+  mg_setup(&c, o);
+  int cyc = mg_solve(&c, kx2, rhs_tmp, phi);
+  mg_free(&c);
+  return cyc;
+}
+
+int ora_mg_discretize(ora* o, double kx2, int lev, double* cf, int* nx, int* ny) {
+  mg_ctx c;
+  mg_setup(&c, o);
+  if (lev < 1 || lev > c.ngrid) { mg_free(&c); return EKM_ERR_ARG; }
+  c.kx2 = kx2;
+  mg_discretize(&c, lev);
+  *nx = c.lev[lev].nx;
+  *ny = c.lev[lev].ny;
+  memcpy(cf, c.lev[lev].cf, sizeof(double) * (size_t)(*nx) * (*ny) * 10);
+  int n = (*nx) * (*ny);
+  mg_free(&c);
+  return n;
+}
+
+/* callbacks for the reference MUDPACK build (oracle/_ref) */
+static const ora* cb_o;
+static double cb_kx2;
+void ora_cb_bind(ora* o, double kx2) { cb_o = o; cb_kx2 = kx2; }
+void ora_cofcr_cb(const double* x, const double* y, double* cxx, double* cxy, double* cyy,
+                  double* cx, double* cy, double* ce) {
+  cofcr(cb_o, cb_kx2, *x, *y, cxx, cxy, cyy, cx, cy, ce);
+}
+void ora_bndcr_cb(const int* kbdy, const double* xory, double* alfa, double* beta,
+                  double* gama, double* gbdy) {
+  bndyc(*kbdy, *xory, alfa, beta, gama, gbdy);
+}
+
+/* ------------------------------------------------------------------------ */
+/* set-up: INIT_DUCT index ranges (src/duct.F90:7-232), RK constants         */
+/* (src/diablo.F90:568-576), wavenumbers (INIT_FFT, called at :563)          */
+/* ------------------------------------------------------------------------ */
+static int supported(const ekm_params* p) {
+  for (int c = 0; c < 3; ++c)
+    for (int s = 0; s < 4; ++s) {
+      int t = p->u_bc[c][s];
+      if (t != 0 && t != 1 && t != 6) return 0;
+    }
+  if (p->n_th) {
+    int t;
+    t = p->th_bc[EKM_YMIN]; if (t != 0 && t != 1 && t != 5 && t != 6) return 0;
+    t = p->th_bc[EKM_YMAX]; if (t != 0 && t != 1 && t != 6) return 0;
+    t = p->th_bc[EKM_ZMIN]; if (t != 0 && t != 1 && t != 6) return 0;
+    t = p->th_bc[EKM_ZMAX]; if (t != 0 && t != 1 && t != 5 && t != 6) return 0;
+  }
+  if (p->f_type != 1 && p->f_type != 2 && p->f_type != 3) return 0; /* 3: no mean forcing */
+  for (int s = 0; s < 4; ++s) if (p->mudpack_bc[s] < 0 || p->mudpack_bc[s] > 2) return 0;
+  if ((p->mudpack_bc[0] == 0) != (p->mudpack_bc[1] == 0)) return 0;
+  if ((p->mudpack_bc[2] == 0) != (p->mudpack_bc[3] == 0)) return 0;
+  if (p->nz + 2 != p->mg_ixp * (1 << (p->mg_iex - 1)) + 1) return 0;
+  if (p->ny + 2 != p->mg_jyq * (1 << (p->mg_jey - 1)) + 1) return 0;
+  if (p->nx < 8 || (p->nx & (p->nx - 1))) return 0;
+  return 1;
+}
+
+static void init_ranges(ora* o) {
+  const ekm_params* p = &o->p;
+  int NY = o->NY, NZ = o->NZ;
+  int wy0 = p->u_bc[2][EKM_YMIN], wy1 = p->u_bc[2][EKM_YMAX];
+  int uz0 = p->u_bc[0][EKM_ZMIN], uz1 = p->u_bc[0][EKM_ZMAX];
+  o->jstart = (wy0 == 0 || wy0 == 1 || wy0 == 6 || wy0 == 7 || wy0 == 3) ? 1 : 2;
+  o->zstart = (uz0 == 1 || uz0 == 6) ? 1 : 2;
+  o->jend = (wy1 == 0 || wy1 == 1 || wy1 == 6) ? NY : NY - 1;
+  o->zend = (uz1 == 0 || uz1 == 1 || uz1 == 6) ? NZ : NZ - 1; /* IC_TYPE 7 */
+  int ty0 = p->th_bc[EKM_YMIN], ty1 = p->th_bc[EKM_YMAX];
+  int tz0 = p->th_bc[EKM_ZMIN], tz1 = p->th_bc[EKM_ZMAX];
+  o->jstart_th = (ty0 == 0 || ty0 == 1 || ty0 == 5 || ty0 == 2 || ty0 == 6 || ty0 == 7 || ty0 == 8) ? 1 : 2;
+  o->zstart_th = (tz0 == 0 || tz0 == 1 || tz0 == 6) ? 1 : 2;
+  o->jend_th = (ty1 == 0 || ty1 == 1 || ty1 == 6 || ty1 == 5) ? NY : NY - 1;
+  o->zend_th = (tz1 == 0 || tz1 == 1 || tz1 == 5 || tz1 == 6) ? NZ : NZ - 1;
+}
+
+static void set_hbar(ora* o) {
+  o->h_bar[0] = o->delta_t * (8.0 / 15.0);
+  o->h_bar[1] = o->delta_t * (2.0 / 15.0);
+  o->h_bar[2] = o->delta_t * (5.0 / 15.0);
+}
+
+ora* ora_create(const ekm_params* p, const double* xpoint, const double* ypoint,
+                const double* th_bar) {
+  if (!supported(p)) return NULL;
+  ora* o = (ora*)calloc(1, sizeof(ora));
+  o->p = *p;
+  o->NX = p->nx; o->NY = p->ny; o->NZ = p->nz;
+  o->NXC = p->nx / 2 + 1; o->NKX = p->nx / 3; o->NX2 = p->nx + 2;
+  o->NY2 = p->ny + 2; o->NZ2 = p->nz + 2; o->NZ3 = p->nz + 3; o->NY3 = p->ny + 3;
+  o->npts = (size_t)o->NX2 * o->NZ2 * o->NY2;
+  size_t pl = (size_t)o->NZ2 * o->NY2, ml = (size_t)o->NZ3 * o->NY3;
+  o->xp = dalloc(pl); o->yp = dalloc(pl); o->thbar = dalloc(pl);
+  memcpy(o->xp, xpoint, pl * sizeof(double));
+  memcpy(o->yp, ypoint, pl * sizeof(double));
+  for (int q = 0; q < 7; ++q) o->met[q] = dalloc(ml * 2);
+  o->jac = dalloc(ml);
+  metrics(o);
+  /* TH_BAR (src/diablo.F90:1152-1160) */
+  for (int j = 0; j < o->NY2; ++j)
+    for (int k = 0; k < o->NZ2; ++k)
+      o->thbar[PL(o, k, j)] = p->cont_strat ? (-o->yp[PL(o, k, j)] + o->yp[PL(o, 0, o->NY + 1)])
+                                            : (th_bar ? th_bar[PL(o, k, j)] : 0.0);
+  o->kx = dalloc(o->NXC); o->kx2 = dalloc(o->NXC);
+  for (int m = 0; m < o->NXC; ++m) {
+    o->kx[m] = 2.0 * M_PI * m / p->lx;
+    o->kx2[m] = o->kx[m] * o->kx[m];
+  }
+  double** f[] = {&o->U[0], &o->U[1], &o->U[2], &o->P, &o->TH, &o->CF[0], &o->CF[1], &o->CF[2],
+                  &o->CFTH, &o->R[0], &o->R[1], &o->R[2], &o->RTH, &o->S1, &o->S2, &o->U2b,
+                  &o->U3b, &o->CS1, &o->PHI};
+  for (size_t q = 0; q < sizeof(f) / sizeof(f[0]); ++q) *f[q] = dalloc(o->npts);
+  init_ranges(o);
+  o->delta_t = p->delta_t;
+  o->delta_t_set = p->delta_t;
+  set_hbar(o);
+  o->beta_bar[0] = 1.0; o->beta_bar[1] = 25.0 / 8.0; o->beta_bar[2] = 9.0 / 4.0;
+  o->zeta_bar[0] = 0.0; o->zeta_bar[1] = -17.0 / 8.0; o->zeta_bar[2] = -5.0 / 4.0;
+  o->rk_step = 1;
+  return o;
+}
+
+void ora_destroy(ora* o) {
+  if (!o) return;
+  free(o->xp); free(o->yp); free(o->thbar); free(o->jac); free(o->kx); free(o->kx2);
+  for (int q = 0; q < 7; ++q) free(o->met[q]);
+  double* f[] = {o->U[0], o->U[1], o->U[2], o->P, o->TH, o->CF[0], o->CF[1], o->CF[2], o->CFTH,
+                 o->R[0], o->R[1], o->R[2], o->RTH, o->S1, o->S2, o->U2b, o->U3b, o->CS1, o->PHI};
+  for (size_t q = 0; q < sizeof(f) / sizeof(f[0]); ++q) free(f[q]);
+  free(o);
+}
+
+/* ------------------------------------------------------------------------ */
+/* ADI boundary rows: APPLY_BC_{1,2,3,TH}_{LOWER,UPPER,LEFT,RIGHT}           */
+/* (src/boundary.F90:124-1760) for IC_TYPE = 7                               */
+/* ------------------------------------------------------------------------ */
+typedef struct sysb { double *a, *b, *c, *g; size_t ld; } sysb;
+
+static void row(sysb* s, int r, double a, double b, double c) {
+  for (size_t i = 0; i < s->ld; ++i) {
+    s->a[i + s->ld * r] = a; s->b[i + s->ld * r] = b; s->c[i + s->ld * r] = c;
+  }
+}
+static void rhsc(sysb* s, int r, double v) { for (size_t i = 0; i < s->ld; ++i) s->g[i + s->ld * r] = v; }
+/* lagged "periodic" value: the field on the same line at index `src` */
+static void rhs_line(sysb* s, int r, const double* u, size_t off, size_t stride, int src) {
+  for (size_t i = 0; i < s->ld; ++i) s->g[i + s->ld * r] = u[off + i + stride * (size_t)src];
+}
+
+/* y-direction rows for line K.  comp: 0..2 velocity, 3 scalar. off/stride
+ * address U(I,K,j) = u[off + I + stride*j]. */
+static void bc_y(const ora* o, int comp, int K, sysb* s, const double* u, size_t off, size_t stride) {
+  const ekm_params* p = &o->p;
+  int NY = o->NY;
+  int lo = comp < 3 ? p->u_bc[comp][EKM_YMIN] : p->th_bc[EKM_YMIN];
+  int hi = comp < 3 ? p->u_bc[comp][EKM_YMAX] : p->th_bc[EKM_YMAX];
+  double c0 = comp < 3 ? p->u_bc_c1[comp][EKM_YMIN] : p->th_bc_c1[EKM_YMIN];
+  double c1 = comp < 3 ? p->u_bc_c1[comp][EKM_YMAX] : p->th_bc_c1[EKM_YMAX];
+  /* LOWER */
+  if (lo == 0) { row(s, 0, 0., 1., 0.); rhsc(s, 0, c0); }
+  else if (lo == 1 || lo == 5) {
+    if (comp == 3) { /* :163-178 */
+      row(s, 0, 0., -1., 1.);
+      rhsc(s, 0, c0 / (0.5 * (C22(o, K, 1, 2) + C22(o, K + 1, 1, 1))) * JAC(o, K, 1));
+    } else if (comp == 2) { row(s, 0, 0., -1., 1.); rhsc(s, 0, c0); } /* :1338-1345 */
+    else { row(s, 1, 0., -1., 1.); rhsc(s, 1, c0); }                  /* :500-506, :868-874 */
+  } else { row(s, 0, 0., 1., 0.); rhs_line(s, 0, u, off, stride, NY); }
+  /* UPPER */
+  if (hi == 0) {
+    row(s, NY + 1, 0., 1., 0.); rhsc(s, NY + 1, c1);
+    if (comp == 1 || comp == 2) { row(s, NY, 0., 1., 0.); rhsc(s, NY, c1); } /* :951-960, :1410-1419 */
+  } else if (hi == 1) {
+    if (comp == 3) { /* :333-346 */
+      row(s, NY, -1., 1., 0.);
+      rhsc(s, NY, 0.5 * c1 * (JAC(o, K, NY) + JAC(o, K, NY - 1)) / C22(o, K, NY - 1, 1));
+      row(s, NY + 1, -1., 1., 0.);
+      rhsc(s, NY + 1, 0.5 * c1 * (JAC(o, K, NY + 1) + JAC(o, K, NY)) / C22(o, K, NY, 1));
+    } else if (comp == 1) { row(s, NY, -1., 1., 0.); rhsc(s, NY, c1); } /* :962-968 */
+    else { row(s, NY + 1, -1., 1., 0.); rhsc(s, NY + 1, c1); }
+  } else { row(s, NY + 1, 0., 1., 0.); rhs_line(s, NY + 1, u, off, stride, 1); }
+}
+
+/* z-direction rows for line J */
+static void bc_z(const ora* o, int comp, int J, sysb* s, const double* u, size_t off, size_t stride) {
+  const ekm_params* p = &o->p;
+  int NZ = o->NZ;
+  int lo = comp < 3 ? p->u_bc[comp][EKM_ZMIN] : p->th_bc[EKM_ZMIN];
+  int hi = comp < 3 ? p->u_bc[comp][EKM_ZMAX] : p->th_bc[EKM_ZMAX];
+  double c0 = comp < 3 ? p->u_bc_c1[comp][EKM_ZMIN] : p->th_bc_c1[EKM_ZMIN];
+  double c1 = comp < 3 ? p->u_bc_c1[comp][EKM_ZMAX] : p->th_bc_c1[EKM_ZMAX];
+  (void)J;
+  /* LEFT */
+  if (lo == 0) {
+    /* U1: C1 on rows 0,1 (:668-683); U2,U3 (IC_TYPE 7): U2_BAR/U3_BAR = 0
+     * (:1062-1075, :1540-1549; VEL_PROFILE sets them to zero with Q = 0,
+     * src/duct.F90:3699-3731); TH: 0 then C1 (:374-384) */
+    double v0 = comp == 0 ? c0 : 0.0, v1 = (comp == 0 || comp == 3) ? c0 : 0.0;
+    row(s, 0, 0., 1., 0.); rhsc(s, 0, v0);
+    row(s, 1, 0., 1., 0.); rhsc(s, 1, v1);
+  } else if (lo == 1) { row(s, 0, 0., -1., 1.); rhsc(s, 0, c0); }
+  else { row(s, 0, 0., 1., 0.); rhs_line(s, 0, u, off, stride, NZ); }
+  /* RIGHT */
+  if (hi == 0) {
+    double vtop = comp == 2 ? c1 : 0.0; /* U3: C1 on NZ+1 and NZ (:1623-1632) */
+    row(s, NZ + 1, 0., 1., 0.); rhsc(s, NZ + 1, vtop);
+    row(s, NZ, 0., 1., 0.); rhsc(s, NZ, c1);
+  } else if (hi == 1) {
+    row(s, NZ + 1, -1., 1., 0.); rhsc(s, NZ + 1, c1);
+    if (comp == 0) { row(s, NZ + 1, 0., 1., 0.); rhsc(s, NZ + 1, 0.0); } /* :792-797 */
+  } else if (hi == 5) { row(s, NZ + 1, -2., 1., 0.); rhsc(s, NZ + 1, 0.0); }   /* :443-449 */
+  else { row(s, NZ + 1, 0., 1., 0.); rhs_line(s, NZ + 1, u, off, stride, 1); }
+}
+
+/* ------------------------------------------------------------------------ */
+/* ADI for one field (theta: :1334-1625; U1: :1640-1835; U2: :1877-2110;     */
+/* U3: :2137-2391)                                                           */
+/* ------------------------------------------------------------------------ */
+static void adi(ora* o, int comp, double* U, double* R, double t1) {
+  int NX2 = o->NX2, NY = o->NY, NZ = o->NZ;
+  int js = comp < 3 ? o->jstart : o->jstart_th, je = comp < 3 ? o->jend : o->jend_th;
+  int zs = comp < 3 ? o->zstart : o->zstart_th, ze = comp < 3 ? o->zend : o->zend_th;
+  double* S1 = o->S1;
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k)
+      for (int i = 0; i < NX2; ++i) {
+        size_t q = IDX(o, i, k, j);
+        S1[q] = R[q] - JAC(o, k, j) * U[q];
+      }
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k)
+      for (int i = 0; i < NX2; ++i) {
+        size_t q = IDX(o, i, k, j), dk = NX2;
+        R[q] = 0.5 * S1[q] + JAC(o, k, j) * U[q] +
+               t1 * (G11(o, k + 1, j, 2) * (U[q + dk] - U[q]) - G11(o, k, j, 2) * (U[q] - U[q - dk]));
+      }
+  /* y sweeps, lines K */
+  size_t sy = (size_t)NX2 * o->NZ2;
+#pragma omp parallel
+  {
+    size_t n = (size_t)NX2 * (NY + 2);
+    sysb s = {dalloc(n), dalloc(n), dalloc(n), dalloc(n), (size_t)NX2};
+    for (size_t t = 0; t < n; ++t) s.b[t] = 1.0;
+#pragma omp for schedule(static)
+    for (int k = zs; k <= ze; ++k) {
+      for (int j = js; j <= je; ++j) {
+        double l = -(t1 * G22(o, k, j, 1)), d = JAC(o, k, j) + t1 * (G22(o, k, j + 1, 1) + G22(o, k, j, 1));
+        double u = -(t1 * G22(o, k, j + 1, 1));
+        for (int i = 0; i < NX2; ++i) {
+          size_t r = i + (size_t)NX2 * j;
+          s.a[r] = l; s.b[r] = d; s.c[r] = u; s.g[r] = R[IDX(o, i, k, j)];
+        }
+      }
+      bc_y(o, comp, k, &s, U, IDX(o, 0, k, 0), sy);
+      ora_thomas_real(s.a, s.b, s.c, s.g, NY + 1, NX2 - 1);
+      for (int j = 0; j <= NY + 1; ++j)
+        for (int i = 0; i < NX2; ++i) U[IDX(o, i, k, j)] = s.g[i + (size_t)NX2 * j];
+    }
+    free(s.a); free(s.b); free(s.c); free(s.g);
+  }
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k)
+      for (int i = 0; i < NX2; ++i) {
+        size_t q = IDX(o, i, k, j), dj = sy;
+        R[q] = 0.5 * S1[q] + JAC(o, k, j) * U[q] +
+               t1 * (G22(o, k, j + 1, 1) * (U[q + dj] - U[q]) - G22(o, k, j, 1) * (U[q] - U[q - dj]));
+      }
+  if (comp < 3) /* IC_TYPE 7: R(I,K,NY+1) = R(I,K,NY) (:1787-1793, :2023-2029) */
+    for (int k = zs; k <= ze; ++k)
+      for (int i = 0; i < NX2; ++i) R[IDX(o, i, k, NY + 1)] = R[IDX(o, i, k, NY)];
+  /* z sweeps, lines J */
+  int hi = comp < 3 ? o->p.u_bc[comp][EKM_ZMAX] : o->p.th_bc[EKM_ZMAX];
+#pragma omp parallel
+  {
+    size_t n = (size_t)NX2 * (NZ + 2);
+    sysb s = {dalloc(n), dalloc(n), dalloc(n), dalloc(n), (size_t)NX2};
+    for (size_t t = 0; t < n; ++t) s.b[t] = 1.0;
+#pragma omp for schedule(static)
+    for (int j = js; j <= je; ++j) {
+      for (int k = zs; k <= ze; ++k) {
+        double l = -(t1 * G11(o, k, j, 2)), d = JAC(o, k, j) + t1 * (G11(o, k + 1, j, 2) + G11(o, k, j, 2));
+        double u = -(t1 * G11(o, k + 1, j, 2));
+        for (int i = 0; i < NX2; ++i) {
+          size_t r = i + (size_t)NX2 * k;
+          s.a[r] = l; s.b[r] = d; s.c[r] = u; s.g[r] = R[IDX(o, i, k, j)];
+        }
+      }
+      bc_z(o, comp, j, &s, U, IDX(o, 0, 0, j), (size_t)NX2);
+      if (hi == 5) {
+        double D = comp == 0 ? G11(o, NZ, j, 2) : 1.0;
+        ora_thomas_real_sp(s.a, s.b, s.c, s.g, D, NZ, NZ + 1, NX2 - 1);
+      } else {
+        ora_thomas_real(s.a, s.b, s.c, s.g, NZ + 1, NX2 - 1);
+      }
+      for (int k = 0; k <= NZ + 1; ++k)
+        for (int i = 0; i < NX2; ++i) U[IDX(o, i, k, j)] = s.g[i + (size_t)NX2 * k];
+    }
+    free(s.a); free(s.b); free(s.c); free(s.g);
+  }
+  if (comp == 3) { /* corners (:1606-1622) */
+    if (o->p.th_bc[EKM_ZMAX] == 1)
+      for (int i = 0; i < NX2; ++i) {
+        U[IDX(o, i, NZ + 1, NY + 1)] = U[IDX(o, i, NZ, NY + 1)];
+        U[IDX(o, i, NZ + 1, 0)] = U[IDX(o, i, NZ, 0)];
+        U[IDX(o, i, 0, NY + 1)] = U[IDX(o, i, 1, NY + 1)];
+        U[IDX(o, i, 0, 0)] = U[IDX(o, i, 1, 0)];
+      }
+    if (o->p.th_bc[EKM_ZMIN] == 6)
+      for (int i = 0; i < NX2; ++i) {
+        U[IDX(o, i, NZ + 1, NY + 1)] = U[IDX(o, i, 1, NY + 1)];
+        U[IDX(o, i, NZ + 1, 0)] = U[IDX(o, i, 1, 0)];
+        U[IDX(o, i, 0, NY + 1)] = U[IDX(o, i, NZ, NY + 1)];
+        U[IDX(o, i, 0, 0)] = U[IDX(o, i, NZ, 0)];
+      }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* nonlinear fluxes (src/duct.F90:683-1120, :1169-1263)                      */
+/* ------------------------------------------------------------------------ */
+static void contravariant(ora* o, const double* U2, const double* U3) {
+  int NX2 = o->NX2;
+#pragma omp parallel for schedule(static)
+  for (int j = o->jstart; j <= o->jend + 1; ++j)
+    for (int k = o->zstart; k <= o->zend + 1; ++k)
+      for (int i = 0; i < NX2; ++i) {
+        size_t q = IDX(o, i, k, j), dj = (size_t)NX2 * o->NZ2, dk = NX2;
+        o->U2b[q] = 0.5 * C22(o, k, j, 1) * (U2[q] + U2[q - dj]) + 0.5 * C21(o, k, j, 1) * (U3[q] + U3[q - dj]);
+        o->U3b[q] = 0.5 * C11(o, k, j, 2) * (U3[q] + U3[q - dk]) + 0.5 * C12(o, k, j, 2) * (U2[q] + U2[q - dk]);
+      }
+}
+
+/* S = J * A * B on the range, zero elsewhere */
+static void xflux(ora* o, double* S, const double* A, const double* B, int js, int je, int zs, int ze) {
+  memset(S, 0, o->npts * sizeof(double));
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k)
+      for (int i = 0; i < o->NX2; ++i) {
+        size_t q = IDX(o, i, k, j);
+        S[q] = JAC(o, k, j) * A[q] * B[q];
+      }
+}
+
+/* S = -(d(U2b q)/deta) + nu d/dzeta(G12 dq/deta) - (d(U3b q)/dzeta) + nu d/deta(G12 dq/dzeta) */
+static void yzflux(ora* o, double* S, const double* Q, double nu, int js, int je, int zs, int ze) {
+  memset(S, 0, o->npts * sizeof(double));
+  int NX2 = o->NX2;
+  size_t dj = (size_t)NX2 * o->NZ2, dk = NX2;
+  const double *U2b = o->U2b, *U3b = o->U3b;
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k)
+      for (int i = 0; i < NX2; ++i) {
+        size_t q = IDX(o, i, k, j);
+        double s1 = 0.5 * U2b[q + dj] * (Q[q] + Q[q + dj]) - 0.5 * U2b[q] * (Q[q] + Q[q - dj]);
+        double s2 = 0.5 * U3b[q + dk] * (Q[q] + Q[q + dk]) - 0.5 * U3b[q] * (Q[q] + Q[q - dk]);
+        s1 = -s1 + 0.25 * nu *
+                       (G12(o, k + 1, j, 2) * (Q[q + dk + dj] + Q[q + dj] - Q[q - dj] - Q[q + dk - dj]) -
+                        G12(o, k, j, 2) * (Q[q - dk + dj] + Q[q + dj] - Q[q - dj] - Q[q - dk - dj]));
+        s2 = -s2 + 0.25 * nu *
+                       (G12(o, k, j + 1, 1) * (Q[q + dk + dj] + Q[q + dk] - Q[q - dk + dj] - Q[q - dk]) -
+                        G12(o, k, j, 1) * (Q[q + dk] + Q[q + dk - dj] - Q[q - dk] - Q[q - dk - dj]));
+        S[q] = s1 + s2;
+      }
+}
+
+/* CF += sgn * (i kx)^p * FFT(S) on the range (p = 1: x-derivative flux) */
+static void accum(ora* o, double* CF, double* S, int ikx, int js, int je, int zs, int ze) {
+  fwd(o, S);
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k)
+      for (int m = 0; m < o->NXC; ++m) {
+        size_t q = CRE(o, m, k, j);
+        if (ikx) { /* CF = CF - CIKXP * CS1 */
+          double kx = o->kx[m];
+          CF[q] = CF[q] - (0.0 * S[q] - kx * S[q + 1]);
+          CF[q + 1] = CF[q + 1] - (0.0 * S[q + 1] + kx * S[q]);
+        } else {
+          CF[q] += S[q];
+          CF[q + 1] += S[q + 1];
+        }
+      }
+}
+
+/* ------------------------------------------------------------------------ */
+/* REM_DIV_CURV (src/duct.F90:2472-2818)                                     */
+/* ------------------------------------------------------------------------ */
+static int rem_div(ora* o) {
+  int NXC = o->NXC, NKX = o->NKX, NY = o->NY, NZ = o->NZ;
+  int js = o->jstart, je = o->jend, zs = o->zstart, ze = o->zend;
+  double *CU1 = o->U[0], *CU2 = o->U[1], *CU3 = o->U[2];
+  double *CR1 = o->PHI, *CS1 = o->CS1, *CU2b = o->U2b, *CU3b = o->U3b;
+  size_t dj = (size_t)o->NX2 * o->NZ2, dk = o->NX2;
+  memset(CR1, 0, o->npts * sizeof(double));
+  memset(CS1, 0, o->npts * sizeof(double));
+  memset(CU2b, 0, o->npts * sizeof(double));
+  memset(CU3b, 0, o->npts * sizeof(double));
+  /* :2511-2522 */
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je + 1; ++j)
+    for (int k = zs; k <= ze + 1; ++k)
+      for (int m = 0; m < NXC; ++m)
+        for (int c = 0; c < 2; ++c) {
+          size_t q = CRE(o, m, k, j) + c;
+          CU2b[q] = 0.5 * C22(o, k, j, 1) * (CU2[q] + CU2[q - dj]) + 0.5 * C21(o, k, j, 1) * (CU3[q] + CU3[q - dj]);
+          CU3b[q] = 0.5 * C11(o, k, j, 2) * (CU3[q] + CU3[q - dk]) + 0.5 * C12(o, k, j, 2) * (CU2[q] + CU2[q - dk]);
+        }
+  /* :2527-2535 */
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; ++j)
+    for (int k = zs - 1; k <= ze; ++k)
+      for (int m = 0; m < NXC; ++m) {
+        size_t q = CRE(o, m, k, j);
+        double kj = o->kx[m] * JAC(o, k, j);
+        /* CIKXP*J*CU1: (0 + i kx) J (a + i b) */
+        double re = (0.0 * CU1[q] - kj * CU1[q + 1]) + (CU2b[q + dj] - CU2b[q]) + (CU3b[q + dk] - CU3b[q]);
+        double im = (0.0 * CU1[q + 1] + kj * CU1[q]) + (CU2b[q + 1 + dj] - CU2b[q + 1]) + (CU3b[q + 1 + dk] - CU3b[q + 1]);
+        CR1[q] = re; CR1[q + 1] = im;
+        CS1[q] = re; CS1[q + 1] = im;
+      }
+  /* :2620-2668, modes above NKX have zero right-hand sides and solve to 0 */
+  size_t pl = (size_t)o->NZ2 * o->NY2;
+  int cyc_sum = 0, cyc_max = 0, nsolve = 0;
+#pragma omp parallel reduction(+ : cyc_sum, nsolve) reduction(max : cyc_max)
+  {
+    double* rhs = dalloc(pl);
+    double* pim = dalloc(pl);
+    double* pre = dalloc(pl);
+    mg_ctx c;
+    mg_setup(&c, o);
+#pragma omp for schedule(dynamic, 1)
+    for (int m = 0; m <= NKX; ++m) {
+      for (int part = 0; part < 2; ++part) { /* imaginary part first */
+        memset(rhs, 0, pl * sizeof(double));
+        for (int k = zs; k <= ze; ++k)
+          for (int j = js; j <= je; ++j)
+            rhs[PL(o, k, j)] = -CR1[CRE(o, m, k, j) + (part == 0 ? 1 : 0)];
+        int cy = mg_solve(&c, o->kx2[m], rhs, part == 0 ? pim : pre);
+        cyc_sum += cy;
+        nsolve += 1;
+        if (cy > cyc_max) cyc_max = cy;
+      }
+      for (int j = 0; j <= NY + 1; ++j)
+        for (int k = 0; k <= NZ + 1; ++k) {
+          double a = pre[PL(o, k, j)], b = pim[PL(o, k, j)];
+          if (o->p.cmplx_fp32) { a = (double)(float)a; b = (double)(float)b; }
+          CR1[CRE(o, m, k, j)] = a;
+          CR1[CRE(o, m, k, j) + 1] = b;
+        }
+    }
+    mg_free(&c);
+    free(rhs); free(pim); free(pre);
+  }
+  o->diag.mg_solves = nsolve;
+  o->diag.mg_cycles = cyc_sum;
+  o->diag.mg_max_cycles = cyc_max;
+  /* :2713-2724 face velocities */
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k)
+      for (int m = 0; m < NXC; ++m)
+        for (int c = 0; c < 2; ++c) {
+          size_t q = CRE(o, m, k, j) + c;
+          CU2b[q] = CU2b[q] - G22(o, k, j, 1) * (CR1[q] - CR1[q - dj]) -
+                    0.25 * G12(o, k, j, 1) * (CR1[q + dk] + CR1[q + dk - dj] - CR1[q - dk] - CR1[q - dk - dj]);
+          CU3b[q] = CU3b[q] - G11(o, k, j, 2) * (CR1[q] - CR1[q - dk]) -
+                    0.25 * G12(o, k, j, 2) * (CR1[q + dj] + CR1[q - dk + dj] - CR1[q - dj] - CR1[q - dk - dj]);
+        }
+  /* :2729-2750 cell-centre velocities */
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k)
+      for (int m = 0; m < NXC; ++m) {
+        size_t q = CRE(o, m, k, j);
+        double kx = o->kx[m];
+        double pr = CR1[q], pi = CR1[q + 1];
+        CU1[q] = CU1[q] - (0.0 * pr - kx * pi);
+        CU1[q + 1] = CU1[q + 1] - (0.0 * pi + kx * pr);
+        for (int c = 0; c < 2; ++c) {
+          size_t r = q + c;
+          CU2[r] = CU2[r] - 0.5 * (C22(o, k, j + 1, 1) * (CR1[r] + CR1[r + dj]) - C22(o, k, j, 1) * (CR1[r] + CR1[r - dj]) +
+                                   C12(o, k + 1, j, 2) * (CR1[r + dk] + CR1[r]) - C12(o, k, j, 2) * (CR1[r] + CR1[r - dk])) /
+                                JAC(o, k, j);
+          CU3[r] = CU3[r] - 0.5 * (C11(o, k + 1, j, 2) * (CR1[r] + CR1[r + dk]) - C11(o, k, j, 2) * (CR1[r] + CR1[r - dk]) +
+                                   C21(o, k, j + 1, 1) * (CR1[r + dj] + CR1[r]) - C21(o, k, j, 1) * (CR1[r] + CR1[r - dj])) /
+                                JAC(o, k, j);
+        }
+      }
+  /* :2754-2800 divergence after the projection */
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k)
+      for (int m = 0; m < NXC; ++m) {
+        size_t q = CRE(o, m, k, j);
+        double kj = o->kx[m] * JAC(o, k, j);
+        CS1[q] = (0.0 * CU1[q] - kj * CU1[q + 1]) + (CU2b[q + dj] - CU2b[q]) + (CU3b[q + dk] - CU3b[q]);
+        CS1[q + 1] = (0.0 * CU1[q + 1] + kj * CU1[q]) + (CU2b[q + 1 + dj] - CU2b[q + 1]) + (CU3b[q + 1 + dk] - CU3b[q + 1]);
+      }
+  inv(o, CS1);
+  double div = 0.0, dmax = 0.0;
+  for (int j = 2; j <= NY - 1; ++j)
+    for (int k = 2; k <= NZ - 1; ++k)
+      for (int i = 1; i <= o->NX + 1; ++i) {
+        double v = fabs(CS1[IDX(o, i, k, j)]);
+        div += v;
+        if (v > dmax) dmax = v;
+      }
+  div = div / ((double)(o->NX + 1) * (double)NY * (double)NZ);
+  o->diag.div_ref = div;
+  o->diag.div_max = dmax;
+  if (isnan(div)) return EKM_ERR_DIV_NAN;
+  if (o->delta_t < 1.0e-4) return EKM_ERR_DT_SMALL;
+  return EKM_OK;
+}
+
+int ora_rem_div_curv(ora* o) { return rem_div(o); }
+int ora_get_phi(ora* o, double* phi) { memcpy(phi, o->PHI, o->npts * sizeof(double)); return EKM_OK; }
+
+/* ------------------------------------------------------------------------ */
+/* courant_curv_mpi (src/duct.F90:3605-3674)                                 */
+/* ------------------------------------------------------------------------ */
+static void courant(ora* o) {
+  contravariant(o, o->U[1], o->U[2]);
+  double dx = o->p.lx / (double)o->NX, cfl = o->p.cfl, dt = 999.0;
+  for (int j = o->jstart; j <= o->jend; ++j)
+    for (int k = o->zstart; k <= o->zend; ++k)
+      for (int i = 0; i < o->NX2; ++i) {
+        size_t q = IDX(o, i, k, j);
+        double dtx = cfl * dx / fabs(o->U[0][q]);
+        double dty = cfl * JAC(o, k, j) / fabs(o->U2b[q]);
+        double dtz = cfl * JAC(o, k, j) / fabs(o->U3b[q]);
+        dt = fmin(dt, fmin(dtx, fmin(dty, dtz)));
+      }
+  if (dt <= 0) return;
+  o->delta_t = dt >= o->delta_t_set ? o->delta_t_set : dt;
+  set_hbar(o);
+}
+
+/* ------------------------------------------------------------------------ */
+/* RK_DUCT (src/duct.F90:237-2469)                                           */
+/* ------------------------------------------------------------------------ */
+int ora_rk_duct(ora* o) {
+  const ekm_params* p = &o->p;
+  int rk = o->rk_step, NXC = o->NXC;
+  int js = o->jstart, je = o->jend, zs = o->zstart, ze = o->zend;
+  int cur_step = o->time_step + 1;
+  double hb = o->h_bar[rk - 1];
+  double TEMP1 = p->nu * hb / 2.0, TEMP3 = o->zeta_bar[rk - 1] * hb, TEMP4 = hb;
+  double TEMP5 = o->beta_bar[rk - 1] * hb;
+  double *CU1 = o->U[0], *CU2 = o->U[1], *CU3 = o->U[2], *CP = o->P, *CTH = o->TH;
+  size_t dj = (size_t)o->NX2 * o->NZ2, dk = o->NX2;
+  int th = p->n_th > 0;
+  /* :298-455 explicit Fourier-space RHS */
+  for (int c = 0; c < 3; ++c) memset(o->R[c], 0, o->npts * sizeof(double));
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k)
+      for (int m = 0; m < NXC; ++m)
+        for (int c = 0; c < 2; ++c) {
+          size_t q = CRE(o, m, k, j) + c;
+          double Jc = JAC(o, k, j);
+          for (int f = 0; f < 3; ++f) {
+            o->R[f][q] = Jc * o->U[f][q];
+            if (rk > 1) o->R[f][q] = o->R[f][q] + TEMP3 * o->CF[f][q];
+          }
+          double cs1 = 0.5 * C11(o, k + 1, j, 2) * (CP[q] + CP[q + dk]) - 0.5 * C11(o, k, j, 2) * (CP[q] + CP[q - dk]) +
+                       0.5 * C21(o, k, j + 1, 1) * (CP[q] + CP[q + dj]) - 0.5 * C21(o, k, j, 1) * (CP[q] + CP[q - dj]);
+          double cs2 = 0.5 * C12(o, k + 1, j, 2) * (CP[q] + CP[q + dk]) - 0.5 * C12(o, k, j, 2) * (CP[q] + CP[q - dk]) +
+                       0.5 * C22(o, k, j + 1, 1) * (CP[q] + CP[q + dj]) - 0.5 * C22(o, k, j, 1) * (CP[q] + CP[q - dj]);
+          o->PHI[q] = cs2; /* scratch: CS2X */
+          o->CS1[q] = cs1;
+        }
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k)
+      for (int m = 0; m < NXC; ++m) {
+        size_t q = CRE(o, m, k, j);
+        double b = TEMP4 * o->kx[m] * JAC(o, k, j);
+        o->R[0][q] = o->R[0][q] - (0.0 * CP[q] - b * CP[q + 1]);
+        o->R[0][q + 1] = o->R[0][q + 1] - (0.0 * CP[q + 1] + b * CP[q]);
+        for (int c = 0; c < 2; ++c) {
+          o->R[1][q + c] = o->R[1][q + c] - TEMP4 * o->PHI[q + c];
+          o->R[2][q + c] = o->R[2][q + c] - TEMP4 * o->CS1[q + c];
+        }
+      }
+  if (p->f_type == 1 || p->f_type == 2) { /* rank 0 owns mode 0 (:411-455) */
+    double px = p->f_type == 1 ? p->px0 : (p->px0 + p->amp_omega0 * cos(p->omega0 * o->time));
+    for (int j = js; j <= je; ++j)
+      for (int k = zs; k <= ze; ++k) {
+        size_t q = CRE(o, 0, k, j);
+        o->R[2][q] = o->R[2][q] - TEMP4 * px * JAC(o, k, j);
+      }
+  }
+  /* :461-475 horizontal viscous terms and Coriolis */
+  double ug = p->u_bc_c1[0][EKM_YMAX], wg = p->wbulk0;
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k)
+      for (int m = 0; m < NXC; ++m) {
+        size_t q = CRE(o, m, k, j);
+        double Jc = JAC(o, k, j), f = p->nu * o->kx2[m] * Jc;
+        for (int c = 0; c < 2; ++c)
+          for (int fld = 0; fld < 3; ++fld) o->CF[fld][q + c] = -(f * o->U[fld][q + c]);
+        double og = (p->coriolis_mean_only && m > 0) ? 0.0 : ug;
+        double ow = (p->coriolis_mean_only && m > 0) ? 0.0 : wg;
+        o->CF[2][q] = o->CF[2][q] + p->f0 * (CU1[q] - og) * Jc;
+        o->CF[2][q + 1] = o->CF[2][q + 1] + p->f0 * CU1[q + 1] * Jc;
+        o->CF[0][q] = o->CF[0][q] - p->f0 * (CU3[q] - ow) * Jc;
+        o->CF[0][q + 1] = o->CF[0][q + 1] - p->f0 * CU3[q + 1] * Jc;
+      }
+  if (th) {
+    double nupr = p->nu / p->pr;
+    /* :517-523 buoyancy */
+#pragma omp parallel for schedule(static)
+    for (int j = js; j <= je; ++j)
+      for (int k = zs; k <= ze; ++k)
+        for (int m = 0; m < NXC; ++m)
+          for (int c = 0; c < 2; ++c) {
+            size_t q = CRE(o, m, k, j) + c;
+            o->CF[1][q] = o->CF[1][q] - JAC(o, k, j) * p->ri_tau * CTH[q];
+          }
+    /* :532-632 scalar RHS */
+    memset(o->RTH, 0, o->npts * sizeof(double));
+    const double* TB = o->thbar;
+    size_t tk = 1, tj = o->NZ2;
+#pragma omp parallel for schedule(static)
+    for (int j = o->jstart_th; j <= o->jend_th; ++j)
+      for (int k = o->zstart_th; k <= o->zend_th; ++k)
+        for (int m = 0; m < NXC; ++m) {
+          size_t q = CRE(o, m, k, j);
+          double Jc = JAC(o, k, j);
+          for (int c = 0; c < 2; ++c) {
+            o->RTH[q + c] = Jc * CTH[q + c];
+            if (rk > 1) o->RTH[q + c] = o->RTH[q + c] + TEMP3 * o->CFTH[q + c];
+            o->CFTH[q + c] = -nupr * Jc * o->kx2[m] * CTH[q + c];
+          }
+          if (p->cont_strat) {
+            for (int c = 0; c < 2; ++c) o->CFTH[q + c] = o->CFTH[q + c] + Jc * CU2[q + c];
+          } else {
+            size_t t = PL(o, k, j);
+            double gb = 0.5 * C12(o, k + 1, j, 2) * (TB[t] + TB[t + tk]) - 0.5 * C12(o, k, j, 2) * (TB[t] + TB[t - tk]) +
+                        0.5 * C22(o, k, j + 1, 1) * (TB[t] + TB[t + tj]) - 0.5 * C22(o, k, j, 1) * (TB[t] + TB[t - tj]);
+            for (int c = 0; c < 2; ++c) o->CFTH[q + c] = o->CFTH[q + c] - CU2[q + c] * gb;
+            double dif = 0.25 * nupr *
+                         (+G12(o, k + 1, j, 2) * (TB[t + tk + tj] + TB[t + tj] - TB[t - tj] - TB[t + tk - tj]) -
+                          G12(o, k, j, 2) * (TB[t - tk + tj] + TB[t + tj] - TB[t - tj] - TB[t - tk - tj]) +
+                          G12(o, k, j + 1, 1) * (TB[t + tk + tj] + TB[t + tk] - TB[t - tk + tj] - TB[t - tk]) -
+                          G12(o, k, j, 1) * (TB[t + tk] + TB[t + tk - tj] - TB[t - tk] - TB[t - tk - tj]) +
+                          4.0 * (G11(o, k + 1, j, 2) * (TB[t + tk] - TB[t]) - G11(o, k, j, 2) * (TB[t] - TB[t - tk])) +
+                          4.0 * (G22(o, k, j + 1, 1) * (TB[t + tj] - TB[t]) - G22(o, k, j, 1) * (TB[t] - TB[t - tj])));
+            o->CFTH[q] = o->CFTH[q] + dif;
+          }
+        }
+  }
+  /* :655-671 velocities and scalar to physical space */
+  for (int f = 0; f < 3; ++f) inv(o, o->U[f]);
+  if (th) inv(o, CTH);
+  double *U1 = CU1, *U2 = CU2, *U3 = CU3;
+  /* :683-1120 momentum fluxes */
+  contravariant(o, U2, U3);
+  for (int f = 0; f < 3; ++f) {
+    xflux(o, o->S1, U1, o->U[f], js, je, zs, ze);
+    accum(o, o->CF[f], o->S1, 1, js, je, zs, ze);
+    yzflux(o, o->S1, o->U[f], p->nu, js, je, zs, ze);
+    accum(o, o->CF[f], o->S1, 0, js, je, zs, ze);
+  }
+  /* :1127-1161 */
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k)
+      for (int m = 0; m < NXC; ++m)
+        for (int c = 0; c < 2; ++c)
+          for (int f = 0; f < 3; ++f) {
+            size_t q = CRE(o, m, k, j) + c;
+            o->R[f][q] = o->R[f][q] + TEMP5 * o->CF[f][q];
+          }
+  for (int f = 0; f < 3; ++f) inv(o, o->R[f]);
+  if (th) { /* :1169-1625 */
+    int tjs = o->jstart_th, tje = o->jend_th, tzs = o->zstart_th, tze = o->zend_th;
+    xflux(o, o->S1, CTH, U1, tjs, tje, tzs, tze);
+    accum(o, o->CFTH, o->S1, 1, tjs, tje, tzs, tze);
+    yzflux(o, o->S1, CTH, p->nu / p->pr, tjs, tje, tzs, tze);
+    accum(o, o->CFTH, o->S1, 0, tjs, tje, tzs, tze);
+#pragma omp parallel for schedule(static)
+    for (int j = tjs; j <= tje; ++j)
+      for (int k = tzs; k <= tze; ++k)
+        for (int m = 0; m < NXC; ++m)
+          for (int c = 0; c < 2; ++c) {
+            size_t q = CRE(o, m, k, j) + c;
+            o->RTH[q] = o->RTH[q] + TEMP5 * o->CFTH[q];
+          }
+    inv(o, o->RTH);
+    adi(o, 3, CTH, o->RTH, TEMP1 / p->pr);
+  }
+  /* :1640-2391 velocity ADI */
+  for (int f = 0; f < 3; ++f) adi(o, f, o->U[f], o->R[f], TEMP1);
+  /* :2417-2419 */
+  if (p->variable_dt && rk == 3 && (cur_step % p->update_dt) == 0) courant(o);
+  /* :2421-2451 */
+  for (int f = 0; f < 3; ++f) fwd(o, o->U[f]);
+  if (th) fwd(o, CTH);
+  int st = rem_div(o);
+  /* :2460-2466 pressure update */
+  double *PHI = o->PHI;
+  for (int j = js - 1; j <= je + 1; ++j)
+    for (int k = zs - 1; k <= ze + 1; ++k)
+      for (int m = 0; m < NXC; ++m)
+        for (int c = 0; c < 2; ++c) {
+          size_t q = CRE(o, m, k, j) + c;
+          CP[q] = CP[q] + PHI[q] / TEMP4;
+        }
+  (void)U1;
+  o->diag.status = st;
+  if (rk == 3) {
+    o->time = o->time + o->delta_t;
+    o->time_step = cur_step;
+    o->rk_step = 1;
+  } else {
+    o->rk_step = rk + 1;
+  }
+  return st;
+}
+
+int ora_time_step(ora* o, int nsteps) {
+  for (int n = 0; n < nsteps; ++n)
+    for (int s = 0; s < 3; ++s) {
+      int st = ora_rk_duct(o);
+      if (st != EKM_OK) return st;
+    }
+  return EKM_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* state in/out                                                              */
+/* ------------------------------------------------------------------------ */
+int ora_set_state(ora* o, const double* cu1, const double* cu2, const double* cu3,
+                  const double* cp, const double* cth, double time, int time_step) {
+  size_t n = o->npts * sizeof(double);
+  memcpy(o->U[0], cu1, n); memcpy(o->U[1], cu2, n); memcpy(o->U[2], cu3, n);
+  memcpy(o->P, cp, n);
+  if (o->p.n_th && cth) memcpy(o->TH, cth, n);
+  o->time = time;
+  o->time_step = time_step;
+  o->rk_step = 1;
+  return EKM_OK;
+}
+
+int ora_get_state(ora* o, double* cu1, double* cu2, double* cu3, double* cp, double* cth) {
+  size_t n = o->npts * sizeof(double);
+  if (cu1) memcpy(cu1, o->U[0], n);
+  if (cu2) memcpy(cu2, o->U[1], n);
+  if (cu3) memcpy(cu3, o->U[2], n);
+  if (cp) memcpy(cp, o->P, n);
+  if (cth && o->p.n_th) memcpy(cth, o->TH, n);
+  return EKM_OK;
+}
+
+/* CREATE_FLOW_DUCT tail (src/duct.F90:3089-3189): forward transforms of the
+ * physical initial fields, then the initial projection; CP starts at zero. */
+int ora_set_initial_physical(ora* o, const double* u1, const double* u2, const double* u3,
+                             const double* th) {
+  size_t n = o->npts * sizeof(double);
+  memcpy(o->U[0], u1, n); memcpy(o->U[1], u2, n); memcpy(o->U[2], u3, n);
+  for (int f = 0; f < 3; ++f) fwd(o, o->U[f]);
+  if (o->p.n_th && th) { memcpy(o->TH, th, n); fwd(o, o->TH); }
+  memset(o->P, 0, n);
+  for (int f = 0; f < 3; ++f) memset(o->CF[f], 0, n);
+  memset(o->CFTH, 0, n);
+  o->time = 0.0; o->time_step = 0; o->rk_step = 1;
+  return rem_div(o);
+}
+
+int ora_get_diag(ora* o, ekm_diag* d) {
+  *d = o->diag;
+  d->delta_t = o->delta_t;
+  d->time = o->time;
+  d->time_step = o->time_step;
+  d->rk_step = o->rk_step;
+  return EKM_OK;
+}

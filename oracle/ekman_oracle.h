@@ -1,0 +1,82 @@
+/*
+ * ekman_oracle.h -- CPU FP64 oracle of the DIABLO duct RK3 substep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithm (jortizta/ekman_bl: src/duct.F90, src/boundary.F90,
+ * src/mg_solver2d.F90, src/mud2cr.f, src/mudcom.f, src/diablo.F90) used as the
+ * checker of the HIP path.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product never links it.
+ *
+ * Pinning (what checks each function; "parity unpinned" marks a function no
+ * independent check covers):
+ *   ora_fft_fwd / ora_fft_inv   numpy rfft/irfft with 1/NX scaling, 2/3 rule,
+ *                               derivative of sin(kx) (tests/test_oracle_fft.py)
+ *   ora_thomas_real(_sp)        dense numpy.linalg.solve (tests/test_oracle_thomas.py)
+ *   ora_metrics                 closed forms on a rectilinear grid, the 1e-8
+ *                               truncation, rundir_unst/GRID_YZ.dat fixture
+ *                               (tests/test_oracle_metrics.py)
+ *   ora_mudpack                 the reference MUDPACK itself (mud2cr.f+mudcom.f
+ *                               compiled into oracle/_ref) at the iterate level,
+ *                               plus a dense scipy solve of the assembled
+ *                               9-point operator (tests/test_oracle_mudpack.py)
+ *   ora_rk_duct                 laminar Ekman-spiral steady state, inertial
+ *                               oscillation period, Stokes-layer decay, the
+ *                               projection invariant J div u = MG residual
+ *                               (tests/test_oracle_physics.py).  The composed
+ *                               operation order of RK_DUCT (lagged periodic
+ *                               rows, corner updates) has no golden vectors:
+ *                               parity unpinned beyond these invariants.
+ */
+#ifndef EKMAN_ORACLE_H
+#define EKMAN_ORACLE_H
+#include "../include/ekman_bl_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ora ora;
+
+ora* ora_create(const ekm_params* p, const double* xpoint, const double* ypoint,
+                const double* th_bar);
+void ora_destroy(ora* o);
+int ora_set_state(ora* o, const double* cu1, const double* cu2, const double* cu3,
+                  const double* cp, const double* cth, double time, int time_step);
+int ora_get_state(ora* o, double* cu1, double* cu2, double* cu3, double* cp, double* cth);
+int ora_set_initial_physical(ora* o, const double* u1, const double* u2,
+                             const double* u3, const double* th);
+int ora_rk_duct(ora* o);
+int ora_time_step(ora* o, int nsteps);
+int ora_rem_div_curv(ora* o);
+int ora_get_phi(ora* o, double* phi);
+int ora_get_diag(ora* o, ekm_diag* d);
+/* metrics: which = 0..6 -> GMAT_11,GMAT_12,GMAT_22,CJOB_11,CJOB_12,CJOB_21,CJOB_22
+ * ((NZ+3)*(NY+3)*2, K fastest then J then face), 7 -> INT_JACOB ((NZ+3)*(NY+3)). */
+int ora_get_metric(ora* o, int which, double* out);
+
+/* components */
+void ora_fft_fwd(double* varp, int nx, long npencil);
+void ora_fft_inv(double* varp, int nx, long npencil);
+void ora_thomas_real(double* a, double* b, double* c, double* g, int ny, int nx);
+void ora_thomas_real_sp(double* a, double* b, double* c, double* g, double d, int k,
+                        int ny, int nx);
+/* MULTIGRID_MUDPACK for one right-hand side: rhs_tmp, phi are (NZ+2)*(NY+2)
+ * planes, K fastest.  Returns the cycle count (iparm(17)). */
+int ora_mudpack(ora* o, double kx2, const double* rhs_tmp, double* phi);
+/* Assembled level-`lev` coefficients cf(i,j,1..10) as MUDPACK stores them
+ * (lev = 1 coarsest .. ngrid finest), for kx2; returns nx*ny or <0. */
+int ora_mg_discretize(ora* o, double kx2, int lev, double* cf, int* nx, int* ny);
+int ora_mg_ngrid(ora* o);
+
+/* Callbacks with MUDPACK's calling convention (all arguments by reference),
+ * bound to the oracle's metrics for the reference build in oracle/_ref. */
+void ora_cb_bind(ora* o, double kx2);
+void ora_cofcr_cb(const double* x, const double* y, double* cxx, double* cxy, double* cyy,
+                  double* cx, double* cy, double* ce);
+void ora_bndcr_cb(const int* kbdy, const double* xory, double* alfa, double* beta,
+                  double* gama, double* gbdy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
