@@ -91,7 +91,8 @@ struct HostMetrics;
 struct MgLevel {
   int nx = 0, ny = 0;
   size_t phi_off = 0, rhs_off = 0;  // offsets inside a solve's workspace (doubles)
-  const double* coef = nullptr;     // c1..c8 [8][ny][nx]
+  size_t c9_off = 0, cf_off = 0;    // per-solve c9; coefficient planes (LDS levels only)
+  const double* coef = nullptr;     // c1..c8 interleaved [ny][nx][8]
   const double* s9 = nullptr;       // c1+c3+c5+c7 before boundary folding
   const double* corr9 = nullptr;    // additive corner correction of c9
   const double* jl = nullptr;       // INT_JACOB at the level's points (0 at specified points)
@@ -111,6 +112,7 @@ struct MgPlan {
   double* ws = nullptr;      // global workspace [nsolve_max][ws_global]
   double* coefmem = nullptr; // all level coefficient arrays
   int* cycles = nullptr;     // [nsolve_max]
+  void* desc = nullptr;      // device copy of the level descriptor
   double* kx2 = nullptr;     // [nsolve_max]
   double* rhs = nullptr;     // [nsolve_max][plane] RHS_TMP
   double* phi = nullptr;     // [nsolve_max][plane]

@@ -249,7 +249,7 @@ __global__ void k_lu(Dims d, Metrics M, Ranges r, BcTable bc, double t1, int dir
   int N = dir == 0 ? d.NY + 1 : d.NZ + 1;  // last row index
   int lo = dir == 0 ? r.js : r.zs, hi = dir == 0 ? r.je : r.ze;
   size_t base = (size_t)L * (N + 1);
-  double bprev = 0.0, cprev = 0.0;
+  double bprev = 0.0, cprev = 0.0, pvprev = 1.0;
   double a_nz = 0.0;
   for (int row = 0; row <= N; ++row) {
     double a = 0.0, b = 1.0, c = 0.0;
@@ -282,14 +282,15 @@ __global__ void k_lu(Dims d, Metrics M, Ranges r, BcTable bc, double t1, int dir
     if (row == N - 1) a_nz = a;
     double mlt = 0.0, pv = b;
     if (row > 0) {
-      mlt = -a / piv[base + row - 1];
-      pv = b + mlt * up[base + row - 1];
+      mlt = -a / pvprev;
+      pv = b + mlt * cprev;
     }
     mult[base + row] = mlt;
     piv[base + row] = pv;
     up[base + row] = c;
     bprev = b;
     cprev = c;
+    pvprev = pv;
   }
 }
 

@@ -1,7 +1,9 @@
 // ekm_solver.hip -- device-resident duct solver and the C ABI
 // (include/ekman_bl_amd.h).  One solver owns one HIP stream and all of its
 // fields; host<->device copies happen only in set/get calls.
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -46,6 +48,11 @@ struct ekm_solver {
   int rk_step = 1, time_step = 0;
   ekm_diag diag;
   std::vector<int> cyc_host;
+  // live timing of the multigrid launch with HIP events on the solver stream
+  int prof = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double prof_ms = 0.0;
+  long prof_launches = 0, prof_solves = 0, prof_cycles = 0;
 };
 
 extern "C" {
@@ -119,6 +126,8 @@ static int free_all(ekm_solver* s) {
     if (q) (void)hipFree(q);
   fft_plan_free(s->fft);
   mg_plan_free(s->mg);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->st) (void)hipStreamDestroy(s->st);
   return EKM_OK;
 }
@@ -204,7 +213,20 @@ extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const doub
   if (hipMalloc(&s->red, 4 * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
   if (fft_plan_init(s->fft, p.nx) != EKM_OK) return fail(EKM_ERR_HIP);
   int nsolve = 2 * (d.NKX + 1);
-  st = mg_plan_build(s->mg, p, d, s->hm, nsolve, 512, 8192);
+  {
+    // one multigrid block per solve; the coarse levels live in LDS, sized by
+    // what one workgroup may allocate on this device
+    int lds_bytes = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      return fail(EKM_ERR_HIP);
+    int block = 512;
+    size_t budget = (size_t)lds_bytes / sizeof(double);
+    if (const char* e = getenv("EKM_MG_BLOCK")) block = atoi(e);
+    if (const char* e = getenv("EKM_MG_LDS_DOUBLES")) budget = std::min(budget, (size_t)atoll(e));
+    if (block < 64 || block > 1024 || (block & (block - 1))) return fail(EKM_ERR_ARG);  // power of two: tree reduction
+    st = mg_plan_build(s->mg, p, d, s->hm, nsolve, block, budget);
+  }
   if (st != EKM_OK) return fail(st);
   s->cyc_host.resize(nsolve);
   s->delta_t = p.delta_t;
@@ -271,7 +293,9 @@ static int rem_div(ekm_solver* s) {
   TRY(launch_faces(d, s->M, s->rv, s->U[1], s->U[2], s->U2b, s->U3b, st));
   TRY(launch_mg_gather(d, s->M, s->rv, s->kx, s->U[0], s->U2b, s->U3b, s->mg.rhs, nmode, st));
   EKM_CHECK(hipMemcpyAsync(s->mg.kx2, s->kx2s, nsolve * sizeof(double), hipMemcpyDeviceToDevice, st));
+  if (s->prof) EKM_CHECK(hipEventRecord(s->ev0, st));
   TRY(mg_solve_ex(s->mg, d, nsolve, 0, st));
+  if (s->prof) EKM_CHECK(hipEventRecord(s->ev1, st));
   TRY(launch_mg_scatter(d, s->mg.phi, nmode, s->p.cmplx_fp32, s->PHI, st));
   TRY(launch_project(d, s->M, s->rv, s->kx, s->PHI, s->U2b, s->U3b, s->U[0], s->U[1], s->U[2], s->S1, st));
   TRY(inv(s, s->S1));
@@ -288,6 +312,14 @@ static int rem_div(ekm_solver* s) {
   s->diag.mg_solves = nsolve;
   s->diag.mg_cycles = sum;
   s->diag.mg_max_cycles = mx;
+  if (s->prof) {
+    float ms = 0.f;
+    EKM_CHECK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->prof_ms += ms;
+    s->prof_launches += 1;
+    s->prof_solves += nsolve;
+    s->prof_cycles += sum;
+  }
   if (std::isnan(s->diag.div_ref)) return EKM_ERR_DIV_NAN;
   if (s->delta_t < 1.0e-4) return EKM_ERR_DT_SMALL;
   return EKM_OK;
@@ -525,4 +557,52 @@ extern "C" int ekm_thomas_real_sp(double* a, double* b, double* c, double* g, do
   int st = thomas_host(a2.data(), b2.data(), c, g2.data(), ny, nx);
   if (st == EKM_OK) std::memcpy(g, g2.data(), g2.size() * sizeof(double));
   return st;
+}
+
+// ---- measurement hooks ----------------------------------------------------------
+extern "C" int ekm_prof_enable(ekm_solver* s, int on) {
+  if (on && !s->ev0) {
+    EKM_CHECK(hipEventCreate(&s->ev0));
+    EKM_CHECK(hipEventCreate(&s->ev1));
+  }
+  s->prof = on;
+  s->prof_ms = 0.0;
+  s->prof_launches = s->prof_solves = s->prof_cycles = 0;
+  return EKM_OK;
+}
+
+// total milliseconds of the multigrid launches since ekm_prof_enable, their
+// count, and the real solves and W cycles they performed
+extern "C" int ekm_prof_get(ekm_solver* s, double* ms, long* launches, long* solves, long* cycles) {
+  *ms = s->prof_ms;
+  *launches = s->prof_launches;
+  *solves = s->prof_solves;
+  *cycles = s->prof_cycles;
+  return EKM_OK;
+}
+
+namespace {
+__global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, size_t n) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+  for (; t < n; t += stride) dst[t] = src[t];
+}
+}  // namespace
+
+// plain device copy of n doubles: the calibration kernel for HBM byte counters
+extern "C" int ekm_copy_dev(double* dst, const double* src, size_t n, void* stream) {
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, (hipStream_t)stream, dst, src, n);
+  EKM_CHECK(hipGetLastError());
+  EKM_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  return EKM_OK;
+}
+
+// allocate / free device memory for callers without a device allocator
+extern "C" int ekm_dev_alloc(void** p, size_t bytes) {
+  EKM_CHECK(hipMalloc(p, bytes));
+  EKM_CHECK(hipMemset(*p, 0, bytes));
+  return EKM_OK;
+}
+extern "C" int ekm_dev_free(void* p) {
+  EKM_CHECK(hipFree(p));
+  return EKM_OK;
 }

@@ -7,8 +7,9 @@ rundir_unst/grid_def:6-44).  `read_input_dat` / `read_grid_def` parse those
 two files with the reference's fixed line order, so a run directory of the
 reference can be pointed at this package unchanged.
 
-`config(n)` builds the five benchmark configurations of this build (see
-DESIGN.md for how their physical description maps onto reference knobs).
+`config(n)` builds the five benchmark configurations (neutral 64x65x65,
+stratified 256x193x257, 512x257x513 and 1024x385x769) on the reference's
+boundary-condition codes.
 """
 from __future__ import annotations
 
@@ -138,7 +139,7 @@ def _ekman_bcs(p: EkmParams, ug: float) -> None:
 
 
 def config(n: int) -> Config:
-    """The benchmark configurations 1..5."""
+    """The benchmark configurations 1, 2, 3 and 5 (4 is a learned-guess study on the grid of 2)."""
     if n == 1:
         p = base_params(64, 63, 63, n_th=0)
         re_d = 500.0

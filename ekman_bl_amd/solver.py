@@ -217,3 +217,20 @@ def host_mg_coefficients(p: EkmParams, xpoint, ypoint, lev: int, kx2: float) -> 
                                           _p(cf), ctypes.byref(nx), ctypes.byref(ny)), "ekm_host_mg_coefficients")
     n = nx.value * ny.value
     return cf[: 9 * n].reshape(9, ny.value, nx.value)
+
+
+# -- measurement hooks ------------------------------------------------------------
+def prof_enable(s: DuctSolver, on: bool = True) -> None:
+    L = lib()
+    L.ekm_prof_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    _check(L.ekm_prof_enable(s.h, 1 if on else 0), "ekm_prof_enable")
+
+
+def prof_get(s: DuctSolver):
+    """(ms, launches, solves, cycles) of the multigrid launches timed since prof_enable."""
+    L = lib()
+    L.ekm_prof_get.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)] + [ctypes.POINTER(ctypes.c_long)] * 3
+    ms = ctypes.c_double()
+    a, b, c = ctypes.c_long(), ctypes.c_long(), ctypes.c_long()
+    _check(L.ekm_prof_get(s.h, ctypes.byref(ms), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "ekm_prof_get")
+    return ms.value, a.value, b.value, c.value

@@ -19,6 +19,8 @@
 #ifndef EKMAN_BL_AMD_H
 #define EKMAN_BL_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -160,6 +162,24 @@ int ekm_fft_x_to_fourier_dev(double* varp, int nx, int npencil, void* stream);
 int ekm_fft_x_to_physical_dev(double* varp, int nx, int npencil, void* stream);
 int ekm_thomas_real_dev(const double* a, const double* b, const double* c, double* g,
                         int ny, int nx, void* stream);
+
+/* ---- measurement and utility ------------------------------------------- */
+/* Time every multigrid launch of the solver with HIP events on its stream
+ * (on = 1 resets the counters). */
+int ekm_prof_enable(ekm_solver* s, int on);
+int ekm_prof_get(ekm_solver* s, double* ms, long* launches, long* solves, long* cycles);
+/* Plain device copy of n doubles (calibration kernel for HBM byte counters). */
+int ekm_copy_dev(double* dst, const double* src, size_t n, void* stream);
+int ekm_dev_alloc(void** p, size_t bytes);
+int ekm_dev_free(void* p);
+/* Host-side tables, no device needed: metrics (15*(NZ+3)*(NY+3): GMAT_11,12,22,
+ * CJOB_11,12,21,22 with two faces each, then INT_JACOB) and the multigrid
+ * coefficients cf(i,j,1..9) of level `lev` (1 = coarsest). */
+int ekm_host_metrics(int nz, int ny, const double* xpoint, const double* ypoint, double* out);
+int ekm_host_mg_coefficients(const ekm_params* p, const double* xpoint, const double* ypoint,
+                             int lev, double kx2, double* cf, int* nx, int* ny);
+/* Message of the last HIP error seen by this thread. */
+const char* ekm_last_error(void);
 
 /* Library version string. */
 const char* ekm_version(void);

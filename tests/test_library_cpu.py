@@ -1,0 +1,139 @@
+"""CPU checks of the product library and its host side (no GPU needed).
+
+- libekman_bl_amd.so loads and exports every function include/*.h declares;
+- the host-built tables (metrics, multigrid coefficients of every level) are
+  bit-for-bit those of the oracle;
+- the product path fails loudly when the library is missing (no fallback);
+- the reference run-directory parser reads the shipped input.dat fixture;
+- the byte contract reproduces its worked values;
+- bench.py refuses more than one GPU.
+"""
+import glob
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from ekman_bl_amd import roofline
+from ekman_bl_amd import solver as S
+from ekman_bl_amd.grid import tanh_grid
+from ekman_bl_amd.params import YMAX, YMIN, ZMAX, ZMIN, EkmParams, base_params, config, read_input_dat
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(ekm_\w+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(S.LIB_PATH):
+        pytest.skip("library not built")
+    L = S.lib()
+    names = _declared_functions()
+    assert len(names) >= 30
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+    assert L.ekm_version().decode().startswith("ekman_bl_amd")
+
+
+def test_product_path_fails_loudly_without_library(tmp_path):
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "from ekman_bl_amd import solver as S\n"
+        "S.LIB_PATH = %r\n"
+        "S._lib = None\n"
+        "try:\n"
+        "    S.lib()\n"
+        "except RuntimeError as e:\n"
+        "    print('raised', e)\n"
+        "    sys.exit(0)\n"
+        "sys.exit(1)\n" % (ROOT, str(tmp_path / "missing.so")))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "not built" in r.stdout
+
+
+def _nan_equal(a, b):
+    return np.array_equal(np.nan_to_num(a, nan=1e300), np.nan_to_num(b, nan=1e300))
+
+
+@pytest.mark.parametrize("n,bcs", [(1, (2, 2, 2, 1)), (1, (2, 2, 2, 2)), (1, (0, 0, 2, 2)), (1, (1, 1, 1, 1)),
+                                   (1, (0, 0, 1, 2)), (1, (2, 2, 0, 0)), (2, (2, 2, 2, 1))])
+def test_host_tables_equal_oracle(n, bcs, oracle_lib):
+    O = oracle_lib
+    if not os.path.exists(S.LIB_PATH):
+        pytest.skip("library not built")
+    cfg = config(n)
+    p = cfg.params
+    for s in range(4):
+        p.mudpack_bc[s] = bcs[s]
+    xp, yp = tanh_grid(p.ny, p.nz, p.ly, p.lz, cfg.gamma)
+    xp = xp + 0.05 * np.sin(0.3 * yp)       # skewed: every metric is nonzero
+    o = O.Oracle(p, xp, yp)
+    hm = S.host_metrics(p.nz, p.ny, xp, yp)
+    for w in range(7):
+        assert _nan_equal(o.metric(w), hm[2 * w:2 * w + 2]), w
+    assert _nan_equal(o.metric(7), hm[14])
+    for lev in range(1, o.mg_ngrid() + 1):
+        for kx2 in (0.0, 3.7):
+            assert np.array_equal(o.mg_discretize(kx2, lev)[:9], S.host_mg_coefficients(p, xp, yp, lev, kx2)), \
+                (lev, kx2)
+
+
+def test_read_input_dat_fixture():
+    """rundir_unst/input.dat (fixture) through the reference's line order
+    (src/diablo.F90:242-337)."""
+    p = base_params(256, 127, 319, n_th=1)
+    vals = read_input_dat(os.path.join(GOLDEN, "input_unst.dat"), p)
+    assert abs(vals["version"] - 1.21) < 1e-12
+    assert p.nu == pytest.approx(1.3e-4) and p.lx == 60 and p.ly == 60 and p.lz == 60
+    assert vals["n_time_steps"] == 30000
+    assert p.delta_t == 3.0 and p.variable_dt == 1 and p.cfl == pytest.approx(0.8) and p.update_dt == 1
+    assert list(p.mudpack_bc) == [0, 0, 2, 2]
+    assert vals["ic_type"] == 7
+    assert p.f_type == 1 and p.f0 == pytest.approx(1.0e-4) and p.omega0 == pytest.approx(0.0146)
+    for c in range(3):
+        assert p.u_bc[c][YMIN] == 0 and p.u_bc[c][YMAX] == 1
+        assert p.u_bc[c][ZMIN] == 6 and p.u_bc[c][ZMAX] == 6
+    assert p.th_bc[YMIN] == 5 and p.th_bc_c1[YMIN] == pytest.approx(1.56e-3)
+    assert p.th_bc[YMAX] == 1 and p.th_bc[ZMIN] == 6 and p.th_bc[ZMAX] == 6
+    assert p.pr == 1.0
+
+
+def test_params_struct_matches_header():
+    """ekm_params_default fills the C struct the ctypes mirror describes."""
+    if not os.path.exists(S.LIB_PATH):
+        pytest.skip("library not built")
+    p = EkmParams()
+    S.lib().ekm_params_default(p)
+    assert p.mg_maxcy == 6 and p.mg_tolmax == 1e-6      # src/mg_solver2d.F90:158,188
+    assert list(p.mudpack_bc) == [2, 2, 2, 2]
+    assert p.cmplx_fp32 == 0 and p.coriolis_mean_only == 0 and p.pr == 1.0
+
+
+def test_byte_contract_worked_values():
+    """B_step = 8 N_pts [(34F+33) + 3 r (15 + 30.5 c)] per point-step for
+    the 256x193x257 stratified case: 2,578 / 3,561 / 4,545 B at c = 2/4/6."""
+    p = config(2).params
+    per_pt = [roofline.step_bytes(p, c) / p.npts for c in (2, 4, 6)]
+    assert [round(v) for v in per_pt] == [2578, 3561, 4545]
+    p1 = config(1).params
+    assert [round(roofline.step_bytes(p1, c) / p1.npts) for c in (2, 4, 6)] == [2334, 3340, 4347]
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr

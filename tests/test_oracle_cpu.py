@@ -1,0 +1,381 @@
+"""CPU checks that pin the oracle (no GPU needed).
+
+Each test checks one oracle function against something independent of the
+oracle's own formulas: the DFT definition (numpy), dense linear algebra
+(numpy / scipy), closed forms on rectilinear grids, the reference's own
+MUDPACK compiled from src/mud2cr.f and src/mudcom.f (oracle/_ref, skipped
+where it is not built), the reference's shipped grid (tests/golden), and the
+projection identity on a uniform grid.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from ekman_bl_amd.grid import tanh_grid
+from ekman_bl_amd.ic import ekman_initial
+from ekman_bl_amd.params import base_params, config
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel(a, b):
+    n = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (n if n > 0 else 1.0)
+
+
+# -- A3: x-FFT ---------------------------------------------------------------
+@pytest.mark.parametrize("nx", [8, 64, 256, 1024])
+def test_fft_is_scaled_truncated_dft(nx, oracle_lib):
+    O = oracle_lib
+    rng = np.random.default_rng(nx)
+    v = rng.standard_normal((11, nx + 2))
+    v[:, nx:] = 0.0
+    a = v.copy()
+    O.fft_fwd(a)
+    ref = np.fft.rfft(v[:, :nx], axis=1) / nx
+    ref[:, nx // 3 + 1:] = 0.0
+    got = a.view(np.complex128)
+    assert np.abs(got - ref).max() < 1e-14 * max(1.0, np.abs(ref).max()) * math.log2(nx)
+    # inverse: unscaled synthesis from modes 0..NX/3 with Hermitian symmetry
+    O.fft_inv(a)
+    back = np.fft.irfft(ref * nx, n=nx, axis=1)
+    assert np.abs(a[:, :nx] - back).max() < 1e-13
+    assert np.all(a[:, nx:] == 0.0)
+
+
+def test_fft_derivative_of_sine(oracle_lib):
+    """d/dx of sin(m x) through i*kx in Fourier space is m cos(m x)."""
+    O = oracle_lib
+    nx, lx, m = 64, 2 * math.pi, 5
+    x = np.arange(nx) * lx / nx
+    v = np.zeros((1, nx + 2))
+    v[0, :nx] = np.sin(m * x)
+    O.fft_fwd(v)
+    c = v.view(np.complex128)[0]
+    kx = 2 * math.pi * np.arange(nx // 2 + 1) / lx
+    c *= 1j * kx
+    O.fft_inv(v)
+    assert np.abs(v[0, :nx] - m * np.cos(m * x)).max() < 1e-12
+
+
+# -- A6: Thomas ----------------------------------------------------------------
+def _dense_solve(a, b, c, g, extra=None):
+    """Solve every batch column of the tridiagonal systems densely."""
+    ny1, nb = g.shape
+    out = np.zeros_like(g)
+    for i in range(nb):
+        M = np.diag(b[:, i]) + np.diag(a[1:, i], -1) + np.diag(c[:-1, i], 1)
+        if extra is not None:
+            r, col, d = extra
+            M[r, col] += d
+        out[:, i] = np.linalg.solve(M, g[:, i])
+    return out
+
+
+def test_thomas_matches_dense(oracle_lib):
+    O = oracle_lib
+    rng = np.random.default_rng(11)
+    ny, nx = 30, 17
+    a = rng.uniform(-1, 0, (ny + 1, nx + 1))
+    c = rng.uniform(-1, 0, (ny + 1, nx + 1))
+    b = 2.5 + rng.uniform(0, 1, (ny + 1, nx + 1))
+    g = rng.standard_normal((ny + 1, nx + 1))
+    ref = _dense_solve(a, b, c, g)
+    x = g.copy()
+    O.thomas_real(a.copy(), b.copy(), c.copy(), x)
+    assert rel(x, ref) < 1e-13
+
+
+def test_thomas_sp_matches_dense(oracle_lib):
+    """THOMAS_REAL_SP eliminates row K+1's extra entry D at column K-1 with
+    row K (src/boundary.F90:17-24) before the plain sweep."""
+    O = oracle_lib
+    rng = np.random.default_rng(12)
+    ny, nx, k, d = 24, 9, 20, 0.37
+    a = rng.uniform(-1, -0.2, (ny + 1, nx + 1))
+    c = rng.uniform(-1, 0, (ny + 1, nx + 1))
+    b = 3.0 + rng.uniform(0, 1, (ny + 1, nx + 1))
+    g = rng.standard_normal((ny + 1, nx + 1))
+    ref = _dense_solve(a, b, c, g, extra=(k + 1, k - 1, d))
+    x = g.copy()
+    O.thomas_real_sp(a.copy(), b.copy(), c.copy(), x, d, k)
+    assert rel(x, ref) < 1e-12
+
+
+# -- A13: metrics ----------------------------------------------------------------
+def _oracle_for_grid(O, ny, nz, xp, yp, bcs=(2, 2, 2, 2)):
+    p = base_params(16, ny, nz)
+    for s in range(4):
+        p.mudpack_bc[s] = bcs[s]
+    return O.Oracle(p, xp, yp), p
+
+
+def test_metrics_uniform_rectilinear_closed_form(oracle_lib):
+    O = oracle_lib
+    ny, nz, dy, dz = 15, 31, 0.5, 0.25
+    y = dy * np.arange(ny + 2)
+    z = dz * np.arange(nz + 2)
+    xp, yp = np.meshgrid(z, y)
+    o, _ = _oracle_for_grid(O, ny, nz, xp, yp)
+    sl = (slice(None), slice(0, ny + 2), slice(0, nz + 2))
+    g11, g12, g22 = o.metric(0)[sl], o.metric(1)[sl], o.metric(2)[sl]
+    c11, c12, c21, c22 = (o.metric(w)[sl] for w in (3, 4, 5, 6))
+    jac = o.metric(7)[:ny + 2, :nz + 2]
+    # interior faces (K,J = 1..NZ+1, 1..NY+1) and the extrapolated ghost rows
+    inner = (slice(None), slice(1, ny + 2), slice(1, nz + 2))
+    assert np.all(g11[inner] == dy / dz) and np.all(g22[inner] == dz / dy)
+    assert np.all(g12[inner] == 0.0) and np.all(c12[inner] == 0.0) and np.all(c21[inner] == 0.0)
+    assert np.all(c11[inner] == dy) and np.all(c22[inner] == dz)
+    assert np.all(jac == dy * dz)
+
+
+def test_metrics_truncate_to_1e8(oracle_lib):
+    """x_zeta etc. are cut to 8 decimals before the metrics are formed
+    (src/diablo.F90:1220-1230): CJOB_22 = x_zeta carries the cut value."""
+    O = oracle_lib
+    ny, nz, dy, dz = 7, 15, 0.5, 0.123456789123
+    xp, yp = np.meshgrid(dz * np.arange(nz + 2), dy * np.arange(ny + 2))
+    o, _ = _oracle_for_grid(O, ny, nz, xp, yp)
+    c22 = o.metric(6)[:, 2:ny, 2:nz]
+    assert np.abs(c22 - 0.12345678).max() < 1e-15
+    assert np.abs(o.metric(7)[2:ny, 2:nz] - 0.12345678 * 0.5).max() < 1e-15
+
+
+def test_metrics_stretched_rectilinear(oracle_lib):
+    """On a tanh-stretched y, uniform z grid the metrics reduce to face
+    spacings: y_eta on the zeta face is y(j+1)-y(j), on the eta face the
+    centred (y(j+1)-y(j-1))/2 (src/diablo.F90:1166-1190)."""
+    O = oracle_lib
+    ny, nz = 31, 15
+    xp, yp = tanh_grid(ny, nz, 20.0, 2 * math.pi, 2.0)
+    o, _ = _oracle_for_grid(O, ny, nz, xp, yp)
+    y, z = yp[:, 0], xp[0, :]
+    t8 = lambda v: np.trunc(v * 1e8) * 1e-8  # noqa: E731
+    dz = t8(z[1] - z[0])
+    c11 = o.metric(3)          # CJOB_11 = y_eta
+    c22 = o.metric(6)          # CJOB_22 = x_zeta
+    for j in range(2, ny):
+        for k in range(2, nz):
+            ye1 = t8(0.25 * (2 * y[j + 1] - 2 * y[j - 1]))
+            ye2 = t8(y[j + 1] - y[j])
+            assert abs(c11[0, j, k] - ye1) <= 1e-15 * abs(ye1)
+            assert abs(c11[1, j, k] - ye2) <= 1e-15 * abs(ye2)
+            assert abs(c22[0, j, k] - dz) <= 1e-15
+    g12 = o.metric(1)[:, 1:ny + 2, 1:nz + 2]
+    assert np.all(g12 == 0.0)
+
+
+def test_shipped_grid_is_tanh_family():
+    """rundir_unst/GRID_YZ.dat (fixture) is the tanh family with gamma 2.3:
+    y uniform in z, z uniform, to the file's printed precision."""
+    d = json.load(open(os.path.join(GOLDEN, "grid_yz_unst.json")))
+    g = d["grid_def"]
+    ny, nz = g["ny"], g["nz"]
+    y, z = np.array(d["y"]), np.array(d["z"])
+    xp, yp = tanh_grid(ny, nz, y[-1], z[-1], 2.3)
+    assert np.abs(yp[:, 0] - y).max() < 1e-12
+    assert np.abs(xp[0, :] - z).max() < 1e-12
+    for j, k, v in d["y_exceptions"]:
+        assert abs(v - y[j]) < 1e-12
+
+
+# -- A10/A11: multigrid --------------------------------------------------------
+@pytest.mark.parametrize("bcs,kx2", [((2, 2, 2, 1), 0.0), ((2, 2, 2, 2), 0.25), ((0, 0, 2, 2), 4.0),
+                                     ((1, 1, 1, 1), 0.0), ((2, 2, 0, 0), 1.0)])
+def test_mudpack_matches_reference_build(bcs, kx2, oracle_lib):
+    """The oracle's MUDPACK restatement against mud2cr itself (compiled from
+    the reference's src/mud2cr.f, src/mudcom.f): bit-for-bit, equal cycles."""
+    O = oracle_lib
+    if not O.ref_available():
+        pytest.skip("oracle/_ref not built (needs the reference tree and amdflang)")
+    cfg = config(1)
+    p = cfg.params
+    for s in range(4):
+        p.mudpack_bc[s] = bcs[s]
+    xp, yp = tanh_grid(p.ny, p.nz, p.ly, p.lz, cfg.gamma)
+    xp = xp + 0.05 * np.sin(0.3 * yp)       # skewed: cross metrics nonzero
+    o = O.Oracle(p, xp, yp)
+    rng = np.random.default_rng(5)
+    rhs = rng.standard_normal((p.ny + 2, p.nz + 2))
+    phi_o, cyc_o = o.mudpack(kx2, rhs)
+    phi_r, cyc_r, err = O.ref_mudpack(o, kx2, rhs)
+    # ierror -1 after intl=1 is MUDPACK's "maxcy reached before tolmax" warning
+    assert err[0] == 0 and err[1] in (0, -1)
+    assert (err[1] == -1) == (cyc_r == p.mg_maxcy)
+    assert cyc_o == cyc_r
+    assert np.array_equal(phi_o, phi_r)
+
+
+def _assemble(cf, nx, ny, bcs):
+    """Sparse matrix of MUDPACK's finest-level 9-point system, one row per
+    grid point (i fastest); Dirichlet rows are identity rows.  In a periodic
+    direction every one of the n points carries its own equation and the
+    virtual neighbours are phi(0) = phi(n-1), phi(n+1) = phi(2)
+    (src/mud2cr.f:1693-1795)."""
+    off = {0: (1, 0), 1: (1, 1), 2: (0, 1), 3: (-1, 1), 4: (-1, 0), 5: (-1, -1), 6: (0, -1), 7: (1, -1)}
+    px, py = bcs[0] == 0, bcs[2] == 0
+    mx, my = nx, ny
+
+    def wrap(v, n, per):
+        if per and v == -1:
+            return n - 2
+        if per and v == n:
+            return 1
+        return v
+
+    idx = lambda i, j: wrap(i, nx, px) + mx * wrap(j, ny, py)  # noqa: E731
+    rows, cols, vals = [], [], []
+    for j in range(my):
+        for i in range(mx):
+            r = idx(i, j)
+            spec = (bcs[0] == 1 and i == 0) or (bcs[1] == 1 and i == nx - 1) or \
+                   (bcs[2] == 1 and j == 0) or (bcs[3] == 1 and j == ny - 1)
+            if spec:
+                rows.append(r); cols.append(r); vals.append(1.0)
+                continue
+            rows.append(r); cols.append(r); vals.append(cf[8, j, i])
+            for q, (di, dj) in off.items():
+                ii, jj = i + di, j + dj
+                inside = (px or 0 <= ii < nx) and (py or 0 <= jj < ny)
+                if not inside:
+                    assert cf[q, j, i] == 0.0, (q, i, j)
+                    continue
+                spec_nb = (not px and ((bcs[0] == 1 and ii == 0) or (bcs[1] == 1 and ii == nx - 1))) or \
+                          (not py and ((bcs[2] == 1 and jj == 0) or (bcs[3] == 1 and jj == ny - 1)))
+                if spec_nb:
+                    continue            # phi = 0 there
+                rows.append(r); cols.append(idx(ii, jj)); vals.append(cf[q, j, i])
+    n = mx * my
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n)), mx, my
+
+
+@pytest.mark.parametrize("bcs,kx2", [((2, 2, 2, 1), 0.0), ((1, 1, 1, 1), 0.0), ((2, 2, 2, 2), 0.5),
+                                     ((0, 0, 2, 1), 0.0), ((2, 2, 0, 0), 0.3)])
+def test_mudpack_converges_to_dense_solution(bcs, kx2, oracle_lib):
+    """With a tight relmax tolerance the W(2,1) cycles converge to the
+    solution of the assembled finest-level 9-point system (sparse direct
+    solve), boundary folding included.  The grid is isotropic in index space
+    (h_y ~ h_z) and skewed so the cross terms are active: point relaxation
+    converges fast there (on strongly stretched grids it does not, which is
+    the reference's own behaviour)."""
+    O = oracle_lib
+    p = base_params(16, 63, 63)
+    for s in range(4):
+        p.mudpack_bc[s] = bcs[s]
+    p.mg_maxcy, p.mg_tolmax = 60, 1e-12
+    xp, yp = tanh_grid(p.ny, p.nz, 2 * math.pi, 2 * math.pi, 0.0)
+    xp = xp + 0.05 * np.sin(yp)
+    o = O.Oracle(p, xp, yp)
+    cf = o.mg_discretize(kx2, o.mg_ngrid())
+    ny, nx = cf.shape[1:]
+    A, mx, my = _assemble(cf, nx, ny, bcs)
+    rng = np.random.default_rng(9)
+    rhs = rng.standard_normal((ny, nx))
+    if bcs[0] == 0:
+        rhs[:, nx - 1] = rhs[:, 0]      # periodic: point nx is point 1
+    if bcs[2] == 0:
+        rhs[ny - 1, :] = rhs[0, :]
+    phi, cyc = o.mudpack(kx2, rhs)
+    b = np.zeros(mx * my)
+    for j in range(my):
+        for i in range(mx):
+            spec = (bcs[0] == 1 and i == 0) or (bcs[1] == 1 and i == nx - 1) or \
+                   (bcs[2] == 1 and j == 0) or (bcs[3] == 1 and j == ny - 1)
+            b[i + mx * j] = 0.0 if spec else -rhs[j, i]
+    x = spla.spsolve(A.tocsc(), b).reshape(my, mx)
+    assert cyc < 60
+    assert rel(phi[:my, :mx], x) < 1e-9
+
+
+def test_point_relaxation_stalls_on_stretched_grid(oracle_lib):
+    """Documents the reference configuration's convergence on the stretched
+    64x65x65 grid: the residual of the assembled system falls by only ~0.85
+    per W(2,1) cycle (index-space anisotropy up to ~40 near the top), so six
+    cycles end far from the discrete solution."""
+    O = oracle_lib
+    cfg = config(1)
+    p = cfg.params
+    xp, yp = tanh_grid(p.ny, p.nz, p.ly, p.lz, cfg.gamma)
+    res = []
+    for cyc in (8, 16):
+        p.mg_maxcy, p.mg_tolmax = cyc, 0.0
+        o = O.Oracle(p, xp, yp)
+        cf = o.mg_discretize(0.0, o.mg_ngrid())
+        ny, nx = cf.shape[1:]
+        A, mx, my = _assemble(cf, nx, ny, tuple(p.mudpack_bc))
+        rhs = np.random.default_rng(9).standard_normal((ny, nx))
+        phi, _ = o.mudpack(0.0, rhs)
+        b = -rhs.copy()
+        b[ny - 1, :] = 0.0
+        res.append(np.abs(A @ phi.ravel() - b.ravel()).max())
+    rate = (res[1] / res[0]) ** (1 / 8)
+    assert 0.6 < rate < 0.95, rate
+
+
+def test_discretisation_exact_on_quadratics(oracle_lib):
+    """On a uniform rectilinear grid the 9-point operator applied to a
+    quadratic in index space equals cxx*2a + cyy*2b + cxy*c + ce*q exactly,
+    with cxx = G11 = dy/dz, cyy = G22 = dz/dy, cxy = 0 (src/duct.F90:3917-3931)."""
+    O = oracle_lib
+    ny, nz, dy, dz = 15, 15, 0.5, 0.25
+    xp, yp = np.meshgrid(dz * np.arange(nz + 2), dy * np.arange(ny + 2))
+    o, p = _oracle_for_grid(O, ny, nz, xp, yp)
+    kx2 = 0.3
+    cf = o.mg_discretize(kx2, o.mg_ngrid())
+    nyy, nxx = cf.shape[1:]
+    i, j = np.meshgrid(np.arange(nxx, dtype=float), np.arange(nyy, dtype=float))
+    a, b, c, d, e = 0.3, -0.7, 0.45, 1.1, -0.2
+    q = a * i * i + b * j * j + c * i * j + d * i + e * j + 2.0
+    off = {0: (1, 0), 1: (1, 1), 2: (0, 1), 3: (-1, 1), 4: (-1, 0), 5: (-1, -1), 6: (0, -1), 7: (1, -1)}
+    lhs = cf[8] * q
+    for k, (di, dj) in off.items():
+        lhs = lhs + cf[k] * np.roll(np.roll(q, -dj, axis=0), -di, axis=1)
+    cxx, cyy, J = dy / dz, dz / dy, dy * dz
+    expect = cxx * 2 * a + cyy * 2 * b - kx2 * J * q
+    inner = (slice(1, nyy - 1), slice(1, nxx - 1))
+    assert np.abs(lhs[inner] - expect[inner]).max() < 1e-11
+
+
+# -- A9: projection invariant ---------------------------------------------------
+def test_projection_on_uniform_grid_leaves_residual_divergence(oracle_lib):
+    """On a uniform rectilinear grid MUDPACK's operator is the divergence of
+    the projection's face gradient, so a tight solve leaves J div u at the
+    multigrid residual (src/duct.F90:2713-2800)."""
+    O = oracle_lib
+    p = base_params(16, 31, 31)
+    p.mg_maxcy, p.mg_tolmax = 60, 1e-15
+    for s, v in enumerate((2, 2, 2, 1)):
+        p.mudpack_bc[s] = v
+    xp, yp = tanh_grid(p.ny, p.nz, 4.0, 4.0, 0.0)
+    o = O.Oracle(p, xp, yp)
+    rng = np.random.default_rng(3)
+    shp = p.shape_phys
+    u = [rng.standard_normal(shp) for _ in range(3)]
+    for a in u:
+        a[:, :, p.nx:] = 0.0
+    o.set_initial_physical(u[0], u[1], u[2], np.zeros(shp))
+    d0 = o.diag().div_ref
+    assert d0 < 1e-11
+
+
+def test_ekman_initial_condition():
+    """The seeded initial field is the laminar Ekman spiral plus bounded
+    uniform noise; padding entries are zero."""
+    cfg = config(1)
+    p = cfg.params
+    xp, yp = tanh_grid(p.ny, p.nz, p.ly, p.lz, cfg.gamma)
+    u1, u2, u3, th = ekman_initial(p, yp, cfg.seed, 0.0, 0.0)
+    eta = yp[:, :, None]
+    assert np.abs(u1[:, :, :p.nx] - (1 - np.exp(-eta) * np.cos(eta))).max() < 1e-15
+    assert np.all(u1[:, :, p.nx:] == 0.0)
+    n1, _, _, _ = ekman_initial(p, yp, cfg.seed, cfg.noise, 0.0)
+    dn = n1[:, :, :p.nx] - u1[:, :, :p.nx]
+    assert np.abs(dn).max() <= 0.5 * cfg.noise + 1e-15
+    assert np.abs(dn).max() > 0.4 * cfg.noise
+    n2, _, _, _ = ekman_initial(p, yp, cfg.seed, cfg.noise, 0.0)
+    assert np.array_equal(n1, n2)
