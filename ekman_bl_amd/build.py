@@ -1,4 +1,5 @@
-"""Build the gfx950 library libekman_bl_amd.so in-tree with hipcc.
+"""Build the gfx950 library libekman_bl_amd.so in-tree with hipcc, and the
+Fortran host (ISO_C_BINDING module + duct_driver) with amdflang when present.
 
 Usage: python -m ekman_bl_amd.build [--jobs N]
 """
@@ -14,6 +15,9 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libekman_bl_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
+FORTRAN = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FORTRAN, "duct_driver")
 
 SOURCES = ["ekm_fft.hip", "ekm_kernels.hip", "ekm_mg.hip", "ekm_solver.hip", "ekm_host.cpp"]
 
@@ -46,7 +50,29 @@ def build(jobs: int = 4, verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    if os.path.exists(FLANG):
+        build_fortran(verbose)
     return LIB
+
+
+def build_fortran(verbose: bool = True) -> str:
+    """ekman_bl_amd.mod + duct_driver, linked to the in-tree library with an
+    $ORIGIN-relative run path."""
+    mod_o = os.path.join(OBJ, "ekman_bl_amd_mod.o")
+    drv_o = os.path.join(OBJ, "duct_driver.o")
+    steps = [
+        [FLANG, "-O2", "-J", OBJ, "-c", os.path.join(FORTRAN, "ekman_bl_amd_mod.f90"), "-o", mod_o],
+        [FLANG, "-O2", "-I", OBJ, "-c", os.path.join(FORTRAN, "duct_driver.f90"), "-o", drv_o],
+        [FLANG, drv_o, mod_o, "-L", HERE, "-lekman_bl_amd", "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER + ".tmp"],
+    ]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    if verbose:
+        print(f"built {DRIVER}")
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -90,12 +90,6 @@ int fft_inverse(const FftPlan& p, double* v, long npencil, hipStream_t st);
 struct HostMetrics;
 struct MgLevel {
   int nx = 0, ny = 0;
-  size_t phi_off = 0, rhs_off = 0;  // offsets inside a solve's workspace (doubles)
-  size_t c9_off = 0, cf_off = 0;    // per-solve c9; coefficient planes (LDS levels only)
-  const double* coef = nullptr;     // c1..c8 interleaved [ny][nx][8]
-  const double* s9 = nullptr;       // c1+c3+c5+c7 before boundary folding
-  const double* corr9 = nullptr;    // additive corner correction of c9
-  const double* jl = nullptr;       // INT_JACOB at the level's points (0 at specified points)
   int in_lds = 0;                   // level lives in LDS inside the solve kernel
 };
 struct MgPlan {
@@ -107,12 +101,12 @@ struct MgPlan {
   MgLevel lev[17];
   size_t ws_global = 0;      // doubles per solve in global memory
   size_t ws_lds = 0;         // doubles per solve in LDS
-  size_t resf_g = 0, phcor_g = 0, resf_l = 0, phcor_l = 0, red_l = 0;
   int lc = 0;                // finest level held in LDS
+  int nops_fmg = 0, nops_w = 0;
   double* ws = nullptr;      // global workspace [nsolve_max][ws_global]
   double* coefmem = nullptr; // all level coefficient arrays
   int* cycles = nullptr;     // [nsolve_max]
-  void* desc = nullptr;      // device copy of the level descriptor
+  void* desc = nullptr;      // device copy of the level descriptor, phase tables and cycle programs
   double* kx2 = nullptr;     // [nsolve_max]
   double* rhs = nullptr;     // [nsolve_max][plane] RHS_TMP
   double* phi = nullptr;     // [nsolve_max][plane]

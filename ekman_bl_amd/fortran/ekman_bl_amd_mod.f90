@@ -1,0 +1,226 @@
+! ekman_bl_amd_mod.f90 -- ISO_C_BINDING interfaces of libekman_bl_amd.so and
+! Fortran wrappers with the reference's subroutine surface.
+!
+! A DIABLO build keeps its own driver and calls these in place of its CPU
+! routines:
+!   RK_DUCT                          (src/duct.F90:237)      -> ekm_rk_duct
+!   REM_DIV_CURV                     (src/duct.F90:2472)     -> ekm_rem_div_curv
+!   FFT_X_TO_FOURIER_OP / _PHYSICAL_OP (calls src/mpi_duct.F90:290,308)
+!                                                            -> ekm_fft_x_to_fourier / _physical
+!   THOMAS_REAL(A,B,C,G,NY,NX)       (src/boundary.F90:53)   -> ekm_thomas_real
+!   THOMAS_REAL_SP(A,B,C,G,D,K,NY,NX) (src/boundary.F90:4)   -> ekm_thomas_real_sp
+!   MULTIGRID_MUDPACK(phi)           (src/mg_solver2d.F90:75) -> ekm_multigrid_mudpack
+! The duct state lives on the GPU behind one opaque handle (ekm_handle); the
+! module-global arrays of the reference are uploaded once (ekm_set_state /
+! ekm_set_initial_physical) and read back only for I/O (ekm_get_state).
+module ekman_bl_amd
+  use iso_c_binding
+  implicit none
+  private :: c_ptr, c_int, c_double
+
+  integer(c_int), parameter :: EKM_OK = 0, EKM_ERR_ARG = -1, EKM_ERR_HIP = -2, &
+       EKM_ERR_UNSUPPORTED = -3, EKM_ERR_DIV_NAN = -4, EKM_ERR_DT_SMALL = -5, EKM_ERR_NODEV = -6
+  integer, parameter :: EKM_YMIN = 1, EKM_YMAX = 2, EKM_ZMIN = 3, EKM_ZMAX = 4
+
+  ! struct ekm_params (include/ekman_bl_amd.h); C arrays [3][4] appear as (4,3)
+  type, bind(c) :: ekm_params
+     integer(c_int) :: nx, ny, nz, n_th
+     integer(c_int) :: mg_ixp, mg_jyq, mg_iex, mg_jey
+     real(c_double) :: nu, lx, ly, lz
+     real(c_double) :: delta_t
+     integer(c_int) :: variable_dt
+     real(c_double) :: cfl
+     integer(c_int) :: update_dt
+     integer(c_int) :: f_type
+     real(c_double) :: wbulk0, px0, omega0, amp_omega0, f0
+     integer(c_int) :: u_bc(4, 3)
+     real(c_double) :: u_bc_c1(4, 3)
+     real(c_double) :: ri_tau, pr
+     integer(c_int) :: cont_strat
+     integer(c_int) :: th_bc(4)
+     real(c_double) :: th_bc_c1(4)
+     integer(c_int) :: mudpack_bc(4)
+     integer(c_int) :: mg_maxcy
+     real(c_double) :: mg_tolmax
+     integer(c_int) :: cmplx_fp32
+     integer(c_int) :: coriolis_mean_only
+     integer(c_int) :: reserved(8)
+  end type ekm_params
+
+  type, bind(c) :: ekm_diag
+     real(c_double) :: div_ref, div_max, delta_t, time
+     integer(c_int) :: time_step, rk_step, mg_solves, mg_cycles, mg_max_cycles, status
+  end type ekm_diag
+
+  type(c_ptr), save :: ekm_handle = c_null_ptr
+
+  interface
+     subroutine ekm_params_default(p) bind(c, name="ekm_params_default")
+       import :: ekm_params
+       type(ekm_params), intent(out) :: p
+     end subroutine ekm_params_default
+
+     integer(c_int) function ekm_create(p, xpoint, ypoint, th_bar, s) bind(c, name="ekm_create")
+       import :: ekm_params, c_double, c_ptr, c_int
+       type(ekm_params), intent(in) :: p
+       real(c_double), intent(in) :: xpoint(*), ypoint(*)
+       type(c_ptr), value :: th_bar
+       type(c_ptr), intent(out) :: s
+     end function ekm_create
+
+     integer(c_int) function ekm_destroy(s) bind(c, name="ekm_destroy")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: s
+     end function ekm_destroy
+
+     integer(c_int) function ekm_set_state(s, cu1, cu2, cu3, cp, cth, time, time_step) bind(c, name="ekm_set_state")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: s
+       real(c_double), intent(in) :: cu1(*), cu2(*), cu3(*), cp(*), cth(*)
+       real(c_double), value :: time
+       integer(c_int), value :: time_step
+     end function ekm_set_state
+
+     integer(c_int) function ekm_get_state(s, cu1, cu2, cu3, cp, cth) bind(c, name="ekm_get_state")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: s
+       real(c_double), intent(out) :: cu1(*), cu2(*), cu3(*), cp(*), cth(*)
+     end function ekm_get_state
+
+     integer(c_int) function ekm_set_initial_physical(s, u1, u2, u3, th) bind(c, name="ekm_set_initial_physical")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: s
+       real(c_double), intent(in) :: u1(*), u2(*), u3(*), th(*)
+     end function ekm_set_initial_physical
+
+     integer(c_int) function ekm_rk_duct(s) bind(c, name="ekm_rk_duct")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: s
+     end function ekm_rk_duct
+
+     integer(c_int) function ekm_time_step(s, nsteps) bind(c, name="ekm_time_step")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: s
+       integer(c_int), value :: nsteps
+     end function ekm_time_step
+
+     integer(c_int) function ekm_rem_div_curv(s) bind(c, name="ekm_rem_div_curv")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: s
+     end function ekm_rem_div_curv
+
+     integer(c_int) function ekm_get_diag(s, d) bind(c, name="ekm_get_diag")
+       import :: c_ptr, c_int, ekm_diag
+       type(c_ptr), value :: s
+       type(ekm_diag), intent(out) :: d
+     end function ekm_get_diag
+
+     integer(c_int) function ekm_sync(s) bind(c, name="ekm_sync")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: s
+     end function ekm_sync
+
+     integer(c_int) function ekm_fft_x_to_fourier(varp, nx, npencil) bind(c, name="ekm_fft_x_to_fourier")
+       import :: c_int, c_double
+       real(c_double), intent(inout) :: varp(*)
+       integer(c_int), value :: nx, npencil
+     end function ekm_fft_x_to_fourier
+
+     integer(c_int) function ekm_fft_x_to_physical(varp, nx, npencil) bind(c, name="ekm_fft_x_to_physical")
+       import :: c_int, c_double
+       real(c_double), intent(inout) :: varp(*)
+       integer(c_int), value :: nx, npencil
+     end function ekm_fft_x_to_physical
+
+     integer(c_int) function ekm_thomas_real(a, b, c, g, ny, nx) bind(c, name="ekm_thomas_real")
+       import :: c_int, c_double
+       real(c_double), intent(inout) :: a(*), b(*), c(*), g(*)
+       integer(c_int), value :: ny, nx
+     end function ekm_thomas_real
+
+     integer(c_int) function ekm_thomas_real_sp(a, b, c, g, d, k, ny, nx) bind(c, name="ekm_thomas_real_sp")
+       import :: c_int, c_double
+       real(c_double), intent(inout) :: a(*), b(*), c(*), g(*)
+       real(c_double), value :: d
+       integer(c_int), value :: k, ny, nx
+     end function ekm_thomas_real_sp
+
+     integer(c_int) function ekm_multigrid_mudpack(s, nsolve, kx2, rhs_tmp, phi, cycles) &
+          bind(c, name="ekm_multigrid_mudpack")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: s
+       integer(c_int), value :: nsolve
+       real(c_double), intent(in) :: kx2(*), rhs_tmp(*)
+       real(c_double), intent(out) :: phi(*)
+       integer(c_int), intent(out) :: cycles(*)
+     end function ekm_multigrid_mudpack
+  end interface
+
+contains
+
+  ! Status check in place of the reference's STOP statements.
+  subroutine ekm_check(st, where)
+    integer(c_int), intent(in) :: st
+    character(len=*), intent(in) :: where
+    if (st /= EKM_OK) then
+       write(6, '(a,a,a,i0)') 'ekman_bl_amd: ', where, ' failed with status ', st
+       if (st == EKM_ERR_DIV_NAN) write(6, *) 'DIV is a NaN'
+       if (st == EKM_ERR_DT_SMALL) write(6, *) 'DELTA_T is zero'
+       stop 1
+    end if
+  end subroutine ekm_check
+
+  ! RK_DUCT (src/duct.F90:237): one RKW3+CN substep of the device state.
+  subroutine RK_DUCT_GPU()
+    call ekm_check(ekm_rk_duct(ekm_handle), 'RK_DUCT')
+  end subroutine RK_DUCT_GPU
+
+  ! REM_DIV_CURV (src/duct.F90:2472): projection of the device state.
+  subroutine REM_DIV_CURV_GPU()
+    call ekm_check(ekm_rem_div_curv(ekm_handle), 'REM_DIV_CURV')
+  end subroutine REM_DIV_CURV_GPU
+
+  ! FFT_X_TO_FOURIER_OP(varp,cvarp,0,NY+1,0,NZP) on an EQUIVALENCEd array:
+  ! varp(0:NX+1,0:NZ+1,0:NY+1), transformed in place.
+  subroutine FFT_X_TO_FOURIER_OP_GPU(varp, nx, nz, ny)
+    integer, intent(in) :: nx, nz, ny
+    real(c_double), intent(inout) :: varp(0:nx+1, 0:nz+1, 0:ny+1)
+    call ekm_check(ekm_fft_x_to_fourier(varp, int(nx, c_int), int((nz+2)*(ny+2), c_int)), 'FFT_X_TO_FOURIER_OP')
+  end subroutine FFT_X_TO_FOURIER_OP_GPU
+
+  subroutine FFT_X_TO_PHYSICAL_OP_GPU(varp, nx, nz, ny)
+    integer, intent(in) :: nx, nz, ny
+    real(c_double), intent(inout) :: varp(0:nx+1, 0:nz+1, 0:ny+1)
+    call ekm_check(ekm_fft_x_to_physical(varp, int(nx, c_int), int((nz+2)*(ny+2), c_int)), 'FFT_X_TO_PHYSICAL_OP')
+  end subroutine FFT_X_TO_PHYSICAL_OP_GPU
+
+  ! THOMAS_REAL(A,B,C,G,NY,NX) (src/boundary.F90:53), A..G(0:NX,0:NY)
+  subroutine THOMAS_REAL_GPU(A, B, C, G, NY, NX)
+    integer, intent(in) :: NY, NX
+    real(c_double), intent(inout) :: A(0:NX, 0:NY), B(0:NX, 0:NY), C(0:NX, 0:NY), G(0:NX, 0:NY)
+    call ekm_check(ekm_thomas_real(A, B, C, G, int(NY, c_int), int(NX, c_int)), 'THOMAS_REAL')
+  end subroutine THOMAS_REAL_GPU
+
+  ! THOMAS_REAL_SP(A,B,C,G,D,K,NY,NX) (src/boundary.F90:4)
+  subroutine THOMAS_REAL_SP_GPU(A, B, C, G, D, K, NY, NX)
+    integer, intent(in) :: K, NY, NX
+    real(c_double), intent(in) :: D
+    real(c_double), intent(inout) :: A(0:NX, 0:NY), B(0:NX, 0:NY), C(0:NX, 0:NY), G(0:NX, 0:NY)
+    call ekm_check(ekm_thomas_real_sp(A, B, C, G, D, int(K, c_int), int(NY, c_int), int(NX, c_int)), 'THOMAS_REAL_SP')
+  end subroutine THOMAS_REAL_SP_GPU
+
+  ! MULTIGRID_MUDPACK(phi) (src/mg_solver2d.F90:75) with its implicit inputs
+  ! made explicit: RHS_TMP(0:NZ+1,0:NY+1) and kx2_c; returns iparm(17).
+  subroutine MULTIGRID_MUDPACK_GPU(phi, rhs_tmp, kx2, nnx, nny, cycles)
+    integer, intent(in) :: nnx, nny
+    real(c_double), intent(in) :: rhs_tmp(nnx, nny), kx2
+    real(c_double), intent(out) :: phi(nnx, nny)
+    integer, intent(out) :: cycles
+    real(c_double) :: k2(1)
+    integer(c_int) :: cyc(1)
+    k2(1) = kx2
+    call ekm_check(ekm_multigrid_mudpack(ekm_handle, 1_c_int, k2, rhs_tmp, phi, cyc), 'MULTIGRID_MUDPACK')
+    cycles = int(cyc(1))
+  end subroutine MULTIGRID_MUDPACK_GPU
+
+end module ekman_bl_amd
