@@ -119,5 +119,21 @@ void mg_plan_free(MgPlan& mg);
 // the initial guesses); cycle counts in mg.cycles
 int mg_solve_ex(MgPlan& mg, const Dims& d, int nsolve, int iguess, hipStream_t st);
 
+// ---- consistent projection (ekm_mgt.hip) --------------------------------------
+struct TightPlan {
+  int fp32 = 0, cm = 0, nsolve_max = 0;
+  long ws_elems = 0, ws64_elems = 0;  // per solve
+  char* mem = nullptr;                // descriptor + coefficient tables
+  void* fac = nullptr;                // line factors [live mode][...]
+  void* ws = nullptr;                 // [nsolve][ws_elems] (float or double)
+  double* ws64 = nullptr;             // [nsolve][ws64_elems]
+};
+// kx2_mode: kx^2 of modes 0..nmode-1 (solve s uses mode s/2)
+int tight_plan_build(TightPlan& tp, const ekm_params& p, const Dims& d, const HostMetrics& hm, const Ranges& rv,
+                     const double* kx2_mode, int nmode, int nsolve_max);
+void tight_plan_free(TightPlan& tp);
+// solves nsolve planes of mg.rhs (RHS_TMP) into mg.phi; V-cycle counts in mg.cycles
+int tight_solve_ex(TightPlan& tp, MgPlan& mg, const Dims& d, int nsolve, hipStream_t st);
+
 // ---- solver -------------------------------------------------------------------
 struct ekm_solver_impl;

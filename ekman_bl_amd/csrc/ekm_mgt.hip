@@ -1,0 +1,593 @@
+// ekm_mgt.hip -- the consistent pressure projection (proj_mode 1 and 2; no
+// counterpart in the reference).
+//
+// The Poisson rows of the cells the divergence diagnostic covers (J=2..NY-1,
+// K=2..NZ-1, src/duct.F90:2790-2800) are the divergence (:2754-2760) of the
+// projection's own face gradient (:2713-2724) and of the CU1 centre
+// correction (:2731), so J div u after the projection equals the residual
+// of the solve on those cells.  Every other row keeps MUDPACK's
+// discretisation (host_mg_level), so the boundary closure and the coarse
+// operators are the reference's.
+//
+// Solver: residual iterations in FP64 on the finest level; each correction
+// is one V(1,1) cycle in T (double: proj_mode 1; float: proj_mode 2, the
+// mixed-precision form) with alternating zebra-line Gauss-Seidel smoothing
+// (z lines of both parities, then y lines of both parities), full weighting
+// and bilinear prolongation; the coarsest level gets 20 smoothing steps.
+// Line smoothing handles the index-space anisotropy of the stretched grids
+// that makes MUDPACK's point relaxation stall.
+//
+// Layout and work split: one workgroup per real solve (2 per live kx mode),
+// as for the MUDPACK replica.  The tridiagonal line matrices depend only on
+// the level and kx, so their Thomas factors (1/pivot, -lower/pivot, upper/
+// pivot) are computed once at set-up per live mode.  A z line (contiguous in
+// memory) is solved by one wavefront: each lane owns a chunk of CM points and
+// the two first-order recurrences of the substitution are composed as affine
+// maps with a wave-level scan.  y lines (stride NZ+2) are solved by lanes over
+// columns, each lane running the substitution down its column, so every load
+// of a wavefront is contiguous across lanes.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ekm_host.h"
+#include "ekm_internal.h"
+
+namespace {
+
+constexpr int kMaxLevT = 16;
+constexpr int kCoarseSweeps = 20;
+
+template <typename T>
+struct TLev {
+  int nx, ny, n;
+  int e_off, r_off, res_off;  // per-solve workspace offsets (elements of T)
+  int f_off;                  // per-mode factor offset: z factors [3][n], then y factors [3][n]
+  const T* cf;                // c1..c8 planes [8][n]
+  const double* s9;           // c9 = ((-kx2*jl) - s9) + corr9
+  const double* corr9;
+  const double* jl;
+  const unsigned char* spec;  // specified (Dirichlet) points
+};
+
+template <typename T>
+struct TDesc {
+  int ngrid, nfx, nfy, maxit;
+  double tol;
+  long ws;          // T elements per solve
+  long ws64;        // doubles per solve: phi (ring) then b
+  long fstride;     // factor elements per mode
+  const T* fac;     // [mode][fstride]
+  const double* cf64;  // finest level c1..c8 in double [8][n]
+  TLev<T> lev[kMaxLevT + 1];
+};
+
+constexpr int kDi[8] = {1, 1, 0, -1, -1, -1, 0, 1};
+constexpr int kDj[8] = {0, 1, 1, 1, 0, -1, -1, -1};
+
+template <typename T>
+__device__ inline T c9_of(const TLev<T>& L, int p, double kx2) {
+  return (T)(((-(kx2 * L.jl[p])) - L.s9[p]) + L.corr9[p]);
+}
+
+// affine map x -> a*x + b; compose(outer, inner) = outer(inner(x))
+template <typename T>
+struct Aff {
+  T a, b;
+};
+template <typename T>
+__device__ inline Aff<T> compose(const Aff<T>& o, const Aff<T>& i) {
+  return Aff<T>{o.a * i.a, o.a * i.b + o.b};
+}
+
+template <typename T>
+__device__ inline T shfl_up(T v, int d) {
+  return __shfl_up(v, d, 64);
+}
+template <typename T>
+__device__ inline T shfl_down(T v, int d) {
+  return __shfl_down(v, d, 64);
+}
+
+// z lines (rows j of parity par) of level L: one wavefront per row.
+template <typename T, int CM>
+__device__ void zlines(const TDesc<T>& g, const TLev<T>& L, T* W, const T* fac, int par) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int nx = L.nx, ny = L.ny, ld = nx + 2, n = L.n;
+  T* e = W + L.e_off;
+  const T* r = W + L.r_off;
+  const T* f1 = fac + L.f_off;
+  const T* fa = f1 + n;
+  const T* fc = fa + n;
+  const int i0 = lane * CM;
+  for (int j = par + 2 * wave; j < ny; j += 2 * nw) {
+    T bk[CM], ak[CM];
+    Aff<T> loc{(T)1, (T)0};
+#pragma unroll
+    for (int k = 0; k < CM; ++k) {
+      const int i = i0 + k;
+      ak[k] = (T)1;
+      bk[k] = (T)0;
+      if (i < nx) {
+        const int p = i + nx * j;
+        const int q = (i + 1) + ld * (j + 1);
+        T s = (T)0;
+        if (!L.spec[p]) {
+          s = r[p] - (L.cf[1 * n + p] * e[q + 1 + ld] + L.cf[2 * n + p] * e[q + ld] + L.cf[3 * n + p] * e[q - 1 + ld] +
+                      L.cf[5 * n + p] * e[q - 1 - ld] + L.cf[6 * n + p] * e[q - ld] + L.cf[7 * n + p] * e[q + 1 - ld]);
+        }
+        ak[k] = fa[p];
+        bk[k] = s * f1[p];
+        loc = compose(Aff<T>{ak[k], bk[k]}, loc);
+      }
+    }
+    // inclusive scan of the forward maps over lanes
+    Aff<T> inc = loc;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      Aff<T> prev{shfl_up(inc.a, d), shfl_up(inc.b, d)};
+      if (lane >= d) inc = compose(inc, prev);
+    }
+    T dp = shfl_up(inc.b, 1);
+    if (lane == 0) dp = (T)0;
+    T dk[CM];
+#pragma unroll
+    for (int k = 0; k < CM; ++k) {
+      dp = ak[k] * dp + bk[k];
+      dk[k] = dp;
+    }
+    // backward: x_i = dp_i - cp_i x_{i+1}, composed from the right
+    Aff<T> locb{(T)1, (T)0};
+    T ck[CM];
+#pragma unroll
+    for (int k = CM - 1; k >= 0; --k) {
+      const int i = i0 + k;
+      ck[k] = (T)0;
+      if (i < nx) {
+        ck[k] = fc[i + nx * j];
+        locb = compose(Aff<T>{-ck[k], dk[k]}, locb);
+      }
+    }
+    Aff<T> incb = locb;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      Aff<T> nxt{shfl_down(incb.a, d), shfl_down(incb.b, d)};
+      if (lane + d < 64) incb = compose(incb, nxt);
+    }
+    T x = shfl_down(incb.b, 1);
+    if (lane == 63) x = (T)0;
+#pragma unroll
+    for (int k = CM - 1; k >= 0; --k) {
+      const int i = i0 + k;
+      if (i < nx) {
+        x = dk[k] - ck[k] * x;
+        e[(i + 1) + ld * (j + 1)] = x;
+      }
+    }
+  }
+}
+
+// y lines (columns i of parity par) of level L: lanes over columns.
+template <typename T>
+__device__ void ylines(const TDesc<T>& g, const TLev<T>& L, T* W, const T* fac, int par) {
+  const int nx = L.nx, ny = L.ny, ld = nx + 2, n = L.n;
+  T* e = W + L.e_off;
+  const T* r = W + L.r_off;
+  const T* f1 = fac + L.f_off + 3 * n;
+  const T* fa = f1 + n;
+  const T* fc = fa + n;
+  for (int i = par + 2 * (int)threadIdx.x; i < nx; i += 2 * (int)blockDim.x) {
+    T dp = (T)0;
+    for (int j = 0; j < ny; ++j) {
+      const int p = i + nx * j;
+      const int q = (i + 1) + ld * (j + 1);
+      T s = (T)0;
+      if (!L.spec[p]) {
+        s = r[p] - (L.cf[0 * n + p] * e[q + 1] + L.cf[1 * n + p] * e[q + 1 + ld] + L.cf[3 * n + p] * e[q - 1 + ld] +
+                    L.cf[4 * n + p] * e[q - 1] + L.cf[5 * n + p] * e[q - 1 - ld] + L.cf[7 * n + p] * e[q + 1 - ld]);
+      }
+      dp = fa[p] * dp + s * f1[p];
+      e[q] = dp;
+    }
+    T x = (T)0;
+    for (int j = ny - 1; j >= 0; --j) {
+      const int p = i + nx * j;
+      const int q = (i + 1) + ld * (j + 1);
+      x = e[q] - fc[p] * x;
+      e[q] = x;
+    }
+  }
+}
+
+template <typename T, int CM>
+__device__ void smooth(const TDesc<T>& g, const TLev<T>& L, T* W, const T* fac) {
+  zlines<T, CM>(g, L, W, fac, 0);
+  __syncthreads();
+  zlines<T, CM>(g, L, W, fac, 1);
+  __syncthreads();
+  ylines<T>(g, L, W, fac, 0);
+  __syncthreads();
+  ylines<T>(g, L, W, fac, 1);
+  __syncthreads();
+}
+
+// res = r - A e on level L (0 at specified points)
+template <typename T>
+__device__ void residual(const TLev<T>& L, T* W, double kx2) {
+  const int nx = L.nx, ld = nx + 2, n = L.n;
+  const T* e = W + L.e_off;
+  const T* r = W + L.r_off;
+  T* res = W + L.res_off;
+  for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    const int j = p / nx, i = p - j * nx;
+    const int q = (i + 1) + ld * (j + 1);
+    T v = (T)0;
+    if (!L.spec[p]) {
+      T acc = (T)0;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) acc += L.cf[c * n + p] * e[q + kDi[c] + ld * kDj[c]];
+      acc += c9_of(L, p, kx2) * e[q];
+      v = r[p] - acc;
+    }
+    res[p] = v;
+  }
+  __syncthreads();
+}
+
+// full weighting of the residual of F onto the right-hand side of C (zero
+// outside the grid; a direction that is not coarsened keeps its index), and
+// C's correction cleared
+template <typename T>
+__device__ void restrict_fw(const TLev<T>& F, const TLev<T>& C, T* W) {
+  const int sx = C.nx < F.nx ? 2 : 1, sy = C.ny < F.ny ? 2 : 1;
+  const T* res = W + F.res_off;
+  T* rc = W + C.r_off;
+  T* ec = W + C.e_off;
+  const T w[3] = {(T)0.25, (T)0.5, (T)0.25};
+  for (int pc = threadIdx.x; pc < C.n; pc += blockDim.x) {
+    const int jc = pc / C.nx, ic = pc - jc * C.nx;
+    T acc = (T)0;
+    if (!C.spec[pc]) {
+      const int i = sx * ic, j = sy * jc;
+      for (int b = -1; b <= 1; ++b)
+        for (int a = -1; a <= 1; ++a) {
+          const int ii = i + a, jj = j + b;
+          if (ii < 0 || ii >= F.nx || jj < 0 || jj >= F.ny) continue;
+          acc += (w[a + 1] * w[b + 1]) * res[ii + F.nx * jj];
+        }
+    }
+    rc[pc] = acc;
+  }
+  for (int t = threadIdx.x; t < (C.nx + 2) * (C.ny + 2); t += blockDim.x) ec[t] = (T)0;
+  __syncthreads();
+}
+
+// e_F += bilinear interpolation of e_C (not at specified points)
+template <typename T>
+__device__ void prolong_add(const TLev<T>& C, const TLev<T>& F, T* W) {
+  const int sx = C.nx < F.nx ? 2 : 1, sy = C.ny < F.ny ? 2 : 1;
+  const T* ec = W + C.e_off;
+  T* ef = W + F.e_off;
+  const int lc = C.nx + 2, lf = F.nx + 2;
+  for (int p = threadIdx.x; p < F.n; p += blockDim.x) {
+    if (F.spec[p]) continue;
+    const int j = p / F.nx, i = p - j * F.nx;
+    const int i0 = i / sx, i1 = (i + sx - 1) / sx, j0 = j / sy, j1 = (j + sy - 1) / sy;
+    const T v = (T)0.25 * ((ec[(i0 + 1) + lc * (j0 + 1)] + ec[(i1 + 1) + lc * (j0 + 1)]) +
+                           (ec[(i0 + 1) + lc * (j1 + 1)] + ec[(i1 + 1) + lc * (j1 + 1)]));
+    ef[(i + 1) + lf * (j + 1)] += v;
+  }
+  __syncthreads();
+}
+
+template <typename T, int CM>
+__global__ void __launch_bounds__(512) k_tight_solve(const TDesc<T>* __restrict__ gp, T* __restrict__ ws,
+                                                     double* __restrict__ ws64, const double* __restrict__ rhs_tmp,
+                                                     double* __restrict__ phi_out, const double* __restrict__ kx2v,
+                                                     int* __restrict__ cycles, size_t plane) {
+  __shared__ double red[512];
+  const TDesc<T>& g = *gp;
+  const int s = blockIdx.x;
+  const double kx2 = kx2v[s];
+  T* W = ws + (size_t)s * g.ws;
+  double* phi = ws64 + (size_t)s * g.ws64;
+  const int ng = g.ngrid;
+  const TLev<T>& F = g.lev[ng];
+  const int nx = F.nx, ny = F.ny, ld = nx + 2, n = F.n;
+  double* b = phi + (size_t)(nx + 2) * (ny + 2);
+  const T* fac = g.fac + (size_t)(s >> 1) * g.fstride;
+  const double* rt = rhs_tmp + (size_t)s * plane;
+  for (int t = threadIdx.x; t < (nx + 2) * (ny + 2); t += blockDim.x) phi[t] = 0.0;
+  for (int p = threadIdx.x; p < n; p += blockDim.x) b[p] = F.spec[p] ? 0.0 : -rt[p];
+  __syncthreads();
+  int it = 0;
+  for (;;) {
+    // FP64 residual of the finest level; its max decides; it seeds the cycle
+    double rmax = 0.0;
+    T* rf = W + F.r_off;
+    T* ef = W + F.e_off;
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+      const int j = p / nx, i = p - j * nx;
+      const int q = (i + 1) + ld * (j + 1);
+      double v = 0.0;
+      if (!F.spec[p]) {
+        double acc = 0.0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc += g.cf64[c * n + p] * phi[q + kDi[c] + ld * kDj[c]];
+        acc += ((((-(kx2 * F.jl[p])) - F.s9[p]) + F.corr9[p])) * phi[q];
+        v = b[p] - acc;
+      }
+      rmax = fmax(rmax, fabs(v));
+      rf[p] = (T)v;
+    }
+    for (int t = threadIdx.x; t < (nx + 2) * (ny + 2); t += blockDim.x) ef[t] = (T)0;
+    red[threadIdx.x] = rmax;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+      __syncthreads();
+    }
+    rmax = red[0];
+    __syncthreads();
+    if (rmax <= g.tol || it >= g.maxit) break;
+    // one V(1,1) cycle on the correction equation: smoothing steps walked in
+    // order (down: ng..2, the coarsest level, up: 2..ng), one call site
+    const int nsm = 2 * (ng - 1) + kCoarseSweeps;
+    for (int k = 0; k < nsm; ++k) {
+      const bool down = k < ng - 1, up = k >= ng - 1 + kCoarseSweeps;
+      const int l = down ? ng - k : (up ? 2 + (k - (ng - 1 + kCoarseSweeps)) : 1);
+      if (up) prolong_add<T>(g.lev[l - 1], g.lev[l], W);
+      smooth<T, CM>(g, g.lev[l], W, fac);
+      if (down) {
+        residual<T>(g.lev[l], W, kx2);
+        restrict_fw<T>(g.lev[l], g.lev[l - 1], W);
+      }
+    }
+    for (int t = threadIdx.x; t < (nx + 2) * (ny + 2); t += blockDim.x) phi[t] += (double)ef[t];
+    __syncthreads();
+    ++it;
+  }
+  double* po = phi_out + (size_t)s * plane;
+  for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    const int j = p / nx, i = p - j * nx;
+    po[p] = phi[(i + 1) + ld * (j + 1)];
+  }
+  if (threadIdx.x == 0) cycles[s] = it;
+}
+
+// Thomas factors of one line: lower/diag/upper -> (1/piv, -lower/piv, upper/piv)
+void line_factors(int len, const double* lo, const double* di, const double* up, double* f1, double* fa,
+                  double* fc) {
+  double cprev = 0.0;
+  for (int k = 0; k < len; ++k) {
+    const double piv = di[k] - lo[k] * cprev;
+    f1[k] = 1.0 / piv;
+    fa[k] = -lo[k] * f1[k];
+    fc[k] = up[k] * f1[k];
+    cprev = fc[k];
+  }
+}
+
+template <typename T>
+int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const HostMetrics& hm, const Ranges& rv,
+                      const double* kx2_mode, int nmode) {
+  const int ng = p.mg_iex > p.mg_jey ? p.mg_iex : p.mg_jey;
+  if (ng > kMaxLevT) return EKM_ERR_UNSUPPORTED;
+  TDesc<T> g;
+  std::memset(&g, 0, sizeof(g));
+  g.ngrid = ng;
+  g.nfx = d.NZ2;
+  g.nfy = d.NY2;
+  g.maxit = p.proj_maxit > 0 ? p.proj_maxit : 50;
+  g.tol = p.proj_tol;
+  std::vector<HostMgLevel> hl(ng + 1);
+  long ws = 0, fstride = 0;
+  for (int k = 1; k <= ng; ++k) {
+    const int ex = k + p.mg_iex - ng, ey = k + p.mg_jey - ng;
+    const int nx = p.mg_ixp * (1 << ((ex > 1 ? ex : 1) - 1)) + 1;
+    const int ny = p.mg_jyq * (1 << ((ey > 1 ? ey : 1) - 1)) + 1;
+    int st = host_mg_level(hm, d.NZ, d.NY, p.mudpack_bc, nx, ny, hl[k]);
+    if (st) return st;
+    TLev<T>& L = g.lev[k];
+    L.nx = nx;
+    L.ny = ny;
+    L.n = nx * ny;
+    L.e_off = (int)ws;
+    ws += (long)(nx + 2) * (ny + 2);
+    L.r_off = (int)ws;
+    ws += L.n;
+    L.res_off = (int)ws;
+    ws += L.n;
+    ws = (ws + 31) & ~31L;
+    L.f_off = (int)fstride;
+    fstride += 6L * L.n;
+  }
+  fstride = (fstride + 63) & ~63L;
+  g.ws = ws;
+  g.ws64 = (long)(d.NZ2 + 2) * (d.NY2 + 2) + (long)d.NZ2 * d.NY2;
+  g.ws64 = (g.ws64 + 31) & ~31L;
+  g.fstride = fstride;
+  // consistent rows of the finest level (cells J=2..NY-1, K=2..NZ-1)
+  {
+    HostMgLevel& L = hl[ng];
+    const int nx = L.nx;
+    const size_t n = (size_t)L.nx * L.ny;
+    const int NZ3 = d.NZ + 3, NY3 = d.NY + 3;
+    auto G = [&](int w, int k, int j, int f) { return hm.m[w][(size_t)k + (size_t)NZ3 * ((size_t)j + (size_t)NY3 * (f - 1))]; };
+    for (int J = 2; J <= d.NY - 1; ++J)
+      for (int K = 2; K <= d.NZ - 1; ++K) {
+        const double tm = (J + 1 <= rv.je) ? 1.0 : 0.0, bm = (J >= rv.js) ? 1.0 : 0.0;
+        const double rm = (K + 1 <= rv.ze) ? 1.0 : 0.0, lm = (K >= rv.zs) ? 1.0 : 0.0;
+        const double a = G(2, K, J + 1, 1) * tm, bb = G(2, K, J, 1) * bm;
+        const double cc = G(0, K + 1, J, 2) * rm, dd = G(0, K, J, 2) * lm;
+        const double e = 0.25 * G(1, K, J + 1, 1) * tm, f = 0.25 * G(1, K, J, 1) * bm;
+        const double gg = 0.25 * G(1, K + 1, J, 2) * rm, h = 0.25 * G(1, K, J, 2) * lm;
+        const size_t q = (size_t)K + (size_t)nx * J;
+        L.c[0 * n + q] = (e - f) + cc;
+        L.c[1 * n + q] = e + gg;
+        L.c[2 * n + q] = (a + gg) - h;
+        L.c[3 * n + q] = -e - h;
+        L.c[4 * n + q] = (-e + f) + dd;
+        L.c[5 * n + q] = f + h;
+        L.c[6 * n + q] = (bb - gg) + h;
+        L.c[7 * n + q] = -f - gg;
+        L.s9[q] = ((a + bb) + cc) + dd;
+        L.corr9[q] = 0.0;
+        L.jl[q] = hm.jac[(size_t)K + (size_t)NZ3 * J];
+      }
+  }
+  // specified points, coefficient tables and line factors
+  size_t tot_n = 0;
+  for (int k = 1; k <= ng; ++k) tot_n += (size_t)g.lev[k].n;
+  std::vector<unsigned char> spec(tot_n);
+  std::vector<T> cfT(8 * tot_n);
+  std::vector<double> s9(tot_n), corr9(tot_n), jl(tot_n);
+  std::vector<T> fac((size_t)nmode * fstride, (T)0);
+  size_t off = 0;
+  std::vector<size_t> loff(ng + 1);
+  for (int k = 1; k <= ng; ++k) {
+    const HostMgLevel& H = hl[k];
+    const int nx = H.nx, ny = H.ny;
+    const size_t n = (size_t)nx * ny;
+    loff[k] = off;
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i)
+        spec[off + i + (size_t)nx * j] = (p.mudpack_bc[0] == 1 && i == 0) || (p.mudpack_bc[1] == 1 && i == nx - 1) ||
+                                         (p.mudpack_bc[2] == 1 && j == 0) || (p.mudpack_bc[3] == 1 && j == ny - 1);
+    for (size_t t = 0; t < 8 * n; ++t) cfT[8 * off + t] = (T)H.c[t];
+    std::memcpy(&s9[off], H.s9.data(), n * sizeof(double));
+    std::memcpy(&corr9[off], H.corr9.data(), n * sizeof(double));
+    std::memcpy(&jl[off], H.jl.data(), n * sizeof(double));
+    // line factors of every live mode
+    std::vector<double> lo(std::max(nx, ny)), di(lo.size()), up(lo.size()), f1(lo.size()), fa(lo.size()), fc(lo.size());
+    for (int m = 0; m < nmode; ++m) {
+      const double kx2 = kx2_mode[m];
+      T* fz = fac.data() + (size_t)m * fstride + g.lev[k].f_off;
+      T* fy = fz + 3 * n;
+      auto c9 = [&](size_t q) { return ((-(kx2 * H.jl[q])) - H.s9[q]) + H.corr9[q]; };
+      for (int j = 0; j < ny; ++j) {
+        for (int i = 0; i < nx; ++i) {
+          const size_t q = i + (size_t)nx * j;
+          if (spec[off + q]) {
+            lo[i] = 0.0; di[i] = 1.0; up[i] = 0.0;
+          } else {
+            lo[i] = H.c[4 * n + q]; di[i] = c9(q); up[i] = H.c[0 * n + q];
+          }
+        }
+        line_factors(nx, lo.data(), di.data(), up.data(), f1.data(), fa.data(), fc.data());
+        for (int i = 0; i < nx; ++i) {
+          const size_t q = i + (size_t)nx * j;
+          fz[q] = (T)f1[i]; fz[n + q] = (T)fa[i]; fz[2 * n + q] = (T)fc[i];
+        }
+      }
+      for (int i = 0; i < nx; ++i) {
+        for (int j = 0; j < ny; ++j) {
+          const size_t q = i + (size_t)nx * j;
+          if (spec[off + q]) {
+            lo[j] = 0.0; di[j] = 1.0; up[j] = 0.0;
+          } else {
+            lo[j] = H.c[6 * n + q]; di[j] = c9(q); up[j] = H.c[2 * n + q];
+          }
+        }
+        line_factors(ny, lo.data(), di.data(), up.data(), f1.data(), fa.data(), fc.data());
+        for (int j = 0; j < ny; ++j) {
+          const size_t q = i + (size_t)nx * j;
+          fy[q] = (T)f1[j]; fy[n + q] = (T)fa[j]; fy[2 * n + q] = (T)fc[j];
+        }
+      }
+    }
+    off += n;
+  }
+  // device copies
+  char* mem = nullptr;
+  const size_t b_spec = (tot_n + 255) & ~(size_t)255;
+  const size_t b_cf = (8 * tot_n * sizeof(T) + 255) & ~(size_t)255;
+  const size_t b_d = (tot_n * sizeof(double) + 255) & ~(size_t)255;
+  const size_t nfine = (size_t)g.lev[ng].n;
+  const size_t b_cf64 = (8 * nfine * sizeof(double) + 255) & ~(size_t)255;
+  const size_t b_desc = (sizeof(TDesc<T>) + 255) & ~(size_t)255;
+  const size_t total = b_desc + b_spec + b_cf + 3 * b_d + b_cf64;
+  EKM_CHECK(hipMalloc(&mem, total));
+  tp.mem = mem;
+  char* cur = mem + b_desc;
+  unsigned char* d_spec = (unsigned char*)cur;
+  cur += b_spec;
+  T* d_cf = (T*)cur;
+  cur += b_cf;
+  double* d_s9 = (double*)cur;
+  cur += b_d;
+  double* d_corr9 = (double*)cur;
+  cur += b_d;
+  double* d_jl = (double*)cur;
+  cur += b_d;
+  double* d_cf64 = (double*)cur;
+  for (int k = 1; k <= ng; ++k) {
+    TLev<T>& L = g.lev[k];
+    L.spec = d_spec + loff[k];
+    L.cf = d_cf + 8 * loff[k];
+    L.s9 = d_s9 + loff[k];
+    L.corr9 = d_corr9 + loff[k];
+    L.jl = d_jl + loff[k];
+  }
+  g.cf64 = d_cf64;
+  EKM_CHECK(hipMalloc(&tp.fac, fac.size() * sizeof(T)));
+  g.fac = (const T*)tp.fac;
+  EKM_CHECK(hipMemcpy(tp.fac, fac.data(), fac.size() * sizeof(T), hipMemcpyHostToDevice));
+  EKM_CHECK(hipMemcpy(mem, &g, sizeof(g), hipMemcpyHostToDevice));
+  EKM_CHECK(hipMemcpy(d_spec, spec.data(), tot_n, hipMemcpyHostToDevice));
+  EKM_CHECK(hipMemcpy(d_cf, cfT.data(), 8 * tot_n * sizeof(T), hipMemcpyHostToDevice));
+  EKM_CHECK(hipMemcpy(d_s9, s9.data(), tot_n * sizeof(double), hipMemcpyHostToDevice));
+  EKM_CHECK(hipMemcpy(d_corr9, corr9.data(), tot_n * sizeof(double), hipMemcpyHostToDevice));
+  EKM_CHECK(hipMemcpy(d_jl, jl.data(), tot_n * sizeof(double), hipMemcpyHostToDevice));
+  EKM_CHECK(hipMemcpy(d_cf64, hl[ng].c.data(), 8 * nfine * sizeof(double), hipMemcpyHostToDevice));
+  tp.ws_elems = g.ws;
+  tp.ws64_elems = g.ws64;
+  return EKM_OK;
+}
+
+}  // namespace
+
+int tight_plan_build(TightPlan& tp, const ekm_params& p, const Dims& d, const HostMetrics& hm, const Ranges& rv,
+                     const double* kx2_mode, int nmode, int nsolve_max) {
+  tp.fp32 = p.proj_mode == 2;
+  const int nf = d.NZ2;
+  tp.cm = nf <= 320 ? 5 : nf <= 576 ? 9 : nf <= 832 ? 13 : 0;
+  if (!tp.cm) return EKM_ERR_UNSUPPORTED;
+  tp.nsolve_max = nsolve_max;
+  int st = tp.fp32 ? tight_build_typed<float>(tp, p, d, hm, rv, kx2_mode, nmode)
+                   : tight_build_typed<double>(tp, p, d, hm, rv, kx2_mode, nmode);
+  if (st) return st;
+  const size_t esz = tp.fp32 ? sizeof(float) : sizeof(double);
+  EKM_CHECK(hipMalloc(&tp.ws, (size_t)nsolve_max * tp.ws_elems * esz));
+  EKM_CHECK(hipMalloc(&tp.ws64, (size_t)nsolve_max * tp.ws64_elems * sizeof(double)));
+  return EKM_OK;
+}
+
+void tight_plan_free(TightPlan& tp) {
+  if (tp.mem) (void)hipFree(tp.mem);
+  if (tp.fac) (void)hipFree(tp.fac);
+  if (tp.ws) (void)hipFree(tp.ws);
+  if (tp.ws64) (void)hipFree(tp.ws64);
+  tp = TightPlan();
+}
+
+int tight_solve_ex(TightPlan& tp, MgPlan& mg, const Dims& d, int nsolve, hipStream_t st) {
+  if (nsolve <= 0) return EKM_OK;
+  if (nsolve > tp.nsolve_max) return EKM_ERR_ARG;
+  const dim3 grid(nsolve), block(512);
+#define LAUNCH(T, CM)                                                                                             \
+  hipLaunchKernelGGL((k_tight_solve<T, CM>), grid, block, 0, st, (const TDesc<T>*)tp.mem, (T*)tp.ws, tp.ws64, \
+                     (const double*)mg.rhs, mg.phi, (const double*)mg.kx2, mg.cycles, d.plane)
+  if (tp.fp32) {
+    if (tp.cm == 5) LAUNCH(float, 5);
+    else if (tp.cm == 9) LAUNCH(float, 9);
+    else LAUNCH(float, 13);
+  } else {
+    if (tp.cm == 5) LAUNCH(double, 5);
+    else if (tp.cm == 9) LAUNCH(double, 9);
+    else LAUNCH(double, 13);
+  }
+#undef LAUNCH
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}

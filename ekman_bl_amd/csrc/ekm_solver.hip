@@ -29,6 +29,7 @@ struct ekm_solver {
   hipStream_t st = nullptr;
   FftPlan fft;
   MgPlan mg;
+  TightPlan tp;  // proj_mode 1, 2
   HostMetrics hm;
   double* met = nullptr;
   Metrics M;
@@ -75,6 +76,9 @@ void ekm_params_default(ekm_params* p) {
   p->mg_maxcy = 6;
   p->mg_tolmax = 1.0e-6;
   for (int s = 0; s < 4; ++s) p->mudpack_bc[s] = 2;
+  p->proj_mode = 0;
+  p->proj_maxit = 50;
+  p->proj_tol = 1.0e-12;
 }
 
 }  // extern "C"
@@ -92,6 +96,18 @@ static int check_params(const ekm_params& p) {
     if (p.mudpack_bc[s] < 0 || p.mudpack_bc[s] > 2) return EKM_ERR_UNSUPPORTED;
   if ((p.mudpack_bc[0] == 0) != (p.mudpack_bc[1] == 0)) return EKM_ERR_ARG;
   if ((p.mudpack_bc[2] == 0) != (p.mudpack_bc[3] == 0)) return EKM_ERR_ARG;
+  if (p.proj_mode < 0 || p.proj_mode > 2) return EKM_ERR_UNSUPPORTED;
+  if (p.proj_mode > 0) {
+    // line smoothing is not cyclic, and the consistent system needs a
+    // specified side to be nonsingular at kx = 0
+    bool dir = false;
+    for (int k = 0; k < 4; ++k) {
+      if (p.mudpack_bc[k] == 0) return EKM_ERR_UNSUPPORTED;
+      if (p.mudpack_bc[k] == 1) dir = true;
+    }
+    if (!dir) return EKM_ERR_UNSUPPORTED;
+    if (!(p.proj_tol > 0.0) || p.proj_maxit < 1) return EKM_ERR_ARG;
+  }
   return EKM_OK;
 }
 
@@ -126,6 +142,7 @@ static int free_all(ekm_solver* s) {
     if (q) (void)hipFree(q);
   fft_plan_free(s->fft);
   mg_plan_free(s->mg);
+  tight_plan_free(s->tp);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->st) (void)hipStreamDestroy(s->st);
@@ -228,6 +245,10 @@ extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const doub
     st = mg_plan_build(s->mg, p, d, s->hm, nsolve, block, budget);
   }
   if (st != EKM_OK) return fail(st);
+  if (p.proj_mode > 0) {
+    st = tight_plan_build(s->tp, p, d, s->hm, s->rv, kx2.data(), d.NKX + 1, nsolve);
+    if (st != EKM_OK) return fail(st);
+  }
   s->cyc_host.resize(nsolve);
   s->delta_t = p.delta_t;
   s->delta_t_set = p.delta_t;
@@ -294,7 +315,10 @@ static int rem_div(ekm_solver* s) {
   TRY(launch_mg_gather(d, s->M, s->rv, s->kx, s->U[0], s->U2b, s->U3b, s->mg.rhs, nmode, st));
   EKM_CHECK(hipMemcpyAsync(s->mg.kx2, s->kx2s, nsolve * sizeof(double), hipMemcpyDeviceToDevice, st));
   if (s->prof) EKM_CHECK(hipEventRecord(s->ev0, st));
-  TRY(mg_solve_ex(s->mg, d, nsolve, 0, st));
+  if (s->p.proj_mode == 0)
+    TRY(mg_solve_ex(s->mg, d, nsolve, 0, st));
+  else
+    TRY(tight_solve_ex(s->tp, s->mg, d, nsolve, st));
   if (s->prof) EKM_CHECK(hipEventRecord(s->ev1, st));
   TRY(launch_mg_scatter(d, s->mg.phi, nmode, s->p.cmplx_fp32, s->PHI, st));
   TRY(launch_project(d, s->M, s->rv, s->kx, s->PHI, s->U2b, s->U3b, s->U[0], s->U[1], s->U[2], s->S1, st));

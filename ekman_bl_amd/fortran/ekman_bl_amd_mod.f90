@@ -44,7 +44,10 @@ module ekman_bl_amd
      real(c_double) :: mg_tolmax
      integer(c_int) :: cmplx_fp32
      integer(c_int) :: coriolis_mean_only
-     integer(c_int) :: reserved(8)
+     integer(c_int) :: proj_mode
+     integer(c_int) :: proj_maxit
+     real(c_double) :: proj_tol
+     integer(c_int) :: reserved(4)
   end type ekm_params
 
   type, bind(c) :: ekm_diag

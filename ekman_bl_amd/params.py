@@ -48,7 +48,10 @@ class EkmParams(ctypes.Structure):
         ("mg_tolmax", ctypes.c_double),
         ("cmplx_fp32", ctypes.c_int),
         ("coriolis_mean_only", ctypes.c_int),
-        ("reserved", ctypes.c_int * 8),
+        ("proj_mode", ctypes.c_int),
+        ("proj_maxit", ctypes.c_int),
+        ("proj_tol", ctypes.c_double),
+        ("reserved", ctypes.c_int * 4),
     ]
 
     # -- convenience ---------------------------------------------------------

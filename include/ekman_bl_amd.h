@@ -72,7 +72,17 @@ typedef struct ekm_params {
   /* options that reproduce or remove reference quirks */
   int cmplx_fp32;          /* 1: round phi through default-kind CMPLX as the reference does (src/duct.F90:2664) */
   int coriolis_mean_only;  /* 1: subtract the geostrophic offsets from mode 0 only; 0 = reference (all modes, src/duct.F90:471-473) */
-  int reserved[8];
+  /* pressure-projection form (no counterpart in the reference):
+   * 0 = MUDPACK replica, W(2,1) cycles to relmax <= mg_tolmax (the drop-in);
+   * 1 = consistent projection: on the cells the divergence diagnostic covers
+   *     (J=2..NY-1, K=2..NZ-1) the Poisson rows are the divergence of the
+   *     projection's own face gradient, so J div u after the projection equals
+   *     the solve's residual; solved by FP32 alternating zebra-line V(1,1)
+   *     cycles inside FP64 iterative refinement until max|residual| <= proj_tol */
+  int proj_mode;
+  int proj_maxit;          /* cycle cap of proj_mode 1 */
+  double proj_tol;         /* absolute residual tolerance of proj_mode 1 */
+  int reserved[4];
 } ekm_params;
 
 /* Fill with the reference defaults that are not grid-size dependent. */

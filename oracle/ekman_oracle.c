@@ -1115,6 +1115,280 @@ static int mg_solve(mg_ctx* c, double kx2, const double* rhs_tmp, double* phiout
 #undef CF
 #undef PH
 
+/* ------------------------------------------------------------------------ */
+/* Consistent projection (proj_mode 1).  Not in the reference: the rows of   */
+/* the cells the divergence diagnostic covers (J=2..NY-1, K=2..NZ-1) are     */
+/* replaced by the divergence (src/duct.F90:2754-2760) of the projection's   */
+/* face gradient (:2713-2724) and the centre correction of CU1 (:2731), so   */
+/* J div u after the projection is the residual of the solve there.  Other   */
+/* rows keep MUDPACK's discretisation.  Solved by alternating zebra-line     */
+/* V(1,1) cycles (full weighting, bilinear prolongation) inside residual     */
+/* iterations until max|residual| <= proj_tol.                               */
+/* ------------------------------------------------------------------------ */
+typedef struct tlev {
+  int nx, ny;
+  double *cf, *e, *r, *res; /* cf: 9 planes [q][j][i]; e: (nx+2)(ny+2) with a zero ring */
+  unsigned char* spec;
+} tlev;
+
+typedef struct tctx {
+  int ngrid, nxa, nxb, nyc, nyd;
+  tlev lev[17];
+  double *lo, *di, *up, *s, *cp, *dp, *phi;
+} tctx;
+
+/* Row of cell (K,J) of the consistent operator, coefficients in MUDPACK's
+ * neighbour order c1 (K+1,J), c2 (K+1,J+1), c3 (K,J+1), c4 (K-1,J+1),
+ * c5 (K-1,J), c6 (K-1,J-1), c7 (K,J-1), c8 (K+1,J-1), c9 (K,J).  Faces the
+ * projection does not correct (outside J=JSTART..JEND, K=ZSTART..ZEND)
+ * contribute nothing. */
+static void tight_row(const ora* o, double kx2, int K, int J, double* c) {
+  const double tm = (J + 1 <= o->jend) ? 1.0 : 0.0, bm = (J >= o->jstart) ? 1.0 : 0.0;
+  const double rm = (K + 1 <= o->zend) ? 1.0 : 0.0, lm = (K >= o->zstart) ? 1.0 : 0.0;
+  const double a = G22(o, K, J + 1, 1) * tm, b = G22(o, K, J, 1) * bm;
+  const double cc = G11(o, K + 1, J, 2) * rm, d = G11(o, K, J, 2) * lm;
+  const double e = 0.25 * G12(o, K, J + 1, 1) * tm, f = 0.25 * G12(o, K, J, 1) * bm;
+  const double g = 0.25 * G12(o, K + 1, J, 2) * rm, h = 0.25 * G12(o, K, J, 2) * lm;
+  c[0] = (e - f) + cc;
+  c[1] = e + g;
+  c[2] = (a + g) - h;
+  c[3] = -e - h;
+  c[4] = (-e + f) + d;
+  c[5] = f + h;
+  c[6] = (b - g) + h;
+  c[7] = -f - g;
+  c[8] = (((-a - b) - cc) - d) - kx2 * JAC(o, K, J);
+}
+
+static void tight_setup(tctx* t, const mg_ctx* c) {
+  memset(t, 0, sizeof(*t));
+  t->ngrid = c->ngrid;
+  t->nxa = c->nxa; t->nxb = c->nxb; t->nyc = c->nyc; t->nyd = c->nyd;
+  size_t mx = 0;
+  for (int k = 1; k <= c->ngrid; ++k) {
+    tlev* L = &t->lev[k];
+    L->nx = c->lev[k].nx; L->ny = c->lev[k].ny;
+    size_t n = (size_t)L->nx * L->ny;
+    L->cf = dalloc(9 * n);
+    L->e = dalloc((size_t)(L->nx + 2) * (L->ny + 2));
+    L->r = dalloc(n);
+    L->res = dalloc(n);
+    L->spec = (unsigned char*)calloc(n, 1);
+    for (int j = 0; j < L->ny; ++j)
+      for (int i = 0; i < L->nx; ++i)
+        L->spec[i + (size_t)L->nx * j] = (t->nxa == 1 && i == 0) || (t->nxb == 1 && i == L->nx - 1) ||
+                                         (t->nyc == 1 && j == 0) || (t->nyd == 1 && j == L->ny - 1);
+    if ((size_t)L->nx > mx) mx = L->nx;
+    if ((size_t)L->ny > mx) mx = L->ny;
+  }
+  t->lo = dalloc(mx); t->di = dalloc(mx); t->up = dalloc(mx);
+  t->s = dalloc(mx); t->cp = dalloc(mx); t->dp = dalloc(mx);
+  t->phi = dalloc((size_t)(c->nfx + 2) * (c->nfy + 2));
+}
+
+static void tight_free(tctx* t) {
+  for (int k = 1; k <= t->ngrid; ++k) {
+    tlev* L = &t->lev[k];
+    free(L->cf); free(L->e); free(L->r); free(L->res); free(L->spec);
+  }
+  free(t->lo); free(t->di); free(t->up); free(t->s); free(t->cp); free(t->dp); free(t->phi);
+}
+
+/* operator coefficients of every level for kx2: MUDPACK's discretisation,
+ * with the consistent rows on the finest level */
+static void tight_discretize(tctx* t, mg_ctx* c, const ora* o, double kx2) {
+  c->kx2 = kx2;
+  for (int k = 1; k <= t->ngrid; ++k) {
+    mg_discretize(c, k);
+    tlev* L = &t->lev[k];
+    size_t n = (size_t)L->nx * L->ny;
+    for (int q = 0; q < 9; ++q) memcpy(L->cf + q * n, c->lev[k].cf + q * n, n * sizeof(double));
+  }
+  tlev* F = &t->lev[t->ngrid];
+  size_t n = (size_t)F->nx * F->ny;
+  double row[9];
+  for (int J = 2; J <= o->NY - 1; ++J)
+    for (int K = 2; K <= o->NZ - 1; ++K) {
+      tight_row(o, kx2, K, J, row);
+      for (int q = 0; q < 9; ++q) F->cf[q * n + K + (size_t)F->nx * J] = row[q];
+    }
+}
+
+#define TE(L, i, j) (L)->e[((i) + 1) + (size_t)((L)->nx + 2) * ((j) + 1)]
+#define TC(L, q, i, j) (L)->cf[(size_t)(q) * (L)->nx * (L)->ny + (i) + (size_t)(L)->nx * (j)]
+
+/* res = r - A e on level L (0 at specified points) */
+static void tight_residual(const tlev* L, const double* e, const double* r, double* res) {
+  const int nx = L->nx, ny = L->ny;
+  static const int di[8] = {1, 1, 0, -1, -1, -1, 0, 1}, dj[8] = {0, 1, 1, 1, 0, -1, -1, -1};
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      size_t p = i + (size_t)nx * j;
+      if (L->spec[p]) { res[p] = 0.0; continue; }
+      double acc = 0.0;
+      for (int q = 0; q < 8; ++q) acc += TC(L, q, i, j) * e[(i + di[q] + 1) + (size_t)(nx + 2) * (j + dj[q] + 1)];
+      acc += TC(L, 8, i, j) * e[(i + 1) + (size_t)(nx + 2) * (j + 1)];
+      res[p] = r[p] - acc;
+    }
+}
+
+static void thomas_line(int n, const double* lo, const double* di, const double* up, const double* s,
+                        double* cp, double* dp, double* x, size_t stride) {
+  cp[0] = up[0] / di[0];
+  dp[0] = s[0] / di[0];
+  for (int i = 1; i < n; ++i) {
+    double den = di[i] - lo[i] * cp[i - 1];
+    cp[i] = up[i] / den;
+    dp[i] = (s[i] - lo[i] * dp[i - 1]) / den;
+  }
+  x[(size_t)(n - 1) * stride] = dp[n - 1];
+  for (int i = n - 2; i >= 0; --i) x[(size_t)i * stride] = dp[i] - cp[i] * x[(size_t)(i + 1) * stride];
+}
+
+/* zebra line Gauss-Seidel: dir 0 lines along i (z) for rows j of parity par,
+ * dir 1 lines along j (y) for columns i of parity par */
+static void tight_lines(tctx* t, tlev* L, int dir, int par) {
+  const int nx = L->nx, ny = L->ny, ld = nx + 2;
+  if (dir == 0) {
+    for (int j = par; j < ny; j += 2) {
+      for (int i = 0; i < nx; ++i) {
+        size_t p = i + (size_t)nx * j;
+        if (L->spec[p]) { t->lo[i] = 0.0; t->di[i] = 1.0; t->up[i] = 0.0; t->s[i] = TE(L, i, j); continue; }
+        t->s[i] = L->r[p] - (TC(L, 1, i, j) * TE(L, i + 1, j + 1) + TC(L, 2, i, j) * TE(L, i, j + 1) +
+                             TC(L, 3, i, j) * TE(L, i - 1, j + 1) + TC(L, 5, i, j) * TE(L, i - 1, j - 1) +
+                             TC(L, 6, i, j) * TE(L, i, j - 1) + TC(L, 7, i, j) * TE(L, i + 1, j - 1));
+        t->lo[i] = TC(L, 4, i, j); t->di[i] = TC(L, 8, i, j); t->up[i] = TC(L, 0, i, j);
+      }
+      thomas_line(nx, t->lo, t->di, t->up, t->s, t->cp, t->dp, &TE(L, 0, j), 1);
+    }
+  } else {
+    for (int i = par; i < nx; i += 2) {
+      for (int j = 0; j < ny; ++j) {
+        size_t p = i + (size_t)nx * j;
+        if (L->spec[p]) { t->lo[j] = 0.0; t->di[j] = 1.0; t->up[j] = 0.0; t->s[j] = TE(L, i, j); continue; }
+        t->s[j] = L->r[p] - (TC(L, 0, i, j) * TE(L, i + 1, j) + TC(L, 1, i, j) * TE(L, i + 1, j + 1) +
+                             TC(L, 3, i, j) * TE(L, i - 1, j + 1) + TC(L, 4, i, j) * TE(L, i - 1, j) +
+                             TC(L, 5, i, j) * TE(L, i - 1, j - 1) + TC(L, 7, i, j) * TE(L, i + 1, j - 1));
+        t->lo[j] = TC(L, 6, i, j); t->di[j] = TC(L, 8, i, j); t->up[j] = TC(L, 2, i, j);
+      }
+      thomas_line(ny, t->lo, t->di, t->up, t->s, t->cp, t->dp, &TE(L, i, 0), (size_t)ld);
+    }
+  }
+}
+
+static void tight_smooth(tctx* t, tlev* L) {
+  tight_lines(t, L, 0, 0); tight_lines(t, L, 0, 1);
+  tight_lines(t, L, 1, 0); tight_lines(t, L, 1, 1);
+}
+
+/* full weighting (1/4,1/2,1/4 per direction, zero outside the grid) of the
+ * fine residual onto the coarse points; a direction that is not coarsened
+ * keeps its index */
+static void tight_restrict(const tlev* F, const double* res, tlev* C) {
+  const int sx = C->nx < F->nx ? 2 : 1, sy = C->ny < F->ny ? 2 : 1;
+  static const double w[3] = {0.25, 0.5, 0.25};
+  for (int jc = 0; jc < C->ny; ++jc)
+    for (int ic = 0; ic < C->nx; ++ic) {
+      size_t pc = ic + (size_t)C->nx * jc;
+      if (C->spec[pc]) { C->r[pc] = 0.0; continue; }
+      int i = sx * ic, j = sy * jc;
+      double acc = 0.0;
+      for (int b = -1; b <= 1; ++b)
+        for (int a = -1; a <= 1; ++a) {
+          int ii = i + a, jj = j + b;
+          if (ii < 0 || ii >= F->nx || jj < 0 || jj >= F->ny) continue;
+          acc += (w[a + 1] * w[b + 1]) * res[ii + (size_t)F->nx * jj];
+        }
+      C->r[pc] = acc;
+    }
+}
+
+/* e_fine += bilinear interpolation of e_coarse (not at specified points) */
+static void tight_prolong(const tlev* C, tlev* F) {
+  const int sx = C->nx < F->nx ? 2 : 1, sy = C->ny < F->ny ? 2 : 1;
+  for (int j = 0; j < F->ny; ++j)
+    for (int i = 0; i < F->nx; ++i) {
+      if (F->spec[i + (size_t)F->nx * j]) continue;
+      int i0 = i / sx, i1 = (i + sx - 1) / sx, j0 = j / sy, j1 = (j + sy - 1) / sy;
+      double v = 0.25 * ((TE(C, i0, j0) + TE(C, i1, j0)) + (TE(C, i0, j1) + TE(C, i1, j1)));
+      TE(F, i, j) += v;
+    }
+}
+
+static void tight_vcycle(tctx* t, int l) {
+  tlev* L = &t->lev[l];
+  if (l == 1) {
+    for (int s = 0; s < 20; ++s) tight_smooth(t, L);
+    return;
+  }
+  tight_smooth(t, L);
+  tight_residual(L, L->e, L->r, L->res);
+  tlev* C = &t->lev[l - 1];
+  tight_restrict(L, L->res, C);
+  memset(C->e, 0, (size_t)(C->nx + 2) * (C->ny + 2) * sizeof(double));
+  tight_vcycle(t, l - 1);
+  tight_prolong(C, L);
+  tight_smooth(t, L);
+}
+
+/* Solves the consistent system for one real right-hand side (MUDPACK's
+ * convention: rhs = -RHS_TMP); returns the number of V cycles. */
+static int tight_solve(tctx* t, mg_ctx* c, const ora* o, double kx2, const double* rhs_tmp, double* phiout) {
+  tight_discretize(t, c, o, kx2);
+  tlev* F = &t->lev[t->ngrid];
+  const int nx = F->nx, ny = F->ny;
+  const size_t n = (size_t)nx * ny;
+  double* b = dalloc(n);
+  double* res = dalloc(n);
+  for (size_t p = 0; p < n; ++p) b[p] = F->spec[p] ? 0.0 : -rhs_tmp[p];
+  memset(t->phi, 0, (size_t)(nx + 2) * (ny + 2) * sizeof(double));
+  int it = 0;
+  const int maxit = o->p.proj_maxit > 0 ? o->p.proj_maxit : 50;
+  for (;;) {
+    tight_residual(F, t->phi, b, res);
+    double rmax = 0.0;
+    for (size_t p = 0; p < n; ++p) rmax = fmax(rmax, fabs(res[p]));
+    if (rmax <= o->p.proj_tol || it >= maxit) break;
+    memcpy(F->r, res, n * sizeof(double));
+    memset(F->e, 0, (size_t)(nx + 2) * (ny + 2) * sizeof(double));
+    tight_vcycle(t, t->ngrid);
+    for (size_t q = 0; q < (size_t)(nx + 2) * (ny + 2); ++q) t->phi[q] += F->e[q];
+    ++it;
+  }
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) phiout[i + (size_t)nx * j] = t->phi[(i + 1) + (size_t)(nx + 2) * (j + 1)];
+  free(b);
+  free(res);
+  return it;
+}
+#undef TE
+#undef TC
+
+int ora_tight_solve(ora* o, double kx2, const double* rhs_tmp, double* phi) {
+  mg_ctx c;
+  tctx t;
+  mg_setup(&c, o);
+  tight_setup(&t, &c);
+  int it = tight_solve(&t, &c, o, kx2, rhs_tmp, phi);
+  tight_free(&t);
+  mg_free(&c);
+  return it;
+}
+
+/* Fine-level operator of proj_mode 1 for kx2: cf[9][NY+2][NZ+2]. */
+int ora_tight_operator(ora* o, double kx2, double* cf) {
+  mg_ctx c;
+  tctx t;
+  mg_setup(&c, o);
+  tight_setup(&t, &c);
+  tight_discretize(&t, &c, o, kx2);
+  tlev* F = &t.lev[t.ngrid];
+  memcpy(cf, F->cf, 9 * (size_t)F->nx * F->ny * sizeof(double));
+  tight_free(&t);
+  mg_free(&c);
+  return EKM_OK;
+}
+
 int ora_mg_ngrid(ora* o) {
   int a = o->p.mg_iex, b = o->p.mg_jey;
   return a > b ? a : b;
@@ -1179,6 +1453,15 @@ static int supported(const ekm_params* p) {
   if (p->nz + 2 != p->mg_ixp * (1 << (p->mg_iex - 1)) + 1) return 0;
   if (p->ny + 2 != p->mg_jyq * (1 << (p->mg_jey - 1)) + 1) return 0;
   if (p->nx < 8 || (p->nx & (p->nx - 1))) return 0;
+  if (p->proj_mode < 0 || p->proj_mode > 1) return 0;
+  if (p->proj_mode == 1) {
+    int dir = 0;
+    for (int s = 0; s < 4; ++s) {
+      if (p->mudpack_bc[s] == 0) return 0;
+      if (p->mudpack_bc[s] == 1) dir = 1;
+    }
+    if (!dir || !(p->proj_tol > 0.0)) return 0;
+  }
   return 1;
 }
 
@@ -1556,7 +1839,9 @@ static int rem_div(ora* o) {
     double* pim = dalloc(pl);
     double* pre = dalloc(pl);
     mg_ctx c;
+    tctx t;
     mg_setup(&c, o);
+    if (o->p.proj_mode == 1) tight_setup(&t, &c);
 #pragma omp for schedule(dynamic, 1)
     for (int m = 0; m <= NKX; ++m) {
       for (int part = 0; part < 2; ++part) { /* imaginary part first */
@@ -1564,7 +1849,8 @@ static int rem_div(ora* o) {
         for (int k = zs; k <= ze; ++k)
           for (int j = js; j <= je; ++j)
             rhs[PL(o, k, j)] = -CR1[CRE(o, m, k, j) + (part == 0 ? 1 : 0)];
-        int cy = mg_solve(&c, o->kx2[m], rhs, part == 0 ? pim : pre);
+        int cy = o->p.proj_mode == 1 ? tight_solve(&t, &c, o, o->kx2[m], rhs, part == 0 ? pim : pre)
+                                     : mg_solve(&c, o->kx2[m], rhs, part == 0 ? pim : pre);
         cyc_sum += cy;
         nsolve += 1;
         if (cy > cyc_max) cyc_max = cy;
@@ -1577,6 +1863,7 @@ static int rem_div(ora* o) {
           CR1[CRE(o, m, k, j) + 1] = b;
         }
     }
+    if (o->p.proj_mode == 1) tight_free(&t);
     mg_free(&c);
     free(rhs); free(pim); free(pre);
   }

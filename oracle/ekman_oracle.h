@@ -67,6 +67,10 @@ int ora_mudpack(ora* o, double kx2, const double* rhs_tmp, double* phi);
  * (lev = 1 coarsest .. ngrid finest), for kx2; returns nx*ny or <0. */
 int ora_mg_discretize(ora* o, double kx2, int lev, double* cf, int* nx, int* ny);
 int ora_mg_ngrid(ora* o);
+/* proj_mode 1: one real solve of the consistent system (returns V cycles),
+ * and its fine-level operator cf[9][NY+2][NZ+2] for kx2. */
+int ora_tight_solve(ora* o, double kx2, const double* rhs_tmp, double* phi);
+int ora_tight_operator(ora* o, double kx2, double* cf);
 
 /* Callbacks with MUDPACK's calling convention (all arguments by reference),
  * bound to the oracle's metrics for the reference build in oracle/_ref. */

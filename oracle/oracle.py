@@ -59,6 +59,8 @@ def lib():
         L.ora_mg_discretize.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _D,
                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.ora_mg_ngrid.argtypes = [ctypes.c_void_p]
+        L.ora_tight_solve.argtypes = [ctypes.c_void_p, ctypes.c_double, _D, _D]
+        L.ora_tight_operator.argtypes = [ctypes.c_void_p, ctypes.c_double, _D]
         L.ora_cb_bind.argtypes = [ctypes.c_void_p, ctypes.c_double]
         _lib = L
     return _lib
@@ -167,6 +169,20 @@ class Oracle:
         phi = np.zeros_like(r)
         cyc = lib().ora_mudpack(self.h, float(kx2), _p(r), _p(phi))
         return phi, cyc
+
+    def tight_solve(self, kx2: float, rhs_tmp: np.ndarray):
+        """One real solve of the consistent projection system (proj_mode 1);
+        rhs_tmp is (NY+2, NZ+2).  Returns (phi, V cycles)."""
+        r = np.ascontiguousarray(rhs_tmp, dtype=np.float64)
+        phi = np.zeros_like(r)
+        it = lib().ora_tight_solve(self.h, float(kx2), _p(r), _p(phi))
+        return phi, it
+
+    def tight_operator(self, kx2: float) -> np.ndarray:
+        """Fine-level operator of proj_mode 1: (9, NY+2, NZ+2)."""
+        cf = np.zeros((9, self.p.ny + 2, self.p.nz + 2))
+        lib().ora_tight_operator(self.h, float(kx2), _p(cf))
+        return cf
 
     def mg_ngrid(self) -> int:
         return lib().ora_mg_ngrid(self.h)

@@ -379,3 +379,83 @@ def test_ekman_initial_condition():
     assert np.abs(dn).max() > 0.4 * cfg.noise
     n2, _, _, _ = ekman_initial(p, yp, cfg.seed, cfg.noise, 0.0)
     assert np.array_equal(n1, n2)
+
+
+# -- proj_mode 1: the consistent projection --------------------------------------
+def _tight_oracle(n=1, tol=1e-13):
+    cfg = config(n)
+    p = cfg.params
+    p.proj_mode, p.proj_tol, p.proj_maxit = 1, tol, 60
+    xp, yp = tanh_grid(p.ny, p.nz, p.ly, p.lz, cfg.gamma)
+    return cfg, p, xp, yp
+
+
+def test_tight_rows_are_divergence_of_projection_gradient(oracle_lib):
+    """On the cells J=2..NY-1, K=2..NZ-1 the operator rows equal J div u of
+    the projection's correction of a field phi, computed here directly from
+    the reference's face and centre corrections (src/duct.F90:2713-2731) and
+    its divergence (:2754-2760), with a skewed grid so every metric term is
+    active."""
+    O = oracle_lib
+    cfg, p, xp, yp = _tight_oracle()
+    xp = xp + 0.05 * np.sin(0.3 * yp)
+    o = O.Oracle(p, xp, yp)
+    kx2 = 0.37
+    cf = o.tight_operator(kx2)
+    G11, G12, G22, JAC = o.metric(0), o.metric(1), o.metric(2), o.metric(7)
+    rng = np.random.default_rng(4)
+    phi = rng.standard_normal((p.ny + 2, p.nz + 2))
+    NY, NZ = p.ny, p.nz
+    js, je, zs, ze = 1, NY, 1, NZ                     # ranges of this BC set
+    F2 = np.zeros((NY + 3, NZ + 3))                   # corrections of CU2b(K,J), CU3b(K,J)
+    F3 = np.zeros((NY + 3, NZ + 3))
+    for J in range(js, je + 1):
+        for K in range(zs, ze + 1):
+            F2[J, K] = G22[0, J, K] * (phi[J, K] - phi[J - 1, K]) + 0.25 * G12[0, J, K] * (
+                phi[J, K + 1] + phi[J - 1, K + 1] - phi[J, K - 1] - phi[J - 1, K - 1])
+            F3[J, K] = G11[1, J, K] * (phi[J, K] - phi[J, K - 1]) + 0.25 * G12[1, J, K] * (
+                phi[J + 1, K] + phi[J + 1, K - 1] - phi[J - 1, K] - phi[J - 1, K - 1])
+    off = {0: (1, 0), 1: (1, 1), 2: (0, 1), 3: (-1, 1), 4: (-1, 0), 5: (-1, -1), 6: (0, -1), 7: (1, -1)}
+    worst = 0.0
+    for J in range(2, NY):
+        for K in range(2, NZ):
+            # J div u of the correction: -(i kx J)(i kx phi) + face differences, with the sign
+            # the solve uses (L phi = div of the subtracted correction)
+            div = (F2[J + 1, K] - F2[J, K]) + (F3[J, K + 1] - F3[J, K]) - kx2 * JAC[J, K] * phi[J, K]
+            row = cf[8, J, K] * phi[J, K] + sum(cf[q, J, K] * phi[J + dj, K + di] for q, (di, dj) in off.items())
+            worst = max(worst, abs(row - div) / (abs(div) + 1e-300))
+    assert worst < 1e-12, worst
+
+
+def test_tight_solve_matches_dense(oracle_lib):
+    O = oracle_lib
+    cfg, p, xp, yp = _tight_oracle()
+    o = O.Oracle(p, xp, yp)
+    for kx2 in (0.0, 0.25):
+        rhs = np.random.default_rng(3).standard_normal((p.ny + 2, p.nz + 2)) * 1e-3
+        phi, it = o.tight_solve(kx2, rhs)
+        cf = o.tight_operator(kx2)
+        A, _, _ = _assemble(np.concatenate([cf, np.zeros((1,) + cf.shape[1:])]), p.nz + 2, p.ny + 2,
+                            tuple(p.mudpack_bc))
+        b = -rhs.copy()
+        b[-1, :] = 0.0                                  # Dirichlet top (MUDPACK_BC(4) = 1)
+        x = spla.spsolve(A.tocsc(), b.ravel()).reshape(b.shape)
+        assert it < 30
+        assert rel(phi, x) < 1e-8
+
+
+def test_tight_projection_removes_divergence(oracle_lib):
+    """proj_mode 1 leaves |J div u| at the solve tolerance on the cells the
+    reference's diagnostic covers; the MUDPACK replica does not."""
+    O = oracle_lib
+    cfg, p, xp, yp = _tight_oracle(1, 1e-11)
+    u1, u2, u3, th = ekman_initial(p, yp, cfg.seed, cfg.noise, cfg.th_noise)
+    o = O.Oracle(p, xp, yp)
+    o.set_initial_physical(u1, u2, u3, th)
+    d = o.diag()
+    assert d.div_max < 1e-10 and d.div_ref < 1e-11
+    p0 = cfg.params.copy()
+    p0.proj_mode = 0
+    o0 = O.Oracle(p0, xp, yp)
+    o0.set_initial_physical(u1, u2, u3, th)
+    assert o0.diag().div_ref > 1e-8
