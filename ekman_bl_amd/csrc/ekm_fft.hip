@@ -6,11 +6,16 @@
 // with modes NKX+1..NX/2 zeroed; inverse from modes 0..NKX (Im c_0 ignored),
 // padding entries NX, NX+1 zeroed.
 //
-// A real pencil of NX doubles is read as NX/2 complex values (even, odd);
-// an LDS-resident radix-2 Stockham FFT of length M = NX/2 runs with one
-// butterfly per thread per pass; the even/odd split is undone (forward) or
-// set up (inverse) in LDS.  P = 256/(M/2) pencils share a 256-thread block,
-// so global loads and stores are contiguous runs of P*(NX+2) doubles.
+// A real pencil of NX doubles is read as M = NX/2 complex values z_k =
+// (u_2k, u_2k+1).  One wavefront transforms one pencil (or 64/M pencils
+// when M < 64): lane l holds elements l, l+64, l+128, ... in registers, the
+// radix-2 decimation-in-frequency stages with partner distance >= 64 pair
+// registers of the same lane and the last six stages pair lanes through
+// cross-lane shuffles, so no pass goes through LDS and no block barrier is
+// needed.  The bit-reversed result is put in order through LDS, where the
+// even/odd split of the real transform (forward) pairs Z_k with Z_{M-k}.
+// The inverse builds Z_k from the modes in registers and runs the same
+// stages on the conjugate.  Loads and stores are contiguous across lanes.
 #include "ekm_internal.h"
 
 #include <cmath>
@@ -18,134 +23,164 @@
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kWaves = 4;  // wavefronts per block
+constexpr int kBlock = 64 * kWaves;
 
 __device__ inline double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 
-// One Stockham radix-2 pass over a pencil of M complex values held in `src`;
-// result in `dst`.  j in [0, M/2).
-template <bool INV>
-__device__ inline void stockham_pass(const double2* src, double2* dst, int M, int ns, int j,
-                                     const double2* __restrict__ tw) {
-  int half = M >> 1;
-  double2 v0 = src[j], v1 = src[j + half];
-  int k = j & (ns - 1);
-  double2 w = tw[k * (half / ns)];  // exp(-2 pi i k / (2 ns))
-  if (INV) w.y = -w.y;
-  v1 = cmul(v1, w);
-  int d = ((j - k) << 1) + k;
-  dst[d] = make_double2(v0.x + v1.x, v0.y + v1.y);
-  dst[d + ns] = make_double2(v0.x - v1.x, v0.y - v1.y);
-}
-
-template <bool INV>
-__device__ inline double2* run_fft(double2* a, double2* b, int M, int j, const double2* tw) {
-  for (int ns = 1; ns < M; ns <<= 1) {
-    stockham_pass<INV>(a, b, M, ns, j, tw);
-    __syncthreads();
-    double2* t = a;
-    a = b;
-    b = t;
+// DIF stages on M = GW*R elements; element e = lane' + GW*r sits in z[r] of
+// lane' (lane' = lane mod GW; GW = 64 whenever R > 1).  tw[t] = exp(-2 pi i
+// t / M), t < M/2.  Partner distances h >= GW pair registers of one lane
+// (compile-time indices), smaller ones pair lanes.
+template <int R>
+__device__ inline void dif(double2 (&z)[R], int GW, int M, int lanep, const double2* __restrict__ tw) {
+#pragma unroll
+  for (int hr = R / 2; hr >= 1; hr >>= 1) {
+    const int h = hr * GW, step = M / (2 * h);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if ((r / hr) % 2 == 0) {
+        const int e = lanep + GW * r;
+        const double2 x = z[r], y = z[r + hr];
+        z[r] = cadd(x, y);
+        z[r + hr] = cmul(csub(x, y), tw[(e % h) * step]);
+      }
+    }
   }
-  return a;
+  for (int h = GW >> 1; h >= 1; h >>= 1) {
+    const int step = M / (2 * h);
+    const bool top = (lanep & h) == 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double2 mine = z[r];
+      double2 other;
+      other.x = __shfl_xor(mine.x, h, 64);
+      other.y = __shfl_xor(mine.y, h, 64);
+      const int e = lanep + GW * r;
+      z[r] = top ? cadd(mine, other) : cmul(csub(other, mine), tw[(e % h) * step]);
+    }
+  }
 }
 
-__global__ void __launch_bounds__(kBlock) k_fft_fwd(double* __restrict__ v, long npencil, int nx,
+__device__ inline int bitrev(int e, int logm) { return (int)(__brev((unsigned)e) >> (32 - logm)); }
+
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_fft_fwd(double* __restrict__ v, long npencil, int nx, int logm,
                                                     const double2* __restrict__ tw,
                                                     const double2* __restrict__ tw2) {
   extern __shared__ double2 lds[];
-  const int M = nx >> 1, T = M >> 1, P = kBlock / T;
-  const int nkx = nx / 3;
-  const int stride = nx + 2;
-  long p0 = (long)blockIdx.x * P;
-  int np = (int)((npencil - p0) < P ? (npencil - p0) : P);
-  // two LDS regions of P*(M+1) values, pencil stride M+1, so either can take
-  // the NX+2 output doubles of every pencil without touching the other
-  const int ld = M + 1;
-  double2* A = lds;
-  double2* B = lds + (size_t)P * ld;
-  double* Ad = reinterpret_cast<double*>(A);
-  // load: the first NX doubles of each pencil are the packed (even, odd) array
-  const double* src = v + p0 * stride;
-  for (int t = threadIdx.x; t < np * stride; t += kBlock) {
-    int p = t / stride, i = t - p * stride;
-    if (i < nx) Ad[(size_t)p * 2 * ld + i] = src[t];
-  }
-  __syncthreads();
-  int p = threadIdx.x / T, j = threadIdx.x - p * T;
-  double2* Z = run_fft<false>(A + (size_t)p * ld, B + (size_t)p * ld, M, j, tw);
-  // the result sits in A or B depending on the pass-count parity (same for all pencils)
-  double2* Zall = (Z == A + (size_t)p * ld) ? A : B;
-  double2* Out = (Zall == A) ? B : A;
+  const int M = nx >> 1, GW = M < 64 ? M : 64, PW = 64 / GW;  // pencils per wave
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lanep = lane % GW, sub = lane / GW;
+  const long pen = ((long)blockIdx.x * kWaves + wave) * PW + sub;
+  const bool live = pen < npencil;
+  const int stride = nx + 2, nkx = nx / 3;
+  const double2* src = reinterpret_cast<const double2*>(v + (live ? pen : 0) * stride);
+  double2 z[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) z[r] = live ? src[lanep + GW * r] : make_double2(0.0, 0.0);
+  dif<R>(z, GW, M, lanep, tw);
+  // into natural order in this wave's LDS slice
+  double2* Z = lds + (size_t)(wave * PW + sub) * M;
+#pragma unroll
+  for (int r = 0; r < R; ++r) Z[bitrev(lanep + GW * r, logm)] = z[r];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   // split: X_k = E_k + W^k O_k, k = 0..M (only k <= NKX kept), scaled by 1/NX
+  if (!live) return;
   const double sc = 1.0 / (double)nx;
-  for (int t = threadIdx.x; t < np * (M + 1); t += kBlock) {
-    int q = t / (M + 1), k = t - q * (M + 1);
+  double2* dst = reinterpret_cast<double2*>(v + pen * stride);
+  for (int k = lanep; k <= M; k += GW) {
     double2 x = make_double2(0.0, 0.0);
     if (k <= nkx) {
-      const double2* Zq = Zall + (size_t)q * ld;
-      double2 z = Zq[k == M ? 0 : k];
-      double2 y = Zq[(M - k) % M];
+      const double2 zk = Z[k == M ? 0 : k];
+      double2 y = Z[(M - k) % M];
       y.y = -y.y;  // conj(Z_{M-k})
-      double2 e = make_double2(0.5 * (z.x + y.x), 0.5 * (z.y + y.y));
-      double2 o = make_double2(0.5 * (z.y - y.y), -0.5 * (z.x - y.x));
-      double2 w = tw2[k];
-      double2 wo = cmul(w, o);
+      const double2 e = make_double2(0.5 * (zk.x + y.x), 0.5 * (zk.y + y.y));
+      const double2 o = make_double2(0.5 * (zk.y - y.y), -0.5 * (zk.x - y.x));
+      const double2 wo = cmul(tw2[k], o);
       x = make_double2((e.x + wo.x) * sc, (e.y + wo.y) * sc);
     }
-    Out[(size_t)q * ld + k] = x;
+    dst[k] = x;
   }
-  __syncthreads();
-  double* dst = v + p0 * stride;
-  const double* Od = reinterpret_cast<const double*>(Out);
-  for (int t = threadIdx.x; t < np * stride; t += kBlock) dst[t] = Od[t];
 }
 
-__global__ void __launch_bounds__(kBlock) k_fft_inv(double* __restrict__ v, long npencil, int nx,
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_fft_inv(double* __restrict__ v, long npencil, int nx, int logm,
                                                     const double2* __restrict__ tw,
                                                     const double2* __restrict__ tw2) {
   extern __shared__ double2 lds[];
-  const int M = nx >> 1, T = M >> 1, P = kBlock / T;
-  const int nkx = nx / 3;
-  const int stride = nx + 2;
-  long p0 = (long)blockIdx.x * P;
-  int np = (int)((npencil - p0) < P ? (npencil - p0) : P);
-  double2* A = lds;                        // P*(M+1) input modes
-  double2* B = lds + (size_t)P * (M + 1);  // P*M work
-  double2* C = B + (size_t)P * M;          // P*M work
-  double* Ad = reinterpret_cast<double*>(A);
-  const double* src = v + p0 * stride;
-  for (int t = threadIdx.x; t < np * stride; t += kBlock) Ad[t] = src[t];
-  __syncthreads();
-  // Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/NX} (X_k - conj X_{M-k}), k < M
-  for (int t = threadIdx.x; t < np * M; t += kBlock) {
-    int q = t / M, k = t - q * M;
-    const double2* Xq = A + (size_t)q * (M + 1);
-    int b = M - k;
-    double2 x = (k <= nkx) ? Xq[k] : make_double2(0.0, 0.0);
+  const int M = nx >> 1, GW = M < 64 ? M : 64, PW = 64 / GW;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lanep = lane % GW, sub = lane / GW;
+  const long pen = ((long)blockIdx.x * kWaves + wave) * PW + sub;
+  const bool live = pen < npencil;
+  const int stride = nx + 2, nkx = nx / 3;
+  const double2* X = reinterpret_cast<const double2*>(v + (live ? pen : 0) * stride);
+  // Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/NX} (X_k - conj X_{M-k}), k < M,
+  // conjugated: the inverse transform is conj(DFT(conj Z))
+  double2 z[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int k = lanep + GW * r, b = M - k;
+    double2 x = (live && k <= nkx) ? X[k] : make_double2(0.0, 0.0);
     if (k == 0) x.y = 0.0;
-    double2 y = (b <= nkx && b < M) ? Xq[b] : make_double2(0.0, 0.0);
+    double2 y = (live && b <= nkx && b < M) ? X[b] : make_double2(0.0, 0.0);
     if (b == 0) y.y = 0.0;
     y.y = -y.y;
-    double2 e = make_double2(x.x + y.x, x.y + y.y);
-    double2 dd = make_double2(x.x - y.x, x.y - y.y);
+    const double2 e = cadd(x, y);
+    const double2 dd = csub(x, y);
     double2 w = tw2[k];
     w.y = -w.y;  // exp(+2 pi i k / NX)
-    double2 o = cmul(dd, w);
-    B[(size_t)q * M + k] = make_double2(e.x - o.y, e.y + o.x);
+    const double2 o = cmul(dd, w);
+    z[r] = make_double2(e.x - o.y, -(e.y + o.x));
   }
-  __syncthreads();
-  int p = threadIdx.x / T, j = threadIdx.x - p * T;
-  double2* Z = run_fft<true>(B + (size_t)p * M, C + (size_t)p * M, M, j, tw);
-  double2* Zall = (Z == B + (size_t)p * M) ? B : C;
-  const double* Zd = reinterpret_cast<const double*>(Zall);
-  double* dst = v + p0 * stride;
-  for (int t = threadIdx.x; t < np * stride; t += kBlock) {
-    int q = t / stride, i = t - q * stride;
-    dst[t] = (i < nx) ? Zd[(size_t)q * 2 * M + i] : 0.0;
+  dif<R>(z, GW, M, lanep, tw);
+  double2* Z = lds + (size_t)(wave * PW + sub) * M;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    double2 t = z[r];
+    t.y = -t.y;
+    Z[bitrev(lanep + GW * r, logm)] = t;
   }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  if (!live) return;
+  double2* dst = reinterpret_cast<double2*>(v + pen * stride);
+  for (int k = lanep; k <= M; k += GW) dst[k] = (k < M) ? Z[k] : make_double2(0.0, 0.0);
+}
+
+template <int R>
+int launch_pair(bool fwd, const FftPlan& p, double* v, long npencil, hipStream_t st) {
+  const int M = p.nx / 2, GW = M < 64 ? M : 64, PW = 64 / GW;
+  const long per_block = (long)kWaves * PW;
+  const long nb = (npencil + per_block - 1) / per_block;
+  const size_t lds = (size_t)kWaves * PW * M * sizeof(double2);
+  int logm = 0;
+  while ((1 << logm) < M) ++logm;
+  if (fwd)
+    hipLaunchKernelGGL(k_fft_fwd<R>, dim3((unsigned)nb), dim3(kBlock), lds, st, v, npencil, p.nx, logm,
+                       (const double2*)p.tw, (const double2*)p.tw2);
+  else
+    hipLaunchKernelGGL(k_fft_inv<R>, dim3((unsigned)nb), dim3(kBlock), lds, st, v, npencil, p.nx, logm,
+                       (const double2*)p.tw, (const double2*)p.tw2);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+
+int launch(bool fwd, const FftPlan& p, double* v, long npencil, hipStream_t st) {
+  if (npencil <= 0) return EKM_OK;
+  const int M = p.nx / 2;
+  if (M <= 64) return launch_pair<1>(fwd, p, v, npencil, st);
+  if (M == 128) return launch_pair<2>(fwd, p, v, npencil, st);
+  if (M == 256) return launch_pair<4>(fwd, p, v, npencil, st);
+  return launch_pair<8>(fwd, p, v, npencil, st);
 }
 
 }  // namespace
@@ -176,22 +211,6 @@ void fft_plan_free(FftPlan& p) {
   p.tw = p.tw2 = nullptr;
 }
 
-int fft_forward(const FftPlan& p, double* v, long npencil, hipStream_t st) {
-  int M = p.nx / 2, T = M / 2, P = kBlock / T;
-  long nb = (npencil + P - 1) / P;
-  size_t lds = (size_t)2 * P * (M + 1) * sizeof(double2);
-  hipLaunchKernelGGL(k_fft_fwd, dim3((unsigned)nb), dim3(kBlock), lds, st, v, npencil, p.nx,
-                     (const double2*)p.tw, (const double2*)p.tw2);
-  EKM_CHECK(hipGetLastError());
-  return EKM_OK;
-}
+int fft_forward(const FftPlan& p, double* v, long npencil, hipStream_t st) { return launch(true, p, v, npencil, st); }
 
-int fft_inverse(const FftPlan& p, double* v, long npencil, hipStream_t st) {
-  int M = p.nx / 2, T = M / 2, P = kBlock / T;
-  long nb = (npencil + P - 1) / P;
-  size_t lds = (size_t)P * ((M + 1) + 2 * M) * sizeof(double2);
-  hipLaunchKernelGGL(k_fft_inv, dim3((unsigned)nb), dim3(kBlock), lds, st, v, npencil, p.nx,
-                     (const double2*)p.tw, (const double2*)p.tw2);
-  EKM_CHECK(hipGetLastError());
-  return EKM_OK;
-}
+int fft_inverse(const FftPlan& p, double* v, long npencil, hipStream_t st) { return launch(false, p, v, npencil, st); }

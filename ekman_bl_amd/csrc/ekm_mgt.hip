@@ -167,35 +167,55 @@ __device__ void zlines(const TDesc<T>& g, const TLev<T>& L, T* W, const T* fac, 
   }
 }
 
-// y lines (columns i of parity par) of level L: lanes over columns.
+// y lines (columns i of parity par) of level L: lanes over columns.  The
+// forward substitution reads the correction of the neighbouring columns and
+// writes its intermediate into the level's residual array (free while
+// smoothing), so its loads carry no dependence on its stores and can run
+// ahead; the back substitution then writes the column.
+template <typename T>
+__device__ __forceinline__ void ylines_fwd(const TLev<T>& L, const T* __restrict__ e, const T* __restrict__ r,
+                                           const T* __restrict__ f1, const T* __restrict__ fa,
+                                           T* __restrict__ tmp, int i) {
+  const int nx = L.nx, ny = L.ny, ld = nx + 2, n = L.n;
+  const T* __restrict__ cf = L.cf;
+  const unsigned char* __restrict__ spec = L.spec;
+  T dp = (T)0;
+#pragma unroll 4
+  for (int j = 0; j < ny; ++j) {
+    const int p = i + nx * j;
+    const int q = (i + 1) + ld * (j + 1);
+    T s = (T)0;
+    if (!spec[p]) {
+      s = r[p] - (cf[0 * n + p] * e[q + 1] + cf[1 * n + p] * e[q + 1 + ld] + cf[3 * n + p] * e[q - 1 + ld] +
+                  cf[4 * n + p] * e[q - 1] + cf[5 * n + p] * e[q - 1 - ld] + cf[7 * n + p] * e[q + 1 - ld]);
+    }
+    dp = fa[p] * dp + s * f1[p];
+    tmp[p] = dp;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void ylines_bwd(const TLev<T>& L, const T* __restrict__ tmp, const T* __restrict__ fc,
+                                           T* __restrict__ e, int i) {
+  const int nx = L.nx, ny = L.ny, ld = nx + 2;
+  T x = (T)0;
+#pragma unroll 4
+  for (int j = ny - 1; j >= 0; --j) {
+    const int p = i + nx * j;
+    x = tmp[p] - fc[p] * x;
+    e[(i + 1) + ld * (j + 1)] = x;
+  }
+}
+
 template <typename T>
 __device__ void ylines(const TDesc<T>& g, const TLev<T>& L, T* W, const T* fac, int par) {
-  const int nx = L.nx, ny = L.ny, ld = nx + 2, n = L.n;
-  T* e = W + L.e_off;
-  const T* r = W + L.r_off;
+  const int nx = L.nx, n = L.n;
   const T* f1 = fac + L.f_off + 3 * n;
   const T* fa = f1 + n;
   const T* fc = fa + n;
   for (int i = par + 2 * (int)threadIdx.x; i < nx; i += 2 * (int)blockDim.x) {
-    T dp = (T)0;
-    for (int j = 0; j < ny; ++j) {
-      const int p = i + nx * j;
-      const int q = (i + 1) + ld * (j + 1);
-      T s = (T)0;
-      if (!L.spec[p]) {
-        s = r[p] - (L.cf[0 * n + p] * e[q + 1] + L.cf[1 * n + p] * e[q + 1 + ld] + L.cf[3 * n + p] * e[q - 1 + ld] +
-                    L.cf[4 * n + p] * e[q - 1] + L.cf[5 * n + p] * e[q - 1 - ld] + L.cf[7 * n + p] * e[q + 1 - ld]);
-      }
-      dp = fa[p] * dp + s * f1[p];
-      e[q] = dp;
-    }
-    T x = (T)0;
-    for (int j = ny - 1; j >= 0; --j) {
-      const int p = i + nx * j;
-      const int q = (i + 1) + ld * (j + 1);
-      x = e[q] - fc[p] * x;
-      e[q] = x;
-    }
+    ylines_fwd<T>(L, W + L.e_off, W + L.r_off, f1, fa, W + L.res_off, i);
+    ylines_bwd<T>(L, W + L.res_off, fc, W + L.e_off, i);
   }
 }
 

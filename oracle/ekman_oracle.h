@@ -8,24 +8,30 @@
  * cpu_baseline leg may load it.  The product never links it.
  *
  * Pinning (what checks each function; "parity unpinned" marks a function no
- * independent check covers):
- *   ora_fft_fwd / ora_fft_inv   numpy rfft/irfft with 1/NX scaling, 2/3 rule,
- *                               derivative of sin(kx) (tests/test_oracle_fft.py)
- *   ora_thomas_real(_sp)        dense numpy.linalg.solve (tests/test_oracle_thomas.py)
- *   ora_metrics                 closed forms on a rectilinear grid, the 1e-8
- *                               truncation, rundir_unst/GRID_YZ.dat fixture
- *                               (tests/test_oracle_metrics.py)
- *   ora_mudpack                 the reference MUDPACK itself (mud2cr.f+mudcom.f
- *                               compiled into oracle/_ref) at the iterate level,
- *                               plus a dense scipy solve of the assembled
- *                               9-point operator (tests/test_oracle_mudpack.py)
- *   ora_rk_duct                 laminar Ekman-spiral steady state, inertial
- *                               oscillation period, Stokes-layer decay, the
- *                               projection invariant J div u = MG residual
- *                               (tests/test_oracle_physics.py).  The composed
- *                               operation order of RK_DUCT (lagged periodic
- *                               rows, corner updates) has no golden vectors:
- *                               parity unpinned beyond these invariants.
+ * independent check covers; tests/test_oracle_cpu.py unless noted):
+ *   ora_fft_fwd / ora_fft_inv   numpy rfft/irfft with 1/NX scaling and the 2/3
+ *                               rule; spectral derivative of sin(m x)
+ *   ora_thomas_real(_sp)        dense numpy.linalg.solve (with the extra
+ *                               (K+1, K-1) entry of the _SP form)
+ *   metrics (ora_get_metric)    closed forms on uniform and stretched
+ *                               rectilinear grids, the 1e-8 truncation; the
+ *                               shipped rundir_unst/GRID_YZ.dat (fixture) is the
+ *                               tanh family the grids come from
+ *   ora_mudpack                 mud2cr itself (src/mud2cr.f + src/mudcom.f built
+ *                               into oracle/_ref by oracle/Makefile), bit for
+ *                               bit with equal cycle counts; convergence to the
+ *                               sparse direct solution of the assembled 9-point
+ *                               system; exactness on quadratics
+ *   ora_tight_solve / _operator the operator rows against the divergence of the
+ *                               projection's face gradient computed directly;
+ *                               the solve against a sparse direct solve
+ *   ora_rem_div_curv            uniform grid: J div u after a tight solve is the
+ *                               multigrid residual; proj_mode 1 removes the
+ *                               divergence to the solve tolerance
+ *   ora_rk_duct / ora_time_step the composed operation order of RK_DUCT (lagged
+ *                               rows, corner updates, dt timing) has no golden
+ *                               vectors: parity unpinned beyond the pieces above
+ *                               (the GPU path is checked against it)
  */
 #ifndef EKMAN_ORACLE_H
 #define EKMAN_ORACLE_H
