@@ -156,6 +156,142 @@ __global__ void __launch_bounds__(kBlock) k_fft_inv(double* __restrict__ v, long
   for (int k = lanep; k <= M; k += GW) dst[k] = (k < M) ? Z[k] : make_double2(0.0, 0.0);
 }
 
+// Fused nonlinear-flux stage of one transported field Q (RK_DUCT
+// src/duct.F90:683-1161 for U1..U3, :1169-1326 for theta): per (K,J)
+// pencil, one wavefront computes in registers
+//   Sx  = J U1 Q                                              (:685-700)
+//   Syz = -(U2b flux)_eta + nu (G12 Q_eta)_zeta
+//         - (U3b flux)_zeta + nu (G12 Q_zeta)_eta             (:721-761)
+// at the pencil's NX points, transforms both (forward, 1/NX, 2/3 rule) and
+// accumulates CF = (CF - i kx Sx) + Syz, CR = CR + TEMP5 CF (:714, :772-777,
+// :1127-1149), so the two flux fields never round-trip through HBM.  The
+// arithmetic of every value is that of the separate flux, transform and
+// accumulation kernels.
+#define MET(arr, k, j, f) M.arr[mf(d, k, j, f)]
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_flux_accum(Dims d, Metrics M, Ranges rg, const double* __restrict__ U1,
+                                                       const double* __restrict__ Q, const double* __restrict__ U2b,
+                                                       const double* __restrict__ U3b, double nu,
+                                                       const double* __restrict__ kxv, double2* __restrict__ CF,
+                                                       double2* __restrict__ CR, double temp5, int logm,
+                                                       const double2* __restrict__ tw,
+                                                       const double2* __restrict__ tw2) {
+  extern __shared__ double2 lds[];
+  const int nx = d.NX, M2 = nx >> 1, GW = M2 < 64 ? M2 : 64, PW = 64 / GW;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lanep = lane % GW, sub = lane / GW;
+  const int nk = rg.ze - rg.zs + 1, nj = rg.je - rg.js + 1;
+  const long npen = (long)nk * nj;
+  const long pen = ((long)blockIdx.x * kWaves + wave) * PW + sub;
+  const bool live = pen < npen;
+  const int k = rg.zs + (int)(live ? pen % nk : 0), j = rg.js + (int)(live ? pen / nk : 0);
+  const size_t dj = (size_t)d.NX2 * d.NZ2, dk = d.NX2;
+  const size_t q0 = (size_t)d.NX2 * ((size_t)k + (size_t)d.NZ2 * j);
+  const double Jc = M.jac[mi(d, k, j)];
+  const double g12kp = MET(g12, k + 1, j, 2), g12k = MET(g12, k, j, 2);
+  const double g12jp = MET(g12, k, j + 1, 1), g12j = MET(g12, k, j, 1);
+  double2 zx[R], zy[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    double sx[2] = {0.0, 0.0}, sy[2] = {0.0, 0.0};
+    if (live) {
+      const size_t q = q0 + 2 * (size_t)(lanep + GW * r);
+      const double2 qc = *reinterpret_cast<const double2*>(Q + q);
+      const double2 u1 = *reinterpret_cast<const double2*>(U1 + q);
+      const double2 qjp = *reinterpret_cast<const double2*>(Q + q + dj);
+      const double2 qjm = *reinterpret_cast<const double2*>(Q + q - dj);
+      const double2 qkp = *reinterpret_cast<const double2*>(Q + q + dk);
+      const double2 qkm = *reinterpret_cast<const double2*>(Q + q - dk);
+      const double2 qpp = *reinterpret_cast<const double2*>(Q + q + dk + dj);
+      const double2 qpm = *reinterpret_cast<const double2*>(Q + q + dk - dj);
+      const double2 qmp = *reinterpret_cast<const double2*>(Q + q - dk + dj);
+      const double2 qmm = *reinterpret_cast<const double2*>(Q + q - dk - dj);
+      const double2 vjp = *reinterpret_cast<const double2*>(U2b + q + dj);
+      const double2 vj = *reinterpret_cast<const double2*>(U2b + q);
+      const double2 wkp = *reinterpret_cast<const double2*>(U3b + q + dk);
+      const double2 wk = *reinterpret_cast<const double2*>(U3b + q);
+      const double QC[2] = {qc.x, qc.y}, UU[2] = {u1.x, u1.y}, QJP[2] = {qjp.x, qjp.y}, QJM[2] = {qjm.x, qjm.y};
+      const double QKP[2] = {qkp.x, qkp.y}, QKM[2] = {qkm.x, qkm.y}, QPP[2] = {qpp.x, qpp.y};
+      const double QPM[2] = {qpm.x, qpm.y}, QMP[2] = {qmp.x, qmp.y}, QMM[2] = {qmm.x, qmm.y};
+      const double VJP[2] = {vjp.x, vjp.y}, VJ[2] = {vj.x, vj.y}, WKP[2] = {wkp.x, wkp.y}, WK[2] = {wk.x, wk.y};
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        sx[h] = Jc * UU[h] * QC[h];
+        double s1 = 0.5 * VJP[h] * (QC[h] + QJP[h]) - 0.5 * VJ[h] * (QC[h] + QJM[h]);
+        double s2 = 0.5 * WKP[h] * (QC[h] + QKP[h]) - 0.5 * WK[h] * (QC[h] + QKM[h]);
+        s1 = -s1 + 0.25 * nu * (g12kp * (QPP[h] + QJP[h] - QJM[h] - QPM[h]) - g12k * (QMP[h] + QJP[h] - QJM[h] - QMM[h]));
+        s2 = -s2 + 0.25 * nu * (g12jp * (QPP[h] + QKP[h] - QMP[h] - QKM[h]) - g12j * (QKP[h] + QPM[h] - QKM[h] - QMM[h]));
+        sy[h] = s1 + s2;
+      }
+    }
+    zx[r] = make_double2(sx[0], sx[1]);
+    zy[r] = make_double2(sy[0], sy[1]);
+  }
+  dif<R>(zx, GW, M2, lanep, tw);
+  dif<R>(zy, GW, M2, lanep, tw);
+  double2* Zx = lds + (size_t)2 * (wave * PW + sub) * M2;
+  double2* Zy = Zx + M2;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int b = bitrev(lanep + GW * r, logm);
+    Zx[b] = zx[r];
+    Zy[b] = zy[r];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  if (!live) return;
+  const double sc = 1.0 / (double)nx;
+  const int nkx = nx / 3;
+  const size_t f0 = (size_t)d.NXC * ((size_t)k + (size_t)d.NZ2 * j);
+  for (int m = lanep; m <= M2; m += GW) {
+    double2 X[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+    if (m <= nkx) {
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const double2* Zt = t == 0 ? Zx : Zy;
+        const double2 zk = Zt[m == M2 ? 0 : m];
+        double2 y = Zt[(M2 - m) % M2];
+        y.y = -y.y;
+        const double2 e = make_double2(0.5 * (zk.x + y.x), 0.5 * (zk.y + y.y));
+        const double2 o = make_double2(0.5 * (zk.y - y.y), -0.5 * (zk.x - y.x));
+        const double2 wo = cmul(tw2[m], o);
+        X[t] = make_double2((e.x + wo.x) * sc, (e.y + wo.y) * sc);
+      }
+    }
+    double2 f = CF[f0 + m];
+    const double w = kxv[m];
+    f.x = f.x - (0.0 * X[0].x - w * X[0].y);
+    f.y = f.y - (0.0 * X[0].y + w * X[0].x);
+    f.x = f.x + X[1].x;
+    f.y = f.y + X[1].y;
+    CF[f0 + m] = f;
+    double2 c = CR[f0 + m];
+    c.x = c.x + temp5 * f.x;
+    c.y = c.y + temp5 * f.y;
+    CR[f0 + m] = c;
+  }
+}
+#undef MET
+
+template <int R>
+int launch_flux_accum_r(const FftPlan& p, const Dims& d, const Metrics& M, const Ranges& rg, const double* U1,
+                        const double* Q, const double* U2b, const double* U3b, double nu, const double* kx,
+                        double* CF, double* CR, double temp5, hipStream_t st) {
+  const int M2 = p.nx / 2, GW = M2 < 64 ? M2 : 64, PW = 64 / GW;
+  const long npen = (long)(rg.ze - rg.zs + 1) * (rg.je - rg.js + 1);
+  if (npen <= 0) return EKM_OK;
+  const long per_block = (long)kWaves * PW;
+  const long nb = (npen + per_block - 1) / per_block;
+  const size_t lds = (size_t)2 * kWaves * PW * M2 * sizeof(double2);
+  int logm = 0;
+  while ((1 << logm) < M2) ++logm;
+  hipLaunchKernelGGL(k_flux_accum<R>, dim3((unsigned)nb), dim3(kBlock), lds, st, d, M, rg, U1, Q, U2b, U3b, nu, kx,
+                     (double2*)CF, (double2*)CR, temp5, logm, (const double2*)p.tw, (const double2*)p.tw2);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+
 template <int R>
 int launch_pair(bool fwd, const FftPlan& p, double* v, long npencil, hipStream_t st) {
   const int M = p.nx / 2, GW = M < 64 ? M : 64, PW = 64 / GW;
@@ -214,3 +350,13 @@ void fft_plan_free(FftPlan& p) {
 int fft_forward(const FftPlan& p, double* v, long npencil, hipStream_t st) { return launch(true, p, v, npencil, st); }
 
 int fft_inverse(const FftPlan& p, double* v, long npencil, hipStream_t st) { return launch(false, p, v, npencil, st); }
+
+int launch_flux_accum(const FftPlan& p, const Dims& d, const Metrics& M, const Ranges& r, const double* U1,
+                      const double* Q, const double* U2b, const double* U3b, double nu, const double* kx, double* CF,
+                      double* CR, double temp5, hipStream_t st) {
+  const int M2 = p.nx / 2;
+  if (M2 <= 64) return launch_flux_accum_r<1>(p, d, M, r, U1, Q, U2b, U3b, nu, kx, CF, CR, temp5, st);
+  if (M2 == 128) return launch_flux_accum_r<2>(p, d, M, r, U1, Q, U2b, U3b, nu, kx, CF, CR, temp5, st);
+  if (M2 == 256) return launch_flux_accum_r<4>(p, d, M, r, U1, Q, U2b, U3b, nu, kx, CF, CR, temp5, st);
+  return launch_flux_accum_r<8>(p, d, M, r, U1, Q, U2b, U3b, nu, kx, CF, CR, temp5, st);
+}

@@ -21,11 +21,6 @@ struct RhsArgs {
 int launch_rhs_fourier(const Dims& d, const Metrics& M, const RhsArgs& a, hipStream_t st);
 int launch_contravariant(const Dims& d, const Metrics& M, const Ranges& r, const double* U2,
                          const double* U3, double* U2b, double* U3b, hipStream_t st);
-int launch_flux(const Dims& d, const Metrics& M, const Ranges& r, const double* U1, const double* Q,
-                const double* U2b, const double* U3b, double nu, double* Sx, double* Syz,
-                hipStream_t st);
-int launch_accum(const Dims& d, const Ranges& r, const double* kx, const double* Sx,
-                 const double* Syz, double* CF, double* CR, double temp5, hipStream_t st);
 // lu: scratch of 7 * NZ2*NY2 doubles
 int launch_adi(const Dims& d, const Metrics& M, const Ranges& r, const BcTable& by,
                const BcTable& bz, double t1, double* U, double* R, double* S, double* lu,

@@ -157,64 +157,6 @@ __global__ void k_contravariant(Dims d, Metrics M, Ranges r, const double* __res
 }
 
 // ---------------------------------------------------------------------------
-// nonlinear fluxes of one transported field Q (:683-1120, :1175-1237):
-// Sx = J U1 Q; Syz = -(U2b flux)_eta + nu (G12 Q_eta)_zeta - (U3b flux)_zeta + nu (G12 Q_zeta)_eta
-// ---------------------------------------------------------------------------
-__global__ void k_flux(Dims d, Metrics M, Ranges r, const double* __restrict__ U1,
-                       const double* __restrict__ Q, const double* __restrict__ U2b,
-                       const double* __restrict__ U3b, double nu, double* __restrict__ Sx,
-                       double* __restrict__ Syz) {
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= d.npts) return;
-  int i = (int)(t % d.NX2);
-  size_t rr = t / d.NX2;
-  int k = (int)(rr % d.NZ2), j = (int)(rr / d.NZ2);
-  if (!(in(j, r.js, r.je) && in(k, r.zs, r.ze))) {
-    Sx[t] = 0.0;
-    Syz[t] = 0.0;
-    return;
-  }
-  (void)i;
-  size_t q = t, dj = (size_t)d.NX2 * d.NZ2, dk = d.NX2;
-  double Jc = JAC(k, j);
-  double qc = Q[q];
-  Sx[q] = Jc * U1[q] * qc;
-  double s1 = 0.5 * U2b[q + dj] * (qc + Q[q + dj]) - 0.5 * U2b[q] * (qc + Q[q - dj]);
-  double s2 = 0.5 * U3b[q + dk] * (qc + Q[q + dk]) - 0.5 * U3b[q] * (qc + Q[q - dk]);
-  double qpp = Q[q + dk + dj], qp0 = Q[q + dk], qpm = Q[q + dk - dj];
-  double q0p = Q[q + dj], q0m = Q[q - dj];
-  double qmp = Q[q - dk + dj], qm0 = Q[q - dk], qmm = Q[q - dk - dj];
-  s1 = -s1 + 0.25 * nu * (G12(k + 1, j, 2) * (qpp + q0p - q0m - qpm) - G12(k, j, 2) * (qmp + q0p - q0m - qmm));
-  s2 = -s2 + 0.25 * nu * (G12(k, j + 1, 1) * (qpp + qp0 - qmp - qm0) - G12(k, j, 1) * (qp0 + qpm - qm0 - qmm));
-  Syz[q] = s1 + s2;
-}
-
-// CF = (CF - i kx Sx) + Syz; CR = CR + TEMP5 CF  (:714, :772-777, :1127-1149)
-__global__ void k_accum(Dims d, Ranges r, const double* __restrict__ kx,
-                        const double2* __restrict__ Sx, const double2* __restrict__ Syz,
-                        double2* __restrict__ CF, double2* __restrict__ CR, double temp5) {
-  int nk = r.ze - r.zs + 1, nj = r.je - r.js + 1;
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t n = (size_t)d.NXC * nk * nj;
-  if (t >= n) return;
-  int m = (int)(t % d.NXC);
-  size_t rr = t / d.NXC;
-  int k = r.zs + (int)(rr % nk), j = r.js + (int)(rr / nk);
-  size_t q = fo(d, m, k, j);
-  double2 f = CF[q], s = Sx[q], y = Syz[q];
-  double w = kx[m];
-  f.x = f.x - (0.0 * s.x - w * s.y);
-  f.y = f.y - (0.0 * s.y + w * s.x);
-  f.x = f.x + y.x;
-  f.y = f.y + y.y;
-  CF[q] = f;
-  double2 c = CR[q];
-  c.x = c.x + temp5 * f.x;
-  c.y = c.y + temp5 * f.y;
-  CR[q] = c;
-}
-
-// ---------------------------------------------------------------------------
 // ADI: explicit half before the y sweep (:1334-1371, :1640-1665, ...)
 // S = R - J U;  R = S/2 + J U + t1 [G11(K+1)(U(K+1)-U) - G11(K)(U-U(K-1))]
 // ---------------------------------------------------------------------------
@@ -664,21 +606,6 @@ int launch_contravariant(const Dims& d, const Metrics& M, const Ranges& r, const
                          const double* U3, double* U2b, double* U3b, hipStream_t st) {
   size_t n = (size_t)d.NX2 * (r.ze - r.zs + 2) * (r.je - r.js + 2);
   hipLaunchKernelGGL(k_contravariant, dim3(nblk(n, 256)), dim3(256), 0, st, d, M, r, U2, U3, U2b, U3b);
-  EKM_CHECK(hipGetLastError());
-  return EKM_OK;
-}
-int launch_flux(const Dims& d, const Metrics& M, const Ranges& r, const double* U1, const double* Q,
-                const double* U2b, const double* U3b, double nu, double* Sx, double* Syz,
-                hipStream_t st) {
-  hipLaunchKernelGGL(k_flux, dim3(nblk(d.npts, 256)), dim3(256), 0, st, d, M, r, U1, Q, U2b, U3b, nu, Sx, Syz);
-  EKM_CHECK(hipGetLastError());
-  return EKM_OK;
-}
-int launch_accum(const Dims& d, const Ranges& r, const double* kx, const double* Sx,
-                 const double* Syz, double* CF, double* CR, double temp5, hipStream_t st) {
-  size_t n = (size_t)d.NXC * (r.ze - r.zs + 1) * (r.je - r.js + 1);
-  hipLaunchKernelGGL(k_accum, dim3(nblk(n, 256)), dim3(256), 0, st, d, r, kx, (const double2*)Sx,
-                     (const double2*)Syz, (double2*)CF, (double2*)CR, temp5);
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }

@@ -423,17 +423,13 @@ extern "C" int ekm_rk_duct(ekm_solver* s) {
   // momentum fluxes (:683-1161)
   TRY(launch_contravariant(d, s->M, s->rv, s->U[1], s->U[2], s->U2b, s->U3b, st));
   for (int f = 0; f < 3; ++f) {
-    TRY(launch_flux(d, s->M, s->rv, s->U[0], s->U[f], s->U2b, s->U3b, p.nu, s->S1, s->S2, st));
-    TRY(fwd(s, s->S1));
-    TRY(fwd(s, s->S2));
-    TRY(launch_accum(d, s->rv, s->kx, s->S1, s->S2, s->CF[f], s->R[f], TEMP5, st));
+    TRY(launch_flux_accum(s->fft, d, s->M, s->rv, s->U[0], s->U[f], s->U2b, s->U3b, p.nu, s->kx, s->CF[f],
+                          s->R[f], TEMP5, st));
   }
   for (int f = 0; f < 3; ++f) TRY(inv(s, s->R[f]));
   if (th) {  // (:1169-1625)
-    TRY(launch_flux(d, s->M, s->rt, s->U[0], s->TH, s->U2b, s->U3b, p.nu / p.pr, s->S1, s->S2, st));
-    TRY(fwd(s, s->S1));
-    TRY(fwd(s, s->S2));
-    TRY(launch_accum(d, s->rt, s->kx, s->S1, s->S2, s->CFTH, s->RTH, TEMP5, st));
+    TRY(launch_flux_accum(s->fft, d, s->M, s->rt, s->U[0], s->TH, s->U2b, s->U3b, p.nu / p.pr, s->kx, s->CFTH,
+                          s->RTH, TEMP5, st));
     TRY(inv(s, s->RTH));
     int corners = (p.th_bc[EKM_ZMAX] == 1 ? 1 : 0) | (p.th_bc[EKM_ZMIN] == 6 ? 2 : 0);
     TRY(launch_adi(d, s->M, s->rt, s->bcy[3], s->bcz[3], TEMP1 / p.pr, s->TH, s->RTH, s->S1, s->lu, corners, st));
