@@ -9,7 +9,8 @@ per-kx pressure projection.  Metric: grid-point-steps per second
 
 The headline form (`value`) is the consistent projection with FP32
 alternating-line V(1,1) cycles inside FP64 residual iterations
-(proj_mode 2, max|residual| <= 1e-12 per real solve); `forms` also times
+(proj_mode 2, max|residual| <= 1e-12 per real solve, each solve started
+from the previous substep's correction); `forms` also times
 the same projection with FP64 cycles (proj_mode 1) and the plain MUDPACK
 replica (proj_mode 0, the reference's W(2,1) cycles to relmax 1e-6, which
 does not reach the divergence bound).  The divergence columns are the
@@ -55,6 +56,7 @@ def run_form(cfg, xp, yp, ic, mode: int, steps: int, warmup: int, tol: float):
     p.proj_mode = mode
     p.proj_tol = tol
     p.proj_maxit = 50
+    p.proj_guess = 1 if mode > 0 else 0
     s = S.DuctSolver(p, xp, yp)
     s.set_initial_physical(*ic)
     for _ in range(warmup):

@@ -136,8 +136,9 @@ struct TightPlan {
 int tight_plan_build(TightPlan& tp, const ekm_params& p, const Dims& d, const HostMetrics& hm, const Ranges& rv,
                      const double* kx2_mode, int nmode, int nsolve_max);
 void tight_plan_free(TightPlan& tp);
-// solves nsolve planes of mg.rhs (RHS_TMP) into mg.phi; V-cycle counts in mg.cycles
-int tight_solve_ex(TightPlan& tp, MgPlan& mg, const Dims& d, int nsolve, hipStream_t st);
+// solves nsolve planes of mg.rhs (RHS_TMP) into mg.phi; V-cycle counts in
+// mg.cycles.  guess != 0: start from guess * (mg.phi on entry), else from zero.
+int tight_solve_ex(TightPlan& tp, MgPlan& mg, const Dims& d, int nsolve, double guess, hipStream_t st);
 
 // ---- solver -------------------------------------------------------------------
 struct ekm_solver_impl;

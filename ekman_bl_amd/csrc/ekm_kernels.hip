@@ -182,21 +182,23 @@ __global__ void k_adi_pre(Dims d, Metrics M, Ranges r, const double* __restrict_
 // Output per (line, row): mult (the reference's modified A), piv (modified B),
 // up (C).  Rows that neither the interior loop nor a BC routine sets keep the
 // [0 1 0 | 0] the reference initialises them to (:1399-1406, :1690-1697).
-__global__ void k_lu(Dims d, Metrics M, Ranges r, BcTable bc, double t1, int dir,
-                     double* __restrict__ mult, double* __restrict__ piv,
-                     double* __restrict__ up, double* __restrict__ spa) {
-  int line0 = dir == 0 ? r.zs : r.js, line1 = dir == 0 ? r.ze : r.je;
-  int L = line0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (L > line1) return;
-  int N = dir == 0 ? d.NY + 1 : d.NZ + 1;  // last row index
-  int lo = dir == 0 ? r.js : r.zs, hi = dir == 0 ? r.je : r.ze;
-  size_t base = (size_t)L * (N + 1);
-  double bprev = 0.0, cprev = 0.0, pvprev = 1.0;
-  double a_nz = 0.0;
-  for (int row = 0; row <= N; ++row) {
+// One workgroup per line: the rows' coefficients are formed in parallel into
+// LDS, one lane runs the elimination recurrence out of LDS, and the factors
+// go back to memory with contiguous stores.
+constexpr int kLuBlock = 256;
+constexpr int kLuMaxRows = 1040;
+__global__ void __launch_bounds__(kLuBlock) k_lu(Dims d, Metrics M, Ranges r, BcTable bc, double t1, int dir,
+                                                  double* __restrict__ mult, double* __restrict__ piv,
+                                                  double* __restrict__ up, double* __restrict__ spa) {
+  __shared__ double sa[kLuMaxRows], sb[kLuMaxRows], sc[kLuMaxRows];
+  const int line0 = dir == 0 ? r.zs : r.js;
+  const int L = line0 + (int)blockIdx.x;
+  const int N = dir == 0 ? d.NY + 1 : d.NZ + 1;  // last row index
+  const int lo = dir == 0 ? r.js : r.zs, hi = dir == 0 ? r.je : r.ze;
+  for (int row = threadIdx.x; row <= N; row += blockDim.x) {
     double a = 0.0, b = 1.0, c = 0.0;
     if (row >= lo && row <= hi) {
-      int k = dir == 0 ? L : row, j = dir == 0 ? row : L;
+      const int k = dir == 0 ? L : row, j = dir == 0 ? row : L;
       if (dir == 0) {
         a = -(t1 * G22(k, j, 1));
         b = JAC(k, j) + t1 * (G22(k, j + 1, 1) + G22(k, j, 1));
@@ -213,26 +215,41 @@ __global__ void k_lu(Dims d, Metrics M, Ranges r, BcTable bc, double t1, int dir
         b = bc.r[e].b;
         c = bc.r[e].c;
       }
-    if (dir == 1 && bc.sp && row == N) {
+    sa[row] = a;
+    sb[row] = b;
+    sc[row] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (dir == 1 && bc.sp) {
       // THOMAS_REAL_SP (src/boundary.F90:17-24): row NZ+1 -= row NZ * D/A(NZ)
-      double D = bc.sp_metric ? G11(d.NZ, L, 2) : 1.0;
-      a = a - bprev * D / a_nz;
-      b = b - cprev * D / a_nz;
+      const double D = bc.sp_metric ? G11(d.NZ, L, 2) : 1.0;
+      const double a_nz = sa[N - 1];
+      sa[N] = sa[N] - sb[N - 1] * D / a_nz;
+      sb[N] = sb[N] - sc[N - 1] * D / a_nz;
       spa[2 * L] = D;
       spa[2 * L + 1] = a_nz;
     }
-    if (row == N - 1) a_nz = a;
-    double mlt = 0.0, pv = b;
-    if (row > 0) {
-      mlt = -a / pvprev;
-      pv = b + mlt * cprev;
+    double cprev = 0.0, pvprev = 1.0;
+    for (int row = 0; row <= N; ++row) {
+      const double a = sa[row], b = sb[row], c = sc[row];
+      double mlt = 0.0, pv = b;
+      if (row > 0) {
+        mlt = -a / pvprev;
+        pv = b + mlt * cprev;
+      }
+      sa[row] = mlt;
+      sb[row] = pv;
+      cprev = c;
+      pvprev = pv;
     }
-    mult[base + row] = mlt;
-    piv[base + row] = pv;
-    up[base + row] = c;
-    bprev = b;
-    cprev = c;
-    pvprev = pv;
+  }
+  __syncthreads();
+  const size_t base = (size_t)L * (N + 1);
+  for (int row = threadIdx.x; row <= N; row += blockDim.x) {
+    mult[base + row] = sa[row];
+    piv[base + row] = sb[row];
+    up[base + row] = sc[row];
   }
 }
 
@@ -249,6 +266,11 @@ __device__ inline double bc_rhs(const Dims& d, const Metrics& M, const BcRow& e,
 // y sweep for lines (i,K): solves in place into U, then emits the z-sweep
 // right-hand side R = S/2 + J U* + t1 [G22(J+1)(U*(J+1)-U*) - G22(J)(U*-U*(J-1))]
 // (:1460-1478 for theta, :1744-1757 for U1, ...).
+// The sweeps below run each line's recurrence in blocks of kSw rows: the
+// right-hand sides and factors of a block are loaded before its recurrence
+// steps, so the memory latency is paid once per block instead of per row.
+constexpr int kSw = 8;
+
 __global__ void k_ysweep(Dims d, Metrics M, Ranges r, BcTable bc, double t1,
                          const double* __restrict__ mult, const double* __restrict__ piv,
                          const double* __restrict__ up, double* __restrict__ U,
@@ -264,25 +286,54 @@ __global__ void k_ysweep(Dims d, Metrics M, Ranges r, BcTable bc, double t1,
   double lag[4];
   for (int e = 0; e < 4; ++e) lag[e] = (bc.r[e].kind == BC_LAGGED) ? U[q0 + dj * bc.r[e].src] : 0.0;
   double g = 0.0;
-  for (int row = 0; row <= N; ++row) {
-    double rhs = (row >= r.js && row <= r.je) ? R[q0 + dj * row] : 0.0;
-    for (int e = 0; e < 4; ++e)
-      if (bc.r[e].row == row) rhs = bc_rhs(d, M, bc.r[e], K, lag[e]);
-    g = (row == 0) ? rhs : rhs + mult[base + row] * g;
-    U[q0 + dj * row] = g;
+  for (int row0 = 0; row0 <= N; row0 += kSw) {
+    double rr[kSw], mm[kSw];
+#pragma unroll
+    for (int u = 0; u < kSw; ++u) {
+      const int row = row0 + u;
+      rr[u] = (row <= N && row >= r.js && row <= r.je) ? R[q0 + dj * row] : 0.0;
+      mm[u] = row <= N ? mult[base + row] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSw; ++u) {
+      const int row = row0 + u;
+      if (row > N) break;
+      double rhs = rr[u];
+      for (int e = 0; e < 4; ++e)
+        if (bc.r[e].row == row) rhs = bc_rhs(d, M, bc.r[e], K, lag[e]);
+      g = (row == 0) ? rhs : rhs + mm[u] * g;
+      U[q0 + dj * row] = g;
+    }
   }
   double x2 = 0.0, x1 = g / piv[base + N];  // x1 = x(row+1), x2 = x(row+2)
   U[q0 + dj * N] = x1;
-  for (int row = N - 1; row >= 0; --row) {
-    double x0 = (U[q0 + dj * row] - up[base + row] * x1) / piv[base + row];
-    U[q0 + dj * row] = x0;
-    int jr = row + 1;  // R at row+1 now has x(row), x(row+1), x(row+2)
-    if (jr >= r.js && jr <= r.je) {
-      size_t q = q0 + dj * jr;
-      R[q] = 0.5 * S[q] + JAC(K, jr) * x1 + t1 * (G22(K, jr + 1, 1) * (x2 - x1) - G22(K, jr, 1) * (x1 - x0));
+  for (int row0 = N - 1; row0 >= 0; row0 -= kSw) {
+    double gg[kSw], uu[kSw], pp[kSw], ss[kSw], jc[kSw], gp[kSw], g0[kSw];
+#pragma unroll
+    for (int u = 0; u < kSw; ++u) {
+      const int row = row0 - u, jr = row + 1;
+      const bool ok = row >= 0;
+      const bool in = ok && jr >= r.js && jr <= r.je;
+      gg[u] = ok ? U[q0 + dj * row] : 0.0;
+      uu[u] = ok ? up[base + row] : 0.0;
+      pp[u] = ok ? piv[base + row] : 1.0;
+      ss[u] = in ? S[q0 + dj * jr] : 0.0;
+      jc[u] = in ? JAC(K, jr) : 0.0;
+      gp[u] = in ? G22(K, jr + 1, 1) : 0.0;
+      g0[u] = in ? G22(K, jr, 1) : 0.0;
     }
-    x2 = x1;
-    x1 = x0;
+#pragma unroll
+    for (int u = 0; u < kSw; ++u) {
+      const int row = row0 - u;
+      if (row < 0) break;
+      const double x0 = (gg[u] - uu[u] * x1) / pp[u];
+      U[q0 + dj * row] = x0;
+      const int jr = row + 1;  // R at row+1 now has x(row), x(row+1), x(row+2)
+      if (jr >= r.js && jr <= r.je)
+        R[q0 + dj * jr] = 0.5 * ss[u] + jc[u] * x1 + t1 * (gp[u] * (x2 - x1) - g0[u] * (x1 - x0));
+      x2 = x1;
+      x1 = x0;
+    }
   }
 }
 
@@ -302,21 +353,47 @@ __global__ void k_zsweep(Dims d, Metrics M, Ranges r, BcTable bc, const double* 
   double lag[4];
   for (int e = 0; e < 4; ++e) lag[e] = (bc.r[e].kind == BC_LAGGED) ? U[q0 + dk * bc.r[e].src] : 0.0;
   double g = 0.0, rprev = 0.0;
-  for (int row = 0; row <= N; ++row) {
-    double rhs = (row >= r.zs && row <= r.ze) ? R[q0 + dk * row] : 0.0;
-    for (int e = 0; e < 4; ++e)
-      if (bc.r[e].row == row) rhs = bc_rhs(d, M, bc.r[e], J, lag[e]);
-    if (bc.sp && row == N) rhs = rhs - rprev * spa[2 * J] / spa[2 * J + 1];
-    g = (row == 0) ? rhs : rhs + mult[base + row] * g;
-    U[q0 + dk * row] = g;
-    rprev = rhs;
+  for (int row0 = 0; row0 <= N; row0 += kSw) {
+    double rr[kSw], mm[kSw];
+#pragma unroll
+    for (int u = 0; u < kSw; ++u) {
+      const int row = row0 + u;
+      rr[u] = (row <= N && row >= r.zs && row <= r.ze) ? R[q0 + dk * row] : 0.0;
+      mm[u] = row <= N ? mult[base + row] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSw; ++u) {
+      const int row = row0 + u;
+      if (row > N) break;
+      double rhs = rr[u];
+      for (int e = 0; e < 4; ++e)
+        if (bc.r[e].row == row) rhs = bc_rhs(d, M, bc.r[e], J, lag[e]);
+      if (bc.sp && row == N) rhs = rhs - rprev * spa[2 * J] / spa[2 * J + 1];
+      g = (row == 0) ? rhs : rhs + mm[u] * g;
+      U[q0 + dk * row] = g;
+      rprev = rhs;
+    }
   }
   double x1 = g / piv[base + N];
   U[q0 + dk * N] = x1;
-  for (int row = N - 1; row >= 0; --row) {
-    double x0 = (U[q0 + dk * row] - up[base + row] * x1) / piv[base + row];
-    U[q0 + dk * row] = x0;
-    x1 = x0;
+  for (int row0 = N - 1; row0 >= 0; row0 -= kSw) {
+    double gg[kSw], uu[kSw], pp[kSw];
+#pragma unroll
+    for (int u = 0; u < kSw; ++u) {
+      const int row = row0 - u;
+      const bool ok = row >= 0;
+      gg[u] = ok ? U[q0 + dk * row] : 0.0;
+      uu[u] = ok ? up[base + row] : 0.0;
+      pp[u] = ok ? piv[base + row] : 1.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSw; ++u) {
+      const int row = row0 - u;
+      if (row < 0) break;
+      const double x0 = (gg[u] - uu[u] * x1) / pp[u];
+      U[q0 + dk * row] = x0;
+      x1 = x0;
+    }
   }
 }
 
@@ -618,9 +695,10 @@ int launch_adi(const Dims& d, const Metrics& M, const Ranges& r, const BcTable& 
   size_t ly = (size_t)d.NZ2 * d.NY2;  // (line, row) entries
   double *my = lu, *py = lu + ly, *uy = lu + 2 * ly;
   double *mz = lu + 3 * ly, *pz = lu + 4 * ly, *uz = lu + 5 * ly, *spa = lu + 6 * ly;
-  hipLaunchKernelGGL(k_lu, dim3(nblk(r.ze - r.zs + 1, 64)), dim3(64), 0, st, d, M, r, by, t1, 0, my, py, uy, spa);
+  if (d.NY + 2 > kLuMaxRows || d.NZ + 2 > kLuMaxRows) return EKM_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(k_lu, dim3(r.ze - r.zs + 1), dim3(kLuBlock), 0, st, d, M, r, by, t1, 0, my, py, uy, spa);
   EKM_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_lu, dim3(nblk(r.je - r.js + 1, 64)), dim3(64), 0, st, d, M, r, bz, t1, 1, mz, pz, uz, spa);
+  hipLaunchKernelGGL(k_lu, dim3(r.je - r.js + 1), dim3(kLuBlock), 0, st, d, M, r, bz, t1, 1, mz, pz, uz, spa);
   EKM_CHECK(hipGetLastError());
   size_t ny_ = (size_t)d.NX2 * (r.ze - r.zs + 1);
   hipLaunchKernelGGL(k_ysweep, dim3(nblk(ny_, 256)), dim3(256), 0, st, d, M, r, by, t1, my, py, uy, U, R,

@@ -89,39 +89,54 @@ __device__ inline T shfl_down(T v, int d) {
   return __shfl_down(v, d, 64);
 }
 
-// z lines (rows j of parity par) of level L: one wavefront per row.
-template <typename T, int CM>
-__device__ void zlines(const TDesc<T>& g, const TLev<T>& L, T* W, const T* fac, int par) {
+// Zebra line relaxation of level L: lines of parity par in direction DIR
+// (0: z lines, rows j, contiguous; 1: y lines, columns i, stride nx+2).  One
+// wavefront per line: lane l owns points l*CM .. l*CM+CM-1 of the line, loads
+// all their operands at once, and the forward and back substitutions of the
+// line's Thomas factors are composed as affine maps with a 64-lane scan, so a
+// line costs one round of memory latency whatever its length.
+template <typename T, int CM, int DIR>
+__device__ void lines(const TLev<T>& L, T* __restrict__ W, const T* __restrict__ fac, int par) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int nx = L.nx, ny = L.ny, ld = nx + 2, n = L.n;
-  T* e = W + L.e_off;
-  const T* r = W + L.r_off;
-  const T* f1 = fac + L.f_off;
+  const int len = DIR == 0 ? nx : ny, nlines = DIR == 0 ? ny : nx;
+  T* __restrict__ e = W + L.e_off;
+  const T* __restrict__ r = W + L.r_off;
+  const T* __restrict__ cf = L.cf;
+  const unsigned char* __restrict__ spec = L.spec;
+  const T* f1 = fac + L.f_off + (DIR == 0 ? 0 : 3 * n);
   const T* fa = f1 + n;
   const T* fc = fa + n;
-  const int i0 = lane * CM;
-  for (int j = par + 2 * wave; j < ny; j += 2 * nw) {
-    T bk[CM], ak[CM];
+  const int k0 = lane * CM;
+  for (int line = par + 2 * wave; line < nlines; line += 2 * nw) {
+    T bk[CM], ak[CM], ck[CM];
     Aff<T> loc{(T)1, (T)0};
 #pragma unroll
-    for (int k = 0; k < CM; ++k) {
-      const int i = i0 + k;
-      ak[k] = (T)1;
-      bk[k] = (T)0;
-      if (i < nx) {
+    for (int c = 0; c < CM; ++c) {
+      const int k = k0 + c;
+      ak[c] = (T)1;
+      bk[c] = (T)0;
+      ck[c] = (T)0;
+      if (k < len) {
+        const int i = DIR == 0 ? k : line, j = DIR == 0 ? line : k;
         const int p = i + nx * j;
         const int q = (i + 1) + ld * (j + 1);
         T s = (T)0;
-        if (!L.spec[p]) {
-          s = r[p] - (L.cf[1 * n + p] * e[q + 1 + ld] + L.cf[2 * n + p] * e[q + ld] + L.cf[3 * n + p] * e[q - 1 + ld] +
-                      L.cf[5 * n + p] * e[q - 1 - ld] + L.cf[6 * n + p] * e[q - ld] + L.cf[7 * n + p] * e[q + 1 - ld]);
+        if (!spec[p]) {
+          if (DIR == 0)  // off-line neighbours: rows j +- 1
+            s = r[p] - (cf[1 * n + p] * e[q + 1 + ld] + cf[2 * n + p] * e[q + ld] + cf[3 * n + p] * e[q - 1 + ld] +
+                        cf[5 * n + p] * e[q - 1 - ld] + cf[6 * n + p] * e[q - ld] + cf[7 * n + p] * e[q + 1 - ld]);
+          else  // columns i +- 1
+            s = r[p] - (cf[0 * n + p] * e[q + 1] + cf[1 * n + p] * e[q + 1 + ld] + cf[3 * n + p] * e[q - 1 + ld] +
+                        cf[4 * n + p] * e[q - 1] + cf[5 * n + p] * e[q - 1 - ld] + cf[7 * n + p] * e[q + 1 - ld]);
         }
-        ak[k] = fa[p];
-        bk[k] = s * f1[p];
-        loc = compose(Aff<T>{ak[k], bk[k]}, loc);
+        ak[c] = fa[p];
+        bk[c] = s * f1[p];
+        ck[c] = fc[p];
+        loc = compose(Aff<T>{ak[c], bk[c]}, loc);
       }
     }
-    // inclusive scan of the forward maps over lanes
+    // forward: dp_k = a_k dp_{k-1} + b_k, inclusive scan over lanes
     Aff<T> inc = loc;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -132,22 +147,15 @@ __device__ void zlines(const TDesc<T>& g, const TLev<T>& L, T* W, const T* fac, 
     if (lane == 0) dp = (T)0;
     T dk[CM];
 #pragma unroll
-    for (int k = 0; k < CM; ++k) {
-      dp = ak[k] * dp + bk[k];
-      dk[k] = dp;
+    for (int c = 0; c < CM; ++c) {
+      dp = ak[c] * dp + bk[c];
+      dk[c] = dp;
     }
-    // backward: x_i = dp_i - cp_i x_{i+1}, composed from the right
+    // backward: x_k = dp_k - cp_k x_{k+1}, composed from the right
     Aff<T> locb{(T)1, (T)0};
-    T ck[CM];
 #pragma unroll
-    for (int k = CM - 1; k >= 0; --k) {
-      const int i = i0 + k;
-      ck[k] = (T)0;
-      if (i < nx) {
-        ck[k] = fc[i + nx * j];
-        locb = compose(Aff<T>{-ck[k], dk[k]}, locb);
-      }
-    }
+    for (int c = CM - 1; c >= 0; --c)
+      if (k0 + c < len) locb = compose(Aff<T>{-ck[c], dk[c]}, locb);
     Aff<T> incb = locb;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -157,77 +165,26 @@ __device__ void zlines(const TDesc<T>& g, const TLev<T>& L, T* W, const T* fac, 
     T x = shfl_down(incb.b, 1);
     if (lane == 63) x = (T)0;
 #pragma unroll
-    for (int k = CM - 1; k >= 0; --k) {
-      const int i = i0 + k;
-      if (i < nx) {
-        x = dk[k] - ck[k] * x;
+    for (int c = CM - 1; c >= 0; --c) {
+      const int k = k0 + c;
+      if (k < len) {
+        x = dk[c] - ck[c] * x;
+        const int i = DIR == 0 ? k : line, j = DIR == 0 ? line : k;
         e[(i + 1) + ld * (j + 1)] = x;
       }
     }
   }
 }
 
-// y lines (columns i of parity par) of level L: lanes over columns.  The
-// forward substitution reads the correction of the neighbouring columns and
-// writes its intermediate into the level's residual array (free while
-// smoothing), so its loads carry no dependence on its stores and can run
-// ahead; the back substitution then writes the column.
-template <typename T>
-__device__ __forceinline__ void ylines_fwd(const TLev<T>& L, const T* __restrict__ e, const T* __restrict__ r,
-                                           const T* __restrict__ f1, const T* __restrict__ fa,
-                                           T* __restrict__ tmp, int i) {
-  const int nx = L.nx, ny = L.ny, ld = nx + 2, n = L.n;
-  const T* __restrict__ cf = L.cf;
-  const unsigned char* __restrict__ spec = L.spec;
-  T dp = (T)0;
-#pragma unroll 4
-  for (int j = 0; j < ny; ++j) {
-    const int p = i + nx * j;
-    const int q = (i + 1) + ld * (j + 1);
-    T s = (T)0;
-    if (!spec[p]) {
-      s = r[p] - (cf[0 * n + p] * e[q + 1] + cf[1 * n + p] * e[q + 1 + ld] + cf[3 * n + p] * e[q - 1 + ld] +
-                  cf[4 * n + p] * e[q - 1] + cf[5 * n + p] * e[q - 1 - ld] + cf[7 * n + p] * e[q + 1 - ld]);
-    }
-    dp = fa[p] * dp + s * f1[p];
-    tmp[p] = dp;
-  }
-}
-
-template <typename T>
-__device__ __forceinline__ void ylines_bwd(const TLev<T>& L, const T* __restrict__ tmp, const T* __restrict__ fc,
-                                           T* __restrict__ e, int i) {
-  const int nx = L.nx, ny = L.ny, ld = nx + 2;
-  T x = (T)0;
-#pragma unroll 4
-  for (int j = ny - 1; j >= 0; --j) {
-    const int p = i + nx * j;
-    x = tmp[p] - fc[p] * x;
-    e[(i + 1) + ld * (j + 1)] = x;
-  }
-}
-
-template <typename T>
-__device__ void ylines(const TDesc<T>& g, const TLev<T>& L, T* W, const T* fac, int par) {
-  const int nx = L.nx, n = L.n;
-  const T* f1 = fac + L.f_off + 3 * n;
-  const T* fa = f1 + n;
-  const T* fc = fa + n;
-  for (int i = par + 2 * (int)threadIdx.x; i < nx; i += 2 * (int)blockDim.x) {
-    ylines_fwd<T>(L, W + L.e_off, W + L.r_off, f1, fa, W + L.res_off, i);
-    ylines_bwd<T>(L, W + L.res_off, fc, W + L.e_off, i);
-  }
-}
-
 template <typename T, int CM>
 __device__ void smooth(const TDesc<T>& g, const TLev<T>& L, T* W, const T* fac) {
-  zlines<T, CM>(g, L, W, fac, 0);
+  lines<T, CM, 0>(L, W, fac, 0);
   __syncthreads();
-  zlines<T, CM>(g, L, W, fac, 1);
+  lines<T, CM, 0>(L, W, fac, 1);
   __syncthreads();
-  ylines<T>(g, L, W, fac, 0);
+  lines<T, CM, 1>(L, W, fac, 0);
   __syncthreads();
-  ylines<T>(g, L, W, fac, 1);
+  lines<T, CM, 1>(L, W, fac, 1);
   __syncthreads();
 }
 
@@ -300,12 +257,12 @@ __device__ void prolong_add(const TLev<T>& C, const TLev<T>& F, T* W) {
   __syncthreads();
 }
 
-template <typename T, int CM>
-__global__ void __launch_bounds__(512) k_tight_solve(const TDesc<T>* __restrict__ gp, T* __restrict__ ws,
+template <typename T, int CM, int NT>
+__global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__ gp, T* __restrict__ ws,
                                                      double* __restrict__ ws64, const double* __restrict__ rhs_tmp,
                                                      double* __restrict__ phi_out, const double* __restrict__ kx2v,
-                                                     int* __restrict__ cycles, size_t plane) {
-  __shared__ double red[512];
+                                                     int* __restrict__ cycles, size_t plane, double guess) {
+  __shared__ double red[NT];
   const TDesc<T>& g = *gp;
   const int s = blockIdx.x;
   const double kx2 = kx2v[s];
@@ -317,7 +274,15 @@ __global__ void __launch_bounds__(512) k_tight_solve(const TDesc<T>* __restrict_
   double* b = phi + (size_t)(nx + 2) * (ny + 2);
   const T* fac = g.fac + (size_t)(s >> 1) * g.fstride;
   const double* rt = rhs_tmp + (size_t)s * plane;
-  for (int t = threadIdx.x; t < (nx + 2) * (ny + 2); t += blockDim.x) phi[t] = 0.0;
+  // phi starts from zero, or from the previous solution of this solve scaled
+  // by `guess` (phi_out still holds it)
+  const double* pg = phi_out + (size_t)s * plane;
+  for (int t = threadIdx.x; t < (nx + 2) * (ny + 2); t += blockDim.x) {
+    const int jj = t / ld - 1, ii = t - (jj + 1) * ld - 1;
+    double v = 0.0;
+    if (guess != 0.0 && ii >= 0 && ii < nx && jj >= 0 && jj < ny && !F.spec[ii + nx * jj]) v = guess * pg[ii + nx * jj];
+    phi[t] = v;
+  }
   for (int p = threadIdx.x; p < n; p += blockDim.x) b[p] = F.spec[p] ? 0.0 : -rt[p];
   __syncthreads();
   int it = 0;
@@ -570,7 +535,7 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
 int tight_plan_build(TightPlan& tp, const ekm_params& p, const Dims& d, const HostMetrics& hm, const Ranges& rv,
                      const double* kx2_mode, int nmode, int nsolve_max) {
   tp.fp32 = p.proj_mode == 2;
-  const int nf = d.NZ2;
+  const int nf = d.NZ2 > d.NY2 ? d.NZ2 : d.NY2;  // longest line of the finest level
   tp.cm = nf <= 320 ? 5 : nf <= 576 ? 9 : nf <= 832 ? 13 : 0;
   if (!tp.cm) return EKM_ERR_UNSUPPORTED;
   tp.nsolve_max = nsolve_max;
@@ -591,13 +556,16 @@ void tight_plan_free(TightPlan& tp) {
   tp = TightPlan();
 }
 
-int tight_solve_ex(TightPlan& tp, MgPlan& mg, const Dims& d, int nsolve, hipStream_t st) {
+int tight_solve_ex(TightPlan& tp, MgPlan& mg, const Dims& d, int nsolve, double guess, hipStream_t st) {
   if (nsolve <= 0) return EKM_OK;
   if (nsolve > tp.nsolve_max) return EKM_ERR_ARG;
-  const dim3 grid(nsolve), block(512);
-#define LAUNCH(T, CM)                                                                                             \
-  hipLaunchKernelGGL((k_tight_solve<T, CM>), grid, block, 0, st, (const TDesc<T>*)tp.mem, (T*)tp.ws, tp.ws64, \
-                     (const double*)mg.rhs, mg.phi, (const double*)mg.kx2, mg.cycles, d.plane)
+  const dim3 grid(nsolve);
+  // 16 wavefronts per workgroup where the line registers fit (FP32 up to 5
+  // points per lane), else 8
+#define LAUNCH(T, CM)                                                                                   \
+  hipLaunchKernelGGL((k_tight_solve<T, CM, (sizeof(T) == 4 && CM <= 5) ? 1024 : 512>), grid,          \
+                     dim3((sizeof(T) == 4 && CM <= 5) ? 1024 : 512), 0, st, (const TDesc<T>*)tp.mem, (T*)tp.ws, tp.ws64, \
+                     (const double*)mg.rhs, mg.phi, (const double*)mg.kx2, mg.cycles, d.plane, guess)
   if (tp.fp32) {
     if (tp.cm == 5) LAUNCH(float, 5);
     else if (tp.cm == 9) LAUNCH(float, 9);

@@ -54,6 +54,8 @@ struct ekm_solver {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double prof_ms = 0.0;
   long prof_launches = 0, prof_solves = 0, prof_cycles = 0;
+  int have_prev_phi = 0;  // mg.phi holds the previous substep's correction
+  double hb_prev = 0.0;
 };
 
 extern "C" {
@@ -283,6 +285,7 @@ extern "C" int ekm_field_device_ptr(ekm_solver* s, int f, void** dptr) {
 
 extern "C" int ekm_set_state(ekm_solver* s, const double* cu1, const double* cu2, const double* cu3,
                              const double* cp, const double* cth, double time, int time_step) {
+  s->have_prev_phi = 0;
   size_t n = s->d.npts * sizeof(double);
   const double* src[4] = {cu1, cu2, cu3, cp};
   double* dst[4] = {s->U[0], s->U[1], s->U[2], s->P};
@@ -307,7 +310,7 @@ extern "C" int ekm_get_state(ekm_solver* s, double* cu1, double* cu2, double* cu
 }
 
 // REM_DIV_CURV (src/duct.F90:2472-2818); phi is left in s->PHI
-static int rem_div(ekm_solver* s) {
+static int rem_div(ekm_solver* s, double guess = 0.0) {
   const Dims& d = s->d;
   hipStream_t st = s->st;
   int nmode = d.NKX + 1, nsolve = 2 * nmode;
@@ -318,7 +321,7 @@ static int rem_div(ekm_solver* s) {
   if (s->p.proj_mode == 0)
     TRY(mg_solve_ex(s->mg, d, nsolve, 0, st));
   else
-    TRY(tight_solve_ex(s->tp, s->mg, d, nsolve, st));
+    TRY(tight_solve_ex(s->tp, s->mg, d, nsolve, guess, st));
   if (s->prof) EKM_CHECK(hipEventRecord(s->ev1, st));
   TRY(launch_mg_scatter(d, s->mg.phi, nmode, s->p.cmplx_fp32, s->PHI, st));
   TRY(launch_project(d, s->M, s->rv, s->kx, s->PHI, s->U2b, s->U3b, s->U[0], s->U[1], s->U[2], s->S1, st));
@@ -349,7 +352,10 @@ static int rem_div(ekm_solver* s) {
   return EKM_OK;
 }
 
-extern "C" int ekm_rem_div_curv(ekm_solver* s) { return rem_div(s); }
+extern "C" int ekm_rem_div_curv(ekm_solver* s) {
+  s->have_prev_phi = 0;
+  return rem_div(s);
+}
 
 extern "C" int ekm_get_phi(ekm_solver* s, double* phi) {
   EKM_CHECK(hipMemcpyAsync(phi, s->PHI, s->d.npts * sizeof(double), hipMemcpyDeviceToHost, s->st));
@@ -374,6 +380,7 @@ extern "C" int ekm_set_initial_physical(ekm_solver* s, const double* u1, const d
   s->time = 0.0;
   s->time_step = 0;
   s->rk_step = 1;
+  s->have_prev_phi = 0;
   return rem_div(s);
 }
 
@@ -456,7 +463,12 @@ extern "C" int ekm_rk_duct(ekm_solver* s) {
   // back to Fourier space and project (:2421-2455)
   for (int f = 0; f < 3; ++f) TRY(fwd(s, s->U[f]));
   if (th) TRY(fwd(s, s->TH));
-  int stat = rem_div(s);
+  // proj_guess: the previous substep's correction, scaled by the ratio of the
+  // RK weights, starts the consistent solve
+  const double guess = (p.proj_mode > 0 && p.proj_guess && s->have_prev_phi) ? hb / s->hb_prev : 0.0;
+  int stat = rem_div(s, guess);
+  s->have_prev_phi = 1;
+  s->hb_prev = hb;
   // pressure update (:2460-2466)
   TRY(launch_pupdate(d, s->rv, s->PHI, s->P, TEMP4, st));
   s->diag.status = stat;
@@ -489,6 +501,7 @@ extern "C" int ekm_get_diag(ekm_solver* s, ekm_diag* dg) {
 extern "C" int ekm_multigrid_mudpack(ekm_solver* s, int nsolve, const double* kx2, const double* rhs_tmp, double* phi,
                                      int* cycles) {
   if (nsolve < 1 || nsolve > s->mg.nsolve_max) return EKM_ERR_ARG;
+  s->have_prev_phi = 0;
   size_t pl = s->d.plane;
   EKM_CHECK(hipMemcpyAsync(s->mg.kx2, kx2, nsolve * sizeof(double), hipMemcpyHostToDevice, s->st));
   EKM_CHECK(hipMemcpyAsync(s->mg.rhs, rhs_tmp, nsolve * pl * sizeof(double), hipMemcpyHostToDevice, s->st));

@@ -47,7 +47,8 @@ module ekman_bl_amd
      integer(c_int) :: proj_mode
      integer(c_int) :: proj_maxit
      real(c_double) :: proj_tol
-     integer(c_int) :: reserved(4)
+     integer(c_int) :: proj_guess
+     integer(c_int) :: reserved(3)
   end type ekm_params
 
   type, bind(c) :: ekm_diag

@@ -51,7 +51,8 @@ class EkmParams(ctypes.Structure):
         ("proj_mode", ctypes.c_int),
         ("proj_maxit", ctypes.c_int),
         ("proj_tol", ctypes.c_double),
-        ("reserved", ctypes.c_int * 4),
+        ("proj_guess", ctypes.c_int),
+        ("reserved", ctypes.c_int * 3),
     ]
 
     # -- convenience ---------------------------------------------------------

@@ -80,9 +80,11 @@ typedef struct ekm_params {
    *     the solve's residual; solved by FP32 alternating zebra-line V(1,1)
    *     cycles inside FP64 iterative refinement until max|residual| <= proj_tol */
   int proj_mode;
-  int proj_maxit;          /* cycle cap of proj_mode 1 */
-  double proj_tol;         /* absolute residual tolerance of proj_mode 1 */
-  int reserved[4];
+  int proj_maxit;          /* cycle cap of proj_mode 1, 2 */
+  double proj_tol;         /* absolute residual tolerance of proj_mode 1, 2 */
+  int proj_guess;          /* proj_mode 1, 2: 1 = start each solve from the previous substep's phi
+                            * scaled by the ratio of the RK weights h_bar (0 = from zero) */
+  int reserved[3];
 } ekm_params;
 
 /* Fill with the reference defaults that are not grid-size dependent. */
