@@ -26,6 +26,7 @@
 // maps with a wave-level scan.  y lines (stride NZ+2) are solved by lanes over
 // columns, each lane running the substitution down its column, so every load
 // of a wavefront is contiguous across lanes.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -36,12 +37,13 @@
 namespace {
 
 constexpr int kMaxLevT = 16;
-constexpr int kCoarseSweeps = 20;
+constexpr int kDirectMax = 128;  // the coarse level solved directly has at most this many points
 
 template <typename T>
 struct TLev {
   int nx, ny, n;
   int e_off, r_off, res_off;  // per-solve workspace offsets (elements of T)
+  int t2_off;                 // second line-solve scratch (the residual array is the first)
   int f_off;                  // per-mode factor offset: z factors [3][n], then y factors [3][n]
   const T* cf;                // c1..c8 planes [8][n]
   const double* s9;           // c9 = ((-kx2*jl) - s9) + corr9
@@ -57,6 +59,8 @@ struct TDesc {
   long ws;          // T elements per solve
   long ws64;        // doubles per solve: phi (ring) then b
   long fstride;     // factor elements per mode
+  int lc;           // coarsest level of the cycle, solved with its inverse
+  long ainv_off;    // per-mode offset of that inverse (n_lc x n_lc, row-major)
   const T* fac;     // [mode][fstride]
   const double* cf64;  // finest level c1..c8 in double [8][n]
   TLev<T> lev[kMaxLevT + 1];
@@ -89,103 +93,231 @@ __device__ inline T shfl_down(T v, int d) {
   return __shfl_down(v, d, 64);
 }
 
-// Zebra line relaxation of level L: lines of parity par in direction DIR
-// (0: z lines, rows j, contiguous; 1: y lines, columns i, stride nx+2).  One
-// wavefront per line: lane l owns points l*CM .. l*CM+CM-1 of the line, loads
-// all their operands at once, and the forward and back substitutions of the
-// line's Thomas factors are composed as affine maps with a 64-lane scan, so a
-// line costs one round of memory latency whatever its length.
-template <typename T, int CM, int DIR>
-__device__ void lines(const TLev<T>& L, T* __restrict__ W, const T* __restrict__ fac, int par) {
+// Zebra line relaxation of level L.  The Thomas factors of every line are
+// precomputed (1/piv, -lower/piv, upper/piv), so a line solve is the forward
+// recurrence dp_k = a_k dp_{k-1} + s_k/piv_k and the back substitution
+// x_k = dp_k - c_k x_{k+1}, both first-order affine recurrences.
+//
+// z lines (rows j of parity par, contiguous in memory): one wavefront per row;
+// point k = 64 c + lane sits in lane `lane` of stripe c, so every load and
+// store of a stripe is one contiguous run; within a stripe the recurrence is a
+// 64-lane scan of affine maps, carried from stripe to stripe.
+template <typename T, int CM>
+__device__ void zlines(const TLev<T>& L, T* __restrict__ W, const T* __restrict__ fac, int par) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int nx = L.nx, ny = L.ny, ld = nx + 2, n = L.n;
-  const int len = DIR == 0 ? nx : ny, nlines = DIR == 0 ? ny : nx;
   T* __restrict__ e = W + L.e_off;
   const T* __restrict__ r = W + L.r_off;
   const T* __restrict__ cf = L.cf;
   const unsigned char* __restrict__ spec = L.spec;
-  const T* f1 = fac + L.f_off + (DIR == 0 ? 0 : 3 * n);
+  const T* f1 = fac + L.f_off;
   const T* fa = f1 + n;
   const T* fc = fa + n;
-  const int k0 = lane * CM;
-  for (int line = par + 2 * wave; line < nlines; line += 2 * nw) {
-    T bk[CM], ak[CM], ck[CM];
-    Aff<T> loc{(T)1, (T)0};
+  for (int j = par + 2 * wave; j < ny; j += 2 * nw) {
+    T ak[CM], bk[CM], ck[CM], dk[CM];
 #pragma unroll
     for (int c = 0; c < CM; ++c) {
-      const int k = k0 + c;
+      const int i = 64 * c + lane;
       ak[c] = (T)1;
       bk[c] = (T)0;
       ck[c] = (T)0;
-      if (k < len) {
-        const int i = DIR == 0 ? k : line, j = DIR == 0 ? line : k;
+      if (i < nx) {
         const int p = i + nx * j;
         const int q = (i + 1) + ld * (j + 1);
         T s = (T)0;
-        if (!spec[p]) {
-          if (DIR == 0)  // off-line neighbours: rows j +- 1
-            s = r[p] - (cf[1 * n + p] * e[q + 1 + ld] + cf[2 * n + p] * e[q + ld] + cf[3 * n + p] * e[q - 1 + ld] +
-                        cf[5 * n + p] * e[q - 1 - ld] + cf[6 * n + p] * e[q - ld] + cf[7 * n + p] * e[q + 1 - ld]);
-          else  // columns i +- 1
-            s = r[p] - (cf[0 * n + p] * e[q + 1] + cf[1 * n + p] * e[q + 1 + ld] + cf[3 * n + p] * e[q - 1 + ld] +
-                        cf[4 * n + p] * e[q - 1] + cf[5 * n + p] * e[q - 1 - ld] + cf[7 * n + p] * e[q + 1 - ld]);
-        }
+        if (!spec[p])
+          s = r[p] - (cf[1 * n + p] * e[q + 1 + ld] + cf[2 * n + p] * e[q + ld] + cf[3 * n + p] * e[q - 1 + ld] +
+                      cf[5 * n + p] * e[q - 1 - ld] + cf[6 * n + p] * e[q - ld] + cf[7 * n + p] * e[q + 1 - ld]);
         ak[c] = fa[p];
         bk[c] = s * f1[p];
         ck[c] = fc[p];
-        loc = compose(Aff<T>{ak[c], bk[c]}, loc);
       }
     }
-    // forward: dp_k = a_k dp_{k-1} + b_k, inclusive scan over lanes
-    Aff<T> inc = loc;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      Aff<T> prev{shfl_up(inc.a, d), shfl_up(inc.b, d)};
-      if (lane >= d) inc = compose(inc, prev);
-    }
-    T dp = shfl_up(inc.b, 1);
-    if (lane == 0) dp = (T)0;
-    T dk[CM];
+    T carry = (T)0;  // dp at the last point of the previous stripe
 #pragma unroll
     for (int c = 0; c < CM; ++c) {
-      dp = ak[c] * dp + bk[c];
-      dk[c] = dp;
-    }
-    // backward: x_k = dp_k - cp_k x_{k+1}, composed from the right
-    Aff<T> locb{(T)1, (T)0};
+      Aff<T> inc{ak[c], bk[c]};
 #pragma unroll
-    for (int c = CM - 1; c >= 0; --c)
-      if (k0 + c < len) locb = compose(Aff<T>{-ck[c], dk[c]}, locb);
-    Aff<T> incb = locb;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      Aff<T> nxt{shfl_down(incb.a, d), shfl_down(incb.b, d)};
-      if (lane + d < 64) incb = compose(incb, nxt);
+      for (int d = 1; d < 64; d <<= 1) {
+        Aff<T> prev{shfl_up(inc.a, d), shfl_up(inc.b, d)};
+        if (lane >= d) inc = compose(inc, prev);
+      }
+      dk[c] = inc.a * carry + inc.b;
+      carry = __shfl(dk[c], 63, 64);
     }
-    T x = shfl_down(incb.b, 1);
-    if (lane == 63) x = (T)0;
+    T xo = (T)0;  // x at the first point of the next stripe
 #pragma unroll
     for (int c = CM - 1; c >= 0; --c) {
-      const int k = k0 + c;
-      if (k < len) {
-        x = dk[c] - ck[c] * x;
-        const int i = DIR == 0 ? k : line, j = DIR == 0 ? line : k;
-        e[(i + 1) + ld * (j + 1)] = x;
+      Aff<T> inc{-ck[c], dk[c]};
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        Aff<T> nxt{shfl_down(inc.a, d), shfl_down(inc.b, d)};
+        if (lane + d < 64) inc = compose(inc, nxt);
       }
+      const T x = inc.a * xo + inc.b;
+      const int i = 64 * c + lane;
+      if (i < nx) e[(i + 1) + ld * (j + 1)] = x;
+      xo = __shfl(x, 0, 64);
     }
   }
 }
 
+// y lines (columns i of parity par, stride nx+2): lanes over 64 columns of a
+// group, so every load of a row is contiguous across lanes; the rows are split
+// into chunks worked by different wavefronts.  Each wavefront runs its
+// chunk's recurrences as affine maps of the unknown value entering the chunk
+// (stored per point in the two scratch arrays), the chunk maps are combined
+// through LDS, and a last pass applies the entering values.
+constexpr int kYMaxTasks = 32;
+constexpr int kYUnroll = 4;
+
+template <typename T>
+__device__ void ylines(const TLev<T>& L, T* __restrict__ W, const T* __restrict__ fac, int par, T* lds) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int nx = L.nx, ny = L.ny, ld = nx + 2, n = L.n;
+  T* __restrict__ e = W + L.e_off;
+  const T* __restrict__ r = W + L.r_off;
+  T* __restrict__ tA = W + L.res_off;
+  T* __restrict__ tB = W + L.t2_off;
+  const T* __restrict__ cf = L.cf;
+  const unsigned char* __restrict__ spec = L.spec;
+  const T* f1 = fac + L.f_off + 3 * n;
+  const T* fa = f1 + n;
+  const T* fc = fa + n;
+  const int ncol = (nx - par + 1) / 2;
+  const int ngrp = (ncol + 63) / 64;
+  int nch = nw / ngrp;
+  if (nch < 1) nch = 1;
+  if (nch > ny) nch = ny;
+  if (ngrp * nch > kYMaxTasks) nch = kYMaxTasks / ngrp;
+  const int cl = (ny + nch - 1) / nch;
+  const int ntask = ngrp * nch;
+  // LDS: per task and lane: chunk map (a, b), then the entering value
+  T* sa = lds;
+  T* sb = sa + kYMaxTasks * 64;
+  T* sv = sb + kYMaxTasks * 64;
+  // forward: dp_j = A_j dp_in + B_j within the chunk
+  for (int task = wave; task < ntask; task += nw) {
+    const int grp = task / nch, ch = task - grp * nch;
+    const int i = par + 2 * (grp * 64 + lane);
+    const int j0 = ch * cl, j1 = min(j0 + cl, ny);
+    T A = (T)1, B = (T)0;
+    if (i < nx) {
+      for (int jb = j0; jb < j1; jb += kYUnroll) {
+        T sv_[kYUnroll], av[kYUnroll], fv[kYUnroll];
+#pragma unroll
+        for (int u = 0; u < kYUnroll; ++u) {
+          const int j = jb + u;
+          sv_[u] = (T)0;
+          av[u] = (T)1;
+          fv[u] = (T)0;
+          if (j < j1) {
+            const int p = i + nx * j;
+            const int q = (i + 1) + ld * (j + 1);
+            if (!spec[p])
+              sv_[u] = r[p] - (cf[0 * n + p] * e[q + 1] + cf[1 * n + p] * e[q + 1 + ld] + cf[3 * n + p] * e[q - 1 + ld] +
+                               cf[4 * n + p] * e[q - 1] + cf[5 * n + p] * e[q - 1 - ld] + cf[7 * n + p] * e[q + 1 - ld]);
+            av[u] = fa[p];
+            fv[u] = f1[p];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kYUnroll; ++u) {
+          const int j = jb + u;
+          if (j < j1) {
+            A = av[u] * A;
+            B = av[u] * B + sv_[u] * fv[u];
+            const int p = i + nx * j;
+            tA[p] = A;
+            tB[p] = B;
+          }
+        }
+      }
+    }
+    sa[task * 64 + lane] = A;
+    sb[task * 64 + lane] = B;
+  }
+  __syncthreads();
+  // entering dp of every chunk, chunks in order
+  for (int grp = wave; grp < ngrp; grp += nw) {
+    T dp = (T)0;
+    for (int ch = 0; ch < nch; ++ch) {
+      const int t = (grp * nch + ch) * 64 + lane;
+      sv[t] = dp;
+      dp = sa[t] * dp + sb[t];
+    }
+  }
+  __syncthreads();
+  // backward: x_j = alpha_j + beta_j x_out within the chunk
+  for (int task = wave; task < ntask; task += nw) {
+    const int grp = task / nch, ch = task - grp * nch;
+    const int i = par + 2 * (grp * 64 + lane);
+    const int j0 = ch * cl, j1 = min(j0 + cl, ny);
+    const T din = sv[task * 64 + lane];
+    T al = (T)0, be = (T)1;
+    if (i < nx) {
+      for (int jb = j1 - 1; jb >= j0; jb -= kYUnroll) {
+        T a_[kYUnroll], b_[kYUnroll], c_[kYUnroll];
+#pragma unroll
+        for (int u = 0; u < kYUnroll; ++u) {
+          const int j = jb - u;
+          if (j >= j0) {
+            const int p = i + nx * j;
+            a_[u] = tA[p];
+            b_[u] = tB[p];
+            c_[u] = fc[p];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kYUnroll; ++u) {
+          const int j = jb - u;
+          if (j >= j0) {
+            const T dp = a_[u] * din + b_[u];
+            al = dp - c_[u] * al;
+            be = -c_[u] * be;
+            const int p = i + nx * j;
+            tA[p] = al;
+            tB[p] = be;
+          }
+        }
+      }
+    }
+    sa[task * 64 + lane] = al;
+    sb[task * 64 + lane] = be;
+  }
+  __syncthreads();
+  // x entering each chunk from above, chunks in reverse order
+  for (int grp = wave; grp < ngrp; grp += nw) {
+    T x = (T)0;
+    for (int ch = nch - 1; ch >= 0; --ch) {
+      const int t = (grp * nch + ch) * 64 + lane;
+      sv[t] = x;
+      x = sa[t] + sb[t] * x;
+    }
+  }
+  __syncthreads();
+  for (int task = wave; task < ntask; task += nw) {
+    const int grp = task / nch, ch = task - grp * nch;
+    const int i = par + 2 * (grp * 64 + lane);
+    if (i >= nx) continue;
+    const int j0 = ch * cl, j1 = min(j0 + cl, ny);
+    const T xo = sv[task * 64 + lane];
+    for (int j = j0; j < j1; ++j) {
+      const int p = i + nx * j;
+      e[(i + 1) + ld * (j + 1)] = tA[p] + tB[p] * xo;
+    }
+  }
+  __syncthreads();
+}
+
 template <typename T, int CM>
-__device__ void smooth(const TDesc<T>& g, const TLev<T>& L, T* W, const T* fac) {
-  lines<T, CM, 0>(L, W, fac, 0);
+__device__ void smooth(const TDesc<T>& g, const TLev<T>& L, T* W, const T* fac, T* lds) {
+  zlines<T, CM>(L, W, fac, 0);
   __syncthreads();
-  lines<T, CM, 0>(L, W, fac, 1);
+  zlines<T, CM>(L, W, fac, 1);
   __syncthreads();
-  lines<T, CM, 1>(L, W, fac, 0);
-  __syncthreads();
-  lines<T, CM, 1>(L, W, fac, 1);
-  __syncthreads();
+  ylines<T>(L, W, fac, 0, lds);
+  ylines<T>(L, W, fac, 1, lds);
 }
 
 // res = r - A e on level L (0 at specified points)
@@ -257,12 +389,29 @@ __device__ void prolong_add(const TLev<T>& C, const TLev<T>& F, T* W) {
   __syncthreads();
 }
 
+// e = A^-1 r on the coarsest level of the cycle (A^-1 precomputed per mode)
+template <typename T>
+__device__ void direct_solve(const TLev<T>& L, T* W, const T* __restrict__ ainv) {
+  const int n = L.n, nx = L.nx, ld = nx + 2;
+  const T* __restrict__ r = W + L.r_off;
+  T* e = W + L.e_off;
+  for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    const T* row = ainv + (size_t)p * n;
+    T acc = (T)0;
+    for (int q = 0; q < n; ++q) acc += row[q] * r[q];
+    const int j = p / nx, i = p - j * nx;
+    e[(i + 1) + ld * (j + 1)] = acc;
+  }
+  __syncthreads();
+}
+
 template <typename T, int CM, int NT>
 __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__ gp, T* __restrict__ ws,
                                                      double* __restrict__ ws64, const double* __restrict__ rhs_tmp,
                                                      double* __restrict__ phi_out, const double* __restrict__ kx2v,
                                                      int* __restrict__ cycles, size_t plane, double guess) {
   __shared__ double red[NT];
+  __shared__ T ylds[3 * kYMaxTasks * 64];
   const TDesc<T>& g = *gp;
   const int s = blockIdx.x;
   const double kx2 = kx2v[s];
@@ -316,13 +465,19 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
     __syncthreads();
     if (rmax <= g.tol || it >= g.maxit) break;
     // one V(1,1) cycle on the correction equation: smoothing steps walked in
-    // order (down: ng..2, the coarsest level, up: 2..ng), one call site
-    const int nsm = 2 * (ng - 1) + kCoarseSweeps;
+    // order (down: ng..lc+1, then the direct solve on lc, up: lc+1..ng), one
+    // call site of the smoother
+    const int lc = g.lc, nsm = 2 * (ng - lc);
+    const T* ainv = fac + g.ainv_off;
+    if (nsm == 0) direct_solve<T>(g.lev[lc], W, ainv);
     for (int k = 0; k < nsm; ++k) {
-      const bool down = k < ng - 1, up = k >= ng - 1 + kCoarseSweeps;
-      const int l = down ? ng - k : (up ? 2 + (k - (ng - 1 + kCoarseSweeps)) : 1);
-      if (up) prolong_add<T>(g.lev[l - 1], g.lev[l], W);
-      smooth<T, CM>(g, g.lev[l], W, fac);
+      const bool down = k < ng - lc;
+      const int l = down ? ng - k : lc + 1 + (k - (ng - lc));
+      if (!down) {
+        if (l == lc + 1) direct_solve<T>(g.lev[lc], W, ainv);
+        prolong_add<T>(g.lev[l - 1], g.lev[l], W);
+      }
+      smooth<T, CM>(g, g.lev[l], W, fac, ylds);
       if (down) {
         residual<T>(g.lev[l], W, kx2);
         restrict_fw<T>(g.lev[l], g.lev[l - 1], W);
@@ -338,6 +493,38 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
     po[p] = phi[(i + 1) + ld * (j + 1)];
   }
   if (threadIdx.x == 0) cycles[s] = it;
+}
+
+// Gauss-Jordan inverse with partial pivoting of a dense n x n matrix
+bool invert_dense(int n, std::vector<double> A, std::vector<double>& inv) {
+  inv.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+  for (int c = 0; c < n; ++c) {
+    int pr = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::fabs(A[(size_t)r * n + c]) > std::fabs(A[(size_t)pr * n + c])) pr = r;
+    if (A[(size_t)pr * n + c] == 0.0) return false;
+    if (pr != c)
+      for (int k = 0; k < n; ++k) {
+        std::swap(A[(size_t)pr * n + k], A[(size_t)c * n + k]);
+        std::swap(inv[(size_t)pr * n + k], inv[(size_t)c * n + k]);
+      }
+    const double rp = 1.0 / A[(size_t)c * n + c];
+    for (int k = 0; k < n; ++k) {
+      A[(size_t)c * n + k] *= rp;
+      inv[(size_t)c * n + k] *= rp;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const double f = A[(size_t)r * n + c];
+      if (f == 0.0) continue;
+      for (int k = 0; k < n; ++k) {
+        A[(size_t)r * n + k] -= f * A[(size_t)c * n + k];
+        inv[(size_t)r * n + k] -= f * inv[(size_t)c * n + k];
+      }
+    }
+  }
+  return true;
 }
 
 // Thomas factors of one line: lower/diag/upper -> (1/piv, -lower/piv, upper/piv)
@@ -383,10 +570,18 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
     ws += L.n;
     L.res_off = (int)ws;
     ws += L.n;
+    L.t2_off = (int)ws;
+    ws += L.n;
     ws = (ws + 31) & ~31L;
     L.f_off = (int)fstride;
     fstride += 6L * L.n;
   }
+  int lc = 1;
+  for (int k = 1; k <= ng; ++k)
+    if (g.lev[k].n <= kDirectMax) lc = k;
+  g.lc = lc;
+  g.ainv_off = fstride;
+  fstride += (long)g.lev[lc].n * g.lev[lc].n;
   fstride = (fstride + 63) & ~63L;
   g.ws = ws;
   g.ws64 = (long)(d.NZ2 + 2) * (d.NY2 + 2) + (long)d.NZ2 * d.NY2;
@@ -479,6 +674,31 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
           const size_t q = i + (size_t)nx * j;
           fy[q] = (T)f1[j]; fy[n + q] = (T)fa[j]; fy[2 * n + q] = (T)fc[j];
         }
+      }
+      // the inverse of the cycle's coarsest operator
+      if (k == lc) {
+        std::vector<double> A(n * n, 0.0);
+        for (int j = 0; j < ny; ++j)
+          for (int i = 0; i < nx; ++i) {
+            const size_t q = i + (size_t)nx * j;
+            if (spec[off + q]) {
+              A[q * n + q] = 1.0;
+              continue;
+            }
+            A[q * n + q] = c9(q);
+            static const int di8[8] = {1, 1, 0, -1, -1, -1, 0, 1}, dj8[8] = {0, 1, 1, 1, 0, -1, -1, -1};
+            for (int c = 0; c < 8; ++c) {
+              const int ii = i + di8[c], jj = j + dj8[c];
+              if (ii < 0 || ii >= nx || jj < 0 || jj >= ny) continue;
+              const size_t qq = ii + (size_t)nx * jj;
+              if (spec[off + qq]) continue;  // the correction is zero there
+              A[q * n + qq] += H.c[c * n + q];
+            }
+          }
+        std::vector<double> inv;
+        if (!invert_dense((int)n, A, inv)) return EKM_ERR_UNSUPPORTED;
+        T* ai = fac.data() + (size_t)m * fstride + g.ainv_off;
+        for (size_t t = 0; t < n * n; ++t) ai[t] = (T)inv[t];
       }
     }
     off += n;
