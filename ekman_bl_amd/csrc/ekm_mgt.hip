@@ -44,11 +44,8 @@ struct TLev {
   int nx, ny, n;
   int e_off, r_off, res_off;  // per-solve workspace offsets (elements of T)
   int t2_off;                 // second line-solve scratch (the residual array is the first)
-  int f_off;                  // per-mode factor offset: z factors [3][n], then y factors [3][n]
-  const T* cf;                // c1..c8 planes [8][n]
-  const double* s9;           // c9 = ((-kx2*jl) - s9) + corr9
-  const double* corr9;
-  const double* jl;
+  int f_off;                  // per-mode offset: z factors [3][n], y factors [3][n], c9 [n]
+  const T* cf;                // c1..c8 planes [8][n]; rows of specified points are 0 (c9 = 1)
   const unsigned char* spec;  // specified (Dirichlet) points
 };
 
@@ -63,16 +60,9 @@ struct TDesc {
   long ainv_off;    // per-mode offset of that inverse (n_lc x n_lc, row-major)
   const T* fac;     // [mode][fstride]
   const double* cf64;  // finest level c1..c8 in double [8][n]
+  const double* c964;  // finest level c9 in double, [mode][n]
   TLev<T> lev[kMaxLevT + 1];
 };
-
-constexpr int kDi[8] = {1, 1, 0, -1, -1, -1, 0, 1};
-constexpr int kDj[8] = {0, 1, 1, 1, 0, -1, -1, -1};
-
-template <typename T>
-__device__ inline T c9_of(const TLev<T>& L, int p, double kx2) {
-  return (T)(((-(kx2 * L.jl[p])) - L.s9[p]) + L.corr9[p]);
-}
 
 // affine map x -> a*x + b; compose(outer, inner) = outer(inner(x))
 template <typename T>
@@ -102,14 +92,13 @@ __device__ inline T shfl_down(T v, int d) {
 // point k = 64 c + lane sits in lane `lane` of stripe c, so every load and
 // store of a stripe is one contiguous run; within a stripe the recurrence is a
 // 64-lane scan of affine maps, carried from stripe to stripe.
-template <typename T, int CM>
+template <typename T, int CM, bool X>
 __device__ void zlines(const TLev<T>& L, T* __restrict__ W, const T* __restrict__ fac, int par) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int nx = L.nx, ny = L.ny, ld = nx + 2, n = L.n;
   T* __restrict__ e = W + L.e_off;
   const T* __restrict__ r = W + L.r_off;
   const T* __restrict__ cf = L.cf;
-  const unsigned char* __restrict__ spec = L.spec;
   const T* f1 = fac + L.f_off;
   const T* fa = f1 + n;
   const T* fc = fa + n;
@@ -124,10 +113,10 @@ __device__ void zlines(const TLev<T>& L, T* __restrict__ W, const T* __restrict_
       if (i < nx) {
         const int p = i + nx * j;
         const int q = (i + 1) + ld * (j + 1);
-        T s = (T)0;
-        if (!spec[p])
-          s = r[p] - (cf[1 * n + p] * e[q + 1 + ld] + cf[2 * n + p] * e[q + ld] + cf[3 * n + p] * e[q - 1 + ld] +
-                      cf[5 * n + p] * e[q - 1 - ld] + cf[6 * n + p] * e[q - ld] + cf[7 * n + p] * e[q + 1 - ld]);
+        T s = r[p] - (cf[2 * n + p] * e[q + ld] + cf[6 * n + p] * e[q - ld]);
+        if (X)
+          s -= (cf[1 * n + p] * e[q + 1 + ld] + cf[3 * n + p] * e[q - 1 + ld]) +
+               (cf[5 * n + p] * e[q - 1 - ld] + cf[7 * n + p] * e[q + 1 - ld]);
         ak[c] = fa[p];
         bk[c] = s * f1[p];
         ck[c] = fc[p];
@@ -171,7 +160,7 @@ __device__ void zlines(const TLev<T>& L, T* __restrict__ W, const T* __restrict_
 constexpr int kYMaxTasks = 32;
 constexpr int kYUnroll = 4;
 
-template <typename T>
+template <typename T, bool X>
 __device__ void ylines(const TLev<T>& L, T* __restrict__ W, const T* __restrict__ fac, int par, T* lds) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int nx = L.nx, ny = L.ny, ld = nx + 2, n = L.n;
@@ -180,7 +169,6 @@ __device__ void ylines(const TLev<T>& L, T* __restrict__ W, const T* __restrict_
   T* __restrict__ tA = W + L.res_off;
   T* __restrict__ tB = W + L.t2_off;
   const T* __restrict__ cf = L.cf;
-  const unsigned char* __restrict__ spec = L.spec;
   const T* f1 = fac + L.f_off + 3 * n;
   const T* fa = f1 + n;
   const T* fc = fa + n;
@@ -214,9 +202,10 @@ __device__ void ylines(const TLev<T>& L, T* __restrict__ W, const T* __restrict_
           if (j < j1) {
             const int p = i + nx * j;
             const int q = (i + 1) + ld * (j + 1);
-            if (!spec[p])
-              sv_[u] = r[p] - (cf[0 * n + p] * e[q + 1] + cf[1 * n + p] * e[q + 1 + ld] + cf[3 * n + p] * e[q - 1 + ld] +
-                               cf[4 * n + p] * e[q - 1] + cf[5 * n + p] * e[q - 1 - ld] + cf[7 * n + p] * e[q + 1 - ld]);
+            sv_[u] = r[p] - (cf[0 * n + p] * e[q + 1] + cf[4 * n + p] * e[q - 1]);
+            if (X)
+              sv_[u] -= (cf[1 * n + p] * e[q + 1 + ld] + cf[3 * n + p] * e[q - 1 + ld]) +
+                        (cf[5 * n + p] * e[q - 1 - ld] + cf[7 * n + p] * e[q + 1 - ld]);
             av[u] = fa[p];
             fv[u] = f1[p];
           }
@@ -310,35 +299,34 @@ __device__ void ylines(const TLev<T>& L, T* __restrict__ W, const T* __restrict_
   __syncthreads();
 }
 
-template <typename T, int CM>
+template <typename T, int CM, bool X>
 __device__ void smooth(const TDesc<T>& g, const TLev<T>& L, T* W, const T* fac, T* lds) {
-  zlines<T, CM>(L, W, fac, 0);
+  zlines<T, CM, X>(L, W, fac, 0);
   __syncthreads();
-  zlines<T, CM>(L, W, fac, 1);
+  zlines<T, CM, X>(L, W, fac, 1);
   __syncthreads();
-  ylines<T>(L, W, fac, 0, lds);
-  ylines<T>(L, W, fac, 1, lds);
+  ylines<T, X>(L, W, fac, 0, lds);
+  ylines<T, X>(L, W, fac, 1, lds);
 }
 
-// res = r - A e on level L (0 at specified points)
-template <typename T>
-__device__ void residual(const TLev<T>& L, T* W, double kx2) {
+// res = r - A e on level L (specified points have the row e = r = 0)
+template <typename T, bool X>
+__device__ void residual(const TLev<T>& L, T* W, const T* __restrict__ fac) {
   const int nx = L.nx, ld = nx + 2, n = L.n;
-  const T* e = W + L.e_off;
-  const T* r = W + L.r_off;
-  T* res = W + L.res_off;
+  const T* __restrict__ e = W + L.e_off;
+  const T* __restrict__ r = W + L.r_off;
+  const T* __restrict__ cf = L.cf;
+  const T* __restrict__ c9 = fac + L.f_off + 6 * n;
+  T* __restrict__ res = W + L.res_off;
   for (int p = threadIdx.x; p < n; p += blockDim.x) {
     const int j = p / nx, i = p - j * nx;
     const int q = (i + 1) + ld * (j + 1);
-    T v = (T)0;
-    if (!L.spec[p]) {
-      T acc = (T)0;
-#pragma unroll
-      for (int c = 0; c < 8; ++c) acc += L.cf[c * n + p] * e[q + kDi[c] + ld * kDj[c]];
-      acc += c9_of(L, p, kx2) * e[q];
-      v = r[p] - acc;
-    }
-    res[p] = v;
+    T acc = (cf[0 * n + p] * e[q + 1] + cf[2 * n + p] * e[q + ld]) + (cf[4 * n + p] * e[q - 1] + cf[6 * n + p] * e[q - ld]);
+    if (X)
+      acc += (cf[1 * n + p] * e[q + 1 + ld] + cf[3 * n + p] * e[q - 1 + ld]) +
+             (cf[5 * n + p] * e[q - 1 - ld] + cf[7 * n + p] * e[q + 1 - ld]);
+    acc += c9[p] * e[q];
+    res[p] = r[p] - acc;
   }
   __syncthreads();
 }
@@ -405,7 +393,7 @@ __device__ void direct_solve(const TLev<T>& L, T* W, const T* __restrict__ ainv)
   __syncthreads();
 }
 
-template <typename T, int CM, int NT>
+template <typename T, int CM, int NT, bool X>
 __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__ gp, T* __restrict__ ws,
                                                      double* __restrict__ ws64, const double* __restrict__ rhs_tmp,
                                                      double* __restrict__ phi_out, const double* __restrict__ kx2v,
@@ -414,7 +402,7 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
   __shared__ T ylds[3 * kYMaxTasks * 64];
   const TDesc<T>& g = *gp;
   const int s = blockIdx.x;
-  const double kx2 = kx2v[s];
+  (void)kx2v;  // the wavenumber enters through the per-mode tables
   T* W = ws + (size_t)s * g.ws;
   double* phi = ws64 + (size_t)s * g.ws64;
   const int ng = g.ngrid;
@@ -422,6 +410,7 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
   const int nx = F.nx, ny = F.ny, ld = nx + 2, n = F.n;
   double* b = phi + (size_t)(nx + 2) * (ny + 2);
   const T* fac = g.fac + (size_t)(s >> 1) * g.fstride;
+  const double* __restrict__ c964 = g.c964 + (size_t)(s >> 1) * n;
   const double* rt = rhs_tmp + (size_t)s * plane;
   // phi starts from zero, or from the previous solution of this solve scaled
   // by `guess` (phi_out still holds it)
@@ -443,14 +432,14 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
     for (int p = threadIdx.x; p < n; p += blockDim.x) {
       const int j = p / nx, i = p - j * nx;
       const int q = (i + 1) + ld * (j + 1);
-      double v = 0.0;
-      if (!F.spec[p]) {
-        double acc = 0.0;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) acc += g.cf64[c * n + p] * phi[q + kDi[c] + ld * kDj[c]];
-        acc += ((((-(kx2 * F.jl[p])) - F.s9[p]) + F.corr9[p])) * phi[q];
-        v = b[p] - acc;
-      }
+      const double* __restrict__ c = g.cf64;
+      double acc = (c[0 * n + p] * phi[q + 1] + c[2 * n + p] * phi[q + ld]) +
+                   (c[4 * n + p] * phi[q - 1] + c[6 * n + p] * phi[q - ld]);
+      if (X)
+        acc += (c[1 * n + p] * phi[q + 1 + ld] + c[3 * n + p] * phi[q - 1 + ld]) +
+               (c[5 * n + p] * phi[q - 1 - ld] + c[7 * n + p] * phi[q + 1 - ld]);
+      acc += c964[p] * phi[q];
+      const double v = b[p] - acc;
       rmax = fmax(rmax, fabs(v));
       rf[p] = (T)v;
     }
@@ -477,9 +466,9 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
         if (l == lc + 1) direct_solve<T>(g.lev[lc], W, ainv);
         prolong_add<T>(g.lev[l - 1], g.lev[l], W);
       }
-      smooth<T, CM>(g, g.lev[l], W, fac, ylds);
+      smooth<T, CM, X>(g, g.lev[l], W, fac, ylds);
       if (down) {
-        residual<T>(g.lev[l], W, kx2);
+        residual<T, X>(g.lev[l], W, fac);
         restrict_fw<T>(g.lev[l], g.lev[l - 1], W);
       }
     }
@@ -574,7 +563,7 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
     ws += L.n;
     ws = (ws + 31) & ~31L;
     L.f_off = (int)fstride;
-    fstride += 6L * L.n;
+    fstride += 7L * L.n;  // z factors, y factors, c9
   }
   int lc = 1;
   for (int k = 1; k <= ng; ++k)
@@ -616,76 +605,94 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
         L.jl[q] = hm.jac[(size_t)K + (size_t)NZ3 * J];
       }
   }
-  // specified points, coefficient tables and line factors
+  // specified points: identity rows (c1..c8 = 0, c9 = 1); cross terms present?
   size_t tot_n = 0;
-  for (int k = 1; k <= ng; ++k) tot_n += (size_t)g.lev[k].n;
-  std::vector<unsigned char> spec(tot_n);
-  std::vector<T> cfT(8 * tot_n);
-  std::vector<double> s9(tot_n), corr9(tot_n), jl(tot_n);
-  std::vector<T> fac((size_t)nmode * fstride, (T)0);
-  size_t off = 0;
   std::vector<size_t> loff(ng + 1);
+  for (int k = 1; k <= ng; ++k) {
+    loff[k] = tot_n;
+    tot_n += (size_t)g.lev[k].n;
+  }
+  std::vector<unsigned char> spec(tot_n);
+  bool cross = false;
+  for (int k = 1; k <= ng; ++k) {
+    HostMgLevel& H = hl[k];
+    const int nx = H.nx, ny = H.ny;
+    const size_t n = (size_t)nx * ny;
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const size_t q = i + (size_t)nx * j;
+        const bool sp = (p.mudpack_bc[0] == 1 && i == 0) || (p.mudpack_bc[1] == 1 && i == nx - 1) ||
+                        (p.mudpack_bc[2] == 1 && j == 0) || (p.mudpack_bc[3] == 1 && j == ny - 1);
+        spec[loff[k] + q] = sp;
+        if (sp) {
+          for (int c = 0; c < 8; ++c) H.c[c * n + q] = 0.0;
+          H.s9[q] = -1.0;
+          H.corr9[q] = 0.0;
+          H.jl[q] = 0.0;
+        }
+        for (int c = 1; c < 8; c += 2)
+          if (H.c[c * n + q] != 0.0) cross = true;
+      }
+  }
+  tp.cross = cross;
+  // coefficient tables, and per live mode: line factors, c9, coarsest inverse
+  const size_t nfine = (size_t)g.lev[ng].n;
+  std::vector<T> cfT(8 * tot_n);
+  std::vector<T> fac((size_t)nmode * fstride, (T)0);
+  std::vector<double> c964((size_t)nmode * nfine);
   for (int k = 1; k <= ng; ++k) {
     const HostMgLevel& H = hl[k];
     const int nx = H.nx, ny = H.ny;
-    const size_t n = (size_t)nx * ny;
-    loff[k] = off;
-    for (int j = 0; j < ny; ++j)
-      for (int i = 0; i < nx; ++i)
-        spec[off + i + (size_t)nx * j] = (p.mudpack_bc[0] == 1 && i == 0) || (p.mudpack_bc[1] == 1 && i == nx - 1) ||
-                                         (p.mudpack_bc[2] == 1 && j == 0) || (p.mudpack_bc[3] == 1 && j == ny - 1);
+    const size_t n = (size_t)nx * ny, off = loff[k];
     for (size_t t = 0; t < 8 * n; ++t) cfT[8 * off + t] = (T)H.c[t];
-    std::memcpy(&s9[off], H.s9.data(), n * sizeof(double));
-    std::memcpy(&corr9[off], H.corr9.data(), n * sizeof(double));
-    std::memcpy(&jl[off], H.jl.data(), n * sizeof(double));
-    // line factors of every live mode
     std::vector<double> lo(std::max(nx, ny)), di(lo.size()), up(lo.size()), f1(lo.size()), fa(lo.size()), fc(lo.size());
     for (int m = 0; m < nmode; ++m) {
       const double kx2 = kx2_mode[m];
       T* fz = fac.data() + (size_t)m * fstride + g.lev[k].f_off;
       T* fy = fz + 3 * n;
+      T* fc9 = fz + 6 * n;
       auto c9 = [&](size_t q) { return ((-(kx2 * H.jl[q])) - H.s9[q]) + H.corr9[q]; };
+      for (size_t q = 0; q < n; ++q) fc9[q] = (T)c9(q);
+      if (k == ng)
+        for (size_t q = 0; q < n; ++q) c964[(size_t)m * nfine + q] = c9(q);
       for (int j = 0; j < ny; ++j) {
         for (int i = 0; i < nx; ++i) {
           const size_t q = i + (size_t)nx * j;
-          if (spec[off + q]) {
-            lo[i] = 0.0; di[i] = 1.0; up[i] = 0.0;
-          } else {
-            lo[i] = H.c[4 * n + q]; di[i] = c9(q); up[i] = H.c[0 * n + q];
-          }
+          lo[i] = H.c[4 * n + q];
+          di[i] = c9(q);
+          up[i] = H.c[0 * n + q];
         }
         line_factors(nx, lo.data(), di.data(), up.data(), f1.data(), fa.data(), fc.data());
         for (int i = 0; i < nx; ++i) {
           const size_t q = i + (size_t)nx * j;
-          fz[q] = (T)f1[i]; fz[n + q] = (T)fa[i]; fz[2 * n + q] = (T)fc[i];
+          // a specified point's correction is zero: 1/piv is stored as 0
+          fz[q] = spec[off + q] ? (T)0 : (T)f1[i];
+          fz[n + q] = (T)fa[i];
+          fz[2 * n + q] = (T)fc[i];
         }
       }
       for (int i = 0; i < nx; ++i) {
         for (int j = 0; j < ny; ++j) {
           const size_t q = i + (size_t)nx * j;
-          if (spec[off + q]) {
-            lo[j] = 0.0; di[j] = 1.0; up[j] = 0.0;
-          } else {
-            lo[j] = H.c[6 * n + q]; di[j] = c9(q); up[j] = H.c[2 * n + q];
-          }
+          lo[j] = H.c[6 * n + q];
+          di[j] = c9(q);
+          up[j] = H.c[2 * n + q];
         }
         line_factors(ny, lo.data(), di.data(), up.data(), f1.data(), fa.data(), fc.data());
         for (int j = 0; j < ny; ++j) {
           const size_t q = i + (size_t)nx * j;
-          fy[q] = (T)f1[j]; fy[n + q] = (T)fa[j]; fy[2 * n + q] = (T)fc[j];
+          fy[q] = spec[off + q] ? (T)0 : (T)f1[j];
+          fy[n + q] = (T)fa[j];
+          fy[2 * n + q] = (T)fc[j];
         }
       }
-      // the inverse of the cycle's coarsest operator
       if (k == lc) {
         std::vector<double> A(n * n, 0.0);
         for (int j = 0; j < ny; ++j)
           for (int i = 0; i < nx; ++i) {
             const size_t q = i + (size_t)nx * j;
-            if (spec[off + q]) {
-              A[q * n + q] = 1.0;
-              continue;
-            }
             A[q * n + q] = c9(q);
+            if (spec[off + q]) continue;
             static const int di8[8] = {1, 1, 0, -1, -1, -1, 0, 1}, dj8[8] = {0, 1, 1, 1, 0, -1, -1, -1};
             for (int c = 0; c < 8; ++c) {
               const int ii = i + di8[c], jj = j + dj8[c];
@@ -701,50 +708,38 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
         for (size_t t = 0; t < n * n; ++t) ai[t] = (T)inv[t];
       }
     }
-    off += n;
   }
-  // device copies
-  char* mem = nullptr;
+  // device copies: descriptor, spec, c1..c8 (T), finest c1..c8 (double), c9 per mode (double), factors
+  const size_t b_desc = (sizeof(TDesc<T>) + 255) & ~(size_t)255;
   const size_t b_spec = (tot_n + 255) & ~(size_t)255;
   const size_t b_cf = (8 * tot_n * sizeof(T) + 255) & ~(size_t)255;
-  const size_t b_d = (tot_n * sizeof(double) + 255) & ~(size_t)255;
-  const size_t nfine = (size_t)g.lev[ng].n;
   const size_t b_cf64 = (8 * nfine * sizeof(double) + 255) & ~(size_t)255;
-  const size_t b_desc = (sizeof(TDesc<T>) + 255) & ~(size_t)255;
-  const size_t total = b_desc + b_spec + b_cf + 3 * b_d + b_cf64;
-  EKM_CHECK(hipMalloc(&mem, total));
+  const size_t b_c964 = (c964.size() * sizeof(double) + 255) & ~(size_t)255;
+  char* mem = nullptr;
+  EKM_CHECK(hipMalloc(&mem, b_desc + b_spec + b_cf + b_cf64 + b_c964));
   tp.mem = mem;
   char* cur = mem + b_desc;
   unsigned char* d_spec = (unsigned char*)cur;
   cur += b_spec;
   T* d_cf = (T*)cur;
   cur += b_cf;
-  double* d_s9 = (double*)cur;
-  cur += b_d;
-  double* d_corr9 = (double*)cur;
-  cur += b_d;
-  double* d_jl = (double*)cur;
-  cur += b_d;
   double* d_cf64 = (double*)cur;
+  cur += b_cf64;
+  double* d_c964 = (double*)cur;
   for (int k = 1; k <= ng; ++k) {
-    TLev<T>& L = g.lev[k];
-    L.spec = d_spec + loff[k];
-    L.cf = d_cf + 8 * loff[k];
-    L.s9 = d_s9 + loff[k];
-    L.corr9 = d_corr9 + loff[k];
-    L.jl = d_jl + loff[k];
+    g.lev[k].spec = d_spec + loff[k];
+    g.lev[k].cf = d_cf + 8 * loff[k];
   }
   g.cf64 = d_cf64;
+  g.c964 = d_c964;
   EKM_CHECK(hipMalloc(&tp.fac, fac.size() * sizeof(T)));
   g.fac = (const T*)tp.fac;
   EKM_CHECK(hipMemcpy(tp.fac, fac.data(), fac.size() * sizeof(T), hipMemcpyHostToDevice));
   EKM_CHECK(hipMemcpy(mem, &g, sizeof(g), hipMemcpyHostToDevice));
   EKM_CHECK(hipMemcpy(d_spec, spec.data(), tot_n, hipMemcpyHostToDevice));
   EKM_CHECK(hipMemcpy(d_cf, cfT.data(), 8 * tot_n * sizeof(T), hipMemcpyHostToDevice));
-  EKM_CHECK(hipMemcpy(d_s9, s9.data(), tot_n * sizeof(double), hipMemcpyHostToDevice));
-  EKM_CHECK(hipMemcpy(d_corr9, corr9.data(), tot_n * sizeof(double), hipMemcpyHostToDevice));
-  EKM_CHECK(hipMemcpy(d_jl, jl.data(), tot_n * sizeof(double), hipMemcpyHostToDevice));
   EKM_CHECK(hipMemcpy(d_cf64, hl[ng].c.data(), 8 * nfine * sizeof(double), hipMemcpyHostToDevice));
+  EKM_CHECK(hipMemcpy(d_c964, c964.data(), c964.size() * sizeof(double), hipMemcpyHostToDevice));
   tp.ws_elems = g.ws;
   tp.ws64_elems = g.ws64;
   return EKM_OK;
@@ -782,10 +777,17 @@ int tight_solve_ex(TightPlan& tp, MgPlan& mg, const Dims& d, int nsolve, double 
   const dim3 grid(nsolve);
   // 16 wavefronts per workgroup where the line registers fit (FP32 up to 5
   // points per lane), else 8
-#define LAUNCH(T, CM)                                                                                   \
-  hipLaunchKernelGGL((k_tight_solve<T, CM, (sizeof(T) == 4 && CM <= 5) ? 1024 : 512>), grid,          \
+#define LAUNCH2(T, CM, X)                                                                               \
+  hipLaunchKernelGGL((k_tight_solve<T, CM, (sizeof(T) == 4 && CM <= 5) ? 1024 : 512, X>), grid,       \
                      dim3((sizeof(T) == 4 && CM <= 5) ? 1024 : 512), 0, st, (const TDesc<T>*)tp.mem, (T*)tp.ws, tp.ws64, \
                      (const double*)mg.rhs, mg.phi, (const double*)mg.kx2, mg.cycles, d.plane, guess)
+#define LAUNCH(T, CM)           \
+  do {                          \
+    if (tp.cross)               \
+      LAUNCH2(T, CM, true);     \
+    else                        \
+      LAUNCH2(T, CM, false);    \
+  } while (0)
   if (tp.fp32) {
     if (tp.cm == 5) LAUNCH(float, 5);
     else if (tp.cm == 9) LAUNCH(float, 9);
@@ -796,6 +798,7 @@ int tight_solve_ex(TightPlan& tp, MgPlan& mg, const Dims& d, int nsolve, double 
     else LAUNCH(double, 13);
   }
 #undef LAUNCH
+#undef LAUNCH2
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }
