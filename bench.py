@@ -10,7 +10,8 @@ per-kx pressure projection.  Metric: grid-point-steps per second
 The headline form (`value`) is the consistent projection with FP32
 alternating-line V(1,1) cycles inside FP64 residual iterations
 (proj_mode 2, max|residual| <= 1e-12 per real solve, each solve started
-from the previous substep's correction); `forms` also times
+from zero: a V(1,1) cycle reduces the residual about a thousandfold, so
+the previous substep's correction as a start saves no cycle); `forms` also times
 the same projection with FP64 cycles (proj_mode 1) and the plain MUDPACK
 replica (proj_mode 0, the reference's W(2,1) cycles to relmax 1e-6, which
 does not reach the divergence bound).  The divergence columns are the
@@ -51,12 +52,12 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_form(cfg, xp, yp, ic, mode: int, steps: int, warmup: int, tol: float):
+def run_form(cfg, xp, yp, ic, mode: int, steps: int, warmup: int, tol: float, guess: int = 0):
     p = cfg.params.copy()
     p.proj_mode = mode
     p.proj_tol = tol
     p.proj_maxit = 50
-    p.proj_guess = 1 if mode > 0 else 0
+    p.proj_guess = guess if mode > 0 else 0
     s = S.DuctSolver(p, xp, yp)
     s.set_initial_physical(*ic)
     for _ in range(warmup):
@@ -116,6 +117,8 @@ def main() -> int:
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--mode", type=int, default=2, help="headline projection form (0, 1, 2)")
     ap.add_argument("--tol", type=float, default=1e-12, help="max |residual| per real solve, modes 1 and 2")
+    ap.add_argument("--guess", type=int, default=0, choices=(0, 1),
+                    help="warm start of modes 1 and 2 (0: from zero, 1: previous substep's correction)")
     ap.add_argument("--side-steps", type=int, default=5, help="timed steps of the other forms (0: skip)")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-oracle work")
     ap.add_argument("--no-cpu", action="store_true")
@@ -144,7 +147,7 @@ def main() -> int:
         L.ekm_dev_free(b)
         log(f"copy kernel: {n * 8} bytes read, {n * 8} bytes written")
     log(f"[bench] {cfg.name}: headline form {args.mode} ({FORM_NAMES[args.mode]}), {args.steps} steps")
-    h = run_form(cfg, xp, yp, ic, args.mode, args.steps, args.warmup, args.tol)
+    h = run_form(cfg, xp, yp, ic, args.mode, args.steps, args.warmup, args.tol, args.guess)
     value = p.npts * args.steps / h["el"] / 1e6
     ms_per_step = h["el"] / args.steps * 1e3
     nsolve = 2 * (p.nx // 3 + 1)
@@ -169,7 +172,7 @@ def main() -> int:
             r = h
         elif args.side_steps > 0:
             log(f"[bench] form {mode} ({FORM_NAMES[mode]}), {args.side_steps} steps")
-            r = run_form(cfg, xp, yp, ic, mode, args.side_steps, 1, args.tol)
+            r = run_form(cfg, xp, yp, ic, mode, args.side_steps, 1, args.tol, args.guess)
         else:
             continue
         forms[str(mode)] = {

@@ -21,10 +21,26 @@ struct RhsArgs {
 int launch_rhs_fourier(const Dims& d, const Metrics& M, const RhsArgs& a, hipStream_t st);
 int launch_contravariant(const Dims& d, const Metrics& M, const Ranges& r, const double* U2,
                          const double* U3, double* U2b, double* U3b, hipStream_t st);
-// lu: scratch of 7 * NZ2*NY2 doubles
-int launch_adi(const Dims& d, const Metrics& M, const Ranges& r, const BcTable& by,
-               const BcTable& bz, double t1, double* U, double* R, double* S, double* lu,
-               int th_corners, hipStream_t st);
+// One field of the batched ADI stage: its index ranges, the boundary rows of
+// its y and z systems, the field, its right-hand side, a scratch field S and
+// 7 * NZ2*NY2 doubles of line factors.
+struct AdiField {
+  Ranges r;
+  BcTable by, bz;
+  double* U;
+  double* R;
+  double* S;
+  double* lu;
+};
+constexpr int kAdiMaxFields = 4;
+struct AdiT1 {
+  double t1[kAdiMaxFields];  // implicit weight per field
+};
+// The ADI sweeps of nf independent fields in one set of launches; fs: nf
+// descriptors in device memory, hf the same on the host (for the grid sizes).
+// th_corners applies the scalar corner rows to field th_field afterwards.
+int launch_adi_batch(const Dims& d, const Metrics& M, const AdiField* fs, const AdiField* hf, int nf,
+                     const AdiT1& t1, int th_field, int th_corners, hipStream_t st);
 int launch_cfl(const Dims& d, const Metrics& M, const Ranges& r, const double* U1, const double* U2,
                const double* U3, double cfl, double dx, unsigned long long* out, hipStream_t st);
 int launch_faces(const Dims& d, const Metrics& M, const Ranges& r, const double* CU2, const double* CU3,

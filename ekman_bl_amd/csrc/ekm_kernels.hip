@@ -160,8 +160,13 @@ __global__ void k_contravariant(Dims d, Metrics M, Ranges r, const double* __res
 // ADI: explicit half before the y sweep (:1334-1371, :1640-1665, ...)
 // S = R - J U;  R = S/2 + J U + t1 [G11(K+1)(U(K+1)-U) - G11(K)(U-U(K-1))]
 // ---------------------------------------------------------------------------
-__global__ void k_adi_pre(Dims d, Metrics M, Ranges r, const double* __restrict__ U,
-                          double* __restrict__ R, double* __restrict__ S, double t1) {
+__global__ void k_adi_pre(Dims d, Metrics M, const AdiField* __restrict__ fs, AdiT1 t1s) {
+  const AdiField& F = fs[blockIdx.y];
+  const Ranges r = F.r;
+  const double t1 = t1s.t1[blockIdx.y];
+  const double* __restrict__ U = F.U;
+  double* __restrict__ R = F.R;
+  double* __restrict__ S = F.S;
   int nk = r.ze - r.zs + 1, nj = r.je - r.js + 1;
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t n = (size_t)d.NX2 * nk * nj;
@@ -187,11 +192,20 @@ __global__ void k_adi_pre(Dims d, Metrics M, Ranges r, const double* __restrict_
 // go back to memory with contiguous stores.
 constexpr int kLuBlock = 256;
 constexpr int kLuMaxRows = 1040;
-__global__ void __launch_bounds__(kLuBlock) k_lu(Dims d, Metrics M, Ranges r, BcTable bc, double t1, int dir,
-                                                  double* __restrict__ mult, double* __restrict__ piv,
-                                                  double* __restrict__ up, double* __restrict__ spa) {
+__global__ void __launch_bounds__(kLuBlock) k_lu(Dims d, Metrics M, const AdiField* __restrict__ fs, AdiT1 t1s,
+                                                  int dir) {
   __shared__ double sa[kLuMaxRows], sb[kLuMaxRows], sc[kLuMaxRows];
+  const AdiField& F = fs[blockIdx.y];
+  const Ranges r = F.r;
+  const BcTable& bc = dir == 0 ? F.by : F.bz;
+  const double t1 = t1s.t1[blockIdx.y];
+  const size_t ly = (size_t)d.NZ2 * d.NY2;
+  double* __restrict__ mult = F.lu + (dir == 0 ? 0 : 3 * ly);
+  double* __restrict__ piv = mult + ly;
+  double* __restrict__ up = mult + 2 * ly;
+  double* __restrict__ spa = F.lu + 6 * ly;
   const int line0 = dir == 0 ? r.zs : r.js;
+  if ((int)blockIdx.x > (dir == 0 ? r.ze - r.zs : r.je - r.js)) return;
   const int L = line0 + (int)blockIdx.x;
   const int N = dir == 0 ? d.NY + 1 : d.NZ + 1;  // last row index
   const int lo = dir == 0 ? r.js : r.zs, hi = dir == 0 ? r.je : r.ze;
@@ -271,10 +285,18 @@ __device__ inline double bc_rhs(const Dims& d, const Metrics& M, const BcRow& e,
 // steps, so the memory latency is paid once per block instead of per row.
 constexpr int kSw = 8;
 
-__global__ void k_ysweep(Dims d, Metrics M, Ranges r, BcTable bc, double t1,
-                         const double* __restrict__ mult, const double* __restrict__ piv,
-                         const double* __restrict__ up, double* __restrict__ U,
-                         double* __restrict__ R, const double* __restrict__ S) {
+__global__ void k_ysweep(Dims d, Metrics M, const AdiField* __restrict__ fs, AdiT1 t1s) {
+  const AdiField& F = fs[blockIdx.y];
+  const Ranges r = F.r;
+  const BcTable& bc = F.by;
+  const double t1 = t1s.t1[blockIdx.y];
+  const size_t ly = (size_t)d.NZ2 * d.NY2;
+  const double* __restrict__ mult = F.lu;
+  const double* __restrict__ piv = F.lu + ly;
+  const double* __restrict__ up = F.lu + 2 * ly;
+  double* __restrict__ U = F.U;
+  double* __restrict__ R = F.R;
+  const double* __restrict__ S = F.S;
   int nk = r.ze - r.zs + 1;
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)d.NX2 * nk) return;
@@ -338,10 +360,17 @@ __global__ void k_ysweep(Dims d, Metrics M, Ranges r, BcTable bc, double t1,
 }
 
 // z sweep for lines (i,J) (:1503-1596 and analogues)
-__global__ void k_zsweep(Dims d, Metrics M, Ranges r, BcTable bc, const double* __restrict__ mult,
-                         const double* __restrict__ piv, const double* __restrict__ up,
-                         const double* __restrict__ spa, double* __restrict__ U,
-                         const double* __restrict__ R) {
+__global__ void k_zsweep(Dims d, Metrics M, const AdiField* __restrict__ fs) {
+  const AdiField& F = fs[blockIdx.y];
+  const Ranges r = F.r;
+  const BcTable& bc = F.bz;
+  const size_t ly = (size_t)d.NZ2 * d.NY2;
+  const double* __restrict__ mult = F.lu + 3 * ly;
+  const double* __restrict__ piv = F.lu + 4 * ly;
+  const double* __restrict__ up = F.lu + 5 * ly;
+  const double* __restrict__ spa = F.lu + 6 * ly;
+  double* __restrict__ U = F.U;
+  const double* __restrict__ R = F.R;
   int nj = r.je - r.js + 1;
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)d.NX2 * nj) return;
@@ -686,30 +715,32 @@ int launch_contravariant(const Dims& d, const Metrics& M, const Ranges& r, const
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }
-int launch_adi(const Dims& d, const Metrics& M, const Ranges& r, const BcTable& by,
-               const BcTable& bz, double t1, double* U, double* R, double* S, double* lu,
-               int th_corners, hipStream_t st) {
-  size_t n = (size_t)d.NX2 * (r.ze - r.zs + 1) * (r.je - r.js + 1);
-  hipLaunchKernelGGL(k_adi_pre, dim3(nblk(n, 256)), dim3(256), 0, st, d, M, r, U, R, S, t1);
-  EKM_CHECK(hipGetLastError());
-  size_t ly = (size_t)d.NZ2 * d.NY2;  // (line, row) entries
-  double *my = lu, *py = lu + ly, *uy = lu + 2 * ly;
-  double *mz = lu + 3 * ly, *pz = lu + 4 * ly, *uz = lu + 5 * ly, *spa = lu + 6 * ly;
+int launch_adi_batch(const Dims& d, const Metrics& M, const AdiField* fs, const AdiField* hf, int nf,
+                     const AdiT1& t1, int th_field, int th_corners, hipStream_t st) {
+  if (nf < 1 || nf > kAdiMaxFields) return EKM_ERR_ARG;
   if (d.NY + 2 > kLuMaxRows || d.NZ + 2 > kLuMaxRows) return EKM_ERR_UNSUPPORTED;
-  hipLaunchKernelGGL(k_lu, dim3(r.ze - r.zs + 1), dim3(kLuBlock), 0, st, d, M, r, by, t1, 0, my, py, uy, spa);
+  size_t npre = 0, nly = 0, nlz = 0, nys = 0, nzs = 0;
+  for (int f = 0; f < nf; ++f) {
+    const Ranges& r = hf[f].r;
+    const size_t nk = (size_t)(r.ze - r.zs + 1), nj = (size_t)(r.je - r.js + 1);
+    npre = std::max(npre, (size_t)d.NX2 * nk * nj);
+    nly = std::max(nly, nk);
+    nlz = std::max(nlz, nj);
+    nys = std::max(nys, (size_t)d.NX2 * nk);
+    nzs = std::max(nzs, (size_t)d.NX2 * nj);
+  }
+  hipLaunchKernelGGL(k_adi_pre, dim3(nblk(npre, 256), nf), dim3(256), 0, st, d, M, fs, t1);
   EKM_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_lu, dim3(r.je - r.js + 1), dim3(kLuBlock), 0, st, d, M, r, bz, t1, 1, mz, pz, uz, spa);
+  hipLaunchKernelGGL(k_lu, dim3((unsigned)nly, nf), dim3(kLuBlock), 0, st, d, M, fs, t1, 0);
   EKM_CHECK(hipGetLastError());
-  size_t ny_ = (size_t)d.NX2 * (r.ze - r.zs + 1);
-  hipLaunchKernelGGL(k_ysweep, dim3(nblk(ny_, 256)), dim3(256), 0, st, d, M, r, by, t1, my, py, uy, U, R,
-                     (const double*)S);
+  hipLaunchKernelGGL(k_lu, dim3((unsigned)nlz, nf), dim3(kLuBlock), 0, st, d, M, fs, t1, 1);
   EKM_CHECK(hipGetLastError());
-  size_t nz_ = (size_t)d.NX2 * (r.je - r.js + 1);
-  hipLaunchKernelGGL(k_zsweep, dim3(nblk(nz_, 256)), dim3(256), 0, st, d, M, r, bz, mz, pz, uz,
-                     (const double*)spa, U, (const double*)R);
+  hipLaunchKernelGGL(k_ysweep, dim3(nblk(nys, 256), nf), dim3(256), 0, st, d, M, fs, t1);
   EKM_CHECK(hipGetLastError());
-  if (th_corners) {
-    hipLaunchKernelGGL(k_th_corners, dim3(nblk(d.NX2, 256)), dim3(256), 0, st, d, U, th_corners);
+  hipLaunchKernelGGL(k_zsweep, dim3(nblk(nzs, 256), nf), dim3(256), 0, st, d, M, fs);
+  EKM_CHECK(hipGetLastError());
+  if (th_corners && th_field >= 0 && th_field < nf) {
+    hipLaunchKernelGGL(k_th_corners, dim3(nblk(d.NX2, 256)), dim3(256), 0, st, d, hf[th_field].U, th_corners);
     EKM_CHECK(hipGetLastError());
   }
   return EKM_OK;

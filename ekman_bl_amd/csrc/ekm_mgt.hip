@@ -159,6 +159,7 @@ __device__ void zlines(const TLev<T>& L, T* __restrict__ W, const T* __restrict_
 // through LDS, and a last pass applies the entering values.
 constexpr int kYMaxTasks = 32;
 constexpr int kYUnroll = 4;
+constexpr int kYApply = 8;  // rows per batch of the last pass
 
 template <typename T, bool X>
 __device__ void ylines(const TLev<T>& L, T* __restrict__ W, const T* __restrict__ fac, int par, T* lds) {
@@ -291,9 +292,17 @@ __device__ void ylines(const TLev<T>& L, T* __restrict__ W, const T* __restrict_
     if (i >= nx) continue;
     const int j0 = ch * cl, j1 = min(j0 + cl, ny);
     const T xo = sv[task * 64 + lane];
-    for (int j = j0; j < j1; ++j) {
-      const int p = i + nx * j;
-      e[(i + 1) + ld * (j + 1)] = tA[p] + tB[p] * xo;
+    for (int jb = j0; jb < j1; jb += kYApply) {
+      T a_[kYApply], b_[kYApply];
+#pragma unroll
+      for (int u = 0; u < kYApply; ++u) {
+        const int j = min(jb + u, j1 - 1);
+        a_[u] = tA[i + nx * j];
+        b_[u] = tB[i + nx * j];
+      }
+#pragma unroll
+      for (int u = 0; u < kYApply; ++u)
+        if (jb + u < j1) e[(i + 1) + ld * (jb + u + 1)] = a_[u] + b_[u] * xo;
     }
   }
   __syncthreads();
@@ -309,7 +318,12 @@ __device__ void smooth(const TDesc<T>& g, const TLev<T>& L, T* W, const T* fac, 
   ylines<T, X>(L, W, fac, 1, lds);
 }
 
-// res = r - A e on level L (specified points have the row e = r = 0)
+// res = r - A e on level L (specified points have the row e = r = 0); each
+// thread issues the loads of kEw points before it uses any
+constexpr int kEw = 4;
+constexpr int kEwR = 2;   // the same for the residual (more loads per point)
+constexpr int kEw64 = 2;  // the same for the FP64 residual of the finest level
+
 template <typename T, bool X>
 __device__ void residual(const TLev<T>& L, T* W, const T* __restrict__ fac) {
   const int nx = L.nx, ld = nx + 2, n = L.n;
@@ -318,15 +332,24 @@ __device__ void residual(const TLev<T>& L, T* W, const T* __restrict__ fac) {
   const T* __restrict__ cf = L.cf;
   const T* __restrict__ c9 = fac + L.f_off + 6 * n;
   T* __restrict__ res = W + L.res_off;
-  for (int p = threadIdx.x; p < n; p += blockDim.x) {
-    const int j = p / nx, i = p - j * nx;
-    const int q = (i + 1) + ld * (j + 1);
-    T acc = (cf[0 * n + p] * e[q + 1] + cf[2 * n + p] * e[q + ld]) + (cf[4 * n + p] * e[q - 1] + cf[6 * n + p] * e[q - ld]);
-    if (X)
-      acc += (cf[1 * n + p] * e[q + 1 + ld] + cf[3 * n + p] * e[q - 1 + ld]) +
-             (cf[5 * n + p] * e[q - 1 - ld] + cf[7 * n + p] * e[q + 1 - ld]);
-    acc += c9[p] * e[q];
-    res[p] = r[p] - acc;
+  const int nt = blockDim.x;
+  for (int p0 = threadIdx.x; p0 < n; p0 += kEwR * nt) {
+    T v[kEwR];
+#pragma unroll
+    for (int u = 0; u < kEwR; ++u) {
+      const int p = min(p0 + u * nt, n - 1);
+      const int j = p / nx, i = p - j * nx;
+      const int q = (i + 1) + ld * (j + 1);
+      T acc = (cf[0 * n + p] * e[q + 1] + cf[2 * n + p] * e[q + ld]) + (cf[4 * n + p] * e[q - 1] + cf[6 * n + p] * e[q - ld]);
+      if (X)
+        acc += (cf[1 * n + p] * e[q + 1 + ld] + cf[3 * n + p] * e[q - 1 + ld]) +
+               (cf[5 * n + p] * e[q - 1 - ld] + cf[7 * n + p] * e[q + 1 - ld]);
+      acc += c9[p] * e[q];
+      v[u] = r[p] - acc;
+    }
+#pragma unroll
+    for (int u = 0; u < kEwR; ++u)
+      if (p0 + u * nt < n) res[p0 + u * nt] = v[u];
   }
   __syncthreads();
 }
@@ -341,19 +364,29 @@ __device__ void restrict_fw(const TLev<T>& F, const TLev<T>& C, T* W) {
   T* rc = W + C.r_off;
   T* ec = W + C.e_off;
   const T w[3] = {(T)0.25, (T)0.5, (T)0.25};
-  for (int pc = threadIdx.x; pc < C.n; pc += blockDim.x) {
-    const int jc = pc / C.nx, ic = pc - jc * C.nx;
-    T acc = (T)0;
-    if (!C.spec[pc]) {
+  const int nt = blockDim.x;
+  for (int p0 = threadIdx.x; p0 < C.n; p0 += kEwR * nt) {
+    T v[kEwR];
+#pragma unroll
+    for (int u = 0; u < kEwR; ++u) {
+      const int pc = min(p0 + u * nt, C.n - 1);
+      const int jc = pc / C.nx, ic = pc - jc * C.nx;
       const int i = sx * ic, j = sy * jc;
+      T acc = (T)0;
+#pragma unroll
       for (int b = -1; b <= 1; ++b)
+#pragma unroll
         for (int a = -1; a <= 1; ++a) {
           const int ii = i + a, jj = j + b;
-          if (ii < 0 || ii >= F.nx || jj < 0 || jj >= F.ny) continue;
-          acc += (w[a + 1] * w[b + 1]) * res[ii + F.nx * jj];
+          const bool in = ii >= 0 && ii < F.nx && jj >= 0 && jj < F.ny;
+          const T x = res[in ? ii + F.nx * jj : 0];
+          acc += in ? (w[a + 1] * w[b + 1]) * x : (T)0;
         }
+      v[u] = C.spec[pc] ? (T)0 : acc;
     }
-    rc[pc] = acc;
+#pragma unroll
+    for (int u = 0; u < kEwR; ++u)
+      if (p0 + u * nt < C.n) rc[p0 + u * nt] = v[u];
   }
   for (int t = threadIdx.x; t < (C.nx + 2) * (C.ny + 2); t += blockDim.x) ec[t] = (T)0;
   __syncthreads();
@@ -366,13 +399,25 @@ __device__ void prolong_add(const TLev<T>& C, const TLev<T>& F, T* W) {
   const T* ec = W + C.e_off;
   T* ef = W + F.e_off;
   const int lc = C.nx + 2, lf = F.nx + 2;
-  for (int p = threadIdx.x; p < F.n; p += blockDim.x) {
-    if (F.spec[p]) continue;
-    const int j = p / F.nx, i = p - j * F.nx;
-    const int i0 = i / sx, i1 = (i + sx - 1) / sx, j0 = j / sy, j1 = (j + sy - 1) / sy;
-    const T v = (T)0.25 * ((ec[(i0 + 1) + lc * (j0 + 1)] + ec[(i1 + 1) + lc * (j0 + 1)]) +
-                           (ec[(i0 + 1) + lc * (j1 + 1)] + ec[(i1 + 1) + lc * (j1 + 1)]));
-    ef[(i + 1) + lf * (j + 1)] += v;
+  const int nt = blockDim.x;
+  for (int p0 = threadIdx.x; p0 < F.n; p0 += kEw * nt) {
+    T v[kEw];
+    int qf[kEw];
+#pragma unroll
+    for (int u = 0; u < kEw; ++u) {
+      const int p = min(p0 + u * nt, F.n - 1);
+      const int j = p / F.nx, i = p - j * F.nx;
+      const int i0 = i / sx, i1 = (i + sx - 1) / sx, j0 = j / sy, j1 = (j + sy - 1) / sy;
+      qf[u] = (i + 1) + lf * (j + 1);
+      const T x = (T)0.25 * ((ec[(i0 + 1) + lc * (j0 + 1)] + ec[(i1 + 1) + lc * (j0 + 1)]) +
+                             (ec[(i0 + 1) + lc * (j1 + 1)] + ec[(i1 + 1) + lc * (j1 + 1)]));
+      v[u] = F.spec[p] ? (T)0 : x + ef[qf[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kEw; ++u) {
+      const int p = p0 + u * nt;
+      if (p < F.n && !F.spec[p]) ef[qf[u]] = v[u];
+    }
   }
   __syncthreads();
 }
@@ -404,11 +449,11 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
   const int s = blockIdx.x;
   (void)kx2v;  // the wavenumber enters through the per-mode tables
   T* W = ws + (size_t)s * g.ws;
-  double* phi = ws64 + (size_t)s * g.ws64;
+  double* __restrict__ phi = ws64 + (size_t)s * g.ws64;
   const int ng = g.ngrid;
   const TLev<T>& F = g.lev[ng];
   const int nx = F.nx, ny = F.ny, ld = nx + 2, n = F.n;
-  double* b = phi + (size_t)(nx + 2) * (ny + 2);
+  double* __restrict__ b = phi + (size_t)(nx + 2) * (ny + 2);
   const T* fac = g.fac + (size_t)(s >> 1) * g.fstride;
   const double* __restrict__ c964 = g.c964 + (size_t)(s >> 1) * n;
   const double* rt = rhs_tmp + (size_t)s * plane;
@@ -427,21 +472,30 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
   for (;;) {
     // FP64 residual of the finest level; its max decides; it seeds the cycle
     double rmax = 0.0;
-    T* rf = W + F.r_off;
-    T* ef = W + F.e_off;
-    for (int p = threadIdx.x; p < n; p += blockDim.x) {
-      const int j = p / nx, i = p - j * nx;
-      const int q = (i + 1) + ld * (j + 1);
-      const double* __restrict__ c = g.cf64;
-      double acc = (c[0 * n + p] * phi[q + 1] + c[2 * n + p] * phi[q + ld]) +
-                   (c[4 * n + p] * phi[q - 1] + c[6 * n + p] * phi[q - ld]);
-      if (X)
-        acc += (c[1 * n + p] * phi[q + 1 + ld] + c[3 * n + p] * phi[q - 1 + ld]) +
-               (c[5 * n + p] * phi[q - 1 - ld] + c[7 * n + p] * phi[q + 1 - ld]);
-      acc += c964[p] * phi[q];
-      const double v = b[p] - acc;
-      rmax = fmax(rmax, fabs(v));
-      rf[p] = (T)v;
+    T* __restrict__ rf = W + F.r_off;
+    T* __restrict__ ef = W + F.e_off;
+    for (int p0 = threadIdx.x; p0 < n; p0 += kEw64 * NT) {
+      double v[kEw64];
+#pragma unroll
+      for (int u = 0; u < kEw64; ++u) {
+        const int p = min(p0 + u * NT, n - 1);
+        const int j = p / nx, i = p - j * nx;
+        const int q = (i + 1) + ld * (j + 1);
+        const double* __restrict__ c = g.cf64;
+        double acc = (c[0 * n + p] * phi[q + 1] + c[2 * n + p] * phi[q + ld]) +
+                     (c[4 * n + p] * phi[q - 1] + c[6 * n + p] * phi[q - ld]);
+        if (X)
+          acc += (c[1 * n + p] * phi[q + 1 + ld] + c[3 * n + p] * phi[q - 1 + ld]) +
+                 (c[5 * n + p] * phi[q - 1 - ld] + c[7 * n + p] * phi[q + 1 - ld]);
+        acc += c964[p] * phi[q];
+        v[u] = b[p] - acc;
+      }
+#pragma unroll
+      for (int u = 0; u < kEw64; ++u)
+        if (p0 + u * NT < n) {
+          rmax = fmax(rmax, fabs(v[u]));
+          rf[p0 + u * NT] = (T)v[u];
+        }
     }
     for (int t = threadIdx.x; t < (nx + 2) * (ny + 2); t += blockDim.x) ef[t] = (T)0;
     red[threadIdx.x] = rmax;
@@ -472,7 +526,17 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
         restrict_fw<T>(g.lev[l], g.lev[l - 1], W);
       }
     }
-    for (int t = threadIdx.x; t < (nx + 2) * (ny + 2); t += blockDim.x) phi[t] += (double)ef[t];
+    for (int t0 = threadIdx.x; t0 < (nx + 2) * (ny + 2); t0 += 2 * kEw * NT) {
+      double v[2 * kEw];
+#pragma unroll
+      for (int u = 0; u < 2 * kEw; ++u) {
+        const int t = min(t0 + u * NT, (nx + 2) * (ny + 2) - 1);
+        v[u] = phi[t] + (double)ef[t];
+      }
+#pragma unroll
+      for (int u = 0; u < 2 * kEw; ++u)
+        if (t0 + u * NT < (nx + 2) * (ny + 2)) phi[t0 + u * NT] = v[u];
+    }
     __syncthreads();
     ++it;
   }
@@ -778,8 +842,8 @@ int tight_solve_ex(TightPlan& tp, MgPlan& mg, const Dims& d, int nsolve, double 
   // 16 wavefronts per workgroup where the line registers fit (FP32 up to 5
   // points per lane), else 8
 #define LAUNCH2(T, CM, X)                                                                               \
-  hipLaunchKernelGGL((k_tight_solve<T, CM, (sizeof(T) == 4 && CM <= 5) ? 1024 : 512, X>), grid,       \
-                     dim3((sizeof(T) == 4 && CM <= 5) ? 1024 : 512), 0, st, (const TDesc<T>*)tp.mem, (T*)tp.ws, tp.ws64, \
+  hipLaunchKernelGGL((k_tight_solve<T, CM, (sizeof(T) == 4 && CM <= 5 && !X) ? 1024 : 512, X>), grid, \
+                     dim3((sizeof(T) == 4 && CM <= 5 && !X) ? 1024 : 512), 0, st, (const TDesc<T>*)tp.mem, (T*)tp.ws, tp.ws64, \
                      (const double*)mg.rhs, mg.phi, (const double*)mg.kx2, mg.cycles, d.plane, guess)
 #define LAUNCH(T, CM)           \
   do {                          \

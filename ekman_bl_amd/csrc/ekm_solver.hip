@@ -41,7 +41,10 @@ struct ekm_solver {
   double* R[3] = {nullptr, nullptr, nullptr};
   double* RTH = nullptr;
   double *S1 = nullptr, *S2 = nullptr, *U2b = nullptr, *U3b = nullptr, *PHI = nullptr;
-  double* lu = nullptr;
+  double* lu = nullptr;     // line factors of the ADI fields, 7 * plane doubles each
+  AdiField adi_h[kAdiMaxFields];
+  AdiField* adi_d = nullptr;  // the same descriptors on the device
+  int adi_nf = 0;
   double* red = nullptr;  // [0] div sum, [1] div max (bits), [2] cfl min (bits)
   BcTable bcy[4], bcz[4];
   double time = 0.0, delta_t = 0.0, delta_t_set = 0.0;
@@ -142,6 +145,7 @@ static int free_all(ekm_solver* s) {
                  s->red, s->met, s->kx, s->kx2, s->kx2s, s->thbar};
   for (double* q : f)
     if (q) (void)hipFree(q);
+  if (s->adi_d) (void)hipFree(s->adi_d);
   fft_plan_free(s->fft);
   mg_plan_free(s->mg);
   tight_plan_free(s->tp);
@@ -228,7 +232,27 @@ extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const doub
     if (hipMalloc(q, d.npts * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
     if (hipMemset(*q, 0, d.npts * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
   }
-  if (hipMalloc(&s->lu, 7 * d.plane * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
+  if (hipMalloc(&s->lu, (size_t)kAdiMaxFields * 7 * d.plane * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
+  // ADI descriptors: U1, U2, U3 and theta; their scratch S fields are the
+  // buffers that are free between the fluxes and the projection
+  {
+    double* Sx[kAdiMaxFields] = {s->S1, s->S2, s->U2b, s->U3b};
+    const int nf = p.n_th ? 4 : 3;
+    for (int f = 0; f < nf; ++f) {
+      AdiField& a = s->adi_h[f];
+      const int c = f < 3 ? f : 3;
+      a.r = f < 3 ? s->rv : s->rt;
+      a.by = s->bcy[c];
+      a.bz = s->bcz[c];
+      a.U = f < 3 ? s->U[f] : s->TH;
+      a.R = f < 3 ? s->R[f] : s->RTH;
+      a.S = Sx[f];
+      a.lu = s->lu + (size_t)f * 7 * d.plane;
+    }
+    s->adi_nf = nf;
+    if (hipMalloc(&s->adi_d, sizeof(s->adi_h)) != hipSuccess) return fail(EKM_ERR_HIP);
+    if (hipMemcpy(s->adi_d, s->adi_h, sizeof(s->adi_h), hipMemcpyHostToDevice) != hipSuccess) return fail(EKM_ERR_HIP);
+  }
   if (hipMalloc(&s->red, 4 * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
   if (fft_plan_init(s->fft, p.nx) != EKM_OK) return fail(EKM_ERR_HIP);
   int nsolve = 2 * (d.NKX + 1);
@@ -438,12 +462,14 @@ extern "C" int ekm_rk_duct(ekm_solver* s) {
     TRY(launch_flux_accum(s->fft, d, s->M, s->rt, s->U[0], s->TH, s->U2b, s->U3b, p.nu / p.pr, s->kx, s->CFTH,
                           s->RTH, TEMP5, st));
     TRY(inv(s, s->RTH));
-    int corners = (p.th_bc[EKM_ZMAX] == 1 ? 1 : 0) | (p.th_bc[EKM_ZMIN] == 6 ? 2 : 0);
-    TRY(launch_adi(d, s->M, s->rt, s->bcy[3], s->bcz[3], TEMP1 / p.pr, s->TH, s->RTH, s->S1, s->lu, corners, st));
   }
-  // velocity ADI (:1640-2391)
-  for (int f = 0; f < 3; ++f)
-    TRY(launch_adi(d, s->M, s->rv, s->bcy[f], s->bcz[f], TEMP1, s->U[f], s->R[f], s->S1, s->lu, 0, st));
+  // ADI of the scalar (:1334-1625) and the velocities (:1640-2391): the four
+  // implicit solves are independent, so they share one set of launches
+  {
+    const int corners = th ? ((p.th_bc[EKM_ZMAX] == 1 ? 1 : 0) | (p.th_bc[EKM_ZMIN] == 6 ? 2 : 0)) : 0;
+    AdiT1 t1{{TEMP1, TEMP1, TEMP1, TEMP1 / p.pr}};
+    TRY(launch_adi_batch(d, s->M, s->adi_d, s->adi_h, s->adi_nf, t1, 3, corners, st));
+  }
   // CFL time step at the third substep (:2417-2419)
   if (p.variable_dt && rk == 3 && (cur_step % p.update_dt) == 0) {
     double init = 999.0;

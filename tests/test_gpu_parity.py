@@ -154,3 +154,33 @@ def test_consistent_projection_matches_oracle(mode, n, oracle_lib):
     assert ds.div_max < 1e-10
     assert max(r) < (5e-9 if mode == 1 else 1e-8)
     s.close()
+
+
+def test_warm_starts_agree():
+    """The warm start of the consistent solve (proj_guess 0: from zero, 1: the
+    previous substep's correction scaled by the RK weights) only changes where
+    the iteration starts: after three steps the velocities and theta agree to
+    the solve tolerance (relative L2 <= 1e-8 at max|residual| <= 1e-12) and
+    the pressure, which sums phi / h_bar over the substeps, to 1e-6."""
+    cfg, p, xp, yp = _setup(1)
+    p.proj_mode, p.proj_tol, p.proj_maxit = 2, 1e-12, 50
+    ic = ekman_initial(p, yp, cfg.seed, cfg.noise, cfg.th_noise)
+    states = {}
+    for g in (0, 1):
+        q = p.copy()
+        q.proj_guess = g
+        s = S.DuctSolver(q, xp, yp)
+        s.set_initial_physical(*ic)
+        cyc = []
+        for _ in range(9):
+            s.rk_duct()
+            d = s.diag()
+            assert d.div_max < 1e-10
+            cyc.append(d.mg_cycles)
+        states[g] = s.get_state()
+        print(f"[warm start] proj_guess {g}: cycles per substep {cyc}", flush=True)
+        s.close()
+    r = _state_rel(states[1], states[0])
+    print(f"[warm start] proj_guess 1 vs 0: {r}", flush=True)
+    assert max(r[:3] + r[4:]) < 1e-8
+    assert r[3] < 1e-6
