@@ -127,6 +127,7 @@ int mg_solve_ex(MgPlan& mg, const Dims& d, int nsolve, int iguess, hipStream_t s
 struct TightPlan {
   int fp32 = 0, cm = 0, nsolve_max = 0;
   bool cross = false;                 // operator has cross-derivative terms (9-point)
+  int lds_bytes = 0;                  // dynamic LDS of the solve kernel (coarse levels)
   long ws_elems = 0, ws64_elems = 0;  // per solve
   char* mem = nullptr;                // descriptor + coefficient tables
   void* fac = nullptr;                // line factors [live mode][...]

@@ -285,7 +285,7 @@ __device__ inline double bc_rhs(const Dims& d, const Metrics& M, const BcRow& e,
 // steps, so the memory latency is paid once per block instead of per row.
 constexpr int kSw = 8;
 
-__global__ void k_ysweep(Dims d, Metrics M, const AdiField* __restrict__ fs, AdiT1 t1s) {
+__global__ void __launch_bounds__(256) k_ysweep(Dims d, Metrics M, const AdiField* __restrict__ fs, AdiT1 t1s) {
   const AdiField& F = fs[blockIdx.y];
   const Ranges r = F.r;
   const BcTable& bc = F.by;
@@ -360,7 +360,7 @@ __global__ void k_ysweep(Dims d, Metrics M, const AdiField* __restrict__ fs, Adi
 }
 
 // z sweep for lines (i,J) (:1503-1596 and analogues)
-__global__ void k_zsweep(Dims d, Metrics M, const AdiField* __restrict__ fs) {
+__global__ void __launch_bounds__(256) k_zsweep(Dims d, Metrics M, const AdiField* __restrict__ fs) {
   const AdiField& F = fs[blockIdx.y];
   const Ranges r = F.r;
   const BcTable& bc = F.bz;

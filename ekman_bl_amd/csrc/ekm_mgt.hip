@@ -44,6 +44,7 @@ struct TLev {
   int nx, ny, n;
   int e_off, r_off, res_off;  // per-solve workspace offsets (elements of T)
   int t2_off;                 // second line-solve scratch (the residual array is the first)
+  int lds;                    // 1: the four arrays above live in the workgroup's LDS
   int f_off;                  // per-mode offset: z factors [3][n], y factors [3][n], c9 [n]
   const T* cf;                // c1..c8 planes [8][n]; rows of specified points are 0 (c9 = 1)
   const unsigned char* spec;  // specified (Dirichlet) points
@@ -64,6 +65,18 @@ struct TDesc {
   TLev<T> lev[kMaxLevT + 1];
 };
 
+// The per-solve workspace: global memory, and the LDS that holds the coarse
+// levels; a level's arrays are at its offsets in one of the two
+template <typename T>
+struct Ws {
+  T* g;
+  T* s;
+};
+template <typename T>
+__device__ inline T* wbase(const TLev<T>& L, const Ws<T>& W) {
+  return L.lds ? W.s : W.g;
+}
+
 // affine map x -> a*x + b; compose(outer, inner) = outer(inner(x))
 template <typename T>
 struct Aff {
@@ -74,13 +87,68 @@ __device__ inline Aff<T> compose(const Aff<T>& o, const Aff<T>& i) {
   return Aff<T>{o.a * i.a, o.a * i.b + o.b};
 }
 
-template <typename T>
-__device__ inline T shfl_up(T v, int d) {
-  return __shfl_up(v, d, 64);
+// Cross-lane moves of the wavefront scans.  dpp<CTRL, RM>(old, v): v of the
+// lane that the DPP control CTRL names, or `old` where that lane lies outside
+// the row or the row is masked out by RM (row_shr:n = 0x110+n, row_shl:n =
+// 0x100+n, row_bcast:15 = 0x142, row_bcast:31 = 0x143; 64-bit values move as
+// two 32-bit halves).
+template <int CTRL, int RM, typename T>
+__device__ inline T dpp(T old, T v) {
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, RM, 0xf, false));
+  } else {
+    const long long o = __double_as_longlong(old), x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)x, CTRL, RM, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(x >> 32), CTRL, RM, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
 }
 template <typename T>
-__device__ inline T shfl_down(T v, int d) {
-  return __shfl_down(v, d, 64);
+__device__ inline T rdlane(T v, int l) {
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+  } else {
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)x, l), hi = __builtin_amdgcn_readlane((int)(x >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
+}
+template <int CTRL, int RM, typename T>
+__device__ inline Aff<T> dpp_aff(const Aff<T>& v) {
+  return Aff<T>{dpp<CTRL, RM>((T)1, v.a), dpp<CTRL, RM>((T)0, v.b)};
+}
+
+// inclusive scan over the lanes in increasing order: lane i gets
+// map_i o map_{i-1} o ... o map_0 (in-row doubling steps, then the row
+// prefixes carried by row_bcast:15 to rows 1 and 3 and row_bcast:31 to rows
+// 2 and 3)
+template <typename T>
+__device__ inline Aff<T> wave_scan_up(Aff<T> v) {
+  v = compose(v, dpp_aff<0x111, 0xf>(v));
+  v = compose(v, dpp_aff<0x112, 0xf>(v));
+  v = compose(v, dpp_aff<0x114, 0xf>(v));
+  v = compose(v, dpp_aff<0x118, 0xf>(v));
+  v = compose(v, dpp_aff<0x142, 0xa>(v));
+  v = compose(v, dpp_aff<0x143, 0xc>(v));
+  return v;
+}
+
+// inclusive scan in decreasing order: lane i gets map_i o ... o map_63
+// (in-row doubling steps, then the suffix maps of the rows above, read from
+// the first lane of each row)
+template <typename T>
+__device__ inline Aff<T> wave_scan_down(Aff<T> v) {
+  v = compose(v, dpp_aff<0x101, 0xf>(v));
+  v = compose(v, dpp_aff<0x102, 0xf>(v));
+  v = compose(v, dpp_aff<0x104, 0xf>(v));
+  v = compose(v, dpp_aff<0x108, 0xf>(v));
+  const Aff<T> r1{rdlane(v.a, 16), rdlane(v.b, 16)};
+  const Aff<T> r2{rdlane(v.a, 32), rdlane(v.b, 32)};
+  const Aff<T> r3{rdlane(v.a, 48), rdlane(v.b, 48)};
+  const Aff<T> s2 = r3, s1 = compose(r2, r3), s0 = compose(r1, s1);
+  const int row = (threadIdx.x & 63) >> 4;
+  const Aff<T> tail = row == 0 ? s0 : row == 1 ? s1 : row == 2 ? s2 : Aff<T>{(T)1, (T)0};
+  return compose(v, tail);
 }
 
 // Zebra line relaxation of level L.  The Thomas factors of every line are
@@ -93,7 +161,8 @@ __device__ inline T shfl_down(T v, int d) {
 // store of a stripe is one contiguous run; within a stripe the recurrence is a
 // 64-lane scan of affine maps, carried from stripe to stripe.
 template <typename T, int CM, bool X>
-__device__ void zlines(const TLev<T>& L, T* __restrict__ W, const T* __restrict__ fac, int par) {
+__device__ void zlines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ fac, int par) {
+  T* W = wbase(L, Wp);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int nx = L.nx, ny = L.ny, ld = nx + 2, n = L.n;
   T* __restrict__ e = W + L.e_off;
@@ -125,28 +194,18 @@ __device__ void zlines(const TLev<T>& L, T* __restrict__ W, const T* __restrict_
     T carry = (T)0;  // dp at the last point of the previous stripe
 #pragma unroll
     for (int c = 0; c < CM; ++c) {
-      Aff<T> inc{ak[c], bk[c]};
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        Aff<T> prev{shfl_up(inc.a, d), shfl_up(inc.b, d)};
-        if (lane >= d) inc = compose(inc, prev);
-      }
+      const Aff<T> inc = wave_scan_up(Aff<T>{ak[c], bk[c]});
       dk[c] = inc.a * carry + inc.b;
-      carry = __shfl(dk[c], 63, 64);
+      carry = rdlane(dk[c], 63);
     }
     T xo = (T)0;  // x at the first point of the next stripe
 #pragma unroll
     for (int c = CM - 1; c >= 0; --c) {
-      Aff<T> inc{-ck[c], dk[c]};
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        Aff<T> nxt{shfl_down(inc.a, d), shfl_down(inc.b, d)};
-        if (lane + d < 64) inc = compose(inc, nxt);
-      }
+      const Aff<T> inc = wave_scan_down(Aff<T>{-ck[c], dk[c]});
       const T x = inc.a * xo + inc.b;
       const int i = 64 * c + lane;
       if (i < nx) e[(i + 1) + ld * (j + 1)] = x;
-      xo = __shfl(x, 0, 64);
+      xo = rdlane(x, 0);
     }
   }
 }
@@ -158,11 +217,12 @@ __device__ void zlines(const TLev<T>& L, T* __restrict__ W, const T* __restrict_
 // (stored per point in the two scratch arrays), the chunk maps are combined
 // through LDS, and a last pass applies the entering values.
 constexpr int kYMaxTasks = 32;
-constexpr int kYUnroll = 4;
+constexpr int kYUnroll = 3;
 constexpr int kYApply = 8;  // rows per batch of the last pass
 
 template <typename T, bool X>
-__device__ void ylines(const TLev<T>& L, T* __restrict__ W, const T* __restrict__ fac, int par, T* lds) {
+__device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ fac, int par, T* lds) {
+  T* W = wbase(L, Wp);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int nx = L.nx, ny = L.ny, ld = nx + 2, n = L.n;
   T* __restrict__ e = W + L.e_off;
@@ -309,7 +369,7 @@ __device__ void ylines(const TLev<T>& L, T* __restrict__ W, const T* __restrict_
 }
 
 template <typename T, int CM, bool X>
-__device__ void smooth(const TDesc<T>& g, const TLev<T>& L, T* W, const T* fac, T* lds) {
+__device__ void smooth(const TDesc<T>& g, const TLev<T>& L, const Ws<T>& W, const T* fac, T* lds) {
   zlines<T, CM, X>(L, W, fac, 0);
   __syncthreads();
   zlines<T, CM, X>(L, W, fac, 1);
@@ -325,7 +385,8 @@ constexpr int kEwR = 2;   // the same for the residual (more loads per point)
 constexpr int kEw64 = 2;  // the same for the FP64 residual of the finest level
 
 template <typename T, bool X>
-__device__ void residual(const TLev<T>& L, T* W, const T* __restrict__ fac) {
+__device__ void residual(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ fac) {
+  T* W = wbase(L, Wp);
   const int nx = L.nx, ld = nx + 2, n = L.n;
   const T* __restrict__ e = W + L.e_off;
   const T* __restrict__ r = W + L.r_off;
@@ -358,11 +419,11 @@ __device__ void residual(const TLev<T>& L, T* W, const T* __restrict__ fac) {
 // outside the grid; a direction that is not coarsened keeps its index), and
 // C's correction cleared
 template <typename T>
-__device__ void restrict_fw(const TLev<T>& F, const TLev<T>& C, T* W) {
+__device__ void restrict_fw(const TLev<T>& F, const TLev<T>& C, const Ws<T>& W) {
   const int sx = C.nx < F.nx ? 2 : 1, sy = C.ny < F.ny ? 2 : 1;
-  const T* res = W + F.res_off;
-  T* rc = W + C.r_off;
-  T* ec = W + C.e_off;
+  const T* __restrict__ res = wbase(F, W) + F.res_off;
+  T* __restrict__ rc = wbase(C, W) + C.r_off;
+  T* __restrict__ ec = wbase(C, W) + C.e_off;
   const T w[3] = {(T)0.25, (T)0.5, (T)0.25};
   const int nt = blockDim.x;
   for (int p0 = threadIdx.x; p0 < C.n; p0 += kEwR * nt) {
@@ -394,10 +455,10 @@ __device__ void restrict_fw(const TLev<T>& F, const TLev<T>& C, T* W) {
 
 // e_F += bilinear interpolation of e_C (not at specified points)
 template <typename T>
-__device__ void prolong_add(const TLev<T>& C, const TLev<T>& F, T* W) {
+__device__ void prolong_add(const TLev<T>& C, const TLev<T>& F, const Ws<T>& W) {
   const int sx = C.nx < F.nx ? 2 : 1, sy = C.ny < F.ny ? 2 : 1;
-  const T* ec = W + C.e_off;
-  T* ef = W + F.e_off;
+  const T* __restrict__ ec = wbase(C, W) + C.e_off;
+  T* __restrict__ ef = wbase(F, W) + F.e_off;
   const int lc = C.nx + 2, lf = F.nx + 2;
   const int nt = blockDim.x;
   for (int p0 = threadIdx.x; p0 < F.n; p0 += kEw * nt) {
@@ -424,7 +485,8 @@ __device__ void prolong_add(const TLev<T>& C, const TLev<T>& F, T* W) {
 
 // e = A^-1 r on the coarsest level of the cycle (A^-1 precomputed per mode)
 template <typename T>
-__device__ void direct_solve(const TLev<T>& L, T* W, const T* __restrict__ ainv) {
+__device__ void direct_solve(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ ainv) {
+  T* W = wbase(L, Wp);
   const int n = L.n, nx = L.nx, ld = nx + 2;
   const T* __restrict__ r = W + L.r_off;
   T* e = W + L.e_off;
@@ -442,13 +504,22 @@ template <typename T, int CM, int NT, bool X>
 __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__ gp, T* __restrict__ ws,
                                                      double* __restrict__ ws64, const double* __restrict__ rhs_tmp,
                                                      double* __restrict__ phi_out, const double* __restrict__ kx2v,
-                                                     int* __restrict__ cycles, size_t plane, double guess) {
+                                                     int* __restrict__ cycles, size_t plane, double guess,
+                                                     int nsolve) {
   __shared__ double red[NT];
   __shared__ T ylds[3 * kYMaxTasks * 64];
+  extern __shared__ __align__(16) unsigned char tight_lds[];
   const TDesc<T>& g = *gp;
-  const int s = blockIdx.x;
+  // workgroups are dealt to the 8 XCDs round robin: solves 2m and 2m+1 (the
+  // two parts of mode m, which read the same per-mode tables) are placed on
+  // one XCD so the second reader finds the tables in that XCD's L2.  The grid
+  // is a multiple of 16; the map is a bijection on it.
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int s = 2 * ((slot >> 1) * 8 + xcd) + (slot & 1);
+  if (s >= nsolve) return;
   (void)kx2v;  // the wavenumber enters through the per-mode tables
   T* W = ws + (size_t)s * g.ws;
+  const Ws<T> Wp{W, reinterpret_cast<T*>(tight_lds)};
   double* __restrict__ phi = ws64 + (size_t)s * g.ws64;
   const int ng = g.ngrid;
   const TLev<T>& F = g.lev[ng];
@@ -512,18 +583,18 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
     // call site of the smoother
     const int lc = g.lc, nsm = 2 * (ng - lc);
     const T* ainv = fac + g.ainv_off;
-    if (nsm == 0) direct_solve<T>(g.lev[lc], W, ainv);
+    if (nsm == 0) direct_solve<T>(g.lev[lc], Wp, ainv);
     for (int k = 0; k < nsm; ++k) {
       const bool down = k < ng - lc;
       const int l = down ? ng - k : lc + 1 + (k - (ng - lc));
       if (!down) {
-        if (l == lc + 1) direct_solve<T>(g.lev[lc], W, ainv);
-        prolong_add<T>(g.lev[l - 1], g.lev[l], W);
+        if (l == lc + 1) direct_solve<T>(g.lev[lc], Wp, ainv);
+        prolong_add<T>(g.lev[l - 1], g.lev[l], Wp);
       }
-      smooth<T, CM, X>(g, g.lev[l], W, fac, ylds);
+      smooth<T, CM, X>(g, g.lev[l], Wp, fac, ylds);
       if (down) {
-        residual<T, X>(g.lev[l], W, fac);
-        restrict_fw<T>(g.lev[l], g.lev[l - 1], W);
+        residual<T, X>(g.lev[l], Wp, fac);
+        restrict_fw<T>(g.lev[l], g.lev[l - 1], Wp);
       }
     }
     for (int t0 = threadIdx.x; t0 < (nx + 2) * (ny + 2); t0 += 2 * kEw * NT) {
@@ -617,15 +688,6 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
     L.nx = nx;
     L.ny = ny;
     L.n = nx * ny;
-    L.e_off = (int)ws;
-    ws += (long)(nx + 2) * (ny + 2);
-    L.r_off = (int)ws;
-    ws += L.n;
-    L.res_off = (int)ws;
-    ws += L.n;
-    L.t2_off = (int)ws;
-    ws += L.n;
-    ws = (ws + 31) & ~31L;
     L.f_off = (int)fstride;
     fstride += 7L * L.n;  // z factors, y factors, c9
   }
@@ -636,7 +698,6 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
   g.ainv_off = fstride;
   fstride += (long)g.lev[lc].n * g.lev[lc].n;
   fstride = (fstride + 63) & ~63L;
-  g.ws = ws;
   g.ws64 = (long)(d.NZ2 + 2) * (d.NY2 + 2) + (long)d.NZ2 * d.NY2;
   g.ws64 = (g.ws64 + 31) & ~31L;
   g.fstride = fstride;
@@ -699,6 +760,37 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
       }
   }
   tp.cross = cross;
+  // workspace: from the coarsest level up, the levels whose correction,
+  // right-hand side and two scratch arrays fit in the LDS left beside the
+  // kernel's static arrays live there; the others (always the finest) in
+  // global memory, one slice per solve
+  {
+    const int nt = (sizeof(T) == 4 && tp.cm <= 5 && !cross) ? 1024 : 512;
+    const long stat = (long)nt * 8 + 3L * kYMaxTasks * 64 * (long)sizeof(T);
+    int dev = 0, maxsh = 0;
+    EKM_CHECK(hipGetDevice(&dev));
+    EKM_CHECK(hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    long budget = (long)maxsh - stat - 256;
+    if (const char* e = getenv("EKM_TIGHT_LDS_BYTES")) budget = std::min(budget, atol(e));
+    long lds = 0;
+    bool fits = true;
+    for (int k = 1; k <= ng; ++k) {
+      TLev<T>& L = g.lev[k];
+      const long need = (((long)(L.nx + 2) * (L.ny + 2) + 3L * L.n + 31) & ~31L) * (long)sizeof(T);
+      fits = fits && k < ng && lds + need <= budget;
+      long& off = fits ? lds : ws;
+      const long base = fits ? off / (long)sizeof(T) : off;
+      L.lds = fits ? 1 : 0;
+      L.e_off = (int)base;
+      L.r_off = (int)(base + (long)(L.nx + 2) * (L.ny + 2));
+      L.res_off = L.r_off + L.n;
+      L.t2_off = L.res_off + L.n;
+      const long elems = ((long)(L.nx + 2) * (L.ny + 2) + 3L * L.n + 31) & ~31L;
+      off += fits ? elems * (long)sizeof(T) : elems;
+    }
+    tp.lds_bytes = (int)lds;
+    g.ws = ws;
+  }
   // coefficient tables, and per live mode: line factors, c9, coarsest inverse
   const size_t nfine = (size_t)g.lev[ng].n;
   std::vector<T> cfT(8 * tot_n);
@@ -838,13 +930,19 @@ void tight_plan_free(TightPlan& tp) {
 int tight_solve_ex(TightPlan& tp, MgPlan& mg, const Dims& d, int nsolve, double guess, hipStream_t st) {
   if (nsolve <= 0) return EKM_OK;
   if (nsolve > tp.nsolve_max) return EKM_ERR_ARG;
-  const dim3 grid(nsolve);
+  // a multiple of 16 workgroups for the XCD placement of the solve pairs
+  const dim3 grid((unsigned)((nsolve + 15) & ~15));
   // 16 wavefronts per workgroup where the line registers fit (FP32 up to 5
   // points per lane), else 8
-#define LAUNCH2(T, CM, X)                                                                               \
-  hipLaunchKernelGGL((k_tight_solve<T, CM, (sizeof(T) == 4 && CM <= 5 && !X) ? 1024 : 512, X>), grid, \
-                     dim3((sizeof(T) == 4 && CM <= 5 && !X) ? 1024 : 512), 0, st, (const TDesc<T>*)tp.mem, (T*)tp.ws, tp.ws64, \
-                     (const double*)mg.rhs, mg.phi, (const double*)mg.kx2, mg.cycles, d.plane, guess)
+#define LAUNCH2(T, CM, X)                                                                                \
+  do {                                                                                                   \
+    constexpr int NT = (sizeof(T) == 4 && CM <= 5 && !X) ? 1024 : 512;                                   \
+    EKM_CHECK(hipFuncSetAttribute((const void*)k_tight_solve<T, CM, NT, X>,                             \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, tp.lds_bytes));            \
+    hipLaunchKernelGGL((k_tight_solve<T, CM, NT, X>), grid, dim3(NT), tp.lds_bytes, st, (const TDesc<T>*)tp.mem, \
+                       (T*)tp.ws, tp.ws64, (const double*)mg.rhs, mg.phi, (const double*)mg.kx2, mg.cycles, \
+                       d.plane, guess, nsolve);                                                          \
+  } while (0)
 #define LAUNCH(T, CM)           \
   do {                          \
     if (tp.cross)               \
