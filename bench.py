@@ -52,12 +52,13 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_form(cfg, xp, yp, ic, mode: int, steps: int, warmup: int, tol: float, guess: int = 0):
+def run_form(cfg, xp, yp, ic, mode: int, steps: int, warmup: int, tol: float, guess: int = 0, krylov: int = 1):
     p = cfg.params.copy()
     p.proj_mode = mode
     p.proj_tol = tol
     p.proj_maxit = 50
     p.proj_guess = guess if mode > 0 else 0
+    p.proj_krylov = krylov if mode > 0 else 0
     s = S.DuctSolver(p, xp, yp)
     s.set_initial_physical(*ic)
     for _ in range(warmup):
@@ -119,6 +120,8 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=1e-12, help="max |residual| per real solve, modes 1 and 2")
     ap.add_argument("--guess", type=int, default=0, choices=(0, 1),
                     help="warm start of modes 1 and 2 (0: from zero, 1: previous substep's correction)")
+    ap.add_argument("--krylov", type=int, default=1, choices=(0, 1),
+                    help="outer iteration of modes 1 and 2 (0: plain refinement, 1: GCR(1))")
     ap.add_argument("--side-steps", type=int, default=5, help="timed steps of the other forms (0: skip)")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-oracle work")
     ap.add_argument("--no-cpu", action="store_true")
@@ -147,7 +150,7 @@ def main() -> int:
         L.ekm_dev_free(b)
         log(f"copy kernel: {n * 8} bytes read, {n * 8} bytes written")
     log(f"[bench] {cfg.name}: headline form {args.mode} ({FORM_NAMES[args.mode]}), {args.steps} steps")
-    h = run_form(cfg, xp, yp, ic, args.mode, args.steps, args.warmup, args.tol, args.guess)
+    h = run_form(cfg, xp, yp, ic, args.mode, args.steps, args.warmup, args.tol, args.guess, args.krylov)
     value = p.npts * args.steps / h["el"] / 1e6
     ms_per_step = h["el"] / args.steps * 1e3
     nsolve = 2 * (p.nx // 3 + 1)
@@ -172,7 +175,7 @@ def main() -> int:
             r = h
         elif args.side_steps > 0:
             log(f"[bench] form {mode} ({FORM_NAMES[mode]}), {args.side_steps} steps")
-            r = run_form(cfg, xp, yp, ic, mode, args.side_steps, 1, args.tol, args.guess)
+            r = run_form(cfg, xp, yp, ic, mode, args.side_steps, 1, args.tol, args.guess, args.krylov)
         else:
             continue
         forms[str(mode)] = {

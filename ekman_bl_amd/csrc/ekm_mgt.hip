@@ -38,6 +38,7 @@ namespace {
 
 constexpr int kMaxLevT = 16;
 constexpr int kDirectMax = 128;  // the coarse level solved directly has at most this many points
+constexpr int kMaxProg = 4 * kMaxLevT + 4;
 
 template <typename T>
 struct TLev {
@@ -55,13 +56,19 @@ struct TDesc {
   int ngrid, nfx, nfy, maxit;
   double tol;
   long ws;          // T elements per solve
-  long ws64;        // doubles per solve: phi (ring) then b
+  long ws64;        // doubles per solve: phi (ring), b; with krylov also r, q, Qp [n] and Zp (ring)
   long fstride;     // factor elements per mode
   int lc;           // coarsest level of the cycle, solved with its inverse
   long ainv_off;    // per-mode offset of that inverse (n_lc x n_lc, row-major)
   const T* fac;     // [mode][fstride]
   const double* cf64;  // finest level c1..c8 in double [8][n]
   const double* c964;  // finest level c9 in double, [mode][n]
+  // the V(1,1) cycle as a program, one step per byte: (level << 2) | op,
+  // op 0 = smooth, residual and restriction to the next coarser level,
+  // 1 = prolongation from the next coarser level and smooth, 2 = direct solve
+  int krylov;  // 1: GCR(1) outer iteration (MG-preconditioned minimal residual with one kept direction)
+  int nprog;
+  unsigned char prog[kMaxProg];
   TLev<T> lev[kMaxLevT + 1];
 };
 
@@ -382,7 +389,8 @@ __device__ void smooth(const TDesc<T>& g, const TLev<T>& L, const Ws<T>& W, cons
 // thread issues the loads of kEw points before it uses any
 constexpr int kEw = 4;
 constexpr int kEwR = 2;   // the same for the residual (more loads per point)
-constexpr int kEw64 = 2;  // the same for the FP64 residual of the finest level
+constexpr int kEw64 = 2;
+constexpr int kEwQ = 1;   // the same for A z of the Krylov step  // the same for the FP64 residual of the finest level
 
 template <typename T, bool X>
 __device__ void residual(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ fac) {
@@ -500,6 +508,23 @@ __device__ void direct_solve(const TLev<T>& L, const Ws<T>& Wp, const T* __restr
   __syncthreads();
 }
 
+// sums of four values over the workgroup (wavefront sums, then one value per
+// wavefront through `red`); every thread gets the totals
+__device__ inline void block_sum4(double* v, double* red) {
+  for (int k = 0; k < 4; ++k)
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane == 0)
+    for (int k = 0; k < 4; ++k) red[4 * wave + k] = v[k];
+  __syncthreads();
+  for (int k = 0; k < 4; ++k) {
+    double a = 0.0;
+    for (int w = 0; w < nw; ++w) a += red[4 * w + k];
+    v[k] = a;
+  }
+  __syncthreads();
+}
+
 template <typename T, int CM, int NT, bool X>
 __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__ gp, T* __restrict__ ws,
                                                      double* __restrict__ ws64, const double* __restrict__ rhs_tmp,
@@ -539,36 +564,99 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
   }
   for (int p = threadIdx.x; p < n; p += blockDim.x) b[p] = F.spec[p] ? 0.0 : -rt[p];
   __syncthreads();
+  const int P = (nx + 2) * (ny + 2);
+  double* __restrict__ rr = b + n;   // krylov: recursive residual
+  double* __restrict__ qv = rr + n;  // A z
+  double* __restrict__ Qp = qv + n;  // kept direction's A z
+  double* __restrict__ Zp = Qp + n;  // kept direction (ring layout)
+  T* __restrict__ rf = W + F.r_off;
+  T* __restrict__ ef = W + F.e_off;
+  const double* __restrict__ c = g.cf64;
   int it = 0;
+  __shared__ double sQQ;  // Qp . Qp of the kept direction
   for (;;) {
-    // FP64 residual of the finest level; its max decides; it seeds the cycle
     double rmax = 0.0;
-    T* __restrict__ rf = W + F.r_off;
-    T* __restrict__ ef = W + F.e_off;
-    for (int p0 = threadIdx.x; p0 < n; p0 += kEw64 * NT) {
-      double v[kEw64];
+    if (!g.krylov || it == 0) {
+      // FP64 residual of the finest level; its max decides; it seeds the cycle
+      for (int p0 = threadIdx.x; p0 < n; p0 += kEw64 * NT) {
+        double v[kEw64];
 #pragma unroll
-      for (int u = 0; u < kEw64; ++u) {
-        const int p = min(p0 + u * NT, n - 1);
-        const int j = p / nx, i = p - j * nx;
-        const int q = (i + 1) + ld * (j + 1);
-        const double* __restrict__ c = g.cf64;
-        double acc = (c[0 * n + p] * phi[q + 1] + c[2 * n + p] * phi[q + ld]) +
-                     (c[4 * n + p] * phi[q - 1] + c[6 * n + p] * phi[q - ld]);
-        if (X)
-          acc += (c[1 * n + p] * phi[q + 1 + ld] + c[3 * n + p] * phi[q - 1 + ld]) +
-                 (c[5 * n + p] * phi[q - 1 - ld] + c[7 * n + p] * phi[q + 1 - ld]);
-        acc += c964[p] * phi[q];
-        v[u] = b[p] - acc;
-      }
-#pragma unroll
-      for (int u = 0; u < kEw64; ++u)
-        if (p0 + u * NT < n) {
-          rmax = fmax(rmax, fabs(v[u]));
-          rf[p0 + u * NT] = (T)v[u];
+        for (int u = 0; u < kEw64; ++u) {
+          const int p = min(p0 + u * NT, n - 1);
+          const int j = p / nx, i = p - j * nx;
+          const int q = (i + 1) + ld * (j + 1);
+          double acc = (c[0 * n + p] * phi[q + 1] + c[2 * n + p] * phi[q + ld]) +
+                       (c[4 * n + p] * phi[q - 1] + c[6 * n + p] * phi[q - ld]);
+          if (X)
+            acc += (c[1 * n + p] * phi[q + 1 + ld] + c[3 * n + p] * phi[q - 1 + ld]) +
+                   (c[5 * n + p] * phi[q - 1 - ld] + c[7 * n + p] * phi[q + 1 - ld]);
+          acc += c964[p] * phi[q];
+          v[u] = b[p] - acc;
         }
+#pragma unroll
+        for (int u = 0; u < kEw64; ++u)
+          if (p0 + u * NT < n) {
+            rmax = fmax(rmax, fabs(v[u]));
+            rf[p0 + u * NT] = (T)v[u];
+            if (g.krylov) rr[p0 + u * NT] = v[u];
+          }
+      }
+      for (int t = threadIdx.x; t < P; t += blockDim.x) ef[t] = (T)0;
+    } else {
+      // GCR(1): z = the cycle's correction, q = A z in FP64; q and z are
+      // made A-orthogonal to the kept direction (beta), then the minimal
+      // residual step alpha = (r.q')/(q'.q') updates phi and r
+      const bool hasq = it > 1;
+      double sm[4] = {0.0, 0.0, 0.0, 0.0};  // q.Qp, q.q, r.q, r.Qp
+      for (int p0 = threadIdx.x; p0 < n; p0 += kEwQ * NT) {
+        double v[kEwQ];
+#pragma unroll
+        for (int u = 0; u < kEwQ; ++u) {
+          const int p = min(p0 + u * NT, n - 1);
+          const int j = p / nx, i = p - j * nx;
+          const int q = (i + 1) + ld * (j + 1);
+          double acc = (c[0 * n + p] * (double)ef[q + 1] + c[2 * n + p] * (double)ef[q + ld]) +
+                       (c[4 * n + p] * (double)ef[q - 1] + c[6 * n + p] * (double)ef[q - ld]);
+          if (X)
+            acc += (c[1 * n + p] * (double)ef[q + 1 + ld] + c[3 * n + p] * (double)ef[q - 1 + ld]) +
+                   (c[5 * n + p] * (double)ef[q - 1 - ld] + c[7 * n + p] * (double)ef[q + 1 - ld]);
+          v[u] = acc + c964[p] * (double)ef[q];
+        }
+#pragma unroll
+        for (int u = 0; u < kEwQ; ++u) {
+          const int p = p0 + u * NT;
+          if (p < n) {
+            const double qp = v[u], rp = rr[p], qk = hasq ? Qp[p] : 0.0;
+            qv[p] = qp;
+            sm[0] += qp * qk;
+            sm[1] += qp * qp;
+            sm[2] += rp * qp;
+            sm[3] += rp * qk;
+          }
+        }
+      }
+      block_sum4(sm, red);
+      const double QQ = hasq ? sQQ : 0.0;
+      const double beta = (hasq && QQ > 0.0) ? sm[0] / QQ : 0.0;
+      const double qq = sm[1] - 2.0 * beta * sm[0] + beta * beta * QQ;
+      const double alpha = qq > 0.0 ? (sm[2] - beta * sm[3]) / qq : 0.0;
+      __syncthreads();  // every thread has read the previous value
+      if (threadIdx.x == 0) sQQ = qq;
+      for (int t = threadIdx.x; t < P; t += blockDim.x) {
+        const double z = (double)ef[t] - (hasq ? beta * Zp[t] : 0.0);
+        phi[t] += alpha * z;
+        Zp[t] = z;
+        ef[t] = (T)0;
+      }
+      for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        const double q2 = qv[p] - (hasq ? beta * Qp[p] : 0.0);
+        const double r2 = rr[p] - alpha * q2;
+        Qp[p] = q2;
+        rr[p] = r2;
+        rf[p] = (T)r2;
+        rmax = fmax(rmax, fabs(r2));
+      }
     }
-    for (int t = threadIdx.x; t < (nx + 2) * (ny + 2); t += blockDim.x) ef[t] = (T)0;
     red[threadIdx.x] = rmax;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
@@ -578,35 +666,35 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
     rmax = red[0];
     __syncthreads();
     if (rmax <= g.tol || it >= g.maxit) break;
-    // one V(1,1) cycle on the correction equation: smoothing steps walked in
-    // order (down: ng..lc+1, then the direct solve on lc, up: lc+1..ng), one
-    // call site of the smoother
-    const int lc = g.lc, nsm = 2 * (ng - lc);
+    // one cycle on the correction equation: the steps of the mode's program
+    // in order, one call site of the smoother
     const T* ainv = fac + g.ainv_off;
-    if (nsm == 0) direct_solve<T>(g.lev[lc], Wp, ainv);
-    for (int k = 0; k < nsm; ++k) {
-      const bool down = k < ng - lc;
-      const int l = down ? ng - k : lc + 1 + (k - (ng - lc));
-      if (!down) {
-        if (l == lc + 1) direct_solve<T>(g.lev[lc], Wp, ainv);
-        prolong_add<T>(g.lev[l - 1], g.lev[l], Wp);
+    for (int k = 0; k < g.nprog; ++k) {
+      const int step = g.prog[k];
+      const int l = step >> 2, op = step & 3;
+      if (op == 2) {
+        direct_solve<T>(g.lev[l], Wp, ainv);
+        continue;
       }
+      if (op == 1) prolong_add<T>(g.lev[l - 1], g.lev[l], Wp);
       smooth<T, CM, X>(g, g.lev[l], Wp, fac, ylds);
-      if (down) {
+      if (op == 0) {
         residual<T, X>(g.lev[l], Wp, fac);
         restrict_fw<T>(g.lev[l], g.lev[l - 1], Wp);
       }
     }
-    for (int t0 = threadIdx.x; t0 < (nx + 2) * (ny + 2); t0 += 2 * kEw * NT) {
-      double v[2 * kEw];
+    if (!g.krylov) {
+      for (int t0 = threadIdx.x; t0 < P; t0 += 2 * kEw * NT) {
+        double v[2 * kEw];
 #pragma unroll
-      for (int u = 0; u < 2 * kEw; ++u) {
-        const int t = min(t0 + u * NT, (nx + 2) * (ny + 2) - 1);
-        v[u] = phi[t] + (double)ef[t];
+        for (int u = 0; u < 2 * kEw; ++u) {
+          const int t = min(t0 + u * NT, P - 1);
+          v[u] = phi[t] + (double)ef[t];
+        }
+#pragma unroll
+        for (int u = 0; u < 2 * kEw; ++u)
+          if (t0 + u * NT < P) phi[t0 + u * NT] = v[u];
       }
-#pragma unroll
-      for (int u = 0; u < 2 * kEw; ++u)
-        if (t0 + u * NT < (nx + 2) * (ny + 2)) phi[t0 + u * NT] = v[u];
     }
     __syncthreads();
     ++it;
@@ -695,10 +783,19 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
   for (int k = 1; k <= ng; ++k)
     if (g.lev[k].n <= kDirectMax) lc = k;
   g.lc = lc;
+  {
+    // the cycle program: down ng..lc+1, direct solve on lc, up lc+1..ng
+    auto put = [&](int l, int op) { g.prog[g.nprog++] = (unsigned char)((l << 2) | op); };
+    for (int l = ng; l > lc; --l) put(l, 0);
+    put(lc, 2);
+    for (int l = lc + 1; l <= ng; ++l) put(l, 1);
+  }
   g.ainv_off = fstride;
   fstride += (long)g.lev[lc].n * g.lev[lc].n;
   fstride = (fstride + 63) & ~63L;
+  g.krylov = p.proj_krylov;
   g.ws64 = (long)(d.NZ2 + 2) * (d.NY2 + 2) + (long)d.NZ2 * d.NY2;
+  if (g.krylov) g.ws64 += 3L * d.NZ2 * d.NY2 + (long)(d.NZ2 + 2) * (d.NY2 + 2);
   g.ws64 = (g.ws64 + 31) & ~31L;
   g.fstride = fstride;
   // consistent rows of the finest level (cells J=2..NY-1, K=2..NZ-1)

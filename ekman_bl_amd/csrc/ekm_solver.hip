@@ -84,6 +84,7 @@ void ekm_params_default(ekm_params* p) {
   p->proj_mode = 0;
   p->proj_maxit = 50;
   p->proj_tol = 1.0e-12;
+  p->proj_krylov = 1;
 }
 
 }  // extern "C"
@@ -112,6 +113,8 @@ static int check_params(const ekm_params& p) {
     }
     if (!dir) return EKM_ERR_UNSUPPORTED;
     if (!(p.proj_tol > 0.0) || p.proj_maxit < 1) return EKM_ERR_ARG;
+    if (p.proj_guess < 0 || p.proj_guess > 1) return EKM_ERR_ARG;
+    if (p.proj_krylov < 0 || p.proj_krylov > 1) return EKM_ERR_ARG;
   }
   return EKM_OK;
 }

@@ -48,7 +48,8 @@ module ekman_bl_amd
      integer(c_int) :: proj_maxit
      real(c_double) :: proj_tol
      integer(c_int) :: proj_guess
-     integer(c_int) :: reserved(3)
+     integer(c_int) :: proj_krylov
+     integer(c_int) :: reserved(2)
   end type ekm_params
 
   type, bind(c) :: ekm_diag

@@ -52,7 +52,8 @@ class EkmParams(ctypes.Structure):
         ("proj_maxit", ctypes.c_int),
         ("proj_tol", ctypes.c_double),
         ("proj_guess", ctypes.c_int),
-        ("reserved", ctypes.c_int * 3),
+        ("proj_krylov", ctypes.c_int),
+        ("reserved", ctypes.c_int * 2),
     ]
 
     # -- convenience ---------------------------------------------------------
@@ -102,6 +103,7 @@ def base_params(nx: int, ny: int, nz: int, n_th: int = 0) -> EkmParams:
     p.mg_maxcy, p.mg_tolmax = 6, 1.0e-6
     for s in range(4):
         p.mudpack_bc[s] = 2
+    p.proj_maxit, p.proj_tol, p.proj_krylov = 50, 1.0e-12, 1   # used by proj_mode 1, 2
     return p
 
 

@@ -84,7 +84,10 @@ typedef struct ekm_params {
   double proj_tol;         /* absolute residual tolerance of proj_mode 1, 2 */
   int proj_guess;          /* proj_mode 1, 2: 1 = start each solve from the previous substep's phi
                             * scaled by the ratio of the RK weights h_bar (0 = from zero) */
-  int reserved[3];
+  int proj_krylov;         /* proj_mode 1, 2: 1 = the outer iteration is GCR(1) (each cycle's
+                            * correction made A-orthogonal to the previous one, then a minimal
+                            * residual step); 0 = plain iterative refinement */
+  int reserved[2];
 } ekm_params;
 
 /* Fill with the reference defaults that are not grid-size dependent. */

@@ -124,23 +124,25 @@ def test_step_matches_oracle(n, steps, oracle_lib):
     s.close()
 
 
-@pytest.mark.parametrize("mode,n", [(1, 1), (2, 1), (1, 2), (2, 2)])
-def test_consistent_projection_matches_oracle(mode, n, oracle_lib):
+@pytest.mark.parametrize("mode,n,krylov", [(1, 1, 0), (2, 1, 0), (1, 2, 0), (2, 2, 0), (2, 1, 1), (2, 2, 1), (1, 2, 1)])
+def test_consistent_projection_matches_oracle(mode, n, krylov, oracle_lib):
     """proj_mode 1 (FP64 line multigrid) and 2 (FP32 cycles inside FP64
-    residual iterations) against the oracle's FP64 consistent solve: both
-    converge to the same discrete solution, so the fields agree to the solve
-    tolerance (relative L2 <= 5e-9 for mode 1, <= 1e-8 for mode 2, at
+    residual iterations), with the plain or the GCR(1) outer iteration,
+    against the oracle's FP64 consistent solve:
+    all converge to the same discrete solution, so the fields agree to the
+    solve tolerance (relative L2 <= 5e-9 for mode 1, <= 1e-8 for mode 2, at
     max|residual| <= 1e-12), and the divergence after the projection is below
     1e-10 on the GPU."""
     O = oracle_lib
     cfg, p, xp, yp = _setup(n)
     p.proj_mode, p.proj_tol, p.proj_maxit = mode, 1e-12, 50
+    p.proj_krylov = krylov
     u1, u2, u3, th = ekman_initial(p, yp, cfg.seed, cfg.noise, cfg.th_noise)
     po = p.copy()
-    po.proj_mode = 1
+    po.proj_mode, po.proj_krylov = 1, 0
     o = O.Oracle(po, xp, yp)
     s = S.DuctSolver(p, xp, yp)
-    print(f"[tight] mode {mode} config {n}", flush=True)
+    print(f"[tight] mode {mode} config {n} krylov {krylov}", flush=True)
     o.set_initial_physical(u1, u2, u3, th)
     s.set_initial_physical(u1, u2, u3, th)
     ds = s.diag()
