@@ -45,7 +45,7 @@ HBM_PEAK_GBS = 8000.0
 DIV_BOUND = 1e-10
 FORM_NAMES = {0: "plain FP64 MUDPACK-replica W(2,1), relmax 1e-6",
               1: "consistent projection, FP64 alternating-line V(1,1)",
-              2: "consistent projection, FP32 alternating-line V(1,1) + FP64 refinement"}
+              2: "consistent projection, FP32 alternating-line V(1,1) cycles in FP64 outer iterations"}
 
 
 def log(msg: str) -> None:
@@ -82,6 +82,28 @@ def run_form(cfg, xp, yp, ic, mode: int, steps: int, warmup: int, tol: float, gu
     s.close()
     return {"mode": mode, "el": el, "steps": steps, "cyc": cyc, "div_ref": max(div_ref), "div_max": max(div_max),
             "mg_ms": mg_ms, "launches": launches, "solves": solves, "cycles": cycles, "dt": dt}
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_tight_solve_hbm_traffic.md")
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per launch of the headline solve kernel, as measured with
+    rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (scaled by the copy-kernel
+    calibration) and recorded in the traffic summary: (bytes, cycles per
+    launch of the measured dispatches), or None when the summary does not
+    cover this workload."""
+    try:
+        kv = {}
+        for line in open(TRAFFIC_FILE):
+            if ":" in line and line.startswith("- `"):
+                k, v = line[3:].split("`", 1)
+                kv[k] = v.split(":", 1)[1].strip().strip("`")
+        if kv.get("workload") != workload:
+            return None
+        return int(kv["traffic_bytes_per_launch"]), float(kv["cycles_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(cfg, xp, yp, ic, budget_s: float):
@@ -163,8 +185,8 @@ def main() -> int:
         kname = "k_mg_solve (batched MUDPACK replica, one launch per projection)"
     else:
         width = 4 if args.mode == 2 else 8
-        launch_bytes = roofline.tight_launch_bytes(p, nsolve, cyc_per_launch, width)
-        step_bytes = roofline.tight_step_bytes(p, cyc_per_launch, width)
+        launch_bytes = roofline.tight_launch_bytes(p, nsolve, cyc_per_launch, width, args.krylov)
+        step_bytes = roofline.tight_step_bytes(p, cyc_per_launch, width, args.krylov)
         kname = "k_tight_solve (batched consistent projection, one launch per projection)"
     achieved = launch_bytes / (avg_ms * 1e-3) / 1e9
     log(f"[bench] {ms_per_step:.2f} ms/step, {value:.1f} Mpt-steps/s, solve {avg_ms:.3f} ms/launch, "
@@ -189,6 +211,10 @@ def main() -> int:
             "div_max_max": r["div_max"],
             "meets_div_bound": bool(r["div_max"] < DIV_BOUND),
         }
+    workload = (f"{cfg.name}: RK3+CN step, {FORM_NAMES[args.mode]}"
+                + (f", max|residual| <= {args.tol:g}" if args.mode else "")
+                + (", GCR(1) outer iteration" if args.mode and args.krylov else ""))
+    tr = measured_traffic(workload) if args.mode == 2 else None
     out = {
         "metric": "Mpt-steps/s at ||div u||<1e-10",
         "value": round(value, 3),
@@ -202,8 +228,7 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64" if args.mode != 2 else "f64 (projection cycles f32)",
         "data": "synthetic: seeded laminar Ekman spiral + 5% uniform noise (splitmix64), theta 1% noise",
-        "config": {"workload": f"{cfg.name}: RK3+CN step, {FORM_NAMES[args.mode]}"
-                               + (f", max|residual| <= {args.tol:g}" if args.mode else ""),
+        "config": {"workload": workload,
                    "nx": p.nx, "n_y": p.ny + 2, "n_z": p.nz + 2, "n_pts": p.npts},
         "div_ref_max": h["div_ref"],
         "div_max_max": h["div_max"],
@@ -212,7 +237,10 @@ def main() -> int:
         "dt_final": h["dt"],
         "step_roofline_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"kernel": kname, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": tr[0] if tr else None,
+                     "traffic_cycles_per_launch": tr[1] if tr else None,
+                     "traffic_source": "profiles/k_tight_solve_hbm_traffic.md" if tr else None,
                      "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(launch_bytes)},
         "forms": forms,
     }

@@ -41,36 +41,45 @@ def step_bytes(p, cycles_per_solve: float) -> float:
 
 # ---------------------------------------------------------------------------
 # Consistent projection (proj_mode 1: FP64 cycles, 2: FP32 cycles inside FP64
-# residual iterations).  Bytes per finest-plane point, element width w of the
+# outer iterations).  Bytes per finest-plane point, element width w of the
 # cycle (4 or 8); the coefficient tables are shared by all solves and not
 # counted, the line factors are shared by the two solves of a mode (half
-# counted per solve).  Per V(1,1) alternating-line cycle:
-#   FP64 residual and seed:  read phi64, b (16) + write r, clear e (2w)
+# counted per solve).  The multigrid part of a V(1,1) alternating-line cycle:
 #   2 smoothing steps x 2 line directions: read r (w) + read/write e (2w)
-#                            + 3 factors / 2 solves (1.5w)        = 18w
+#   + 3 factors / 2 solves (1.5w)                                   = 18w
 #   residual (3w), restriction (1.5w), prolongation (2.25w)
-#   -> multigrid part 24.75w on the finest level, x 4/3 over all levels = 33w
-#   phi64 += e: read e (w) + read/write phi64 (16)
-# => 32 + 36w per cycle.  Per solve once: read RHS_TMP, write b, clear phi64,
-# write phi (32) and the final residual check (16 + 2w).
+#   -> 24.75w on the finest level, x 4/3 over all levels            = 33w
+# Plain iterative refinement, per cycle:
+#   FP64 residual and seed: read phi, b (16) + write r, clear e (2w)
+#   phi += e: read e (w) + read/write phi (16)
+#   => 32 + 36w.  Per solve once: read RHS_TMP, write b, clear phi, write
+#   phi (32) and the final residual check (16 + 2w).
+# GCR(1) outer iteration (proj_krylov 1), per cycle after the first:
+#   q = A z: read z (w), r, Qp (16), write q (8)                    = 24 + w
+#   update: read z (w), Zp, phi (16), write phi, Zp (16), clear z (w);
+#   read q, Qp, r (24), write Qp, r (16), r in the cycle width (w)  = 72 + 4w
+#   => 96 + 37w with the multigrid part.  Per solve once: read RHS_TMP,
+#   write b, clear phi, write phi (32), the first residual: read phi, b,
+#   write r (24) and its FP32 copy and the cleared e (2w) => 56 + 2w.
 # ---------------------------------------------------------------------------
-def tight_cycle_bytes_per_point(width: int) -> float:
-    return 32.0 + 36.0 * width
+def tight_cycle_bytes_per_point(width: int, krylov: int = 0) -> float:
+    return (96.0 + 37.0 * width) if krylov else (32.0 + 36.0 * width)
 
 
-def tight_setup_bytes_per_point(width: int) -> float:
-    return 48.0 + 2.0 * width
+def tight_setup_bytes_per_point(width: int, krylov: int = 0) -> float:
+    return (56.0 + 2.0 * width) if krylov else (48.0 + 2.0 * width)
 
 
-def tight_launch_bytes(p, nsolve: int, cycles_total: float, width: int) -> float:
+def tight_launch_bytes(p, nsolve: int, cycles_total: float, width: int, krylov: int = 0) -> float:
     """Algorithmic bytes of one consistent-projection launch (all solves)."""
     plane = (p.ny + 2) * (p.nz + 2)
-    return plane * (nsolve * tight_setup_bytes_per_point(width) + cycles_total * tight_cycle_bytes_per_point(width))
+    return plane * (nsolve * tight_setup_bytes_per_point(width, krylov) +
+                    cycles_total * tight_cycle_bytes_per_point(width, krylov))
 
 
-def tight_step_bytes(p, cycles_per_projection: float, width: int) -> float:
+def tight_step_bytes(p, cycles_per_projection: float, width: int, krylov: int = 0) -> float:
     """Algorithmic bytes of one time step with the consistent projection."""
     F = fields(p)
     npts = p.nx * (p.ny + 2) * (p.nz + 2)
     nsolve = 2 * (p.nx // 3 + 1)
-    return 8.0 * npts * (34 * F + 33) + 3 * tight_launch_bytes(p, nsolve, cycles_per_projection, width)
+    return 8.0 * npts * (34 * F + 33) + 3 * tight_launch_bytes(p, nsolve, cycles_per_projection, width, krylov)
