@@ -66,6 +66,65 @@ __device__ inline void dif(double2 (&z)[R], int GW, int M, int lanep, const doub
   }
 }
 
+// Twiddles of the six lane stages when a wavefront holds one transform
+// (GW = 64): the stage with partner distance h uses tw[(lane mod h) * M/(2h)]
+// whatever the register, so all six are loaded up front, together with the
+// data, instead of one dependent load per stage.
+struct LaneTw {
+  double2 w[6];
+};
+__device__ inline LaneTw lane_twiddles(int lanep, int M, const double2* __restrict__ tw) {
+  LaneTw t;
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    const int h = 32 >> s;
+    t.w[s] = tw[(lanep & (h - 1)) * (M / (2 * h))];
+  }
+  return t;
+}
+
+// dif<R> for GW = 64 with the lane-stage twiddles given
+template <int R>
+__device__ inline void dif64(double2 (&z)[R], int M, int lanep, const double2* __restrict__ tw, const LaneTw& lt) {
+#pragma unroll
+  for (int hr = R / 2; hr >= 1; hr >>= 1) {
+    const int h = hr * 64, step = M / (2 * h);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if ((r / hr) % 2 == 0) {
+        const int e = lanep + 64 * r;
+        const double2 x = z[r], y = z[r + hr];
+        z[r] = cadd(x, y);
+        z[r + hr] = cmul(csub(x, y), tw[(e % h) * step]);
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    const int h = 32 >> s;
+    const bool top = (lanep & h) == 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double2 mine = z[r];
+      double2 other;
+      other.x = __shfl_xor(mine.x, h, 64);
+      other.y = __shfl_xor(mine.y, h, 64);
+      z[r] = top ? cadd(mine, other) : cmul(csub(other, mine), lt.w[s]);
+    }
+  }
+}
+
+// the transform of z: dif64 when a wavefront holds one transform, else the
+// general form
+template <int R>
+__device__ inline void dif_any(double2 (&z)[R], int GW, int M, int lanep, const double2* __restrict__ tw,
+                               const LaneTw& lt) {
+  if (GW == 64)
+    dif64<R>(z, M, lanep, tw, lt);
+  else
+    dif<R>(z, GW, M, lanep, tw);
+}
+
 __device__ inline int bitrev(int e, int logm) { return (int)(__brev((unsigned)e) >> (32 - logm)); }
 
 template <int R>
@@ -80,10 +139,11 @@ __global__ void __launch_bounds__(kBlock) k_fft_fwd(double* __restrict__ v, long
   const bool live = pen < npencil;
   const int stride = nx + 2, nkx = nx / 3;
   const double2* src = reinterpret_cast<const double2*>(v + (live ? pen : 0) * stride);
+  const LaneTw lt = lane_twiddles(lanep, M, tw);
   double2 z[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) z[r] = live ? src[lanep + GW * r] : make_double2(0.0, 0.0);
-  dif<R>(z, GW, M, lanep, tw);
+  dif_any<R>(z, GW, M, lanep, tw, lt);
   // into natural order in this wave's LDS slice
   double2* Z = lds + (size_t)(wave * PW + sub) * M;
 #pragma unroll
@@ -95,7 +155,13 @@ __global__ void __launch_bounds__(kBlock) k_fft_fwd(double* __restrict__ v, long
   if (!live) return;
   const double sc = 1.0 / (double)nx;
   double2* dst = reinterpret_cast<double2*>(v + pen * stride);
-  for (int k = lanep; k <= M; k += GW) {
+  double2 w2[R + 1];
+#pragma unroll
+  for (int i = 0; i <= R; ++i) w2[i] = tw2[min(lanep + GW * i, M)];
+#pragma unroll
+  for (int i = 0; i <= R; ++i) {
+    const int k = lanep + GW * i;
+    if (k > M) break;
     double2 x = make_double2(0.0, 0.0);
     if (k <= nkx) {
       const double2 zk = Z[k == M ? 0 : k];
@@ -103,7 +169,7 @@ __global__ void __launch_bounds__(kBlock) k_fft_fwd(double* __restrict__ v, long
       y.y = -y.y;  // conj(Z_{M-k})
       const double2 e = make_double2(0.5 * (zk.x + y.x), 0.5 * (zk.y + y.y));
       const double2 o = make_double2(0.5 * (zk.y - y.y), -0.5 * (zk.x - y.x));
-      const double2 wo = cmul(tw2[k], o);
+      const double2 wo = cmul(w2[i], o);
       x = make_double2((e.x + wo.x) * sc, (e.y + wo.y) * sc);
     }
     dst[k] = x;
@@ -124,6 +190,7 @@ __global__ void __launch_bounds__(kBlock) k_fft_inv(double* __restrict__ v, long
   const double2* X = reinterpret_cast<const double2*>(v + (live ? pen : 0) * stride);
   // Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/NX} (X_k - conj X_{M-k}), k < M,
   // conjugated: the inverse transform is conj(DFT(conj Z))
+  const LaneTw lt = lane_twiddles(lanep, M, tw);
   double2 z[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -140,7 +207,7 @@ __global__ void __launch_bounds__(kBlock) k_fft_inv(double* __restrict__ v, long
     const double2 o = cmul(dd, w);
     z[r] = make_double2(e.x - o.y, -(e.y + o.x));
   }
-  dif<R>(z, GW, M, lanep, tw);
+  dif_any<R>(z, GW, M, lanep, tw, lt);
   double2* Z = lds + (size_t)(wave * PW + sub) * M;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -190,6 +257,7 @@ __global__ void __launch_bounds__(kBlock) k_flux_accum(Dims d, Metrics M, Ranges
   const double Jc = M.jac[mi(d, k, j)];
   const double g12kp = MET(g12, k + 1, j, 2), g12k = MET(g12, k, j, 2);
   const double g12jp = MET(g12, k, j + 1, 1), g12j = MET(g12, k, j, 1);
+  const LaneTw lt = lane_twiddles(lanep, M2, tw);
   double2 zx[R], zy[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -227,8 +295,8 @@ __global__ void __launch_bounds__(kBlock) k_flux_accum(Dims d, Metrics M, Ranges
     zx[r] = make_double2(sx[0], sx[1]);
     zy[r] = make_double2(sy[0], sy[1]);
   }
-  dif<R>(zx, GW, M2, lanep, tw);
-  dif<R>(zy, GW, M2, lanep, tw);
+  dif_any<R>(zx, GW, M2, lanep, tw, lt);
+  dif_any<R>(zy, GW, M2, lanep, tw, lt);
   double2* Zx = lds + (size_t)2 * (wave * PW + sub) * M2;
   double2* Zy = Zx + M2;
 #pragma unroll
@@ -244,7 +312,22 @@ __global__ void __launch_bounds__(kBlock) k_flux_accum(Dims d, Metrics M, Ranges
   const double sc = 1.0 / (double)nx;
   const int nkx = nx / 3;
   const size_t f0 = (size_t)d.NXC * ((size_t)k + (size_t)d.NZ2 * j);
-  for (int m = lanep; m <= M2; m += GW) {
+  // the outputs m = lanep + GW i (i <= R): all global operands are loaded
+  // first, then combined, so the lane pays one memory round trip
+  double2 cf[R + 1], cr[R + 1], w2[R + 1];
+  double kw[R + 1];
+#pragma unroll
+  for (int i = 0; i <= R; ++i) {
+    const int m = min(lanep + GW * i, M2);
+    cf[i] = CF[f0 + m];
+    cr[i] = CR[f0 + m];
+    w2[i] = tw2[m];
+    kw[i] = kxv[m];
+  }
+#pragma unroll
+  for (int i = 0; i <= R; ++i) {
+    const int m = lanep + GW * i;
+    if (m > M2) break;
     double2 X[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
     if (m <= nkx) {
 #pragma unroll
@@ -255,18 +338,18 @@ __global__ void __launch_bounds__(kBlock) k_flux_accum(Dims d, Metrics M, Ranges
         y.y = -y.y;
         const double2 e = make_double2(0.5 * (zk.x + y.x), 0.5 * (zk.y + y.y));
         const double2 o = make_double2(0.5 * (zk.y - y.y), -0.5 * (zk.x - y.x));
-        const double2 wo = cmul(tw2[m], o);
+        const double2 wo = cmul(w2[i], o);
         X[t] = make_double2((e.x + wo.x) * sc, (e.y + wo.y) * sc);
       }
     }
-    double2 f = CF[f0 + m];
-    const double w = kxv[m];
+    double2 f = cf[i];
+    const double w = kw[i];
     f.x = f.x - (0.0 * X[0].x - w * X[0].y);
     f.y = f.y - (0.0 * X[0].y + w * X[0].x);
     f.x = f.x + X[1].x;
     f.y = f.y + X[1].y;
     CF[f0 + m] = f;
-    double2 c = CR[f0 + m];
+    double2 c = cr[i];
     c.x = c.x + temp5 * f.x;
     c.y = c.y + temp5 * f.y;
     CR[f0 + m] = c;

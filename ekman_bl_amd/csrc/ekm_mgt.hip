@@ -226,6 +226,7 @@ __device__ void zlines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
 constexpr int kYMaxTasks = 32;
 constexpr int kYUnroll = 3;
 constexpr int kYApply = 8;  // rows per batch of the last pass
+constexpr int kYBack = 6;   // rows per batch of the backward pass
 
 template <typename T, bool X>
 __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ fac, int par, T* lds) {
@@ -313,10 +314,10 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
     const T din = sv[task * 64 + lane];
     T al = (T)0, be = (T)1;
     if (i < nx) {
-      for (int jb = j1 - 1; jb >= j0; jb -= kYUnroll) {
-        T a_[kYUnroll], b_[kYUnroll], c_[kYUnroll];
+      for (int jb = j1 - 1; jb >= j0; jb -= kYBack) {
+        T a_[kYBack], b_[kYBack], c_[kYBack];
 #pragma unroll
-        for (int u = 0; u < kYUnroll; ++u) {
+        for (int u = 0; u < kYBack; ++u) {
           const int j = jb - u;
           if (j >= j0) {
             const int p = i + nx * j;
@@ -326,7 +327,7 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
           }
         }
 #pragma unroll
-        for (int u = 0; u < kYUnroll; ++u) {
+        for (int u = 0; u < kYBack; ++u) {
           const int j = jb - u;
           if (j >= j0) {
             const T dp = a_[u] * din + b_[u];
