@@ -526,6 +526,78 @@ __device__ inline void block_sum4(double* v, double* red) {
   __syncthreads();
 }
 
+// One GCR(1) step after a cycle (the cycle's correction z sits in the finest
+// level's e): q = A z in FP64; q and z are made A-orthogonal to the kept
+// direction (beta), then the minimal residual step alpha = (r.q')/(q'.q')
+// updates phi, r and the kept direction, and r seeds the next cycle in the
+// cycle's precision.  Returns this thread's max |r|.  Kept out of line so its
+// pointers and scalars are not live across the cycle.
+template <typename T, int NT, bool X>
+__device__ __noinline__ double gcr_step(const TDesc<T>& g, const TLev<T>& F, double* __restrict__ phi, T* W,
+                                        const double* __restrict__ c964, double* red, int it) {
+  __shared__ double sQQ;  // Qp . Qp of the kept direction
+  const int nx = F.nx, ny = F.ny, ld = nx + 2, n = F.n, P = (nx + 2) * (ny + 2);
+  double* __restrict__ rr = phi + P + n;  // after phi (ring) and b
+  double* __restrict__ qv = rr + n;       // A z
+  double* __restrict__ Qp = qv + n;       // kept direction's A z
+  double* __restrict__ Zp = Qp + n;       // kept direction (ring layout)
+  T* __restrict__ rf = W + F.r_off;
+  T* __restrict__ ef = W + F.e_off;
+  const double* __restrict__ c = g.cf64;
+  double rmax = 0.0;
+  const bool hasq = it > 1;
+  double sm[4] = {0.0, 0.0, 0.0, 0.0};  // q.Qp, q.q, r.q, r.Qp
+  for (int p0 = threadIdx.x; p0 < n; p0 += kEwQ * NT) {
+    double v[kEwQ];
+#pragma unroll
+    for (int u = 0; u < kEwQ; ++u) {
+      const int p = min(p0 + u * NT, n - 1);
+      const int j = p / nx, i = p - j * nx;
+      const int q = (i + 1) + ld * (j + 1);
+      double acc = (c[0 * n + p] * (double)ef[q + 1] + c[2 * n + p] * (double)ef[q + ld]) +
+                   (c[4 * n + p] * (double)ef[q - 1] + c[6 * n + p] * (double)ef[q - ld]);
+      if (X)
+        acc += (c[1 * n + p] * (double)ef[q + 1 + ld] + c[3 * n + p] * (double)ef[q - 1 + ld]) +
+               (c[5 * n + p] * (double)ef[q - 1 - ld] + c[7 * n + p] * (double)ef[q + 1 - ld]);
+      v[u] = acc + c964[p] * (double)ef[q];
+    }
+#pragma unroll
+    for (int u = 0; u < kEwQ; ++u) {
+      const int p = p0 + u * NT;
+      if (p < n) {
+        const double qp = v[u], rp = rr[p], qk = hasq ? Qp[p] : 0.0;
+        qv[p] = qp;
+        sm[0] += qp * qk;
+        sm[1] += qp * qp;
+        sm[2] += rp * qp;
+        sm[3] += rp * qk;
+      }
+    }
+  }
+  block_sum4(sm, red);
+  const double QQ = hasq ? sQQ : 0.0;
+  const double beta = (hasq && QQ > 0.0) ? sm[0] / QQ : 0.0;
+  const double qq = sm[1] - 2.0 * beta * sm[0] + beta * beta * QQ;
+  const double alpha = qq > 0.0 ? (sm[2] - beta * sm[3]) / qq : 0.0;
+  __syncthreads();  // every thread has read the previous value
+  if (threadIdx.x == 0) sQQ = qq;
+  for (int t = threadIdx.x; t < P; t += blockDim.x) {
+    const double z = (double)ef[t] - (hasq ? beta * Zp[t] : 0.0);
+    phi[t] += alpha * z;
+    Zp[t] = z;
+    ef[t] = (T)0;
+  }
+  for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    const double q2 = qv[p] - (hasq ? beta * Qp[p] : 0.0);
+    const double r2 = rr[p] - alpha * q2;
+    Qp[p] = q2;
+    rr[p] = r2;
+    rf[p] = (T)r2;
+    rmax = fmax(rmax, fabs(r2));
+  }
+  return rmax;
+}
+
 template <typename T, int CM, int NT, bool X>
 __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__ gp, T* __restrict__ ws,
                                                      double* __restrict__ ws64, const double* __restrict__ rhs_tmp,
@@ -566,15 +638,11 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
   for (int p = threadIdx.x; p < n; p += blockDim.x) b[p] = F.spec[p] ? 0.0 : -rt[p];
   __syncthreads();
   const int P = (nx + 2) * (ny + 2);
-  double* __restrict__ rr = b + n;   // krylov: recursive residual
-  double* __restrict__ qv = rr + n;  // A z
-  double* __restrict__ Qp = qv + n;  // kept direction's A z
-  double* __restrict__ Zp = Qp + n;  // kept direction (ring layout)
+  double* __restrict__ rr = b + n;  // krylov: the recursive residual (first iteration)
   T* __restrict__ rf = W + F.r_off;
   T* __restrict__ ef = W + F.e_off;
   const double* __restrict__ c = g.cf64;
   int it = 0;
-  __shared__ double sQQ;  // Qp . Qp of the kept direction
   for (;;) {
     double rmax = 0.0;
     if (!g.krylov || it == 0) {
@@ -604,59 +672,7 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
       }
       for (int t = threadIdx.x; t < P; t += blockDim.x) ef[t] = (T)0;
     } else {
-      // GCR(1): z = the cycle's correction, q = A z in FP64; q and z are
-      // made A-orthogonal to the kept direction (beta), then the minimal
-      // residual step alpha = (r.q')/(q'.q') updates phi and r
-      const bool hasq = it > 1;
-      double sm[4] = {0.0, 0.0, 0.0, 0.0};  // q.Qp, q.q, r.q, r.Qp
-      for (int p0 = threadIdx.x; p0 < n; p0 += kEwQ * NT) {
-        double v[kEwQ];
-#pragma unroll
-        for (int u = 0; u < kEwQ; ++u) {
-          const int p = min(p0 + u * NT, n - 1);
-          const int j = p / nx, i = p - j * nx;
-          const int q = (i + 1) + ld * (j + 1);
-          double acc = (c[0 * n + p] * (double)ef[q + 1] + c[2 * n + p] * (double)ef[q + ld]) +
-                       (c[4 * n + p] * (double)ef[q - 1] + c[6 * n + p] * (double)ef[q - ld]);
-          if (X)
-            acc += (c[1 * n + p] * (double)ef[q + 1 + ld] + c[3 * n + p] * (double)ef[q - 1 + ld]) +
-                   (c[5 * n + p] * (double)ef[q - 1 - ld] + c[7 * n + p] * (double)ef[q + 1 - ld]);
-          v[u] = acc + c964[p] * (double)ef[q];
-        }
-#pragma unroll
-        for (int u = 0; u < kEwQ; ++u) {
-          const int p = p0 + u * NT;
-          if (p < n) {
-            const double qp = v[u], rp = rr[p], qk = hasq ? Qp[p] : 0.0;
-            qv[p] = qp;
-            sm[0] += qp * qk;
-            sm[1] += qp * qp;
-            sm[2] += rp * qp;
-            sm[3] += rp * qk;
-          }
-        }
-      }
-      block_sum4(sm, red);
-      const double QQ = hasq ? sQQ : 0.0;
-      const double beta = (hasq && QQ > 0.0) ? sm[0] / QQ : 0.0;
-      const double qq = sm[1] - 2.0 * beta * sm[0] + beta * beta * QQ;
-      const double alpha = qq > 0.0 ? (sm[2] - beta * sm[3]) / qq : 0.0;
-      __syncthreads();  // every thread has read the previous value
-      if (threadIdx.x == 0) sQQ = qq;
-      for (int t = threadIdx.x; t < P; t += blockDim.x) {
-        const double z = (double)ef[t] - (hasq ? beta * Zp[t] : 0.0);
-        phi[t] += alpha * z;
-        Zp[t] = z;
-        ef[t] = (T)0;
-      }
-      for (int p = threadIdx.x; p < n; p += blockDim.x) {
-        const double q2 = qv[p] - (hasq ? beta * Qp[p] : 0.0);
-        const double r2 = rr[p] - alpha * q2;
-        Qp[p] = q2;
-        rr[p] = r2;
-        rf[p] = (T)r2;
-        rmax = fmax(rmax, fabs(r2));
-      }
+      rmax = gcr_step<T, NT, X>(g, F, phi, W, c964, red, it);
     }
     red[threadIdx.x] = rmax;
     __syncthreads();
