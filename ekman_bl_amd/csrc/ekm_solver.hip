@@ -12,6 +12,7 @@
 #include "ekm_host.h"
 #include "ekm_internal.h"
 #include "ekm_kernels.h"
+#include "ekm_les.h"
 
 static thread_local std::string g_last_error;
 
@@ -57,6 +58,12 @@ struct ekm_solver {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double prof_ms = 0.0;
   long prof_launches = 0, prof_solves = 0, prof_cycles = 0;
+  // LES (src/les_chan.F90): model fields and scratch, and the reference's
+  // CREATE_NEW_FLOW / FIRST_TIME, which decide when the model runs and when
+  // its coefficient is recomputed
+  LesFields les{};
+  LesScratch lesw{};
+  int create_new_flow = 1, first_time = 1;
   int have_prev_phi = 0;  // mg.phi holds the previous substep's correction
   double hb_prev = 0.0;
 };
@@ -103,6 +110,8 @@ static int check_params(const ekm_params& p) {
   if ((p.mudpack_bc[0] == 0) != (p.mudpack_bc[1] == 0)) return EKM_ERR_ARG;
   if ((p.mudpack_bc[2] == 0) != (p.mudpack_bc[3] == 0)) return EKM_ERR_ARG;
   if (p.proj_mode < 0 || p.proj_mode > 2) return EKM_ERR_UNSUPPORTED;
+  if (p.les_model != 0 && p.les_model != 2) return EKM_ERR_UNSUPPORTED;  // dynamic Smagorinsky only
+  if (p.les_model && (p.nx > 4096)) return EKM_ERR_UNSUPPORTED;
   if (p.proj_mode > 0) {
     // line smoothing is not cyclic, and the consistent system needs a
     // specified side to be nonsingular at kx = 0
@@ -149,6 +158,13 @@ static int free_all(ekm_solver* s) {
   for (double* q : f)
     if (q) (void)hipFree(q);
   if (s->adi_d) (void)hipFree(s->adi_d);
+  {
+    double* lf[] = {s->les.S[0], s->les.S[1], s->les.S[2], s->les.S[3], s->les.S[4], s->les.S[5], s->les.sabs,
+                    s->les.nut, s->les.delta, s->les.cdyn, s->lesw.ub[0], s->lesw.ub[1], s->lesw.ub[2],
+                    s->lesw.s2b, s->lesw.t1, s->lesw.m0, s->lesw.fm0, s->lesw.ft2, s->lesw.nsum, s->lesw.dsum};
+    for (double* q : lf)
+      if (q) (void)hipFree(q);
+  }
   fft_plan_free(s->fft);
   mg_plan_free(s->mg);
   tight_plan_free(s->tp);
@@ -256,6 +272,29 @@ extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const doub
     if (hipMalloc(&s->adi_d, sizeof(s->adi_h)) != hipSuccess) return fail(EKM_ERR_HIP);
     if (hipMemcpy(s->adi_d, s->adi_h, sizeof(s->adi_h), hipMemcpyHostToDevice) != hipSuccess) return fail(EKM_ERR_HIP);
   }
+  if (p.les_model) {
+    // subgrid model fields and the filter width Delta = (DX J)^(1/3) at the
+    // cell centres (src/les_chan.F90:343-348, beta = 1)
+    double** pf[] = {&s->les.S[0], &s->les.S[1], &s->les.S[2], &s->les.S[3], &s->les.S[4], &s->les.S[5],
+                     &s->les.sabs, &s->les.nut, &s->lesw.ub[0], &s->lesw.ub[1], &s->lesw.ub[2], &s->lesw.s2b,
+                     &s->lesw.t1, &s->lesw.m0, &s->lesw.fm0, &s->lesw.ft2};
+    for (double** q : pf) {
+      if (hipMalloc(q, d.npts * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
+      if (hipMemset(*q, 0, d.npts * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
+    }
+    double** pp[] = {&s->les.delta, &s->les.cdyn, &s->lesw.nsum, &s->lesw.dsum};
+    for (double** q : pp) {
+      if (hipMalloc(q, d.plane * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
+      if (hipMemset(*q, 0, d.plane * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
+    }
+    std::vector<double> dl(d.plane);
+    const double dx = p.lx / (double)d.NX;
+    for (int j = 0; j < d.NY2; ++j)
+      for (int k = 0; k < d.NZ2; ++k)
+        dl[(size_t)k + (size_t)d.NZ2 * j] = std::pow(dx * s->hm.jac[(size_t)k + (size_t)d.NZ3 * j], 1.0 / 3.0);
+    if (hipMemcpy(s->les.delta, dl.data(), d.plane * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(EKM_ERR_HIP);
+  }
   if (hipMalloc(&s->red, 4 * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
   if (fft_plan_init(s->fft, p.nx) != EKM_OK) return fail(EKM_ERR_HIP);
   int nsolve = 2 * (d.NKX + 1);
@@ -313,6 +352,8 @@ extern "C" int ekm_field_device_ptr(ekm_solver* s, int f, void** dptr) {
 extern "C" int ekm_set_state(ekm_solver* s, const double* cu1, const double* cu2, const double* cu3,
                              const double* cp, const double* cth, double time, int time_step) {
   s->have_prev_phi = 0;
+  s->create_new_flow = 0;  // a restart: the subgrid model runs from the first step
+  s->first_time = 1;
   size_t n = s->d.npts * sizeof(double);
   const double* src[4] = {cu1, cu2, cu3, cp};
   double* dst[4] = {s->U[0], s->U[1], s->U[2], s->P};
@@ -322,6 +363,16 @@ extern "C" int ekm_set_state(ekm_solver* s, const double* cu1, const double* cu2
   s->time = time;
   s->time_step = time_step;
   s->rk_step = 1;
+  return EKM_OK;
+}
+
+// NU_T (physical, the layout of a field) and C_DYN ((NZ+2) x (NY+2), K
+// fastest) of the subgrid model (src/modules.F90:431-483 les_chan_var)
+extern "C" int ekm_get_les(ekm_solver* s, double* nut, double* cdyn) {
+  if (!s->p.les_model) return EKM_ERR_ARG;
+  if (nut) EKM_CHECK(hipMemcpyAsync(nut, s->les.nut, s->d.npts * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  if (cdyn) EKM_CHECK(hipMemcpyAsync(cdyn, s->les.cdyn, s->d.plane * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  EKM_CHECK(hipStreamSynchronize(s->st));
   return EKM_OK;
 }
 
@@ -408,6 +459,8 @@ extern "C" int ekm_set_initial_physical(ekm_solver* s, const double* u1, const d
   s->time_step = 0;
   s->rk_step = 1;
   s->have_prev_phi = 0;
+  s->create_new_flow = 1;
+  s->first_time = 1;
   return rem_div(s);
 }
 
@@ -452,7 +505,14 @@ extern "C" int ekm_rk_duct(ekm_solver* s) {
   a.px = p.f_type == 1 ? p.px0 : p.px0 + p.amp_omega0 * std::cos(p.omega0 * s->time);
   TRY(launch_rhs_fourier(d, s->M, a, st));
   // velocities and scalar to physical space (:655-671)
-  for (int f = 0; f < 3; ++f) TRY(inv(s, s->U[f]));
+  // the subgrid model (:641-657), which leaves the velocities in physical
+  // space, or the velocities to physical space (:658-671)
+  if (p.les_model && (!s->create_new_flow || cur_step > 1)) {
+    const bool refresh = ((cur_step % 5) == 0 && rk == 1) || s->first_time;
+    TRY(les_chan_run(s->fft, d, s->M, s->rv, s->kx, s->U, s->CF, s->les, s->lesw, refresh, st));
+  } else {
+    for (int f = 0; f < 3; ++f) TRY(inv(s, s->U[f]));
+  }
   if (th) TRY(inv(s, s->TH));
   // momentum fluxes (:683-1161)
   TRY(launch_contravariant(d, s->M, s->rv, s->U[1], s->U[2], s->U2b, s->U3b, st));
@@ -505,6 +565,7 @@ extern "C" int ekm_rk_duct(ekm_solver* s) {
     s->time = s->time + s->delta_t;
     s->time_step = cur_step;
     s->rk_step = 1;
+    s->first_time = 0;
   } else {
     s->rk_step = rk + 1;
   }

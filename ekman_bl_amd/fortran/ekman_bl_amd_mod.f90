@@ -49,7 +49,8 @@ module ekman_bl_amd
      real(c_double) :: proj_tol
      integer(c_int) :: proj_guess
      integer(c_int) :: proj_krylov
-     integer(c_int) :: reserved(2)
+     integer(c_int) :: les_model
+     integer(c_int) :: reserved(1)
   end type ekm_params
 
   type, bind(c) :: ekm_diag

@@ -53,7 +53,8 @@ class EkmParams(ctypes.Structure):
         ("proj_tol", ctypes.c_double),
         ("proj_guess", ctypes.c_int),
         ("proj_krylov", ctypes.c_int),
-        ("reserved", ctypes.c_int * 2),
+        ("les_model", ctypes.c_int),
+        ("reserved", ctypes.c_int * 1),
     ]
 
     # -- convenience ---------------------------------------------------------

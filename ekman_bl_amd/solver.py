@@ -64,6 +64,7 @@ def lib():
         L.ekm_time_step.argtypes = [vp, ctypes.c_int]
         L.ekm_set_state.argtypes = [vp, _D, _D, _D, _D, _D, ctypes.c_double, ctypes.c_int]
         L.ekm_get_state.argtypes = [vp, _D, _D, _D, _D, _D]
+        L.ekm_get_les.argtypes = [vp, _D, _D]
         L.ekm_set_initial_physical.argtypes = [vp, _D, _D, _D, _D]
         L.ekm_get_phi.argtypes = [vp, _D]
         L.ekm_get_diag.argtypes = [vp, ctypes.POINTER(EkmDiag)]
@@ -137,6 +138,14 @@ class DuctSolver:
         out = [self._zc() for _ in range(5)]
         _check(lib().ekm_get_state(self.h, *[_c(a) for a in out]), "ekm_get_state")
         return out  # cu1, cu2, cu3, cp, cth
+
+    def get_les(self):
+        """(NU_T physical (NY+2, NZ+2, NX+2), C_DYN (NY+2, NZ+2)) of the subgrid model."""
+        p = self.p
+        nut = np.zeros((p.ny + 2, p.nz + 2, p.nx + 2))
+        cdyn = np.zeros((p.ny + 2, p.nz + 2))
+        _check(lib().ekm_get_les(self.h, _p(nut), _p(cdyn)), "ekm_get_les")
+        return nut, cdyn
 
     def set_initial_physical(self, u1, u2, u3, th):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u1, u2, u3, th)]

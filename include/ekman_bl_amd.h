@@ -87,7 +87,9 @@ typedef struct ekm_params {
   int proj_krylov;         /* proj_mode 1, 2: 1 = the outer iteration is GCR(1) (each cycle's
                             * correction made A-orthogonal to the previous one, then a minimal
                             * residual step); 0 = plain iterative refinement */
-  int reserved[2];
+  int les_model;           /* LES_MODEL_TYPE of the velocity subgrid model (src/les_chan.F90):
+                            * 0 = none (DNS), 2 = dynamic Smagorinsky */
+  int reserved[1];
 } ekm_params;
 
 /* Fill with the reference defaults that are not grid-size dependent. */
@@ -125,6 +127,15 @@ int ekm_destroy(ekm_solver* s);
 int ekm_set_state(ekm_solver* s, const double* cu1, const double* cu2, const double* cu3,
                   const double* cp, const double* cth, double time, int time_step);
 int ekm_get_state(ekm_solver* s, double* cu1, double* cu2, double* cu3, double* cp, double* cth);
+
+/* The subgrid model's eddy viscosity NU_T ((NX+2)*(NZ+2)*(NY+2) doubles,
+ * physical layout) and dynamic coefficient C_DYN ((NZ+2)*(NY+2), K fastest)
+ * after the last substep with les_model = 2 (module les_chan_var,
+ * src/modules.F90:431-483, set by les_chan, src/les_chan.F90:332-831).
+ * set_state counts as a restart (the model runs from the first step, with the
+ * coefficient computed in every substep of that step); set_initial_physical
+ * as a new flow (the model starts at the second step). */
+int ekm_get_les(ekm_solver* s, double* nut, double* cdyn);
 
 /* Physical-space initial fields (U1,U2,U3,TH on (NX+2)*(NZ+2)*(NY+2) doubles):
  * forward x-FFT with 2/3 truncation, CP = 0, then the initial projection

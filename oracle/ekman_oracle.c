@@ -58,6 +58,11 @@ struct ora {
   double h_bar[3], beta_bar[3], zeta_bar[3];
   int rk_step, time_step;
   ekm_diag diag;
+  /* LES (src/les_chan.F90 model 2): strain / stress Sij (physical, 6 P
+   * fields), |S|, filtered velocity, eddy viscosity, filter width and the
+   * dynamic coefficient; CREATE_NEW_FLOW and FIRST_TIME of the reference */
+  double *Sij[6], *SABS, *UBT[3], *NUT, *DELTA, *CDYN;
+  int create_new_flow, first_time;
 };
 
 #define IDX(o, i, k, j) ((size_t)(i) + (size_t)(o)->NX2 * ((size_t)(k) + (size_t)(o)->NZ2 * (size_t)(j)))
@@ -1454,6 +1459,7 @@ static int supported(const ekm_params* p) {
   if (p->ny + 2 != p->mg_jyq * (1 << (p->mg_jey - 1)) + 1) return 0;
   if (p->nx < 8 || (p->nx & (p->nx - 1))) return 0;
   if (p->proj_mode < 0 || p->proj_mode > 1) return 0;
+  if (p->les_model != 0 && p->les_model != 2) return 0;
   if (p->proj_mode == 1) {
     int dir = 0;
     for (int s = 0; s < 4; ++s) {
@@ -1518,6 +1524,19 @@ ora* ora_create(const ekm_params* p, const double* xpoint, const double* ypoint,
                   &o->CFTH, &o->R[0], &o->R[1], &o->R[2], &o->RTH, &o->S1, &o->S2, &o->U2b,
                   &o->U3b, &o->CS1, &o->PHI};
   for (size_t q = 0; q < sizeof(f) / sizeof(f[0]); ++q) *f[q] = dalloc(o->npts);
+  if (p->les_model) {
+    for (int q = 0; q < 6; ++q) o->Sij[q] = dalloc(o->npts);
+    for (int q = 0; q < 3; ++q) o->UBT[q] = dalloc(o->npts);
+    o->SABS = dalloc(o->npts);
+    o->NUT = dalloc(o->npts);
+    o->DELTA = dalloc(pl);
+    o->CDYN = dalloc(pl);
+    /* filter width at cell centres (les_chan.F90:343-348, beta = 1) */
+    for (int j = 0; j <= o->NY + 1; ++j)
+      for (int k = 0; k <= o->NZ + 1; ++k) o->DELTA[PL(o, k, j)] = pow(p->lx / (double)o->NX * JAC(o, k, j), 1.0 / 3.0);
+  }
+  o->create_new_flow = 1;
+  o->first_time = 1;
   init_ranges(o);
   o->delta_t = p->delta_t;
   o->delta_t_set = p->delta_t;
@@ -1535,6 +1554,9 @@ void ora_destroy(ora* o) {
   double* f[] = {o->U[0], o->U[1], o->U[2], o->P, o->TH, o->CF[0], o->CF[1], o->CF[2], o->CFTH,
                  o->R[0], o->R[1], o->R[2], o->RTH, o->S1, o->S2, o->U2b, o->U3b, o->CS1, o->PHI};
   for (size_t q = 0; q < sizeof(f) / sizeof(f[0]); ++q) free(f[q]);
+  for (int q = 0; q < 6; ++q) free(o->Sij[q]);
+  for (int q = 0; q < 3; ++q) free(o->UBT[q]);
+  free(o->SABS); free(o->NUT); free(o->DELTA); free(o->CDYN);
   free(o);
 }
 
@@ -1955,6 +1977,232 @@ static void courant(ora* o) {
 /* ------------------------------------------------------------------------ */
 /* RK_DUCT (src/duct.F90:237-2469)                                           */
 /* ------------------------------------------------------------------------ */
+/* ------------------------------------------------------------------------ */
+/* LES, dynamic Smagorinsky model of the velocity (src/les_chan.F90 model 2, */
+/* LES_IMPLICIT = .FALSE.)                                                   */
+/* ------------------------------------------------------------------------ */
+#define LES_ALPHA 2.44950 /* test / LES filter width ratio (les_chan.F90:37) */
+
+/* x test filter [1/4 1/2 1/4], periodic over i = 0..NX-1, every (k, j)
+ * (les_filter_chan filter_type 2, src/les_chan.F90:1719-1838, through
+ * FILTER_VAR, src/mpi_duct.F90:834-852); the padding i = NX, NX+1 is kept */
+static void les_filter(const ora* o, const double* a, double* b) {
+  const int NX = o->NX;
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j <= o->NY + 1; ++j)
+    for (int k = 0; k <= o->NZ + 1; ++k) {
+      const double* A = a + IDX(o, 0, k, j);
+      double* B = b + IDX(o, 0, k, j);
+      double tmp[4096 + 2];
+      for (int i = 0; i < NX; ++i) {
+        const int im = i == 0 ? NX - 1 : i - 1, ip = i == NX - 1 ? 0 : i + 1;
+        tmp[i] = ((0.0 * A[im] + 0.25 * A[im]) + 0.5 * A[i] + 0.25 * A[ip]) + 0.0 * A[ip];
+      }
+      for (int i = 0; i < NX; ++i) B[i] = tmp[i];
+      if (a != b) { B[NX] = A[NX]; B[NX + 1] = A[NX + 1]; }
+    }
+}
+
+/* the two face-difference operators of the strain (compute_strain_chan,
+ * src/les_chan.F90:1623-1717) at (k, j) on a field with strides dk, dj:
+ * D2 uses CJOB_12 / CJOB_22, D3 uses CJOB_11 / CJOB_21 */
+static inline double les_d2(const ora* o, const double* f, size_t q, int k, int j, size_t dk, size_t dj) {
+  return 0.5 * C12(o, k + 1, j, 2) * (f[q] + f[q + dk]) - 0.5 * C12(o, k, j, 2) * (f[q] + f[q - dk]) +
+         0.5 * C22(o, k, j + 1, 1) * (f[q] + f[q + dj]) - 0.5 * C22(o, k, j, 1) * (f[q] + f[q - dj]);
+}
+static inline double les_d3(const ora* o, const double* f, size_t q, int k, int j, size_t dk, size_t dj) {
+  return 0.5 * C11(o, k + 1, j, 2) * (f[q] + f[q + dk]) - 0.5 * C11(o, k, j, 2) * (f[q] + f[q - dk]) +
+         0.5 * C21(o, k, j + 1, 1) * (f[q] + f[q + dj]) - 0.5 * C21(o, k, j, 1) * (f[q] + f[q - dj]);
+}
+
+static void les_ghosts(const ora* o, double* a) {
+  for (int k = 0; k <= o->NZ + 1; ++k)
+    for (int i = 0; i < o->NX2; ++i) {
+      a[IDX(o, i, k, o->NY + 1)] = a[IDX(o, i, k, o->NY)];
+      a[IDX(o, i, k, 0)] = a[IDX(o, i, k, 1)];
+    }
+  for (int j = 0; j <= o->NY + 1; ++j)
+    for (int i = 0; i < o->NX2; ++i) {
+      a[IDX(o, i, o->NZ + 1, j)] = a[IDX(o, i, o->NZ, j)];
+      a[IDX(o, i, 0, j)] = a[IDX(o, i, 1, j)];
+    }
+}
+
+/* les_chan (src/les_chan.F90:1-1621, model 2): on entry CU1..3 in Fourier
+ * space, CF1..3 holding the explicit terms; on exit U1..3 in physical
+ * space, NU_T set and the divergence of the subgrid stress added to CF. */
+static void les_chan(ora* o, int cur_step, int rk) {
+  const int js = o->jstart, je = o->jend, zs = o->zstart, ze = o->zend, NXC = o->NXC;
+  const size_t dj = (size_t)o->NX2 * o->NZ2, dk = o->NX2;
+  /* strain rate in Fourier space (compute_strain_chan :1623-1717), interior */
+  for (int q = 0; q < 6; ++q) memset(o->Sij[q], 0, o->npts * sizeof(double));
+  const double *CU1 = o->U[0], *CU2 = o->U[1], *CU3 = o->U[2];
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k) {
+      const double Jc = JAC(o, k, j);
+      for (int m = 0; m < NXC; ++m) {
+        const double kx = o->kx[m];
+        for (int c = 0; c < 2; ++c) {
+          const size_t q = CRE(o, m, k, j) + c, qo = CRE(o, m, k, j) + (1 - c);
+          const double sg = c == 0 ? -1.0 : 1.0; /* (i kx f)_re = -kx f_im, (i kx f)_im = kx f_re */
+          o->Sij[0][q] = sg * kx * CU1[qo];
+          o->Sij[1][q] = les_d2(o, CU2, q, k, j, dk, dj) / Jc;
+          o->Sij[2][q] = les_d3(o, CU3, q, k, j, dk, dj) / Jc;
+          o->Sij[3][q] = 0.5 * (les_d2(o, CU1, q, k, j, dk, dj) / Jc + sg * kx * CU2[qo]);
+          o->Sij[4][q] = 0.5 * (les_d3(o, CU1, q, k, j, dk, dj) / Jc + sg * kx * CU3[qo]);
+          o->Sij[5][q] = 0.5 * (les_d3(o, CU2, q, k, j, dk, dj) + les_d2(o, CU3, q, k, j, dk, dj)) / Jc;
+        }
+      }
+    }
+  for (int q = 0; q < 6; ++q) inv(o, o->Sij[q]);
+  /* |S| at the interior points (:351-364), extended to the y ghosts (:366-371) */
+  memset(o->SABS, 0, o->npts * sizeof(double));
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k)
+      for (int i = 0; i < o->NX2; ++i) {
+        const size_t q = IDX(o, i, k, j);
+        const double s1 = o->Sij[0][q], s2 = o->Sij[1][q], s3 = o->Sij[2][q];
+        const double s4 = o->Sij[3][q], s5 = o->Sij[4][q], s6 = o->Sij[5][q];
+        o->SABS[q] = sqrt(2.0 * s1 * s1 + 4.0 * s4 * s4 + 4.0 * s5 * s5 + 2.0 * s2 * s2 + 4.0 * s6 * s6 + 2.0 * s3 * s3);
+      }
+  for (int k = 0; k <= o->NZ + 1; ++k)
+    for (int i = 0; i < o->NX2; ++i) {
+      o->SABS[IDX(o, i, k, 0)] = o->SABS[IDX(o, i, k, 1)];
+      o->SABS[IDX(o, i, k, o->NY + 1)] = o->SABS[IDX(o, i, k, o->NY)];
+    }
+  /* velocities to physical space and test-filtered (:412-464) */
+  for (int f = 0; f < 3; ++f) inv(o, o->U[f]);
+  for (int f = 0; f < 3; ++f) les_filter(o, o->U[f], o->UBT[f]);
+  const double *U1 = o->U[0], *U2 = o->U[1], *U3 = o->U[2];
+  /* dynamic coefficient every 5th step at the first substep, or on the
+   * first step of a run (:475-814) */
+  if (((cur_step % 5) == 0 && rk == 1) || o->first_time) {
+    const size_t np = o->npts;
+    double* S2B = dalloc(np);
+    double* T1 = dalloc(np);
+    double* T2 = dalloc(np);
+    double* M = dalloc(np);
+    double* nsum = dalloc((size_t)o->NZ2 * o->NY2);
+    double* dsum = dalloc((size_t)o->NZ2 * o->NY2);
+    les_filter(o, o->SABS, S2B);
+    static const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+    for (int ij = 0; ij < 6; ++ij) {
+      const double cross = ij < 3 ? 1.0 : 2.0;
+      les_filter(o, o->Sij[ij], T1);
+      for (int j = 0; j <= o->NY + 1; ++j)
+        for (int k = 0; k <= o->NZ + 1; ++k) {
+          const double D = o->DELTA[PL(o, k, j)];
+          for (int i = 0; i < o->NX2; ++i) {
+            const size_t q = IDX(o, i, k, j);
+            T1[q] = T1[q] * ((LES_ALPHA * D) * (LES_ALPHA * D)) * S2B[q];
+            M[q] = (D * D) * o->SABS[q] * o->Sij[ij][q];
+          }
+        }
+      les_filter(o, M, M);
+      for (size_t q = 0; q < np; ++q) M[q] = T1[q] - M[q];
+      const double *Ua = o->U[ia[ij]], *Ub = o->U[ib[ij]];
+      for (size_t q = 0; q < np; ++q) T2[q] = Ua[q] * Ub[q];
+      les_filter(o, T2, T2);
+      const double *Ba = o->UBT[ia[ij]], *Bb = o->UBT[ib[ij]];
+      for (int j = 0; j <= o->NY + 1; ++j)
+        for (int k = 0; k <= o->NZ + 1; ++k) {
+          double sn = 0.0, sd = 0.0;
+          for (int i = 0; i < o->NX; ++i) {
+            const size_t q = IDX(o, i, k, j);
+            sn += M[q] * (T2[q] - Ba[q] * Bb[q]);
+            sd += M[q] * M[q];
+          }
+          dsum[PL(o, k, j)] += cross * sd;
+          nsum[PL(o, k, j)] += cross * sn;
+        }
+    }
+    memset(o->CDYN, 0, (size_t)o->NZ2 * o->NY2 * sizeof(double));
+    for (int j = js; j <= je; ++j)
+      for (int k = zs; k <= ze; ++k) {
+        const size_t t = PL(o, k, j);
+        o->CDYN[t] = dsum[t] != 0.0 ? -0.5 * nsum[t] / dsum[t] : 0.0;
+      }
+    for (size_t t = 0; t < (size_t)o->NZ2 * o->NY2; ++t)
+      if (o->CDYN[t] < 0.0) o->CDYN[t] = 0.0;
+    free(S2B); free(T1); free(T2); free(M); free(nsum); free(dsum);
+  }
+  /* eddy viscosity (:825-831) */
+  memset(o->NUT, 0, o->npts * sizeof(double));
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k) {
+      const size_t t = PL(o, k, j);
+      const double D = o->DELTA[t];
+      for (int i = 0; i < o->NX2; ++i) o->NUT[IDX(o, i, k, j)] = o->CDYN[t] * (D * D) * o->SABS[IDX(o, i, k, j)];
+    }
+  /* subgrid stress, the implicit diagonal parts removed (:833-972) */
+  for (int ij = 0; ij < 6; ++ij) {
+    double* S = o->Sij[ij];
+    if (ij == 0 || ij == 1 || ij == 2) {
+      const int j0 = ij == 2 ? 0 : 1;
+      for (int j = j0; j <= o->NY + 1; ++j)
+        for (int k = 1; k <= o->NZ + 1; ++k) {
+          const size_t t = PL(o, k, j);
+          const double D = o->DELTA[t], C = o->CDYN[t];
+          for (int i = 0; i < o->NX2; ++i) {
+            const size_t q = IDX(o, i, k, j);
+            S[q] = ij == 0 ? ((-2.0 * C) * (D * D)) * o->SABS[q] * S[q] : 0.0;
+          }
+        }
+    } else {
+      const double* V = ij == 5 ? U2 : U1;
+      for (int j = js; j <= je; ++j)
+        for (int k = zs; k <= ze; ++k) {
+          const size_t t = PL(o, k, j);
+          const double D = o->DELTA[t], C = o->CDYN[t], Jc = JAC(o, k, j);
+          for (int i = 0; i < o->NX2; ++i) {
+            const size_t q = IDX(o, i, k, j);
+            const double g = ij == 3 ? les_d2(o, V, q, k, j, dk, dj) : les_d3(o, V, q, k, j, dk, dj);
+            S[q] = ((-2.0 * C) * (D * D)) * o->SABS[q] * (S[q] - (0.5 * g) / Jc);
+          }
+        }
+    }
+    les_ghosts(o, S);
+  }
+  (void)U3;
+  /* to Fourier space (the stress of ij = 2, 3 is zero) and the explicit
+   * divergence into CF (:1169-1246, :1313-1356) */
+  fwd(o, o->Sij[0]);
+  fwd(o, o->Sij[3]);
+  fwd(o, o->Sij[4]);
+  fwd(o, o->Sij[5]);
+  memset(o->Sij[1], 0, o->npts * sizeof(double));
+  memset(o->Sij[2], 0, o->npts * sizeof(double));
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k) {
+      const double Jc = JAC(o, k, j);
+      for (int m = 0; m < NXC; ++m) {
+        const double kx = o->kx[m];
+        for (int c = 0; c < 2; ++c) {
+          const size_t q = CRE(o, m, k, j) + c, qo = CRE(o, m, k, j) + (1 - c);
+          const double sg = c == 0 ? -1.0 : 1.0;
+          const double *C1 = o->Sij[0], *C2 = o->Sij[1], *C3 = o->Sij[2], *C4 = o->Sij[3], *C5 = o->Sij[4],
+                       *C6 = o->Sij[5];
+          o->CF[0][q] = ((o->CF[0][q] - Jc * (sg * kx * C1[qo])) - les_d2(o, C4, q, k, j, dk, dj)) -
+                        les_d3(o, C5, q, k, j, dk, dj);
+          o->CF[1][q] = ((o->CF[1][q] - Jc * (sg * kx * C4[qo])) - les_d2(o, C2, q, k, j, dk, dj)) -
+                        les_d3(o, C6, q, k, j, dk, dj);
+          o->CF[2][q] = ((o->CF[2][q] - Jc * (sg * kx * C5[qo])) - les_d2(o, C6, q, k, j, dk, dj)) -
+                        les_d3(o, C3, q, k, j, dk, dj);
+        }
+      }
+    }
+}
+
+int ora_get_les(ora* o, double* nut, double* cdyn) {
+  if (!o->p.les_model) return EKM_ERR_ARG;
+  if (nut) memcpy(nut, o->NUT, o->npts * sizeof(double));
+  if (cdyn) memcpy(cdyn, o->CDYN, (size_t)o->NZ2 * o->NY2 * sizeof(double));
+  return EKM_OK;
+}
+
 int ora_rk_duct(ora* o) {
   const ekm_params* p = &o->p;
   int rk = o->rk_step, NXC = o->NXC;
@@ -2068,8 +2316,13 @@ int ora_rk_duct(ora* o) {
           }
         }
   }
-  /* :655-671 velocities and scalar to physical space */
-  for (int f = 0; f < 3; ++f) inv(o, o->U[f]);
+  /* :641-671 the subgrid model (which leaves the velocities in physical
+   * space), or the velocities to physical space; the scalar to physical space */
+  const int les = p->les_model && (!o->create_new_flow || cur_step > 1);
+  if (les)
+    les_chan(o, cur_step, rk);
+  else
+    for (int f = 0; f < 3; ++f) inv(o, o->U[f]);
   if (th) inv(o, CTH);
   double *U1 = CU1, *U2 = CU2, *U3 = CU3;
   /* :683-1120 momentum fluxes */
@@ -2131,6 +2384,7 @@ int ora_rk_duct(ora* o) {
     o->time = o->time + o->delta_t;
     o->time_step = cur_step;
     o->rk_step = 1;
+    o->first_time = 0;
   } else {
     o->rk_step = rk + 1;
   }
@@ -2158,6 +2412,8 @@ int ora_set_state(ora* o, const double* cu1, const double* cu2, const double* cu
   o->time = time;
   o->time_step = time_step;
   o->rk_step = 1;
+  o->create_new_flow = 0; /* a restart: the subgrid model runs from the first step */
+  o->first_time = 1;
   return EKM_OK;
 }
 
@@ -2183,6 +2439,8 @@ int ora_set_initial_physical(ora* o, const double* u1, const double* u2, const d
   for (int f = 0; f < 3; ++f) memset(o->CF[f], 0, n);
   memset(o->CFTH, 0, n);
   o->time = 0.0; o->time_step = 0; o->rk_step = 1;
+  o->create_new_flow = 1;
+  o->first_time = 1;
   return rem_div(o);
 }
 

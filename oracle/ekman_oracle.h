@@ -52,6 +52,7 @@ int ora_get_state(ora* o, double* cu1, double* cu2, double* cu3, double* cp, dou
 int ora_set_initial_physical(ora* o, const double* u1, const double* u2,
                              const double* u3, const double* th);
 int ora_rk_duct(ora* o);
+int ora_get_les(ora* o, double* nut, double* cdyn);
 int ora_time_step(ora* o, int nsteps);
 int ora_rem_div_curv(ora* o);
 int ora_get_phi(ora* o, double* phi);

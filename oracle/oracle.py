@@ -47,6 +47,7 @@ def lib():
         L.ora_time_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ora_set_state.argtypes = [ctypes.c_void_p, _D, _D, _D, _D, _D, ctypes.c_double, ctypes.c_int]
         L.ora_get_state.argtypes = [ctypes.c_void_p, _D, _D, _D, _D, _D]
+        L.ora_get_les.argtypes = [ctypes.c_void_p, _D, _D]
         L.ora_set_initial_physical.argtypes = [ctypes.c_void_p, _D, _D, _D, _D]
         L.ora_get_phi.argtypes = [ctypes.c_void_p, _D]
         L.ora_get_diag.argtypes = [ctypes.c_void_p, ctypes.POINTER(EkmDiag)]
@@ -126,6 +127,16 @@ class Oracle:
         st = lib().ora_set_state(self.h, *[_c(a) for a in arrs], _c(th), float(time), int(time_step))
         if st:
             raise RuntimeError(f"ora_set_state: {st}")
+
+    def get_les(self):
+        """(NU_T physical (NY+2, NZ+2, NX+2), C_DYN (NY+2, NZ+2))."""
+        p = self.p
+        nut = np.zeros((p.ny + 2, p.nz + 2, p.nx + 2))
+        cdyn = np.zeros((p.ny + 2, p.nz + 2))
+        st = lib().ora_get_les(self.h, _p(nut), _p(cdyn))
+        if st != 0:
+            raise RuntimeError(f"ora_get_les: {st}")
+        return nut, cdyn
 
     def get_state(self):
         out = [self._zeros_c() for _ in range(5)]

@@ -459,3 +459,64 @@ def test_tight_projection_removes_divergence(oracle_lib):
     o0 = O.Oracle(p0, xp, yp)
     o0.set_initial_physical(u1, u2, u3, th)
     assert o0.diag().div_ref > 1e-8
+
+
+# ---------------------------------------------------------------------------
+# LES: dynamic Smagorinsky model of the velocity (src/les_chan.F90 model 2)
+# ---------------------------------------------------------------------------
+def _les_pair(noise, time_step=3):
+    """Oracle objects without and with the subgrid model, both restarted from
+    the same projected state of config 1 (a restart, so the model runs from
+    the first step and recomputes its coefficient there)."""
+    from oracle import oracle as O
+    cfg = config(1)
+    p = cfg.params
+    xp, yp = tanh_grid(p.ny, p.nz, p.ly, p.lz, cfg.gamma)
+    ic = ekman_initial(p, yp, cfg.seed, noise, cfg.th_noise)
+    o0 = O.Oracle(p, xp, yp)
+    o0.set_initial_physical(*ic)
+    st = o0.get_state()
+    pl = p.copy()
+    pl.les_model = 2
+    oL = O.Oracle(pl, xp, yp)
+    oL.set_state(*st[:4], st[4], time=0.0, time_step=time_step)
+    oD = O.Oracle(p, xp, yp)
+    oD.set_state(*st[:4], st[4], time=0.0, time_step=time_step)
+    return p, oD, oL
+
+
+def test_les_vanishes_without_x_fluctuations(oracle_lib):
+    """On an x-independent flow (the laminar Ekman spiral) the test-filtered
+    products equal the products of the filtered velocities, so the dynamic
+    coefficient is zero up to rounding and a substep with the model is the
+    plain substep (fields agree to 1e-12 relative L2)."""
+    p, oD, oL = _les_pair(noise=0.0)
+    oD.rk_duct()
+    oL.rk_duct()
+    nut, cdyn = oL.get_les()
+    assert np.abs(cdyn).max() < 1e-10
+    assert np.abs(nut).max() < 1e-12
+    for a, b in zip(oL.get_state(), oD.get_state()):
+        n = np.linalg.norm(b)
+        assert np.linalg.norm(a - b) <= 1e-12 * (n if n > 0 else 1.0)
+
+
+def test_les_coefficient_and_viscosity_signs(oracle_lib):
+    """With resolved x fluctuations (5% noise) the clipped coefficient is
+    nonnegative, positive somewhere, and zero outside the interior index
+    ranges; NU_T = C_DYN Delta^2 |S| is nonnegative and zero outside the
+    interior (src/les_chan.F90:798-831)."""
+    p, oD, oL = _les_pair(noise=0.05)
+    oL.rk_duct()
+    nut, cdyn = oL.get_les()
+    assert (cdyn >= 0.0).all() and (nut >= 0.0).all()
+    assert cdyn.max() > 0.0 and nut.max() > 0.0
+    # config 1: interior rows J = 1..NY, K = 1..NZ (ranges of its BC types)
+    assert np.all(cdyn[0, :] == 0.0) and np.all(cdyn[-1, :] == 0.0)
+    assert np.all(cdyn[:, 0] == 0.0) and np.all(cdyn[:, -1] == 0.0)
+    assert np.all(nut[0] == 0.0) and np.all(nut[-1] == 0.0)
+    # the model changes the step
+    a, b = oL.get_state()[0], None
+    oD.rk_duct()
+    b = oD.get_state()[0]
+    assert np.linalg.norm(a - b) > 0.0
