@@ -234,15 +234,24 @@ __global__ void __launch_bounds__(kBlock) k_fft_inv(double* __restrict__ v, long
 // :1127-1149), so the two flux fields never round-trip through HBM.  The
 // arithmetic of every value is that of the separate flux, transform and
 // accumulation kernels.
+// LES adds the subgrid model's eddy-viscosity cross terms (:990-1120) to Syz
+// before the transform: with the face averages of NU_T and GL = GMAT_12
+// (U1), GMAT_12_y (U2) or GMAT_12_z (U3),
+//   Sl = 1/4 [GL(K+1) nu(K+1/2) Q_eta(K+1/2) - GL(K) nu(K-1/2) Q_eta(K-1/2)
+//           + GL(J+1) nu(J+1/2) Q_zeta(J+1/2) - GL(J) nu(J-1/2) Q_zeta(J-1/2)].
+// The reference transforms Sl separately and adds it to CF afterwards; the
+// transform is linear, so the two differ by rounding only.
 #define MET(arr, k, j, f) M.arr[mf(d, k, j, f)]
-template <int R>
+template <int R, bool LES>
 __global__ void __launch_bounds__(kBlock) k_flux_accum(Dims d, Metrics M, Ranges rg, const double* __restrict__ U1,
                                                        const double* __restrict__ Q, const double* __restrict__ U2b,
                                                        const double* __restrict__ U3b, double nu,
                                                        const double* __restrict__ kxv, double2* __restrict__ CF,
                                                        double2* __restrict__ CR, double temp5, int logm,
                                                        const double2* __restrict__ tw,
-                                                       const double2* __restrict__ tw2) {
+                                                       const double2* __restrict__ tw2,
+                                                       const double* __restrict__ nut,
+                                                       const double* __restrict__ gl) {
   extern __shared__ double2 lds[];
   const int nx = d.NX, M2 = nx >> 1, GW = M2 < 64 ? M2 : 64, PW = 64 / GW;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -257,6 +266,13 @@ __global__ void __launch_bounds__(kBlock) k_flux_accum(Dims d, Metrics M, Ranges
   const double Jc = M.jac[mi(d, k, j)];
   const double g12kp = MET(g12, k + 1, j, 2), g12k = MET(g12, k, j, 2);
   const double g12jp = MET(g12, k, j + 1, 1), g12j = MET(g12, k, j, 1);
+  double glkp = 0.0, glk = 0.0, gljp = 0.0, glj = 0.0;
+  if constexpr (LES) {
+    glkp = gl[mf(d, k + 1, j, 2)];
+    glk = gl[mf(d, k, j, 2)];
+    gljp = gl[mf(d, k, j + 1, 1)];
+    glj = gl[mf(d, k, j, 1)];
+  }
   const LaneTw lt = lane_twiddles(lanep, M2, tw);
   double2 zx[R], zy[R];
 #pragma unroll
@@ -282,6 +298,16 @@ __global__ void __launch_bounds__(kBlock) k_flux_accum(Dims d, Metrics M, Ranges
       const double QKP[2] = {qkp.x, qkp.y}, QKM[2] = {qkm.x, qkm.y}, QPP[2] = {qpp.x, qpp.y};
       const double QPM[2] = {qpm.x, qpm.y}, QMP[2] = {qmp.x, qmp.y}, QMM[2] = {qmm.x, qmm.y};
       const double VJP[2] = {vjp.x, vjp.y}, VJ[2] = {vj.x, vj.y}, WKP[2] = {wkp.x, wkp.y}, WK[2] = {wk.x, wk.y};
+      double NC[2] = {0.0, 0.0}, NKP[2] = {0.0, 0.0}, NKM[2] = {0.0, 0.0}, NJP[2] = {0.0, 0.0}, NJM[2] = {0.0, 0.0};
+      if constexpr (LES) {
+        const double2 nc = *reinterpret_cast<const double2*>(nut + q);
+        const double2 nkp = *reinterpret_cast<const double2*>(nut + q + dk);
+        const double2 nkm = *reinterpret_cast<const double2*>(nut + q - dk);
+        const double2 njp = *reinterpret_cast<const double2*>(nut + q + dj);
+        const double2 njm = *reinterpret_cast<const double2*>(nut + q - dj);
+        NC[0] = nc.x; NC[1] = nc.y; NKP[0] = nkp.x; NKP[1] = nkp.y; NKM[0] = nkm.x; NKM[1] = nkm.y;
+        NJP[0] = njp.x; NJP[1] = njp.y; NJM[0] = njm.x; NJM[1] = njm.y;
+      }
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         sx[h] = Jc * UU[h] * QC[h];
@@ -290,6 +316,11 @@ __global__ void __launch_bounds__(kBlock) k_flux_accum(Dims d, Metrics M, Ranges
         s1 = -s1 + 0.25 * nu * (g12kp * (QPP[h] + QJP[h] - QJM[h] - QPM[h]) - g12k * (QMP[h] + QJP[h] - QJM[h] - QMM[h]));
         s2 = -s2 + 0.25 * nu * (g12jp * (QPP[h] + QKP[h] - QMP[h] - QKM[h]) - g12j * (QKP[h] + QPM[h] - QKM[h] - QMM[h]));
         sy[h] = s1 + s2;
+        if constexpr (LES)
+          sy[h] = sy[h] + 0.25 * (glkp * 0.5 * (NKP[h] + NC[h]) * (QPP[h] + QJP[h] - QJM[h] - QPM[h]) -
+                                  glk * 0.5 * (NC[h] + NKM[h]) * (QMP[h] + QJP[h] - QJM[h] - QMM[h]) +
+                                  gljp * 0.5 * (NC[h] + NJP[h]) * (QPP[h] + QKP[h] - QMP[h] - QKM[h]) -
+                                  glj * 0.5 * (NC[h] + NJM[h]) * (QKP[h] + QPM[h] - QKM[h] - QMM[h]));
       }
     }
     zx[r] = make_double2(sx[0], sx[1]);
@@ -360,7 +391,7 @@ __global__ void __launch_bounds__(kBlock) k_flux_accum(Dims d, Metrics M, Ranges
 template <int R>
 int launch_flux_accum_r(const FftPlan& p, const Dims& d, const Metrics& M, const Ranges& rg, const double* U1,
                         const double* Q, const double* U2b, const double* U3b, double nu, const double* kx,
-                        double* CF, double* CR, double temp5, hipStream_t st) {
+                        double* CF, double* CR, double temp5, const double* nut, const double* gl, hipStream_t st) {
   const int M2 = p.nx / 2, GW = M2 < 64 ? M2 : 64, PW = 64 / GW;
   const long npen = (long)(rg.ze - rg.zs + 1) * (rg.je - rg.js + 1);
   if (npen <= 0) return EKM_OK;
@@ -369,8 +400,14 @@ int launch_flux_accum_r(const FftPlan& p, const Dims& d, const Metrics& M, const
   const size_t lds = (size_t)2 * kWaves * PW * M2 * sizeof(double2);
   int logm = 0;
   while ((1 << logm) < M2) ++logm;
-  hipLaunchKernelGGL(k_flux_accum<R>, dim3((unsigned)nb), dim3(kBlock), lds, st, d, M, rg, U1, Q, U2b, U3b, nu, kx,
-                     (double2*)CF, (double2*)CR, temp5, logm, (const double2*)p.tw, (const double2*)p.tw2);
+  if (nut)
+    hipLaunchKernelGGL((k_flux_accum<R, true>), dim3((unsigned)nb), dim3(kBlock), lds, st, d, M, rg, U1, Q, U2b, U3b,
+                       nu, kx, (double2*)CF, (double2*)CR, temp5, logm, (const double2*)p.tw, (const double2*)p.tw2,
+                       nut, gl);
+  else
+    hipLaunchKernelGGL((k_flux_accum<R, false>), dim3((unsigned)nb), dim3(kBlock), lds, st, d, M, rg, U1, Q, U2b,
+                       U3b, nu, kx, (double2*)CF, (double2*)CR, temp5, logm, (const double2*)p.tw,
+                       (const double2*)p.tw2, nullptr, nullptr);
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }
@@ -436,10 +473,11 @@ int fft_inverse(const FftPlan& p, double* v, long npencil, hipStream_t st) { ret
 
 int launch_flux_accum(const FftPlan& p, const Dims& d, const Metrics& M, const Ranges& r, const double* U1,
                       const double* Q, const double* U2b, const double* U3b, double nu, const double* kx, double* CF,
-                      double* CR, double temp5, hipStream_t st) {
+                      double* CR, double temp5, hipStream_t st, const double* nut, const double* gl) {
   const int M2 = p.nx / 2;
-  if (M2 <= 64) return launch_flux_accum_r<1>(p, d, M, r, U1, Q, U2b, U3b, nu, kx, CF, CR, temp5, st);
-  if (M2 == 128) return launch_flux_accum_r<2>(p, d, M, r, U1, Q, U2b, U3b, nu, kx, CF, CR, temp5, st);
-  if (M2 == 256) return launch_flux_accum_r<4>(p, d, M, r, U1, Q, U2b, U3b, nu, kx, CF, CR, temp5, st);
-  return launch_flux_accum_r<8>(p, d, M, r, U1, Q, U2b, U3b, nu, kx, CF, CR, temp5, st);
+  if (nut && !gl) return EKM_ERR_ARG;
+  if (M2 <= 64) return launch_flux_accum_r<1>(p, d, M, r, U1, Q, U2b, U3b, nu, kx, CF, CR, temp5, nut, gl, st);
+  if (M2 == 128) return launch_flux_accum_r<2>(p, d, M, r, U1, Q, U2b, U3b, nu, kx, CF, CR, temp5, nut, gl, st);
+  if (M2 == 256) return launch_flux_accum_r<4>(p, d, M, r, U1, Q, U2b, U3b, nu, kx, CF, CR, temp5, nut, gl, st);
+  return launch_flux_accum_r<8>(p, d, M, r, U1, Q, U2b, U3b, nu, kx, CF, CR, temp5, nut, gl, st);
 }

@@ -66,7 +66,7 @@ int host_metrics(int NZ, int NY, const double* xp, const double* yp, HostMetrics
         ye(k, j, f) = tr(ye(k, j, f));
       }
   size_t mn = (size_t)NZ3 * NY3;
-  for (int w = 0; w < 7; ++w) out.m[w].assign(mn * 2, NAN);
+  for (int w = 0; w < 13; ++w) out.m[w].assign(mn * 2, NAN);
   out.jac.assign(mn, NAN);
   auto M = [&](int w, int k, int j, int f) -> double& { return out.m[w][(size_t)k + (size_t)NZ3 * ((size_t)j + (size_t)NY3 * (f - 1))]; };
   auto JC = [&](int k, int j) -> double& { return out.jac[(size_t)k + (size_t)NZ3 * j]; };
@@ -85,6 +85,14 @@ int host_metrics(int NZ, int NY, const double* xp, const double* yp, HostMetrics
         M(4, k, j, f) = zy * J;
         M(5, k, j, f) = ex * J;
         M(6, k, j, f) = ey * J;
+        // subgrid-stress variants (:1379-1403): U3's stress doubles the
+        // zeta_x / eta_x products, U2's the zeta_y / eta_y products
+        M(7, k, j, f) = (2.0 * zx * zx + zy * zy) * J;
+        M(8, k, j, f) = (2.0 * zx * ex + zy * ey) * J;
+        M(9, k, j, f) = (2.0 * ex * ex + ey * ey) * J;
+        M(10, k, j, f) = (zx * zx + 2.0 * zy * zy) * J;
+        M(11, k, j, f) = (zx * ex + 2.0 * zy * ey) * J;
+        M(12, k, j, f) = (ex * ex + 2.0 * ey * ey) * J;
       }
   // cell Jacobian and its extrapolation (:1268-1292)
   for (int j = 1; j <= NY; ++j)
@@ -100,18 +108,19 @@ int host_metrics(int NZ, int NY, const double* xp, const double* yp, HostMetrics
   }
   JC(0, 0) = 0.5 * (2.0 * JC(1, 0) - JC(2, 0) + 2.0 * JC(0, 1) - JC(0, 2));
   JC(NZ + 1, NY + 1) = 0.5 * (2.0 * JC(NZ, NY + 1) - JC(NZ - 1, NY + 1) + 2.0 * JC(NZ + 1, NY) - JC(NZ + 1, NY - 1));
-  // ghost faces (:1294-1340); GMAT_12 keeps its extrapolated corner because
-  // the reference's corner formula assigns GMAT_22 twice
-  for (int w = 0; w < 7; ++w)
+  // ghost faces (:1294-1340, variants :1405-1448); GMAT_12 and its variants
+  // keep their extrapolated corner because the reference's corner formulas
+  // assign GMAT_22 (GMAT_22_z, GMAT_22_y) twice
+  for (int w = 0; w < 13; ++w)
     for (int f = 1; f <= 2; ++f) {
       for (int j = 0; j <= NY + 1; ++j) M(w, 0, j, f) = 2.0 * M(w, 1, j, f) - M(w, 2, j, f);
       for (int k = 0; k <= NZ + 1; ++k) M(w, k, 0, f) = 2.0 * M(w, k, 1, f) - M(w, k, 2, f);
     }
-  for (int w : {0, 2, 3, 4, 5, 6})
+  for (int w : {0, 2, 3, 4, 5, 6, 7, 9, 10, 12})
     for (int f = 1; f <= 2; ++f)
       M(w, 0, 0, f) = 0.5 * (2.0 * M(w, 0, 1, f) - M(w, 0, 2, f) + 2.0 * M(w, 1, 0, f) - M(w, 2, 0, f));
-  // copies for the coefficient callback (:1343-1376)
-  for (int w = 0; w < 7; ++w)
+  // copies for the coefficient callback (:1343-1376, variants :1450-1474)
+  for (int w = 0; w < 13; ++w)
     for (int f = 1; f <= 2; ++f) {
       for (int k = 0; k <= NZ + 1; ++k) M(w, k, NY + 2, f) = M(w, k, NY + 1, f);
       for (int j = 0; j <= NY + 1; ++j) M(w, NZ + 2, j, f) = M(w, NZ + 1, j, f);
@@ -383,6 +392,15 @@ extern "C" int ekm_host_metrics(int nz, int ny, const double* xpoint, const doub
   size_t mn = (size_t)(nz + 3) * (ny + 3);
   for (int w = 0; w < 7; ++w) std::memcpy(out + (size_t)w * 2 * mn, hm.m[w].data(), 2 * mn * sizeof(double));
   std::memcpy(out + 14 * mn, hm.jac.data(), mn * sizeof(double));
+  return EKM_OK;
+}
+
+extern "C" int ekm_host_les_metrics(int nz, int ny, const double* xpoint, const double* ypoint, double* out) {
+  HostMetrics hm;
+  int st = host_metrics(nz, ny, xpoint, ypoint, hm);
+  if (st) return st;
+  size_t mn = (size_t)(nz + 3) * (ny + 3);
+  for (int w = 0; w < 6; ++w) std::memcpy(out + (size_t)w * 2 * mn, hm.m[7 + w].data(), 2 * mn * sizeof(double));
   return EKM_OK;
 }
 

@@ -5,7 +5,9 @@
 #include "ekm_internal.h"
 
 struct HostMetrics {
-  std::vector<double> m[7];  // GMAT_11,12,22, CJOB_11,12,21,22: (0:NZ+2,0:NY+2,1:2)
+  // GMAT_11,12,22, CJOB_11,12,21,22, then the subgrid-stress variants
+  // GMAT_11_z,12_z,22_z,11_y,12_y,22_y: (0:NZ+2,0:NY+2,1:2)
+  std::vector<double> m[13];
   std::vector<double> jac;   // INT_JACOB (0:NZ+2,0:NY+2)
 };
 

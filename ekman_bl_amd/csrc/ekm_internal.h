@@ -85,10 +85,13 @@ void fft_plan_free(FftPlan& p);
 // in-place transforms of npencil pencils of stride NX+2
 int fft_forward(const FftPlan& p, double* v, long npencil, hipStream_t st);
 int fft_inverse(const FftPlan& p, double* v, long npencil, hipStream_t st);
-// fused flux -> forward transform -> CF/CR accumulation of one transported field
+// fused flux -> forward transform -> CF/CR accumulation of one transported
+// field; nut (with its cross-term metric gl) adds the subgrid viscosity's
+// cross terms
 int launch_flux_accum(const FftPlan& p, const Dims& d, const Metrics& M, const Ranges& r, const double* U1,
                       const double* Q, const double* U2b, const double* U3b, double nu, const double* kx, double* CF,
-                      double* CR, double temp5, hipStream_t st);
+                      double* CR, double temp5, hipStream_t st, const double* nut = nullptr,
+                      const double* gl = nullptr);
 
 // ---- multigrid --------------------------------------------------------------
 struct HostMetrics;

@@ -23,7 +23,11 @@ int launch_contravariant(const Dims& d, const Metrics& M, const Ranges& r, const
                          const double* U3, double* U2b, double* U3b, hipStream_t st);
 // One field of the batched ADI stage: its index ranges, the boundary rows of
 // its y and z systems, the field, its right-hand side, a scratch field S and
-// 7 * NZ2*NY2 doubles of line factors.
+// 7 * NZ2*NY2 doubles of line factors.  With a subgrid viscosity (nut not
+// null) the systems differ from point to point along x: gz and gy are the
+// metrics of its z- and y-direction terms (GMAT_11 / GMAT_22 for U1, the _y
+// variants for U2, the _z variants for U3) and pv is a P field of scratch for
+// the pivots; lu is then unused.
 struct AdiField {
   Ranges r;
   BcTable by, bz;
@@ -31,10 +35,15 @@ struct AdiField {
   double* R;
   double* S;
   double* lu;
+  const double* nut;
+  const double* gz;
+  const double* gy;
+  double* pv;
 };
 constexpr int kAdiMaxFields = 4;
 struct AdiT1 {
   double t1[kAdiMaxFields];  // implicit weight per field
+  double t2[kAdiMaxFields];  // subgrid-viscosity weight H_BAR/2 per field
 };
 // The ADI sweeps of nf independent fields in one set of launches; fs: nf
 // descriptors in device memory, hf the same on the host (for the grid sizes).

@@ -8,8 +8,9 @@
 // consecutive kx modes the same way.  The ADI sweeps give every thread one
 // (i,k) or (i,j) line: for a fixed row the threads of a wavefront touch
 // consecutive i, so each step of the sequential recurrence is a coalesced
-// row load.  The tridiagonal factors do not depend on i (no eddy viscosity in
-// the plain step), so they are computed once per line by k_lu and shared.
+// row load.  Without eddy viscosity the tridiagonal factors do not depend on
+// i, so they are computed once per line by k_lu and shared; with it every
+// point's system is formed inside its sweep.
 #include "ekm_kernels.h"
 
 namespace {
@@ -178,7 +179,15 @@ __global__ void k_adi_pre(Dims d, Metrics M, const AdiField* __restrict__ fs, Ad
   double Jc = JAC(k, j), u = U[q];
   double s = R[q] - Jc * u;
   S[q] = s;
-  R[q] = 0.5 * s + Jc * u + t1 * (G11(k + 1, j, 2) * (U[q + dk] - u) - G11(k, j, 2) * (u - U[q - dk]));
+  double rv = 0.5 * s + Jc * u + t1 * (G11(k + 1, j, 2) * (U[q + dk] - u) - G11(k, j, 2) * (u - U[q - dk]));
+  if (F.nut) {  // subgrid viscosity, z direction (:1667-1680 and analogues)
+    const double* __restrict__ nut = F.nut;
+    const double* __restrict__ gz = F.gz;
+    const double t2 = t1s.t2[blockIdx.y], nc = nut[q];
+    rv = rv + t2 * (0.5 * (nut[q + dk] + nc) * (gz[mf(d, k + 1, j, 2)] * (U[q + dk] - u)) -
+                    0.5 * (nc + nut[q - dk]) * (gz[mf(d, k, j, 2)] * (u - U[q - dk])));
+  }
+  R[q] = rv;
 }
 
 // Per-line factorisation of one sweep (THOMAS_REAL forward elimination,
@@ -196,6 +205,7 @@ __global__ void __launch_bounds__(kLuBlock) k_lu(Dims d, Metrics M, const AdiFie
                                                   int dir) {
   __shared__ double sa[kLuMaxRows], sb[kLuMaxRows], sc[kLuMaxRows];
   const AdiField& F = fs[blockIdx.y];
+  if (F.nut) return;  // point-dependent systems: factored inside the sweeps
   const Ranges r = F.r;
   const BcTable& bc = dir == 0 ? F.by : F.bz;
   const double t1 = t1s.t1[blockIdx.y];
@@ -287,6 +297,7 @@ constexpr int kSw = 8;
 
 __global__ void __launch_bounds__(256) k_ysweep(Dims d, Metrics M, const AdiField* __restrict__ fs, AdiT1 t1s) {
   const AdiField& F = fs[blockIdx.y];
+  if (F.nut) return;
   const Ranges r = F.r;
   const BcTable& bc = F.by;
   const double t1 = t1s.t1[blockIdx.y];
@@ -362,6 +373,7 @@ __global__ void __launch_bounds__(256) k_ysweep(Dims d, Metrics M, const AdiFiel
 // z sweep for lines (i,J) (:1503-1596 and analogues)
 __global__ void __launch_bounds__(256) k_zsweep(Dims d, Metrics M, const AdiField* __restrict__ fs) {
   const AdiField& F = fs[blockIdx.y];
+  if (F.nut) return;
   const Ranges r = F.r;
   const BcTable& bc = F.bz;
   const size_t ly = (size_t)d.NZ2 * d.NY2;
@@ -422,6 +434,243 @@ __global__ void __launch_bounds__(256) k_zsweep(Dims d, Metrics M, const AdiFiel
       const double x0 = (gg[u] - uu[u] * x1) / pp[u];
       U[q0 + dk * row] = x0;
       x1 = x0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sweeps with the subgrid viscosity (fields with nut): every (i, line) has its
+// own system, so each thread forms its rows as it eliminates -- the molecular
+// row, then the eddy-viscosity face terms (:1718-1731, :1819-1832 and the U2,
+// U3 analogues), then the boundary rows of APPLY_BC_* override -- and keeps
+// its pivots in the scratch field pv for the back substitution, which forms
+// the upper coefficients again.  Thomas arithmetic as THOMAS_REAL(_SP)
+// (src/boundary.F90:1-84).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_ysweep_les(Dims d, Metrics M, const AdiField* __restrict__ fs, AdiT1 t1s) {
+  const AdiField& F = fs[blockIdx.y];
+  if (!F.nut) return;
+  const Ranges r = F.r;
+  const BcTable& bc = F.by;
+  const double t1 = t1s.t1[blockIdx.y], t2 = t1s.t2[blockIdx.y], t2h = t2 * 0.5;
+  double* __restrict__ U = F.U;
+  double* __restrict__ R = F.R;
+  const double* __restrict__ S = F.S;
+  const double* __restrict__ nut = F.nut;
+  const double* __restrict__ gy = F.gy;
+  double* __restrict__ pv = F.pv;
+  int nk = r.ze - r.zs + 1;
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)d.NX2 * nk) return;
+  int i = (int)(t % d.NX2), K = r.zs + (int)(t / d.NX2);
+  const int N = d.NY + 1;
+  const size_t dj = (size_t)d.NX2 * d.NZ2;
+  const size_t q0 = ph(d, i, K, 0);
+  double lag[4];
+  for (int e = 0; e < 4; ++e) lag[e] = (bc.r[e].kind == BC_LAGGED) ? U[q0 + dj * bc.r[e].src] : 0.0;
+  double g = 0.0, pvp = 1.0, cp = 0.0;
+  double nm = 0.0, nc = nut[q0];  // NU_T at row-1 and row
+  for (int row0 = 0; row0 <= N; row0 += kSw) {
+    double rr[kSw], nn[kSw];
+#pragma unroll
+    for (int u = 0; u < kSw; ++u) {
+      const int row = row0 + u;
+      rr[u] = (row <= N && row >= r.js && row <= r.je) ? R[q0 + dj * row] : 0.0;
+      nn[u] = row + 1 <= N ? nut[q0 + dj * (row + 1)] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSw; ++u) {
+      const int row = row0 + u;
+      if (row > N) break;
+      const double np = nn[u];
+      double a = 0.0, b = 1.0, c = 0.0, rhs = rr[u];
+      if (row >= r.js && row <= r.je) {
+        const double gp = G22(K, row + 1, 1), g0 = G22(K, row, 1);
+        a = -(t1 * g0);
+        b = JAC(K, row) + t1 * (gp + g0);
+        c = -(t1 * gp);
+        const double wm = t2h * (nc + nm) * gy[mf(d, K, row, 1)];
+        const double wp = t2h * (nc + np) * gy[mf(d, K, row + 1, 1)];
+        a = a - wm;
+        b = b + wp + wm;
+        c = c - wp;
+      }
+      for (int e = 0; e < 4; ++e)
+        if (bc.r[e].row == row) {
+          a = bc.r[e].a;
+          b = bc.r[e].b;
+          c = bc.r[e].c;
+          rhs = bc_rhs(d, M, bc.r[e], K, lag[e]);
+        }
+      double p = b;
+      if (row == 0) {
+        g = rhs;
+      } else {
+        const double mlt = -a / pvp;
+        p = b + mlt * cp;
+        g = rhs + mlt * g;
+      }
+      U[q0 + dj * row] = g;
+      pv[q0 + dj * row] = p;
+      pvp = p;
+      cp = c;
+      nm = nc;
+      nc = np;
+    }
+  }
+  double x2 = 0.0, x1 = g / pvp;  // x1 = x(row+1), x2 = x(row+2)
+  U[q0 + dj * N] = x1;
+  double n1 = nut[q0 + dj * N], n2 = 0.0;  // NU_T at row+1, row+2
+  for (int row0 = N - 1; row0 >= 0; row0 -= kSw) {
+    double gg[kSw], pp[kSw], nl[kSw], ss[kSw], jc[kSw], gp[kSw], g0[kSw], lp[kSw], l0[kSw];
+#pragma unroll
+    for (int u = 0; u < kSw; ++u) {
+      const int row = row0 - u, jr = row + 1;
+      const bool ok = row >= 0;
+      const bool in = ok && jr >= r.js && jr <= r.je;
+      gg[u] = ok ? U[q0 + dj * row] : 0.0;
+      pp[u] = ok ? pv[q0 + dj * row] : 1.0;
+      nl[u] = ok ? nut[q0 + dj * row] : 0.0;
+      ss[u] = in ? S[q0 + dj * jr] : 0.0;
+      jc[u] = in ? JAC(K, jr) : 0.0;
+      gp[u] = in ? G22(K, jr + 1, 1) : 0.0;
+      g0[u] = ok ? G22(K, jr, 1) : 0.0;
+      lp[u] = in ? gy[mf(d, K, jr + 1, 1)] : 0.0;
+      l0[u] = ok ? gy[mf(d, K, jr, 1)] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSw; ++u) {
+      const int row = row0 - u;
+      if (row < 0) break;
+      const double n0 = nl[u];
+      double c = 0.0;
+      if (row >= r.js && row <= r.je) c = -(t1 * g0[u]) - t2h * (n0 + n1) * l0[u];
+      for (int e = 0; e < 4; ++e)
+        if (bc.r[e].row == row) c = bc.r[e].c;
+      const double x0 = (gg[u] - c * x1) / pp[u];
+      U[q0 + dj * row] = x0;
+      const int jr = row + 1;  // R at row+1 now has x(row), x(row+1), x(row+2)
+      if (jr >= r.js && jr <= r.je) {
+        double rv = 0.5 * ss[u] + jc[u] * x1 + t1 * (gp[u] * (x2 - x1) - g0[u] * (x1 - x0));
+        rv = rv + t2 * (0.5 * (n2 + n1) * (lp[u] * (x2 - x1)) - 0.5 * (n0 + n1) * (l0[u] * (x1 - x0)));
+        R[q0 + dj * jr] = rv;
+      }
+      x2 = x1;
+      x1 = x0;
+      n2 = n1;
+      n1 = n0;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_zsweep_les(Dims d, Metrics M, const AdiField* __restrict__ fs, AdiT1 t1s) {
+  const AdiField& F = fs[blockIdx.y];
+  if (!F.nut) return;
+  const Ranges r = F.r;
+  const BcTable& bc = F.bz;
+  const double t1 = t1s.t1[blockIdx.y], t2h = t1s.t2[blockIdx.y] * 0.5;
+  double* __restrict__ U = F.U;
+  const double* __restrict__ R = F.R;
+  const double* __restrict__ nut = F.nut;
+  const double* __restrict__ gz = F.gz;
+  double* __restrict__ pv = F.pv;
+  int nj = r.je - r.js + 1;
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)d.NX2 * nj) return;
+  int i = (int)(t % d.NX2), J = r.js + (int)(t / d.NX2);
+  const int N = d.NZ + 1;
+  const size_t dk = d.NX2;
+  const size_t q0 = ph(d, i, 0, J);
+  double lag[4];
+  for (int e = 0; e < 4; ++e) lag[e] = (bc.r[e].kind == BC_LAGGED) ? U[q0 + dk * bc.r[e].src] : 0.0;
+  const double D = bc.sp_metric ? G11(d.NZ, J, 2) : 1.0;
+  double g = 0.0, pvp = 1.0, cp = 0.0;
+  double ap = 0.0, bp = 1.0, rp = 0.0;  // previous row before elimination (THOMAS_REAL_SP)
+  double nm = 0.0, nc = nut[q0];
+  for (int row0 = 0; row0 <= N; row0 += kSw) {
+    double rr[kSw], nn[kSw];
+#pragma unroll
+    for (int u = 0; u < kSw; ++u) {
+      const int row = row0 + u;
+      rr[u] = (row <= N && row >= r.zs && row <= r.ze) ? R[q0 + dk * row] : 0.0;
+      nn[u] = row + 1 <= N ? nut[q0 + dk * (row + 1)] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSw; ++u) {
+      const int row = row0 + u;
+      if (row > N) break;
+      const double np = nn[u];
+      double a = 0.0, b = 1.0, c = 0.0, rhs = rr[u];
+      if (row >= r.zs && row <= r.ze) {
+        const double gp = G11(row + 1, J, 2), g0 = G11(row, J, 2);
+        a = -(t1 * g0);
+        b = JAC(row, J) + t1 * (gp + g0);
+        c = -(t1 * gp);
+        const double wm = t2h * (nc + nm) * gz[mf(d, row, J, 2)];
+        const double wp = t2h * (np + nc) * gz[mf(d, row + 1, J, 2)];
+        a = a - wm;
+        b = b + wp + wm;
+        c = c - wp;
+      }
+      for (int e = 0; e < 4; ++e)
+        if (bc.r[e].row == row) {
+          a = bc.r[e].a;
+          b = bc.r[e].b;
+          c = bc.r[e].c;
+          rhs = bc_rhs(d, M, bc.r[e], J, lag[e]);
+        }
+      if (bc.sp && row == N) {
+        a = a - bp * D / ap;
+        b = b - cp * D / ap;
+        rhs = rhs - rp * D / ap;
+      }
+      double p = b;
+      if (row == 0) {
+        g = rhs;
+      } else {
+        const double mlt = -a / pvp;
+        p = b + mlt * cp;
+        g = rhs + mlt * g;
+      }
+      U[q0 + dk * row] = g;
+      pv[q0 + dk * row] = p;
+      ap = a;
+      bp = b;
+      rp = rhs;
+      pvp = p;
+      cp = c;
+      nm = nc;
+      nc = np;
+    }
+  }
+  double x1 = g / pvp;
+  U[q0 + dk * N] = x1;
+  double n1 = nut[q0 + dk * N];
+  for (int row0 = N - 1; row0 >= 0; row0 -= kSw) {
+    double gg[kSw], pp[kSw], nl[kSw], gk[kSw], lk[kSw];
+#pragma unroll
+    for (int u = 0; u < kSw; ++u) {
+      const int row = row0 - u;
+      const bool ok = row >= 0;
+      gg[u] = ok ? U[q0 + dk * row] : 0.0;
+      pp[u] = ok ? pv[q0 + dk * row] : 1.0;
+      nl[u] = ok ? nut[q0 + dk * row] : 0.0;
+      gk[u] = ok ? G11(row + 1, J, 2) : 0.0;
+      lk[u] = ok ? gz[mf(d, row + 1, J, 2)] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSw; ++u) {
+      const int row = row0 - u;
+      if (row < 0) break;
+      const double n0 = nl[u];
+      double c = 0.0;
+      if (row >= r.zs && row <= r.ze) c = -(t1 * gk[u]) - t2h * (n1 + n0) * lk[u];
+      for (int e = 0; e < 4; ++e)
+        if (bc.r[e].row == row) c = bc.r[e].c;
+      const double x0 = (gg[u] - c * x1) / pp[u];
+      U[q0 + dk * row] = x0;
+      x1 = x0;
+      n1 = n0;
     }
   }
 }
@@ -720,7 +969,14 @@ int launch_adi_batch(const Dims& d, const Metrics& M, const AdiField* fs, const 
   if (nf < 1 || nf > kAdiMaxFields) return EKM_ERR_ARG;
   if (d.NY + 2 > kLuMaxRows || d.NZ + 2 > kLuMaxRows) return EKM_ERR_UNSUPPORTED;
   size_t npre = 0, nly = 0, nlz = 0, nys = 0, nzs = 0;
+  bool plain = false, les = false;
   for (int f = 0; f < nf; ++f) {
+    if (hf[f].nut) {
+      if (!hf[f].gz || !hf[f].gy || !hf[f].pv) return EKM_ERR_ARG;
+      les = true;
+    } else {
+      plain = true;
+    }
     const Ranges& r = hf[f].r;
     const size_t nk = (size_t)(r.ze - r.zs + 1), nj = (size_t)(r.je - r.js + 1);
     npre = std::max(npre, (size_t)d.NX2 * nk * nj);
@@ -731,14 +987,26 @@ int launch_adi_batch(const Dims& d, const Metrics& M, const AdiField* fs, const 
   }
   hipLaunchKernelGGL(k_adi_pre, dim3(nblk(npre, 256), nf), dim3(256), 0, st, d, M, fs, t1);
   EKM_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_lu, dim3((unsigned)nly, nf), dim3(kLuBlock), 0, st, d, M, fs, t1, 0);
-  EKM_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_lu, dim3((unsigned)nlz, nf), dim3(kLuBlock), 0, st, d, M, fs, t1, 1);
-  EKM_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_ysweep, dim3(nblk(nys, 256), nf), dim3(256), 0, st, d, M, fs, t1);
-  EKM_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_zsweep, dim3(nblk(nzs, 256), nf), dim3(256), 0, st, d, M, fs);
-  EKM_CHECK(hipGetLastError());
+  if (plain) {
+    hipLaunchKernelGGL(k_lu, dim3((unsigned)nly, nf), dim3(kLuBlock), 0, st, d, M, fs, t1, 0);
+    EKM_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_lu, dim3((unsigned)nlz, nf), dim3(kLuBlock), 0, st, d, M, fs, t1, 1);
+    EKM_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_ysweep, dim3(nblk(nys, 256), nf), dim3(256), 0, st, d, M, fs, t1);
+    EKM_CHECK(hipGetLastError());
+  }
+  if (les) {
+    hipLaunchKernelGGL(k_ysweep_les, dim3(nblk(nys, 256), nf), dim3(256), 0, st, d, M, fs, t1);
+    EKM_CHECK(hipGetLastError());
+  }
+  if (plain) {
+    hipLaunchKernelGGL(k_zsweep, dim3(nblk(nzs, 256), nf), dim3(256), 0, st, d, M, fs);
+    EKM_CHECK(hipGetLastError());
+  }
+  if (les) {
+    hipLaunchKernelGGL(k_zsweep_les, dim3(nblk(nzs, 256), nf), dim3(256), 0, st, d, M, fs, t1);
+    EKM_CHECK(hipGetLastError());
+  }
   if (th_corners && th_field >= 0 && th_field < nf) {
     hipLaunchKernelGGL(k_th_corners, dim3(nblk(d.NX2, 256)), dim3(256), 0, st, d, hf[th_field].U, th_corners);
     EKM_CHECK(hipGetLastError());

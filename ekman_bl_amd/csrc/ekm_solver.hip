@@ -63,6 +63,9 @@ struct ekm_solver {
   // its coefficient is recomputed
   LesFields les{};
   LesScratch lesw{};
+  double* lesmet = nullptr;           // GMAT_11_z,12_z,22_z,11_y,12_y,22_y (two faces each)
+  AdiField adi_hl[kAdiMaxFields];     // ADI descriptors with the subgrid viscosity
+  AdiField* adi_dl = nullptr;
   int create_new_flow = 1, first_time = 1;
   int have_prev_phi = 0;  // mg.phi holds the previous substep's correction
   double hb_prev = 0.0;
@@ -112,6 +115,8 @@ static int check_params(const ekm_params& p) {
   if (p.proj_mode < 0 || p.proj_mode > 2) return EKM_ERR_UNSUPPORTED;
   if (p.les_model != 0 && p.les_model != 2) return EKM_ERR_UNSUPPORTED;  // dynamic Smagorinsky only
   if (p.les_model && (p.nx > 4096)) return EKM_ERR_UNSUPPORTED;
+  // the scalar's subgrid model (src/les_chan_th.F90) is not implemented
+  if (p.les_model && p.n_th) return EKM_ERR_UNSUPPORTED;
   if (p.proj_mode > 0) {
     // line smoothing is not cyclic, and the consistent system needs a
     // specified side to be nonsingular at kx = 0
@@ -154,10 +159,11 @@ static void set_hbar(ekm_solver* s) {
 static int free_all(ekm_solver* s) {
   double* f[] = {s->U[0], s->U[1], s->U[2], s->P, s->TH, s->CF[0], s->CF[1], s->CF[2], s->CFTH,
                  s->R[0], s->R[1], s->R[2], s->RTH, s->S1, s->S2, s->U2b, s->U3b, s->PHI, s->lu,
-                 s->red, s->met, s->kx, s->kx2, s->kx2s, s->thbar};
+                 s->red, s->met, s->kx, s->kx2, s->kx2s, s->thbar, s->lesmet};
   for (double* q : f)
     if (q) (void)hipFree(q);
   if (s->adi_d) (void)hipFree(s->adi_d);
+  if (s->adi_dl) (void)hipFree(s->adi_dl);
   {
     double* lf[] = {s->les.S[0], s->les.S[1], s->les.S[2], s->les.S[3], s->les.S[4], s->les.S[5], s->les.sabs,
                     s->les.nut, s->les.delta, s->les.cdyn, s->lesw.ub[0], s->lesw.ub[1], s->lesw.ub[2],
@@ -172,6 +178,18 @@ static int free_all(ekm_solver* s) {
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->st) (void)hipStreamDestroy(s->st);
   return EKM_OK;
+}
+
+// metric of the subgrid-viscosity terms of velocity component c (0..2):
+// which 0: z direction (GMAT_11, GMAT_11_y, GMAT_11_z for U1, U2, U3), 1: y
+// direction (GMAT_22, GMAT_22_y, GMAT_22_z), 2: the explicit cross terms
+// (GMAT_12, GMAT_12_y, GMAT_12_z) (src/duct.F90:990-1120, :1667-2391)
+static const double* les_metric(const ekm_solver* s, int c, int which) {
+  if (c == 0) return which == 0 ? s->M.g11 : which == 1 ? s->M.g22 : s->M.g12;
+  const size_t mn = s->d.mplane;
+  const int base = c == 2 ? 0 : 3;  // _z variants first, then _y
+  const int w = which == 0 ? 0 : which == 1 ? 2 : 1;
+  return s->lesmet + (size_t)(base + w) * 2 * mn;
 }
 
 static int fwd(ekm_solver* s, double* f) { return fft_forward(s->fft, f, (long)s->d.plane, s->st); }
@@ -267,6 +285,9 @@ extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const doub
       a.R = f < 3 ? s->R[f] : s->RTH;
       a.S = Sx[f];
       a.lu = s->lu + (size_t)f * 7 * d.plane;
+      a.nut = nullptr;
+      a.gz = a.gy = nullptr;
+      a.pv = nullptr;
     }
     s->adi_nf = nf;
     if (hipMalloc(&s->adi_d, sizeof(s->adi_h)) != hipSuccess) return fail(EKM_ERR_HIP);
@@ -293,6 +314,27 @@ extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const doub
       for (int k = 0; k < d.NZ2; ++k)
         dl[(size_t)k + (size_t)d.NZ2 * j] = std::pow(dx * s->hm.jac[(size_t)k + (size_t)d.NZ3 * j], 1.0 / 3.0);
     if (hipMemcpy(s->les.delta, dl.data(), d.plane * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(EKM_ERR_HIP);
+    // metrics of the subgrid stress (src/diablo.F90:1379-1474) and the ADI
+    // descriptors of the velocities with NU_T; their pivot scratch fields are
+    // the coefficient scratch, which is free once the model has run
+    if (hipMalloc(&s->lesmet, 12 * mn * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
+    for (int w = 0; w < 6; ++w)
+      if (hipMemcpy(s->lesmet + (size_t)w * 2 * mn, s->hm.m[7 + w].data(), 2 * mn * sizeof(double),
+                    hipMemcpyHostToDevice) != hipSuccess)
+        return fail(EKM_ERR_HIP);
+    double* pvs[3] = {s->lesw.t1, s->lesw.m0, s->lesw.fm0};
+    for (int f = 0; f < kAdiMaxFields; ++f) {
+      s->adi_hl[f] = s->adi_h[f];
+      if (f >= 3) continue;
+      AdiField& a = s->adi_hl[f];
+      a.nut = s->les.nut;
+      a.gz = les_metric(s, f, 0);
+      a.gy = les_metric(s, f, 1);
+      a.pv = pvs[f];
+    }
+    if (hipMalloc(&s->adi_dl, sizeof(s->adi_hl)) != hipSuccess) return fail(EKM_ERR_HIP);
+    if (hipMemcpy(s->adi_dl, s->adi_hl, sizeof(s->adi_hl), hipMemcpyHostToDevice) != hipSuccess)
       return fail(EKM_ERR_HIP);
   }
   if (hipMalloc(&s->red, 4 * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
@@ -507,7 +549,8 @@ extern "C" int ekm_rk_duct(ekm_solver* s) {
   // velocities and scalar to physical space (:655-671)
   // the subgrid model (:641-657), which leaves the velocities in physical
   // space, or the velocities to physical space (:658-671)
-  if (p.les_model && (!s->create_new_flow || cur_step > 1)) {
+  const bool les_on = p.les_model && (!s->create_new_flow || cur_step > 1);
+  if (les_on) {
     const bool refresh = ((cur_step % 5) == 0 && rk == 1) || s->first_time;
     TRY(les_chan_run(s->fft, d, s->M, s->rv, s->kx, s->U, s->CF, s->les, s->lesw, refresh, st));
   } else {
@@ -516,9 +559,10 @@ extern "C" int ekm_rk_duct(ekm_solver* s) {
   if (th) TRY(inv(s, s->TH));
   // momentum fluxes (:683-1161)
   TRY(launch_contravariant(d, s->M, s->rv, s->U[1], s->U[2], s->U2b, s->U3b, st));
+  // (with the model: its eddy-viscosity cross terms, :990-1120)
   for (int f = 0; f < 3; ++f) {
     TRY(launch_flux_accum(s->fft, d, s->M, s->rv, s->U[0], s->U[f], s->U2b, s->U3b, p.nu, s->kx, s->CF[f],
-                          s->R[f], TEMP5, st));
+                          s->R[f], TEMP5, st, les_on ? s->les.nut : nullptr, les_on ? les_metric(s, f, 2) : nullptr));
   }
   for (int f = 0; f < 3; ++f) TRY(inv(s, s->R[f]));
   if (th) {  // (:1169-1625)
@@ -530,8 +574,9 @@ extern "C" int ekm_rk_duct(ekm_solver* s) {
   // implicit solves are independent, so they share one set of launches
   {
     const int corners = th ? ((p.th_bc[EKM_ZMAX] == 1 ? 1 : 0) | (p.th_bc[EKM_ZMIN] == 6 ? 2 : 0)) : 0;
-    AdiT1 t1{{TEMP1, TEMP1, TEMP1, TEMP1 / p.pr}};
-    TRY(launch_adi_batch(d, s->M, s->adi_d, s->adi_h, s->adi_nf, t1, 3, corners, st));
+    AdiT1 t1{{TEMP1, TEMP1, TEMP1, TEMP1 / p.pr}, {hb / 2.0, hb / 2.0, hb / 2.0, hb / 2.0}};
+    TRY(launch_adi_batch(d, s->M, les_on ? s->adi_dl : s->adi_d, les_on ? s->adi_hl : s->adi_h, s->adi_nf, t1, 3,
+                         corners, st));
   }
   // CFL time step at the third substep (:2417-2419)
   if (p.variable_dt && rk == 3 && (cur_step % p.update_dt) == 0) {

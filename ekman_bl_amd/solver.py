@@ -78,6 +78,7 @@ def lib():
         L.ekm_thomas_real_sp.argtypes = [_D, _D, _D, _D, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.ekm_thomas_real_dev.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]
         L.ekm_host_metrics.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D]
+        L.ekm_host_les_metrics.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D]
         L.ekm_host_mg_coefficients.argtypes = [ctypes.POINTER(EkmParams), _D, _D, ctypes.c_int, ctypes.c_double,
                                                _D, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         _lib = L
@@ -214,6 +215,17 @@ def host_metrics(nz: int, ny: int, xpoint: np.ndarray, ypoint: np.ndarray) -> np
     _check(lib().ekm_host_metrics(nz, ny, _p(np.ascontiguousarray(xpoint, dtype=np.float64)),
                                   _p(np.ascontiguousarray(ypoint, dtype=np.float64)), _p(out)), "ekm_host_metrics")
     return out.reshape(15, ny + 3, nz + 3)
+
+
+def host_les_metrics(nz: int, ny: int, xpoint: np.ndarray, ypoint: np.ndarray) -> np.ndarray:
+    """Subgrid-stress metric variants as the library computes them:
+    (6, 2, NY+3, NZ+3) = GMAT_11_z, 12_z, 22_z, 11_y, 12_y, 22_y."""
+    mn = (nz + 3) * (ny + 3)
+    out = np.zeros(12 * mn)
+    _check(lib().ekm_host_les_metrics(nz, ny, _p(np.ascontiguousarray(xpoint, dtype=np.float64)),
+                                      _p(np.ascontiguousarray(ypoint, dtype=np.float64)), _p(out)),
+           "ekm_host_les_metrics")
+    return out.reshape(6, 2, ny + 3, nz + 3)
 
 
 def host_mg_coefficients(p: EkmParams, xpoint, ypoint, lev: int, kx2: float) -> np.ndarray:

@@ -202,6 +202,9 @@ int ekm_dev_free(void* p);
  * CJOB_11,12,21,22 with two faces each, then INT_JACOB) and the multigrid
  * coefficients cf(i,j,1..9) of level `lev` (1 = coarsest). */
 int ekm_host_metrics(int nz, int ny, const double* xpoint, const double* ypoint, double* out);
+/* The subgrid-stress metric variants of JACOBI_TRANS (src/diablo.F90:1379-1474):
+ * 12*(NZ+3)*(NY+3) = GMAT_11_z,12_z,22_z,11_y,12_y,22_y with two faces each. */
+int ekm_host_les_metrics(int nz, int ny, const double* xpoint, const double* ypoint, double* out);
 int ekm_host_mg_coefficients(const ekm_params* p, const double* xpoint, const double* ypoint,
                              int lev, double kx2, double* cf, int* nx, int* ny);
 /* Message of the last HIP error seen by this thread. */

@@ -45,7 +45,8 @@ struct ora {
   size_t npts; /* doubles per 3D field */
   /* grid and metrics (JACOBI_TRANS) */
   double *xp, *yp;               /* (0:NZ+1,0:NY+1) */
-  double* met[7];                /* (0:NZ+2,0:NY+2,1:2): G11 G12 G22 C11 C12 C21 C22 */
+  double* met[13];               /* (0:NZ+2,0:NY+2,1:2): G11 G12 G22 C11 C12 C21 C22, then the
+                                  * LES variants G11_z G12_z G22_z G11_y G12_y G22_y */
   double* jac;                   /* INT_JACOB (0:NZ+2,0:NY+2) */
   double* thbar;                 /* TH_BAR (0:NZ+1,0:NY+1) */
   double *kx, *kx2;              /* KXP, KX2P (0:NX/2) */
@@ -79,6 +80,7 @@ struct ora {
 #define C21(o, k, j, f) ((o)->met[5][MF(o, k, j, f)])
 #define C22(o, k, j, f) ((o)->met[6][MF(o, k, j, f)])
 #define JAC(o, k, j) ((o)->jac[MI(o, k, j)])
+#define NMET 13
 
 static double* dalloc(size_t n) {
   double* a = (double*)malloc(n * sizeof(double));
@@ -318,7 +320,7 @@ static void metrics(ora* o) {
         ye[L(k, j, f)] = trunc8(ye[L(k, j, f)]);
       }
   /* :1233-1264 */
-  for (int q = 0; q < 7; ++q)
+  for (int q = 0; q < NMET; ++q)
     for (size_t t = 0; t < (size_t)o->NZ3 * o->NY3 * 2; ++t) o->met[q][t] = NAN;
   for (size_t t = 0; t < (size_t)o->NZ3 * o->NY3; ++t) o->jac[t] = NAN;
   for (int j = 1; j <= NY + 1; ++j)
@@ -335,6 +337,14 @@ static void metrics(ora* o) {
         C12(o, k, j, f) = zy * J;
         C21(o, k, j, f) = ex * J;
         C22(o, k, j, f) = ey * J;
+        /* :1379-1403 LES variants: the stress of U3 doubles the zeta_x and
+         * eta_x products, that of U2 the zeta_y and eta_y products */
+        o->met[7][MF(o, k, j, f)] = (2.0 * zx * zx + zy * zy) * J;
+        o->met[8][MF(o, k, j, f)] = (2.0 * zx * ex + zy * ey) * J;
+        o->met[9][MF(o, k, j, f)] = (2.0 * ex * ex + ey * ey) * J;
+        o->met[10][MF(o, k, j, f)] = (zx * zx + 2.0 * zy * zy) * J;
+        o->met[11][MF(o, k, j, f)] = (zx * ex + 2.0 * zy * ey) * J;
+        o->met[12][MF(o, k, j, f)] = (ex * ex + 2.0 * ey * ey) * J;
       }
   /* :1268-1292 cell Jacobian */
   for (int j = 1; j <= NY; ++j)
@@ -352,23 +362,26 @@ static void metrics(ora* o) {
   JAC(o, NZ + 1, NY + 1) = 0.5 * (2.0 * JAC(o, NZ, NY + 1) - JAC(o, NZ - 1, NY + 1) +
                                   2.0 * JAC(o, NZ + 1, NY) - JAC(o, NZ + 1, NY - 1));
   /* :1294-1340 ghost metrics, same statement order (GMAT_12 corner is not
-   * re-set: the reference assigns GMAT_22 twice at :1322-1326) */
+   * re-set: the reference assigns GMAT_22 twice at :1322-1326); the LES
+   * variants follow the same rules (:1405-1474, GMAT_22_z and GMAT_22_y are
+   * the ones assigned twice there) */
   for (int j = 0; j <= NY + 1; ++j)
     for (int f = 1; f <= 2; ++f)
-      for (int t = 0; t < 7; ++t) {
+      for (int t = 0; t < NMET; ++t) {
         double* a = o->met[t];
         a[MF(o, 0, j, f)] = 2.0 * a[MF(o, 1, j, f)] - a[MF(o, 2, j, f)];
       }
   for (int k = 0; k <= NZ + 1; ++k)
     for (int f = 1; f <= 2; ++f)
-      for (int t = 0; t < 7; ++t) {
+      for (int t = 0; t < NMET; ++t) {
         double* a = o->met[t];
         a[MF(o, k, 0, f)] = 2.0 * a[MF(o, k, 1, f)] - a[MF(o, k, 2, f)];
       }
   for (int f = 1; f <= 2; ++f) {
-    /* GMAT_11, GMAT_22, CJOB_11, CJOB_12, CJOB_21, CJOB_22 */
-    int list[6] = {0, 2, 3, 4, 5, 6};
-    for (int t = 0; t < 6; ++t) {
+    /* GMAT_11, GMAT_22, CJOB_11, CJOB_12, CJOB_21, CJOB_22, GMAT_11_z,
+     * GMAT_22_z, GMAT_11_y, GMAT_22_y */
+    int list[10] = {0, 2, 3, 4, 5, 6, 7, 9, 10, 12};
+    for (int t = 0; t < 10; ++t) {
       double* a = o->met[list[t]];
       a[MF(o, 0, 0, f)] = 0.5 * (2.0 * a[MF(o, 0, 1, f)] - a[MF(o, 0, 2, f)] +
                                  2.0 * a[MF(o, 1, 0, f)] - a[MF(o, 2, 0, f)]);
@@ -376,7 +389,7 @@ static void metrics(ora* o) {
   }
   /* :1343-1376 copies for the multigrid coefficient callback */
   for (int f = 1; f <= 2; ++f) {
-    for (int q = 0; q < 7; ++q) {
+    for (int q = 0; q < NMET; ++q) {
       double* a = o->met[q];
       for (int k = 0; k <= NZ + 1; ++k) a[MF(o, k, NY + 2, f)] = a[MF(o, k, NY + 1, f)];
       for (int j = 0; j <= NY + 1; ++j) a[MF(o, NZ + 2, j, f)] = a[MF(o, NZ + 1, j, f)];
@@ -390,9 +403,9 @@ static void metrics(ora* o) {
 }
 
 int ora_get_metric(ora* o, int which, double* out) {
-  if (which < 0 || which > 7) return EKM_ERR_ARG;
-  size_t n = (size_t)o->NZ3 * o->NY3 * (which < 7 ? 2 : 1);
-  memcpy(out, which < 7 ? o->met[which] : o->jac, n * sizeof(double));
+  if (which < 0 || which > 13) return EKM_ERR_ARG;
+  size_t n = (size_t)o->NZ3 * o->NY3 * (which == 7 ? 1 : 2);
+  memcpy(out, which < 7 ? o->met[which] : which == 7 ? o->jac : o->met[which - 1], n * sizeof(double));
   return EKM_OK;
 }
 
@@ -1460,6 +1473,7 @@ static int supported(const ekm_params* p) {
   if (p->nx < 8 || (p->nx & (p->nx - 1))) return 0;
   if (p->proj_mode < 0 || p->proj_mode > 1) return 0;
   if (p->les_model != 0 && p->les_model != 2) return 0;
+  if (p->les_model && p->n_th) return 0; /* les_chan_th is not restated */
   if (p->proj_mode == 1) {
     int dir = 0;
     for (int s = 0; s < 4; ++s) {
@@ -1507,7 +1521,7 @@ ora* ora_create(const ekm_params* p, const double* xpoint, const double* ypoint,
   o->xp = dalloc(pl); o->yp = dalloc(pl); o->thbar = dalloc(pl);
   memcpy(o->xp, xpoint, pl * sizeof(double));
   memcpy(o->yp, ypoint, pl * sizeof(double));
-  for (int q = 0; q < 7; ++q) o->met[q] = dalloc(ml * 2);
+  for (int q = 0; q < NMET; ++q) o->met[q] = dalloc(ml * 2);
   o->jac = dalloc(ml);
   metrics(o);
   /* TH_BAR (src/diablo.F90:1152-1160) */
@@ -1550,7 +1564,7 @@ ora* ora_create(const ekm_params* p, const double* xpoint, const double* ypoint,
 void ora_destroy(ora* o) {
   if (!o) return;
   free(o->xp); free(o->yp); free(o->thbar); free(o->jac); free(o->kx); free(o->kx2);
-  for (int q = 0; q < 7; ++q) free(o->met[q]);
+  for (int q = 0; q < NMET; ++q) free(o->met[q]);
   double* f[] = {o->U[0], o->U[1], o->U[2], o->P, o->TH, o->CF[0], o->CF[1], o->CF[2], o->CFTH,
                  o->R[0], o->R[1], o->R[2], o->RTH, o->S1, o->S2, o->U2b, o->U3b, o->CS1, o->PHI};
   for (size_t q = 0; q < sizeof(f) / sizeof(f[0]); ++q) free(f[q]);
@@ -1645,7 +1659,18 @@ static void bc_z(const ora* o, int comp, int J, sysb* s, const double* u, size_t
 /* ADI for one field (theta: :1334-1625; U1: :1640-1835; U2: :1877-2110;     */
 /* U3: :2137-2391)                                                           */
 /* ------------------------------------------------------------------------ */
-static void adi(ora* o, int comp, double* U, double* R, double t1) {
+/* the z- and y-direction metrics of the subgrid viscosity of velocity
+ * component c: GMAT_11 / GMAT_22 for U1 (:1667-1731), the _y variants for U2
+ * (:1916-1928, :1963-1975), the _z variants for U3 (:2177-2236) */
+static const double* les_gz(const ora* o, int c) { return o->met[c == 0 ? 0 : c == 1 ? 10 : 7]; }
+static const double* les_gy(const ora* o, int c) { return o->met[c == 0 ? 2 : c == 1 ? 12 : 9]; }
+
+/* nut != NULL adds the subgrid viscosity (:1667-1680, :1718-1731,
+ * :1771-1784, :1819-1832 and their U2, U3 analogues) with weight t2 =
+ * H_BAR/2: GZ is the z-direction metric (GMAT_11, GMAT_11_y, GMAT_11_z for
+ * U1, U2, U3), GY the y-direction one (GMAT_22, GMAT_22_y, GMAT_22_z). */
+static void adi(ora* o, int comp, double* U, double* R, double t1, const double* nut, const double* GZ,
+                const double* GY, double t2) {
   int NX2 = o->NX2, NY = o->NY, NZ = o->NZ;
   int js = comp < 3 ? o->jstart : o->jstart_th, je = comp < 3 ? o->jend : o->jend_th;
   int zs = comp < 3 ? o->zstart : o->zstart_th, ze = comp < 3 ? o->zend : o->zend_th;
@@ -1664,6 +1689,9 @@ static void adi(ora* o, int comp, double* U, double* R, double t1) {
         size_t q = IDX(o, i, k, j), dk = NX2;
         R[q] = 0.5 * S1[q] + JAC(o, k, j) * U[q] +
                t1 * (G11(o, k + 1, j, 2) * (U[q + dk] - U[q]) - G11(o, k, j, 2) * (U[q] - U[q - dk]));
+        if (nut)
+          R[q] = R[q] + t2 * (0.5 * (nut[q + dk] + nut[q]) * (GZ[MF(o, k + 1, j, 2)] * (U[q + dk] - U[q])) -
+                              0.5 * (nut[q] + nut[q - dk]) * (GZ[MF(o, k, j, 2)] * (U[q] - U[q - dk])));
       }
   /* y sweeps, lines K */
   size_t sy = (size_t)NX2 * o->NZ2;
@@ -1681,6 +1709,15 @@ static void adi(ora* o, int comp, double* U, double* R, double t1) {
           size_t r = i + (size_t)NX2 * j;
           s.a[r] = l; s.b[r] = d; s.c[r] = u; s.g[r] = R[IDX(o, i, k, j)];
         }
+        if (nut)
+          for (int i = 0; i < NX2; ++i) {
+            size_t r = i + (size_t)NX2 * j, q = IDX(o, i, k, j);
+            double wm = t2 * 0.5 * (nut[q] + nut[q - sy]) * GY[MF(o, k, j, 1)];
+            double wp = t2 * 0.5 * (nut[q] + nut[q + sy]) * GY[MF(o, k, j + 1, 1)];
+            s.a[r] = s.a[r] - wm;
+            s.b[r] = s.b[r] + wp + wm;
+            s.c[r] = s.c[r] - wp;
+          }
       }
       bc_y(o, comp, k, &s, U, IDX(o, 0, k, 0), sy);
       ora_thomas_real(s.a, s.b, s.c, s.g, NY + 1, NX2 - 1);
@@ -1696,6 +1733,9 @@ static void adi(ora* o, int comp, double* U, double* R, double t1) {
         size_t q = IDX(o, i, k, j), dj = sy;
         R[q] = 0.5 * S1[q] + JAC(o, k, j) * U[q] +
                t1 * (G22(o, k, j + 1, 1) * (U[q + dj] - U[q]) - G22(o, k, j, 1) * (U[q] - U[q - dj]));
+        if (nut)
+          R[q] = R[q] + t2 * (0.5 * (nut[q + dj] + nut[q]) * (GY[MF(o, k, j + 1, 1)] * (U[q + dj] - U[q])) -
+                              0.5 * (nut[q - dj] + nut[q]) * (GY[MF(o, k, j, 1)] * (U[q] - U[q - dj])));
       }
   if (comp < 3) /* IC_TYPE 7: R(I,K,NY+1) = R(I,K,NY) (:1787-1793, :2023-2029) */
     for (int k = zs; k <= ze; ++k)
@@ -1716,6 +1756,15 @@ static void adi(ora* o, int comp, double* U, double* R, double t1) {
           size_t r = i + (size_t)NX2 * k;
           s.a[r] = l; s.b[r] = d; s.c[r] = u; s.g[r] = R[IDX(o, i, k, j)];
         }
+        if (nut)
+          for (int i = 0; i < NX2; ++i) {
+            size_t r = i + (size_t)NX2 * k, q = IDX(o, i, k, j), dk = NX2;
+            double wm = t2 * 0.5 * (nut[q] + nut[q - dk]) * GZ[MF(o, k, j, 2)];
+            double wp = t2 * 0.5 * (nut[q + dk] + nut[q]) * GZ[MF(o, k + 1, j, 2)];
+            s.a[r] = s.a[r] - wm;
+            s.b[r] = s.b[r] + wp + wm;
+            s.c[r] = s.c[r] - wp;
+          }
       }
       bc_z(o, comp, j, &s, U, IDX(o, 0, 0, j), (size_t)NX2);
       if (hi == 5) {
@@ -1745,6 +1794,15 @@ static void adi(ora* o, int comp, double* U, double* R, double t1) {
         U[IDX(o, i, 0, 0)] = U[IDX(o, i, NZ, 0)];
       }
   }
+}
+
+/* One field's ADI on caller arrays (U, R physical, in place), with the
+ * subgrid viscosity nut when it is not NULL: the building block the
+ * substep runs, exposed so its eddy-viscosity terms can be checked alone. */
+int ora_adi_field(ora* o, int comp, double* U, double* R, double t1, const double* nut, double t2) {
+  if (comp < 0 || comp > 3 || (comp == 3 && (!o->p.n_th || nut))) return EKM_ERR_ARG;
+  adi(o, comp, U, R, t1, nut, comp < 3 ? les_gz(o, comp) : NULL, comp < 3 ? les_gy(o, comp) : NULL, t2);
+  return EKM_OK;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1794,6 +1852,32 @@ static void yzflux(ora* o, double* S, const double* Q, double nu, int js, int je
                        (G12(o, k, j + 1, 1) * (Q[q + dk + dj] + Q[q + dk] - Q[q - dk + dj] - Q[q - dk]) -
                         G12(o, k, j, 1) * (Q[q + dk] + Q[q + dk - dj] - Q[q - dk] - Q[q - dk - dj]));
         S[q] = s1 + s2;
+      }
+}
+
+/* Eddy-viscosity cross terms of the subgrid model (:990-1120): with the
+ * face averages of NU_T, S = 1/4 [ GL(K+1) nu(K+1/2) (Q_eta at K+1/2)
+ * - GL(K) nu(K-1/2) (Q_eta at K-1/2) + GL(J+1) nu(J+1/2) (Q_zeta at J+1/2)
+ * - GL(J) nu(J-1/2) (Q_zeta at J-1/2) ]; GL is GMAT_12 for U1, GMAT_12_y for
+ * U2 and GMAT_12_z for U3. */
+static void les_xterm(ora* o, double* S, const double* Q, const double* GL, int js, int je, int zs, int ze) {
+  memset(S, 0, o->npts * sizeof(double));
+  int NX2 = o->NX2;
+  size_t dj = (size_t)NX2 * o->NZ2, dk = NX2;
+  const double* N = o->NUT;
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k)
+      for (int i = 0; i < NX2; ++i) {
+        size_t q = IDX(o, i, k, j);
+        S[q] = 0.25 * (GL[MF(o, k + 1, j, 2)] * 0.5 * (N[q + dk] + N[q]) *
+                           (Q[q + dk + dj] + Q[q + dj] - Q[q - dj] - Q[q + dk - dj]) -
+                       GL[MF(o, k, j, 2)] * 0.5 * (N[q] + N[q - dk]) *
+                           (Q[q - dk + dj] + Q[q + dj] - Q[q - dj] - Q[q - dk - dj]) +
+                       GL[MF(o, k, j + 1, 1)] * 0.5 * (N[q] + N[q + dj]) *
+                           (Q[q + dk + dj] + Q[q + dk] - Q[q - dk + dj] - Q[q - dk]) -
+                       GL[MF(o, k, j, 1)] * 0.5 * (N[q] + N[q - dj]) *
+                           (Q[q + dk] + Q[q + dk - dj] - Q[q - dk] - Q[q - dk - dj]));
       }
 }
 
@@ -2333,6 +2417,11 @@ int ora_rk_duct(ora* o) {
     yzflux(o, o->S1, o->U[f], p->nu, js, je, zs, ze);
     accum(o, o->CF[f], o->S1, 0, js, je, zs, ze);
   }
+  if (les) /* :990-1120 */
+    for (int f = 0; f < 3; ++f) {
+      les_xterm(o, o->S1, o->U[f], o->met[f == 0 ? 1 : f == 1 ? 11 : 8], js, je, zs, ze);
+      accum(o, o->CF[f], o->S1, 0, js, je, zs, ze);
+    }
   /* :1127-1161 */
 #pragma omp parallel for schedule(static)
   for (int j = js; j <= je; ++j)
@@ -2359,10 +2448,11 @@ int ora_rk_duct(ora* o) {
             o->RTH[q] = o->RTH[q] + TEMP5 * o->CFTH[q];
           }
     inv(o, o->RTH);
-    adi(o, 3, CTH, o->RTH, TEMP1 / p->pr);
+    adi(o, 3, CTH, o->RTH, TEMP1 / p->pr, NULL, NULL, NULL, 0.0);
   }
   /* :1640-2391 velocity ADI */
-  for (int f = 0; f < 3; ++f) adi(o, f, o->U[f], o->R[f], TEMP1);
+  for (int f = 0; f < 3; ++f)
+    adi(o, f, o->U[f], o->R[f], TEMP1, les ? o->NUT : NULL, les_gz(o, f), les_gy(o, f), hb / 2.0);
   /* :2417-2419 */
   if (p->variable_dt && rk == 3 && (cur_step % p->update_dt) == 0) courant(o);
   /* :2421-2451 */

@@ -28,6 +28,19 @@
  *   ora_rem_div_curv            uniform grid: J div u after a tight solve is the
  *                               multigrid residual; proj_mode 1 removes the
  *                               divergence to the solve tolerance
+ *   LES metric variants         rectilinear grids: GMAT_11_z = 2 GMAT_11,
+ *   (ora_get_metric 8..13)      GMAT_11_y = GMAT_11, GMAT_22_z = GMAT_22,
+ *                               GMAT_22_y = 2 GMAT_22, GMAT_12_* = 0; skewed
+ *                               grids: X_z + X_y = 3 X for X = 11, 12, 22
+ *   les_chan (LES model 2)      x-independent flow: C_DYN = 0 and the substep
+ *                               is the plain substep; C_DYN >= 0 after clipping,
+ *                               NU_T >= 0 and zero outside the interior
+ *   ora_adi_field with NU_T     a constant NU_T on U1 (whose subgrid metrics are
+ *                               the molecular ones) is the plain ADI with
+ *                               t1 + t2 NU_T; the U2/U3 metric selection and the
+ *                               explicit cross terms (les_xterm): parity unpinned
+ *                               beyond the reference text (the GPU path is
+ *                               checked against them)
  *   ora_rk_duct / ora_time_step the composed operation order of RK_DUCT (lagged
  *                               rows, corner updates, dt timing) has no golden
  *                               vectors: parity unpinned beyond the pieces above
@@ -53,12 +66,16 @@ int ora_set_initial_physical(ora* o, const double* u1, const double* u2,
                              const double* u3, const double* th);
 int ora_rk_duct(ora* o);
 int ora_get_les(ora* o, double* nut, double* cdyn);
+/* one field's ADI (comp 0..2 velocity, 3 scalar) on physical U, R in place,
+ * with the subgrid viscosity nut (physical P field) unless NULL */
+int ora_adi_field(ora* o, int comp, double* U, double* R, double t1, const double* nut, double t2);
 int ora_time_step(ora* o, int nsteps);
 int ora_rem_div_curv(ora* o);
 int ora_get_phi(ora* o, double* phi);
 int ora_get_diag(ora* o, ekm_diag* d);
 /* metrics: which = 0..6 -> GMAT_11,GMAT_12,GMAT_22,CJOB_11,CJOB_12,CJOB_21,CJOB_22
- * ((NZ+3)*(NY+3)*2, K fastest then J then face), 7 -> INT_JACOB ((NZ+3)*(NY+3)). */
+ * ((NZ+3)*(NY+3)*2, K fastest then J then face), 7 -> INT_JACOB ((NZ+3)*(NY+3)),
+ * 8..13 -> GMAT_11_z,GMAT_12_z,GMAT_22_z,GMAT_11_y,GMAT_12_y,GMAT_22_y (two faces). */
 int ora_get_metric(ora* o, int which, double* out);
 
 /* components */

@@ -48,6 +48,7 @@ def lib():
         L.ora_set_state.argtypes = [ctypes.c_void_p, _D, _D, _D, _D, _D, ctypes.c_double, ctypes.c_int]
         L.ora_get_state.argtypes = [ctypes.c_void_p, _D, _D, _D, _D, _D]
         L.ora_get_les.argtypes = [ctypes.c_void_p, _D, _D]
+        L.ora_adi_field.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.c_double, _D, ctypes.c_double]
         L.ora_set_initial_physical.argtypes = [ctypes.c_void_p, _D, _D, _D, _D]
         L.ora_get_phi.argtypes = [ctypes.c_void_p, _D]
         L.ora_get_diag.argtypes = [ctypes.c_void_p, ctypes.POINTER(EkmDiag)]
@@ -166,11 +167,23 @@ class Oracle:
         lib().ora_get_diag(self.h, ctypes.byref(d))
         return d
 
+    def adi_field(self, comp: int, u: np.ndarray, r: np.ndarray, t1: float, nut=None, t2: float = 0.0):
+        """One field's ADI (physical (NY+2, NZ+2, NX+2) arrays, returned as new
+        arrays), with the subgrid viscosity nut when given."""
+        u = np.array(u, dtype=np.float64, order="C")
+        r = np.array(r, dtype=np.float64, order="C")
+        nu = None if nut is None else np.ascontiguousarray(nut, dtype=np.float64)
+        st = lib().ora_adi_field(self.h, comp, _p(u), _p(r), float(t1), None if nu is None else _p(nu), float(t2))
+        if st != 0:
+            raise RuntimeError(f"ora_adi_field: {st}")
+        return u, r
+
     def metric(self, which: int) -> np.ndarray:
         """0..6: GMAT_11, GMAT_12, GMAT_22, CJOB_11, CJOB_12, CJOB_21, CJOB_22 as
-        (2, NY+3, NZ+3); 7: INT_JACOB as (NY+3, NZ+3)."""
+        (2, NY+3, NZ+3); 7: INT_JACOB as (NY+3, NZ+3); 8..13: the LES variants
+        GMAT_11_z, GMAT_12_z, GMAT_22_z, GMAT_11_y, GMAT_12_y, GMAT_22_y."""
         n3z, n3y = self.p.nz + 3, self.p.ny + 3
-        out = np.zeros((2, n3y, n3z) if which < 7 else (n3y, n3z))
+        out = np.zeros((2, n3y, n3z) if which != 7 else (n3y, n3z))
         lib().ora_get_metric(self.h, which, _p(out))
         return out
 

@@ -188,15 +188,21 @@ def test_warm_starts_agree():
     assert r[3] < 1e-6
 
 
-def test_les_matches_oracle(oracle_lib):
-    """Dynamic Smagorinsky subgrid model (les_model 2, src/les_chan.F90) on
-    the GPU against the oracle: both restart from the same projected state of
-    config 1 with 5% noise at step 3, so the two steps run the coefficient
-    refresh of a run's first step (all three substeps) and of every fifth
-    step (substep 1 of step 5).  Fields, NU_T and C_DYN agree to 1e-10
-    relative L2."""
+@pytest.mark.parametrize("skew", [0.0, 0.05])
+def test_les_matches_oracle(skew, oracle_lib):
+    """Dynamic Smagorinsky subgrid model (les_model 2, src/les_chan.F90) and
+    its eddy viscosity in the momentum equations (explicit cross terms and
+    the implicit ADI terms, src/duct.F90:990-1120, :1667-2391) on the GPU
+    against the oracle: both restart from the same projected state of config
+    1 with 5% noise at step 3, so the two steps run the coefficient refresh of
+    a run's first step (all three substeps) and of every fifth step (substep 1
+    of step 5).  skew > 0 shears the grid (z += skew sin(0.3 y)) so that
+    GMAT_12 and its variants, and with them the cross terms, are nonzero.
+    Fields, NU_T and C_DYN agree to 1e-10 relative L2; the model changes the
+    fields by about 1e-5 to 1e-4 relative in this case."""
     O = oracle_lib
     cfg, p, xp, yp = _setup(1)
+    xp = xp + skew * np.sin(0.3 * yp)
     ic = ekman_initial(p, yp, cfg.seed, 0.05, cfg.th_noise)
     o0 = O.Oracle(p, xp, yp)
     o0.set_initial_physical(*ic)

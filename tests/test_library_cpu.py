@@ -85,6 +85,9 @@ def test_host_tables_equal_oracle(n, bcs, oracle_lib):
     for w in range(7):
         assert _nan_equal(o.metric(w), hm[2 * w:2 * w + 2]), w
     assert _nan_equal(o.metric(7), hm[14])
+    lm = S.host_les_metrics(p.nz, p.ny, xp, yp)
+    for w in range(6):
+        assert _nan_equal(o.metric(8 + w), lm[w]), w
     for lev in range(1, o.mg_ngrid() + 1):
         for kx2 in (0.0, 3.7):
             assert np.array_equal(o.mg_discretize(kx2, lev)[:9], S.host_mg_coefficients(p, xp, yp, lev, kx2)), \

@@ -170,6 +170,76 @@ def test_metrics_stretched_rectilinear(oracle_lib):
     assert np.all(g12 == 0.0)
 
 
+def test_les_metric_variants_rectilinear_closed_form(oracle_lib):
+    """Subgrid-stress metrics (src/diablo.F90:1379-1474) on a uniform
+    rectilinear grid: the U3 stress doubles the zeta_x products and the U2
+    stress the eta_y products, so GMAT_11_z = 2 GMAT_11, GMAT_11_y = GMAT_11,
+    GMAT_22_z = GMAT_22, GMAT_22_y = 2 GMAT_22 and both GMAT_12 variants are 0
+    on every face, ghosts included."""
+    O = oracle_lib
+    ny, nz, dy, dz = 15, 31, 0.5, 0.25
+    xp, yp = np.meshgrid(dz * np.arange(nz + 2), dy * np.arange(ny + 2))
+    o, _ = _oracle_for_grid(O, ny, nz, xp, yp)
+    g11, g22 = o.metric(0), o.metric(2)
+    g11z, g12z, g22z, g11y, g12y, g22y = (o.metric(w) for w in range(8, 14))
+    assert np.all(g11 == dy / dz) and np.all(g22 == dz / dy)
+    assert np.array_equal(g11z, 2 * g11) and np.array_equal(g11y, g11)
+    assert np.array_equal(g22z, g22) and np.array_equal(g22y, 2 * g22)
+    assert np.all(g12z == 0.0) and np.all(g12y == 0.0)
+
+
+def test_les_metric_variants_sum_to_three_times_molecular(oracle_lib):
+    """On a skewed grid every metric is nonzero; the _z and _y variants put
+    the factor 2 on complementary products, so X_z + X_y = 3 X for X = GMAT_11,
+    GMAT_12, GMAT_22 on every face, including the extrapolated ghosts and the
+    corner rules (which treat the variants as the reference treats X)."""
+    O = oracle_lib
+    ny, nz = 31, 15
+    xp, yp = tanh_grid(ny, nz, 20.0, 2 * math.pi, 2.0)
+    xp, yp = xp + 0.05 * np.sin(0.3 * yp), yp + 0.02 * np.sin(0.5 * xp)
+    o, _ = _oracle_for_grid(O, ny, nz, xp, yp)
+    for base, vz, vy in ((0, 8, 11), (1, 9, 12), (2, 10, 13)):
+        g, gz, gy = o.metric(base), o.metric(vz), o.metric(vy)
+        fin = np.isfinite(g)
+        assert np.array_equal(fin, np.isfinite(gz)) and np.array_equal(fin, np.isfinite(gy))
+        assert np.abs(g[fin]).max() > 0.0
+        err = np.abs(gz[fin] + gy[fin] - 3.0 * g[fin]).max()
+        assert err <= 1e-12 * np.abs(g[fin]).max(), (base, err)
+        assert np.abs(gz[fin] - g[fin]).max() > 1e-6 * np.abs(g[fin]).max()
+        assert np.abs(gy[fin] - g[fin]).max() > 1e-6 * np.abs(g[fin]).max()
+
+
+def test_les_adi_constant_viscosity_on_u1_is_plain_adi(oracle_lib):
+    """U1's subgrid terms use the molecular metrics GMAT_11 / GMAT_22
+    (src/duct.F90:1667-1731, :1771-1832), so with a constant eddy viscosity
+    c the ADI with weight t2 equals the plain ADI with t1 + t2 c: the
+    explicit pre-terms, the matrix rows and the emitted right-hand side all
+    scale the same face averages.  A varying c(i) along x does the same line
+    by line.  Skewed config 1 grid, random U and R."""
+    from oracle import oracle as O
+    cfg = config(1)
+    p = cfg.params
+    xp, yp = tanh_grid(p.ny, p.nz, p.ly, p.lz, cfg.gamma)
+    xp = xp + 0.05 * np.sin(0.3 * yp)
+    o = O.Oracle(p, xp, yp)
+    rng = np.random.default_rng(7)
+    shape = (p.ny + 2, p.nz + 2, p.nx + 2)
+    u = rng.standard_normal(shape)
+    r = rng.standard_normal(shape)
+    t1, t2 = 2.0e-3, 0.05
+    ci = 1.0e-2 * (1.0 + rng.random(p.nx + 2))
+    for c in (np.full(p.nx + 2, 3.0e-2), ci):
+        nut = np.broadcast_to(c, shape).copy()
+        ul, rl = o.adi_field(0, u, r, t1, nut, t2)
+        for i in (0, 5, p.nx):
+            up, _ = o.adi_field(0, u, r, t1 + t2 * c[i])
+            err = np.abs(ul[..., i] - up[..., i]).max()
+            assert err <= 1e-12 * np.abs(up[..., i]).max(), (i, err)
+    # the viscosity changes the solution (the check is not vacuous)
+    up, _ = o.adi_field(0, u, r, t1)
+    assert np.abs(ul - up).max() > 1e-3 * np.abs(up).max()
+
+
 def test_shipped_grid_is_tanh_family():
     """rundir_unst/GRID_YZ.dat (fixture) is the tanh family with gamma 2.3:
     y uniform in z, z uniform, to the file's printed precision."""
