@@ -1,7 +1,9 @@
-// ekm_les.hip -- the dynamic Smagorinsky subgrid model of the velocity
+// ekm_les.hip -- the dynamic Smagorinsky subgrid models of the velocity
 // (reference: src/les_chan.F90 les_chan, LES_MODEL_TYPE 2 with
 // LES_IMPLICIT = .FALSE.; compute_strain_chan :1623-1717; les_filter_chan
-// :1719-1838 through FILTER_VAR, src/mpi_duct.F90:834-852).
+// :1719-1838 through FILTER_VAR, src/mpi_duct.F90:834-852) and of the scalar
+// (src/les_chan_th.F90 les_chan_th, LES_MODEL_TYPE_TH 2; compute_scalar_grad
+// :462-535).
 //
 // Every kernel is an HBM-bound FP64 pass over device-resident fields with
 // consecutive threads on consecutive x (physical) or kx (Fourier) points, so
@@ -259,9 +261,134 @@ __global__ void k_les_cf(Dims d, Metrics M, Ranges r, const double* __restrict__
   upd(CF3, C5, d2c(d, M, C6, t, k, j, dk, dj), z);
 }
 
+// ---------------------------------------------------------------------------
+// the scalar model (src/les_chan_th.F90, model 2)
+// ---------------------------------------------------------------------------
+// scalar gradient in Fourier space at the velocity interior, zero elsewhere
+// (compute_scalar_grad :480-496): G0 = i kx TH, G1 = D2(TH)/J, G2 = D3(TH)/J
+__global__ void k_les_thgrad(Dims d, Metrics M, Ranges r, const double* __restrict__ kxv, const double2* CTH,
+                             double2* G0, double2* G1, double2* G2) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n = (size_t)d.NXC * d.NZ2 * d.NY2;
+  if (t >= n) return;
+  const int m = (int)(t % d.NXC);
+  const size_t rr = t / d.NXC;
+  const int k = (int)(rr % d.NZ2), j = (int)(rr / d.NZ2);
+  const size_t dk = d.NXC, dj = (size_t)d.NXC * d.NZ2;
+  double2 g0 = make_double2(0.0, 0.0), g1 = g0, g2 = g0;
+  if (in(j, r.js, r.je) && in(k, r.zs, r.ze)) {
+    const double Jc = JAC(k, j);
+    g0 = ikx(kxv[m], CTH[t]);
+    const double2 a = d2c(d, M, CTH, t, k, j, dk, dj), b = d3c(d, M, CTH, t, k, j, dk, dj);
+    g1 = make_double2(a.x / Jc, a.y / Jc);
+    g2 = make_double2(b.x / Jc, b.y / Jc);
+  }
+  G0[t] = g0;
+  G1[t] = g1;
+  G2[t] = g2;
+}
+
+// the background gradient added to dTH/dy at the velocity interior (:521-531)
+__global__ void k_les_thbg(Dims d, Metrics M, Ranges r, const double* __restrict__ thbar, double* __restrict__ G1) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.npts) return;
+  const size_t pl = t / d.NX2;
+  const int k = (int)(pl % d.NZ2), j = (int)(pl / d.NZ2);
+  if (!(in(j, r.js, r.je) && in(k, r.zs, r.ze))) return;
+  G1[t] = G1[t] + d2<double>(d, M, thbar, pl, k, j, 1, (size_t)d.NZ2) / JAC(k, j);
+}
+
+// KAPPA_T = C_DYN_TH Delta^2 |S| on the scalar's interior, zero elsewhere
+// (:326-334), and the x flux -C_DYN_TH Delta^2 |S| dTH/dx in place of G0
+// (:341-351; the y and z components the reference also forms are unused)
+__global__ void k_les_kappa(Dims d, Ranges rt, const double* __restrict__ delta, const double* __restrict__ cdt,
+                            const double* __restrict__ sabs, double* __restrict__ kappa, double* __restrict__ G0) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.npts) return;
+  const size_t pl = t / d.NX2;
+  const int k = (int)(pl % d.NZ2), j = (int)(pl / d.NZ2);
+  const double D = delta[pl], C = cdt[pl], s = sabs[t];
+  kappa[t] = (in(j, rt.js, rt.je) && in(k, rt.zs, rt.ze)) ? C * (D * D) * s : 0.0;
+  G0[t] = -(C * (D * D) * s * G0[t]);
+}
+
+// CFTH -= J i kx (flux)^ on the scalar's interior (:411-419)
+__global__ void k_les_thcf(Dims d, Metrics M, Ranges rt, const double* __restrict__ kxv, const double2* F,
+                           double2* CFTH) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n = (size_t)d.NXC * d.NZ2 * d.NY2;
+  if (t >= n) return;
+  const int m = (int)(t % d.NXC);
+  const size_t rr = t / d.NXC;
+  const int k = (int)(rr % d.NZ2), j = (int)(rr / d.NZ2);
+  if (!(in(j, rt.js, rt.je) && in(k, rt.zs, rt.ze))) return;
+  const double a = JAC(k, j) * kxv[m];
+  const double2 f = F[t];
+  double2 c = CFTH[t];
+  c.x = c.x - (0.0 * f.x - a * f.y);
+  c.y = c.y - (0.0 * f.y + a * f.x);
+  CFTH[t] = c;
+}
+
 inline unsigned nblk(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
+
+int les_chan_th_run(const FftPlan& fft, const Dims& d, const Metrics& M, const Ranges& rv, const Ranges& rt,
+                    const double* kx, const double* thbar, double* const U[3], double* TH, double* CFTH,
+                    const LesFields& L, const LesScratch& w, bool refresh, hipStream_t st) {
+  const size_t nf = (size_t)d.NXC * d.NZ2 * d.NY2, np = d.npts;
+  const long npen = (long)d.plane;
+  double* G[3] = {L.S[0], L.S[1], L.S[2]};
+  hipLaunchKernelGGL(k_les_thgrad, dim3(nblk(nf, 256)), dim3(256), 0, st, d, M, rv, kx, (const double2*)TH,
+                     (double2*)G[0], (double2*)G[1], (double2*)G[2]);
+  EKM_CHECK(hipGetLastError());
+  for (int c = 0; c < 3; ++c) {
+    const int e = fft_inverse(fft, G[c], npen, st);
+    if (e != EKM_OK) return e;
+  }
+  hipLaunchKernelGGL(k_les_thbg, dim3(nblk(np, 256)), dim3(256), 0, st, d, M, rv, thbar, G[1]);
+  EKM_CHECK(hipGetLastError());
+  {
+    const int e = fft_inverse(fft, TH, npen, st);
+    if (e != EKM_OK) return e;
+  }
+  if (refresh) {
+    // test-filtered scalar, then the three contractions with the velocity
+    // model's filtered velocities and |S| (:155-321)
+    double* thb = L.S[3];
+    hipLaunchKernelGGL(k_les_filter, dim3(nblk(np, 256)), dim3(256), 0, st, d, (const double*)TH, thb);
+    EKM_CHECK(hipMemsetAsync(w.nsum, 0, d.plane * sizeof(double), st));
+    EKM_CHECK(hipMemsetAsync(w.dsum, 0, d.plane * sizeof(double), st));
+    for (int ij = 0; ij < 3; ++ij) {
+      hipLaunchKernelGGL(k_les_filter, dim3(nblk(np, 256)), dim3(256), 0, st, d, (const double*)G[ij], w.t1);
+      hipLaunchKernelGGL(k_les_mprep, dim3(nblk(np, 256)), dim3(256), 0, st, d, (const double*)L.delta,
+                         (const double*)G[ij], (const double*)L.sabs, (const double*)w.s2b, w.t1, w.m0);
+      hipLaunchKernelGGL(k_les_filter, dim3(nblk(np, 256)), dim3(256), 0, st, d, (const double*)w.m0, w.fm0);
+      hipLaunchKernelGGL(k_les_prod, dim3(nblk(np, 256)), dim3(256), 0, st, d, (const double*)U[ij],
+                         (const double*)TH, w.m0);
+      hipLaunchKernelGGL(k_les_filter, dim3(nblk(np, 256)), dim3(256), 0, st, d, (const double*)w.m0, w.ft2);
+      hipLaunchKernelGGL(k_les_numden, dim3(nblk(d.plane, 4)), dim3(256), 0, st, d, (const double*)w.t1,
+                         (const double*)w.fm0, (const double*)w.ft2, (const double*)thb, (const double*)w.ub[ij],
+                         1.0, w.nsum, w.dsum);
+      EKM_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_les_cdyn, dim3(nblk(d.plane, 256)), dim3(256), 0, st, d, rv, (const double*)w.nsum,
+                       (const double*)w.dsum, L.cdyn_th);
+    EKM_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_les_kappa, dim3(nblk(np, 256)), dim3(256), 0, st, d, rt, (const double*)L.delta,
+                     (const double*)L.cdyn_th, (const double*)L.sabs, L.kappa, G[0]);
+  EKM_CHECK(hipGetLastError());
+  {
+    const int e = fft_forward(fft, G[0], npen, st);
+    if (e != EKM_OK) return e;
+  }
+  hipLaunchKernelGGL(k_les_thcf, dim3(nblk(nf, 256)), dim3(256), 0, st, d, M, rt, kx, (const double2*)G[0],
+                     (double2*)CFTH);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
 
 int les_chan_run(const FftPlan& fft, const Dims& d, const Metrics& M, const Ranges& r, const double* kx,
                  double* const U[3], double* const CF[3], const LesFields& L, const LesScratch& w, bool refresh,

@@ -115,8 +115,10 @@ static int check_params(const ekm_params& p) {
   if (p.proj_mode < 0 || p.proj_mode > 2) return EKM_ERR_UNSUPPORTED;
   if (p.les_model != 0 && p.les_model != 2) return EKM_ERR_UNSUPPORTED;  // dynamic Smagorinsky only
   if (p.les_model && (p.nx > 4096)) return EKM_ERR_UNSUPPORTED;
-  // the scalar's subgrid model (src/les_chan_th.F90) is not implemented
-  if (p.les_model && p.n_th) return EKM_ERR_UNSUPPORTED;
+  // the scalar's subgrid model with a non-uniform background (cont_strat 0)
+  // adds KAPPA_T indexed by Fourier mode to the background diffusion
+  // (src/duct.F90:610-618); that form is not supported
+  if (p.les_model && p.n_th && !p.cont_strat) return EKM_ERR_UNSUPPORTED;
   if (p.proj_mode > 0) {
     // line smoothing is not cyclic, and the consistent system needs a
     // specified side to be nonsingular at kx = 0
@@ -166,7 +168,8 @@ static int free_all(ekm_solver* s) {
   if (s->adi_dl) (void)hipFree(s->adi_dl);
   {
     double* lf[] = {s->les.S[0], s->les.S[1], s->les.S[2], s->les.S[3], s->les.S[4], s->les.S[5], s->les.sabs,
-                    s->les.nut, s->les.delta, s->les.cdyn, s->lesw.ub[0], s->lesw.ub[1], s->lesw.ub[2],
+                    s->les.nut, s->les.delta, s->les.cdyn, s->les.kappa, s->les.cdyn_th,
+                    s->lesw.ub[0], s->lesw.ub[1], s->lesw.ub[2],
                     s->lesw.s2b, s->lesw.t1, s->lesw.m0, s->lesw.fm0, s->lesw.ft2, s->lesw.nsum, s->lesw.dsum};
     for (double* q : lf)
       if (q) (void)hipFree(q);
@@ -308,6 +311,14 @@ extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const doub
       if (hipMalloc(q, d.plane * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
       if (hipMemset(*q, 0, d.plane * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
     }
+    if (p.n_th) {  // the scalar model's KAPPA_T and C_DYN_TH
+      if (hipMalloc(&s->les.kappa, d.npts * sizeof(double)) != hipSuccess ||
+          hipMalloc(&s->les.cdyn_th, d.plane * sizeof(double)) != hipSuccess)
+        return fail(EKM_ERR_HIP);
+      if (hipMemset(s->les.kappa, 0, d.npts * sizeof(double)) != hipSuccess ||
+          hipMemset(s->les.cdyn_th, 0, d.plane * sizeof(double)) != hipSuccess)
+        return fail(EKM_ERR_HIP);
+    }
     std::vector<double> dl(d.plane);
     const double dx = p.lx / (double)d.NX;
     for (int j = 0; j < d.NY2; ++j)
@@ -323,14 +334,16 @@ extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const doub
       if (hipMemcpy(s->lesmet + (size_t)w * 2 * mn, s->hm.m[7 + w].data(), 2 * mn * sizeof(double),
                     hipMemcpyHostToDevice) != hipSuccess)
         return fail(EKM_ERR_HIP);
-    double* pvs[3] = {s->lesw.t1, s->lesw.m0, s->lesw.fm0};
+    double* pvs[kAdiMaxFields] = {s->lesw.t1, s->lesw.m0, s->lesw.fm0, s->lesw.ft2};
     for (int f = 0; f < kAdiMaxFields; ++f) {
       s->adi_hl[f] = s->adi_h[f];
-      if (f >= 3) continue;
+      if (f >= s->adi_nf) continue;
       AdiField& a = s->adi_hl[f];
-      a.nut = s->les.nut;
-      a.gz = les_metric(s, f, 0);
-      a.gy = les_metric(s, f, 1);
+      // the scalar's terms carry KAPPA_T with the molecular metrics
+      // (src/duct.F90:1373-1385, :1437-1450, :1489-1500, :1557-1570)
+      a.nut = f < 3 ? s->les.nut : s->les.kappa;
+      a.gz = f < 3 ? les_metric(s, f, 0) : s->M.g11;
+      a.gy = f < 3 ? les_metric(s, f, 1) : s->M.g22;
       a.pv = pvs[f];
     }
     if (hipMalloc(&s->adi_dl, sizeof(s->adi_hl)) != hipSuccess) return fail(EKM_ERR_HIP);
@@ -410,6 +423,15 @@ extern "C" int ekm_set_state(ekm_solver* s, const double* cu1, const double* cu2
 
 // NU_T (physical, the layout of a field) and C_DYN ((NZ+2) x (NY+2), K
 // fastest) of the subgrid model (src/modules.F90:431-483 les_chan_var)
+extern "C" int ekm_get_les_th(ekm_solver* s, double* kappa, double* cdyn_th) {
+  if (!s->p.les_model || !s->p.n_th) return EKM_ERR_ARG;
+  if (kappa) EKM_CHECK(hipMemcpyAsync(kappa, s->les.kappa, s->d.npts * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  if (cdyn_th)
+    EKM_CHECK(hipMemcpyAsync(cdyn_th, s->les.cdyn_th, s->d.plane * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  EKM_CHECK(hipStreamSynchronize(s->st));
+  return EKM_OK;
+}
+
 extern "C" int ekm_get_les(ekm_solver* s, double* nut, double* cdyn) {
   if (!s->p.les_model) return EKM_ERR_ARG;
   if (nut) EKM_CHECK(hipMemcpyAsync(nut, s->les.nut, s->d.npts * sizeof(double), hipMemcpyDeviceToHost, s->st));
@@ -556,7 +578,15 @@ extern "C" int ekm_rk_duct(ekm_solver* s) {
   } else {
     for (int f = 0; f < 3; ++f) TRY(inv(s, s->U[f]));
   }
-  if (th) TRY(inv(s, s->TH));
+  // the scalar's subgrid model (:650-654), which leaves the scalar in
+  // physical space, or the scalar to physical space
+  if (th && les_on) {
+    const bool refresh_th = ((cur_step % 10) == 0 && rk == 1) || s->first_time;
+    TRY(les_chan_th_run(s->fft, d, s->M, s->rv, s->rt, s->kx, s->thbar, s->U, s->TH, s->CFTH, s->les, s->lesw,
+                        refresh_th, st));
+  } else if (th) {
+    TRY(inv(s, s->TH));
+  }
   // momentum fluxes (:683-1161)
   TRY(launch_contravariant(d, s->M, s->rv, s->U[1], s->U[2], s->U2b, s->U3b, st));
   // (with the model: its eddy-viscosity cross terms, :990-1120)
@@ -567,7 +597,7 @@ extern "C" int ekm_rk_duct(ekm_solver* s) {
   for (int f = 0; f < 3; ++f) TRY(inv(s, s->R[f]));
   if (th) {  // (:1169-1625)
     TRY(launch_flux_accum(s->fft, d, s->M, s->rt, s->U[0], s->TH, s->U2b, s->U3b, p.nu / p.pr, s->kx, s->CFTH,
-                          s->RTH, TEMP5, st));
+                          s->RTH, TEMP5, st, les_on ? s->les.kappa : nullptr, les_on ? s->M.g12 : nullptr));
     TRY(inv(s, s->RTH));
   }
   // ADI of the scalar (:1334-1625) and the velocities (:1640-2391): the four

@@ -93,6 +93,19 @@ module ekman_bl_amd
        real(c_double), intent(out) :: cu1(*), cu2(*), cu3(*), cp(*), cth(*)
      end function ekm_get_state
 
+     ! the subgrid models' NU_T / C_DYN and KAPPA_T / C_DYN_TH (les_model = 2)
+     integer(c_int) function ekm_get_les(s, nut, cdyn) bind(c, name="ekm_get_les")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: s
+       real(c_double), intent(out) :: nut(*), cdyn(*)
+     end function ekm_get_les
+
+     integer(c_int) function ekm_get_les_th(s, kappa, cdyn_th) bind(c, name="ekm_get_les_th")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: s
+       real(c_double), intent(out) :: kappa(*), cdyn_th(*)
+     end function ekm_get_les_th
+
      integer(c_int) function ekm_set_initial_physical(s, u1, u2, u3, th) bind(c, name="ekm_set_initial_physical")
        import :: c_ptr, c_int, c_double
        type(c_ptr), value :: s

@@ -65,6 +65,7 @@ def lib():
         L.ekm_set_state.argtypes = [vp, _D, _D, _D, _D, _D, ctypes.c_double, ctypes.c_int]
         L.ekm_get_state.argtypes = [vp, _D, _D, _D, _D, _D]
         L.ekm_get_les.argtypes = [vp, _D, _D]
+        L.ekm_get_les_th.argtypes = [vp, _D, _D]
         L.ekm_set_initial_physical.argtypes = [vp, _D, _D, _D, _D]
         L.ekm_get_phi.argtypes = [vp, _D]
         L.ekm_get_diag.argtypes = [vp, ctypes.POINTER(EkmDiag)]
@@ -147,6 +148,14 @@ class DuctSolver:
         cdyn = np.zeros((p.ny + 2, p.nz + 2))
         _check(lib().ekm_get_les(self.h, _p(nut), _p(cdyn)), "ekm_get_les")
         return nut, cdyn
+
+    def get_les_th(self):
+        """(KAPPA_T physical (NY+2, NZ+2, NX+2), C_DYN_TH (NY+2, NZ+2)) of the scalar model."""
+        p = self.p
+        kap = np.zeros((p.ny + 2, p.nz + 2, p.nx + 2))
+        cdt = np.zeros((p.ny + 2, p.nz + 2))
+        _check(lib().ekm_get_les_th(self.h, _p(kap), _p(cdt)), "ekm_get_les_th")
+        return kap, cdt
 
     def set_initial_physical(self, u1, u2, u3, th):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u1, u2, u3, th)]

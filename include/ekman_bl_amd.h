@@ -136,6 +136,13 @@ int ekm_get_state(ekm_solver* s, double* cu1, double* cu2, double* cu3, double* 
  * coefficient computed in every substep of that step); set_initial_physical
  * as a new flow (the model starts at the second step). */
 int ekm_get_les(ekm_solver* s, double* nut, double* cdyn);
+/* The scalar subgrid model's eddy diffusivity KAPPA_T ((NX+2)*(NZ+2)*(NY+2),
+ * physical layout) and dynamic coefficient C_DYN_TH ((NZ+2)*(NY+2)) after the
+ * last substep with les_model = 2 and a scalar (KAPPA_T and C_DYN_TH of
+ * module run_variable / les_chan_var, set by les_chan_th,
+ * src/les_chan_th.F90:146-334; the coefficient is recomputed every tenth
+ * step).  Requires cont_strat = 1 with a scalar. */
+int ekm_get_les_th(ekm_solver* s, double* kappa, double* cdyn_th);
 
 /* Physical-space initial fields (U1,U2,U3,TH on (NX+2)*(NZ+2)*(NY+2) doubles):
  * forward x-FFT with 2/3 truncation, CP = 0, then the initial projection

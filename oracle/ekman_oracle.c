@@ -63,6 +63,10 @@ struct ora {
    * fields), |S|, filtered velocity, eddy viscosity, filter width and the
    * dynamic coefficient; CREATE_NEW_FLOW and FIRST_TIME of the reference */
   double *Sij[6], *SABS, *UBT[3], *NUT, *DELTA, *CDYN;
+  /* scalar model (src/les_chan_th.F90 model 2): the test-filtered |S| kept
+   * from the velocity model's refresh (S_2BAR), the eddy diffusivity
+   * KAPPA_T and the dynamic coefficient C_DYN_TH */
+  double *S2B, *KAPT, *CDYNT;
   int create_new_flow, first_time;
 };
 
@@ -1473,7 +1477,10 @@ static int supported(const ekm_params* p) {
   if (p->nx < 8 || (p->nx & (p->nx - 1))) return 0;
   if (p->proj_mode < 0 || p->proj_mode > 1) return 0;
   if (p->les_model != 0 && p->les_model != 2) return 0;
-  if (p->les_model && p->n_th) return 0; /* les_chan_th is not restated */
+  /* the scalar model with a non-uniform background (cont_strat 0) adds
+   * KAPPA_T indexed by Fourier mode to the background diffusion
+   * (src/duct.F90:610-618); not restated */
+  if (p->les_model && p->n_th && !p->cont_strat) return 0;
   if (p->proj_mode == 1) {
     int dir = 0;
     for (int s = 0; s < 4; ++s) {
@@ -1541,6 +1548,9 @@ ora* ora_create(const ekm_params* p, const double* xpoint, const double* ypoint,
   if (p->les_model) {
     for (int q = 0; q < 6; ++q) o->Sij[q] = dalloc(o->npts);
     for (int q = 0; q < 3; ++q) o->UBT[q] = dalloc(o->npts);
+    o->S2B = dalloc(o->npts);
+    o->KAPT = dalloc(o->npts);
+    o->CDYNT = dalloc(pl);
     o->SABS = dalloc(o->npts);
     o->NUT = dalloc(o->npts);
     o->DELTA = dalloc(pl);
@@ -1571,6 +1581,7 @@ void ora_destroy(ora* o) {
   for (int q = 0; q < 6; ++q) free(o->Sij[q]);
   for (int q = 0; q < 3; ++q) free(o->UBT[q]);
   free(o->SABS); free(o->NUT); free(o->DELTA); free(o->CDYN);
+  free(o->S2B); free(o->KAPT); free(o->CDYNT);
   free(o);
 }
 
@@ -1662,8 +1673,8 @@ static void bc_z(const ora* o, int comp, int J, sysb* s, const double* u, size_t
 /* the z- and y-direction metrics of the subgrid viscosity of velocity
  * component c: GMAT_11 / GMAT_22 for U1 (:1667-1731), the _y variants for U2
  * (:1916-1928, :1963-1975), the _z variants for U3 (:2177-2236) */
-static const double* les_gz(const ora* o, int c) { return o->met[c == 0 ? 0 : c == 1 ? 10 : 7]; }
-static const double* les_gy(const ora* o, int c) { return o->met[c == 0 ? 2 : c == 1 ? 12 : 9]; }
+static const double* les_gz(const ora* o, int c) { return o->met[c == 0 || c == 3 ? 0 : c == 1 ? 10 : 7]; }
+static const double* les_gy(const ora* o, int c) { return o->met[c == 0 || c == 3 ? 2 : c == 1 ? 12 : 9]; }
 
 /* nut != NULL adds the subgrid viscosity (:1667-1680, :1718-1731,
  * :1771-1784, :1819-1832 and their U2, U3 analogues) with weight t2 =
@@ -1800,8 +1811,8 @@ static void adi(ora* o, int comp, double* U, double* R, double t1, const double*
  * subgrid viscosity nut when it is not NULL: the building block the
  * substep runs, exposed so its eddy-viscosity terms can be checked alone. */
 int ora_adi_field(ora* o, int comp, double* U, double* R, double t1, const double* nut, double t2) {
-  if (comp < 0 || comp > 3 || (comp == 3 && (!o->p.n_th || nut))) return EKM_ERR_ARG;
-  adi(o, comp, U, R, t1, nut, comp < 3 ? les_gz(o, comp) : NULL, comp < 3 ? les_gy(o, comp) : NULL, t2);
+  if (comp < 0 || comp > 3 || (comp == 3 && !o->p.n_th)) return EKM_ERR_ARG;
+  adi(o, comp, U, R, t1, nut, les_gz(o, comp), les_gy(o, comp), t2);
   return EKM_OK;
 }
 
@@ -1860,11 +1871,11 @@ static void yzflux(ora* o, double* S, const double* Q, double nu, int js, int je
  * - GL(K) nu(K-1/2) (Q_eta at K-1/2) + GL(J+1) nu(J+1/2) (Q_zeta at J+1/2)
  * - GL(J) nu(J-1/2) (Q_zeta at J-1/2) ]; GL is GMAT_12 for U1, GMAT_12_y for
  * U2 and GMAT_12_z for U3. */
-static void les_xterm(ora* o, double* S, const double* Q, const double* GL, int js, int je, int zs, int ze) {
+static void les_xterm(ora* o, double* S, const double* Q, const double* N, const double* GL, int js, int je, int zs,
+                      int ze) {
   memset(S, 0, o->npts * sizeof(double));
   int NX2 = o->NX2;
   size_t dj = (size_t)NX2 * o->NZ2, dk = NX2;
-  const double* N = o->NUT;
 #pragma omp parallel for schedule(static)
   for (int j = js; j <= je; ++j)
     for (int k = zs; k <= ze; ++k)
@@ -2164,13 +2175,13 @@ static void les_chan(ora* o, int cur_step, int rk) {
    * first step of a run (:475-814) */
   if (((cur_step % 5) == 0 && rk == 1) || o->first_time) {
     const size_t np = o->npts;
-    double* S2B = dalloc(np);
+    double* S2B = o->S2B;
     double* T1 = dalloc(np);
     double* T2 = dalloc(np);
     double* M = dalloc(np);
     double* nsum = dalloc((size_t)o->NZ2 * o->NY2);
     double* dsum = dalloc((size_t)o->NZ2 * o->NY2);
-    les_filter(o, o->SABS, S2B);
+    les_filter(o, o->SABS, S2B); /* kept for the scalar model */
     static const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
     for (int ij = 0; ij < 6; ++ij) {
       const double cross = ij < 3 ? 1.0 : 2.0;
@@ -2210,7 +2221,7 @@ static void les_chan(ora* o, int cur_step, int rk) {
       }
     for (size_t t = 0; t < (size_t)o->NZ2 * o->NY2; ++t)
       if (o->CDYN[t] < 0.0) o->CDYN[t] = 0.0;
-    free(S2B); free(T1); free(T2); free(M); free(nsum); free(dsum);
+    free(T1); free(T2); free(M); free(nsum); free(dsum);
   }
   /* eddy viscosity (:825-831) */
   memset(o->NUT, 0, o->npts * sizeof(double));
@@ -2278,6 +2289,139 @@ static void les_chan(ora* o, int cur_step, int rk) {
         }
       }
     }
+}
+
+/* les_chan_th (src/les_chan_th.F90 model 2, scalar 1): on entry CTH in
+ * Fourier space and U1..3, |S| and (on a refresh) the filtered velocities
+ * and |S| from les_chan; on exit TH in physical space, KAPPA_T set and the x
+ * part of the subgrid scalar flux added to CFTH (:411-419).  The y and z
+ * parts act through KAPPA_T in the cross terms and the ADI. */
+static void les_chan_th(ora* o, int cur_step, int rk) {
+  const int js = o->jstart, je = o->jend, zs = o->zstart, ze = o->zend, NXC = o->NXC;
+  const size_t dj = (size_t)o->NX2 * o->NZ2, dk = o->NX2, np = o->npts;
+  double *G0 = o->Sij[0], *G1 = o->Sij[1], *G2 = o->Sij[2], *CTH = o->TH;
+  /* scalar gradient (compute_scalar_grad :480-496) in Fourier space at the
+   * velocity interior; the other rows carry nothing the model uses (their
+   * C_DYN_TH and KAPPA_T are zero), so they are zero here */
+  memset(G0, 0, np * sizeof(double));
+  memset(G1, 0, np * sizeof(double));
+  memset(G2, 0, np * sizeof(double));
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k) {
+      const double Jc = JAC(o, k, j);
+      for (int m = 0; m < NXC; ++m) {
+        const double kx = o->kx[m];
+        for (int c = 0; c < 2; ++c) {
+          const size_t q = CRE(o, m, k, j) + c, qo = CRE(o, m, k, j) + (1 - c);
+          const double sg = c == 0 ? -1.0 : 1.0;
+          G0[q] = sg * kx * CTH[qo];
+          G1[q] = les_d2(o, CTH, q, k, j, dk, dj) / Jc;
+          G2[q] = les_d3(o, CTH, q, k, j, dk, dj) / Jc;
+        }
+      }
+    }
+  inv(o, G0);
+  inv(o, G1);
+  inv(o, G2);
+  /* background gradient added to d/dy (:521-531) */
+  const double* TB = o->thbar;
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k) {
+      const double b = les_d2(o, TB, PL(o, k, j), k, j, 1, (size_t)o->NZ2) / JAC(o, k, j);
+      for (int i = 0; i < o->NX2; ++i) G1[IDX(o, i, k, j)] = G1[IDX(o, i, k, j)] + b;
+    }
+  inv(o, CTH); /* :144 */
+  const double* TH = CTH;
+  /* dynamic coefficient every 10th step at the first substep, or on the
+   * first step of a run (:146-321); U_BAR_TIL and S_2BAR are the velocity
+   * model's, refreshed in the same substep */
+  if (((cur_step % 10) == 0 && rk == 1) || o->first_time) {
+    double* THB = dalloc(np);
+    double* T1 = dalloc(np);
+    double* T2 = dalloc(np);
+    double* M = dalloc(np);
+    double* nsum = dalloc((size_t)o->NZ2 * o->NY2);
+    double* dsum = dalloc((size_t)o->NZ2 * o->NY2);
+    les_filter(o, TH, THB);
+    const double* G[3] = {G0, G1, G2};
+    for (int ij = 0; ij < 3; ++ij) {
+      les_filter(o, G[ij], T1);
+      for (int j = 0; j <= o->NY + 1; ++j)
+        for (int k = 0; k <= o->NZ + 1; ++k) {
+          const double D = o->DELTA[PL(o, k, j)];
+          for (int i = 0; i < o->NX2; ++i) {
+            const size_t q = IDX(o, i, k, j);
+            T1[q] = T1[q] * ((LES_ALPHA * D) * (LES_ALPHA * D)) * o->S2B[q];
+            M[q] = (D * D) * o->SABS[q] * G[ij][q];
+          }
+        }
+      les_filter(o, M, M);
+      for (size_t q = 0; q < np; ++q) M[q] = T1[q] - M[q];
+      const double* Ui = o->U[ij];
+      for (size_t q = 0; q < np; ++q) T2[q] = Ui[q] * TH[q];
+      les_filter(o, T2, T2);
+      const double* B = o->UBT[ij];
+      for (int j = 0; j <= o->NY + 1; ++j)
+        for (int k = 0; k <= o->NZ + 1; ++k) {
+          double sn = 0.0, sd = 0.0;
+          for (int i = 0; i < o->NX; ++i) {
+            const size_t q = IDX(o, i, k, j);
+            sn += M[q] * (T2[q] - THB[q] * B[q]);
+            sd += M[q] * M[q];
+          }
+          dsum[PL(o, k, j)] += sd;
+          nsum[PL(o, k, j)] += sn;
+        }
+    }
+    memset(o->CDYNT, 0, (size_t)o->NZ2 * o->NY2 * sizeof(double));
+    for (int j = js; j <= je; ++j)
+      for (int k = zs; k <= ze; ++k) {
+        const size_t t = PL(o, k, j);
+        o->CDYNT[t] = dsum[t] != 0.0 ? -0.5 * nsum[t] / dsum[t] : 0.0;
+      }
+    for (size_t t = 0; t < (size_t)o->NZ2 * o->NY2; ++t)
+      if (o->CDYNT[t] < 0.0) o->CDYNT[t] = 0.0;
+    free(THB); free(T1); free(T2); free(M); free(nsum); free(dsum);
+  }
+  /* eddy diffusivity on the scalar's interior (:326-334) and the x flux
+   * -C_DYN_TH Delta^2 |S| dTH/dx (:341-351; the y and z components, which
+   * the reference also forms, are not used) */
+  memset(o->KAPT, 0, np * sizeof(double));
+  for (int j = o->jstart_th; j <= o->jend_th; ++j)
+    for (int k = o->zstart_th; k <= o->zend_th; ++k) {
+      const size_t t = PL(o, k, j);
+      const double D = o->DELTA[t];
+      for (int i = 0; i < o->NX2; ++i) o->KAPT[IDX(o, i, k, j)] = o->CDYNT[t] * (D * D) * o->SABS[IDX(o, i, k, j)];
+    }
+  for (int j = 0; j <= o->NY + 1; ++j)
+    for (int k = 0; k <= o->NZ + 1; ++k) {
+      const size_t t = PL(o, k, j);
+      const double D = o->DELTA[t];
+      for (int i = 0; i < o->NX2; ++i) {
+        const size_t q = IDX(o, i, k, j);
+        G0[q] = -(o->CDYNT[t] * (D * D) * o->SABS[q] * G0[q]);
+      }
+    }
+  fwd(o, G0);
+  /* CFTH -= J i kx (flux)^ (:411-419) */
+  for (int j = o->jstart_th; j <= o->jend_th; ++j)
+    for (int k = o->zstart_th; k <= o->zend_th; ++k) {
+      const double Jc = JAC(o, k, j);
+      for (int m = 0; m < NXC; ++m) {
+        const size_t q = CRE(o, m, k, j);
+        const double a = Jc * o->kx[m];
+        o->CFTH[q] = o->CFTH[q] - (0.0 * G0[q] - a * G0[q + 1]);
+        o->CFTH[q + 1] = o->CFTH[q + 1] - (0.0 * G0[q + 1] + a * G0[q]);
+      }
+    }
+}
+
+int ora_get_les_th(ora* o, double* kappa, double* cdyn_th) {
+  if (!o->p.les_model || !o->p.n_th) return EKM_ERR_ARG;
+  if (kappa) memcpy(kappa, o->KAPT, o->npts * sizeof(double));
+  if (cdyn_th) memcpy(cdyn_th, o->CDYNT, (size_t)o->NZ2 * o->NY2 * sizeof(double));
+  return EKM_OK;
 }
 
 int ora_get_les(ora* o, double* nut, double* cdyn) {
@@ -2407,7 +2551,10 @@ int ora_rk_duct(ora* o) {
     les_chan(o, cur_step, rk);
   else
     for (int f = 0; f < 3; ++f) inv(o, o->U[f]);
-  if (th) inv(o, CTH);
+  if (th && les)
+    les_chan_th(o, cur_step, rk); /* :650-654, leaves TH physical */
+  else if (th)
+    inv(o, CTH);
   double *U1 = CU1, *U2 = CU2, *U3 = CU3;
   /* :683-1120 momentum fluxes */
   contravariant(o, U2, U3);
@@ -2419,7 +2566,7 @@ int ora_rk_duct(ora* o) {
   }
   if (les) /* :990-1120 */
     for (int f = 0; f < 3; ++f) {
-      les_xterm(o, o->S1, o->U[f], o->met[f == 0 ? 1 : f == 1 ? 11 : 8], js, je, zs, ze);
+      les_xterm(o, o->S1, o->U[f], o->NUT, o->met[f == 0 ? 1 : f == 1 ? 11 : 8], js, je, zs, ze);
       accum(o, o->CF[f], o->S1, 0, js, je, zs, ze);
     }
   /* :1127-1161 */
@@ -2439,6 +2586,10 @@ int ora_rk_duct(ora* o) {
     accum(o, o->CFTH, o->S1, 1, tjs, tje, tzs, tze);
     yzflux(o, o->S1, CTH, p->nu / p->pr, tjs, tje, tzs, tze);
     accum(o, o->CFTH, o->S1, 0, tjs, tje, tzs, tze);
+    if (les) { /* KAPPA_T cross terms (:1266-1311) */
+      les_xterm(o, o->S1, CTH, o->KAPT, o->met[1], tjs, tje, tzs, tze);
+      accum(o, o->CFTH, o->S1, 0, tjs, tje, tzs, tze);
+    }
 #pragma omp parallel for schedule(static)
     for (int j = tjs; j <= tje; ++j)
       for (int k = tzs; k <= tze; ++k)
@@ -2448,7 +2599,7 @@ int ora_rk_duct(ora* o) {
             o->RTH[q] = o->RTH[q] + TEMP5 * o->CFTH[q];
           }
     inv(o, o->RTH);
-    adi(o, 3, CTH, o->RTH, TEMP1 / p->pr, NULL, NULL, NULL, 0.0);
+    adi(o, 3, CTH, o->RTH, TEMP1 / p->pr, les ? o->KAPT : NULL, les_gz(o, 3), les_gy(o, 3), hb / 2.0);
   }
   /* :1640-2391 velocity ADI */
   for (int f = 0; f < 3; ++f)

@@ -66,6 +66,8 @@ int ora_set_initial_physical(ora* o, const double* u1, const double* u2,
                              const double* u3, const double* th);
 int ora_rk_duct(ora* o);
 int ora_get_les(ora* o, double* nut, double* cdyn);
+/* the scalar model: KAPPA_T (physical P field) and C_DYN_TH ((NZ+2)*(NY+2)) */
+int ora_get_les_th(ora* o, double* kappa, double* cdyn_th);
 /* one field's ADI (comp 0..2 velocity, 3 scalar) on physical U, R in place,
  * with the subgrid viscosity nut (physical P field) unless NULL */
 int ora_adi_field(ora* o, int comp, double* U, double* R, double t1, const double* nut, double t2);

@@ -48,6 +48,7 @@ def lib():
         L.ora_set_state.argtypes = [ctypes.c_void_p, _D, _D, _D, _D, _D, ctypes.c_double, ctypes.c_int]
         L.ora_get_state.argtypes = [ctypes.c_void_p, _D, _D, _D, _D, _D]
         L.ora_get_les.argtypes = [ctypes.c_void_p, _D, _D]
+        L.ora_get_les_th.argtypes = [ctypes.c_void_p, _D, _D]
         L.ora_adi_field.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.c_double, _D, ctypes.c_double]
         L.ora_set_initial_physical.argtypes = [ctypes.c_void_p, _D, _D, _D, _D]
         L.ora_get_phi.argtypes = [ctypes.c_void_p, _D]
@@ -138,6 +139,16 @@ class Oracle:
         if st != 0:
             raise RuntimeError(f"ora_get_les: {st}")
         return nut, cdyn
+
+    def get_les_th(self):
+        """(KAPPA_T physical (NY+2, NZ+2, NX+2), C_DYN_TH (NY+2, NZ+2))."""
+        p = self.p
+        kap = np.zeros((p.ny + 2, p.nz + 2, p.nx + 2))
+        cdt = np.zeros((p.ny + 2, p.nz + 2))
+        st = lib().ora_get_les_th(self.h, _p(kap), _p(cdt))
+        if st != 0:
+            raise RuntimeError(f"ora_get_les_th: {st}")
+        return kap, cdt
 
     def get_state(self):
         out = [self._zeros_c() for _ in range(5)]

@@ -590,3 +590,82 @@ def test_les_coefficient_and_viscosity_signs(oracle_lib):
     oD.rk_duct()
     b = oD.get_state()[0]
     assert np.linalg.norm(a - b) > 0.0
+
+
+def _les_th_pair(noise, th_noise, shift=0.0, time_step=3):
+    """Oracle objects without and with the subgrid models on a small
+    stratified grid (config 2's physics, 32 x 33 x 33 points), both restarted
+    from the same projected state; `shift` adds a constant to the scalar."""
+    from oracle import oracle as O
+    from ekman_bl_amd.params import mg_factors
+    cfg = config(2)
+    p = cfg.params
+    p.nx, p.ny, p.nz = 32, 31, 31
+    p.mg_ixp, p.mg_iex = mg_factors(p.nz + 2)
+    p.mg_jyq, p.mg_jey = mg_factors(p.ny + 2)
+    xp, yp = tanh_grid(p.ny, p.nz, p.ly, p.lz, cfg.gamma)
+    ic = ekman_initial(p, yp, cfg.seed, noise, th_noise)
+    o0 = O.Oracle(p, xp, yp)
+    o0.set_initial_physical(*ic)
+    st = o0.get_state()
+    cth = st[4].copy()
+    cth[..., 0] += shift
+    pl = p.copy()
+    pl.les_model = 2
+    oL = O.Oracle(pl, xp, yp)
+    oL.set_state(*st[:4], cth, time=0.0, time_step=time_step)
+    oD = O.Oracle(p, xp, yp)
+    oD.set_state(*st[:4], cth, time=0.0, time_step=time_step)
+    return p, oD, oL
+
+
+def test_les_th_vanishes_without_x_fluctuations(oracle_lib):
+    """The scalar model (src/les_chan_th.F90, model 2) on an x-independent
+    stratified flow: the filtered products equal the products of the filtered
+    fields, so C_DYN_TH and KAPPA_T vanish up to rounding and a substep with
+    both models is the plain substep for every field, scalar included."""
+    p, oD, oL = _les_th_pair(0.0, 0.0)
+    oD.rk_duct()
+    oL.rk_duct()
+    kap, cdt = oL.get_les_th()
+    assert np.abs(cdt).max() < 1e-10 and np.abs(kap).max() < 1e-12
+    for a, b in zip(oL.get_state(), oD.get_state()):
+        n = np.linalg.norm(b)
+        assert np.linalg.norm(a - b) <= 1e-12 * (n if n > 0 else 1.0)
+
+
+def test_les_th_coefficient_and_diffusivity_signs(oracle_lib):
+    """With x fluctuations of velocity and scalar the clipped C_DYN_TH is
+    nonnegative, positive somewhere and zero outside the interior rows;
+    KAPPA_T = C_DYN_TH Delta^2 |S| is nonnegative and zero outside the
+    scalar's interior (src/les_chan_th.F90:300-334); the model changes the
+    scalar."""
+    p, oD, oL = _les_th_pair(0.05, 0.01)
+    oL.rk_duct()
+    kap, cdt = oL.get_les_th()
+    assert (cdt >= 0.0).all() and (kap >= 0.0).all()
+    assert cdt.max() > 0.0 and kap.max() > 0.0
+    assert np.all(cdt[0, :] == 0.0) and np.all(cdt[-1, :] == 0.0)
+    assert np.all(cdt[:, 0] == 0.0) and np.all(cdt[:, -1] == 0.0)
+    assert np.all(kap[0] == 0.0) and np.all(kap[-1] == 0.0)
+    assert np.all(kap[:, 0] == 0.0) and np.all(kap[:, -1] == 0.0)
+    oD.rk_duct()
+    a, b = oL.get_state()[4], oD.get_state()[4]
+    assert np.linalg.norm(a - b) > 1e-9 * np.linalg.norm(b)
+
+
+def test_les_th_coefficient_invariant_under_scalar_shift(oracle_lib):
+    """A constant added to the scalar leaves its gradient unchanged (the
+    difference operators annihilate constants on the rectilinear grid) and
+    cancels in the resolved flux filter(u_i th) - filter(u_i) filter(th), so
+    C_DYN_TH and KAPPA_T of the first substep do not change (to rounding).
+    A filtered velocity paired with the wrong product would break this."""
+    _, _, o1 = _les_th_pair(0.05, 0.01)
+    _, _, o2 = _les_th_pair(0.05, 0.01, shift=3.0)
+    o1.rk_duct()
+    o2.rk_duct()
+    k1, c1 = o1.get_les_th()
+    k2, c2 = o2.get_les_th()
+    assert c1.max() > 0.0
+    assert np.abs(c1 - c2).max() <= 1e-8 * c1.max()
+    assert np.abs(k1 - k2).max() <= 1e-8 * k1.max()
