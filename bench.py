@@ -84,6 +84,14 @@ def run_form(cfg, xp, yp, ic, mode: int, steps: int, warmup: int, tol: float, gu
             "mg_ms": mg_ms, "launches": launches, "solves": solves, "cycles": cycles, "dt": dt}
 
 
+def workload_name(cfg, mode: int, tol: float, krylov: int) -> str:
+    return (f"{cfg.name}: RK3+CN step"
+            + (", dynamic Smagorinsky LES of velocity and scalar" if cfg.params.les_model else "")
+            + f", {FORM_NAMES[mode]}"
+            + (f", max|residual| <= {tol:g}" if mode else "")
+            + (", GCR(1) outer iteration" if mode and krylov else ""))
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_tight_solve_hbm_traffic.md")
 
 
@@ -211,9 +219,7 @@ def main() -> int:
             "div_max_max": r["div_max"],
             "meets_div_bound": bool(r["div_max"] < DIV_BOUND),
         }
-    workload = (f"{cfg.name}: RK3+CN step, {FORM_NAMES[args.mode]}"
-                + (f", max|residual| <= {args.tol:g}" if args.mode else "")
-                + (", GCR(1) outer iteration" if args.mode and args.krylov else ""))
+    workload = workload_name(cfg, args.mode, args.tol, args.krylov)
     tr = measured_traffic(workload) if args.mode == 2 else None
     out = {
         "metric": "Mpt-steps/s at ||div u||<1e-10",

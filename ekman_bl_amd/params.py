@@ -172,6 +172,8 @@ def config(n: int) -> Config:
         p.th_bc[YMAX] = 1; p.th_bc_c1[YMAX] = 0.0
         p.th_bc[ZMIN] = 1; p.th_bc[ZMAX] = 1
         p.delta_t, p.cfl = 0.1, 0.5
+        if n == 5:
+            p.les_model = 2   # dynamic Smagorinsky for the velocity and the scalar
         steps = {2: 1000, 3: 500, 5: 2000}[n]
         return Config(f"config{n}_stratified_{dims[0]}x{dims[1]+2}x{dims[2]+2}", p, 2.0, steps, 2, 0.05, 0.01)
     raise ValueError(f"no configuration {n}")
