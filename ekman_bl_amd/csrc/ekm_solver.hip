@@ -654,6 +654,13 @@ extern "C" int ekm_time_step(ekm_solver* s, int nsteps) {
   return EKM_OK;
 }
 
+extern "C" int ekm_get_solve_cycles(ekm_solver* s, int* cycles, int n) {
+  const int ns = (int)s->cyc_host.size();
+  if (!cycles || n < ns) return EKM_ERR_ARG;
+  for (int q = 0; q < ns; ++q) cycles[q] = s->cyc_host[q];
+  return EKM_OK;
+}
+
 extern "C" int ekm_get_diag(ekm_solver* s, ekm_diag* dg) {
   *dg = s->diag;
   dg->delta_t = s->delta_t;

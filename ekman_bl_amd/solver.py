@@ -66,6 +66,7 @@ def lib():
         L.ekm_get_state.argtypes = [vp, _D, _D, _D, _D, _D]
         L.ekm_get_les.argtypes = [vp, _D, _D]
         L.ekm_get_les_th.argtypes = [vp, _D, _D]
+        L.ekm_get_solve_cycles.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
         L.ekm_set_initial_physical.argtypes = [vp, _D, _D, _D, _D]
         L.ekm_get_phi.argtypes = [vp, _D]
         L.ekm_get_diag.argtypes = [vp, ctypes.POINTER(EkmDiag)]
@@ -141,6 +142,26 @@ class DuctSolver:
         _check(lib().ekm_get_state(self.h, *[_c(a) for a in out]), "ekm_get_state")
         return out  # cu1, cu2, cu3, cp, cth
 
+    def save_flow(self, run_dir: str, kind: str = "saved", backup_index=None):
+        """Write the state as the reference's SAVE_FLOW / BACKUP_FLOW would
+        (src/diablo.F90:753-916) under run_dir; returns the paths."""
+        from . import restart
+        cu1, cu2, cu3, cp, cth = self.get_state()
+        d = self.diag()
+        p = self.p
+        flow = restart.Flow(p.nx, p.ny, p.nz, d.time, d.time_step, cu1, cu2, cu3, cp, [cth] if p.n_th else [])
+        return restart.save_flow(run_dir, flow, kind, backup_index)
+
+    def read_flow(self, run_dir: str, nranks: int = 1, kind: str = "saved", backup_index=None):
+        """Restart from the files READ_FLOW reads (src/diablo.F90:918-1000),
+        written by nranks ranks; a restart in the sense of set_state."""
+        from . import restart
+        p = self.p
+        fl = restart.read_flow(run_dir, p.nx, p.ny, p.nz, p.n_th, nranks, kind, backup_index)
+        self.set_state(fl.cu1, fl.cu2, fl.cu3, fl.cp, fl.cth[0] if p.n_th else None, time=fl.time,
+                       time_step=fl.time_step)
+        return fl
+
     def get_les(self):
         """(NU_T physical (NY+2, NZ+2, NX+2), C_DYN (NY+2, NZ+2)) of the subgrid model."""
         p = self.p
@@ -148,6 +169,14 @@ class DuctSolver:
         cdyn = np.zeros((p.ny + 2, p.nz + 2))
         _check(lib().ekm_get_les(self.h, _p(nut), _p(cdyn)), "ekm_get_les")
         return nut, cdyn
+
+    def solve_cycles(self) -> np.ndarray:
+        """Cycles of every real solve of the last projection (2 per kx mode)."""
+        n = 2 * (self.p.nx // 3 + 1)
+        out = np.zeros(n, dtype=np.int32)
+        _check(lib().ekm_get_solve_cycles(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n),
+               "ekm_get_solve_cycles")
+        return out
 
     def get_les_th(self):
         """(KAPPA_T physical (NY+2, NZ+2, NX+2), C_DYN_TH (NY+2, NZ+2)) of the scalar model."""

@@ -160,6 +160,11 @@ int ekm_time_step(ekm_solver* s, int nsteps);
 int ekm_rem_div_curv(ekm_solver* s);
 int ekm_get_phi(ekm_solver* s, double* phi);
 int ekm_get_diag(ekm_solver* s, ekm_diag* d);
+/* Cycle count of every real solve of the last projection (MUDPACK's iparm(17)
+ * per MULTIGRID_MUDPACK call, src/mg_solver2d.F90:195-199), n >= mg_solves
+ * entries; solves 2m and 2m+1 belong to kx mode m (imaginary, then real part
+ * of the right-hand side, src/duct.F90:2641-2660). */
+int ekm_get_solve_cycles(ekm_solver* s, int* cycles, int n);
 /* Blocks until queued work on the solver's stream is done. */
 int ekm_sync(ekm_solver* s);
 /* Device pointer of state field f (0=U1,1=U2,2=U3,3=P,4=TH); the field's
