@@ -438,6 +438,10 @@ __global__ void __launch_bounds__(256) k_zsweep(Dims d, Metrics M, const AdiFiel
   }
 }
 
+// rows per batch of the sweeps with a subgrid viscosity, which keep more
+// operands per row in flight than the plain sweeps
+constexpr int kSwL = 6;
+
 // ---------------------------------------------------------------------------
 // Sweeps with the subgrid viscosity (fields with nut): every (i, line) has its
 // own system, so each thread forms its rows as it eliminates -- the molecular
@@ -470,16 +474,16 @@ __global__ void __launch_bounds__(256) k_ysweep_les(Dims d, Metrics M, const Adi
   for (int e = 0; e < 4; ++e) lag[e] = (bc.r[e].kind == BC_LAGGED) ? U[q0 + dj * bc.r[e].src] : 0.0;
   double g = 0.0, pvp = 1.0, cp = 0.0;
   double nm = 0.0, nc = nut[q0];  // NU_T at row-1 and row
-  for (int row0 = 0; row0 <= N; row0 += kSw) {
-    double rr[kSw], nn[kSw];
+  for (int row0 = 0; row0 <= N; row0 += kSwL) {
+    double rr[kSwL], nn[kSwL];
 #pragma unroll
-    for (int u = 0; u < kSw; ++u) {
+    for (int u = 0; u < kSwL; ++u) {
       const int row = row0 + u;
       rr[u] = (row <= N && row >= r.js && row <= r.je) ? R[q0 + dj * row] : 0.0;
       nn[u] = row + 1 <= N ? nut[q0 + dj * (row + 1)] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < kSw; ++u) {
+    for (int u = 0; u < kSwL; ++u) {
       const int row = row0 + u;
       if (row > N) break;
       const double np = nn[u];
@@ -521,10 +525,10 @@ __global__ void __launch_bounds__(256) k_ysweep_les(Dims d, Metrics M, const Adi
   double x2 = 0.0, x1 = g / pvp;  // x1 = x(row+1), x2 = x(row+2)
   U[q0 + dj * N] = x1;
   double n1 = nut[q0 + dj * N], n2 = 0.0;  // NU_T at row+1, row+2
-  for (int row0 = N - 1; row0 >= 0; row0 -= kSw) {
-    double gg[kSw], pp[kSw], nl[kSw], ss[kSw], jc[kSw], gp[kSw], g0[kSw], lp[kSw], l0[kSw];
+  for (int row0 = N - 1; row0 >= 0; row0 -= kSwL) {
+    double gg[kSwL], pp[kSwL], nl[kSwL], ss[kSwL], jc[kSwL], gp[kSwL], g0[kSwL], lp[kSwL], l0[kSwL];
 #pragma unroll
-    for (int u = 0; u < kSw; ++u) {
+    for (int u = 0; u < kSwL; ++u) {
       const int row = row0 - u, jr = row + 1;
       const bool ok = row >= 0;
       const bool in = ok && jr >= r.js && jr <= r.je;
@@ -539,7 +543,7 @@ __global__ void __launch_bounds__(256) k_ysweep_les(Dims d, Metrics M, const Adi
       l0[u] = ok ? gy[mf(d, K, jr, 1)] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < kSw; ++u) {
+    for (int u = 0; u < kSwL; ++u) {
       const int row = row0 - u;
       if (row < 0) break;
       const double n0 = nl[u];
@@ -587,16 +591,16 @@ __global__ void __launch_bounds__(256) k_zsweep_les(Dims d, Metrics M, const Adi
   double g = 0.0, pvp = 1.0, cp = 0.0;
   double ap = 0.0, bp = 1.0, rp = 0.0;  // previous row before elimination (THOMAS_REAL_SP)
   double nm = 0.0, nc = nut[q0];
-  for (int row0 = 0; row0 <= N; row0 += kSw) {
-    double rr[kSw], nn[kSw];
+  for (int row0 = 0; row0 <= N; row0 += kSwL) {
+    double rr[kSwL], nn[kSwL];
 #pragma unroll
-    for (int u = 0; u < kSw; ++u) {
+    for (int u = 0; u < kSwL; ++u) {
       const int row = row0 + u;
       rr[u] = (row <= N && row >= r.zs && row <= r.ze) ? R[q0 + dk * row] : 0.0;
       nn[u] = row + 1 <= N ? nut[q0 + dk * (row + 1)] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < kSw; ++u) {
+    for (int u = 0; u < kSwL; ++u) {
       const int row = row0 + u;
       if (row > N) break;
       const double np = nn[u];
@@ -646,10 +650,10 @@ __global__ void __launch_bounds__(256) k_zsweep_les(Dims d, Metrics M, const Adi
   double x1 = g / pvp;
   U[q0 + dk * N] = x1;
   double n1 = nut[q0 + dk * N];
-  for (int row0 = N - 1; row0 >= 0; row0 -= kSw) {
-    double gg[kSw], pp[kSw], nl[kSw], gk[kSw], lk[kSw];
+  for (int row0 = N - 1; row0 >= 0; row0 -= kSwL) {
+    double gg[kSwL], pp[kSwL], nl[kSwL], gk[kSwL], lk[kSwL];
 #pragma unroll
-    for (int u = 0; u < kSw; ++u) {
+    for (int u = 0; u < kSwL; ++u) {
       const int row = row0 - u;
       const bool ok = row >= 0;
       gg[u] = ok ? U[q0 + dk * row] : 0.0;
@@ -659,7 +663,7 @@ __global__ void __launch_bounds__(256) k_zsweep_les(Dims d, Metrics M, const Adi
       lk[u] = ok ? gz[mf(d, row + 1, J, 2)] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < kSw; ++u) {
+    for (int u = 0; u < kSwL; ++u) {
       const int row = row0 - u;
       if (row < 0) break;
       const double n0 = nl[u];

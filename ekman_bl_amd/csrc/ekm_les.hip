@@ -201,10 +201,8 @@ __global__ void k_les_stress(Dims d, Metrics M, Ranges r, const double* __restri
   L.nut[t] = inner ? C * (D * D) * s : 0.0;
   const double f = ((-2.0 * C) * (D * D)) * s;
   if (j >= 1 && k >= 1) L.S[0][t] = f * L.S[0][t];
-  if (k >= 1) {
-    L.S[1][t] = 0.0;
-    L.S[2][t] = 0.0;
-  }
+  // components 2 and 3 of the stress are zero (:866-880): they are neither
+  // transformed nor read below, so S[1] and S[2] are left as they are
   if (inner) {
     const size_t dk = d.NX2, dj = (size_t)d.NX2 * d.NZ2;
     const double Jc = JAC(k, j);
@@ -265,9 +263,11 @@ __global__ void k_les_cf(Dims d, Metrics M, Ranges r, const double* __restrict__
 // the scalar model (src/les_chan_th.F90, model 2)
 // ---------------------------------------------------------------------------
 // scalar gradient in Fourier space at the velocity interior, zero elsewhere
-// (compute_scalar_grad :480-496): G0 = i kx TH, G1 = D2(TH)/J, G2 = D3(TH)/J
+// (compute_scalar_grad :480-496): G0 = i kx TH, G1 = D2(TH)/J, G2 = D3(TH)/J.
+// G1 and G2 enter only the dynamic coefficient, so they are formed (full)
+// only on the substeps that recompute it.
 __global__ void k_les_thgrad(Dims d, Metrics M, Ranges r, const double* __restrict__ kxv, const double2* CTH,
-                             double2* G0, double2* G1, double2* G2) {
+                             double2* G0, double2* G1, double2* G2, int full) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t n = (size_t)d.NXC * d.NZ2 * d.NY2;
   if (t >= n) return;
@@ -279,13 +279,17 @@ __global__ void k_les_thgrad(Dims d, Metrics M, Ranges r, const double* __restri
   if (in(j, r.js, r.je) && in(k, r.zs, r.ze)) {
     const double Jc = JAC(k, j);
     g0 = ikx(kxv[m], CTH[t]);
-    const double2 a = d2c(d, M, CTH, t, k, j, dk, dj), b = d3c(d, M, CTH, t, k, j, dk, dj);
-    g1 = make_double2(a.x / Jc, a.y / Jc);
-    g2 = make_double2(b.x / Jc, b.y / Jc);
+    if (full) {
+      const double2 a = d2c(d, M, CTH, t, k, j, dk, dj), b = d3c(d, M, CTH, t, k, j, dk, dj);
+      g1 = make_double2(a.x / Jc, a.y / Jc);
+      g2 = make_double2(b.x / Jc, b.y / Jc);
+    }
   }
   G0[t] = g0;
-  G1[t] = g1;
-  G2[t] = g2;
+  if (full) {
+    G1[t] = g1;
+    G2[t] = g2;
+  }
 }
 
 // the background gradient added to dTH/dy at the velocity interior (:521-531)
@@ -341,14 +345,16 @@ int les_chan_th_run(const FftPlan& fft, const Dims& d, const Metrics& M, const R
   const long npen = (long)d.plane;
   double* G[3] = {L.S[0], L.S[1], L.S[2]};
   hipLaunchKernelGGL(k_les_thgrad, dim3(nblk(nf, 256)), dim3(256), 0, st, d, M, rv, kx, (const double2*)TH,
-                     (double2*)G[0], (double2*)G[1], (double2*)G[2]);
+                     (double2*)G[0], (double2*)G[1], (double2*)G[2], refresh ? 1 : 0);
   EKM_CHECK(hipGetLastError());
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < (refresh ? 3 : 1); ++c) {
     const int e = fft_inverse(fft, G[c], npen, st);
     if (e != EKM_OK) return e;
   }
-  hipLaunchKernelGGL(k_les_thbg, dim3(nblk(np, 256)), dim3(256), 0, st, d, M, rv, thbar, G[1]);
-  EKM_CHECK(hipGetLastError());
+  if (refresh) {
+    hipLaunchKernelGGL(k_les_thbg, dim3(nblk(np, 256)), dim3(256), 0, st, d, M, rv, thbar, G[1]);
+    EKM_CHECK(hipGetLastError());
+  }
   {
     const int e = fft_inverse(fft, TH, npen, st);
     if (e != EKM_OK) return e;

@@ -787,6 +787,8 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
     const int ex = k + p.mg_iex - ng, ey = k + p.mg_jey - ng;
     const int nx = p.mg_ixp * (1 << ((ex > 1 ? ex : 1) - 1)) + 1;
     const int ny = p.mg_jyq * (1 << ((ey > 1 ? ey : 1) - 1)) + 1;
+    // the y-line smoother keeps one LDS slot per (64-column group, chunk)
+    if ((nx + 1) / 2 > 64 * kYMaxTasks) return EKM_ERR_UNSUPPORTED;
     int st = host_mg_level(hm, d.NZ, d.NY, p.mudpack_bc, nx, ny, hl[k]);
     if (st) return st;
     TLev<T>& L = g.lev[k];
