@@ -62,5 +62,8 @@ int launch_project(const Dims& d, const Metrics& M, const Ranges& r, const doubl
                    hipStream_t st);
 int launch_pupdate(const Dims& d, const Ranges& r, const double* PHI, double* CP, double temp4, hipStream_t st);
 int launch_div_reduce(const Dims& d, const double* S, double* out, hipStream_t st);
+// flow-rate integral of the F_TYPE 0 controller into out[0]
+int launch_flow_rate(const Dims& d, const Ranges& r, const double* CU3, const double* dy, double* out,
+                     hipStream_t st);
 int launch_thomas_generic(const double* A, const double* B, const double* C, double* G, double* W, int ny,
                           int nx, hipStream_t st);

@@ -889,6 +889,29 @@ __global__ void k_pupdate(Dims d, Ranges r, const double2* __restrict__ P, doubl
   CP[q] = c;
 }
 
+// flow rate of the F_TYPE 0 controller (:416-428): the sum over J = JSTART
+// .. JEND-1, K = ZSTART+1 .. ZEND-1 of (W0(K,J+1) + W0(K,J)) dy(K,J) / 2 with
+// W0 the real part of the mean (kx = 0) mode of U3 and dy(K,J) =
+// ypoint(K,J+1) - ypoint(K,J); one workgroup, written to out[0]
+__global__ void __launch_bounds__(256) k_flow_rate(Dims d, Ranges r, const double2* __restrict__ CU3,
+                                                   const double* __restrict__ dy, double* __restrict__ out) {
+  __shared__ double red[256];
+  const int nk = r.ze - r.zs - 1, nj = r.je - r.js;
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < nk * nj; t += blockDim.x) {
+    const int k = r.zs + 1 + t % nk, j = r.js + t / nk;
+    const double w1 = CU3[fo(d, 0, k, j + 1)].x, w0 = CU3[fo(d, 0, k, j)].x;
+    acc += (w1 + w0) * dy[(size_t)k + (size_t)d.NZ2 * j] / 2;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
 // divergence diagnostic (:2790-2800): sum and max of |div| over
 // J = 2..NY-1, K = 2..NZ-1, I = 1..NX+1 of the physical field
 __global__ void k_div_reduce(Dims d, const double* __restrict__ S, double* __restrict__ out) {
@@ -1059,6 +1082,13 @@ int launch_pupdate(const Dims& d, const Ranges& r, const double* PHI, double* CP
   size_t n = (size_t)d.NXC * (r.ze - r.zs + 3) * (r.je - r.js + 3);
   hipLaunchKernelGGL(k_pupdate, dim3(nblk(n, 256)), dim3(256), 0, st, d, r, (const double2*)PHI, (double2*)CP,
                      temp4);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_flow_rate(const Dims& d, const Ranges& r, const double* CU3, const double* dy, double* out,
+                     hipStream_t st) {
+  if (r.ze - r.zs - 1 < 1 || r.je - r.js < 1) return EKM_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(k_flow_rate, dim3(1), dim3(256), 0, st, d, r, (const double2*)CU3, dy, out);
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }

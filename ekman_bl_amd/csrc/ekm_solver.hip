@@ -46,7 +46,7 @@ struct ekm_solver {
   AdiField adi_h[kAdiMaxFields];
   AdiField* adi_d = nullptr;  // the same descriptors on the device
   int adi_nf = 0;
-  double* red = nullptr;  // [0] div sum, [1] div max (bits), [2] cfl min (bits)
+  double* red = nullptr;  // [0] div sum, [1] div max (bits), [2] cfl min (bits), [3] flow rate
   BcTable bcy[4], bcz[4];
   double time = 0.0, delta_t = 0.0, delta_t_set = 0.0;
   double h_bar[3], beta_bar[3], zeta_bar[3];
@@ -68,6 +68,11 @@ struct ekm_solver {
   AdiField* adi_dl = nullptr;
   int create_new_flow = 1, first_time = 1;
   int have_prev_phi = 0;  // mg.phi holds the previous substep's correction
+  // F_TYPE 0 flow-rate controller (src/duct.F90:414-436): PX0 and Qn, started
+  // at Q0 = Qn = WBULK0 (src/diablo.F90:359-361); y spacings of its integral
+  double px0 = 0.0, qn = 0.0;
+  double* fdy = nullptr;
+  double q_den = 1.0;  // (ypoint(1,JEND) - ypoint(1,JSTART)) * (ZEND-ZSTART-1)
   double hb_prev = 0.0;
 };
 
@@ -107,7 +112,7 @@ static int check_params(const ekm_params& p) {
   if (p.nz + 2 != p.mg_ixp * (1 << (p.mg_iex - 1)) + 1) return EKM_ERR_ARG;   // src/mg_solver2d.F90:138-140
   if (p.ny + 2 != p.mg_jyq * (1 << (p.mg_jey - 1)) + 1) return EKM_ERR_ARG;
   if (p.mg_ixp < 2 || p.mg_jyq < 2 || p.mg_maxcy < 1) return EKM_ERR_ARG;
-  if (p.f_type != 1 && p.f_type != 2 && p.f_type != 3) return EKM_ERR_UNSUPPORTED;
+  if (p.f_type < 0 || p.f_type > 3) return EKM_ERR_UNSUPPORTED;  // 3: no mean forcing
   for (int s = 0; s < 4; ++s)
     if (p.mudpack_bc[s] < 0 || p.mudpack_bc[s] > 2) return EKM_ERR_UNSUPPORTED;
   if ((p.mudpack_bc[0] == 0) != (p.mudpack_bc[1] == 0)) return EKM_ERR_ARG;
@@ -161,7 +166,7 @@ static void set_hbar(ekm_solver* s) {
 static int free_all(ekm_solver* s) {
   double* f[] = {s->U[0], s->U[1], s->U[2], s->P, s->TH, s->CF[0], s->CF[1], s->CF[2], s->CFTH,
                  s->R[0], s->R[1], s->R[2], s->RTH, s->S1, s->S2, s->U2b, s->U3b, s->PHI, s->lu,
-                 s->red, s->met, s->kx, s->kx2, s->kx2s, s->thbar, s->lesmet};
+                 s->red, s->met, s->kx, s->kx2, s->kx2s, s->thbar, s->lesmet, s->fdy};
   for (double* q : f)
     if (q) (void)hipFree(q);
   if (s->adi_d) (void)hipFree(s->adi_d);
@@ -351,6 +356,21 @@ extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const doub
       return fail(EKM_ERR_HIP);
   }
   if (hipMalloc(&s->red, 4 * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
+  s->px0 = p.px0;
+  s->qn = p.wbulk0;
+  if (p.f_type == 0) {
+    std::vector<double> dy(d.plane, 0.0);
+    for (int j = 0; j + 1 < d.NY2; ++j)
+      for (int k = 0; k < d.NZ2; ++k)
+        dy[(size_t)k + (size_t)d.NZ2 * j] = ypoint[(size_t)k + (size_t)d.NZ2 * (j + 1)] - ypoint[(size_t)k + (size_t)d.NZ2 * j];
+    const Ranges& r = s->rv;
+    const double span = ypoint[1 + (size_t)d.NZ2 * r.je] - ypoint[1 + (size_t)d.NZ2 * r.js];
+    if (r.ze - r.zs - 1 < 1 || r.je - r.js < 1 || !(span != 0.0)) return fail(EKM_ERR_UNSUPPORTED);
+    s->q_den = span;
+    if (hipMalloc(&s->fdy, d.plane * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
+    if (hipMemcpy(s->fdy, dy.data(), d.plane * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(EKM_ERR_HIP);
+  }
   if (fft_plan_init(s->fft, p.nx) != EKM_OK) return fail(EKM_ERR_HIP);
   int nsolve = 2 * (d.NKX + 1);
   {
@@ -423,6 +443,12 @@ extern "C" int ekm_set_state(ekm_solver* s, const double* cu1, const double* cu2
 
 // NU_T (physical, the layout of a field) and C_DYN ((NZ+2) x (NY+2), K
 // fastest) of the subgrid model (src/modules.F90:431-483 les_chan_var)
+extern "C" int ekm_get_forcing(ekm_solver* s, double* px0, double* qn) {
+  if (px0) *px0 = s->p.f_type == 0 ? s->px0 : s->p.px0;
+  if (qn) *qn = s->qn;
+  return EKM_OK;
+}
+
 extern "C" int ekm_get_les_th(ekm_solver* s, double* kappa, double* cdyn_th) {
   if (!s->p.les_model || !s->p.n_th) return EKM_ERR_ARG;
   if (kappa) EKM_CHECK(hipMemcpyAsync(kappa, s->les.kappa, s->d.npts * sizeof(double), hipMemcpyDeviceToHost, s->st));
@@ -557,7 +583,7 @@ extern "C" int ekm_rk_duct(ekm_solver* s) {
   a.th = th;
   a.cont_strat = p.cont_strat;
   a.cor_mean_only = p.coriolis_mean_only;
-  a.px_on = (p.f_type == 1 || p.f_type == 2);
+  a.px_on = (p.f_type == 0 || p.f_type == 1 || p.f_type == 2);
   a.temp3 = TEMP3;
   a.temp4 = TEMP4;
   a.nu = p.nu;
@@ -567,6 +593,19 @@ extern "C" int ekm_rk_duct(ekm_solver* s) {
   a.ri_tau = p.ri_tau;
   a.nupr = p.nu / p.pr;
   a.px = p.f_type == 1 ? p.px0 : p.px0 + p.amp_omega0 * std::cos(p.omega0 * s->time);
+  if (p.f_type == 0) {
+    // the flow-rate controller (:414-436) on the state entering the substep
+    TRY(launch_flow_rate(d, s->rv, s->U[2], s->fdy, s->red + 3, st));
+    double sum;
+    EKM_CHECK(hipMemcpyAsync(&sum, s->red + 3, sizeof(double), hipMemcpyDeviceToHost, st));
+    EKM_CHECK(hipStreamSynchronize(st));
+    const double qn1 = sum / s->q_den / (double)(float)(s->rv.ze - s->rv.zs - 1);
+    // WEIGHT = 0.1 is assigned from a default-real literal (src/diablo.F90:361)
+    const double weight = (double)0.1f;
+    s->px0 = s->px0 - (weight / TEMP4) * (s->qn + p.wbulk0 - 2 * qn1);
+    s->qn = qn1;
+    a.px = s->px0;
+  }
   TRY(launch_rhs_fourier(d, s->M, a, st));
   // velocities and scalar to physical space (:655-671)
   // the subgrid model (:641-657), which leaves the velocities in physical

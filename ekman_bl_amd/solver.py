@@ -67,6 +67,7 @@ def lib():
         L.ekm_get_les.argtypes = [vp, _D, _D]
         L.ekm_get_les_th.argtypes = [vp, _D, _D]
         L.ekm_get_solve_cycles.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+        L.ekm_get_forcing.argtypes = [vp, _D, _D]
         L.ekm_set_initial_physical.argtypes = [vp, _D, _D, _D, _D]
         L.ekm_get_phi.argtypes = [vp, _D]
         L.ekm_get_diag.argtypes = [vp, ctypes.POINTER(EkmDiag)]
@@ -169,6 +170,12 @@ class DuctSolver:
         cdyn = np.zeros((p.ny + 2, p.nz + 2))
         _check(lib().ekm_get_les(self.h, _p(nut), _p(cdyn)), "ekm_get_les")
         return nut, cdyn
+
+    def forcing(self):
+        """(PX0, Qn) of the F_TYPE 0 flow-rate controller."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().ekm_get_forcing(self.h, ctypes.byref(a), ctypes.byref(b)), "ekm_get_forcing")
+        return a.value, b.value
 
     def solve_cycles(self) -> np.ndarray:
         """Cycles of every real solve of the last projection (2 per kx mode)."""

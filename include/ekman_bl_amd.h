@@ -55,7 +55,8 @@ typedef struct ekm_params {
   double cfl;              /* CFL */
   int update_dt;           /* UPDATE_DT */
   /* forcing */
-  int f_type;              /* F_TYPE: 0,1,2 (mean-flow forcing on mode 0) */
+  int f_type;              /* F_TYPE: 0 constant flow rate, 1 constant and 2 oscillating
+                              pressure gradient on mode 0, 3 none (src/duct.F90:411-455) */
   double wbulk0, px0, omega0, amp_omega0, f0;
   /* velocity BCs: [component 0=U1,1=U2(V),2=U3(W)][slot] type and C1 */
   int u_bc[3][4];
@@ -165,6 +166,11 @@ int ekm_get_diag(ekm_solver* s, ekm_diag* d);
  * entries; solves 2m and 2m+1 belong to kx mode m (imaginary, then real part
  * of the right-hand side, src/duct.F90:2641-2660). */
 int ekm_get_solve_cycles(ekm_solver* s, int* cycles, int n);
+/* Mean pressure gradient PX0 and last flow rate Qn of the F_TYPE 0 flow-rate
+ * controller (PX0, Qn of module run_variable, updated every substep at
+ * src/duct.F90:414-436; both start from PX0 and Q0 = WBULK0,
+ * src/diablo.F90:359-361); for F_TYPE 1, 2, 3 px0 is the fixed PX0. */
+int ekm_get_forcing(ekm_solver* s, double* px0, double* qn);
 /* Blocks until queued work on the solver's stream is done. */
 int ekm_sync(ekm_solver* s);
 /* Device pointer of state field f (0=U1,1=U2,2=U3,3=P,4=TH); the field's

@@ -68,6 +68,10 @@ struct ora {
    * KAPPA_T and the dynamic coefficient C_DYN_TH */
   double *S2B, *KAPT, *CDYNT;
   int create_new_flow, first_time;
+  /* F_TYPE 0 flow-rate controller (src/duct.F90:414-436): the running mean
+   * pressure gradient PX0 and the previous flow rate Qn, set at start-up as
+   * Q0 = Qn = WBULK0 (src/diablo.F90:359-361) */
+  double px0, qn;
 };
 
 #define IDX(o, i, k, j) ((size_t)(i) + (size_t)(o)->NX2 * ((size_t)(k) + (size_t)(o)->NZ2 * (size_t)(j)))
@@ -1468,7 +1472,7 @@ static int supported(const ekm_params* p) {
     t = p->th_bc[EKM_ZMIN]; if (t != 0 && t != 1 && t != 6) return 0;
     t = p->th_bc[EKM_ZMAX]; if (t != 0 && t != 1 && t != 5 && t != 6) return 0;
   }
-  if (p->f_type != 1 && p->f_type != 2 && p->f_type != 3) return 0; /* 3: no mean forcing */
+  if (p->f_type < 0 || p->f_type > 3) return 0; /* 3: no mean forcing */
   for (int s = 0; s < 4; ++s) if (p->mudpack_bc[s] < 0 || p->mudpack_bc[s] > 2) return 0;
   if ((p->mudpack_bc[0] == 0) != (p->mudpack_bc[1] == 0)) return 0;
   if ((p->mudpack_bc[2] == 0) != (p->mudpack_bc[3] == 0)) return 0;
@@ -1561,6 +1565,8 @@ ora* ora_create(const ekm_params* p, const double* xpoint, const double* ypoint,
   }
   o->create_new_flow = 1;
   o->first_time = 1;
+  o->px0 = p->px0;
+  o->qn = p->wbulk0;
   init_ranges(o);
   o->delta_t = p->delta_t;
   o->delta_t_set = p->delta_t;
@@ -2417,6 +2423,12 @@ static void les_chan_th(ora* o, int cur_step, int rk) {
     }
 }
 
+int ora_get_forcing(ora* o, double* px0, double* qn) {
+  if (px0) *px0 = o->p.f_type == 0 ? o->px0 : o->p.px0;
+  if (qn) *qn = o->qn;
+  return EKM_OK;
+}
+
 int ora_get_les_th(ora* o, double* kappa, double* cdyn_th) {
   if (!o->p.les_model || !o->p.n_th) return EKM_ERR_ARG;
   if (kappa) memcpy(kappa, o->KAPT, o->npts * sizeof(double));
@@ -2475,8 +2487,20 @@ int ora_rk_duct(ora* o) {
           o->R[2][q + c] = o->R[2][q + c] - TEMP4 * o->CS1[q + c];
         }
       }
-  if (p->f_type == 1 || p->f_type == 2) { /* rank 0 owns mode 0 (:411-455) */
-    double px = p->f_type == 1 ? p->px0 : (p->px0 + p->amp_omega0 * cos(p->omega0 * o->time));
+  if (p->f_type == 0) { /* constant flow rate (:414-436) */
+    double qn1 = 0.0;
+    for (int j = js; j <= je - 1; ++j)
+      for (int k = zs + 1; k <= ze - 1; ++k)
+        qn1 = qn1 + (CU3[CRE(o, 0, k, j + 1)] + CU3[CRE(o, 0, k, j)]) * (o->yp[PL(o, k, j + 1)] - o->yp[PL(o, k, j)]) / 2;
+    qn1 = qn1 / (o->yp[PL(o, 1, je)] - o->yp[PL(o, 1, js)]) / (double)(float)(ze - zs - 1);
+    /* WEIGHT = 0.1 is assigned from a default-real literal (src/diablo.F90:361) */
+    const double weight = (double)0.1f, q0 = p->wbulk0;
+    o->px0 = o->px0 - (weight / TEMP4) * (o->qn + q0 - 2 * qn1);
+    o->qn = qn1;
+  }
+  if (p->f_type == 0 || p->f_type == 1 || p->f_type == 2) { /* rank 0 owns mode 0 (:411-455) */
+    double px = p->f_type == 0 ? o->px0
+              : p->f_type == 1 ? p->px0 : (p->px0 + p->amp_omega0 * cos(p->omega0 * o->time));
     for (int j = js; j <= je; ++j)
       for (int k = zs; k <= ze; ++k) {
         size_t q = CRE(o, 0, k, j);

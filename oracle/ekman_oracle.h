@@ -68,6 +68,8 @@ int ora_rk_duct(ora* o);
 int ora_get_les(ora* o, double* nut, double* cdyn);
 /* the scalar model: KAPPA_T (physical P field) and C_DYN_TH ((NZ+2)*(NY+2)) */
 int ora_get_les_th(ora* o, double* kappa, double* cdyn_th);
+/* the F_TYPE 0 controller: current PX0 and the last flow rate Qn */
+int ora_get_forcing(ora* o, double* px0, double* qn);
 /* one field's ADI (comp 0..2 velocity, 3 scalar) on physical U, R in place,
  * with the subgrid viscosity nut (physical P field) unless NULL */
 int ora_adi_field(ora* o, int comp, double* U, double* R, double t1, const double* nut, double t2);

@@ -49,6 +49,7 @@ def lib():
         L.ora_get_state.argtypes = [ctypes.c_void_p, _D, _D, _D, _D, _D]
         L.ora_get_les.argtypes = [ctypes.c_void_p, _D, _D]
         L.ora_get_les_th.argtypes = [ctypes.c_void_p, _D, _D]
+        L.ora_get_forcing.argtypes = [ctypes.c_void_p, _D, _D]
         L.ora_adi_field.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.c_double, _D, ctypes.c_double]
         L.ora_set_initial_physical.argtypes = [ctypes.c_void_p, _D, _D, _D, _D]
         L.ora_get_phi.argtypes = [ctypes.c_void_p, _D]
@@ -139,6 +140,12 @@ class Oracle:
         if st != 0:
             raise RuntimeError(f"ora_get_les: {st}")
         return nut, cdyn
+
+    def forcing(self):
+        """(PX0, Qn) of the F_TYPE 0 flow-rate controller."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        lib().ora_get_forcing(self.h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
 
     def get_les_th(self):
         """(KAPPA_T physical (NY+2, NZ+2, NX+2), C_DYN_TH (NY+2, NZ+2))."""

@@ -275,3 +275,29 @@ def test_les_scalar_matches_oracle(skew, oracle_lib):
         assert rel(ct_g, ct_o) < 1e-10 and rel(kap_g, kap_o) < 1e-10
     assert ct_o.max() > 0.0
     s.close()
+
+
+def test_flow_rate_controller_matches_oracle(oracle_lib):
+    """F_TYPE 0 (constant flow rate, src/duct.F90:414-436) on the GPU against
+    the oracle: config 1 with WBULK0 = 0.3, from the projected initial state,
+    six substeps; fields to 1e-10 relative L2 and the controller's PX0 and Qn
+    to 1e-12 (the flow-rate sum is reduced in another order on the GPU)."""
+    O = oracle_lib
+    cfg, p, xp, yp = _setup(1)
+    p.f_type, p.wbulk0, p.px0 = 0, 0.3, 0.01
+    ic = ekman_initial(p, yp, cfg.seed, 0.05, cfg.th_noise)
+    o = O.Oracle(p, xp, yp)
+    s = S.DuctSolver(p, xp, yp)
+    o.set_initial_physical(*ic)
+    s.set_initial_physical(*ic)
+    for sub in range(6):
+        o.rk_duct()
+        s.rk_duct()
+        r = _state_rel(s.get_state(), o.get_state())
+        (pg, qg), (po, qo) = s.forcing(), o.forcing()
+        print(f"[f_type 0] substep {sub}: fields {max(r):.2e}, px0 {pg:.10e} vs {po:.10e}, qn {qg:.10e} vs {qo:.10e}",
+              flush=True)
+        assert max(r) < 1e-10
+        assert abs(pg - po) <= 1e-12 * max(1.0, abs(po)) and abs(qg - qo) <= 1e-12 * max(1.0, abs(qo))
+    assert po != 0.01
+    s.close()

@@ -669,3 +669,33 @@ def test_les_th_coefficient_invariant_under_scalar_shift(oracle_lib):
     assert c1.max() > 0.0
     assert np.abs(c1 - c2).max() <= 1e-8 * c1.max()
     assert np.abs(k1 - k2).max() <= 1e-8 * k1.max()
+
+
+def test_flow_rate_controller_closed_form(oracle_lib):
+    """F_TYPE 0 (src/duct.F90:414-436, constants src/diablo.F90:359-361): with
+    a uniform mean W = c on a grid whose y lines do not depend on z, the flow
+    rate integral is c, so one substep moves the pressure gradient by
+    -(WEIGHT/TEMP4)(Qn + Q0 - 2c), WEIGHT the single-precision 0.1 the
+    reference assigns, TEMP4 = H_BAR(1) = 8/15 DELTA_T, Qn = Q0 = WBULK0."""
+    from oracle import oracle as O
+    cfg = config(1)
+    p = cfg.params
+    p.f_type, p.wbulk0, p.px0, p.variable_dt = 0, 0.3, 0.01, 0
+    xp, yp = tanh_grid(p.ny, p.nz, p.ly, p.lz, cfg.gamma)
+    o = O.Oracle(p, xp, yp)
+    z = np.zeros(p.shape_fourier, dtype=np.complex128)
+    cu3 = z.copy()
+    c = 0.5
+    cu3[..., 0] = c
+    o.set_state(z, z, cu3, z, z, time=0.0, time_step=0)
+    assert o.forcing() == (0.01, 0.3)
+    o.rk_duct()
+    px0, qn = o.forcing()
+    t4 = p.delta_t * (8.0 / 15.0)
+    w = float(np.float32(0.1))
+    assert abs(qn - c) <= 1e-14
+    assert abs(px0 - (0.01 - (w / t4) * (0.3 + 0.3 - 2 * c))) <= 1e-13
+    # the gradient acts on the mean mode: a second substep keeps integrating
+    o.rk_duct()
+    px1, qn1 = o.forcing()
+    assert px1 != px0 and abs(qn1 - c) < 0.5
