@@ -159,7 +159,7 @@ def main() -> int:
                     help="run the plain copy kernel on this many MiB once (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the duct solver runs on one GPU; --gpus {args.gpus} is not supported "
               "(the duct is not sharded across GPUs)", file=sys.stderr)
         return 2
     cfg = config(args.config)

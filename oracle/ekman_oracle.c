@@ -425,8 +425,9 @@ int ora_get_metric(ora* o, int which, double* out) {
 
 /* Boundary callback.  The reference bndcr (src/mg_solver2d.F90:206-255)
  * implements kbdy=3,4 (y boundaries) as dphi/dy = 0: alfa=0, beta=1, gama=0,
- * gbdy=0.  kbdy=1,2 (mixed at the z ends) is this build's generalisation for
- * free-slip z ends: dphi/dz = 0, alfa=1, beta=0, gama=0, gbdy=0. */
+ * gbdy=0.  kbdy=1,2 (mixed at the z ends, which the reference leaves out) are
+ * generalised here for free-slip z ends: dphi/dz = 0, alfa=1, beta=0, gama=0,
+ * gbdy=0. */
 static void bndyc(int kbdy, double xory, double* alfa, double* beta, double* gama,
                   double* gbdy) {
   (void)xory;
