@@ -301,3 +301,64 @@ def test_flow_rate_controller_matches_oracle(oracle_lib):
         assert abs(pg - po) <= 1e-12 * max(1.0, abs(po)) and abs(qg - qo) <= 1e-12 * max(1.0, abs(qo))
     assert po != 0.01
     s.close()
+
+
+def _small_stratified():
+    """Config 2's physics on 32 x 33 x 33 points."""
+    from ekman_bl_amd.params import mg_factors
+    cfg = config(2)
+    p = cfg.params
+    p.nx, p.ny, p.nz = 32, 31, 31
+    p.mg_ixp, p.mg_iex = mg_factors(p.nz + 2)
+    p.mg_jyq, p.mg_jey = mg_factors(p.ny + 2)
+    return cfg, p
+
+
+@pytest.mark.parametrize("case", ["shipped", "theta_zmax_sp", "dirichlet_z"])
+def test_boundary_rows_match_oracle(case, oracle_lib):
+    """The boundary-row types the configurations do not use, on the GPU
+    against the oracle (APPLY_BC_*, src/boundary.F90:124-1760), 2 steps from
+    the projected initial state of a small stratified case:
+    - shipped: the reference run's set (rundir_unst/input.dat): lagged
+      periodic z rows (type 6) for the velocity and the scalar, scalar bottom
+      type 5, periodic multigrid z ends (MUDPACK_BC 0, 0) with the plain
+      projection (the line-smoothed one needs non-periodic ends);
+    - theta_zmax_sp: scalar z-max type 5, the THOMAS_REAL_SP elimination
+      (src/boundary.F90:4-47);
+    - dirichlet_z: Dirichlet z ends for all fields (ZSTART = 2, the C1 rows on
+      two points).
+    The rows live in the ADI, so every case uses the plain projection, which
+    the oracle restates exactly; fields agree to 1e-10 relative L2."""
+    from ekman_bl_amd.params import YMIN, ZMIN, ZMAX
+    O = oracle_lib
+    cfg, p = _small_stratified()
+    if case == "shipped":
+        for c in range(3):
+            p.u_bc[c][ZMIN] = p.u_bc[c][ZMAX] = 6
+        p.th_bc[YMIN] = 5
+        p.th_bc[ZMIN] = p.th_bc[ZMAX] = 6
+        p.mudpack_bc[0] = p.mudpack_bc[1] = 0
+        p.proj_mode = 0
+    elif case == "theta_zmax_sp":
+        p.th_bc[ZMAX] = 5
+    else:
+        for c in range(3):
+            p.u_bc[c][ZMIN] = p.u_bc[c][ZMAX] = 0
+            p.u_bc_c1[c][ZMIN] = p.u_bc_c1[c][ZMAX] = 0.0
+        p.u_bc_c1[0][ZMIN] = 0.2
+        p.th_bc[ZMIN] = p.th_bc[ZMAX] = 0
+        p.th_bc_c1[ZMIN], p.th_bc_c1[ZMAX] = 0.01, -0.01
+    p.proj_mode = 0
+    xp, yp = tanh_grid(p.ny, p.nz, p.ly, p.lz, cfg.gamma)
+    ic = ekman_initial(p, yp, cfg.seed, cfg.noise, cfg.th_noise)
+    o = O.Oracle(p, xp, yp)
+    s = S.DuctSolver(p, xp, yp)
+    o.set_initial_physical(*ic)
+    s.set_initial_physical(*ic)
+    for sub in range(6):
+        assert o.rk_duct() == 0
+        s.rk_duct()
+        r = _state_rel(s.get_state(), o.get_state())
+        print(f"[bc {case}] substep {sub}: {max(r):.2e}", flush=True)
+        assert max(r) < 1e-10
+    s.close()
