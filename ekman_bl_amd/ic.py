@@ -12,6 +12,11 @@ src/duct.F90:471-473): u = G (1 - e^-eta cos eta), w = -G e^-eta sin eta,
 eta = y / delta, delta = sqrt(2 nu / f); plus uniform noise of amplitude
 `noise` * G in [-1/2, 1/2).  Scalar: zero perturbation plus `th_noise` noise
 (the background TH_BAR carries the stratification).
+
+reference_ic7 gives the form of the reference's own IC_TYPE 7 start
+(CREATE_FLOW_DUCT / CREATE_TH_DUCT): uniform W = WBULK0 with damped Fourier
+kicks, scalar zero, with the same counter-based noise in place of
+RANDOM_NUMBER.
 """
 from __future__ import annotations
 
@@ -59,3 +64,61 @@ def ekman_initial(p, ypoint: np.ndarray, seed: int, noise: float, th_noise: floa
     for f in (u1, u2, u3, th):
         f[:, :, p.nx:] = 0.0
     return u1, u2, u3, th
+
+
+def index_ranges(p):
+    """(JSTART, JEND, ZSTART, ZEND) of INIT_DUCT for the velocity (src/duct.F90:
+    27-129, IC_TYPE 7 forms): from U3's y boundary types and U1's z types."""
+    wy0, wy1 = p.u_bc[2][0], p.u_bc[2][1]
+    uz0, uz1 = p.u_bc[0][2], p.u_bc[0][3]
+    js = 1 if wy0 in (0, 1, 3, 6, 7) else 2
+    je = p.ny if wy1 in (0, 1, 6) else p.ny - 1
+    zs = 1 if uz0 in (1, 6) else 2
+    ze = p.nz if uz1 in (0, 1, 6) else p.nz - 1
+    return js, je, zs, ze
+
+
+def reference_ic7_fourier(p, kick: float, seed: int):
+    """The form of CREATE_FLOW_DUCT / CREATE_TH_DUCT for IC_TYPE 7 (src/duct.F90:
+    3026-3137, 2854-2864) in Fourier space, (NY+2, NZ+2, NX/2+1) complex:
+    U1 = U2 = 0, U3 = WBULK0 (the mean mode), TH = 0; then for every mode
+    I = 1..NX/2 and interior J, K, uniform kicks (RNUM - 1/2) KICK added to the
+    real part of each velocity component (a REAL added to a COMPLEX), and the
+    three components multiplied by DAMP_FACT = exp(-6 J^2 / NY^2).  The kicks
+    come from the splitmix64 stream (seed, component), visited in the
+    reference's I, J, K order, in place of RANDOM_NUMBER."""
+    ny2, nz2, nxc = p.ny + 2, p.nz + 2, p.nx // 2 + 1
+    cu = [np.zeros((ny2, nz2, nxc), dtype=np.complex128) for _ in range(3)]
+    cu[2][..., 0] = p.wbulk0
+    js, je, zs, ze = index_ranges(p)
+    nj, nk, ni = je - js + 1, ze - zs + 1, nxc - 1
+    n = ni * nj * nk
+    if n > 0 and kick != 0.0:
+        j = np.arange(js, je + 1)
+        damp = np.exp(-6.0 * (j.astype(np.float64) ** 2) / float(p.ny ** 2))
+        for c in range(3):
+            r = uniform01(seed, 10 + c, n).reshape(ni, nj, nk) - 0.5   # I outer, then J, then K
+            block = cu[c][js:je + 1, zs:ze + 1, 1:]
+            block.real += np.transpose(r, (1, 2, 0)) * kick
+            block *= damp[:, None, None]
+    return cu[0], cu[1], cu[2], np.zeros_like(cu[0]), np.zeros_like(cu[0])
+
+
+def fourier_to_physical(p, cf: np.ndarray) -> np.ndarray:
+    """Inverse x transform in the library's convention (forward scaled 1/NX,
+    modes above NX/3 dropped), padded to NX+2: (NY+2, NZ+2, NX+2)."""
+    c = cf.copy()
+    c[..., p.nx // 3 + 1:] = 0.0
+    u = np.fft.irfft(c, n=p.nx, axis=-1) * p.nx
+    out = np.zeros(u.shape[:-1] + (p.nx + 2,))
+    out[..., :p.nx] = u
+    return out
+
+
+def reference_ic7(p, kick: float, seed: int):
+    """Physical (u1, u2, u3, th) of reference_ic7_fourier, for
+    set_initial_physical (which then projects, as CREATE_FLOW_DUCT's
+    REM_DIV_CURV does).  Modes above NX/3 carry kicks in the reference until
+    its first forward transform drops them; here they are dropped at once."""
+    f = reference_ic7_fourier(p, kick, seed)
+    return tuple(fourier_to_physical(p, a) for a in (f[0], f[1], f[2], f[4]))

@@ -699,3 +699,37 @@ def test_flow_rate_controller_closed_form(oracle_lib):
     o.rk_duct()
     px1, qn1 = o.forcing()
     assert px1 != px0 and abs(qn1 - c) < 0.5
+
+
+def test_reference_ic7_form():
+    """IC_TYPE 7 (src/duct.F90:3026-3137, scalar :2854-2864): U3 = WBULK0 in
+    the mean mode everywhere, U1 = U2 = TH = 0 there; the kicks only touch the
+    real parts of modes 1..NX/2 at the interior points, within
+    +-KICK/2 * exp(-6 J^2/NY^2), and fill that range; the physical fields
+    transform back to the same modes (forward scaled 1/NX, modes <= NX/3)."""
+    from ekman_bl_amd import ic
+    p = config(1).params
+    p.wbulk0 = 0.7
+    kick = 0.1
+    cu1, cu2, cu3, cp, cth = ic.reference_ic7_fourier(p, kick, 3)
+    assert np.all(cu3[..., 0] == 0.7) and np.all(cu1[..., 0] == 0) and np.all(cu2[..., 0] == 0)
+    assert not cp.any() and not cth.any()
+    js, je, zs, ze = ic.index_ranges(p)
+    assert (js, je, zs, ze) == (1, p.ny, 1, p.nz)   # config 1: W no-slip bottom, Neumann z ends
+    for c in (cu1, cu2, cu3):
+        kicks = c[..., 1:]
+        assert not kicks.imag.any()
+        inner = np.zeros(kicks.shape, bool)
+        inner[js:je + 1, zs:ze + 1, :] = True
+        assert not kicks[~inner].any()
+        j = np.arange(p.ny + 2)[:, None, None]
+        bound = 0.5 * kick * np.exp(-6.0 * j ** 2 / p.ny ** 2)
+        assert np.all(np.abs(kicks.real) <= bound + 1e-15)
+        # uniform draws: the damped amplitude fills its range near the wall
+        assert np.abs(kicks.real[1]).max() > 0.9 * bound[1, 0, 0]
+    u1, u2, u3, th = ic.reference_ic7(p, kick, 3)
+    back = np.fft.rfft(u3[..., :p.nx], axis=-1) / p.nx
+    m = p.nx // 3 + 1
+    assert np.abs(back[..., :m] - cu3[..., :m]).max() < 1e-14
+    assert np.abs(back[..., m:]).max() < 1e-14
+    assert not th.any()
