@@ -62,6 +62,9 @@ int launch_project(const Dims& d, const Metrics& M, const Ranges& r, const doubl
                    hipStream_t st);
 int launch_pupdate(const Dims& d, const Ranges& r, const double* PHI, double* CP, double temp4, hipStream_t st);
 int launch_div_reduce(const Dims& d, const double* S, double* out, hipStream_t st);
+// x statistics (16 planes, see k_xstats) of the Fourier state; TH may be null
+int launch_xstats(const Dims& d, const double* U1, const double* U2, const double* U3, const double* P,
+                  const double* TH, double* out, hipStream_t st);
 // flow-rate integral of the F_TYPE 0 controller into out[0]
 int launch_flow_rate(const Dims& d, const Ranges& r, const double* CU3, const double* dy, double* out,
                      hipStream_t st);

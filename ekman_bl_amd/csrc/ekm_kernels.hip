@@ -889,6 +889,58 @@ __global__ void k_pupdate(Dims d, Ranges r, const double2* __restrict__ P, doubl
   CP[q] = c;
 }
 
+// x statistics of SAVE_STATS_CURVI (src/flow_statistics.F90:163-215, :420-495)
+// at every (K,J), from the Fourier state by Parseval: with the forward
+// transform scaled by 1/NX and the modes above NX/3 zero,
+//   sum_i f'_i g'_i = 2 NX sum_{m>=1} Re(f_m conj g_m)   (i over the NX points).
+// One wavefront per (K,J) pencil, lanes over the modes.  Planes of out
+// ((NZ+2)*(NY+2) each): 0-4 the means of U1, U2, U3, P, TH (the real parts of
+// mode 0); 5-8 the reference's urms, vrms, wrms, thrms (un-normalised sums of
+// squared fluctuations); 9-15 uv, uw, wv, pu (= w'p'), pv, thv, thw (sums / NX).
+constexpr int kXStats = 16;
+__global__ void __launch_bounds__(256) k_xstats(Dims d, const double2* __restrict__ U1,
+                                                const double2* __restrict__ U2, const double2* __restrict__ U3,
+                                                const double2* __restrict__ P, const double2* __restrict__ TH,
+                                                double* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const size_t pen = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (pen >= d.plane) return;
+  const size_t f0 = pen * d.NXC;
+  double a[11];
+  for (int q = 0; q < 11; ++q) a[q] = 0.0;
+  for (int m = 1 + lane; m < d.NXC; m += 64) {
+    const double2 u = U1[f0 + m], v = U2[f0 + m], w = U3[f0 + m], p = P[f0 + m];
+    const double2 t = TH ? TH[f0 + m] : make_double2(0.0, 0.0);
+    // the Nyquist mode has no conjugate partner
+    const double wgt = (2 * m == d.NX) ? 1.0 : 2.0;
+    auto re = [](double2 x, double2 y) { return x.x * y.x + x.y * y.y; };
+    a[0] += wgt * re(u, u);
+    a[1] += wgt * re(v, v);
+    a[2] += wgt * re(w, w);
+    a[3] += wgt * re(t, t);
+    a[4] += wgt * re(u, v);
+    a[5] += wgt * re(u, w);
+    a[6] += wgt * re(w, v);
+    a[7] += wgt * re(w, p);
+    a[8] += wgt * re(v, p);
+    a[9] += wgt * re(t, v);
+    a[10] += wgt * re(t, w);
+  }
+  for (int o = 32; o > 0; o >>= 1)
+    for (int q = 0; q < 11; ++q) a[q] += __shfl_xor(a[q], o, 64);
+  if (lane == 0) {
+    const size_t pl = d.plane;
+    const double nx = (double)d.NX;
+    out[0 * pl + pen] = U1[f0].x;
+    out[1 * pl + pen] = U2[f0].x;
+    out[2 * pl + pen] = U3[f0].x;
+    out[3 * pl + pen] = P[f0].x;
+    out[4 * pl + pen] = TH ? TH[f0].x : 0.0;
+    for (int q = 0; q < 4; ++q) out[(5 + q) * pl + pen] = nx * a[q];
+    for (int q = 4; q < 11; ++q) out[(5 + q) * pl + pen] = a[q];
+  }
+}
+
 // flow rate of the F_TYPE 0 controller (:416-428): the sum over J = JSTART
 // .. JEND-1, K = ZSTART+1 .. ZEND-1 of (W0(K,J+1) + W0(K,J)) dy(K,J) / 2 with
 // W0 the real part of the mean (kx = 0) mode of U3 and dy(K,J) =
@@ -1089,6 +1141,13 @@ int launch_flow_rate(const Dims& d, const Ranges& r, const double* CU3, const do
                      hipStream_t st) {
   if (r.ze - r.zs - 1 < 1 || r.je - r.js < 1) return EKM_ERR_UNSUPPORTED;
   hipLaunchKernelGGL(k_flow_rate, dim3(1), dim3(256), 0, st, d, r, (const double2*)CU3, dy, out);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_xstats(const Dims& d, const double* U1, const double* U2, const double* U3, const double* P,
+                  const double* TH, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_xstats, dim3(nblk(d.plane, 4)), dim3(256), 0, st, d, (const double2*)U1, (const double2*)U2,
+                     (const double2*)U3, (const double2*)P, (const double2*)TH, out);
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }

@@ -443,6 +443,18 @@ extern "C" int ekm_set_state(ekm_solver* s, const double* cu1, const double* cu2
 
 // NU_T (physical, the layout of a field) and C_DYN ((NZ+2) x (NY+2), K
 // fastest) of the subgrid model (src/modules.F90:431-483 les_chan_var)
+extern "C" int ekm_get_xstats(ekm_solver* s, double* out) {
+  if (!out) return EKM_ERR_ARG;
+  const size_t n = 16 * s->d.plane * sizeof(double);
+  double* dev = nullptr;
+  EKM_CHECK(hipMalloc((void**)&dev, n));
+  int st = launch_xstats(s->d, s->U[0], s->U[1], s->U[2], s->P, s->p.n_th ? s->TH : nullptr, dev, s->st);
+  if (st == EKM_OK && hipMemcpyAsync(out, dev, n, hipMemcpyDeviceToHost, s->st) != hipSuccess) st = EKM_ERR_HIP;
+  if (hipStreamSynchronize(s->st) != hipSuccess) st = EKM_ERR_HIP;
+  (void)hipFree(dev);
+  return st;
+}
+
 extern "C" int ekm_get_forcing(ekm_solver* s, double* px0, double* qn) {
   if (px0) *px0 = s->p.f_type == 0 ? s->px0 : s->p.px0;
   if (qn) *qn = s->qn;

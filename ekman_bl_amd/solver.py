@@ -68,6 +68,7 @@ def lib():
         L.ekm_get_les_th.argtypes = [vp, _D, _D]
         L.ekm_get_solve_cycles.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
         L.ekm_get_forcing.argtypes = [vp, _D, _D]
+        L.ekm_get_xstats.argtypes = [vp, _D]
         L.ekm_set_initial_physical.argtypes = [vp, _D, _D, _D, _D]
         L.ekm_get_phi.argtypes = [vp, _D]
         L.ekm_get_diag.argtypes = [vp, ctypes.POINTER(EkmDiag)]
@@ -170,6 +171,15 @@ class DuctSolver:
         cdyn = np.zeros((p.ny + 2, p.nz + 2))
         _check(lib().ekm_get_les(self.h, _p(nut), _p(cdyn)), "ekm_get_les")
         return nut, cdyn
+
+    XSTATS = ("u1", "u2", "u3", "p", "th", "urms", "vrms", "wrms", "thrms", "uv", "uw", "wv", "pu", "pv", "thv", "thw")
+
+    def x_stats(self) -> dict:
+        """x statistics of the current state (ekm_get_xstats): name -> (NY+2, NZ+2)."""
+        p = self.p
+        out = np.zeros((16, p.ny + 2, p.nz + 2))
+        _check(lib().ekm_get_xstats(self.h, _p(out)), "ekm_get_xstats")
+        return dict(zip(self.XSTATS, out))
 
     def forcing(self):
         """(PX0, Qn) of the F_TYPE 0 flow-rate controller."""

@@ -171,6 +171,16 @@ int ekm_get_solve_cycles(ekm_solver* s, int* cycles, int n);
  * src/duct.F90:414-436; both start from PX0 and Q0 = WBULK0,
  * src/diablo.F90:359-361); for F_TYPE 1, 2, 3 px0 is the fixed PX0. */
 int ekm_get_forcing(ekm_solver* s, double* px0, double* qn);
+/* x statistics of the current state, the moments of SAVE_STATS_CURVI
+ * (src/flow_statistics.F90:163-215, :420-495), computed on the device from the
+ * Fourier fields: 16 planes of (NZ+2)*(NY+2) (K fastest) for every K, J:
+ * 0-4 the x means of U1, U2, U3, P, TH; 5-8 urms, vrms, wrms, thrms as the
+ * reference forms them (sums over x of the squared fluctuations, not divided
+ * by NX); 9-15 uv, uw, wv, pu (= w'p'), pv, thv, thw (sums over x divided by
+ * NX).  The sums run over the NX physical points (the reference's loops also
+ * visit the two padding entries of the physical array, whose contents its FFT
+ * leaves undefined). */
+int ekm_get_xstats(ekm_solver* s, double* out);
 /* Blocks until queued work on the solver's stream is done. */
 int ekm_sync(ekm_solver* s);
 /* Device pointer of state field f (0=U1,1=U2,2=U3,3=P,4=TH); the field's

@@ -2424,6 +2424,44 @@ static void les_chan_th(ora* o, int cur_step, int rk) {
     }
 }
 
+/* x statistics of SAVE_STATS_CURVI (src/flow_statistics.F90:163-215,
+ * :420-495) by direct sums over the NX physical points of copies of the
+ * fields transformed to physical space; same 16-plane layout as the library's
+ * ekm_get_xstats (means of U1, U2, U3, P, TH; urms, vrms, wrms, thrms as sums
+ * of squared fluctuations; uv, uw, wv, pu = w'p', pv, thv, thw as sums / NX). */
+int ora_xstats(ora* o, double* out) {
+  const size_t np = o->npts, pl = (size_t)o->NZ2 * o->NY2;
+  double* F[5];
+  const double* src[5] = {o->U[0], o->U[1], o->U[2], o->P, o->p.n_th ? o->TH : NULL};
+  for (int q = 0; q < 5; ++q) {
+    F[q] = dalloc(np);
+    if (src[q]) memcpy(F[q], src[q], np * sizeof(double));
+    else memset(F[q], 0, np * sizeof(double));
+    inv(o, F[q]);
+  }
+  const int NX = o->NX;
+  for (int j = 0; j < o->NY2; ++j)
+    for (int k = 0; k < o->NZ2; ++k) {
+      const size_t t = PL(o, k, j);
+      double mean[5];
+      for (int q = 0; q < 5; ++q) mean[q] = src[q] ? src[q][CRE(o, 0, k, j)] : 0.0; /* dble(CU(0,K,J)) */
+      double s[11] = {0};
+      for (int i = 0; i < NX; ++i) {
+        const size_t q = IDX(o, i, k, j);
+        const double u = F[0][q] - mean[0], v = F[1][q] - mean[1], w = F[2][q] - mean[2];
+        const double p = F[3][q] - mean[3], th = F[4][q] - mean[4];
+        s[0] += u * u; s[1] += v * v; s[2] += w * w; s[3] += th * th;
+        s[4] += u * v; s[5] += u * w; s[6] += w * v; s[7] += w * p; s[8] += v * p;
+        s[9] += th * v; s[10] += th * w;
+      }
+      for (int q = 0; q < 5; ++q) out[q * pl + t] = mean[q];
+      for (int q = 0; q < 4; ++q) out[(5 + q) * pl + t] = s[q];
+      for (int q = 4; q < 11; ++q) out[(5 + q) * pl + t] = s[q] / (double)NX;
+    }
+  for (int q = 0; q < 5; ++q) free(F[q]);
+  return EKM_OK;
+}
+
 int ora_get_forcing(ora* o, double* px0, double* qn) {
   if (px0) *px0 = o->p.f_type == 0 ? o->px0 : o->p.px0;
   if (qn) *qn = o->qn;

@@ -70,6 +70,8 @@ int ora_get_les(ora* o, double* nut, double* cdyn);
 int ora_get_les_th(ora* o, double* kappa, double* cdyn_th);
 /* the F_TYPE 0 controller: current PX0 and the last flow rate Qn */
 int ora_get_forcing(ora* o, double* px0, double* qn);
+/* x statistics (16 planes of (NZ+2)*(NY+2)), direct sums in physical space */
+int ora_xstats(ora* o, double* out);
 /* one field's ADI (comp 0..2 velocity, 3 scalar) on physical U, R in place,
  * with the subgrid viscosity nut (physical P field) unless NULL */
 int ora_adi_field(ora* o, int comp, double* U, double* R, double t1, const double* nut, double t2);

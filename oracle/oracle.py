@@ -50,6 +50,7 @@ def lib():
         L.ora_get_les.argtypes = [ctypes.c_void_p, _D, _D]
         L.ora_get_les_th.argtypes = [ctypes.c_void_p, _D, _D]
         L.ora_get_forcing.argtypes = [ctypes.c_void_p, _D, _D]
+        L.ora_xstats.argtypes = [ctypes.c_void_p, _D]
         L.ora_adi_field.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.c_double, _D, ctypes.c_double]
         L.ora_set_initial_physical.argtypes = [ctypes.c_void_p, _D, _D, _D, _D]
         L.ora_get_phi.argtypes = [ctypes.c_void_p, _D]
@@ -140,6 +141,15 @@ class Oracle:
         if st != 0:
             raise RuntimeError(f"ora_get_les: {st}")
         return nut, cdyn
+
+    XSTATS = ("u1", "u2", "u3", "p", "th", "urms", "vrms", "wrms", "thrms", "uv", "uw", "wv", "pu", "pv", "thv", "thw")
+
+    def x_stats(self) -> dict:
+        """x statistics by direct sums in physical space: name -> (NY+2, NZ+2)."""
+        p = self.p
+        out = np.zeros((16, p.ny + 2, p.nz + 2))
+        lib().ora_xstats(self.h, _p(out))
+        return dict(zip(self.XSTATS, out))
 
     def forcing(self):
         """(PX0, Qn) of the F_TYPE 0 flow-rate controller."""

@@ -362,3 +362,28 @@ def test_boundary_rows_match_oracle(case, oracle_lib):
         print(f"[bc {case}] substep {sub}: {max(r):.2e}", flush=True)
         assert max(r) < 1e-10
     s.close()
+
+
+def test_x_statistics_match_oracle(oracle_lib):
+    """The x statistics of SAVE_STATS_CURVI (ekm_get_xstats): the GPU forms
+    them from the Fourier state by Parseval, the oracle by direct sums over
+    the physical points; after one step of the small stratified case every
+    plane agrees to 1e-10 relative (max norm over the plane's scale)."""
+    O = oracle_lib
+    cfg, p = _small_stratified()
+    xp, yp = tanh_grid(p.ny, p.nz, p.ly, p.lz, cfg.gamma)
+    ic = ekman_initial(p, yp, cfg.seed, cfg.noise, cfg.th_noise)
+    o = O.Oracle(p, xp, yp)
+    s = S.DuctSolver(p, xp, yp)
+    o.set_initial_physical(*ic)
+    s.set_initial_physical(*ic)
+    o.time_step(1)
+    s.time_step(1)
+    sg, so = s.x_stats(), o.x_stats()
+    for name in S.DuctSolver.XSTATS:
+        scale = np.abs(so[name]).max()
+        err = np.abs(sg[name] - so[name]).max()
+        print(f"[xstats] {name}: scale {scale:.3e} err {err:.2e}", flush=True)
+        assert err <= 1e-10 * max(scale, 1e-300), name
+    assert np.abs(so["urms"]).max() > 0 and np.abs(so["thv"]).max() > 0
+    s.close()

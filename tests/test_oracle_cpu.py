@@ -733,3 +733,29 @@ def test_reference_ic7_form():
     assert np.abs(back[..., :m] - cu3[..., :m]).max() < 1e-14
     assert np.abs(back[..., m:]).max() < 1e-14
     assert not th.any()
+
+
+def test_x_statistics_single_mode_closed_form(oracle_lib):
+    """SAVE_STATS_CURVI's x moments (src/flow_statistics.F90:163-215, 420-495)
+    as the oracle forms them (direct sums over the NX physical points): with
+    mode 1 of U1 equal to a and of U2 equal to b (u = 2a cos x, v = 2|b|
+    cos(x + arg b)) and means c, d, the sums are urms = 2 NX a^2, vrms =
+    2 NX |b|^2, uv = 2 Re(a conj b), wrms = uw = 0, and the means are c, d."""
+    from oracle import oracle as O
+    cfg = config(1)
+    p = cfg.params
+    xp, yp = tanh_grid(p.ny, p.nz, p.ly, p.lz, cfg.gamma)
+    o = O.Oracle(p, xp, yp)
+    z = np.zeros(p.shape_fourier, dtype=np.complex128)
+    a, b, c, d = 0.3, 0.2 - 0.1j, 1.5, -0.25
+    cu1, cu2 = z.copy(), z.copy()
+    cu1[..., 1], cu2[..., 1] = a, b
+    cu1[..., 0], cu2[..., 0] = c, d
+    o.set_state(cu1, cu2, z, z, z, time=0.0, time_step=0)
+    st = o.x_stats()
+    nx = p.nx
+    assert np.allclose(st["u1"], c, rtol=0, atol=1e-15) and np.allclose(st["u2"], d, rtol=0, atol=1e-15)
+    assert np.allclose(st["urms"], 2 * nx * a * a, rtol=1e-13, atol=0)
+    assert np.allclose(st["vrms"], 2 * nx * abs(b) ** 2, rtol=1e-13, atol=0)
+    assert np.allclose(st["uv"], 2 * (a * np.conj(b)).real, rtol=1e-13, atol=0)
+    assert np.abs(st["wrms"]).max() == 0 and np.abs(st["uw"]).max() == 0
