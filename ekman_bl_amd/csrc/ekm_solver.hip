@@ -119,7 +119,6 @@ static int check_params(const ekm_params& p) {
   if ((p.mudpack_bc[2] == 0) != (p.mudpack_bc[3] == 0)) return EKM_ERR_ARG;
   if (p.proj_mode < 0 || p.proj_mode > 2) return EKM_ERR_UNSUPPORTED;
   if (p.les_model != 0 && p.les_model != 2) return EKM_ERR_UNSUPPORTED;  // dynamic Smagorinsky only
-  if (p.les_model && (p.nx > 4096)) return EKM_ERR_UNSUPPORTED;
   // the scalar's subgrid model with a non-uniform background (cont_strat 0)
   // adds KAPPA_T indexed by Fourier mode to the background diffusion
   // (src/duct.F90:610-618); that form is not supported

@@ -106,6 +106,28 @@ module ekman_bl_amd
        real(c_double), intent(out) :: kappa(*), cdyn_th(*)
      end function ekm_get_les_th
 
+     ! x statistics of SAVE_STATS_CURVI (16 planes of (NZ+2)*(NY+2))
+     integer(c_int) function ekm_get_xstats(s, out) bind(c, name="ekm_get_xstats")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: s
+       real(c_double), intent(out) :: out(*)
+     end function ekm_get_xstats
+
+     ! PX0 and Qn of the F_TYPE 0 flow-rate controller
+     integer(c_int) function ekm_get_forcing(s, px0, qn) bind(c, name="ekm_get_forcing")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: s
+       real(c_double), intent(out) :: px0, qn
+     end function ekm_get_forcing
+
+     ! cycles of every real solve of the last projection (MUDPACK's iparm(17))
+     integer(c_int) function ekm_get_solve_cycles(s, cycles, n) bind(c, name="ekm_get_solve_cycles")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: s
+       integer(c_int), intent(out) :: cycles(*)
+       integer(c_int), value :: n
+     end function ekm_get_solve_cycles
+
      integer(c_int) function ekm_set_initial_physical(s, u1, u2, u3, th) bind(c, name="ekm_set_initial_physical")
        import :: c_ptr, c_int, c_double
        type(c_ptr), value :: s
