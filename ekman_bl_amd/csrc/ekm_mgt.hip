@@ -46,8 +46,9 @@ struct TLev {
   int e_off, r_off, res_off;  // per-solve workspace offsets (elements of T)
   int t2_off;                 // second line-solve scratch (the residual array is the first)
   int lds;                    // 1: the four arrays above live in the workgroup's LDS
-  int f_off;                  // per-mode offset: z factors [3][n], y factors [3][n], c9 [n]
+  int f_off;                  // per-mode offset: z factors [3][n], y factors [3][n] (column-parity layout), c9 [n]
   const T* cf;                // c1..c8 planes [8][n]; rows of specified points are 0 (c9 = 1)
+  const T* cfy;               // the same planes in the column-parity layout of the y-line smoother
   const unsigned char* spec;  // specified (Dirichlet) points
 };
 
@@ -223,6 +224,18 @@ __device__ void zlines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
 // chunk's recurrences as affine maps of the unknown value entering the chunk
 // (stored per point in the two scratch arrays), the chunk maps are combined
 // through LDS, and a last pass applies the entering values.
+//
+// The line factors, the coefficient planes the smoother reads and the two
+// scratch arrays are kept in a column-parity layout: the columns of parity 0
+// (i = 2c) first, row by row (c + ncol0 j), then those of parity 1
+// (ncol0 ny + c + ncol1 j).  A wavefront's row load then covers whole cache
+// lines; in the natural layout (i + nx j) every load of one parity fetched
+// the lines of both.  The right-hand side and the correction keep the natural
+// layout the z lines and the transfers use.
+__host__ __device__ inline int ysplit(int i, int j, int nx, int ny) {
+  const int ncol0 = (nx + 1) >> 1;
+  return (i & 1) ? ncol0 * ny + (i >> 1) + (nx >> 1) * j : (i >> 1) + ncol0 * j;
+}
 constexpr int kYMaxTasks = 32;
 constexpr int kYUnroll = 3;
 constexpr int kYApply = 8;  // rows per batch of the last pass
@@ -237,11 +250,13 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
   const T* __restrict__ r = W + L.r_off;
   T* __restrict__ tA = W + L.res_off;
   T* __restrict__ tB = W + L.t2_off;
-  const T* __restrict__ cf = L.cf;
+  const T* __restrict__ cf = L.cfy;
   const T* f1 = fac + L.f_off + 3 * n;
   const T* fa = f1 + n;
   const T* fc = fa + n;
   const int ncol = (nx - par + 1) / 2;
+  // column-parity layout: this parity's block starts at pb, rows ncol apart
+  const int pb = par ? ((nx + 1) >> 1) * ny : 0;
   const int ngrp = (ncol + 63) / 64;
   int nch = nw / ngrp;
   if (nch < 1) nch = 1;
@@ -256,7 +271,8 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
   // forward: dp_j = A_j dp_in + B_j within the chunk
   for (int task = wave; task < ntask; task += nw) {
     const int grp = task / nch, ch = task - grp * nch;
-    const int i = par + 2 * (grp * 64 + lane);
+    const int col = grp * 64 + lane;
+    const int i = par + 2 * col;
     const int j0 = ch * cl, j1 = min(j0 + cl, ny);
     T A = (T)1, B = (T)0;
     if (i < nx) {
@@ -270,13 +286,14 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
           fv[u] = (T)0;
           if (j < j1) {
             const int p = i + nx * j;
+            const int ps = pb + col + ncol * j;
             const int q = (i + 1) + ld * (j + 1);
-            sv_[u] = r[p] - (cf[0 * n + p] * e[q + 1] + cf[4 * n + p] * e[q - 1]);
+            sv_[u] = r[p] - (cf[0 * n + ps] * e[q + 1] + cf[4 * n + ps] * e[q - 1]);
             if (X)
-              sv_[u] -= (cf[1 * n + p] * e[q + 1 + ld] + cf[3 * n + p] * e[q - 1 + ld]) +
-                        (cf[5 * n + p] * e[q - 1 - ld] + cf[7 * n + p] * e[q + 1 - ld]);
-            av[u] = fa[p];
-            fv[u] = f1[p];
+              sv_[u] -= (cf[1 * n + ps] * e[q + 1 + ld] + cf[3 * n + ps] * e[q - 1 + ld]) +
+                        (cf[5 * n + ps] * e[q - 1 - ld] + cf[7 * n + ps] * e[q + 1 - ld]);
+            av[u] = fa[ps];
+            fv[u] = f1[ps];
           }
         }
 #pragma unroll
@@ -285,9 +302,9 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
           if (j < j1) {
             A = av[u] * A;
             B = av[u] * B + sv_[u] * fv[u];
-            const int p = i + nx * j;
-            tA[p] = A;
-            tB[p] = B;
+            const int ps = pb + col + ncol * j;
+            tA[ps] = A;
+            tB[ps] = B;
           }
         }
       }
@@ -309,7 +326,8 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
   // backward: x_j = alpha_j + beta_j x_out within the chunk
   for (int task = wave; task < ntask; task += nw) {
     const int grp = task / nch, ch = task - grp * nch;
-    const int i = par + 2 * (grp * 64 + lane);
+    const int col = grp * 64 + lane;
+    const int i = par + 2 * col;
     const int j0 = ch * cl, j1 = min(j0 + cl, ny);
     const T din = sv[task * 64 + lane];
     T al = (T)0, be = (T)1;
@@ -320,10 +338,10 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
         for (int u = 0; u < kYBack; ++u) {
           const int j = jb - u;
           if (j >= j0) {
-            const int p = i + nx * j;
-            a_[u] = tA[p];
-            b_[u] = tB[p];
-            c_[u] = fc[p];
+            const int ps = pb + col + ncol * j;
+            a_[u] = tA[ps];
+            b_[u] = tB[ps];
+            c_[u] = fc[ps];
           }
         }
 #pragma unroll
@@ -333,9 +351,9 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
             const T dp = a_[u] * din + b_[u];
             al = dp - c_[u] * al;
             be = -c_[u] * be;
-            const int p = i + nx * j;
-            tA[p] = al;
-            tB[p] = be;
+            const int ps = pb + col + ncol * j;
+            tA[ps] = al;
+            tB[ps] = be;
           }
         }
       }
@@ -356,7 +374,8 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
   __syncthreads();
   for (int task = wave; task < ntask; task += nw) {
     const int grp = task / nch, ch = task - grp * nch;
-    const int i = par + 2 * (grp * 64 + lane);
+    const int col = grp * 64 + lane;
+    const int i = par + 2 * col;
     if (i >= nx) continue;
     const int j0 = ch * cl, j1 = min(j0 + cl, ny);
     const T xo = sv[task * 64 + lane];
@@ -365,8 +384,8 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
 #pragma unroll
       for (int u = 0; u < kYApply; ++u) {
         const int j = min(jb + u, j1 - 1);
-        a_[u] = tA[i + nx * j];
-        b_[u] = tB[i + nx * j];
+        a_[u] = tA[pb + col + ncol * j];
+        b_[u] = tB[pb + col + ncol * j];
       }
 #pragma unroll
       for (int u = 0; u < kYApply; ++u)
@@ -909,7 +928,7 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
   }
   // coefficient tables, and per live mode: line factors, c9, coarsest inverse
   const size_t nfine = (size_t)g.lev[ng].n;
-  std::vector<T> cfT(8 * tot_n);
+  std::vector<T> cfT(8 * tot_n), cfyT(8 * tot_n);
   std::vector<T> fac((size_t)nmode * fstride, (T)0);
   std::vector<double> c964((size_t)nmode * nfine);
   for (int k = 1; k <= ng; ++k) {
@@ -917,6 +936,10 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
     const int nx = H.nx, ny = H.ny;
     const size_t n = (size_t)nx * ny, off = loff[k];
     for (size_t t = 0; t < 8 * n; ++t) cfT[8 * off + t] = (T)H.c[t];
+    for (int c = 0; c < 8; ++c)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+          cfyT[8 * off + c * n + (size_t)ysplit(i, j, nx, ny)] = (T)H.c[c * n + i + (size_t)nx * j];
     std::vector<double> lo(std::max(nx, ny)), di(lo.size()), up(lo.size()), f1(lo.size()), fa(lo.size()), fc(lo.size());
     for (int m = 0; m < nmode; ++m) {
       const double kx2 = kx2_mode[m];
@@ -953,9 +976,10 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
         line_factors(ny, lo.data(), di.data(), up.data(), f1.data(), fa.data(), fc.data());
         for (int j = 0; j < ny; ++j) {
           const size_t q = i + (size_t)nx * j;
-          fy[q] = spec[off + q] ? (T)0 : (T)f1[j];
-          fy[n + q] = (T)fa[j];
-          fy[2 * n + q] = (T)fc[j];
+          const size_t qs = (size_t)ysplit(i, j, nx, ny);
+          fy[qs] = spec[off + q] ? (T)0 : (T)f1[j];
+          fy[n + qs] = (T)fa[j];
+          fy[2 * n + qs] = (T)fc[j];
         }
       }
       if (k == lc) {
@@ -988,12 +1012,14 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
   const size_t b_cf64 = (8 * nfine * sizeof(double) + 255) & ~(size_t)255;
   const size_t b_c964 = (c964.size() * sizeof(double) + 255) & ~(size_t)255;
   char* mem = nullptr;
-  EKM_CHECK(hipMalloc(&mem, b_desc + b_spec + b_cf + b_cf64 + b_c964));
+  EKM_CHECK(hipMalloc(&mem, b_desc + b_spec + 2 * b_cf + b_cf64 + b_c964));
   tp.mem = mem;
   char* cur = mem + b_desc;
   unsigned char* d_spec = (unsigned char*)cur;
   cur += b_spec;
   T* d_cf = (T*)cur;
+  cur += b_cf;
+  T* d_cfy = (T*)cur;
   cur += b_cf;
   double* d_cf64 = (double*)cur;
   cur += b_cf64;
@@ -1001,6 +1027,7 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
   for (int k = 1; k <= ng; ++k) {
     g.lev[k].spec = d_spec + loff[k];
     g.lev[k].cf = d_cf + 8 * loff[k];
+    g.lev[k].cfy = d_cfy + 8 * loff[k];
   }
   g.cf64 = d_cf64;
   g.c964 = d_c964;
@@ -1010,6 +1037,7 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
   EKM_CHECK(hipMemcpy(mem, &g, sizeof(g), hipMemcpyHostToDevice));
   EKM_CHECK(hipMemcpy(d_spec, spec.data(), tot_n, hipMemcpyHostToDevice));
   EKM_CHECK(hipMemcpy(d_cf, cfT.data(), 8 * tot_n * sizeof(T), hipMemcpyHostToDevice));
+  EKM_CHECK(hipMemcpy(d_cfy, cfyT.data(), 8 * tot_n * sizeof(T), hipMemcpyHostToDevice));
   EKM_CHECK(hipMemcpy(d_cf64, hl[ng].c.data(), 8 * nfine * sizeof(double), hipMemcpyHostToDevice));
   EKM_CHECK(hipMemcpy(d_c964, c964.data(), c964.size() * sizeof(double), hipMemcpyHostToDevice));
   tp.ws_elems = g.ws;
