@@ -167,8 +167,10 @@ __device__ inline Aff<T> wave_scan_down(Aff<T> v) {
 // z lines (rows j of parity par, contiguous in memory): one wavefront per row;
 // point k = 64 c + lane sits in lane `lane` of stripe c, so every load and
 // store of a stripe is one contiguous run; within a stripe the recurrence is a
-// 64-lane scan of affine maps, carried from stripe to stripe.
-template <typename T, int CM, bool X>
+// 64-lane scan of affine maps, carried from stripe to stripe.  ZERO: the
+// correction is zero on the neighbouring rows (the first sweep of a cycle's
+// down leg), so they are not read.
+template <typename T, int CM, bool X, bool ZERO = false>
 __device__ void zlines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ fac, int par) {
   T* W = wbase(L, Wp);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -190,10 +192,13 @@ __device__ void zlines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
       if (i < nx) {
         const int p = i + nx * j;
         const int q = (i + 1) + ld * (j + 1);
-        T s = r[p] - (cf[2 * n + p] * e[q + ld] + cf[6 * n + p] * e[q - ld]);
-        if (X)
-          s -= (cf[1 * n + p] * e[q + 1 + ld] + cf[3 * n + p] * e[q - 1 + ld]) +
-               (cf[5 * n + p] * e[q - 1 - ld] + cf[7 * n + p] * e[q + 1 - ld]);
+        T s = r[p];
+        if (!ZERO) {
+          s -= cf[2 * n + p] * e[q + ld] + cf[6 * n + p] * e[q - ld];
+          if (X)
+            s -= (cf[1 * n + p] * e[q + 1 + ld] + cf[3 * n + p] * e[q - 1 + ld]) +
+                 (cf[5 * n + p] * e[q - 1 - ld] + cf[7 * n + p] * e[q + 1 - ld]);
+        }
         ak[c] = fa[p];
         bk[c] = s * f1[p];
         ck[c] = fc[p];
@@ -395,9 +400,18 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
   __syncthreads();
 }
 
+// One alternating zebra sweep.  The rows of parity 0 are solved first and
+// read only the rows of parity 1, so a correction that is zero on entry (the
+// down leg: nothing has been written since the level was restricted to) is
+// not read at all, and the prolongation of the up leg need only have updated
+// the rows of parity 1.  The interior of the level's correction is fully
+// rewritten by the two z sweeps; its ring stays zero.
 template <typename T, int CM, bool X>
-__device__ void smooth(const TDesc<T>& g, const TLev<T>& L, const Ws<T>& W, const T* fac, T* lds) {
-  zlines<T, CM, X>(L, W, fac, 0);
+__device__ void smooth(const TDesc<T>& g, const TLev<T>& L, const Ws<T>& W, const T* fac, T* lds, bool down) {
+  if (down)
+    zlines<T, CM, X, true>(L, W, fac, 0);
+  else
+    zlines<T, CM, X>(L, W, fac, 0);
   __syncthreads();
   zlines<T, CM, X>(L, W, fac, 1);
   __syncthreads();
@@ -444,14 +458,13 @@ __device__ void residual(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict_
 }
 
 // full weighting of the residual of F onto the right-hand side of C (zero
-// outside the grid; a direction that is not coarsened keeps its index), and
-// C's correction cleared
+// outside the grid; a direction that is not coarsened keeps its index).  C's
+// correction is not cleared: the down leg's first sweep treats it as zero.
 template <typename T>
 __device__ void restrict_fw(const TLev<T>& F, const TLev<T>& C, const Ws<T>& W) {
   const int sx = C.nx < F.nx ? 2 : 1, sy = C.ny < F.ny ? 2 : 1;
   const T* __restrict__ res = wbase(F, W) + F.res_off;
   T* __restrict__ rc = wbase(C, W) + C.r_off;
-  T* __restrict__ ec = wbase(C, W) + C.e_off;
   const T w[3] = {(T)0.25, (T)0.5, (T)0.25};
   const int nt = blockDim.x;
   for (int p0 = threadIdx.x; p0 < C.n; p0 += kEwR * nt) {
@@ -477,11 +490,13 @@ __device__ void restrict_fw(const TLev<T>& F, const TLev<T>& C, const Ws<T>& W) 
     for (int u = 0; u < kEwR; ++u)
       if (p0 + u * nt < C.n) rc[p0 + u * nt] = v[u];
   }
-  for (int t = threadIdx.x; t < (C.nx + 2) * (C.ny + 2); t += blockDim.x) ec[t] = (T)0;
   __syncthreads();
 }
 
-// e_F += bilinear interpolation of e_C (not at specified points)
+// e_F += bilinear interpolation of e_C (not at specified points), on the
+// rows of parity 1 only: the smoothing that follows solves the rows of
+// parity 0 first, reading only those of parity 1, and overwrites every row
+// before anything else reads the correction
 template <typename T>
 __device__ void prolong_add(const TLev<T>& C, const TLev<T>& F, const Ws<T>& W) {
   const int sx = C.nx < F.nx ? 2 : 1, sy = C.ny < F.ny ? 2 : 1;
@@ -489,13 +504,18 @@ __device__ void prolong_add(const TLev<T>& C, const TLev<T>& F, const Ws<T>& W) 
   T* __restrict__ ef = wbase(F, W) + F.e_off;
   const int lc = C.nx + 2, lf = F.nx + 2;
   const int nt = blockDim.x;
-  for (int p0 = threadIdx.x; p0 < F.n; p0 += kEw * nt) {
+  const int nodd = F.nx * (F.ny >> 1);  // points of rows 1, 3, ...
+  for (int p0 = threadIdx.x; p0 < nodd; p0 += kEw * nt) {
     T v[kEw];
     int qf[kEw];
+    int pf[kEw];
 #pragma unroll
     for (int u = 0; u < kEw; ++u) {
-      const int p = min(p0 + u * nt, F.n - 1);
-      const int j = p / F.nx, i = p - j * F.nx;
+      const int po = min(p0 + u * nt, nodd - 1);
+      const int jr = po / F.nx, i = po - jr * F.nx;
+      const int j = 2 * jr + 1;
+      const int p = i + F.nx * j;
+      pf[u] = p;
       const int i0 = i / sx, i1 = (i + sx - 1) / sx, j0 = j / sy, j1 = (j + sy - 1) / sy;
       qf[u] = (i + 1) + lf * (j + 1);
       const T x = (T)0.25 * ((ec[(i0 + 1) + lc * (j0 + 1)] + ec[(i1 + 1) + lc * (j0 + 1)]) +
@@ -503,10 +523,8 @@ __device__ void prolong_add(const TLev<T>& C, const TLev<T>& F, const Ws<T>& W) 
       v[u] = F.spec[p] ? (T)0 : x + ef[qf[u]];
     }
 #pragma unroll
-    for (int u = 0; u < kEw; ++u) {
-      const int p = p0 + u * nt;
-      if (p < F.n && !F.spec[p]) ef[qf[u]] = v[u];
-    }
+    for (int u = 0; u < kEw; ++u)
+      if (p0 + u * nt < nodd && !F.spec[pf[u]]) ef[qf[u]] = v[u];
   }
   __syncthreads();
 }
@@ -604,7 +622,6 @@ __device__ __noinline__ double gcr_step(const TDesc<T>& g, const TLev<T>& F, dou
     const double z = (double)ef[t] - (hasq ? beta * Zp[t] : 0.0);
     phi[t] += alpha * z;
     Zp[t] = z;
-    ef[t] = (T)0;
   }
   for (int p = threadIdx.x; p < n; p += blockDim.x) {
     const double q2 = qv[p] - (hasq ? beta * Qp[p] : 0.0);
@@ -655,6 +672,13 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
     phi[t] = v;
   }
   for (int p = threadIdx.x; p < n; p += blockDim.x) b[p] = F.spec[p] ? 0.0 : -rt[p];
+  // every level's correction starts at zero: the rings stay zero, the interiors
+  // are rewritten by every sweep
+  for (int l = 1; l <= ng; ++l) {
+    const TLev<T>& L = g.lev[l];
+    T* el = wbase(L, Wp) + L.e_off;
+    for (int t = threadIdx.x; t < (L.nx + 2) * (L.ny + 2); t += blockDim.x) el[t] = (T)0;
+  }
   __syncthreads();
   const int P = (nx + 2) * (ny + 2);
   double* __restrict__ rr = b + n;  // krylov: the recursive residual (first iteration)
@@ -689,17 +713,14 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
             if (g.krylov) rr[p0 + u * NT] = v[u];
           }
       }
-      for (int t = threadIdx.x; t < P; t += blockDim.x) ef[t] = (T)0;
     } else {
       rmax = gcr_step<T, NT, X>(g, F, phi, W, c964, red, it);
     }
-    red[threadIdx.x] = rmax;
+    // max over the workgroup: wavefront maxima, then one value per wavefront
+    for (int o = 32; o > 0; o >>= 1) rmax = fmax(rmax, __shfl_xor(rmax, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = rmax;
     __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
-      __syncthreads();
-    }
-    rmax = red[0];
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) rmax = fmax(rmax, red[w]);
     __syncthreads();
     if (rmax <= g.tol || it >= g.maxit) break;
     // one cycle on the correction equation: the steps of the mode's program
@@ -713,7 +734,7 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
         continue;
       }
       if (op == 1) prolong_add<T>(g.lev[l - 1], g.lev[l], Wp);
-      smooth<T, CM, X>(g, g.lev[l], Wp, fac, ylds);
+      smooth<T, CM, X>(g, g.lev[l], Wp, fac, ylds, op == 0);
       if (op == 0) {
         residual<T, X>(g.lev[l], Wp, fac);
         restrict_fw<T>(g.lev[l], g.lev[l - 1], Wp);
