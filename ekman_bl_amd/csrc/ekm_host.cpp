@@ -395,6 +395,33 @@ extern "C" int ekm_host_metrics(int nz, int ny, const double* xpoint, const doub
   return EKM_OK;
 }
 
+// SPONGE_SIGMA(J), J = 0..NY+1 (src/diablo.F90:1533-1556): zero below
+// NY_S_T; from NY_S_T up, parabolic (type 1) in ypoint(NZ+1, J) or
+// exponential (type 2, co_b = 30) in the row index, both 0 at NY_S_T and 1 at
+// NY+1.  The spans pass through REAL(), a default (single precision) real.
+void host_sponge_sigma(int type, int nz, int ny, int ny_s_t, const double* ypoint, double* sig) {
+  for (int j = 0; j <= ny + 1; ++j) sig[j] = 0.0;
+  if (ny_s_t < 0 || ny_s_t > ny + 1) return;
+  auto y = [&](int j) { return ypoint[(size_t)(nz + 1) + (size_t)(nz + 2) * j]; };
+  if (type == 1) {
+    const double b = 1.0 / (double)(float)(y(ny + 1) - y(ny_s_t));
+    for (int j = ny_s_t; j <= ny + 1; ++j) {
+      const double xx = (y(j) - y(ny_s_t)) * b;
+      sig[j] = xx * xx;
+    }
+  } else if (type == 2) {
+    const double co_b = 30.0;
+    const double b = std::log(co_b) / (double)(float)(ny + 1 - ny_s_t);
+    for (int j = ny_s_t; j <= ny + 1; ++j) sig[j] = (std::exp(b * (double)(j - ny_s_t)) - 1.0) / (co_b - 1.0);
+  }
+}
+
+extern "C" int ekm_host_sponge_sigma(int type, int nz, int ny, int ny_s_t, const double* ypoint, double* sig) {
+  if (type < 0 || type > 2 || nz < 1 || ny < 1) return EKM_ERR_ARG;
+  host_sponge_sigma(type, nz, ny, ny_s_t, ypoint, sig);
+  return EKM_OK;
+}
+
 extern "C" int ekm_host_les_metrics(int nz, int ny, const double* xpoint, const double* ypoint, double* out) {
   HostMetrics hm;
   int st = host_metrics(nz, ny, xpoint, ypoint, hm);

@@ -22,3 +22,5 @@ struct HostMgLevel {
 int host_metrics(int NZ, int NY, const double* xp, const double* yp, HostMetrics& out);
 int host_mg_level(const HostMetrics& hm, int NZ, int NY, const int bc[4], int nx, int ny, HostMgLevel& L);
 int host_bc_table(const ekm_params& p, int comp, int dir, BcTable& t);
+// SPONGE_SIGMA(0:NY+1) of the vertical sponge (src/diablo.F90:1533-1556)
+void host_sponge_sigma(int type, int nz, int ny, int ny_s_t, const double* ypoint, double* sig);

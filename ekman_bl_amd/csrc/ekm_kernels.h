@@ -68,5 +68,23 @@ int launch_xstats(const Dims& d, const double* U1, const double* U2, const doubl
 // flow-rate integral of the F_TYPE 0 controller into out[0]
 int launch_flow_rate(const Dims& d, const Ranges& r, const double* CU3, const double* dy, double* out,
                      hipStream_t st);
+// sponge_th (src/duct.F90:3196-3362) with the streamwise profile off; see
+// k_sponge_fluct / k_sponge_mean
+struct SpongeArgs {
+  double2* cf[3];
+  double2* cfth;
+  double2* cu[3];
+  const double2* cth;
+  const double* sigma;  // SPONGE_SIGMA(0:NY+1)
+  double amp;           // Sponge_Amp = 0.1 / DELTA_T
+  Ranges r;             // the velocity ranges JSTART..JEND, ZSTART..ZEND
+  int j0;               // first row of the fluctuation damping (max(JSTART, NY_S_T))
+  int ny_sponge;        // first row of the mean-U3 average
+  double t1, t2, t3;    // mode-0 targets V_BC_YMAX_C1, U_BC_YMAX_C1, WBULK0
+  int zmax6;            // type-6 z-max velocity rows with RI_TAU != 0
+  int zero_cu2;         // ... and MOD(2*TIME_STEP, SAVE_STATS_INT) = 0
+  double jst;           // INT_JACOB(ZEND+1, JEND+1)
+};
+int launch_sponge(const Dims& d, const Metrics& M, const SpongeArgs& a, hipStream_t st);
 int launch_thomas_generic(const double* A, const double* B, const double* C, double* G, double* W, int ny,
                           int nx, hipStream_t st);

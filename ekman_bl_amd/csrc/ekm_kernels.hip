@@ -964,6 +964,92 @@ __global__ void __launch_bounds__(256) k_flow_rate(Dims d, Ranges r, const doubl
   if (threadIdx.x == 0) out[0] = red[0];
 }
 
+// sponge_th (src/duct.F90:3196-3362) with SPONGE_SIGMA_OUT = 0 (the
+// streamwise sponge off), so its inflow/outflow updates are the identity.
+// Fluctuations (:3228-3236), modes 1 and up: CF -= (Sponge_Amp SIGMA(J)) CU
+// INT_JACOB for the three velocities and the scalar.  SIGMA is zero below
+// NY_S_T, where the update is the identity, so only rows from j0 are visited.
+__global__ void k_sponge_fluct(Dims d, Metrics M, SpongeArgs a) {
+  const int nm = d.NXC - 1, nk = a.r.ze - a.r.zs + 1;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nm * nk * (a.r.je - a.j0 + 1)) return;
+  const int m = 1 + (int)(t % nm);
+  const size_t rr = t / nm;
+  const int k = a.r.zs + (int)(rr % nk), j = a.j0 + (int)(rr / nk);
+  const size_t q = fo(d, m, k, j);
+  const double Jc = JAC(k, j), as = a.amp * a.sigma[j];
+  for (int f = 0; f < 3; ++f) {
+    const double2 u = a.cu[f][q];
+    double2 c = a.cf[f][q];
+    c.x = c.x - (as * u.x) * Jc;
+    c.y = c.y - (as * u.y) * Jc;
+    a.cf[f][q] = c;
+  }
+  const double2 u = a.cth[q];
+  double2 c = a.cfth[q];
+  c.x = c.x - (as * u.x) * Jc;
+  c.y = c.y - (as * u.y) * Jc;
+  a.cfth[q] = c;
+}
+
+// Mode 0 of sponge_th, one workgroup: the mean of Re CU3 over J = NY_sponge..
+// JEND, K = ZSTART..ZEND (:3261-3272) decides the dynamic sponge (:3283-3305):
+// beyond 0.1% of WBULK0 the three velocities relax to their targets and the
+// scalar to zero, else only CU1 and CU2.  With type-6 z-max rows and
+// stratification (:3324-3332), CF2 of mode 0 relaxes on the whole plane with
+// the Jacobian at the exit indices of the loops above, and CU2X(0,:,:) is
+// zeroed on the steps where MOD(2*TIME_STEP, SAVE_STATS_INT) = 0.
+__global__ void __launch_bounds__(1024) k_sponge_mean(Dims d, Metrics M, SpongeArgs a) {
+  __shared__ double red[1024];
+  const int nk = a.r.ze - a.r.zs + 1, nj = a.r.je - a.ny_sponge + 1;
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < nk * nj; t += blockDim.x)
+    acc += a.cu[2][fo(d, 0, a.r.zs + t % nk, a.ny_sponge + t / nk)].x;
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double mean3 = red[0] / (double)(nk * nj);
+  const bool dyn = fabs(mean3 - a.t3) > a.t3 * 0.001;
+  const int nji = a.r.je - a.r.js + 1;
+  for (int t = threadIdx.x; t < nk * nji; t += blockDim.x) {
+    const int k = a.r.zs + t % nk, j = a.r.js + t / nk;
+    const size_t q = fo(d, 0, k, j);
+    const double Jc = JAC(k, j), as = a.amp * a.sigma[j];
+    const double2 u1 = a.cu[0][q], u2 = a.cu[1][q];
+    double2 c1 = a.cf[0][q], c2 = a.cf[1][q];
+    c1.x = c1.x - (as * (u1.x - a.t1)) * Jc;
+    c1.y = c1.y - (as * u1.y) * Jc;
+    c2.x = c2.x - (as * (u2.x - a.t2)) * Jc;
+    c2.y = c2.y - (as * u2.y) * Jc;
+    a.cf[0][q] = c1;
+    a.cf[1][q] = c2;
+    if (dyn) {
+      const double2 u3 = a.cu[2][q], th = a.cth[q];
+      double2 c3 = a.cf[2][q], ct = a.cfth[q];
+      c3.x = c3.x - (as * (u3.x - a.t3)) * Jc;
+      c3.y = c3.y - (as * u3.y) * Jc;
+      ct.x = ct.x - (as * th.x) * Jc;
+      ct.y = ct.y - (as * th.y) * Jc;
+      a.cf[2][q] = c3;
+      a.cfth[q] = ct;
+    }
+  }
+  if (!a.zmax6) return;
+  __syncthreads();
+  for (int t = threadIdx.x; t < (int)d.plane; t += blockDim.x) {
+    const size_t q = fo(d, 0, t % d.NZ2, t / d.NZ2);
+    const double2 u2 = a.cu[1][q];
+    double2 c2 = a.cf[1][q];
+    c2.x = c2.x - (a.amp * (u2.x - a.t2)) * a.jst;
+    c2.y = c2.y - (a.amp * u2.y) * a.jst;
+    a.cf[1][q] = c2;
+    if (a.zero_cu2) a.cu[1][q] = make_double2(0.0, 0.0);
+  }
+}
+
 // divergence diagnostic (:2790-2800): sum and max of |div| over
 // J = 2..NY-1, K = 2..NZ-1, I = 1..NX+1 of the physical field
 __global__ void k_div_reduce(Dims d, const double* __restrict__ S, double* __restrict__ out) {
@@ -1148,6 +1234,16 @@ int launch_xstats(const Dims& d, const double* U1, const double* U2, const doubl
                   const double* TH, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_xstats, dim3(nblk(d.plane, 4)), dim3(256), 0, st, d, (const double2*)U1, (const double2*)U2,
                      (const double2*)U3, (const double2*)P, (const double2*)TH, out);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_sponge(const Dims& d, const Metrics& M, const SpongeArgs& a, hipStream_t st) {
+  if (a.r.je >= a.j0 && d.NXC > 1) {
+    const size_t n = (size_t)(d.NXC - 1) * (a.r.ze - a.r.zs + 1) * (a.r.je - a.j0 + 1);
+    hipLaunchKernelGGL(k_sponge_fluct, dim3(nblk(n, 256)), dim3(256), 0, st, d, M, a);
+    EKM_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_sponge_mean, dim3(1), dim3(1024), 0, st, d, M, a);
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }

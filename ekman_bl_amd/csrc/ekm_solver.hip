@@ -74,6 +74,10 @@ struct ekm_solver {
   double* fdy = nullptr;
   double q_den = 1.0;  // (ypoint(1,JEND) - ypoint(1,JSTART)) * (ZEND-ZSTART-1)
   double hb_prev = 0.0;
+  // sponge_th (src/duct.F90:3196-3362): SPONGE_SIGMA(0:NY+1) and
+  // INT_JACOB(ZEND+1, JEND+1), the Jacobian its type-6 branch reads
+  double* sponge = nullptr;
+  double sponge_jst = 0.0;
 };
 
 extern "C" {
@@ -100,6 +104,7 @@ void ekm_params_default(ekm_params* p) {
   p->proj_maxit = 50;
   p->proj_tol = 1.0e-12;
   p->proj_krylov = 1;
+  p->ny_s_t = p->nz_s_r = p->save_stats_int = 1 << 30;  // no sponge rows
 }
 
 }  // extern "C"
@@ -123,6 +128,15 @@ static int check_params(const ekm_params& p) {
   // adds KAPPA_T indexed by Fourier mode to the background diffusion
   // (src/duct.F90:610-618); that form is not supported
   if (p.les_model && p.n_th && !p.cont_strat) return EKM_ERR_UNSUPPORTED;
+  if (p.sponge_type < 0 || p.sponge_type > 2) return EKM_ERR_UNSUPPORTED;
+  if (p.sponge_type) {
+    // the streamwise sponge also scales the scalar's diffusion by
+    // 1 + 500 SPONGE_SIGMA_OUT (src/duct.F90:1354,1529-1536) and the type-8
+    // rows; that form is not supported
+    if (p.nz_s_r < p.nz + 1) return EKM_ERR_UNSUPPORTED;
+    // NY_S_T = NY+1 makes the profile 0/0 (src/diablo.F90:1538,1550)
+    if (p.ny_s_t < 0 || p.ny_s_t == p.ny + 1 || p.save_stats_int < 1) return EKM_ERR_ARG;
+  }
   if (p.proj_mode > 0) {
     // line smoothing is not cyclic, and the consistent system needs a
     // specified side to be nonsingular at kx = 0
@@ -165,7 +179,7 @@ static void set_hbar(ekm_solver* s) {
 static int free_all(ekm_solver* s) {
   double* f[] = {s->U[0], s->U[1], s->U[2], s->P, s->TH, s->CF[0], s->CF[1], s->CF[2], s->CFTH,
                  s->R[0], s->R[1], s->R[2], s->RTH, s->S1, s->S2, s->U2b, s->U3b, s->PHI, s->lu,
-                 s->red, s->met, s->kx, s->kx2, s->kx2s, s->thbar, s->lesmet, s->fdy};
+                 s->red, s->met, s->kx, s->kx2, s->kx2s, s->thbar, s->lesmet, s->fdy, s->sponge};
   for (double* q : f)
     if (q) (void)hipFree(q);
   if (s->adi_d) (void)hipFree(s->adi_d);
@@ -357,6 +371,14 @@ extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const doub
   if (hipMalloc(&s->red, 4 * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
   s->px0 = p.px0;
   s->qn = p.wbulk0;
+  if (p.sponge_type && p.n_th) {
+    std::vector<double> sig(d.NY2);
+    host_sponge_sigma(p.sponge_type, d.NZ, d.NY, p.ny_s_t, ypoint, sig.data());
+    if (hipMalloc(&s->sponge, d.NY2 * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
+    if (hipMemcpy(s->sponge, sig.data(), d.NY2 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(EKM_ERR_HIP);
+    s->sponge_jst = s->hm.jac[(size_t)(s->rv.ze + 1) + (size_t)d.NZ3 * (s->rv.je + 1)];
+  }
   if (p.f_type == 0) {
     std::vector<double> dy(d.plane, 0.0);
     for (int j = 0; j + 1 < d.NY2; ++j)
@@ -618,6 +640,28 @@ extern "C" int ekm_rk_duct(ekm_solver* s) {
     a.px = s->px0;
   }
   TRY(launch_rhs_fourier(d, s->M, a, st));
+  // sponge_th, called after the scalar's explicit terms (:635)
+  if (th && p.sponge_type) {
+    SpongeArgs g;
+    for (int f = 0; f < 3; ++f) {
+      g.cf[f] = (double2*)s->CF[f];
+      g.cu[f] = (double2*)s->U[f];
+    }
+    g.cfth = (double2*)s->CFTH;
+    g.cth = (const double2*)s->TH;
+    g.sigma = s->sponge;
+    g.amp = 0.1 / s->delta_t;
+    g.r = s->rv;
+    g.j0 = std::max(s->rv.js, p.ny_s_t);
+    g.ny_sponge = p.ny_s_t > p.ny ? 2 * p.ny / 3 : p.ny_s_t;
+    g.t1 = p.u_bc_c1[1][EKM_YMAX];
+    g.t2 = p.u_bc_c1[0][EKM_YMAX];
+    g.t3 = p.wbulk0;
+    g.zmax6 = p.u_bc[0][EKM_ZMAX] == 6 && p.u_bc[1][EKM_ZMAX] == 6 && p.u_bc[2][EKM_ZMAX] == 6 && p.ri_tau != 0.0;
+    g.zero_cu2 = (2 * cur_step) % p.save_stats_int == 0;
+    g.jst = s->sponge_jst;
+    TRY(launch_sponge(d, s->M, g, st));
+  }
   // velocities and scalar to physical space (:655-671)
   // the subgrid model (:641-657), which leaves the velocities in physical
   // space, or the velocities to physical space (:658-671)

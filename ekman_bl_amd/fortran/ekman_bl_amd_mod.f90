@@ -50,6 +50,10 @@ module ekman_bl_amd
      integer(c_int) :: proj_guess
      integer(c_int) :: proj_krylov
      integer(c_int) :: les_model
+     integer(c_int) :: sponge_type
+     integer(c_int) :: ny_s_t
+     integer(c_int) :: nz_s_r
+     integer(c_int) :: save_stats_int
      integer(c_int) :: reserved(1)
   end type ekm_params
 

@@ -54,6 +54,10 @@ class EkmParams(ctypes.Structure):
         ("proj_guess", ctypes.c_int),
         ("proj_krylov", ctypes.c_int),
         ("les_model", ctypes.c_int),
+        ("sponge_type", ctypes.c_int),
+        ("ny_s_t", ctypes.c_int),
+        ("nz_s_r", ctypes.c_int),
+        ("save_stats_int", ctypes.c_int),
         ("reserved", ctypes.c_int * 1),
     ]
 
@@ -105,6 +109,7 @@ def base_params(nx: int, ny: int, nz: int, n_th: int = 0) -> EkmParams:
     for s in range(4):
         p.mudpack_bc[s] = 2
     p.proj_maxit, p.proj_tol, p.proj_krylov = 50, 1.0e-12, 1   # used by proj_mode 1, 2
+    p.ny_s_t = p.nz_s_r = p.save_stats_int = 1 << 30             # no sponge rows
     return p
 
 
@@ -190,6 +195,8 @@ def read_grid_def(path: str) -> dict:
         line = line.split("!")[0]
         for name, expr in re.findall(r"PARAMETER\s*\(\s*(\w+)\s*=\s*([^,\)]+)", line, re.I):
             expr = expr.strip()
+            # names defined on earlier lines (NZ_S_L = NZ + 1 - NZ_S_R)
+            expr = re.sub(r"[A-Za-z_]\w*", lambda m: str(out.get(m.group(0).lower(), m.group(0))), expr)
             if re.fullmatch(r"[0-9+\-*/ ()]+", expr):
                 out[name.lower()] = int(eval(expr))  # noqa: S307 - digits and operators only
         m = re.search(r"PARAMETER\s*\((.*)\)", line, re.I)
@@ -239,12 +246,14 @@ def read_input_dat(path: str, p: EkmParams) -> dict:
     mud_use, a, b, c, d = next(it)[:5]
     out["mudpack_use"] = mud_use
     p.mudpack_bc[0], p.mudpack_bc[1], p.mudpack_bc[2], p.mudpack_bc[3] = a, b, c, d
-    next(it)  # I/O
+    io = next(it)  # VERBOSITY, SAVE_FLOW_INT, SAVE_STATS_INT, BACKUP_FLOW_INT, MOVIE
+    p.save_stats_int = int(io[2])
     ics = next(it)
     out["ic_type"] = ics[1]
     ft, wb, px0, om, amp, _beta, f0, sponge = next(it)[:8]
     p.f_type, p.wbulk0, p.px0, p.omega0, p.amp_omega0, p.f0 = ft, wb, px0, om, amp, f0
     out["sponge_type"] = sponge
+    p.sponge_type = int(sponge)
     # BCs: U,V,W lower; U,V,W upper; U,V,W left; U,V,W right (:284-306)
     for slot in (YMIN, YMAX, ZMIN, ZMAX):
         for comp in range(3):
@@ -270,5 +279,8 @@ def params_from_run_dir(grid_def: str, input_dat: str) -> EkmParams:
     p = base_params(g["nx"], g["ny"], g["nz"], n_th=g.get("n_th", 0))
     if "iixp" in g:
         p.mg_ixp, p.mg_jyq, p.mg_iex, p.mg_jey = g["iixp"], g["jjyq"], g["iiex"], g["jjey"]
+    for key in ("ny_s_t", "nz_s_r"):
+        if key in g:
+            setattr(p, key, g[key])
     read_input_dat(input_dat, p)
     return p

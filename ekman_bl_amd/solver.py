@@ -83,6 +83,7 @@ def lib():
         L.ekm_thomas_real_dev.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]
         L.ekm_host_metrics.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D]
         L.ekm_host_les_metrics.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D]
+        L.ekm_host_sponge_sigma.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D]
         L.ekm_host_mg_coefficients.argtypes = [ctypes.POINTER(EkmParams), _D, _D, ctypes.c_int, ctypes.c_double,
                                                _D, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         _lib = L
@@ -270,6 +271,15 @@ def host_metrics(nz: int, ny: int, xpoint: np.ndarray, ypoint: np.ndarray) -> np
     _check(lib().ekm_host_metrics(nz, ny, _p(np.ascontiguousarray(xpoint, dtype=np.float64)),
                                   _p(np.ascontiguousarray(ypoint, dtype=np.float64)), _p(out)), "ekm_host_metrics")
     return out.reshape(15, ny + 3, nz + 3)
+
+
+def host_sponge_sigma(sponge_type: int, nz: int, ny: int, ny_s_t: int, ypoint: np.ndarray) -> np.ndarray:
+    """SPONGE_SIGMA(0:NY+1) of the vertical sponge (src/diablo.F90:1533-1556)."""
+    out = np.zeros(ny + 2)
+    _check(lib().ekm_host_sponge_sigma(sponge_type, nz, ny, ny_s_t,
+                                       _p(np.ascontiguousarray(ypoint, dtype=np.float64)), _p(out)),
+           "ekm_host_sponge_sigma")
+    return out
 
 
 def host_les_metrics(nz: int, ny: int, xpoint: np.ndarray, ypoint: np.ndarray) -> np.ndarray:

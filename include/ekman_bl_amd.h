@@ -90,6 +90,15 @@ typedef struct ekm_params {
                             * residual step); 0 = plain iterative refinement */
   int les_model;           /* LES_MODEL_TYPE of the velocity subgrid model (src/les_chan.F90):
                             * 0 = none (DNS), 2 = dynamic Smagorinsky */
+  int sponge_type;         /* SPONGE_TYPE (input.dat, src/diablo.F90:278): 0 = none, 1 = parabolic,
+                            * 2 = exponential Rayleigh damping (sponge_th, src/duct.F90:3196-3362,
+                            * applied when N_TH = 1) */
+  int ny_s_t;              /* NY_S_T (grid_def): first row J of the vertical sponge */
+  int nz_s_r;              /* NZ_S_R (grid_def): the streamwise sponge, which needs NZ_S_R >= NZ+1
+                            * (off) when sponge_type != 0 */
+  int save_stats_int;      /* SAVE_STATS_INT (input.dat, src/diablo.F90:274); the sponge's
+                            * stratified type-6 branch zeroes CU2X(0,:,:) when
+                            * MOD(2*TIME_STEP, SAVE_STATS_INT) = 0 */
   int reserved[1];
 } ekm_params;
 
@@ -233,6 +242,10 @@ int ekm_host_metrics(int nz, int ny, const double* xpoint, const double* ypoint,
 /* The subgrid-stress metric variants of JACOBI_TRANS (src/diablo.F90:1379-1474):
  * 12*(NZ+3)*(NY+3) = GMAT_11_z,12_z,22_z,11_y,12_y,22_y with two faces each. */
 int ekm_host_les_metrics(int nz, int ny, const double* xpoint, const double* ypoint, double* out);
+/* SPONGE_SIGMA(J), J = 0..NY+1, of the vertical sponge from ypoint(NZ+1, J)
+ * (replaces the profile part of JACOBI_TRANS, src/diablo.F90:1533-1556;
+ * type 1 parabolic, 2 exponential, 0 zero). */
+int ekm_host_sponge_sigma(int type, int nz, int ny, int ny_s_t, const double* ypoint, double* sig);
 int ekm_host_mg_coefficients(const ekm_params* p, const double* xpoint, const double* ypoint,
                              int lev, double kx2, double* cf, int* nx, int* ny);
 /* Message of the last HIP error seen by this thread. */

@@ -72,6 +72,8 @@ struct ora {
    * pressure gradient PX0 and the previous flow rate Qn, set at start-up as
    * Q0 = Qn = WBULK0 (src/diablo.F90:359-361) */
   double px0, qn;
+  /* SPONGE_SIGMA (0:NY+1) of the vertical sponge (sponge_th) */
+  double* sigma;
 };
 
 #define IDX(o, i, k, j) ((size_t)(i) + (size_t)(o)->NX2 * ((size_t)(k) + (size_t)(o)->NZ2 * (size_t)(j)))
@@ -1486,6 +1488,11 @@ static int supported(const ekm_params* p) {
    * KAPPA_T indexed by Fourier mode to the background diffusion
    * (src/duct.F90:610-618); not restated */
   if (p->les_model && p->n_th && !p->cont_strat) return 0;
+  /* sponge: the vertical profile only (the streamwise one, NZ_S_R < NZ+1,
+   * also scales the scalar's diffusion and is not restated) */
+  if (p->sponge_type < 0 || p->sponge_type > 2) return 0;
+  if (p->sponge_type && (p->nz_s_r < p->nz + 1 || p->ny_s_t < 0 || p->ny_s_t == p->ny + 1 || p->save_stats_int < 1))
+    return 0;
   if (p->proj_mode == 1) {
     int dir = 0;
     for (int s = 0; s < 4; ++s) {
@@ -1519,6 +1526,8 @@ static void set_hbar(ora* o) {
   o->h_bar[1] = o->delta_t * (2.0 / 15.0);
   o->h_bar[2] = o->delta_t * (5.0 / 15.0);
 }
+
+static void sponge_sigma(ora* o);
 
 ora* ora_create(const ekm_params* p, const double* xpoint, const double* ypoint,
                 const double* th_bar) {
@@ -1568,6 +1577,8 @@ ora* ora_create(const ekm_params* p, const double* xpoint, const double* ypoint,
   o->first_time = 1;
   o->px0 = p->px0;
   o->qn = p->wbulk0;
+  o->sigma = dalloc((size_t)o->NY + 2);
+  sponge_sigma(o);
   init_ranges(o);
   o->delta_t = p->delta_t;
   o->delta_t_set = p->delta_t;
@@ -1580,7 +1591,7 @@ ora* ora_create(const ekm_params* p, const double* xpoint, const double* ypoint,
 
 void ora_destroy(ora* o) {
   if (!o) return;
-  free(o->xp); free(o->yp); free(o->thbar); free(o->jac); free(o->kx); free(o->kx2);
+  free(o->xp); free(o->yp); free(o->thbar); free(o->jac); free(o->kx); free(o->kx2); free(o->sigma);
   for (int q = 0; q < NMET; ++q) free(o->met[q]);
   double* f[] = {o->U[0], o->U[1], o->U[2], o->P, o->TH, o->CF[0], o->CF[1], o->CF[2], o->CFTH,
                  o->R[0], o->R[1], o->R[2], o->RTH, o->S1, o->S2, o->U2b, o->U3b, o->CS1, o->PHI};
@@ -2462,6 +2473,11 @@ int ora_xstats(ora* o, double* out) {
   return EKM_OK;
 }
 
+int ora_get_sponge(ora* o, double* sigma) {
+  memcpy(sigma, o->sigma, ((size_t)o->NY + 2) * sizeof(double));
+  return EKM_OK;
+}
+
 int ora_get_forcing(ora* o, double* px0, double* qn) {
   if (px0) *px0 = o->p.f_type == 0 ? o->px0 : o->p.px0;
   if (qn) *qn = o->qn;
@@ -2480,6 +2496,97 @@ int ora_get_les(ora* o, double* nut, double* cdyn) {
   if (nut) memcpy(nut, o->NUT, o->npts * sizeof(double));
   if (cdyn) memcpy(cdyn, o->CDYN, (size_t)o->NZ2 * o->NY2 * sizeof(double));
   return EKM_OK;
+}
+
+/* SPONGE_SIGMA(J), J = 0..NY+1 (src/diablo.F90:1533-1556): zero below NY_S_T;
+ * type 1: ((ypoint(NZ+1,J) - ypoint(NZ+1,NY_S_T)) b)^2 with b = 1/REAL(span),
+ * type 2: (exp(b (J - NY_S_T)) - 1)/(co_b - 1), co_b = 30,
+ * b = log(co_b)/REAL(NY+1-NY_S_T).  REAL() is a default (single) real. */
+static void sponge_sigma(ora* o) {
+  const ekm_params* p = &o->p;
+  const int NY = o->NY, nys = p->ny_s_t, kt = o->NZ + 1;
+  for (int j = 0; j <= NY + 1; ++j) o->sigma[j] = 0.0;
+  if (!p->sponge_type || nys < 0 || nys > NY + 1) return;
+  if (p->sponge_type == 1) {
+    const double y0 = o->yp[PL(o, kt, nys)];
+    const double b = 1.0 / (double)(float)(o->yp[PL(o, kt, NY + 1)] - y0);
+    for (int j = nys; j <= NY + 1; ++j) {
+      const double xx = (o->yp[PL(o, kt, j)] - y0) * b;
+      o->sigma[j] = xx * xx;
+    }
+  } else {
+    const double co_b = 30.0, b = log(co_b) / (double)(float)(NY + 1 - nys);
+    for (int j = nys; j <= NY + 1; ++j) o->sigma[j] = (exp(b * (double)(j - nys)) - 1.0) / (co_b - 1.0);
+  }
+}
+
+/* sponge_th (src/duct.F90:3196-3362) for SPONGE_SIGMA_OUT = 0 (NZ_S_R >=
+ * NZ+1), where its inflow/outflow updates are the identity.  DYNAMIC_SPONE
+ * is .TRUE. and Sponge_Amp = 0.1/DELTA_T. */
+static void sponge_th(ora* o, int cur_step) {
+  const ekm_params* p = &o->p;
+  const int js = o->jstart, je = o->jend, zs = o->zstart, ze = o->zend;
+  const double A = 0.1 / o->delta_t;
+  double *CU1 = o->U[0], *CU2 = o->U[1], *CU3 = o->U[2], *CTH = o->TH;
+  double *CF1 = o->CF[0], *CF2 = o->CF[1], *CF3 = o->CF[2], *CFT = o->CFTH;
+  /* fluctuations, I = 1..NX2P (:3228-3236) */
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k)
+      for (int m = 1; m < o->NXC; ++m)
+        for (int c = 0; c < 2; ++c) {
+          const size_t q = CRE(o, m, k, j) + c;
+          for (int f = 0; f < 3; ++f)
+            o->CF[f][q] = o->CF[f][q] - A * o->sigma[j] * (o->U[f][q] - 0.0) * JAC(o, k, j);
+          CFT[q] = CFT[q] - A * o->sigma[j] * (CTH[q] - 0.0) * JAC(o, k, j);
+        }
+  /* mode 0 (rank 0): the mean of CU3 over the top rows (:3261-3276) */
+  const int nysp = p->ny_s_t > o->NY ? 2 * o->NY / 3 : p->ny_s_t;
+  double s3 = 0.0;
+  for (int J = nysp; J <= je; ++J)
+    for (int K = zs; K <= ze; ++K) s3 = s3 + CU3[CRE(o, 0, K, J)];
+  s3 = s3 / (double)((ze - zs + 1) * (je - nysp + 1));
+  const double t1 = p->u_bc_c1[1][EKM_YMAX], t2 = p->u_bc_c1[0][EKM_YMAX], t3 = p->wbulk0;
+  const int dyn = fabs(s3 - t3) > t3 * 0.001;
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k) {
+      const size_t q = CRE(o, 0, k, j);
+      const double Jc = JAC(o, k, j);
+      /* a complex minus a real target changes the real part only */
+      if (dyn) { /* :3283-3292 */
+        CF1[q] = CF1[q] - A * o->sigma[j] * (CU1[q] - t1) * Jc;
+        CF1[q + 1] = CF1[q + 1] - A * o->sigma[j] * CU1[q + 1] * Jc;
+        CF2[q] = CF2[q] - A * o->sigma[j] * (CU2[q] - t2) * Jc;
+        CF2[q + 1] = CF2[q + 1] - A * o->sigma[j] * CU2[q + 1] * Jc;
+        CF3[q] = CF3[q] - A * o->sigma[j] * (CU3[q] - t3) * Jc;
+        CF3[q + 1] = CF3[q + 1] - A * o->sigma[j] * CU3[q + 1] * Jc;
+        CFT[q] = CFT[q] - A * o->sigma[j] * (CTH[q] - 0.0) * Jc;
+        CFT[q + 1] = CFT[q + 1] - A * o->sigma[j] * (CTH[q + 1] - 0.0) * Jc;
+      } else { /* :3294-3301 */
+        CF2[q] = CF2[q] - A * o->sigma[j] * (CU2[q] - t2) * Jc;
+        CF2[q + 1] = CF2[q + 1] - A * o->sigma[j] * CU2[q + 1] * Jc;
+        CF1[q] = CF1[q] - A * o->sigma[j] * (CU1[q] - t1) * Jc;
+        CF1[q + 1] = CF1[q + 1] - A * o->sigma[j] * CU1[q + 1] * Jc;
+      }
+    }
+  /* :3324-3332: the whole mode-0 plane of CF2, with INT_JACOB(K,J) read
+   * after the loops above (K = ZEND+1, J = JEND+1); CU2X(0,:,:) zeroed on
+   * the steps where MOD(2*TIME_STEP, SAVE_STATS_INT) = 0 */
+  if (p->u_bc[0][EKM_ZMAX] == 6 && p->u_bc[1][EKM_ZMAX] == 6 && p->u_bc[2][EKM_ZMAX] == 6 && p->ri_tau != 0.0) {
+    const double Jst = JAC(o, ze + 1, je + 1);
+    for (int J = 0; J <= o->NY + 1; ++J)
+      for (int K = 0; K <= o->NZ + 1; ++K) {
+        const size_t q = CRE(o, 0, K, J);
+        CF2[q] = CF2[q] - A * (CU2[q] - t2) * Jst;
+        CF2[q + 1] = CF2[q + 1] - A * CU2[q + 1] * Jst;
+      }
+    if ((2 * cur_step) % p->save_stats_int == 0)
+      for (int J = 0; J <= o->NY + 1; ++J)
+        for (int K = 0; K <= o->NZ + 1; ++K) {
+          const size_t q = CRE(o, 0, K, J);
+          CU2[q] = 0.0;
+          CU2[q + 1] = 0.0;
+        }
+  }
 }
 
 int ora_rk_duct(ora* o) {
@@ -2607,6 +2714,7 @@ int ora_rk_duct(ora* o) {
           }
         }
   }
+  if (th && p->sponge_type) sponge_th(o, cur_step); /* :635, after the scalar's explicit terms */
   /* :641-671 the subgrid model (which leaves the velocities in physical
    * space), or the velocities to physical space; the scalar to physical space */
   const int les = p->les_model && (!o->create_new_flow || cur_step > 1);

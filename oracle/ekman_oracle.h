@@ -41,6 +41,15 @@
  *                               explicit cross terms (les_xterm): parity unpinned
  *                               beyond the reference text (the GPU path is
  *                               checked against them)
+ *   sponge_sigma / sponge_th    SPONGE_SIGMA closed forms (0 at NY_S_T, 1 at
+ *                               NY+1, type 1 on the K = NZ+1 column, type 2 a
+ *                               constant growth factor); sigma = 0 and a state
+ *                               at the targets leave the substep bit-identical
+ *                               to the substep without a sponge; the damping
+ *                               lowers the top rows' fluctuation energy.  The
+ *                               stratified type-6 branch (whole-plane CF2
+ *                               relaxation, CU2X(0,:,:) zeroing): parity
+ *                               unpinned beyond the reference text
  *   ora_rk_duct / ora_time_step the composed operation order of RK_DUCT (lagged
  *                               rows, corner updates, dt timing) has no golden
  *                               vectors: parity unpinned beyond the pieces above
@@ -70,6 +79,8 @@ int ora_get_les(ora* o, double* nut, double* cdyn);
 int ora_get_les_th(ora* o, double* kappa, double* cdyn_th);
 /* the F_TYPE 0 controller: current PX0 and the last flow rate Qn */
 int ora_get_forcing(ora* o, double* px0, double* qn);
+/* SPONGE_SIGMA(0:NY+1) of the vertical sponge (zero without one) */
+int ora_get_sponge(ora* o, double* sigma);
 /* x statistics (16 planes of (NZ+2)*(NY+2)), direct sums in physical space */
 int ora_xstats(ora* o, double* out);
 /* one field's ADI (comp 0..2 velocity, 3 scalar) on physical U, R in place,

@@ -50,6 +50,7 @@ def lib():
         L.ora_get_les.argtypes = [ctypes.c_void_p, _D, _D]
         L.ora_get_les_th.argtypes = [ctypes.c_void_p, _D, _D]
         L.ora_get_forcing.argtypes = [ctypes.c_void_p, _D, _D]
+        L.ora_get_sponge.argtypes = [ctypes.c_void_p, _D]
         L.ora_xstats.argtypes = [ctypes.c_void_p, _D]
         L.ora_adi_field.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.c_double, _D, ctypes.c_double]
         L.ora_set_initial_physical.argtypes = [ctypes.c_void_p, _D, _D, _D, _D]
@@ -150,6 +151,12 @@ class Oracle:
         out = np.zeros((16, p.ny + 2, p.nz + 2))
         lib().ora_xstats(self.h, _p(out))
         return dict(zip(self.XSTATS, out))
+
+    def sponge_sigma(self):
+        """SPONGE_SIGMA(0:NY+1) of the vertical sponge."""
+        out = np.zeros(self.p.ny + 2)
+        lib().ora_get_sponge(self.h, _p(out))
+        return out
 
     def forcing(self):
         """(PX0, Qn) of the F_TYPE 0 flow-rate controller."""

@@ -364,6 +364,45 @@ def test_boundary_rows_match_oracle(case, oracle_lib):
     s.close()
 
 
+@pytest.mark.parametrize("case", ["type2_zero_cu2", "type1"])
+def test_sponge_matches_oracle(case, oracle_lib):
+    """sponge_th (src/duct.F90:3196-3362) on the GPU against the oracle, on the
+    reference run's form (rundir_unst): type-6 z rows for the velocity and the
+    scalar, scalar bottom type 5, periodic multigrid z ends with the plain
+    projection, stratification, F_TYPE 1 with WBULK0 = 0.05 (the dynamic
+    mean-flow sponge is taken) and a vertical sponge on the top rows.
+    type2_zero_cu2: SPONGE_TYPE 2 with SAVE_STATS_INT = 2, so the type-6
+    branch also zeroes CU2X(0,:,:) in every substep; type1: SPONGE_TYPE 1,
+    without the zeroing.  Six substeps, fields to 1e-10 relative L2."""
+    from ekman_bl_amd.params import YMIN, ZMIN, ZMAX
+    O = oracle_lib
+    cfg, p = _small_stratified()
+    for c in range(3):
+        p.u_bc[c][ZMIN] = p.u_bc[c][ZMAX] = 6
+    p.th_bc[YMIN] = 5
+    p.th_bc[ZMIN] = p.th_bc[ZMAX] = 6
+    p.mudpack_bc[0] = p.mudpack_bc[1] = 0
+    p.proj_mode = 0
+    p.f_type, p.wbulk0, p.px0 = 1, 0.05, 0.0
+    p.sponge_type = 2 if case.startswith("type2") else 1
+    p.ny_s_t = p.ny - 6
+    p.nz_s_r = p.nz + 2
+    p.save_stats_int = 2 if "zero" in case else 1 << 30
+    xp, yp = tanh_grid(p.ny, p.nz, p.ly, p.lz, cfg.gamma)
+    ic = ekman_initial(p, yp, cfg.seed, cfg.noise, cfg.th_noise)
+    o = O.Oracle(p, xp, yp)
+    s = S.DuctSolver(p, xp, yp)
+    o.set_initial_physical(*ic)
+    s.set_initial_physical(*ic)
+    for sub in range(6):
+        assert o.rk_duct() == 0
+        s.rk_duct()
+        r = _state_rel(s.get_state(), o.get_state())
+        print(f"[sponge {case}] substep {sub}: {max(r):.2e}", flush=True)
+        assert max(r) < 1e-10
+    s.close()
+
+
 def test_x_statistics_match_oracle(oracle_lib):
     """The x statistics of SAVE_STATS_CURVI (ekm_get_xstats): the GPU forms
     them from the Fourier state by Parseval, the oracle by direct sums over

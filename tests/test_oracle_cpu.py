@@ -759,3 +759,133 @@ def test_x_statistics_single_mode_closed_form(oracle_lib):
     assert np.allclose(st["vrms"], 2 * nx * abs(b) ** 2, rtol=1e-13, atol=0)
     assert np.allclose(st["uv"], 2 * (a * np.conj(b)).real, rtol=1e-13, atol=0)
     assert np.abs(st["wrms"]).max() == 0 and np.abs(st["uw"]).max() == 0
+
+
+# ---------------------------------------------------------------------------
+# sponge_th (src/duct.F90:3196-3362) and its profile (src/diablo.F90:1533-1556)
+# ---------------------------------------------------------------------------
+def _sponge_small(sponge_type=2, ny_s_t=None, ri_tau=0.0):
+    """A 16 x 33 x 17 stratified-config case with a vertical sponge on the top
+    rows, the plain projection, and the z rows of config 2 (not type 6, so the
+    sponge's type-6 branch stays off); ri_tau = 0 keeps the scalar passive."""
+    from ekman_bl_amd.params import mg_factors
+    cfg = config(2)
+    p = cfg.params
+    p.nx, p.ny, p.nz = 16, 31, 15
+    p.mg_ixp, p.mg_iex = mg_factors(p.nz + 2)
+    p.mg_jyq, p.mg_jey = mg_factors(p.ny + 2)
+    p.proj_mode = 0
+    p.ri_tau = ri_tau
+    p.sponge_type = sponge_type
+    p.ny_s_t = p.ny - 7 if ny_s_t is None else ny_s_t
+    p.nz_s_r = p.nz + 2
+    p.save_stats_int = 1 << 30
+    xp, yp = tanh_grid(p.ny, p.nz, p.ly, p.lz, cfg.gamma)
+    return cfg, p, xp, yp
+
+
+@pytest.mark.parametrize("sponge_type", [1, 2])
+def test_sponge_profile_closed_forms(sponge_type, oracle_lib):
+    """SPONGE_SIGMA is 0 below and at NY_S_T and 1 at NY+1 for both types;
+    type 1 is the square of the height above ypoint(NZ+1, NY_S_T) over the
+    span (the span passes through REAL(), single precision), so on a grid
+    whose y varies with K only the column K = NZ+1 reproduces it; type 2 has
+    sigma + 1/29 growing by the constant factor 30^(1/(NY+1-NY_S_T)) per row.
+    The library's host profile equals the oracle's bit for bit."""
+    from ekman_bl_amd import solver as S
+    O = oracle_lib
+    cfg, p, xp, yp = _sponge_small(sponge_type)
+    p.n_th = 1
+    jf = np.arange(p.ny + 2)[:, None] / (p.ny + 1)
+    yp = yp + 0.2 * p.ly * (xp / p.lz) * jf ** 2  # y depends on K, nonlinearly in J
+    o = O.Oracle(p, xp, yp)
+    sig = o.sponge_sigma()
+    nys, ny = p.ny_s_t, p.ny
+    assert np.all(sig[:nys + 1] == 0.0)
+    assert np.all(np.diff(sig[nys:]) > 0)
+    if sponge_type == 1:
+        assert abs(sig[ny + 1] - 1.0) < 5e-7
+        span = np.float32(yp[ny + 1, p.nz + 1] - yp[nys, p.nz + 1])
+        h = (yp[nys:, p.nz + 1] - yp[nys, p.nz + 1]) / float(span)
+        np.testing.assert_allclose(sig[nys:], h * h, rtol=1e-14, atol=0)
+        h0 = (yp[nys:, 0] - yp[nys, 0]) / (yp[ny + 1, 0] - yp[nys, 0])
+        assert np.max(np.abs(sig[nys:] - h0 * h0)) > 1e-3
+    else:
+        assert abs(sig[ny + 1] - 1.0) < 1e-14
+        ratio = (sig[nys + 1:] + 1 / 29) / (sig[nys:-1] + 1 / 29)
+        np.testing.assert_allclose(ratio, 30.0 ** (1.0 / (ny + 1 - nys)), rtol=1e-13)
+    host = S.host_sponge_sigma(sponge_type, p.nz, p.ny, nys, yp)
+    assert np.array_equal(host, sig)
+
+
+def _oracle_run(O, p, xp, yp, state=None, ic=None, substeps=1):
+    o = O.Oracle(p, xp, yp)
+    if state is not None:
+        o.set_state(*state)
+    else:
+        o.set_initial_physical(*ic)
+    for _ in range(substeps):
+        assert o.rk_duct() == 0
+    return o.get_state()
+
+
+def test_sponge_without_rows_is_the_identity(oracle_lib):
+    """With NY_S_T above NY+1 every SPONGE_SIGMA is zero and, without the
+    type-6 branch, sponge_th changes nothing: the run equals the run without
+    a sponge bit for bit (the mean-U3 average then covers J >= 2 NY/3)."""
+    O = oracle_lib
+    cfg, p, xp, yp = _sponge_small(2, ny_s_t=1 << 20)
+    ic = ekman_initial(p, yp, cfg.seed, cfg.noise, cfg.th_noise)
+    a = _oracle_run(O, p, xp, yp, ic=ic, substeps=3)
+    p.sponge_type = 0
+    b = _oracle_run(O, p, xp, yp, ic=ic, substeps=3)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_sponge_leaves_a_state_at_its_targets_alone(oracle_lib):
+    """A state whose fluctuations vanish and whose mean velocities equal the
+    sponge's targets (CU1 -> V_BC_YMAX_C1, CU2 -> U_BC_YMAX_C1, CU3 -> WBULK0,
+    src/duct.F90:3279-3281) is a fixed point of the damping: one substep
+    equals the substep without a sponge bit for bit.  Swapping two targets
+    breaks it (checked by moving U_BC_YMAX_C1)."""
+    from ekman_bl_amd.params import YMAX
+    O = oracle_lib
+    cfg, p, xp, yp = _sponge_small(1)
+    p.u_bc_c1[1][YMAX] = 0.3      # V_BC_YMAX_C1: target of CU1
+    p.u_bc_c1[0][YMAX] = -0.2     # U_BC_YMAX_C1: target of CU2
+    p.wbulk0 = 0.7
+    shape = (p.ny + 2, p.nz + 2, p.nx // 2 + 1)
+    fl = [np.zeros(shape, complex) for _ in range(5)]
+    fl[0][..., 0], fl[1][..., 0], fl[2][..., 0] = 0.3, -0.2, 0.7
+    fl[4][..., 0] = 0.01 * np.arange(p.ny + 2)[:, None]
+    a = _oracle_run(O, p, xp, yp, state=fl)
+    p.sponge_type = 0
+    b = _oracle_run(O, p, xp, yp, state=fl)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    p.sponge_type = 1
+    p.u_bc_c1[0][YMAX] = 0.3
+    c = _oracle_run(O, p, xp, yp, state=fl)
+    assert not np.array_equal(c[1], b[1])
+
+
+def test_sponge_damps_the_top_fluctuations(oracle_lib):
+    """Rayleigh damping with Sponge_Amp = 0.1/DELTA_T: after two steps the
+    fluctuation energy (modes >= 1) on the sponge rows is lower than without
+    the sponge, and the rows below NY_S_T - 4 are nearly untouched."""
+    O = oracle_lib
+    cfg, p, xp, yp = _sponge_small(2)
+    ic = ekman_initial(p, yp, cfg.seed, 0.2, cfg.th_noise)
+    a = _oracle_run(O, p, xp, yp, ic=ic, substeps=6)
+    p.sponge_type = 0
+    b = _oracle_run(O, p, xp, yp, ic=ic, substeps=6)
+    top = slice(p.ny_s_t + 2, p.ny + 1)
+    low = slice(1, p.ny_s_t - 4)
+    for f in range(3):
+        ea = np.sum(np.abs(a[f][top, :, 1:]) ** 2)
+        eb = np.sum(np.abs(b[f][top, :, 1:]) ** 2)
+        print(f"[sponge] field {f}: top fluctuation energy {ea:.3e} with, {eb:.3e} without", flush=True)
+        assert ea < 0.97 * eb
+    ra = np.sum(np.abs(a[0][low] - b[0][low]) ** 2) / np.sum(np.abs(b[0][low]) ** 2)
+    assert ra < 1e-3
