@@ -424,7 +424,8 @@ __device__ void smooth(const TDesc<T>& g, const TLev<T>& L, const Ws<T>& W, cons
 constexpr int kEw = 4;
 constexpr int kEwR = 2;   // the same for the residual (more loads per point)
 constexpr int kEw64 = 2;
-constexpr int kEwQ = 1;   // the same for A z of the Krylov step  // the same for the FP64 residual of the finest level
+constexpr int kEwQ = 1;   // the same for A z of the Krylov step (2 needs scratch)
+constexpr int kEwG = 2;   // the same for the Krylov step's update pass
 
 template <typename T, bool X>
 __device__ void residual(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ fac) {
@@ -618,18 +619,41 @@ __device__ __noinline__ double gcr_step(const TDesc<T>& g, const TLev<T>& F, dou
   const double alpha = qq > 0.0 ? (sm[2] - beta * sm[3]) / qq : 0.0;
   __syncthreads();  // every thread has read the previous value
   if (threadIdx.x == 0) sQQ = qq;
-  for (int t = threadIdx.x; t < P; t += blockDim.x) {
-    const double z = (double)ef[t] - (hasq ? beta * Zp[t] : 0.0);
-    phi[t] += alpha * z;
-    Zp[t] = z;
-  }
-  for (int p = threadIdx.x; p < n; p += blockDim.x) {
-    const double q2 = qv[p] - (hasq ? beta * Qp[p] : 0.0);
-    const double r2 = rr[p] - alpha * q2;
-    Qp[p] = q2;
-    rr[p] = r2;
-    rf[p] = (T)r2;
-    rmax = fmax(rmax, fabs(r2));
+  // one pass over the interior: phi and the kept direction (ring layout),
+  // then A z and the residual.  The rings of phi and of the direction are
+  // never visited: ef is zero there, so the update would keep them zero
+  // (phi's ring is cleared at the start; the direction's ring is never read).
+  for (int p0 = threadIdx.x; p0 < n; p0 += kEwG * NT) {
+    double zv[kEwG], zk[kEwG], ph[kEwG], qn[kEwG], qk[kEwG], rk[kEwG];
+    int qi[kEwG];
+#pragma unroll
+    for (int u = 0; u < kEwG; ++u) {
+      const int p = min(p0 + u * NT, n - 1);
+      const int j = p / nx, i = p - j * nx;
+      const int q = (i + 1) + ld * (j + 1);
+      qi[u] = q;
+      zv[u] = (double)ef[q];
+      zk[u] = hasq ? Zp[q] : 0.0;
+      ph[u] = phi[q];
+      qn[u] = qv[p];
+      qk[u] = hasq ? Qp[p] : 0.0;
+      rk[u] = rr[p];
+    }
+#pragma unroll
+    for (int u = 0; u < kEwG; ++u) {
+      const int p = p0 + u * NT;
+      if (p < n) {
+        const double z = zv[u] - (hasq ? beta * zk[u] : 0.0);
+        phi[qi[u]] = ph[u] + alpha * z;
+        Zp[qi[u]] = z;
+        const double q2 = qn[u] - (hasq ? beta * qk[u] : 0.0);
+        const double r2 = rk[u] - alpha * q2;
+        Qp[p] = q2;
+        rr[p] = r2;
+        rf[p] = (T)r2;
+        rmax = fmax(rmax, fabs(r2));
+      }
+    }
   }
   return rmax;
 }
