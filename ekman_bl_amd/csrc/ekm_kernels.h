@@ -86,5 +86,16 @@ struct SpongeArgs {
   double jst;           // INT_JACOB(ZEND+1, JEND+1)
 };
 int launch_sponge(const Dims& d, const Metrics& M, const SpongeArgs& a, hipStream_t st);
+// the mean-gradient statistics of SAVE_STATS_CURVI; see k_mean_grads
+struct MeanGradArgs {
+  const double2 *u1, *u2, *u3, *th;  // the Fourier state (th null without a scalar)
+  const double* thbar;               // TH_BAR plane, null with CONT_STRAT
+  Ranges rv;                         // the velocity ranges
+  double ri_tau;                     // RI_TAU(1)
+  double ke_scale;                   // 1, or (AMP_OMEGA0/OMEGA0)**(-2) when AMP_OMEGA0 != 0
+  int cont_strat;
+  double* out;                       // 8 planes of (NZ+2)*(NY+2), then THBULK
+};
+int launch_mean_grads(const Dims& d, const Metrics& M, const MeanGradArgs& a, hipStream_t st);
 int launch_thomas_generic(const double* A, const double* B, const double* C, double* G, double* W, int ny,
                           int nx, hipStream_t st);

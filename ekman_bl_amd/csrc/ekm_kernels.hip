@@ -1050,6 +1050,101 @@ __global__ void __launch_bounds__(1024) k_sponge_mean(Dims d, Metrics M, SpongeA
   }
 }
 
+// The mean-gradient statistics of SAVE_STATS_CURVI (src/flow_statistics.F90)
+// from the x means (real parts of mode 0), one workgroup.  Planes (K fastest):
+// 0 dudy, 1 dwdy over ZSTART..ZEND x JSTART..JEND (:228-240), 2 dudz,
+// 3 dwdz (:250-260), 4 dthdy with the background gradient when CONT_STRAT
+// is off (:464-480), 5 dthdz (:495-503), 6 Rig (:520-551), 7 the mean-flow
+// kinetic energy TKE (:164-175); then THBULK (:517-553).  The derivatives are
+// the contravariant face metrics averaged to the cell (0.125 times four faces)
+// times index differences, over INT_JACOB; the reference's row and column
+// fills follow in its order.  Points it never writes stay zero.
+__device__ inline double cell_avg(const Dims& d, const double* c, int k, int j) {
+  return 0.125 * (((c[mf(d, k, j, 1)] + c[mf(d, k, j + 1, 1)]) + c[mf(d, k, j, 2)]) + c[mf(d, k + 1, j, 2)]);
+}
+__global__ void __launch_bounds__(1024) k_mean_grads(Dims d, Metrics M, MeanGradArgs a) {
+  __shared__ double red[1024];
+  const int NZ = d.NZ, NY = d.NY, NZ2 = d.NZ2;
+  const int pl = (int)d.plane;
+  double* o = a.out;
+  auto P = [&](int f, int k, int j) -> double& { return o[(size_t)f * pl + k + NZ2 * j]; };
+  auto m0 = [&](const double2* f, int k, int j) { return f[fo(d, 0, k, j)].x; };
+  for (int t = threadIdx.x; t < 8 * pl; t += blockDim.x) o[t] = 0.0;
+  __syncthreads();
+  for (int t = threadIdx.x; t < pl; t += blockDim.x) {
+    const int k = t % NZ2, j = t / NZ2;
+    const double u1 = m0(a.u1, k, j), u2 = m0(a.u2, k, j), u3 = m0(a.u3, k, j);
+    P(7, k, j) = ((u1 * u1 + u2 * u2) + u3 * u3) * a.ke_scale;
+    if (k >= 1 && k <= NZ && j >= 1 && j <= NY) {
+      const double Jc = JAC(k, j);
+      const double a11 = cell_avg(d, M.c11, k, j), a12 = cell_avg(d, M.c12, k, j);
+      const double a21 = cell_avg(d, M.c21, k, j), a22 = cell_avg(d, M.c22, k, j);
+      if (in(k, a.rv.zs, a.rv.ze) && in(j, a.rv.js, a.rv.je)) {
+        P(0, k, j) = (a12 * (m0(a.u1, k + 1, j) - m0(a.u1, k - 1, j)) + a22 * (m0(a.u1, k, j + 1) - m0(a.u1, k, j - 1))) / Jc;
+        P(1, k, j) = (a12 * (m0(a.u3, k + 1, j) - m0(a.u3, k - 1, j)) + a22 * (m0(a.u3, k, j + 1) - m0(a.u3, k, j - 1))) / Jc;
+      }
+      P(2, k, j) = (a21 * (m0(a.u1, k, j + 1) - m0(a.u1, k, j - 1)) + a11 * (m0(a.u1, k + 1, j) - m0(a.u1, k - 1, j))) / Jc;
+      P(3, k, j) = (a21 * (m0(a.u3, k, j + 1) - m0(a.u3, k, j - 1)) + a11 * (m0(a.u3, k + 1, j) - m0(a.u3, k - 1, j))) / Jc;
+      if (a.th) {
+        double g = (a12 * (m0(a.th, k + 1, j) - m0(a.th, k - 1, j)) + a22 * (m0(a.th, k, j + 1) - m0(a.th, k, j - 1))) / Jc;
+        if (a.thbar) {
+          const double* T = a.thbar;
+          const int q = k + NZ2 * j;
+          g = g + (a12 * (T[q + 1] - T[q - 1]) + a22 * (T[q + NZ2] - T[q - NZ2])) / Jc;
+        }
+        P(4, k, j) = g;
+        P(5, k, j) = (a21 * (m0(a.th, k, j + 1) - m0(a.th, k, j - 1)) + a11 * (m0(a.th, k + 1, j) - m0(a.th, k - 1, j))) / Jc;
+      }
+    }
+  }
+  __syncthreads();
+  // rows 0 and NY+1 of the velocity gradients (:242-249, :262-267); columns 0
+  // and NZ+1 of the scalar gradients (:482-486, :505-508)
+  for (int k = 1 + (int)threadIdx.x; k <= NZ; k += blockDim.x)
+    for (int f = 0; f < 4; ++f) {
+      P(f, k, 0) = P(f, k, 1);
+      P(f, k, NY + 1) = P(f, k, NY);
+    }
+  if (a.th)
+    for (int j = 1 + (int)threadIdx.x; j <= NY; j += blockDim.x)
+      for (int f = 4; f < 6; ++f) {
+        P(f, 0, j) = P(f, 1, j);
+        P(f, NZ + 1, j) = P(f, NZ, j);
+      }
+  __syncthreads();
+  // column 1 of dudz, dwdz (:269-272); rows 0 and NY+1 of the scalar
+  // gradients, extrapolated (:488-491, :510-513)
+  for (int j = 1 + (int)threadIdx.x; j <= NY; j += blockDim.x) {
+    P(3, 1, j) = P(3, 2, j);
+    P(2, 1, j) = P(2, 2, j);
+  }
+  if (a.th)
+    for (int k = threadIdx.x; k <= NZ + 1; k += blockDim.x)
+      for (int f = 4; f < 6; ++f) {
+        P(f, k, 0) = 2.0 * P(f, k, 1) - P(f, k, 2);
+        P(f, k, NY + 1) = 2.0 * P(f, k, NY) - P(f, k, NY - 1);
+      }
+  __syncthreads();
+  // gradient Richardson number and THBULK (:517-553)
+  double acc = 0.0;
+  if (a.th) {
+    for (int t = threadIdx.x; t < NZ * NY; t += blockDim.x) {
+      const int k = 1 + t % NZ, j = 1 + t / NZ;
+      acc += 0.25 * (((m0(a.th, k, j) + m0(a.th, k - 1, j)) + m0(a.th, k - 1, j - 1)) + m0(a.th, k, j - 1));
+      const double gth = a.cont_strat ? P(4, k, j) - 1.0 : P(4, k, j);
+      const double dw = P(1, k, j), du = P(0, k, j);
+      P(6, k, j) = dw == 0.0 ? -(a.ri_tau * gth) * 1.0e18 : -(a.ri_tau * gth) / (dw * dw + du * du);
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) o[(size_t)8 * pl] = a.th ? red[0] / (double)(float)(NY * NZ) : 0.0;
+}
+
 // divergence diagnostic (:2790-2800): sum and max of |div| over
 // J = 2..NY-1, K = 2..NZ-1, I = 1..NX+1 of the physical field
 __global__ void k_div_reduce(Dims d, const double* __restrict__ S, double* __restrict__ out) {
@@ -1244,6 +1339,11 @@ int launch_sponge(const Dims& d, const Metrics& M, const SpongeArgs& a, hipStrea
     EKM_CHECK(hipGetLastError());
   }
   hipLaunchKernelGGL(k_sponge_mean, dim3(1), dim3(1024), 0, st, d, M, a);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_mean_grads(const Dims& d, const Metrics& M, const MeanGradArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_mean_grads, dim3(1), dim3(1024), 0, st, d, M, a);
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }

@@ -476,6 +476,28 @@ extern "C" int ekm_get_xstats(ekm_solver* s, double* out) {
   return st;
 }
 
+extern "C" int ekm_get_mean_grads(ekm_solver* s, double* out) {
+  if (!out) return EKM_ERR_ARG;
+  const ekm_params& p = s->p;
+  const size_t n = (8 * s->d.plane + 1) * sizeof(double);
+  MeanGradArgs a;
+  a.u1 = (const double2*)s->U[0];
+  a.u2 = (const double2*)s->U[1];
+  a.u3 = (const double2*)s->U[2];
+  a.th = p.n_th ? (const double2*)s->TH : nullptr;
+  a.thbar = (p.n_th && !p.cont_strat) ? s->thbar : nullptr;
+  a.rv = s->rv;
+  a.ri_tau = p.ri_tau;
+  a.ke_scale = p.amp_omega0 != 0.0 ? std::pow(p.amp_omega0 / p.omega0, -2.0) : 1.0;
+  a.cont_strat = p.cont_strat;
+  EKM_CHECK(hipMalloc((void**)&a.out, n));
+  int st = launch_mean_grads(s->d, s->M, a, s->st);
+  if (st == EKM_OK && hipMemcpyAsync(out, a.out, n, hipMemcpyDeviceToHost, s->st) != hipSuccess) st = EKM_ERR_HIP;
+  if (hipStreamSynchronize(s->st) != hipSuccess) st = EKM_ERR_HIP;
+  (void)hipFree(a.out);
+  return st;
+}
+
 extern "C" int ekm_get_forcing(ekm_solver* s, double* px0, double* qn) {
   if (px0) *px0 = s->p.f_type == 0 ? s->px0 : s->p.px0;
   if (qn) *qn = s->qn;

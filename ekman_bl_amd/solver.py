@@ -69,6 +69,7 @@ def lib():
         L.ekm_get_solve_cycles.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
         L.ekm_get_forcing.argtypes = [vp, _D, _D]
         L.ekm_get_xstats.argtypes = [vp, _D]
+        L.ekm_get_mean_grads.argtypes = [vp, _D]
         L.ekm_set_initial_physical.argtypes = [vp, _D, _D, _D, _D]
         L.ekm_get_phi.argtypes = [vp, _D]
         L.ekm_get_diag.argtypes = [vp, ctypes.POINTER(EkmDiag)]
@@ -181,6 +182,16 @@ class DuctSolver:
         out = np.zeros((16, p.ny + 2, p.nz + 2))
         _check(lib().ekm_get_xstats(self.h, _p(out)), "ekm_get_xstats")
         return dict(zip(self.XSTATS, out))
+
+    MEAN_GRADS = ("dudy", "dwdy", "dudz", "dwdz", "dthdy", "dthdz", "rig", "tke")
+
+    def mean_grads(self):
+        """SAVE_STATS_CURVI's mean gradients of the current state
+        (ekm_get_mean_grads): (dict name -> (NY+2, NZ+2), THBULK)."""
+        p = self.p
+        out = np.zeros(8 * (p.ny + 2) * (p.nz + 2) + 1)
+        _check(lib().ekm_get_mean_grads(self.h, _p(out)), "ekm_get_mean_grads")
+        return dict(zip(self.MEAN_GRADS, out[:-1].reshape(8, p.ny + 2, p.nz + 2))), out[-1]
 
     def forcing(self):
         """(PX0, Qn) of the F_TYPE 0 flow-rate controller."""

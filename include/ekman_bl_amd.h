@@ -190,6 +190,14 @@ int ekm_get_forcing(ekm_solver* s, double* px0, double* qn);
  * visit the two padding entries of the physical array, whose contents its FFT
  * leaves undefined). */
 int ekm_get_xstats(ekm_solver* s, double* out);
+/* The mean-gradient statistics of SAVE_STATS_CURVI from the x means of the
+ * current state (src/flow_statistics.F90:164-175, 228-275, 455-553): 8 planes
+ * of (NZ+2)*(NY+2) doubles (K fastest) -- 0 dudy, 1 dwdy, 2 dudz, 3 dwdz,
+ * 4 dthdy (with the background gradient when cont_strat = 0), 5 dthdz, 6 the
+ * gradient Richardson number Rig, 7 the mean-flow kinetic energy TKE -- then
+ * THBULK (1 double).  The reference's row/column fills are applied; points it
+ * never writes are zero; the scalar planes are zero when N_TH = 0. */
+int ekm_get_mean_grads(ekm_solver* s, double* out);
 /* Blocks until queued work on the solver's stream is done. */
 int ekm_sync(ekm_solver* s);
 /* Device pointer of state field f (0=U1,1=U2,2=U3,3=P,4=TH); the field's

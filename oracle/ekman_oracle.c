@@ -2473,6 +2473,103 @@ int ora_xstats(ora* o, double* out) {
   return EKM_OK;
 }
 
+/* SAVE_STATS_CURVI's mean gradients, loop by loop (src/flow_statistics.F90:
+ * 164-175 TKE, 228-249 dudy/dwdy, 250-272 dudz/dwdz, 464-491 dthdy, 495-513
+ * dthdz, 517-553 THBULK and Rig), on the real parts of mode 0 */
+int ora_mean_grads(ora* o, double* out) {
+  const ekm_params* p = &o->p;
+  const int NZ = o->NZ, NY = o->NY;
+  const size_t pl = (size_t)o->NZ2 * o->NY2;
+  memset(out, 0, (8 * pl + 1) * sizeof(double));
+  double *dudy = out, *dwdy = out + pl, *dudz = out + 2 * pl, *dwdz = out + 3 * pl;
+  double *dthdy = out + 4 * pl, *dthdz = out + 5 * pl, *rig = out + 6 * pl, *tke = out + 7 * pl;
+#define M0(f, k, j) ((f)[CRE(o, 0, k, j)])
+#define A4(c, k, j) (0.125 * (c(o, k, j, 1) + c(o, k, j + 1, 1) + c(o, k, j, 2) + c(o, k + 1, j, 2)))
+  double *U1 = o->U[0], *U2 = o->U[1], *U3 = o->U[2], *TH = o->TH;
+  for (int k = 0; k <= NZ + 1; ++k)
+    for (int j = 0; j <= NY + 1; ++j) {
+      tke[PL(o, k, j)] = (M0(U1, k, j) * M0(U1, k, j) + M0(U2, k, j) * M0(U2, k, j)) + M0(U3, k, j) * M0(U3, k, j);
+      if (p->amp_omega0 != 0.0) tke[PL(o, k, j)] = tke[PL(o, k, j)] * pow(p->amp_omega0 / p->omega0, -2.0);
+    }
+  for (int k = o->zstart; k <= o->zend; ++k)
+    for (int j = o->jstart; j <= o->jend; ++j) {
+      dudy[PL(o, k, j)] = (A4(C12, k, j) * (M0(U1, k + 1, j) - M0(U1, k - 1, j)) +
+                           A4(C22, k, j) * (M0(U1, k, j + 1) - M0(U1, k, j - 1))) / JAC(o, k, j);
+      dwdy[PL(o, k, j)] = (A4(C12, k, j) * (M0(U3, k + 1, j) - M0(U3, k - 1, j)) +
+                           A4(C22, k, j) * (M0(U3, k, j + 1) - M0(U3, k, j - 1))) / JAC(o, k, j);
+    }
+  for (int k = 1; k <= NZ; ++k) {
+    dudy[PL(o, k, 0)] = dudy[PL(o, k, 1)];
+    dwdy[PL(o, k, 0)] = dwdy[PL(o, k, 1)];
+    dudy[PL(o, k, NY + 1)] = dudy[PL(o, k, NY)];
+    dwdy[PL(o, k, NY + 1)] = dwdy[PL(o, k, NY)];
+  }
+  for (int k = 1; k <= NZ; ++k)
+    for (int j = 1; j <= NY; ++j) {
+      dudz[PL(o, k, j)] = (A4(C21, k, j) * (M0(U1, k, j + 1) - M0(U1, k, j - 1)) +
+                           A4(C11, k, j) * (M0(U1, k + 1, j) - M0(U1, k - 1, j))) / JAC(o, k, j);
+      dwdz[PL(o, k, j)] = (A4(C21, k, j) * (M0(U3, k, j + 1) - M0(U3, k, j - 1)) +
+                           A4(C11, k, j) * (M0(U3, k + 1, j) - M0(U3, k - 1, j))) / JAC(o, k, j);
+    }
+  for (int k = 1; k <= NZ; ++k) {
+    dudz[PL(o, k, 0)] = dudz[PL(o, k, 1)];
+    dwdz[PL(o, k, 0)] = dwdz[PL(o, k, 1)];
+    dudz[PL(o, k, NY + 1)] = dudz[PL(o, k, NY)];
+    dwdz[PL(o, k, NY + 1)] = dwdz[PL(o, k, NY)];
+  }
+  for (int j = 1; j <= NY; ++j) {
+    dwdz[PL(o, 1, j)] = dwdz[PL(o, 2, j)];
+    dudz[PL(o, 1, j)] = dudz[PL(o, 2, j)];
+  }
+  if (p->n_th) {
+    const double* TB = o->thbar;
+    for (int k = 1; k <= NZ; ++k)
+      for (int j = 1; j <= NY; ++j) {
+        size_t q = PL(o, k, j);
+        dthdy[q] = (A4(C12, k, j) * (M0(TH, k + 1, j) - M0(TH, k - 1, j)) +
+                    A4(C22, k, j) * (M0(TH, k, j + 1) - M0(TH, k, j - 1))) / JAC(o, k, j);
+        if (!p->cont_strat)
+          dthdy[q] = dthdy[q] + (A4(C12, k, j) * (TB[PL(o, k + 1, j)] - TB[PL(o, k - 1, j)]) +
+                                 A4(C22, k, j) * (TB[PL(o, k, j + 1)] - TB[PL(o, k, j - 1)])) / JAC(o, k, j);
+      }
+    for (int j = 1; j <= NY; ++j) {
+      dthdy[PL(o, 0, j)] = dthdy[PL(o, 1, j)];
+      dthdy[PL(o, NZ + 1, j)] = dthdy[PL(o, NZ, j)];
+    }
+    for (int k = 0; k <= NZ + 1; ++k) {
+      dthdy[PL(o, k, 0)] = 2.0 * dthdy[PL(o, k, 1)] - dthdy[PL(o, k, 2)];
+      dthdy[PL(o, k, NY + 1)] = 2.0 * dthdy[PL(o, k, NY)] - dthdy[PL(o, k, NY - 1)];
+    }
+    for (int k = 1; k <= NZ; ++k)
+      for (int j = 1; j <= NY; ++j)
+        dthdz[PL(o, k, j)] = (A4(C21, k, j) * (M0(TH, k, j + 1) - M0(TH, k, j - 1)) +
+                              A4(C11, k, j) * (M0(TH, k + 1, j) - M0(TH, k - 1, j))) / JAC(o, k, j);
+    for (int j = 1; j <= NY; ++j) {
+      dthdz[PL(o, 0, j)] = dthdz[PL(o, 1, j)];
+      dthdz[PL(o, NZ + 1, j)] = dthdz[PL(o, NZ, j)];
+    }
+    for (int k = 0; k <= NZ + 1; ++k) {
+      dthdz[PL(o, k, 0)] = 2.0 * dthdz[PL(o, k, 1)] - dthdz[PL(o, k, 2)];
+      dthdz[PL(o, k, NY + 1)] = 2.0 * dthdz[PL(o, k, NY)] - dthdz[PL(o, k, NY - 1)];
+    }
+    double ub = 0.0;
+    for (int j = 1; j <= NY; ++j)
+      for (int k = 1; k <= NZ; ++k) {
+        size_t q = PL(o, k, j);
+        ub = ub + 0.25 * (M0(TH, k, j) + M0(TH, k - 1, j) + M0(TH, k - 1, j - 1) + M0(TH, k, j - 1));
+        double g = p->cont_strat ? dthdy[q] - 1.0 : dthdy[q];
+        if (dwdy[q] == 0.0)
+          rig[q] = -p->ri_tau * g * 1.0e18;
+        else
+          rig[q] = -p->ri_tau * g / (dwdy[q] * dwdy[q] + dudy[q] * dudy[q]);
+      }
+    out[8 * pl] = ub / (double)(float)(NY * NZ);
+  }
+#undef M0
+#undef A4
+  return EKM_OK;
+}
+
 int ora_get_sponge(ora* o, double* sigma) {
   memcpy(sigma, o->sigma, ((size_t)o->NY + 2) * sizeof(double));
   return EKM_OK;

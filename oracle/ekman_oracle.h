@@ -41,6 +41,9 @@
  *                               explicit cross terms (les_xterm): parity unpinned
  *                               beyond the reference text (the GPU path is
  *                               checked against them)
+ *   ora_mean_grads              sheared affine grid: linear mean fields give
+ *                               their exact derivatives, Rig, TKE, the fills
+ *                               and THBULK in closed form
  *   sponge_sigma / sponge_th    SPONGE_SIGMA closed forms (0 at NY_S_T, 1 at
  *                               NY+1, type 1 on the K = NZ+1 column, type 2 a
  *                               constant growth factor); sigma = 0 and a state
@@ -83,6 +86,9 @@ int ora_get_forcing(ora* o, double* px0, double* qn);
 int ora_get_sponge(ora* o, double* sigma);
 /* x statistics (16 planes of (NZ+2)*(NY+2)), direct sums in physical space */
 int ora_xstats(ora* o, double* out);
+/* the mean-gradient statistics of SAVE_STATS_CURVI: 8 planes of
+ * (NZ+2)*(NY+2) (dudy, dwdy, dudz, dwdz, dthdy, dthdz, Rig, TKE), then THBULK */
+int ora_mean_grads(ora* o, double* out);
 /* one field's ADI (comp 0..2 velocity, 3 scalar) on physical U, R in place,
  * with the subgrid viscosity nut (physical P field) unless NULL */
 int ora_adi_field(ora* o, int comp, double* U, double* R, double t1, const double* nut, double t2);

@@ -51,6 +51,7 @@ def lib():
         L.ora_get_les_th.argtypes = [ctypes.c_void_p, _D, _D]
         L.ora_get_forcing.argtypes = [ctypes.c_void_p, _D, _D]
         L.ora_get_sponge.argtypes = [ctypes.c_void_p, _D]
+        L.ora_mean_grads.argtypes = [ctypes.c_void_p, _D]
         L.ora_xstats.argtypes = [ctypes.c_void_p, _D]
         L.ora_adi_field.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.c_double, _D, ctypes.c_double]
         L.ora_set_initial_physical.argtypes = [ctypes.c_void_p, _D, _D, _D, _D]
@@ -151,6 +152,15 @@ class Oracle:
         out = np.zeros((16, p.ny + 2, p.nz + 2))
         lib().ora_xstats(self.h, _p(out))
         return dict(zip(self.XSTATS, out))
+
+    MEAN_GRADS = ("dudy", "dwdy", "dudz", "dwdz", "dthdy", "dthdz", "rig", "tke")
+
+    def mean_grads(self):
+        """SAVE_STATS_CURVI's mean gradients: (dict name -> (NY+2, NZ+2), THBULK)."""
+        p = self.p
+        out = np.zeros(8 * (p.ny + 2) * (p.nz + 2) + 1)
+        lib().ora_mean_grads(self.h, _p(out))
+        return dict(zip(self.MEAN_GRADS, out[:-1].reshape(8, p.ny + 2, p.nz + 2))), out[-1]
 
     def sponge_sigma(self):
         """SPONGE_SIGMA(0:NY+1) of the vertical sponge."""

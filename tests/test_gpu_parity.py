@@ -403,6 +403,36 @@ def test_sponge_matches_oracle(case, oracle_lib):
     s.close()
 
 
+@pytest.mark.parametrize("cont_strat", [1, 0])
+def test_mean_gradients_match_oracle(cont_strat, oracle_lib):
+    """The mean-gradient statistics of SAVE_STATS_CURVI (ekm_get_mean_grads)
+    after one step of the small stratified case, on the sheared grid, against
+    the oracle's loop-by-loop restatement: every plane to 1e-12 of its scale
+    and THBULK to 1e-12.  cont_strat 0 adds the background gradient of a
+    non-uniform TH_BAR to dthdy."""
+    O = oracle_lib
+    cfg, p = _small_stratified()
+    p.cont_strat = cont_strat
+    xp, yp = tanh_grid(p.ny, p.nz, p.ly, p.lz, cfg.gamma)
+    xp = xp + 0.05 * np.sin(0.3 * yp)
+    tb = None if cont_strat else -yp + 0.1 * np.sin(xp)
+    ic = ekman_initial(p, yp, cfg.seed, cfg.noise, cfg.th_noise)
+    o = O.Oracle(p, xp, yp, th_bar=tb)
+    s = S.DuctSolver(p, xp, yp, th_bar=tb)
+    o.set_initial_physical(*ic)
+    s.set_initial_physical(*ic)
+    o.time_step(1)
+    s.time_step(1)
+    (mg, tg), (mo, to) = s.mean_grads(), o.mean_grads()
+    for name in S.DuctSolver.MEAN_GRADS:
+        scale = max(np.max(np.abs(mo[name])), 1e-300)
+        err = np.max(np.abs(mg[name] - mo[name])) / scale
+        print(f"[mean grads cont_strat={cont_strat}] {name}: {err:.2e} of {scale:.3e}", flush=True)
+        assert err < 1e-12, name
+    assert abs(tg - to) <= 1e-12 * max(1.0, abs(to))
+    s.close()
+
+
 def test_x_statistics_match_oracle(oracle_lib):
     """The x statistics of SAVE_STATS_CURVI (ekm_get_xstats): the GPU forms
     them from the Fourier state by Parseval, the oracle by direct sums over

@@ -889,3 +889,48 @@ def test_sponge_damps_the_top_fluctuations(oracle_lib):
         assert ea < 0.97 * eb
     ra = np.sum(np.abs(a[0][low] - b[0][low]) ** 2) / np.sum(np.abs(b[0][low]) ** 2)
     assert ra < 1e-3
+
+
+# ---------------------------------------------------------------------------
+# SAVE_STATS_CURVI mean gradients (src/flow_statistics.F90:164-175, 228-275, 455-553)
+# ---------------------------------------------------------------------------
+def test_mean_gradients_closed_forms(oracle_lib):
+    """On a sheared affine grid (z = dz K + s y, y = dy J) the cell-averaged
+    contravariant metrics times central index differences give the exact
+    derivatives of linear fields: with U1 = a y + b z, U3 = c y + e z and
+    theta = g y + h z the planes are dudy = a, dudz = b, dwdy = c, dwdz = e,
+    dthdy = g, dthdz = h (to the metrics' 1e-8 truncation); a swapped metric
+    pair (12 for 21) or a sign fails on the shear.  With CONT_STRAT,
+    Rig = -RI_TAU (g - 1)/(a^2 + c^2); TKE = U1^2 + U2^2 + U3^2; the scalar
+    rows extrapolate linearly (still g); THBULK is the mean over the cells of
+    theta at the cell corner (K-1/2, J-1/2)."""
+    O = oracle_lib
+    ny, nz, dy, dz, sh = 15, 15, 0.5, 0.25, 0.3
+    J = np.arange(ny + 2)[:, None] * np.ones((1, nz + 2))
+    K = np.ones((ny + 2, 1)) * np.arange(nz + 2)[None, :]
+    yp = dy * J
+    xp = dz * K + sh * yp
+    p = base_params(16, ny, nz, n_th=1)
+    p.cont_strat, p.ri_tau = 1, 0.04
+    o = O.Oracle(p, xp, yp)
+    a, b, c, e, g, h = 0.7, -0.4, 0.3, 0.9, -0.2, 0.5
+    F = [np.zeros((ny + 2, nz + 2, 9), complex) for _ in range(5)]
+    F[0][..., 0] = a * yp + b * xp
+    F[1][..., 0] = 0.1
+    F[2][..., 0] = c * yp + e * xp
+    F[4][..., 0] = g * yp + h * xp
+    o.set_state(*F)
+    m, thb = o.mean_grads()
+    inner = (slice(2, ny), slice(2, nz))
+    for name, v in (("dudy", a), ("dudz", b), ("dwdy", c), ("dwdz", e), ("dthdy", g), ("dthdz", h)):
+        np.testing.assert_allclose(m[name][inner], v, rtol=0, atol=1e-6, err_msg=name)
+    np.testing.assert_allclose(m["rig"][inner], -p.ri_tau * (g - 1.0) / (a * a + c * c), rtol=1e-5)
+    np.testing.assert_allclose(m["tke"], F[0][..., 0].real ** 2 + 0.01 + F[2][..., 0].real ** 2, rtol=1e-14)
+    # fills: velocity rows 0 and NY+1 copy rows 1 and NY; scalar rows extrapolate
+    assert np.array_equal(m["dudy"][0, 1:nz + 1], m["dudy"][1, 1:nz + 1])
+    assert np.array_equal(m["dwdz"][ny + 1, 2:nz + 1], m["dwdz"][ny, 2:nz + 1])  # column 1 is rewritten after
+    np.testing.assert_allclose(m["dthdy"][0, 2:nz], g, atol=1e-6)
+    np.testing.assert_allclose(m["dthdz"][ny + 1, 2:nz], h, atol=1e-6)
+    yc = dy * (np.arange(1, ny + 1) - 0.5)[:, None]
+    zc = dz * (np.arange(1, nz + 1) - 0.5)[None, :] + sh * yc
+    np.testing.assert_allclose(thb, np.mean(g * yc + h * zc), rtol=1e-12)
