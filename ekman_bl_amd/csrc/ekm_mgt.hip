@@ -695,7 +695,24 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
     if (guess != 0.0 && ii >= 0 && ii < nx && jj >= 0 && jj < ny && !F.spec[ii + nx * jj]) v = guess * pg[ii + nx * jj];
     phi[t] = v;
   }
-  for (int p = threadIdx.x; p < n; p += blockDim.x) b[p] = F.spec[p] ? 0.0 : -rt[p];
+  const int P = (nx + 2) * (ny + 2);
+  double* __restrict__ rr = b + n;  // krylov: the recursive residual (first iteration)
+  T* __restrict__ rf = W + F.r_off;
+  T* __restrict__ ef = W + F.e_off;
+  // From phi = 0 the first residual is b itself (A 0 adds only signed zeros),
+  // so it is formed here instead of by the residual pass; b is kept only
+  // where a later residual reads it.
+  const bool zero_start = guess == 0.0;
+  double rmax0 = 0.0;
+  for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    const double bv = F.spec[p] ? 0.0 : -rt[p];
+    if (!zero_start || !g.krylov) b[p] = bv;
+    if (zero_start) {
+      rmax0 = fmax(rmax0, fabs(bv));
+      rf[p] = (T)bv;
+      if (g.krylov) rr[p] = bv;
+    }
+  }
   // every level's correction starts at zero: the rings stay zero, the interiors
   // are rewritten by every sweep
   for (int l = 1; l <= ng; ++l) {
@@ -704,15 +721,13 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
     for (int t = threadIdx.x; t < (L.nx + 2) * (L.ny + 2); t += blockDim.x) el[t] = (T)0;
   }
   __syncthreads();
-  const int P = (nx + 2) * (ny + 2);
-  double* __restrict__ rr = b + n;  // krylov: the recursive residual (first iteration)
-  T* __restrict__ rf = W + F.r_off;
-  T* __restrict__ ef = W + F.e_off;
   const double* __restrict__ c = g.cf64;
   int it = 0;
   for (;;) {
     double rmax = 0.0;
-    if (!g.krylov || it == 0) {
+    if (it == 0 && zero_start) {
+      rmax = rmax0;
+    } else if (!g.krylov || it == 0) {
       // FP64 residual of the finest level; its max decides; it seeds the cycle
       for (int p0 = threadIdx.x; p0 < n; p0 += kEw64 * NT) {
         double v[kEw64];
