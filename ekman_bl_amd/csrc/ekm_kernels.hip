@@ -761,48 +761,75 @@ __global__ void k_faces(Dims d, Metrics M, Ranges r, const double2* __restrict__
 // RHS_TMP planes of the per-mode solves (:2527-2535 and :2620-2656): solve
 // s = 2m + part, part 0 the imaginary and part 1 the real component; the
 // divergence is formed on the fly from the face fluxes.
-__global__ void k_mg_gather(Dims d, Metrics M, Ranges r, const double* __restrict__ kx,
-                            const double2* __restrict__ CU1, const double2* __restrict__ F2,
-                            const double2* __restrict__ F3, double* __restrict__ rhs, int nmode) {
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t n = (size_t)nmode * d.NZ2 * d.NY2;
-  if (t >= n) return;
-  int m = (int)(t % nmode);
-  size_t rr = t / nmode;
-  int k = (int)(rr % d.NZ2), j = (int)(rr / d.NZ2);
-  double vi = 0.0, vr = 0.0;
-  if (in(j, r.js, r.je) && in(k, r.zs, r.ze)) {
-    size_t q = fo(d, m, k, j), dk = d.NXC, dj = (size_t)d.NXC * d.NZ2;
-    double kj = kx[m] * JAC(k, j);
-    double2 u = CU1[q], f2 = F2[q], f2p = F2[q + dj], f3 = F3[q], f3p = F3[q + dk];
-    double re = (0.0 * u.x - kj * u.y) + (f2p.x - f2.x) + (f3p.x - f3.x);
-    double im = (0.0 * u.y + kj * u.x) + (f2p.y - f2.y) + (f3p.y - f3.y);
-    vi = -im;
-    vr = -re;
+// Both transfers between the Fourier fields (modes contiguous) and the solve
+// planes (points contiguous) go through an LDS tile of 32 modes x 32 plane
+// points, so the loads and the stores of a wavefront are both contiguous.
+constexpr int kTileM = 32, kTileP = 32;
+__global__ void __launch_bounds__(256) k_mg_gather(Dims d, Metrics M, Ranges r, const double* __restrict__ kx,
+                                                   const double2* __restrict__ CU1, const double2* __restrict__ F2,
+                                                   const double2* __restrict__ F3, double* __restrict__ rhs,
+                                                   int nmode) {
+  __shared__ double2 tile[kTileP][kTileM + 1];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int m0 = blockIdx.x * kTileM;
+  const size_t p0 = (size_t)blockIdx.y * kTileP;
+  for (int pp = ty; pp < kTileP; pp += 8) {  // lanes along the modes
+    const size_t p = p0 + pp;
+    const int m = m0 + tx;
+    double vi = 0.0, vr = 0.0;
+    if (p < d.plane && m < nmode) {
+      const int k = (int)(p % d.NZ2), j = (int)(p / d.NZ2);
+      if (in(j, r.js, r.je) && in(k, r.zs, r.ze)) {
+        size_t q = fo(d, m, k, j), dk = d.NXC, dj = (size_t)d.NXC * d.NZ2;
+        double kj = kx[m] * JAC(k, j);
+        double2 u = CU1[q], f2 = F2[q], f2p = F2[q + dj], f3 = F3[q], f3p = F3[q + dk];
+        double re = (0.0 * u.x - kj * u.y) + (f2p.x - f2.x) + (f3p.x - f3.x);
+        double im = (0.0 * u.y + kj * u.x) + (f2p.y - f2.y) + (f3p.y - f3.y);
+        vi = -im;
+        vr = -re;
+      }
+    }
+    tile[pp][tx] = make_double2(vi, vr);
   }
-  size_t p = (size_t)k + (size_t)d.NZ2 * j;
-  rhs[(size_t)(2 * m) * d.plane + p] = vi;
-  rhs[(size_t)(2 * m + 1) * d.plane + p] = vr;
+  __syncthreads();
+  for (int mm = ty; mm < kTileM; mm += 8) {  // lanes along the plane points
+    const int m = m0 + mm;
+    const size_t p = p0 + tx;
+    if (m < nmode && p < d.plane) {
+      const double2 v = tile[tx][mm];
+      rhs[(size_t)(2 * m) * d.plane + p] = v.x;
+      rhs[(size_t)(2 * m + 1) * d.plane + p] = v.y;
+    }
+  }
 }
 
 // phi = CMPLX(P_TMP2, P_TMP) (:2662-2666), modes above NKX are zero
-__global__ void k_mg_scatter(Dims d, const double* __restrict__ phi, int nmode, int fp32,
-                             double2* __restrict__ PHI) {
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t n = (size_t)d.NXC * d.NZ2 * d.NY2;
-  if (t >= n) return;
-  int m = (int)(t % d.NXC);
-  size_t p = t / d.NXC;  // k + NZ2*j
-  double2 v = make_double2(0.0, 0.0);
-  if (m < nmode) {
-    v.x = phi[(size_t)(2 * m + 1) * d.plane + p];
-    v.y = phi[(size_t)(2 * m) * d.plane + p];
-    if (fp32) {
-      v.x = (double)(float)v.x;
-      v.y = (double)(float)v.y;
+__global__ void __launch_bounds__(256) k_mg_scatter(Dims d, const double* __restrict__ phi, int nmode, int fp32,
+                                                    double2* __restrict__ PHI) {
+  __shared__ double2 tile[kTileM][kTileP + 1];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int m0 = blockIdx.x * kTileM;
+  const size_t p0 = (size_t)blockIdx.y * kTileP;
+  for (int mm = ty; mm < kTileM; mm += 8) {  // lanes along the plane points
+    const int m = m0 + mm;
+    const size_t p = p0 + tx;
+    double2 v = make_double2(0.0, 0.0);
+    if (m < nmode && p < d.plane) {
+      v.x = phi[(size_t)(2 * m + 1) * d.plane + p];
+      v.y = phi[(size_t)(2 * m) * d.plane + p];
+      if (fp32) {
+        v.x = (double)(float)v.x;
+        v.y = (double)(float)v.y;
+      }
     }
+    tile[mm][tx] = v;
   }
-  PHI[t] = v;
+  __syncthreads();
+  for (int pp = ty; pp < kTileP; pp += 8) {  // lanes along the modes; modes above NKX get zero
+    const int m = m0 + tx;
+    const size_t p = p0 + pp;
+    if (m < d.NXC && p < d.plane) PHI[(size_t)m + (size_t)d.NXC * p] = tile[tx][pp];
+  }
 }
 
 __device__ inline double2 face2(const Dims& d, const Metrics& M, const Ranges& r, const double2* F2,
@@ -1289,15 +1316,15 @@ int launch_faces(const Dims& d, const Metrics& M, const Ranges& r, const double*
 }
 int launch_mg_gather(const Dims& d, const Metrics& M, const Ranges& r, const double* kx, const double* CU1,
                      const double* F2, const double* F3, double* rhs, int nmode, hipStream_t st) {
-  size_t n = (size_t)nmode * d.NZ2 * d.NY2;
-  hipLaunchKernelGGL(k_mg_gather, dim3(nblk(n, 256)), dim3(256), 0, st, d, M, r, kx, (const double2*)CU1,
+  const dim3 grid((unsigned)((nmode + kTileM - 1) / kTileM), (unsigned)((d.plane + kTileP - 1) / kTileP));
+  hipLaunchKernelGGL(k_mg_gather, grid, dim3(256), 0, st, d, M, r, kx, (const double2*)CU1,
                      (const double2*)F2, (const double2*)F3, rhs, nmode);
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }
 int launch_mg_scatter(const Dims& d, const double* phi, int nmode, int fp32, double* PHI, hipStream_t st) {
-  size_t n = (size_t)d.NXC * d.NZ2 * d.NY2;
-  hipLaunchKernelGGL(k_mg_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, d, phi, nmode, fp32, (double2*)PHI);
+  const dim3 grid((unsigned)((d.NXC + kTileM - 1) / kTileM), (unsigned)((d.plane + kTileP - 1) / kTileP));
+  hipLaunchKernelGGL(k_mg_scatter, grid, dim3(256), 0, st, d, phi, nmode, fp32, (double2*)PHI);
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }
