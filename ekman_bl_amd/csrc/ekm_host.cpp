@@ -416,6 +416,44 @@ void host_sponge_sigma(int type, int nz, int ny, int ny_s_t, const double* ypoin
   }
 }
 
+// SPONGE_SIGMA_OUT(K), K = 0..NZ+1 (src/diablo.F90:1489-1523): zero unless
+// NZ_S_R < NZ+1; from NZ_S_R up and from NZ_S_L down, parabolic (type 1) in
+// xpoint(K, NY+1) scaled by the right span, or exponential (type 2, co_b = 25)
+// in the index distance; the spans pass through REAL() (single precision).
+void host_sponge_sigma_out(int type, int nz, int ny, int nz_s_r, int nz_s_l, const double* xpoint, double* sig) {
+  for (int k = 0; k <= nz + 1; ++k) sig[k] = 0.0;
+  if (nz_s_r >= nz + 1 || nz_s_r < 0 || nz_s_l < 0 || nz_s_l > nz + 1) return;
+  auto x = [&](int k) { return xpoint[(size_t)k + (size_t)(nz + 2) * (ny + 1)]; };
+  if (type == 1) {
+    const double b = 1.0 / (double)(float)(x(nz + 1) - x(nz_s_r));
+    for (int k = 0; k <= nz + 1; ++k) {
+      if (k >= nz_s_r) {
+        const double xx = (x(k) - x(nz_s_r)) * b;
+        sig[k] = xx * xx;
+      } else if (k <= nz_s_l) {
+        const double xx = (x(nz_s_l) - x(k)) * b;
+        sig[k] = xx * xx;
+      }
+    }
+  } else if (type == 2) {
+    const double co_b = 25.0;
+    const double b = std::log(co_b) / (double)(float)(nz + 1 - nz_s_r);
+    for (int k = 0; k <= nz + 1; ++k) {
+      if (k >= nz_s_r)
+        sig[k] = (std::exp(b * (double)std::abs(k - nz_s_r)) - 1.0) / (co_b - 1.0);
+      else if (k <= nz_s_l)
+        sig[k] = (std::exp(b * (double)std::abs(nz_s_l - k)) - 1.0) / (co_b - 1.0);
+    }
+  }
+}
+
+extern "C" int ekm_host_sponge_sigma_out(int type, int nz, int ny, int nz_s_r, int nz_s_l, const double* xpoint,
+                                         double* sig) {
+  if (type < 0 || type > 2 || nz < 1 || ny < 1) return EKM_ERR_ARG;
+  host_sponge_sigma_out(type, nz, ny, nz_s_r, nz_s_l, xpoint, sig);
+  return EKM_OK;
+}
+
 extern "C" int ekm_host_sponge_sigma(int type, int nz, int ny, int ny_s_t, const double* ypoint, double* sig) {
   if (type < 0 || type > 2 || nz < 1 || ny < 1) return EKM_ERR_ARG;
   host_sponge_sigma(type, nz, ny, ny_s_t, ypoint, sig);

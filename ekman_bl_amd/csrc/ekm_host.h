@@ -24,3 +24,5 @@ int host_mg_level(const HostMetrics& hm, int NZ, int NY, const int bc[4], int nx
 int host_bc_table(const ekm_params& p, int comp, int dir, BcTable& t);
 // SPONGE_SIGMA(0:NY+1) of the vertical sponge (src/diablo.F90:1533-1556)
 void host_sponge_sigma(int type, int nz, int ny, int ny_s_t, const double* ypoint, double* sig);
+// SPONGE_SIGMA_OUT(0:NZ+1) of the streamwise sponge (src/diablo.F90:1489-1523)
+void host_sponge_sigma_out(int type, int nz, int ny, int nz_s_r, int nz_s_l, const double* xpoint, double* sig);

@@ -76,9 +76,11 @@ struct SpongeArgs {
   double2* cu[3];
   const double2* cth;
   const double* sigma;  // SPONGE_SIGMA(0:NY+1)
+  const double* sigma_out;  // SPONGE_SIGMA_OUT(0:NZ+1), null when the streamwise profile is off
   double amp;           // Sponge_Amp = 0.1 / DELTA_T
   Ranges r;             // the velocity ranges JSTART..JEND, ZSTART..ZEND
-  int j0;               // first row of the fluctuation damping (max(JSTART, NY_S_T))
+  int j0;               // first row of the fluctuation damping (max(JSTART, NY_S_T), or
+                        // JSTART with the streamwise profile)
   int ny_sponge;        // first row of the mean-U3 average
   double t1, t2, t3;    // mode-0 targets V_BC_YMAX_C1, U_BC_YMAX_C1, WBULK0
   int zmax6;            // type-6 z-max velocity rows with RI_TAU != 0

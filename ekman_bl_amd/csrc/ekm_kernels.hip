@@ -991,11 +991,11 @@ __global__ void __launch_bounds__(256) k_flow_rate(Dims d, Ranges r, const doubl
   if (threadIdx.x == 0) out[0] = red[0];
 }
 
-// sponge_th (src/duct.F90:3196-3362) with SPONGE_SIGMA_OUT = 0 (the
-// streamwise sponge off), so its inflow/outflow updates are the identity.
-// Fluctuations (:3228-3236), modes 1 and up: CF -= (Sponge_Amp SIGMA(J)) CU
-// INT_JACOB for the three velocities and the scalar.  SIGMA is zero below
-// NY_S_T, where the update is the identity, so only rows from j0 are visited.
+// sponge_th (src/duct.F90:3196-3362).  Fluctuations, modes 1 and up: CF -=
+// (Sponge_Amp SIGMA(J)) CU INT_JACOB for the three velocities and the scalar
+// (:3228-3236), then the same with SIGMA_OUT(K) when the streamwise profile is
+// on (:3240-3250).  Without it, SIGMA is zero below NY_S_T, where the update is
+// the identity, so only rows from j0 are visited.
 __global__ void k_sponge_fluct(Dims d, Metrics M, SpongeArgs a) {
   const int nm = d.NXC - 1, nk = a.r.ze - a.r.zs + 1;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1005,18 +1005,19 @@ __global__ void k_sponge_fluct(Dims d, Metrics M, SpongeArgs a) {
   const int k = a.r.zs + (int)(rr % nk), j = a.j0 + (int)(rr / nk);
   const size_t q = fo(d, m, k, j);
   const double Jc = JAC(k, j), as = a.amp * a.sigma[j];
-  for (int f = 0; f < 3; ++f) {
-    const double2 u = a.cu[f][q];
-    double2 c = a.cf[f][q];
+  const double ao = a.sigma_out ? a.amp * a.sigma_out[k] : 0.0;
+  for (int f = 0; f < 4; ++f) {
+    double2* cf = f < 3 ? a.cf[f] : a.cfth;
+    const double2 u = f < 3 ? a.cu[f][q] : a.cth[q];
+    double2 c = cf[q];
     c.x = c.x - (as * u.x) * Jc;
     c.y = c.y - (as * u.y) * Jc;
-    a.cf[f][q] = c;
+    if (a.sigma_out) {
+      c.x = c.x - (ao * u.x) * Jc;
+      c.y = c.y - (ao * u.y) * Jc;
+    }
+    cf[q] = c;
   }
-  const double2 u = a.cth[q];
-  double2 c = a.cfth[q];
-  c.x = c.x - (as * u.x) * Jc;
-  c.y = c.y - (as * u.y) * Jc;
-  a.cfth[q] = c;
 }
 
 // Mode 0 of sponge_th, one workgroup: the mean of Re CU3 over J = NY_sponge..
@@ -1025,7 +1026,9 @@ __global__ void k_sponge_fluct(Dims d, Metrics M, SpongeArgs a) {
 // scalar to zero, else only CU1 and CU2.  With type-6 z-max rows and
 // stratification (:3324-3332), CF2 of mode 0 relaxes on the whole plane with
 // the Jacobian at the exit indices of the loops above, and CU2X(0,:,:) is
-// zeroed on the steps where MOD(2*TIME_STEP, SAVE_STATS_INT) = 0.
+// zeroed on the steps where MOD(2*TIME_STEP, SAVE_STATS_INT) = 0.  Last, the
+// streamwise profile's inflow/outflow terms with twice the amplitude
+// (:3350-3356), on the possibly zeroed CU2.
 __global__ void __launch_bounds__(1024) k_sponge_mean(Dims d, Metrics M, SpongeArgs a) {
   __shared__ double red[1024];
   const int nk = a.r.ze - a.r.zs + 1, nj = a.r.je - a.ny_sponge + 1;
@@ -1064,16 +1067,33 @@ __global__ void __launch_bounds__(1024) k_sponge_mean(Dims d, Metrics M, SpongeA
       a.cfth[q] = ct;
     }
   }
-  if (!a.zmax6) return;
+  if (a.zmax6) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < (int)d.plane; t += blockDim.x) {
+      const size_t q = fo(d, 0, t % d.NZ2, t / d.NZ2);
+      const double2 u2 = a.cu[1][q];
+      double2 c2 = a.cf[1][q];
+      c2.x = c2.x - (a.amp * (u2.x - a.t2)) * a.jst;
+      c2.y = c2.y - (a.amp * u2.y) * a.jst;
+      a.cf[1][q] = c2;
+      if (a.zero_cu2) a.cu[1][q] = make_double2(0.0, 0.0);
+    }
+  }
+  if (!a.sigma_out) return;
   __syncthreads();
-  for (int t = threadIdx.x; t < (int)d.plane; t += blockDim.x) {
-    const size_t q = fo(d, 0, t % d.NZ2, t / d.NZ2);
-    const double2 u2 = a.cu[1][q];
-    double2 c2 = a.cf[1][q];
-    c2.x = c2.x - (a.amp * (u2.x - a.t2)) * a.jst;
-    c2.y = c2.y - (a.amp * u2.y) * a.jst;
-    a.cf[1][q] = c2;
-    if (a.zero_cu2) a.cu[1][q] = make_double2(0.0, 0.0);
+  for (int t = threadIdx.x; t < nk * nji; t += blockDim.x) {
+    const int k = a.r.zs + t % nk, j = a.r.js + t / nk;
+    const size_t q = fo(d, 0, k, j);
+    const double Jc = JAC(k, j), ao = (2.0 * a.amp) * a.sigma_out[k];
+    const double tg[4] = {0.0, 0.0, a.t3, 0.0};
+    for (int f = 0; f < 4; ++f) {
+      double2* cf = f < 3 ? a.cf[f] : a.cfth;
+      const double2 u = f < 3 ? a.cu[f][q] : a.cth[q];
+      double2 c = cf[q];
+      c.x = c.x - (ao * (u.x - tg[f])) * Jc;
+      c.y = c.y - (ao * u.y) * Jc;
+      cf[q] = c;
+    }
   }
 }
 

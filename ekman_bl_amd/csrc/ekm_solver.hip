@@ -78,6 +78,7 @@ struct ekm_solver {
   // INT_JACOB(ZEND+1, JEND+1), the Jacobian its type-6 branch reads
   double* sponge = nullptr;
   double sponge_jst = 0.0;
+  double* sponge_out = nullptr;  // SPONGE_SIGMA_OUT(0:NZ+1), when the streamwise profile is on
 };
 
 extern "C" {
@@ -105,6 +106,7 @@ void ekm_params_default(ekm_params* p) {
   p->proj_tol = 1.0e-12;
   p->proj_krylov = 1;
   p->ny_s_t = p->nz_s_r = p->save_stats_int = 1 << 30;  // no sponge rows
+  p->nz_s_l = -(1 << 30);
 }
 
 }  // extern "C"
@@ -130,12 +132,10 @@ static int check_params(const ekm_params& p) {
   if (p.les_model && p.n_th && !p.cont_strat) return EKM_ERR_UNSUPPORTED;
   if (p.sponge_type < 0 || p.sponge_type > 2) return EKM_ERR_UNSUPPORTED;
   if (p.sponge_type) {
-    // the streamwise sponge also scales the scalar's diffusion by
-    // 1 + 500 SPONGE_SIGMA_OUT (src/duct.F90:1354,1529-1536) and the type-8
-    // rows; that form is not supported
-    if (p.nz_s_r < p.nz + 1) return EKM_ERR_UNSUPPORTED;
-    // NY_S_T = NY+1 makes the profile 0/0 (src/diablo.F90:1538,1550)
+    // NY_S_T = NY+1 makes the vertical profile 0/0 (src/diablo.F90:1538,1550);
+    // the streamwise one (NZ_S_R < NZ+1) reads xpoint(NZ_S_R) and xpoint(NZ_S_L)
     if (p.ny_s_t < 0 || p.ny_s_t == p.ny + 1 || p.save_stats_int < 1) return EKM_ERR_ARG;
+    if (p.nz_s_r < p.nz + 1 && (p.nz_s_r < 0 || p.nz_s_l < 0 || p.nz_s_l > p.nz + 1)) return EKM_ERR_ARG;
   }
   if (p.proj_mode > 0) {
     // line smoothing is not cyclic, and the consistent system needs a
@@ -179,7 +179,7 @@ static void set_hbar(ekm_solver* s) {
 static int free_all(ekm_solver* s) {
   double* f[] = {s->U[0], s->U[1], s->U[2], s->P, s->TH, s->CF[0], s->CF[1], s->CF[2], s->CFTH,
                  s->R[0], s->R[1], s->R[2], s->RTH, s->S1, s->S2, s->U2b, s->U3b, s->PHI, s->lu,
-                 s->red, s->met, s->kx, s->kx2, s->kx2s, s->thbar, s->lesmet, s->fdy, s->sponge};
+                 s->red, s->met, s->kx, s->kx2, s->kx2s, s->thbar, s->lesmet, s->fdy, s->sponge, s->sponge_out};
   for (double* q : f)
     if (q) (void)hipFree(q);
   if (s->adi_d) (void)hipFree(s->adi_d);
@@ -378,6 +378,13 @@ extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const doub
     if (hipMemcpy(s->sponge, sig.data(), d.NY2 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
       return fail(EKM_ERR_HIP);
     s->sponge_jst = s->hm.jac[(size_t)(s->rv.ze + 1) + (size_t)d.NZ3 * (s->rv.je + 1)];
+    if (p.nz_s_r < p.nz + 1) {
+      std::vector<double> so(d.NZ2);
+      host_sponge_sigma_out(p.sponge_type, d.NZ, d.NY, p.nz_s_r, p.nz_s_l, xpoint, so.data());
+      if (hipMalloc(&s->sponge_out, d.NZ2 * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
+      if (hipMemcpy(s->sponge_out, so.data(), d.NZ2 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(EKM_ERR_HIP);
+    }
   }
   if (p.f_type == 0) {
     std::vector<double> dy(d.plane, 0.0);
@@ -672,9 +679,10 @@ extern "C" int ekm_rk_duct(ekm_solver* s) {
     g.cfth = (double2*)s->CFTH;
     g.cth = (const double2*)s->TH;
     g.sigma = s->sponge;
+    g.sigma_out = s->sponge_out;
     g.amp = 0.1 / s->delta_t;
     g.r = s->rv;
-    g.j0 = std::max(s->rv.js, p.ny_s_t);
+    g.j0 = s->sponge_out ? s->rv.js : std::max(s->rv.js, p.ny_s_t);
     g.ny_sponge = p.ny_s_t > p.ny ? 2 * p.ny / 3 : p.ny_s_t;
     g.t1 = p.u_bc_c1[1][EKM_YMAX];
     g.t2 = p.u_bc_c1[0][EKM_YMAX];

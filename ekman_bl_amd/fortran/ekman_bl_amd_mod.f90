@@ -54,7 +54,7 @@ module ekman_bl_amd
      integer(c_int) :: ny_s_t
      integer(c_int) :: nz_s_r
      integer(c_int) :: save_stats_int
-     integer(c_int) :: reserved(1)
+     integer(c_int) :: nz_s_l
   end type ekm_params
 
   type, bind(c) :: ekm_diag

@@ -58,7 +58,7 @@ class EkmParams(ctypes.Structure):
         ("ny_s_t", ctypes.c_int),
         ("nz_s_r", ctypes.c_int),
         ("save_stats_int", ctypes.c_int),
-        ("reserved", ctypes.c_int * 1),
+        ("nz_s_l", ctypes.c_int),
     ]
 
     # -- convenience ---------------------------------------------------------
@@ -110,6 +110,7 @@ def base_params(nx: int, ny: int, nz: int, n_th: int = 0) -> EkmParams:
         p.mudpack_bc[s] = 2
     p.proj_maxit, p.proj_tol, p.proj_krylov = 50, 1.0e-12, 1   # used by proj_mode 1, 2
     p.ny_s_t = p.nz_s_r = p.save_stats_int = 1 << 30             # no sponge rows
+    p.nz_s_l = -(1 << 30)
     return p
 
 
@@ -279,7 +280,7 @@ def params_from_run_dir(grid_def: str, input_dat: str) -> EkmParams:
     p = base_params(g["nx"], g["ny"], g["nz"], n_th=g.get("n_th", 0))
     if "iixp" in g:
         p.mg_ixp, p.mg_jyq, p.mg_iex, p.mg_jey = g["iixp"], g["jjyq"], g["iiex"], g["jjey"]
-    for key in ("ny_s_t", "nz_s_r"):
+    for key in ("ny_s_t", "nz_s_r", "nz_s_l"):
         if key in g:
             setattr(p, key, g[key])
     read_input_dat(input_dat, p)

@@ -85,6 +85,7 @@ def lib():
         L.ekm_host_metrics.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D]
         L.ekm_host_les_metrics.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D]
         L.ekm_host_sponge_sigma.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D]
+        L.ekm_host_sponge_sigma_out.argtypes = [ctypes.c_int] * 5 + [_D, _D]
         L.ekm_host_mg_coefficients.argtypes = [ctypes.POINTER(EkmParams), _D, _D, ctypes.c_int, ctypes.c_double,
                                                _D, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         _lib = L
@@ -290,6 +291,16 @@ def host_sponge_sigma(sponge_type: int, nz: int, ny: int, ny_s_t: int, ypoint: n
     _check(lib().ekm_host_sponge_sigma(sponge_type, nz, ny, ny_s_t,
                                        _p(np.ascontiguousarray(ypoint, dtype=np.float64)), _p(out)),
            "ekm_host_sponge_sigma")
+    return out
+
+
+def host_sponge_sigma_out(sponge_type: int, nz: int, ny: int, nz_s_r: int, nz_s_l: int,
+                          xpoint: np.ndarray) -> np.ndarray:
+    """SPONGE_SIGMA_OUT(0:NZ+1) of the streamwise sponge (src/diablo.F90:1489-1523)."""
+    out = np.zeros(nz + 2)
+    _check(lib().ekm_host_sponge_sigma_out(sponge_type, nz, ny, nz_s_r, nz_s_l,
+                                           _p(np.ascontiguousarray(xpoint, dtype=np.float64)), _p(out)),
+           "ekm_host_sponge_sigma_out")
     return out
 
 

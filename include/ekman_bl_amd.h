@@ -94,12 +94,13 @@ typedef struct ekm_params {
                             * 2 = exponential Rayleigh damping (sponge_th, src/duct.F90:3196-3362,
                             * applied when N_TH = 1) */
   int ny_s_t;              /* NY_S_T (grid_def): first row J of the vertical sponge */
-  int nz_s_r;              /* NZ_S_R (grid_def): the streamwise sponge, which needs NZ_S_R >= NZ+1
-                            * (off) when sponge_type != 0 */
+  int nz_s_r;              /* NZ_S_R (grid_def): first K of the right streamwise sponge; the
+                            * streamwise profile is on when NZ_S_R < NZ+1 */
   int save_stats_int;      /* SAVE_STATS_INT (input.dat, src/diablo.F90:274); the sponge's
                             * stratified type-6 branch zeroes CU2X(0,:,:) when
                             * MOD(2*TIME_STEP, SAVE_STATS_INT) = 0 */
-  int reserved[1];
+  int nz_s_l;              /* NZ_S_L (grid_def, NZ+1-NZ_S_R in the shipped file): last K of the
+                            * left streamwise sponge */
 } ekm_params;
 
 /* Fill with the reference defaults that are not grid-size dependent. */
@@ -254,6 +255,10 @@ int ekm_host_les_metrics(int nz, int ny, const double* xpoint, const double* ypo
  * (replaces the profile part of JACOBI_TRANS, src/diablo.F90:1533-1556;
  * type 1 parabolic, 2 exponential, 0 zero). */
 int ekm_host_sponge_sigma(int type, int nz, int ny, int ny_s_t, const double* ypoint, double* sig);
+/* SPONGE_SIGMA_OUT(K), K = 0..NZ+1, of the streamwise sponge from
+ * xpoint(K, NY+1) (src/diablo.F90:1489-1523; zero unless NZ_S_R < NZ+1). */
+int ekm_host_sponge_sigma_out(int type, int nz, int ny, int nz_s_r, int nz_s_l, const double* xpoint,
+                              double* sig);
 int ekm_host_mg_coefficients(const ekm_params* p, const double* xpoint, const double* ypoint,
                              int lev, double kx2, double* cf, int* nx, int* ny);
 /* Message of the last HIP error seen by this thread. */

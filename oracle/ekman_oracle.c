@@ -72,8 +72,8 @@ struct ora {
    * pressure gradient PX0 and the previous flow rate Qn, set at start-up as
    * Q0 = Qn = WBULK0 (src/diablo.F90:359-361) */
   double px0, qn;
-  /* SPONGE_SIGMA (0:NY+1) of the vertical sponge (sponge_th) */
-  double* sigma;
+  /* SPONGE_SIGMA (0:NY+1) and SPONGE_SIGMA_OUT (0:NZ+1) of sponge_th */
+  double *sigma, *sigma_out;
 };
 
 #define IDX(o, i, k, j) ((size_t)(i) + (size_t)(o)->NX2 * ((size_t)(k) + (size_t)(o)->NZ2 * (size_t)(j)))
@@ -1491,8 +1491,8 @@ static int supported(const ekm_params* p) {
   /* sponge: the vertical profile only (the streamwise one, NZ_S_R < NZ+1,
    * also scales the scalar's diffusion and is not restated) */
   if (p->sponge_type < 0 || p->sponge_type > 2) return 0;
-  if (p->sponge_type && (p->nz_s_r < p->nz + 1 || p->ny_s_t < 0 || p->ny_s_t == p->ny + 1 || p->save_stats_int < 1))
-    return 0;
+  if (p->sponge_type && (p->ny_s_t < 0 || p->ny_s_t == p->ny + 1 || p->save_stats_int < 1)) return 0;
+  if (p->sponge_type && p->nz_s_r < p->nz + 1 && (p->nz_s_r < 0 || p->nz_s_l < 0 || p->nz_s_l > p->nz + 1)) return 0;
   if (p->proj_mode == 1) {
     int dir = 0;
     for (int s = 0; s < 4; ++s) {
@@ -1578,6 +1578,7 @@ ora* ora_create(const ekm_params* p, const double* xpoint, const double* ypoint,
   o->px0 = p->px0;
   o->qn = p->wbulk0;
   o->sigma = dalloc((size_t)o->NY + 2);
+  o->sigma_out = dalloc((size_t)o->NZ + 2);
   sponge_sigma(o);
   init_ranges(o);
   o->delta_t = p->delta_t;
@@ -1591,7 +1592,7 @@ ora* ora_create(const ekm_params* p, const double* xpoint, const double* ypoint,
 
 void ora_destroy(ora* o) {
   if (!o) return;
-  free(o->xp); free(o->yp); free(o->thbar); free(o->jac); free(o->kx); free(o->kx2); free(o->sigma);
+  free(o->xp); free(o->yp); free(o->thbar); free(o->jac); free(o->kx); free(o->kx2); free(o->sigma); free(o->sigma_out);
   for (int q = 0; q < NMET; ++q) free(o->met[q]);
   double* f[] = {o->U[0], o->U[1], o->U[2], o->P, o->TH, o->CF[0], o->CF[1], o->CF[2], o->CFTH,
                  o->R[0], o->R[1], o->R[2], o->RTH, o->S1, o->S2, o->U2b, o->U3b, o->CS1, o->PHI};
@@ -2575,6 +2576,11 @@ int ora_get_sponge(ora* o, double* sigma) {
   return EKM_OK;
 }
 
+int ora_get_sponge_out(ora* o, double* sigma_out) {
+  memcpy(sigma_out, o->sigma_out, ((size_t)o->NZ + 2) * sizeof(double));
+  return EKM_OK;
+}
+
 int ora_get_forcing(ora* o, double* px0, double* qn) {
   if (px0) *px0 = o->p.f_type == 0 ? o->px0 : o->p.px0;
   if (qn) *qn = o->qn;
@@ -2603,8 +2609,10 @@ static void sponge_sigma(ora* o) {
   const ekm_params* p = &o->p;
   const int NY = o->NY, nys = p->ny_s_t, kt = o->NZ + 1;
   for (int j = 0; j <= NY + 1; ++j) o->sigma[j] = 0.0;
-  if (!p->sponge_type || nys < 0 || nys > NY + 1) return;
-  if (p->sponge_type == 1) {
+  if (!p->sponge_type) return;
+  if (nys < 0 || nys > NY + 1) {
+    /* no vertical rows */
+  } else if (p->sponge_type == 1) {
     const double y0 = o->yp[PL(o, kt, nys)];
     const double b = 1.0 / (double)(float)(o->yp[PL(o, kt, NY + 1)] - y0);
     for (int j = nys; j <= NY + 1; ++j) {
@@ -2615,11 +2623,39 @@ static void sponge_sigma(ora* o) {
     const double co_b = 30.0, b = log(co_b) / (double)(float)(NY + 1 - nys);
     for (int j = nys; j <= NY + 1; ++j) o->sigma[j] = (exp(b * (double)(j - nys)) - 1.0) / (co_b - 1.0);
   }
+  /* SPONGE_SIGMA_OUT(K) (src/diablo.F90:1489-1523), from xpoint(K, NY+1) */
+  const int NZ = o->NZ, zr = p->nz_s_r, zl = p->nz_s_l;
+  for (int k = 0; k <= NZ + 1; ++k) o->sigma_out[k] = 0.0;
+  if (zr >= NZ + 1) return;
+#define XT(k) (o->xp[PL(o, k, NY + 1)])
+  if (p->sponge_type == 1) {
+    const double b = 1.0 / (double)(float)(XT(NZ + 1) - XT(zr));
+    for (int k = 0; k <= NZ + 1; ++k) {
+      double xx;
+      if (k >= zr) {
+        xx = (XT(k) - XT(zr)) * b;
+        o->sigma_out[k] = xx * xx;
+      } else if (k <= zl) {
+        xx = (XT(zl) - XT(k)) * b;
+        o->sigma_out[k] = xx * xx;
+      }
+    }
+  } else {
+    const double co_b = 25.0, b = log(co_b) / (double)(float)(NZ + 1 - zr);
+    for (int k = 0; k <= NZ + 1; ++k) {
+      if (k >= zr)
+        o->sigma_out[k] = (exp(b * (double)abs(k - zr)) - 1.0) / (co_b - 1.0);
+      else if (k <= zl)
+        o->sigma_out[k] = (exp(b * (double)abs(zl - k)) - 1.0) / (co_b - 1.0);
+    }
+  }
+#undef XT
 }
 
-/* sponge_th (src/duct.F90:3196-3362) for SPONGE_SIGMA_OUT = 0 (NZ_S_R >=
- * NZ+1), where its inflow/outflow updates are the identity.  DYNAMIC_SPONE
- * is .TRUE. and Sponge_Amp = 0.1/DELTA_T. */
+/* sponge_th (src/duct.F90:3196-3362): DYNAMIC_SPONE is .TRUE. and
+ * Sponge_Amp = 0.1/DELTA_T; with the streamwise profile off SPONGE_SIGMA_OUT
+ * is zero and its inflow/outflow updates are the identity.  (ART_VISCOSITY
+ * enters RK_DUCT only under WAVE_ABS, which is .FALSE., src/diablo.F90:1486.) */
 static void sponge_th(ora* o, int cur_step) {
   const ekm_params* p = &o->p;
   const int js = o->jstart, je = o->jend, zs = o->zstart, ze = o->zend;
@@ -2635,6 +2671,16 @@ static void sponge_th(ora* o, int cur_step) {
           for (int f = 0; f < 3; ++f)
             o->CF[f][q] = o->CF[f][q] - A * o->sigma[j] * (o->U[f][q] - 0.0) * JAC(o, k, j);
           CFT[q] = CFT[q] - A * o->sigma[j] * (CTH[q] - 0.0) * JAC(o, k, j);
+        }
+  /* inflow/outflow (:3240-3250), a separate loop */
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k)
+      for (int m = 1; m < o->NXC; ++m)
+        for (int c = 0; c < 2; ++c) {
+          const size_t q = CRE(o, m, k, j) + c;
+          for (int f = 0; f < 3; ++f)
+            o->CF[f][q] = o->CF[f][q] - A * o->sigma_out[k] * (o->U[f][q] - 0.0) * JAC(o, k, j);
+          CFT[q] = CFT[q] - A * o->sigma_out[k] * (CTH[q] - 0.0) * JAC(o, k, j);
         }
   /* mode 0 (rank 0): the mean of CU3 over the top rows (:3261-3276) */
   const int nysp = p->ny_s_t > o->NY ? 2 * o->NY / 3 : p->ny_s_t;
@@ -2684,6 +2730,20 @@ static void sponge_th(ora* o, int cur_step) {
           CU2[q + 1] = 0.0;
         }
   }
+  /* inflow/outflow of mode 0 (:3350-3356) */
+  for (int j = js; j <= je; ++j)
+    for (int k = zs; k <= ze; ++k) {
+      const size_t q = CRE(o, 0, k, j);
+      const double Jc = JAC(o, k, j);
+      CF1[q] = CF1[q] - 2 * A * o->sigma_out[k] * (CU1[q] - 0.0) * Jc;
+      CF1[q + 1] = CF1[q + 1] - 2 * A * o->sigma_out[k] * CU1[q + 1] * Jc;
+      CF2[q] = CF2[q] - 2 * A * o->sigma_out[k] * (CU2[q] - 0.0) * Jc;
+      CF2[q + 1] = CF2[q + 1] - 2 * A * o->sigma_out[k] * CU2[q + 1] * Jc;
+      CF3[q] = CF3[q] - 2 * A * o->sigma_out[k] * (CU3[q] - t3) * Jc;
+      CF3[q + 1] = CF3[q + 1] - 2 * A * o->sigma_out[k] * CU3[q + 1] * Jc;
+      CFT[q] = CFT[q] - 2 * A * o->sigma_out[k] * (CTH[q] - 0.0) * Jc;
+      CFT[q + 1] = CFT[q + 1] - 2 * A * o->sigma_out[k] * (CTH[q + 1] - 0.0) * Jc;
+    }
 }
 
 int ora_rk_duct(ora* o) {

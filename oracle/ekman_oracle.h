@@ -44,7 +44,8 @@
  *   ora_mean_grads              sheared affine grid: linear mean fields give
  *                               their exact derivatives, Rig, TKE, the fills
  *                               and THBULK in closed form
- *   sponge_sigma / sponge_th    SPONGE_SIGMA closed forms (0 at NY_S_T, 1 at
+ *   sponge_sigma / sponge_th    SPONGE_SIGMA and SPONGE_SIGMA_OUT closed forms;
+ *                               SPONGE_SIGMA: (0 at NY_S_T, 1 at
  *                               NY+1, type 1 on the K = NZ+1 column, type 2 a
  *                               constant growth factor); sigma = 0 and a state
  *                               at the targets leave the substep bit-identical
@@ -84,6 +85,8 @@ int ora_get_les_th(ora* o, double* kappa, double* cdyn_th);
 int ora_get_forcing(ora* o, double* px0, double* qn);
 /* SPONGE_SIGMA(0:NY+1) of the vertical sponge (zero without one) */
 int ora_get_sponge(ora* o, double* sigma);
+/* SPONGE_SIGMA_OUT(0:NZ+1) of the streamwise sponge (zero without one) */
+int ora_get_sponge_out(ora* o, double* sigma_out);
 /* x statistics (16 planes of (NZ+2)*(NY+2)), direct sums in physical space */
 int ora_xstats(ora* o, double* out);
 /* the mean-gradient statistics of SAVE_STATS_CURVI: 8 planes of

@@ -51,6 +51,7 @@ def lib():
         L.ora_get_les_th.argtypes = [ctypes.c_void_p, _D, _D]
         L.ora_get_forcing.argtypes = [ctypes.c_void_p, _D, _D]
         L.ora_get_sponge.argtypes = [ctypes.c_void_p, _D]
+        L.ora_get_sponge_out.argtypes = [ctypes.c_void_p, _D]
         L.ora_mean_grads.argtypes = [ctypes.c_void_p, _D]
         L.ora_xstats.argtypes = [ctypes.c_void_p, _D]
         L.ora_adi_field.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.c_double, _D, ctypes.c_double]
@@ -166,6 +167,12 @@ class Oracle:
         """SPONGE_SIGMA(0:NY+1) of the vertical sponge."""
         out = np.zeros(self.p.ny + 2)
         lib().ora_get_sponge(self.h, _p(out))
+        return out
+
+    def sponge_sigma_out(self):
+        """SPONGE_SIGMA_OUT(0:NZ+1) of the streamwise sponge."""
+        out = np.zeros(self.p.nz + 2)
+        lib().ora_get_sponge_out(self.h, _p(out))
         return out
 
     def forcing(self):

@@ -364,7 +364,7 @@ def test_boundary_rows_match_oracle(case, oracle_lib):
     s.close()
 
 
-@pytest.mark.parametrize("case", ["type2_zero_cu2", "type1"])
+@pytest.mark.parametrize("case", ["type2_zero_cu2", "type1", "type2_streamwise_zero_cu2"])
 def test_sponge_matches_oracle(case, oracle_lib):
     """sponge_th (src/duct.F90:3196-3362) on the GPU against the oracle, on the
     reference run's form (rundir_unst): type-6 z rows for the velocity and the
@@ -373,7 +373,9 @@ def test_sponge_matches_oracle(case, oracle_lib):
     mean-flow sponge is taken) and a vertical sponge on the top rows.
     type2_zero_cu2: SPONGE_TYPE 2 with SAVE_STATS_INT = 2, so the type-6
     branch also zeroes CU2X(0,:,:) in every substep; type1: SPONGE_TYPE 1,
-    without the zeroing.  Six substeps, fields to 1e-10 relative L2."""
+    without the zeroing; type2_streamwise_zero_cu2: also the streamwise profile
+    (NZ_S_R = NZ-5, NZ_S_L = NZ+1-NZ_S_R) and its inflow/outflow terms, which
+    act on mode 0 after the zeroing.  Six substeps, fields to 1e-10 relative L2."""
     from ekman_bl_amd.params import YMIN, ZMIN, ZMAX
     O = oracle_lib
     cfg, p = _small_stratified()
@@ -388,6 +390,9 @@ def test_sponge_matches_oracle(case, oracle_lib):
     p.ny_s_t = p.ny - 6
     p.nz_s_r = p.nz + 2
     p.save_stats_int = 2 if "zero" in case else 1 << 30
+    if "streamwise" in case:
+        p.nz_s_r = p.nz - 5
+        p.nz_s_l = p.nz + 1 - p.nz_s_r
     xp, yp = tanh_grid(p.ny, p.nz, p.ly, p.lz, cfg.gamma)
     ic = ekman_initial(p, yp, cfg.seed, cfg.noise, cfg.th_noise)
     o = O.Oracle(p, xp, yp)

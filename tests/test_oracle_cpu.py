@@ -934,3 +934,44 @@ def test_mean_gradients_closed_forms(oracle_lib):
     yc = dy * (np.arange(1, ny + 1) - 0.5)[:, None]
     zc = dz * (np.arange(1, nz + 1) - 0.5)[None, :] + sh * yc
     np.testing.assert_allclose(thb, np.mean(g * yc + h * zc), rtol=1e-12)
+
+
+@pytest.mark.parametrize("sponge_type", [1, 2])
+def test_sponge_out_profile_closed_forms(sponge_type, oracle_lib):
+    """SPONGE_SIGMA_OUT (src/diablo.F90:1489-1523): zero between NZ_S_L and
+    NZ_S_R, 0 at both and 1 at NZ+1; type 1 the squared distance along
+    xpoint(K, NY+1) over the right span (single precision), read on the top
+    row of a grid whose z varies with J, so on a uniform top row the two sides
+    mirror each other; type 2 a constant growth factor 25^(1/(NZ+1-NZ_S_R))
+    of sigma + 1/24 on both sides.  The library's host profile equals the
+    oracle's bit for bit; NZ_S_R >= NZ+1 turns it off."""
+    from ekman_bl_amd import solver as S
+    O = oracle_lib
+    cfg, p, xp, yp = _sponge_small(sponge_type)
+    p.n_th = 1
+    p.nz_s_r = p.nz - 4
+    p.nz_s_l = p.nz + 1 - p.nz_s_r
+    jf = np.arange(p.ny + 2)[:, None] / (p.ny + 1)
+    xp = xp + 0.3 * np.sin(2.0 * np.pi * jf) * (xp / p.lz) ** 2  # nonlinear in z except at J = 0, NY+1
+    o = O.Oracle(p, xp, yp)
+    so = o.sponge_sigma_out()
+    zr, zl, nz = p.nz_s_r, p.nz_s_l, p.nz
+    assert np.all(so[zl:zr + 1] == 0.0)
+    assert np.all(np.diff(so[zr:]) > 0) and np.all(np.diff(so[:zl + 1]) < 0)
+    if sponge_type == 1:
+        assert abs(so[nz + 1] - 1.0) < 5e-7
+        xt = xp[p.ny + 1]
+        span = float(np.float32(xt[nz + 1] - xt[zr]))
+        np.testing.assert_allclose(so[zr:], ((xt[zr:] - xt[zr]) / span) ** 2, rtol=1e-14, atol=0)
+        np.testing.assert_allclose(so[:zl + 1][::-1], so[zr:], rtol=1e-12, atol=1e-15)
+        xm = xp[p.ny // 2]
+        assert np.max(np.abs(so[zr:] - ((xm[zr:] - xm[zr]) / (xm[nz + 1] - xm[zr])) ** 2)) > 1e-4
+    else:
+        assert abs(so[nz + 1] - 1.0) < 1e-14
+        ratio = (so[zr + 1:] + 1 / 24) / (so[zr:-1] + 1 / 24)
+        np.testing.assert_allclose(ratio, 25.0 ** (1.0 / (nz + 1 - zr)), rtol=1e-13)
+        np.testing.assert_allclose(so[:zl + 1][::-1], so[zr:], rtol=1e-15)
+    host = S.host_sponge_sigma_out(sponge_type, p.nz, p.ny, zr, zl, xp)
+    assert np.array_equal(host, so)
+    p.nz_s_r = p.nz + 2
+    assert np.all(O.Oracle(p, xp, yp).sponge_sigma_out() == 0.0)
