@@ -3,8 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
  * algorithm (jortizta/ekman_bl: src/duct.F90, src/boundary.F90,
- * src/mg_solver2d.F90, src/mud2cr.f, src/mudcom.f, src/diablo.F90) used as the
- * checker of the HIP path.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * src/mg_solver2d.F90, src/mud2cr.f, src/mudcom.f, src/diablo.F90) against
+ * which the HIP path is tested.  Only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load it.  The product never links it.
  *
  * Pinning (what checks each function; "parity unpinned" marks a function no

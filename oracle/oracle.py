@@ -1,6 +1,6 @@
 """ctypes wrapper of the CPU oracle (oracle/libekman_oracle.so).
 
-TEST INFRASTRUCTURE ONLY: the checker of the HIP path.  Imported by tests/,
+TEST INFRASTRUCTURE ONLY: the oracle the HIP path is tested against.  Imported by tests/,
 __graft_entry__.smoke() and bench.py's cpu_baseline leg, nowhere else.
 """
 from __future__ import annotations
