@@ -83,6 +83,19 @@ __device__ inline LaneTw lane_twiddles(int lanep, int M, const double2* __restri
   return t;
 }
 
+// The value of lane (lane ^ H) for H = 1, 2: a quad_perm DPP move
+// ([1,0,3,2] = 0xB1, [2,3,0,1] = 0x4E) in the vector ALU, where a shuffle
+// would go through the LDS permute unit; the source lane is always inside
+// the quad, so the bound control never applies.
+template <int H>
+__device__ inline double quad_xor(double v) {
+  constexpr int ctrl = H == 1 ? 0xB1 : 0x4E;
+  const long long x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)x, ctrl, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), ctrl, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // dif<R> for GW = 64 with the lane-stage twiddles given
 template <int R>
 __device__ inline void dif64(double2 (&z)[R], int M, int lanep, const double2* __restrict__ tw, const LaneTw& lt) {
@@ -107,8 +120,16 @@ __device__ inline void dif64(double2 (&z)[R], int M, int lanep, const double2* _
     for (int r = 0; r < R; ++r) {
       const double2 mine = z[r];
       double2 other;
-      other.x = __shfl_xor(mine.x, h, 64);
-      other.y = __shfl_xor(mine.y, h, 64);
+      if (h == 2) {
+        other.x = quad_xor<2>(mine.x);
+        other.y = quad_xor<2>(mine.y);
+      } else if (h == 1) {
+        other.x = quad_xor<1>(mine.x);
+        other.y = quad_xor<1>(mine.y);
+      } else {
+        other.x = __shfl_xor(mine.x, h, 64);
+        other.y = __shfl_xor(mine.y, h, 64);
+      }
       z[r] = top ? cadd(mine, other) : cmul(csub(other, mine), lt.w[s]);
     }
   }
