@@ -99,5 +99,9 @@ struct MeanGradArgs {
   double* out;                       // 8 planes of (NZ+2)*(NY+2), then THBULK
 };
 int launch_mean_grads(const Dims& d, const Metrics& M, const MeanGradArgs& a, hipStream_t st);
+// the higher-order x moments (13 planes, see k_xmoments) from the physical
+// velocities U1p..U3p and the Fourier state's means
+int launch_xmoments(const Dims& d, const double* U1p, const double* U2p, const double* U3p, const double* U1,
+                    const double* U2, const double* U3, double* out, hipStream_t st);
 int launch_thomas_generic(const double* A, const double* B, const double* C, double* G, double* W, int ny,
                           int nx, hipStream_t st);

@@ -1192,6 +1192,39 @@ __global__ void __launch_bounds__(1024) k_mean_grads(Dims d, Metrics M, MeanGrad
   if (threadIdx.x == 0) o[(size_t)8 * pl] = a.th ? red[0] / (double)(float)(NY * NZ) : 0.0;
 }
 
+// high_oder_momentums_curvi (src/flow_statistics.F90:1026-1095): one
+// wavefront per (K,J) pencil sums, over the NX physical points, the powers of
+// the velocity fluctuations about the mode-0 means (real parts of the Fourier
+// state), divided by NX.  Planes: UU, VV, WW, UUU, VVV, WWW, UUUU, VVVV, WWWW,
+// UUV, WWV, UUUV, WWWV.
+__global__ void __launch_bounds__(256) k_xmoments(Dims d, const double* __restrict__ U1p,
+                                                  const double* __restrict__ U2p, const double* __restrict__ U3p,
+                                                  const double2* __restrict__ U1, const double2* __restrict__ U2,
+                                                  const double2* __restrict__ U3, double* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const size_t pen = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (pen >= d.plane) return;
+  const int k = (int)(pen % d.NZ2), j = (int)(pen / d.NZ2);
+  const double mu = U1[fo(d, 0, k, j)].x, mv = U2[fo(d, 0, k, j)].x, mw = U3[fo(d, 0, k, j)].x;
+  double s[13];
+#pragma unroll
+  for (int q = 0; q < 13; ++q) s[q] = 0.0;
+  for (int i = lane; i < d.NX; i += 64) {
+    const size_t q = ph(d, i, k, j);
+    const double u = U1p[q] - mu, v = U2p[q] - mv, w = U3p[q] - mw;
+    s[0] += u * u; s[1] += v * v; s[2] += w * w;
+    s[3] += u * u * u; s[4] += v * v * v; s[5] += w * w * w;
+    s[6] += (u * u) * (u * u); s[7] += (v * v) * (v * v); s[8] += (w * w) * (w * w);
+    s[9] += u * u * v; s[10] += w * w * v; s[11] += u * u * u * v; s[12] += w * w * w * v;
+  }
+#pragma unroll
+  for (int q = 0; q < 13; ++q) {
+    double v = s[q];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) out[(size_t)q * d.plane + pen] = v / (double)d.NX;
+  }
+}
+
 // divergence diagnostic (:2790-2800): sum and max of |div| over
 // J = 2..NY-1, K = 2..NZ-1, I = 1..NX+1 of the physical field
 __global__ void k_div_reduce(Dims d, const double* __restrict__ S, double* __restrict__ out) {
@@ -1391,6 +1424,13 @@ int launch_sponge(const Dims& d, const Metrics& M, const SpongeArgs& a, hipStrea
 }
 int launch_mean_grads(const Dims& d, const Metrics& M, const MeanGradArgs& a, hipStream_t st) {
   hipLaunchKernelGGL(k_mean_grads, dim3(1), dim3(1024), 0, st, d, M, a);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_xmoments(const Dims& d, const double* U1p, const double* U2p, const double* U3p, const double* U1,
+                    const double* U2, const double* U3, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_xmoments, dim3(nblk(d.plane, 4)), dim3(256), 0, st, d, U1p, U2p, U3p, (const double2*)U1,
+                     (const double2*)U2, (const double2*)U3, out);
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }

@@ -505,6 +505,27 @@ extern "C" int ekm_get_mean_grads(ekm_solver* s, double* out) {
   return st;
 }
 
+extern "C" int ekm_get_xmoments(ekm_solver* s, double* out) {
+  if (!out) return EKM_ERR_ARG;
+  const Dims& d = s->d;
+  const size_t n = 13 * d.plane * sizeof(double);
+  double* tmp = nullptr;  // three physical velocity fields, then the planes
+  EKM_CHECK(hipMalloc((void**)&tmp, 3 * d.npts * sizeof(double) + n));
+  double* o = tmp + 3 * d.npts;
+  int st = EKM_OK;
+  for (int f = 0; f < 3 && st == EKM_OK; ++f) {
+    if (hipMemcpyAsync(tmp + f * d.npts, s->U[f], d.npts * sizeof(double), hipMemcpyDeviceToDevice, s->st) != hipSuccess)
+      st = EKM_ERR_HIP;
+    else
+      st = inv(s, tmp + f * d.npts);
+  }
+  if (st == EKM_OK) st = launch_xmoments(d, tmp, tmp + d.npts, tmp + 2 * d.npts, s->U[0], s->U[1], s->U[2], o, s->st);
+  if (st == EKM_OK && hipMemcpyAsync(out, o, n, hipMemcpyDeviceToHost, s->st) != hipSuccess) st = EKM_ERR_HIP;
+  if (hipStreamSynchronize(s->st) != hipSuccess) st = EKM_ERR_HIP;
+  (void)hipFree(tmp);
+  return st;
+}
+
 extern "C" int ekm_get_forcing(ekm_solver* s, double* px0, double* qn) {
   if (px0) *px0 = s->p.f_type == 0 ? s->px0 : s->p.px0;
   if (qn) *qn = s->qn;

@@ -70,6 +70,7 @@ def lib():
         L.ekm_get_forcing.argtypes = [vp, _D, _D]
         L.ekm_get_xstats.argtypes = [vp, _D]
         L.ekm_get_mean_grads.argtypes = [vp, _D]
+        L.ekm_get_xmoments.argtypes = [vp, _D]
         L.ekm_set_initial_physical.argtypes = [vp, _D, _D, _D, _D]
         L.ekm_get_phi.argtypes = [vp, _D]
         L.ekm_get_diag.argtypes = [vp, ctypes.POINTER(EkmDiag)]
@@ -185,6 +186,14 @@ class DuctSolver:
         return dict(zip(self.XSTATS, out))
 
     MEAN_GRADS = ("dudy", "dwdy", "dudz", "dwdz", "dthdy", "dthdz", "rig", "tke")
+    XMOMENTS = ("uu", "vv", "ww", "uuu", "vvv", "www", "uuuu", "vvvv", "wwww", "uuv", "wwv", "uuuv", "wwwv")
+
+    def x_moments(self) -> dict:
+        """The higher-order x moments (ekm_get_xmoments): name -> (NY+2, NZ+2)."""
+        p = self.p
+        out = np.zeros((13, p.ny + 2, p.nz + 2))
+        _check(lib().ekm_get_xmoments(self.h, _p(out)), "ekm_get_xmoments")
+        return dict(zip(self.XMOMENTS, out))
 
     def mean_grads(self):
         """SAVE_STATS_CURVI's mean gradients of the current state

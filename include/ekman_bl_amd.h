@@ -199,6 +199,12 @@ int ekm_get_xstats(ekm_solver* s, double* out);
  * THBULK (1 double).  The reference's row/column fills are applied; points it
  * never writes are zero; the scalar planes are zero when N_TH = 0. */
 int ekm_get_mean_grads(ekm_solver* s, double* out);
+/* The higher-order x moments of high_oder_momentums_curvi
+ * (src/flow_statistics.F90:1026-1095): 13 planes of (NZ+2)*(NY+2) doubles
+ * (K fastest) -- UU, VV, WW, UUU, VVV, WWW, UUUU, VVVV, WWWW, UUV, WWV, UUUV,
+ * WWWV -- the x averages over the NX physical points of powers of the velocity
+ * fluctuations about the mode-0 means. */
+int ekm_get_xmoments(ekm_solver* s, double* out);
 /* Blocks until queued work on the solver's stream is done. */
 int ekm_sync(ekm_solver* s);
 /* Device pointer of state field f (0=U1,1=U2,2=U3,3=P,4=TH); the field's

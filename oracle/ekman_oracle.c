@@ -2571,6 +2571,37 @@ int ora_mean_grads(ora* o, double* out) {
   return EKM_OK;
 }
 
+/* high_oder_momentums_curvi (src/flow_statistics.F90:1026-1095): per (K,J)
+ * the x averages of powers of the velocity fluctuations about the mode-0
+ * means, in the order UU, VV, WW, UUU, VVV, WWW, UUUU, VVVV, WWWW, UUV, WWV,
+ * UUUV, WWWV; direct sums over the NX physical points */
+int ora_xmoments(ora* o, double* out) {
+  const size_t np = o->npts, pl = (size_t)o->NZ2 * o->NY2;
+  double* F[3];
+  for (int q = 0; q < 3; ++q) {
+    F[q] = dalloc(np);
+    memcpy(F[q], o->U[q], np * sizeof(double));
+    inv(o, F[q]);
+  }
+  const int NX = o->NX;
+  for (int j = 0; j < o->NY2; ++j)
+    for (int k = 0; k < o->NZ2; ++k) {
+      const double mu = o->U[0][CRE(o, 0, k, j)], mv = o->U[1][CRE(o, 0, k, j)], mw = o->U[2][CRE(o, 0, k, j)];
+      double s[13] = {0};
+      for (int i = 0; i < NX; ++i) {
+        const size_t q = IDX(o, i, k, j);
+        const double u = F[0][q] - mu, v = F[1][q] - mv, w = F[2][q] - mw;
+        s[0] += u * u; s[1] += v * v; s[2] += w * w;
+        s[3] += u * u * u; s[4] += v * v * v; s[5] += w * w * w;
+        s[6] += (u * u) * (u * u); s[7] += (v * v) * (v * v); s[8] += (w * w) * (w * w);
+        s[9] += u * u * v; s[10] += w * w * v; s[11] += u * u * u * v; s[12] += w * w * w * v;
+      }
+      for (int q = 0; q < 13; ++q) out[q * pl + PL(o, k, j)] = s[q] / (double)NX;
+    }
+  for (int q = 0; q < 3; ++q) free(F[q]);
+  return EKM_OK;
+}
+
 int ora_get_sponge(ora* o, double* sigma) {
   memcpy(sigma, o->sigma, ((size_t)o->NY + 2) * sizeof(double));
   return EKM_OK;

@@ -41,6 +41,8 @@
  *                               explicit cross terms (les_xterm): parity unpinned
  *                               beyond the reference text (the GPU path is
  *                               checked against them)
+ *   ora_xmoments                one x mode about non-zero means: the cos^n
+ *                               averages in closed form
  *   ora_mean_grads              sheared affine grid: linear mean fields give
  *                               their exact derivatives, Rig, TKE, the fills
  *                               and THBULK in closed form
@@ -92,6 +94,9 @@ int ora_xstats(ora* o, double* out);
 /* the mean-gradient statistics of SAVE_STATS_CURVI: 8 planes of
  * (NZ+2)*(NY+2) (dudy, dwdy, dudz, dwdz, dthdy, dthdz, Rig, TKE), then THBULK */
 int ora_mean_grads(ora* o, double* out);
+/* the higher-order x moments of the velocity fluctuations (13 planes of
+ * (NZ+2)*(NY+2): UU VV WW UUU VVV WWW UUUU VVVV WWWW UUV WWV UUUV WWWV) */
+int ora_xmoments(ora* o, double* out);
 /* one field's ADI (comp 0..2 velocity, 3 scalar) on physical U, R in place,
  * with the subgrid viscosity nut (physical P field) unless NULL */
 int ora_adi_field(ora* o, int comp, double* U, double* R, double t1, const double* nut, double t2);

@@ -53,6 +53,7 @@ def lib():
         L.ora_get_sponge.argtypes = [ctypes.c_void_p, _D]
         L.ora_get_sponge_out.argtypes = [ctypes.c_void_p, _D]
         L.ora_mean_grads.argtypes = [ctypes.c_void_p, _D]
+        L.ora_xmoments.argtypes = [ctypes.c_void_p, _D]
         L.ora_xstats.argtypes = [ctypes.c_void_p, _D]
         L.ora_adi_field.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.c_double, _D, ctypes.c_double]
         L.ora_set_initial_physical.argtypes = [ctypes.c_void_p, _D, _D, _D, _D]
@@ -155,6 +156,14 @@ class Oracle:
         return dict(zip(self.XSTATS, out))
 
     MEAN_GRADS = ("dudy", "dwdy", "dudz", "dwdz", "dthdy", "dthdz", "rig", "tke")
+    XMOMENTS = ("uu", "vv", "ww", "uuu", "vvv", "www", "uuuu", "vvvv", "wwww", "uuv", "wwv", "uuuv", "wwwv")
+
+    def x_moments(self):
+        """high_oder_momentums_curvi's x moments: name -> (NY+2, NZ+2)."""
+        p = self.p
+        out = np.zeros((13, p.ny + 2, p.nz + 2))
+        lib().ora_xmoments(self.h, _p(out))
+        return dict(zip(self.XMOMENTS, out))
 
     def mean_grads(self):
         """SAVE_STATS_CURVI's mean gradients: (dict name -> (NY+2, NZ+2), THBULK)."""

@@ -438,6 +438,29 @@ def test_mean_gradients_match_oracle(cont_strat, oracle_lib):
     s.close()
 
 
+def test_x_moments_match_oracle(oracle_lib):
+    """The higher-order x moments (ekm_get_xmoments: the GPU's wavefront sums
+    against the oracle's sequential ones) after one step of the small
+    stratified case: every plane to 1e-12 of its scale."""
+    O = oracle_lib
+    cfg, p = _small_stratified()
+    xp, yp = tanh_grid(p.ny, p.nz, p.ly, p.lz, cfg.gamma)
+    ic = ekman_initial(p, yp, cfg.seed, cfg.noise, cfg.th_noise)
+    o = O.Oracle(p, xp, yp)
+    s = S.DuctSolver(p, xp, yp)
+    o.set_initial_physical(*ic)
+    s.set_initial_physical(*ic)
+    o.time_step(1)
+    s.time_step(1)
+    xg, xo = s.x_moments(), o.x_moments()
+    for name in S.DuctSolver.XMOMENTS:
+        scale = max(np.max(np.abs(xo[name])), 1e-300)
+        err = np.max(np.abs(xg[name] - xo[name])) / scale
+        print(f"[x moments] {name}: {err:.2e} of {scale:.3e}", flush=True)
+        assert err < 1e-12, name
+    s.close()
+
+
 def test_x_statistics_match_oracle(oracle_lib):
     """The x statistics of SAVE_STATS_CURVI (ekm_get_xstats): the GPU forms
     them from the Fourier state by Parseval, the oracle by direct sums over

@@ -975,3 +975,31 @@ def test_sponge_out_profile_closed_forms(sponge_type, oracle_lib):
     assert np.array_equal(host, so)
     p.nz_s_r = p.nz + 2
     assert np.all(O.Oracle(p, xp, yp).sponge_sigma_out() == 0.0)
+
+
+def test_x_moments_single_mode_closed_forms(oracle_lib):
+    """high_oder_momentums_curvi (src/flow_statistics.F90:1026-1095) on a state
+    whose fluctuations are one x mode m: u' = a cos, v' = b cos, w' = c cos
+    (CU = amplitude/2 at mode m), means U, V, W on mode 0.  Averages over the
+    NX points: cos^2 -> 1/2, cos^3 -> 0, cos^4 -> 3/8; so UU = a^2/2,
+    UUUU = 3 a^4/8, UUU = UUV = 0, UUUV = 3 a^3 b/8, WWWV = 3 c^3 b/8.  The
+    means must be removed first (a mean left in fails UUU)."""
+    O = oracle_lib
+    p = base_params(16, 7, 7)
+    ny, nz = p.ny, p.nz
+    y = 0.5 * np.arange(ny + 2)
+    z = 0.25 * np.arange(nz + 2)
+    xp, yp = np.meshgrid(z, y)
+    o = O.Oracle(p, xp, yp)
+    a, b, c, m = 0.6, -0.3, 0.9, 3
+    F = [np.zeros((ny + 2, nz + 2, 9), complex) for _ in range(5)]
+    for f, (amp, mean) in enumerate(((a, 1.5), (b, -0.2), (c, 0.7))):
+        F[f][..., 0] = mean
+        F[f][..., m] = amp / 2
+    o.set_state(*F)
+    x = o.x_moments()
+    exp = {"uu": a * a / 2, "vv": b * b / 2, "ww": c * c / 2, "uuu": 0.0, "vvv": 0.0, "www": 0.0,
+           "uuuu": 3 * a ** 4 / 8, "vvvv": 3 * b ** 4 / 8, "wwww": 3 * c ** 4 / 8, "uuv": 0.0, "wwv": 0.0,
+           "uuuv": 3 * a ** 3 * b / 8, "wwwv": 3 * c ** 3 * b / 8}
+    for name, v in exp.items():
+        np.testing.assert_allclose(x[name], v, rtol=0, atol=1e-14, err_msg=name)
