@@ -117,6 +117,39 @@ module ekman_bl_amd
        real(c_double), intent(out) :: out(*)
      end function ekm_get_xstats
 
+     ! the mean-gradient statistics of SAVE_STATS_CURVI: 8 planes (dudy, dwdy,
+     ! dudz, dwdz, dthdy, dthdz, Rig, TKE) of (NZ+2)*(NY+2), then THBULK
+     integer(c_int) function ekm_get_mean_grads(s, out) bind(c, name="ekm_get_mean_grads")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: s
+       real(c_double), intent(out) :: out(*)
+     end function ekm_get_mean_grads
+
+     ! the higher-order x moments of high_oder_momentums_curvi: 13 planes
+     ! (UU VV WW UUU VVV WWW UUUU VVVV WWWW UUV WWV UUUV WWWV)
+     integer(c_int) function ekm_get_xmoments(s, out) bind(c, name="ekm_get_xmoments")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: s
+       real(c_double), intent(out) :: out(*)
+     end function ekm_get_xmoments
+
+     ! SPONGE_SIGMA(0:NY+1) and SPONGE_SIGMA_OUT(0:NZ+1) of sponge_th's profiles
+     integer(c_int) function ekm_host_sponge_sigma(stype, nz, ny, ny_s_t, ypoint, sig) &
+         bind(c, name="ekm_host_sponge_sigma")
+       import :: c_int, c_double
+       integer(c_int), value :: stype, nz, ny, ny_s_t
+       real(c_double), intent(in) :: ypoint(*)
+       real(c_double), intent(out) :: sig(*)
+     end function ekm_host_sponge_sigma
+
+     integer(c_int) function ekm_host_sponge_sigma_out(stype, nz, ny, nz_s_r, nz_s_l, xpoint, sig) &
+         bind(c, name="ekm_host_sponge_sigma_out")
+       import :: c_int, c_double
+       integer(c_int), value :: stype, nz, ny, nz_s_r, nz_s_l
+       real(c_double), intent(in) :: xpoint(*)
+       real(c_double), intent(out) :: sig(*)
+     end function ekm_host_sponge_sigma_out
+
      ! PX0 and Qn of the F_TYPE 0 flow-rate controller
      integer(c_int) function ekm_get_forcing(s, px0, qn) bind(c, name="ekm_get_forcing")
        import :: c_ptr, c_int, c_double
@@ -199,6 +232,109 @@ module ekman_bl_amd
        real(c_double), intent(out) :: phi(*)
        integer(c_int), intent(out) :: cycles(*)
      end function ekm_multigrid_mudpack
+
+     ! the projection's correction PHI (physical layout, (NX+2)*(NZ+2)*(NY+2))
+     integer(c_int) function ekm_get_phi(s, phi) bind(c, name="ekm_get_phi")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: s
+       real(c_double), intent(out) :: phi(*)
+     end function ekm_get_phi
+
+     ! device address of state field f (0..4: CU1, CU2, CU3, CP, CTH)
+     integer(c_int) function ekm_field_device_ptr(s, f, dptr) bind(c, name="ekm_field_device_ptr")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: s
+       integer(c_int), value :: f
+       type(c_ptr), intent(out) :: dptr
+     end function ekm_field_device_ptr
+
+     ! device memory and the device-pointer component drop-ins
+     integer(c_int) function ekm_dev_alloc(p, bytes) bind(c, name="ekm_dev_alloc")
+       import :: c_ptr, c_int, c_size_t
+       type(c_ptr), intent(out) :: p
+       integer(c_size_t), value :: bytes
+     end function ekm_dev_alloc
+
+     integer(c_int) function ekm_dev_free(p) bind(c, name="ekm_dev_free")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: p
+     end function ekm_dev_free
+
+     integer(c_int) function ekm_copy_dev(dst, src, n, stream) bind(c, name="ekm_copy_dev")
+       import :: c_ptr, c_int, c_size_t
+       type(c_ptr), value :: dst, src, stream
+       integer(c_size_t), value :: n
+     end function ekm_copy_dev
+
+     integer(c_int) function ekm_fft_x_to_fourier_dev(varp, nx, npencil, stream) &
+         bind(c, name="ekm_fft_x_to_fourier_dev")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: varp, stream
+       integer(c_int), value :: nx, npencil
+     end function ekm_fft_x_to_fourier_dev
+
+     integer(c_int) function ekm_fft_x_to_physical_dev(varp, nx, npencil, stream) &
+         bind(c, name="ekm_fft_x_to_physical_dev")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: varp, stream
+       integer(c_int), value :: nx, npencil
+     end function ekm_fft_x_to_physical_dev
+
+     integer(c_int) function ekm_thomas_real_dev(a, b, c, g, ny, nx, stream) bind(c, name="ekm_thomas_real_dev")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: a, b, c, g, stream
+       integer(c_int), value :: ny, nx
+     end function ekm_thomas_real_dev
+
+     ! host-side tables (no device needed)
+     integer(c_int) function ekm_host_metrics(nz, ny, xpoint, ypoint, out) bind(c, name="ekm_host_metrics")
+       import :: c_int, c_double
+       integer(c_int), value :: nz, ny
+       real(c_double), intent(in) :: xpoint(*), ypoint(*)
+       real(c_double), intent(out) :: out(*)
+     end function ekm_host_metrics
+
+     integer(c_int) function ekm_host_les_metrics(nz, ny, xpoint, ypoint, out) bind(c, name="ekm_host_les_metrics")
+       import :: c_int, c_double
+       integer(c_int), value :: nz, ny
+       real(c_double), intent(in) :: xpoint(*), ypoint(*)
+       real(c_double), intent(out) :: out(*)
+     end function ekm_host_les_metrics
+
+     integer(c_int) function ekm_host_mg_coefficients(p, xpoint, ypoint, lev, kx2, cf, nx, ny) &
+         bind(c, name="ekm_host_mg_coefficients")
+       import :: c_int, c_double, ekm_params
+       type(ekm_params), intent(in) :: p
+       real(c_double), intent(in) :: xpoint(*), ypoint(*)
+       integer(c_int), value :: lev
+       real(c_double), value :: kx2
+       real(c_double), intent(out) :: cf(*)
+       integer(c_int), intent(out) :: nx, ny
+     end function ekm_host_mg_coefficients
+
+     ! timing of the projection launches (HIP events on the solver's stream)
+     integer(c_int) function ekm_prof_enable(s, on) bind(c, name="ekm_prof_enable")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: s
+       integer(c_int), value :: on
+     end function ekm_prof_enable
+
+     integer(c_int) function ekm_prof_get(s, ms, launches, solves, cycles) bind(c, name="ekm_prof_get")
+       import :: c_ptr, c_int, c_double, c_long
+       type(c_ptr), value :: s
+       real(c_double), intent(out) :: ms
+       integer(c_long), intent(out) :: launches, solves, cycles
+     end function ekm_prof_get
+
+     ! C strings: the last error message and the version (convert with
+     ! c_f_pointer and the terminating NUL)
+     type(c_ptr) function ekm_last_error() bind(c, name="ekm_last_error")
+       import :: c_ptr
+     end function ekm_last_error
+
+     type(c_ptr) function ekm_version() bind(c, name="ekm_version")
+       import :: c_ptr
+     end function ekm_version
   end interface
 
 contains

@@ -153,3 +153,18 @@ def test_bench_finds_measured_traffic_of_headline_workload():
     cfg = config(2)
     tr = bench.measured_traffic(bench.workload_name(cfg, 2, 1e-12, 1))
     assert tr is not None and tr[0] > 1e9 and tr[1] > 0
+
+
+def test_fortran_module_binds_every_entry_point():
+    """Every extern "C" entry point that include/ekman_bl_amd.h declares has a
+    BIND(C) interface of the same name in the Fortran module the reference's
+    driver uses (ekman_bl_amd/fortran/ekman_bl_amd_mod.f90), and the module
+    names no symbol the header lacks."""
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    h = open(os.path.join(root, "include", "ekman_bl_amd.h")).read()
+    f = open(os.path.join(root, "ekman_bl_amd", "fortran", "ekman_bl_amd_mod.f90")).read()
+    decl = set(re.findall(r"\b(ekm_\w+)\s*\(", re.sub(r"/\*.*?\*/", "", h, flags=re.S)))
+    bound = set(re.findall(r'bind\(c,\s*name="(ekm_\w+)"\)', f))
+    assert decl - bound == set()
+    assert bound - decl == set()
