@@ -268,7 +268,7 @@ __global__ void __launch_bounds__(kBlock) k_flux_accum(Dims d, Metrics M, Ranges
                                                        const double* __restrict__ Q, const double* __restrict__ U2b,
                                                        const double* __restrict__ U3b, double nu,
                                                        const double* __restrict__ kxv, double2* __restrict__ CF,
-                                                       double2* __restrict__ CR, double temp5, int logm,
+                                                       double2* __restrict__ CR, const double* __restrict__ temp5p, int logm,
                                                        const double2* __restrict__ tw,
                                                        const double2* __restrict__ tw2,
                                                        const double* __restrict__ nut,
@@ -402,6 +402,7 @@ __global__ void __launch_bounds__(kBlock) k_flux_accum(Dims d, Metrics M, Ranges
     f.y = f.y + X[1].y;
     CF[f0 + m] = f;
     double2 c = cr[i];
+    const double temp5 = *temp5p;
     c.x = c.x + temp5 * f.x;
     c.y = c.y + temp5 * f.y;
     CR[f0 + m] = c;
@@ -412,7 +413,7 @@ __global__ void __launch_bounds__(kBlock) k_flux_accum(Dims d, Metrics M, Ranges
 template <int R>
 int launch_flux_accum_r(const FftPlan& p, const Dims& d, const Metrics& M, const Ranges& rg, const double* U1,
                         const double* Q, const double* U2b, const double* U3b, double nu, const double* kx,
-                        double* CF, double* CR, double temp5, const double* nut, const double* gl, hipStream_t st) {
+                        double* CF, double* CR, const double* temp5, const double* nut, const double* gl, hipStream_t st) {
   const int M2 = p.nx / 2, GW = M2 < 64 ? M2 : 64, PW = 64 / GW;
   const long npen = (long)(rg.ze - rg.zs + 1) * (rg.je - rg.js + 1);
   if (npen <= 0) return EKM_OK;
@@ -494,7 +495,7 @@ int fft_inverse(const FftPlan& p, double* v, long npencil, hipStream_t st) { ret
 
 int launch_flux_accum(const FftPlan& p, const Dims& d, const Metrics& M, const Ranges& r, const double* U1,
                       const double* Q, const double* U2b, const double* U3b, double nu, const double* kx, double* CF,
-                      double* CR, double temp5, hipStream_t st, const double* nut, const double* gl) {
+                      double* CR, const double* temp5, hipStream_t st, const double* nut, const double* gl) {
   const int M2 = p.nx / 2;
   if (nut && !gl) return EKM_ERR_ARG;
   if (M2 <= 64) return launch_flux_accum_r<1>(p, d, M, r, U1, Q, U2b, U3b, nu, kx, CF, CR, temp5, nut, gl, st);

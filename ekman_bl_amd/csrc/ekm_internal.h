@@ -51,6 +51,32 @@ struct Ranges {
   int js, je, zs, ze;
 };
 
+// Run state of a solver that changes from substep to substep, kept in device
+// memory and updated by one-thread control kernels (ekm_kernels.hip), so that
+// a substep needs no host round trip and can be replayed from a HIP graph:
+// DELTA_T and H_BAR after the CFL update (src/duct.F90:2417-2419), TIME, the
+// TEMP1..5 of the substep in flight (src/duct.F90:283-287), the mean pressure
+// gradient, the F_TYPE 0 controller state (src/duct.F90:414-436), the
+// reduction accumulators and the diagnostics of the last projection with a
+// sticky status word (the reference's STOPs, src/duct.F90:2811-2812).
+// Every kernel that reads one of these takes a pointer to this block.
+struct StepState {
+  double delta_t, delta_t_set, time;
+  double h_bar[3];
+  double hb, temp1, temp3, temp4, temp5;  // of the substep in flight
+  double px;         // PX0 (+ AMP_OMEGA0 cos(OMEGA0 TIME)) of the substep in flight
+  double amp;        // Sponge_Amp = 0.1 / DELTA_T
+  double guess;      // start of the consistent solve: 0 (from zero) or the scale of mg.phi
+  double hb_prev;    // H_BAR of the previous substep
+  double px0, qn;    // F_TYPE 0 controller
+  // reduction accumulators; each is reset by the kernel that consumes it:
+  // 0 sum |J div u|, 1 max |J div u| (bits), 2 CFL minimum (bits), 3 flow rate
+  double red[4];
+  double div_ref, div_max;
+  int mg_solves, mg_cycles, mg_max_cycles, status;
+  long long prof_launches, prof_solves, prof_cycles;  // totals since the last reset
+};
+
 // One boundary row of an ADI system, the net effect of the APPLY_BC_*
 // routines of src/boundary.F90 (IC_TYPE = 7) after all their assignments.
 enum BcKind : int {
@@ -90,7 +116,7 @@ int fft_inverse(const FftPlan& p, double* v, long npencil, hipStream_t st);
 // cross terms
 int launch_flux_accum(const FftPlan& p, const Dims& d, const Metrics& M, const Ranges& r, const double* U1,
                       const double* Q, const double* U2b, const double* U3b, double nu, const double* kx, double* CF,
-                      double* CR, double temp5, hipStream_t st, const double* nut = nullptr,
+                      double* CR, const double* temp5, hipStream_t st, const double* nut = nullptr,
                       const double* gl = nullptr);
 
 // ---- multigrid --------------------------------------------------------------
@@ -142,8 +168,9 @@ int tight_plan_build(TightPlan& tp, const ekm_params& p, const Dims& d, const Ho
                      const double* kx2_mode, int nmode, int nsolve_max);
 void tight_plan_free(TightPlan& tp);
 // solves nsolve planes of mg.rhs (RHS_TMP) into mg.phi; V-cycle counts in
-// mg.cycles.  guess != 0: start from guess * (mg.phi on entry), else from zero.
-int tight_solve_ex(TightPlan& tp, MgPlan& mg, const Dims& d, int nsolve, double guess, hipStream_t st);
+// mg.cycles.  guess (device pointer, may be null): when *guess != 0, start
+// from *guess * (mg.phi on entry), else from zero.
+int tight_solve_ex(TightPlan& tp, MgPlan& mg, const Dims& d, int nsolve, const double* guess, hipStream_t st);
 
 // ---- solver -------------------------------------------------------------------
 struct ekm_solver_impl;

@@ -15,7 +15,8 @@ struct RhsArgs {
   const double* thbar;
   Ranges rv, rt;
   int rk, th, cont_strat, cor_mean_only, px_on;
-  double temp3, temp4, nu, f0, ug, wg, ri_tau, nupr, px;
+  double nu, f0, ug, wg, ri_tau, nupr;
+  const StepState* ss;  // TEMP3, TEMP4 and the mean pressure gradient of the substep
 };
 
 int launch_rhs_fourier(const Dims& d, const Metrics& M, const RhsArgs& a, hipStream_t st);
@@ -41,9 +42,12 @@ struct AdiField {
   double* pv;
 };
 constexpr int kAdiMaxFields = 4;
+// Implicit weights of the ADI stage: TEMP1 / div[f] per field (div = 1, or
+// Pr for the scalar) and the subgrid-viscosity weight H_BAR/2, from the
+// substep's TEMP1 and H_BAR in the step state.
 struct AdiT1 {
-  double t1[kAdiMaxFields];  // implicit weight per field
-  double t2[kAdiMaxFields];  // subgrid-viscosity weight H_BAR/2 per field
+  const StepState* ss;
+  double div[kAdiMaxFields];
 };
 // The ADI sweeps of nf independent fields in one set of launches; fs: nf
 // descriptors in device memory, hf the same on the host (for the grid sizes).
@@ -60,7 +64,8 @@ int launch_mg_scatter(const Dims& d, const double* phi, int nmode, int fp32, dou
 int launch_project(const Dims& d, const Metrics& M, const Ranges& r, const double* kx, const double* PHI,
                    const double* F2, const double* F3, double* CU1, double* CU2, double* CU3, double* DIV,
                    hipStream_t st);
-int launch_pupdate(const Dims& d, const Ranges& r, const double* PHI, double* CP, double temp4, hipStream_t st);
+int launch_pupdate(const Dims& d, const Ranges& r, const double* PHI, double* CP, const StepState* ss,
+                   hipStream_t st);
 int launch_div_reduce(const Dims& d, const double* S, double* out, hipStream_t st);
 // x statistics (16 planes, see k_xstats) of the Fourier state; TH may be null
 int launch_xstats(const Dims& d, const double* U1, const double* U2, const double* U3, const double* P,
@@ -77,7 +82,7 @@ struct SpongeArgs {
   const double2* cth;
   const double* sigma;  // SPONGE_SIGMA(0:NY+1)
   const double* sigma_out;  // SPONGE_SIGMA_OUT(0:NZ+1), null when the streamwise profile is off
-  double amp;           // Sponge_Amp = 0.1 / DELTA_T
+  const StepState* ss;   // Sponge_Amp = 0.1 / DELTA_T of the substep
   Ranges r;             // the velocity ranges JSTART..JEND, ZSTART..ZEND
   int j0;               // first row of the fluctuation damping (max(JSTART, NY_S_T), or
                         // JSTART with the streamwise profile)
@@ -105,3 +110,26 @@ int launch_xmoments(const Dims& d, const double* U1p, const double* U2p, const d
                     const double* U2, const double* U3, double* out, hipStream_t st);
 int launch_thomas_generic(const double* A, const double* B, const double* C, double* G, double* W, int ny,
                           int nx, hipStream_t st);
+
+// ---- step-state control kernels (one thread each) ---------------------------
+// constants of RK substep rk (1..3) from H_BAR: TEMP1, TEMP3, TEMP4, TEMP5
+// (src/duct.F90:283-287), the mean pressure gradient (F_TYPE 1, 2;
+// src/duct.F90:431-453), Sponge_Amp and the start of the consistent solve
+struct StepConsts {
+  double nu;
+  double beta[3], zeta[3];   // BETA_BAR, ZETA_BAR (src/diablo.F90:568-576)
+  int f_type;
+  double px0, amp_omega0, omega0;
+  int guess_on;              // start the solve from H_BAR / H_BAR_prev times the previous phi
+};
+int launch_step_begin(StepState* ss, int rk, const StepConsts& c, hipStream_t st);
+// F_TYPE 0 controller update from the flow rate in red[3] (src/duct.F90:414-436)
+int launch_flow_ctrl(StepState* ss, double q_den, int nzz, double wbulk0, hipStream_t st);
+// DELTA_T = min(CFL minimum in red[2], DELTA_T_SET) and H_BAR (src/duct.F90:2417-2419)
+int launch_dt_update(StepState* ss, hipStream_t st);
+// diagnostics of a projection: the reference's DIV and the max from red[0..1]
+// (src/duct.F90:2790-2800), cycle totals of the nsolve solves, the status
+// checks of src/duct.F90:2811-2812 (sticky)
+int launch_div_final(StepState* ss, const int* cycles, int nsolve, double denom, hipStream_t st);
+// end of a substep: H_BAR_prev, and TIME += DELTA_T after the third (src/diablo.F90:152-157)
+int launch_step_end(StepState* ss, int rk, hipStream_t st);

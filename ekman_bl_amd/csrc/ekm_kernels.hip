@@ -45,6 +45,7 @@ __global__ void k_rhs_fourier(Dims d, Metrics M, RhsArgs a) {
   int k = (int)(r % d.NZ2), j = (int)(r / d.NZ2);
   size_t q = t;
   const size_t dk = d.NXC, dj = (size_t)d.NXC * d.NZ2;
+  const double temp3 = a.ss->temp3, temp4 = a.ss->temp4;
   if (in(j, a.rv.js, a.rv.je) && in(k, a.rv.zs, a.rv.ze)) {
     double Jc = JAC(k, j);
     const double2* CP = a.cp;
@@ -55,8 +56,8 @@ __global__ void k_rhs_fourier(Dims d, Metrics M, RhsArgs a) {
       cr[f] = make_double2(Jc * cu[f].x, Jc * cu[f].y);
       if (a.rk > 1) {
         double2 o = a.cf[f][q];
-        cr[f].x = cr[f].x + a.temp3 * o.x;
-        cr[f].y = cr[f].y + a.temp3 * o.y;
+        cr[f].x = cr[f].x + temp3 * o.x;
+        cr[f].y = cr[f].y + temp3 * o.y;
       }
     }
     // pressure gradient as explicit Euler (:366-409)
@@ -69,14 +70,14 @@ __global__ void k_rhs_fourier(Dims d, Metrics M, RhsArgs a) {
     cs1.y = a11p * (p.y + pkp.y) - a11 * (p.y + pkm.y) + a21p * (p.y + pjp.y) - a21 * (p.y + pjm.y);
     cs2.x = a12p * (p.x + pkp.x) - a12 * (p.x + pkm.x) + a22p * (p.x + pjp.x) - a22 * (p.x + pjm.x);
     cs2.y = a12p * (p.y + pkp.y) - a12 * (p.y + pkm.y) + a22p * (p.y + pjp.y) - a22 * (p.y + pjm.y);
-    double b = a.temp4 * a.kx[m] * Jc;
+    double b = temp4 * a.kx[m] * Jc;
     cr[0].x = cr[0].x - (0.0 * p.x - b * p.y);
     cr[0].y = cr[0].y - (0.0 * p.y + b * p.x);
-    cr[1].x = cr[1].x - a.temp4 * cs2.x;
-    cr[1].y = cr[1].y - a.temp4 * cs2.y;
-    cr[2].x = cr[2].x - a.temp4 * cs1.x;
-    cr[2].y = cr[2].y - a.temp4 * cs1.y;
-    if (m == 0 && a.px_on) cr[2].x = cr[2].x - a.temp4 * a.px * Jc;  // (:431-453)
+    cr[1].x = cr[1].x - temp4 * cs2.x;
+    cr[1].y = cr[1].y - temp4 * cs2.y;
+    cr[2].x = cr[2].x - temp4 * cs1.x;
+    cr[2].y = cr[2].y - temp4 * cs1.y;
+    if (m == 0 && a.px_on) cr[2].x = cr[2].x - temp4 * a.ss->px * Jc;  // (:431-453)
     // horizontal viscous terms and rotation (:461-475)
     double fv = a.nu * a.kx2[m] * Jc;
     for (int f = 0; f < 3; ++f) cf[f] = make_double2(-(fv * cu[f].x), -(fv * cu[f].y));
@@ -107,8 +108,8 @@ __global__ void k_rhs_fourier(Dims d, Metrics M, RhsArgs a) {
     double2 crth = make_double2(Jc * c.x, Jc * c.y);
     if (a.rk > 1) {
       double2 o = a.cfth[q];
-      crth.x = crth.x + a.temp3 * o.x;
-      crth.y = crth.y + a.temp3 * o.y;
+      crth.x = crth.x + temp3 * o.x;
+      crth.y = crth.y + temp3 * o.y;
     }
     double fth = -a.nupr * Jc * a.kx2[m];
     double2 cfth = make_double2(fth * c.x, fth * c.y);
@@ -164,7 +165,7 @@ __global__ void k_contravariant(Dims d, Metrics M, Ranges r, const double* __res
 __global__ void k_adi_pre(Dims d, Metrics M, const AdiField* __restrict__ fs, AdiT1 t1s) {
   const AdiField& F = fs[blockIdx.y];
   const Ranges r = F.r;
-  const double t1 = t1s.t1[blockIdx.y];
+  const double t1 = t1s.ss->temp1 / t1s.div[blockIdx.y];
   const double* __restrict__ U = F.U;
   double* __restrict__ R = F.R;
   double* __restrict__ S = F.S;
@@ -183,7 +184,7 @@ __global__ void k_adi_pre(Dims d, Metrics M, const AdiField* __restrict__ fs, Ad
   if (F.nut) {  // subgrid viscosity, z direction (:1667-1680 and analogues)
     const double* __restrict__ nut = F.nut;
     const double* __restrict__ gz = F.gz;
-    const double t2 = t1s.t2[blockIdx.y], nc = nut[q];
+    const double t2 = t1s.ss->hb / 2.0, nc = nut[q];
     rv = rv + t2 * (0.5 * (nut[q + dk] + nc) * (gz[mf(d, k + 1, j, 2)] * (U[q + dk] - u)) -
                     0.5 * (nc + nut[q - dk]) * (gz[mf(d, k, j, 2)] * (u - U[q - dk])));
   }
@@ -208,7 +209,7 @@ __global__ void __launch_bounds__(kLuBlock) k_lu(Dims d, Metrics M, const AdiFie
   if (F.nut) return;  // point-dependent systems: factored inside the sweeps
   const Ranges r = F.r;
   const BcTable& bc = dir == 0 ? F.by : F.bz;
-  const double t1 = t1s.t1[blockIdx.y];
+  const double t1 = t1s.ss->temp1 / t1s.div[blockIdx.y];
   const size_t ly = (size_t)d.NZ2 * d.NY2;
   double* __restrict__ mult = F.lu + (dir == 0 ? 0 : 3 * ly);
   double* __restrict__ piv = mult + ly;
@@ -300,7 +301,7 @@ __global__ void __launch_bounds__(256) k_ysweep(Dims d, Metrics M, const AdiFiel
   if (F.nut) return;
   const Ranges r = F.r;
   const BcTable& bc = F.by;
-  const double t1 = t1s.t1[blockIdx.y];
+  const double t1 = t1s.ss->temp1 / t1s.div[blockIdx.y];
   const size_t ly = (size_t)d.NZ2 * d.NY2;
   const double* __restrict__ mult = F.lu;
   const double* __restrict__ piv = F.lu + ly;
@@ -456,7 +457,7 @@ __global__ void __launch_bounds__(256) k_ysweep_les(Dims d, Metrics M, const Adi
   if (!F.nut) return;
   const Ranges r = F.r;
   const BcTable& bc = F.by;
-  const double t1 = t1s.t1[blockIdx.y], t2 = t1s.t2[blockIdx.y], t2h = t2 * 0.5;
+  const double t1 = t1s.ss->temp1 / t1s.div[blockIdx.y], t2 = t1s.ss->hb / 2.0, t2h = t2 * 0.5;
   double* __restrict__ U = F.U;
   double* __restrict__ R = F.R;
   const double* __restrict__ S = F.S;
@@ -572,7 +573,7 @@ __global__ void __launch_bounds__(256) k_zsweep_les(Dims d, Metrics M, const Adi
   if (!F.nut) return;
   const Ranges r = F.r;
   const BcTable& bc = F.bz;
-  const double t1 = t1s.t1[blockIdx.y], t2h = t1s.t2[blockIdx.y] * 0.5;
+  const double t1 = t1s.ss->temp1 / t1s.div[blockIdx.y], t2h = (t1s.ss->hb / 2.0) * 0.5;
   double* __restrict__ U = F.U;
   const double* __restrict__ R = F.R;
   const double* __restrict__ nut = F.nut;
@@ -901,7 +902,7 @@ __global__ void k_project(Dims d, Metrics M, Ranges r, const double* __restrict_
 
 // CPX += phi / TEMP4 over J = js-1..je+1, K = zs-1..ze+1 (:2460-2466)
 __global__ void k_pupdate(Dims d, Ranges r, const double2* __restrict__ P, double2* __restrict__ CP,
-                          double temp4) {
+                          const StepState* __restrict__ ss) {
   int nk = r.ze - r.zs + 3, nj = r.je - r.js + 3;
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t n = (size_t)d.NXC * nk * nj;
@@ -910,6 +911,7 @@ __global__ void k_pupdate(Dims d, Ranges r, const double2* __restrict__ P, doubl
   size_t rr = t / d.NXC;
   int k = r.zs - 1 + (int)(rr % nk), j = r.js - 1 + (int)(rr / nk);
   size_t q = fo(d, m, k, j);
+  const double temp4 = ss->temp4;
   double2 c = CP[q], p = P[q];
   c.x = c.x + p.x / temp4;
   c.y = c.y + p.y / temp4;
@@ -1004,8 +1006,8 @@ __global__ void k_sponge_fluct(Dims d, Metrics M, SpongeArgs a) {
   const size_t rr = t / nm;
   const int k = a.r.zs + (int)(rr % nk), j = a.j0 + (int)(rr / nk);
   const size_t q = fo(d, m, k, j);
-  const double Jc = JAC(k, j), as = a.amp * a.sigma[j];
-  const double ao = a.sigma_out ? a.amp * a.sigma_out[k] : 0.0;
+  const double Jc = JAC(k, j), as = a.ss->amp * a.sigma[j];
+  const double ao = a.sigma_out ? a.ss->amp * a.sigma_out[k] : 0.0;
   for (int f = 0; f < 4; ++f) {
     double2* cf = f < 3 ? a.cf[f] : a.cfth;
     const double2 u = f < 3 ? a.cu[f][q] : a.cth[q];
@@ -1047,7 +1049,7 @@ __global__ void __launch_bounds__(1024) k_sponge_mean(Dims d, Metrics M, SpongeA
   for (int t = threadIdx.x; t < nk * nji; t += blockDim.x) {
     const int k = a.r.zs + t % nk, j = a.r.js + t / nk;
     const size_t q = fo(d, 0, k, j);
-    const double Jc = JAC(k, j), as = a.amp * a.sigma[j];
+    const double Jc = JAC(k, j), as = a.ss->amp * a.sigma[j];
     const double2 u1 = a.cu[0][q], u2 = a.cu[1][q];
     double2 c1 = a.cf[0][q], c2 = a.cf[1][q];
     c1.x = c1.x - (as * (u1.x - a.t1)) * Jc;
@@ -1073,8 +1075,8 @@ __global__ void __launch_bounds__(1024) k_sponge_mean(Dims d, Metrics M, SpongeA
       const size_t q = fo(d, 0, t % d.NZ2, t / d.NZ2);
       const double2 u2 = a.cu[1][q];
       double2 c2 = a.cf[1][q];
-      c2.x = c2.x - (a.amp * (u2.x - a.t2)) * a.jst;
-      c2.y = c2.y - (a.amp * u2.y) * a.jst;
+      c2.x = c2.x - (a.ss->amp * (u2.x - a.t2)) * a.jst;
+      c2.y = c2.y - (a.ss->amp * u2.y) * a.jst;
       a.cf[1][q] = c2;
       if (a.zero_cu2) a.cu[1][q] = make_double2(0.0, 0.0);
     }
@@ -1084,7 +1086,7 @@ __global__ void __launch_bounds__(1024) k_sponge_mean(Dims d, Metrics M, SpongeA
   for (int t = threadIdx.x; t < nk * nji; t += blockDim.x) {
     const int k = a.r.zs + t % nk, j = a.r.js + t / nk;
     const size_t q = fo(d, 0, k, j);
-    const double Jc = JAC(k, j), ao = (2.0 * a.amp) * a.sigma_out[k];
+    const double Jc = JAC(k, j), ao = (2.0 * a.ss->amp) * a.sigma_out[k];
     const double tg[4] = {0.0, 0.0, a.t3, 0.0};
     for (int f = 0; f < 4; ++f) {
       double2* cf = f < 3 ? a.cf[f] : a.cfth;
@@ -1391,10 +1393,11 @@ int launch_project(const Dims& d, const Metrics& M, const Ranges& r, const doubl
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }
-int launch_pupdate(const Dims& d, const Ranges& r, const double* PHI, double* CP, double temp4, hipStream_t st) {
+int launch_pupdate(const Dims& d, const Ranges& r, const double* PHI, double* CP, const StepState* ss,
+                   hipStream_t st) {
   size_t n = (size_t)d.NXC * (r.ze - r.zs + 3) * (r.je - r.js + 3);
   hipLaunchKernelGGL(k_pupdate, dim3(nblk(n, 256)), dim3(256), 0, st, d, r, (const double2*)PHI, (double2*)CP,
-                     temp4);
+                     ss);
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }
@@ -1442,6 +1445,128 @@ int launch_div_reduce(const Dims& d, const double* S, double* out, hipStream_t s
 int launch_thomas_generic(const double* A, const double* B, const double* C, double* G, double* W, int ny,
                           int nx, hipStream_t st) {
   hipLaunchKernelGGL(k_thomas_generic, dim3(nblk((size_t)nx + 1, 256)), dim3(256), 0, st, A, B, C, G, W, ny, nx);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// step-state control kernels: one thread each, launched between the substep's
+// data-parallel kernels, so the substep's scalar decisions stay on the device
+// ---------------------------------------------------------------------------
+namespace {
+
+// constants of RK substep rk (src/duct.F90:283-287, :431-453; sponge_th's
+// Sponge_Amp, src/duct.F90:3220)
+__global__ void k_step_begin(StepState* ss, int rk, StepConsts c) {
+  const double hb = ss->h_bar[rk - 1];
+  ss->hb = hb;
+  ss->temp1 = c.nu * hb / 2.0;
+  ss->temp3 = c.zeta[rk - 1] * hb;
+  ss->temp4 = hb;
+  ss->temp5 = c.beta[rk - 1] * hb;
+  if (c.f_type == 0)
+    ss->px = ss->px0;  // the controller updates it before the RHS
+  else if (c.f_type == 1)
+    ss->px = c.px0;
+  else
+    ss->px = c.px0 + c.amp_omega0 * cos(c.omega0 * ss->time);
+  ss->amp = 0.1 / ss->delta_t;
+  ss->guess = c.guess_on ? hb / ss->hb_prev : 0.0;
+}
+
+// F_TYPE 0: PX0 <- PX0 - (WEIGHT/TEMP4)(Qn + Q0 - 2 Qn1) with the flow rate of
+// the state entering the substep (src/duct.F90:414-436); WEIGHT is the
+// single-precision 0.1 the reference assigns (src/diablo.F90:361)
+__global__ void k_flow_ctrl(StepState* ss, double q_den, int nzz, double wbulk0) {
+  const double sum = ss->red[3];
+  ss->red[3] = 0.0;
+  const double qn1 = sum / q_den / (double)(float)nzz;
+  const double weight = (double)0.1f;
+  ss->px0 = ss->px0 - (weight / ss->temp4) * (ss->qn + wbulk0 - 2 * qn1);
+  ss->qn = qn1;
+  ss->px = ss->px0;
+}
+
+// DELTA_T from the CFL minimum, capped at DELTA_T_SET, and H_BAR
+// (src/duct.F90:2417-2419, :3605-3674; src/diablo.F90:568-576)
+__global__ void k_dt_update(StepState* ss) {
+  const double dt = ss->red[2];
+  ss->red[2] = 999.0;
+  if (dt > 0) {
+    const double d = dt >= ss->delta_t_set ? ss->delta_t_set : dt;
+    ss->delta_t = d;
+    ss->h_bar[0] = d * (8.0 / 15.0);
+    ss->h_bar[1] = d * (2.0 / 15.0);
+    ss->h_bar[2] = d * (5.0 / 15.0);
+  }
+}
+
+// the projection's diagnostics: DIV (src/duct.F90:2790-2800), its max, the
+// cycle totals, and the STOP conditions of src/duct.F90:2811-2812 as a sticky
+// status (the first failure is kept)
+__global__ void __launch_bounds__(64) k_div_final(StepState* ss, const int* __restrict__ cycles, int nsolve,
+                                                  double denom) {
+  int sum = 0, mx = 0;
+  for (int q = threadIdx.x; q < nsolve; q += 64) {
+    const int c = cycles[q];
+    sum += c;
+    mx = c > mx ? c : mx;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    sum += __shfl_xor(sum, o);
+    const int m2 = __shfl_xor(mx, o);
+    mx = m2 > mx ? m2 : mx;
+  }
+  if (threadIdx.x != 0) return;
+  const double dr = ss->red[0] / denom;
+  ss->div_ref = dr;
+  ss->div_max = ss->red[1];
+  ss->red[0] = 0.0;
+  ss->red[1] = 0.0;
+  ss->mg_solves = nsolve;
+  ss->mg_cycles = sum;
+  ss->mg_max_cycles = mx;
+  ss->prof_launches += 1;
+  ss->prof_solves += nsolve;
+  ss->prof_cycles += sum;
+  int st = EKM_OK;
+  if (isnan(dr))
+    st = EKM_ERR_DIV_NAN;
+  else if (ss->delta_t < 1.0e-4)
+    st = EKM_ERR_DT_SMALL;
+  if (ss->status == EKM_OK) ss->status = st;
+}
+
+__global__ void k_step_end(StepState* ss, int rk) {
+  ss->hb_prev = ss->hb;
+  if (rk == 3) ss->time = ss->time + ss->delta_t;
+}
+
+}  // namespace
+
+int launch_step_begin(StepState* ss, int rk, const StepConsts& c, hipStream_t st) {
+  if (rk < 1 || rk > 3) return EKM_ERR_ARG;
+  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, st, ss, rk, c);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_flow_ctrl(StepState* ss, double q_den, int nzz, double wbulk0, hipStream_t st) {
+  hipLaunchKernelGGL(k_flow_ctrl, dim3(1), dim3(1), 0, st, ss, q_den, nzz, wbulk0);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_dt_update(StepState* ss, hipStream_t st) {
+  hipLaunchKernelGGL(k_dt_update, dim3(1), dim3(1), 0, st, ss);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_div_final(StepState* ss, const int* cycles, int nsolve, double denom, hipStream_t st) {
+  hipLaunchKernelGGL(k_div_final, dim3(1), dim3(64), 0, st, ss, cycles, nsolve, denom);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+int launch_step_end(StepState* ss, int rk, hipStream_t st) {
+  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, st, ss, rk);
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }

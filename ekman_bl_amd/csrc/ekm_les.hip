@@ -177,14 +177,19 @@ __global__ void __launch_bounds__(256) k_les_numden(Dims d, const double* __rest
 }
 
 // C_DYN = max(0, -1/2 num / den) on the interior points, zero elsewhere (:798-814)
-__global__ void k_les_cdyn(Dims d, Ranges r, const double* __restrict__ nsum, const double* __restrict__ dsum,
+// The plane sums are cleared here after they are read, so they start at zero
+// at the next refresh (they are zero when the solver is created).
+__global__ void k_les_cdyn(Dims d, Ranges r, double* __restrict__ nsum, double* __restrict__ dsum,
                            double* __restrict__ cdyn) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= d.plane) return;
   const int k = (int)(t % d.NZ2), j = (int)(t / d.NZ2);
   double c = 0.0;
-  if (in(j, r.js, r.je) && in(k, r.zs, r.ze)) c = dsum[t] != 0.0 ? -0.5 * nsum[t] / dsum[t] : 0.0;
+  const double ns = nsum[t], ds = dsum[t];
+  if (in(j, r.js, r.je) && in(k, r.zs, r.ze)) c = ds != 0.0 ? -0.5 * ns / ds : 0.0;
   cdyn[t] = c < 0.0 ? 0.0 : c;
+  nsum[t] = 0.0;
+  dsum[t] = 0.0;
 }
 
 // eddy viscosity (:825-831) and the subgrid stress with the parts that the
@@ -364,8 +369,6 @@ int les_chan_th_run(const FftPlan& fft, const Dims& d, const Metrics& M, const R
     // model's filtered velocities and |S| (:155-321)
     double* thb = L.S[3];
     hipLaunchKernelGGL(k_les_filter, dim3(nblk(np, 256)), dim3(256), 0, st, d, (const double*)TH, thb);
-    EKM_CHECK(hipMemsetAsync(w.nsum, 0, d.plane * sizeof(double), st));
-    EKM_CHECK(hipMemsetAsync(w.dsum, 0, d.plane * sizeof(double), st));
     for (int ij = 0; ij < 3; ++ij) {
       hipLaunchKernelGGL(k_les_filter, dim3(nblk(np, 256)), dim3(256), 0, st, d, (const double*)G[ij], w.t1);
       hipLaunchKernelGGL(k_les_mprep, dim3(nblk(np, 256)), dim3(256), 0, st, d, (const double*)L.delta,
@@ -379,8 +382,8 @@ int les_chan_th_run(const FftPlan& fft, const Dims& d, const Metrics& M, const R
                          1.0, w.nsum, w.dsum);
       EKM_CHECK(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_les_cdyn, dim3(nblk(d.plane, 256)), dim3(256), 0, st, d, rv, (const double*)w.nsum,
-                       (const double*)w.dsum, L.cdyn_th);
+    hipLaunchKernelGGL(k_les_cdyn, dim3(nblk(d.plane, 256)), dim3(256), 0, st, d, rv, w.nsum,
+                       w.dsum, L.cdyn_th);
     EKM_CHECK(hipGetLastError());
   }
   hipLaunchKernelGGL(k_les_kappa, dim3(nblk(np, 256)), dim3(256), 0, st, d, rt, (const double*)L.delta,
@@ -420,8 +423,6 @@ int les_chan_run(const FftPlan& fft, const Dims& d, const Metrics& M, const Rang
     for (int c = 0; c < 3; ++c)
       hipLaunchKernelGGL(k_les_filter, dim3(nblk(np, 256)), dim3(256), 0, st, d, (const double*)U[c], w.ub[c]);
     hipLaunchKernelGGL(k_les_filter, dim3(nblk(np, 256)), dim3(256), 0, st, d, (const double*)L.sabs, w.s2b);
-    EKM_CHECK(hipMemsetAsync(w.nsum, 0, d.plane * sizeof(double), st));
-    EKM_CHECK(hipMemsetAsync(w.dsum, 0, d.plane * sizeof(double), st));
     static const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
     for (int ij = 0; ij < 6; ++ij) {
       hipLaunchKernelGGL(k_les_filter, dim3(nblk(np, 256)), dim3(256), 0, st, d, (const double*)L.S[ij], w.t1);
@@ -436,8 +437,8 @@ int les_chan_run(const FftPlan& fft, const Dims& d, const Metrics& M, const Rang
                          (const double*)w.ub[ib[ij]], ij < 3 ? 1.0 : 2.0, w.nsum, w.dsum);
       EKM_CHECK(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_les_cdyn, dim3(nblk(d.plane, 256)), dim3(256), 0, st, d, r, (const double*)w.nsum,
-                       (const double*)w.dsum, L.cdyn);
+    hipLaunchKernelGGL(k_les_cdyn, dim3(nblk(d.plane, 256)), dim3(256), 0, st, d, r, w.nsum,
+                       w.dsum, L.cdyn);
     EKM_CHECK(hipGetLastError());
   }
   hipLaunchKernelGGL(k_les_stress, dim3(nblk(np, 256)), dim3(256), 0, st, d, M, r, (const double*)L.delta,

@@ -662,12 +662,13 @@ template <typename T, int CM, int NT, bool X>
 __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__ gp, T* __restrict__ ws,
                                                      double* __restrict__ ws64, const double* __restrict__ rhs_tmp,
                                                      double* __restrict__ phi_out, const double* __restrict__ kx2v,
-                                                     int* __restrict__ cycles, size_t plane, double guess,
-                                                     int nsolve) {
+                                                     int* __restrict__ cycles, size_t plane,
+                                                     const double* __restrict__ guessp, int nsolve) {
   __shared__ double red[NT];
   __shared__ T ylds[3 * kYMaxTasks * 64];
   extern __shared__ __align__(16) unsigned char tight_lds[];
   const TDesc<T>& g = *gp;
+  const double guess = guessp ? *guessp : 0.0;
   // workgroups are dealt to the 8 XCDs round robin: solves 2m and 2m+1 (the
   // two parts of mode m, which read the same per-mode tables) are placed on
   // one XCD so the second reader finds the tables in that XCD's L2.  The grid
@@ -1131,7 +1132,7 @@ void tight_plan_free(TightPlan& tp) {
   tp = TightPlan();
 }
 
-int tight_solve_ex(TightPlan& tp, MgPlan& mg, const Dims& d, int nsolve, double guess, hipStream_t st) {
+int tight_solve_ex(TightPlan& tp, MgPlan& mg, const Dims& d, int nsolve, const double* guess, hipStream_t st) {
   if (nsolve <= 0) return EKM_OK;
   if (nsolve > tp.nsolve_max) return EKM_ERR_ARG;
   // a multiple of 16 workgroups for the XCD placement of the solve pairs

@@ -3,6 +3,8 @@
 // fields; host<->device copies happen only in set/get calls.
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
+#include <map>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
@@ -46,18 +48,26 @@ struct ekm_solver {
   AdiField adi_h[kAdiMaxFields];
   AdiField* adi_d = nullptr;  // the same descriptors on the device
   int adi_nf = 0;
-  double* red = nullptr;  // [0] div sum, [1] div max (bits), [2] cfl min (bits), [3] flow rate
+  // run state that changes from substep to substep (DELTA_T, H_BAR, TIME,
+  // TEMP1..5, controller, reductions, diagnostics, status) lives on the
+  // device; the host reads it only when a caller asks (ekm_get_diag, ...)
+  StepState* ss = nullptr;
+  double* red = nullptr;  // ss->red
   BcTable bcy[4], bcz[4];
-  double time = 0.0, delta_t = 0.0, delta_t_set = 0.0;
-  double h_bar[3], beta_bar[3], zeta_bar[3];
+  double beta_bar[3], zeta_bar[3];
   int rk_step = 1, time_step = 0;
-  ekm_diag diag;
+  int status_seen = EKM_OK;  // the sticky status as last read back
   std::vector<int> cyc_host;
+  int kx2_dirty = 1;  // mg.kx2 does not hold the solver's wavenumbers
+  // captured substeps, one executable graph per host-decided variant
+  std::map<unsigned, hipGraphExec_t> graphs;
   // live timing of the multigrid launch with HIP events on the solver stream
+  // (eager substeps only): pairs recorded since the last fold
   int prof = 0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> evs;
+  size_t ev_used = 0;
   double prof_ms = 0.0;
-  long prof_launches = 0, prof_solves = 0, prof_cycles = 0;
+  long prof_timed = 0;
   // LES (src/les_chan.F90): model fields and scratch, and the reference's
   // CREATE_NEW_FLOW / FIRST_TIME, which decide when the model runs and when
   // its coefficient is recomputed
@@ -68,12 +78,11 @@ struct ekm_solver {
   AdiField* adi_dl = nullptr;
   int create_new_flow = 1, first_time = 1;
   int have_prev_phi = 0;  // mg.phi holds the previous substep's correction
-  // F_TYPE 0 flow-rate controller (src/duct.F90:414-436): PX0 and Qn, started
-  // at Q0 = Qn = WBULK0 (src/diablo.F90:359-361); y spacings of its integral
-  double px0 = 0.0, qn = 0.0;
+  // F_TYPE 0 flow-rate controller (src/duct.F90:414-436): its PX0 and Qn are
+  // in the step state, started at Q0 = Qn = WBULK0 (src/diablo.F90:359-361);
+  // y spacings of its integral
   double* fdy = nullptr;
   double q_den = 1.0;  // (ypoint(1,JEND) - ypoint(1,JSTART)) * (ZEND-ZSTART-1)
-  double hb_prev = 0.0;
   // sponge_th (src/duct.F90:3196-3362): SPONGE_SIGMA(0:NY+1) and
   // INT_JACOB(ZEND+1, JEND+1), the Jacobian its type-6 branch reads
   double* sponge = nullptr;
@@ -120,6 +129,9 @@ static int check_params(const ekm_params& p) {
   if (p.ny + 2 != p.mg_jyq * (1 << (p.mg_jey - 1)) + 1) return EKM_ERR_ARG;
   if (p.mg_ixp < 2 || p.mg_jyq < 2 || p.mg_maxcy < 1) return EKM_ERR_ARG;
   if (p.f_type < 0 || p.f_type > 3) return EKM_ERR_UNSUPPORTED;  // 3: no mean forcing
+  // MOD(TIME_STEP, UPDATE_DT) decides the CFL update (src/duct.F90:2417)
+  if (p.variable_dt && p.update_dt < 1) return EKM_ERR_ARG;
+  if (!(p.delta_t > 0.0)) return EKM_ERR_ARG;
   for (int s = 0; s < 4; ++s)
     if (p.mudpack_bc[s] < 0 || p.mudpack_bc[s] > 2) return EKM_ERR_UNSUPPORTED;
   if ((p.mudpack_bc[0] == 0) != (p.mudpack_bc[1] == 0)) return EKM_ERR_ARG;
@@ -170,16 +182,62 @@ static void init_ranges(const ekm_params& p, Ranges& rv, Ranges& rt) {
   rt.ze = (tz1 == 0 || tz1 == 1 || tz1 == 5 || tz1 == 6) ? NZ : NZ - 1;
 }
 
-static void set_hbar(ekm_solver* s) {
-  s->h_bar[0] = s->delta_t * (8.0 / 15.0);
-  s->h_bar[1] = s->delta_t * (2.0 / 15.0);
-  s->h_bar[2] = s->delta_t * (5.0 / 15.0);
+// device address of a field of the step state
+#define SS_PTR(s, field) \
+  (reinterpret_cast<decltype(&(static_cast<StepState*>(nullptr))->field)>(reinterpret_cast<char*>((s)->ss) + \
+                                                                         offsetof(StepState, field)))
+
+// host -> step state, on the solver's stream (the caller's value is on its
+// stack, so the copy completes before returning)
+static int ss_put(ekm_solver* s, size_t off, const void* v, size_t n) {
+  EKM_CHECK(hipMemcpyAsync(reinterpret_cast<char*>(s->ss) + off, v, n, hipMemcpyHostToDevice, s->st));
+  EKM_CHECK(hipStreamSynchronize(s->st));
+  return EKM_OK;
+}
+
+// waits for the stream and reads the step state; records its sticky status
+static int ss_get(ekm_solver* s, StepState* h) {
+  EKM_CHECK(hipMemcpyAsync(h, s->ss, sizeof(StepState), hipMemcpyDeviceToHost, s->st));
+  EKM_CHECK(hipStreamSynchronize(s->st));
+  s->status_seen = h->status;
+  return EKM_OK;
+}
+
+// a new run (initial condition or restart): TIME and TIME_STEP as given, the
+// status cleared
+static int ss_restart(ekm_solver* s, double time) {
+  const int ok = EKM_OK;
+  if (ss_put(s, offsetof(StepState, time), &time, sizeof(double)) != EKM_OK) return EKM_ERR_HIP;
+  if (ss_put(s, offsetof(StepState, status), &ok, sizeof(int)) != EKM_OK) return EKM_ERR_HIP;
+  s->status_seen = EKM_OK;
+  return EKM_OK;
+}
+
+// the solver's wavenumbers back into mg.kx2 after the multigrid drop-in used it
+static int restore_kx2(ekm_solver* s) {
+  if (!s->kx2_dirty) return EKM_OK;
+  EKM_CHECK(hipMemcpyAsync(s->mg.kx2, s->kx2s, 2 * (s->d.NKX + 1) * sizeof(double), hipMemcpyDeviceToDevice, s->st));
+  s->kx2_dirty = 0;
+  return EKM_OK;
+}
+
+// sums the recorded solve timings (waits for the stream)
+static int fold_events(ekm_solver* s) {
+  if (!s->ev_used) return EKM_OK;
+  EKM_CHECK(hipStreamSynchronize(s->st));
+  for (size_t q = 0; q < s->ev_used; ++q) {
+    float ms = 0.f;
+    EKM_CHECK(hipEventElapsedTime(&ms, s->evs[2 * q], s->evs[2 * q + 1]));
+    s->prof_ms += ms;
+  }
+  s->ev_used = 0;
+  return EKM_OK;
 }
 
 static int free_all(ekm_solver* s) {
   double* f[] = {s->U[0], s->U[1], s->U[2], s->P, s->TH, s->CF[0], s->CF[1], s->CF[2], s->CFTH,
                  s->R[0], s->R[1], s->R[2], s->RTH, s->S1, s->S2, s->U2b, s->U3b, s->PHI, s->lu,
-                 s->red, s->met, s->kx, s->kx2, s->kx2s, s->thbar, s->lesmet, s->fdy, s->sponge, s->sponge_out};
+                 s->met, s->kx, s->kx2, s->kx2s, s->thbar, s->lesmet, s->fdy, s->sponge, s->sponge_out};
   for (double* q : f)
     if (q) (void)hipFree(q);
   if (s->adi_d) (void)hipFree(s->adi_d);
@@ -195,8 +253,10 @@ static int free_all(ekm_solver* s) {
   fft_plan_free(s->fft);
   mg_plan_free(s->mg);
   tight_plan_free(s->tp);
-  if (s->ev0) (void)hipEventDestroy(s->ev0);
-  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->ss) (void)hipFree(s->ss);
+  for (auto& g : s->graphs) (void)hipGraphExecDestroy(g.second);
+  s->graphs.clear();
+  for (hipEvent_t e : s->evs) (void)hipEventDestroy(e);
   if (s->st) (void)hipStreamDestroy(s->st);
   return EKM_OK;
 }
@@ -227,6 +287,9 @@ extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const doub
   if (!pp || !xpoint || !ypoint || !out) return EKM_ERR_ARG;
   *out = nullptr;
   TRY(check_params(*pp));
+  // a scalar without the constant stratification reads its background TH_BAR
+  // (src/diablo.F90:1152-1160), which the caller must then pass
+  if (pp->n_th && !pp->cont_strat && !th_bar) return EKM_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return EKM_ERR_NODEV;
   ekm_solver* s = new ekm_solver();
@@ -279,7 +342,7 @@ extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const doub
   for (int j = 0; j < d.NY2; ++j)
     for (int k = 0; k < d.NZ2; ++k) {
       size_t q = (size_t)k + (size_t)d.NZ2 * j;
-      tb[q] = p.cont_strat ? (-ypoint[q] + ypoint[(size_t)d.NZ2 * (d.NY + 1)]) : (th_bar ? th_bar[q] : 0.0);
+      tb[q] = p.cont_strat ? (-ypoint[q] + ypoint[(size_t)d.NZ2 * (d.NY + 1)]) : (p.n_th ? th_bar[q] : 0.0);
     }
   if (hipMalloc(&s->thbar, d.plane * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
   (void)hipMemcpy(s->thbar, tb.data(), d.plane * sizeof(double), hipMemcpyHostToDevice);
@@ -368,9 +431,20 @@ extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const doub
     if (hipMemcpy(s->adi_dl, s->adi_hl, sizeof(s->adi_hl), hipMemcpyHostToDevice) != hipSuccess)
       return fail(EKM_ERR_HIP);
   }
-  if (hipMalloc(&s->red, 4 * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
-  s->px0 = p.px0;
-  s->qn = p.wbulk0;
+  {
+    StepState h;
+    std::memset(&h, 0, sizeof(h));
+    h.delta_t = h.delta_t_set = p.delta_t;
+    h.h_bar[0] = h.delta_t * (8.0 / 15.0);
+    h.h_bar[1] = h.delta_t * (2.0 / 15.0);
+    h.h_bar[2] = h.delta_t * (5.0 / 15.0);
+    h.px0 = p.px0;
+    h.qn = p.wbulk0;
+    h.red[2] = 999.0;  // the CFL minimum starts above any time step
+    if (hipMalloc(&s->ss, sizeof(StepState)) != hipSuccess) return fail(EKM_ERR_HIP);
+    if (hipMemcpy(s->ss, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) return fail(EKM_ERR_HIP);
+    s->red = SS_PTR(s, red[0]);
+  }
   if (p.sponge_type && p.n_th) {
     std::vector<double> sig(d.NY2);
     host_sponge_sigma(p.sponge_type, d.NZ, d.NY, p.ny_s_t, ypoint, sig.data());
@@ -421,12 +495,8 @@ extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const doub
     if (st != EKM_OK) return fail(st);
   }
   s->cyc_host.resize(nsolve);
-  s->delta_t = p.delta_t;
-  s->delta_t_set = p.delta_t;
-  set_hbar(s);
   s->beta_bar[0] = 1.0; s->beta_bar[1] = 25.0 / 8.0; s->beta_bar[2] = 9.0 / 4.0;
   s->zeta_bar[0] = 0.0; s->zeta_bar[1] = -17.0 / 8.0; s->zeta_bar[2] = -5.0 / 4.0;
-  std::memset(&s->diag, 0, sizeof(s->diag));
   if (hipStreamSynchronize(s->st) != hipSuccess) return fail(EKM_ERR_HIP);
   *out = s;
   return EKM_OK;
@@ -441,8 +511,9 @@ extern "C" int ekm_destroy(ekm_solver* s) {
 }
 
 extern "C" int ekm_sync(ekm_solver* s) {
-  EKM_CHECK(hipStreamSynchronize(s->st));
-  return EKM_OK;
+  StepState h;
+  TRY(ss_get(s, &h));
+  return h.status;
 }
 
 extern "C" int ekm_field_device_ptr(ekm_solver* s, int f, void** dptr) {
@@ -463,7 +534,7 @@ extern "C" int ekm_set_state(ekm_solver* s, const double* cu1, const double* cu2
   for (int q = 0; q < 4; ++q) EKM_CHECK(hipMemcpyAsync(dst[q], src[q], n, hipMemcpyHostToDevice, s->st));
   if (s->p.n_th && cth) EKM_CHECK(hipMemcpyAsync(s->TH, cth, n, hipMemcpyHostToDevice, s->st));
   EKM_CHECK(hipStreamSynchronize(s->st));
-  s->time = time;
+  TRY(ss_restart(s, time));
   s->time_step = time_step;
   s->rk_step = 1;
   return EKM_OK;
@@ -527,8 +598,10 @@ extern "C" int ekm_get_xmoments(ekm_solver* s, double* out) {
 }
 
 extern "C" int ekm_get_forcing(ekm_solver* s, double* px0, double* qn) {
-  if (px0) *px0 = s->p.f_type == 0 ? s->px0 : s->p.px0;
-  if (qn) *qn = s->qn;
+  StepState h;
+  TRY(ss_get(s, &h));
+  if (px0) *px0 = s->p.f_type == 0 ? h.px0 : s->p.px0;
+  if (qn) *qn = h.qn;
   return EKM_OK;
 }
 
@@ -560,52 +633,52 @@ extern "C" int ekm_get_state(ekm_solver* s, double* cu1, double* cu2, double* cu
   return EKM_OK;
 }
 
-// REM_DIV_CURV (src/duct.F90:2472-2818); phi is left in s->PHI
-static int rem_div(ekm_solver* s, double guess = 0.0) {
+// REM_DIV_CURV (src/duct.F90:2472-2818); phi is left in s->PHI.  The
+// diagnostics and the status checks (src/duct.F90:2790-2812) are formed on
+// the device into the step state; nothing is read back here.  guess: device
+// pointer to the scale of the consistent solve's start (null: from zero).
+// timed: HIP events around the solve launch (not inside a graph capture).
+static int rem_div(ekm_solver* s, const double* guess, bool timed) {
   const Dims& d = s->d;
   hipStream_t st = s->st;
   int nmode = d.NKX + 1, nsolve = 2 * nmode;
+  TRY(restore_kx2(s));
   TRY(launch_faces(d, s->M, s->rv, s->U[1], s->U[2], s->U2b, s->U3b, st));
   TRY(launch_mg_gather(d, s->M, s->rv, s->kx, s->U[0], s->U2b, s->U3b, s->mg.rhs, nmode, st));
-  EKM_CHECK(hipMemcpyAsync(s->mg.kx2, s->kx2s, nsolve * sizeof(double), hipMemcpyDeviceToDevice, st));
-  if (s->prof) EKM_CHECK(hipEventRecord(s->ev0, st));
+  if (timed) {
+    if (s->evs.size() < 2 * (s->ev_used + 1)) {
+      hipEvent_t a, b;
+      EKM_CHECK(hipEventCreate(&a));
+      EKM_CHECK(hipEventCreate(&b));
+      s->evs.push_back(a);
+      s->evs.push_back(b);
+    }
+    EKM_CHECK(hipEventRecord(s->evs[2 * s->ev_used], st));
+  }
   if (s->p.proj_mode == 0)
     TRY(mg_solve_ex(s->mg, d, nsolve, 0, st));
   else
     TRY(tight_solve_ex(s->tp, s->mg, d, nsolve, guess, st));
-  if (s->prof) EKM_CHECK(hipEventRecord(s->ev1, st));
+  if (timed) {
+    EKM_CHECK(hipEventRecord(s->evs[2 * s->ev_used + 1], st));
+    s->ev_used += 1;
+    s->prof_timed += 1;
+    if (s->ev_used >= 64) TRY(fold_events(s));
+  }
   TRY(launch_mg_scatter(d, s->mg.phi, nmode, s->p.cmplx_fp32, s->PHI, st));
   TRY(launch_project(d, s->M, s->rv, s->kx, s->PHI, s->U2b, s->U3b, s->U[0], s->U[1], s->U[2], s->S1, st));
   TRY(inv(s, s->S1));
-  EKM_CHECK(hipMemsetAsync(s->red, 0, 2 * sizeof(double), st));
   TRY(launch_div_reduce(d, s->S1, s->red, st));
-  double h[2];
-  EKM_CHECK(hipMemcpyAsync(h, s->red, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
-  EKM_CHECK(hipMemcpyAsync(s->cyc_host.data(), s->mg.cycles, nsolve * sizeof(int), hipMemcpyDeviceToHost, st));
-  EKM_CHECK(hipStreamSynchronize(st));
-  s->diag.div_ref = h[0] / ((double)(d.NX + 1) * (double)d.NY * (double)d.NZ);
-  s->diag.div_max = h[1];
-  int sum = 0, mx = 0;
-  for (int c : s->cyc_host) { sum += c; mx = c > mx ? c : mx; }
-  s->diag.mg_solves = nsolve;
-  s->diag.mg_cycles = sum;
-  s->diag.mg_max_cycles = mx;
-  if (s->prof) {
-    float ms = 0.f;
-    EKM_CHECK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-    s->prof_ms += ms;
-    s->prof_launches += 1;
-    s->prof_solves += nsolve;
-    s->prof_cycles += sum;
-  }
-  if (std::isnan(s->diag.div_ref)) return EKM_ERR_DIV_NAN;
-  if (s->delta_t < 1.0e-4) return EKM_ERR_DT_SMALL;
+  TRY(launch_div_final(s->ss, s->mg.cycles, nsolve, (double)(d.NX + 1) * (double)d.NY * (double)d.NZ, st));
   return EKM_OK;
 }
 
 extern "C" int ekm_rem_div_curv(ekm_solver* s) {
   s->have_prev_phi = 0;
-  return rem_div(s);
+  TRY(rem_div(s, nullptr, s->prof != 0));
+  StepState h;
+  TRY(ss_get(s, &h));
+  return h.status;
 }
 
 extern "C" int ekm_get_phi(ekm_solver* s, double* phi) {
@@ -628,25 +701,72 @@ extern "C" int ekm_set_initial_physical(ekm_solver* s, const double* u1, const d
   }
   double* z[] = {s->P, s->CF[0], s->CF[1], s->CF[2], s->CFTH};
   for (double* q : z) EKM_CHECK(hipMemsetAsync(q, 0, n, s->st));
-  s->time = 0.0;
+  TRY(ss_restart(s, 0.0));
   s->time_step = 0;
   s->rk_step = 1;
   s->have_prev_phi = 0;
   s->create_new_flow = 1;
   s->first_time = 1;
-  return rem_div(s);
+  TRY(rem_div(s, nullptr, false));
+  StepState h;
+  TRY(ss_get(s, &h));
+  return h.status;
 }
 
-// RK_DUCT (src/duct.F90:237-2469)
-extern "C" int ekm_rk_duct(ekm_solver* s) {
+// What the host decides for a substep from its counters alone: the RK stage,
+// when the subgrid models run and refresh their coefficients, the sponge's
+// CU2 zeroing, the CFL update and the warm start.  A captured substep graph
+// is keyed by it.
+struct SubstepPlan {
+  int rk, cur_step;
+  bool les_on, refresh, refresh_th, zero_cu2, cfl_now, guess_on;
+  unsigned key() const {
+    return (unsigned)rk | (les_on << 2) | (refresh << 3) | (refresh_th << 4) | (zero_cu2 << 5) | (cfl_now << 6) |
+           (guess_on << 7);
+  }
+};
+
+static SubstepPlan plan_substep(const ekm_solver* s) {
+  const ekm_params& p = s->p;
+  SubstepPlan q{};
+  q.rk = s->rk_step;
+  q.cur_step = s->time_step + 1;
+  // the subgrid model is not run on the first step of a new flow (:641-657)
+  q.les_on = p.les_model && (!s->create_new_flow || q.cur_step > 1);
+  q.refresh = q.les_on && (((q.cur_step % 5) == 0 && q.rk == 1) || s->first_time);
+  q.refresh_th = q.les_on && p.n_th && (((q.cur_step % 10) == 0 && q.rk == 1) || s->first_time);
+  q.zero_cu2 = p.n_th && p.sponge_type && (2 * q.cur_step) % p.save_stats_int == 0;
+  // CFL time step at the third substep (:2417-2419)
+  q.cfl_now = p.variable_dt && q.rk == 3 && (q.cur_step % p.update_dt) == 0;
+  // proj_guess: the previous substep's correction, scaled by the ratio of the
+  // RK weights, starts the consistent solve
+  q.guess_on = p.proj_mode > 0 && p.proj_guess && s->have_prev_phi;
+  return q;
+}
+
+// RK_DUCT (src/duct.F90:237-2469): enqueues one substep on the solver's
+// stream.  Only kernels are enqueued (no host round trip, no memset), so the
+// same sequence runs eagerly or is captured into a graph.
+static int enqueue_substep(ekm_solver* s, const SubstepPlan& q, bool timed) {
   const ekm_params& p = s->p;
   const Dims& d = s->d;
   hipStream_t st = s->st;
-  const int rk = s->rk_step, cur_step = s->time_step + 1;
-  const double hb = s->h_bar[rk - 1];
-  const double TEMP1 = p.nu * hb / 2.0, TEMP3 = s->zeta_bar[rk - 1] * hb, TEMP4 = hb;
-  const double TEMP5 = s->beta_bar[rk - 1] * hb;
+  const int rk = q.rk;
   const int th = p.n_th > 0;
+  {
+    StepConsts c;
+    c.nu = p.nu;
+    for (int r = 0; r < 3; ++r) {
+      c.beta[r] = s->beta_bar[r];
+      c.zeta[r] = s->zeta_bar[r];
+    }
+    c.f_type = p.f_type;
+    c.px0 = p.px0;
+    c.amp_omega0 = p.amp_omega0;
+    c.omega0 = p.omega0;
+    c.guess_on = q.guess_on;
+    TRY(launch_step_begin(s->ss, rk, c, st));
+  }
   RhsArgs a;
   for (int f = 0; f < 3; ++f) {
     a.cu[f] = (const double2*)s->U[f];
@@ -667,27 +787,17 @@ extern "C" int ekm_rk_duct(ekm_solver* s) {
   a.cont_strat = p.cont_strat;
   a.cor_mean_only = p.coriolis_mean_only;
   a.px_on = (p.f_type == 0 || p.f_type == 1 || p.f_type == 2);
-  a.temp3 = TEMP3;
-  a.temp4 = TEMP4;
   a.nu = p.nu;
   a.f0 = p.f0;
   a.ug = p.u_bc_c1[0][EKM_YMAX];
   a.wg = p.wbulk0;
   a.ri_tau = p.ri_tau;
   a.nupr = p.nu / p.pr;
-  a.px = p.f_type == 1 ? p.px0 : p.px0 + p.amp_omega0 * std::cos(p.omega0 * s->time);
+  a.ss = s->ss;
   if (p.f_type == 0) {
     // the flow-rate controller (:414-436) on the state entering the substep
     TRY(launch_flow_rate(d, s->rv, s->U[2], s->fdy, s->red + 3, st));
-    double sum;
-    EKM_CHECK(hipMemcpyAsync(&sum, s->red + 3, sizeof(double), hipMemcpyDeviceToHost, st));
-    EKM_CHECK(hipStreamSynchronize(st));
-    const double qn1 = sum / s->q_den / (double)(float)(s->rv.ze - s->rv.zs - 1);
-    // WEIGHT = 0.1 is assigned from a default-real literal (src/diablo.F90:361)
-    const double weight = (double)0.1f;
-    s->px0 = s->px0 - (weight / TEMP4) * (s->qn + p.wbulk0 - 2 * qn1);
-    s->qn = qn1;
-    a.px = s->px0;
+    TRY(launch_flow_ctrl(s->ss, s->q_den, s->rv.ze - s->rv.zs - 1, p.wbulk0, st));
   }
   TRY(launch_rhs_fourier(d, s->M, a, st));
   // sponge_th, called after the scalar's explicit terms (:635)
@@ -701,7 +811,7 @@ extern "C" int ekm_rk_duct(ekm_solver* s) {
     g.cth = (const double2*)s->TH;
     g.sigma = s->sponge;
     g.sigma_out = s->sponge_out;
-    g.amp = 0.1 / s->delta_t;
+    g.ss = s->ss;
     g.r = s->rv;
     g.j0 = s->sponge_out ? s->rv.js : std::max(s->rv.js, p.ny_s_t);
     g.ny_sponge = p.ny_s_t > p.ny ? 2 * p.ny / 3 : p.ny_s_t;
@@ -709,109 +819,148 @@ extern "C" int ekm_rk_duct(ekm_solver* s) {
     g.t2 = p.u_bc_c1[0][EKM_YMAX];
     g.t3 = p.wbulk0;
     g.zmax6 = p.u_bc[0][EKM_ZMAX] == 6 && p.u_bc[1][EKM_ZMAX] == 6 && p.u_bc[2][EKM_ZMAX] == 6 && p.ri_tau != 0.0;
-    g.zero_cu2 = (2 * cur_step) % p.save_stats_int == 0;
+    g.zero_cu2 = q.zero_cu2;
     g.jst = s->sponge_jst;
     TRY(launch_sponge(d, s->M, g, st));
   }
-  // velocities and scalar to physical space (:655-671)
   // the subgrid model (:641-657), which leaves the velocities in physical
   // space, or the velocities to physical space (:658-671)
-  const bool les_on = p.les_model && (!s->create_new_flow || cur_step > 1);
-  if (les_on) {
-    const bool refresh = ((cur_step % 5) == 0 && rk == 1) || s->first_time;
-    TRY(les_chan_run(s->fft, d, s->M, s->rv, s->kx, s->U, s->CF, s->les, s->lesw, refresh, st));
+  if (q.les_on) {
+    TRY(les_chan_run(s->fft, d, s->M, s->rv, s->kx, s->U, s->CF, s->les, s->lesw, q.refresh, st));
   } else {
     for (int f = 0; f < 3; ++f) TRY(inv(s, s->U[f]));
   }
   // the scalar's subgrid model (:650-654), which leaves the scalar in
   // physical space, or the scalar to physical space
-  if (th && les_on) {
-    const bool refresh_th = ((cur_step % 10) == 0 && rk == 1) || s->first_time;
+  if (th && q.les_on) {
     TRY(les_chan_th_run(s->fft, d, s->M, s->rv, s->rt, s->kx, s->thbar, s->U, s->TH, s->CFTH, s->les, s->lesw,
-                        refresh_th, st));
+                        q.refresh_th, st));
   } else if (th) {
     TRY(inv(s, s->TH));
   }
   // momentum fluxes (:683-1161)
   TRY(launch_contravariant(d, s->M, s->rv, s->U[1], s->U[2], s->U2b, s->U3b, st));
   // (with the model: its eddy-viscosity cross terms, :990-1120)
+  const double* temp5 = SS_PTR(s, temp5);
   for (int f = 0; f < 3; ++f) {
     TRY(launch_flux_accum(s->fft, d, s->M, s->rv, s->U[0], s->U[f], s->U2b, s->U3b, p.nu, s->kx, s->CF[f],
-                          s->R[f], TEMP5, st, les_on ? s->les.nut : nullptr, les_on ? les_metric(s, f, 2) : nullptr));
+                          s->R[f], temp5, st, q.les_on ? s->les.nut : nullptr,
+                          q.les_on ? les_metric(s, f, 2) : nullptr));
   }
   for (int f = 0; f < 3; ++f) TRY(inv(s, s->R[f]));
   if (th) {  // (:1169-1625)
     TRY(launch_flux_accum(s->fft, d, s->M, s->rt, s->U[0], s->TH, s->U2b, s->U3b, p.nu / p.pr, s->kx, s->CFTH,
-                          s->RTH, TEMP5, st, les_on ? s->les.kappa : nullptr, les_on ? s->M.g12 : nullptr));
+                          s->RTH, temp5, st, q.les_on ? s->les.kappa : nullptr, q.les_on ? s->M.g12 : nullptr));
     TRY(inv(s, s->RTH));
   }
   // ADI of the scalar (:1334-1625) and the velocities (:1640-2391): the four
   // implicit solves are independent, so they share one set of launches
   {
     const int corners = th ? ((p.th_bc[EKM_ZMAX] == 1 ? 1 : 0) | (p.th_bc[EKM_ZMIN] == 6 ? 2 : 0)) : 0;
-    AdiT1 t1{{TEMP1, TEMP1, TEMP1, TEMP1 / p.pr}, {hb / 2.0, hb / 2.0, hb / 2.0, hb / 2.0}};
-    TRY(launch_adi_batch(d, s->M, les_on ? s->adi_dl : s->adi_d, les_on ? s->adi_hl : s->adi_h, s->adi_nf, t1, 3,
-                         corners, st));
+    AdiT1 t1{s->ss, {1.0, 1.0, 1.0, p.pr}};
+    TRY(launch_adi_batch(d, s->M, q.les_on ? s->adi_dl : s->adi_d, q.les_on ? s->adi_hl : s->adi_h, s->adi_nf, t1,
+                         3, corners, st));
   }
   // CFL time step at the third substep (:2417-2419)
-  if (p.variable_dt && rk == 3 && (cur_step % p.update_dt) == 0) {
-    double init = 999.0;
-    unsigned long long bits;
-    std::memcpy(&bits, &init, sizeof(bits));
-    EKM_CHECK(hipMemcpyAsync(s->red + 2, &bits, sizeof(bits), hipMemcpyHostToDevice, st));
+  if (q.cfl_now) {
     TRY(launch_cfl(d, s->M, s->rv, s->U[0], s->U[1], s->U[2], p.cfl, p.lx / (double)d.NX,
                    reinterpret_cast<unsigned long long*>(s->red + 2), st));
-    double dt;
-    EKM_CHECK(hipMemcpyAsync(&dt, s->red + 2, sizeof(double), hipMemcpyDeviceToHost, st));
-    EKM_CHECK(hipStreamSynchronize(st));
-    if (dt > 0) {
-      s->delta_t = dt >= s->delta_t_set ? s->delta_t_set : dt;
-      set_hbar(s);
-    }
+    TRY(launch_dt_update(s->ss, st));
   }
   // back to Fourier space and project (:2421-2455)
   for (int f = 0; f < 3; ++f) TRY(fwd(s, s->U[f]));
   if (th) TRY(fwd(s, s->TH));
-  // proj_guess: the previous substep's correction, scaled by the ratio of the
-  // RK weights, starts the consistent solve
-  const double guess = (p.proj_mode > 0 && p.proj_guess && s->have_prev_phi) ? hb / s->hb_prev : 0.0;
-  int stat = rem_div(s, guess);
-  s->have_prev_phi = 1;
-  s->hb_prev = hb;
+  TRY(rem_div(s, q.guess_on ? SS_PTR(s, guess) : nullptr, timed));
   // pressure update (:2460-2466)
-  TRY(launch_pupdate(d, s->rv, s->PHI, s->P, TEMP4, st));
-  s->diag.status = stat;
-  if (rk == 3) {
-    s->time = s->time + s->delta_t;
-    s->time_step = cur_step;
+  TRY(launch_pupdate(d, s->rv, s->PHI, s->P, s->ss, st));
+  TRY(launch_step_end(s->ss, rk, st));
+  return EKM_OK;
+}
+
+// the host counters after a substep (src/diablo.F90:152-157)
+static void advance_counters(ekm_solver* s, const SubstepPlan& q) {
+  s->have_prev_phi = 1;
+  if (q.rk == 3) {
+    s->time_step = q.cur_step;
     s->rk_step = 1;
     s->first_time = 0;
   } else {
-    s->rk_step = rk + 1;
+    s->rk_step = q.rk + 1;
   }
-  return stat;
 }
 
-extern "C" int ekm_time_step(ekm_solver* s, int nsteps) {
-  for (int n = 0; n < nsteps; ++n)
-    for (int q = 0; q < 3; ++q) TRY(ekm_rk_duct(s));
-  EKM_CHECK(hipStreamSynchronize(s->st));
-  return EKM_OK;
+// One substep, launched eagerly.  Returns launch errors, or the sticky status
+// of the run as last read back (the status of this substep's projection is
+// formed on the device and read by ekm_get_diag, ekm_time_step, ekm_sync).
+extern "C" int ekm_rk_duct(ekm_solver* s) {
+  const SubstepPlan q = plan_substep(s);
+  TRY(enqueue_substep(s, q, s->prof != 0));
+  advance_counters(s, q);
+  return s->status_seen;
 }
+
+// One substep replayed from a HIP graph: the substep's variant is captured on
+// first use (the kernels and their order are those of ekm_rk_duct) and
+// launched as one graph after that.
+extern "C" int ekm_rk_duct_graph_launch(ekm_solver* s) {
+  const SubstepPlan q = plan_substep(s);
+  TRY(restore_kx2(s));
+  auto it = s->graphs.find(q.key());
+  if (it == s->graphs.end()) {
+    EKM_CHECK(hipStreamBeginCapture(s->st, hipStreamCaptureModeRelaxed));
+    const int e = enqueue_substep(s, q, false);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(s->st, &g);
+    if (e != EKM_OK || ce != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      if (e != EKM_OK) return e;
+      EKM_CHECK(ce);
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    EKM_CHECK(ie);
+    it = s->graphs.emplace(q.key(), ex).first;
+  }
+  EKM_CHECK(hipGraphLaunch(it->second, s->st));
+  advance_counters(s, q);
+  return s->status_seen;
+}
+
+// n full time steps (three substeps each); graph = 1 replays captured
+// substeps.  Waits for the device and returns the run's status.
+extern "C" int ekm_time_step_ex(ekm_solver* s, int nsteps, int graph) {
+  for (int n = 0; n < nsteps; ++n)
+    for (int q = 0; q < 3; ++q) TRY(graph ? ekm_rk_duct_graph_launch(s) : ekm_rk_duct(s));
+  StepState h;
+  TRY(ss_get(s, &h));
+  return h.status;
+}
+
+extern "C" int ekm_time_step(ekm_solver* s, int nsteps) { return ekm_time_step_ex(s, nsteps, 0); }
 
 extern "C" int ekm_get_solve_cycles(ekm_solver* s, int* cycles, int n) {
   const int ns = (int)s->cyc_host.size();
   if (!cycles || n < ns) return EKM_ERR_ARG;
+  EKM_CHECK(hipMemcpyAsync(s->cyc_host.data(), s->mg.cycles, ns * sizeof(int), hipMemcpyDeviceToHost, s->st));
+  EKM_CHECK(hipStreamSynchronize(s->st));
   for (int q = 0; q < ns; ++q) cycles[q] = s->cyc_host[q];
   return EKM_OK;
 }
 
 extern "C" int ekm_get_diag(ekm_solver* s, ekm_diag* dg) {
-  *dg = s->diag;
-  dg->delta_t = s->delta_t;
-  dg->time = s->time;
+  StepState h;
+  TRY(ss_get(s, &h));
+  dg->div_ref = h.div_ref;
+  dg->div_max = h.div_max;
+  dg->delta_t = h.delta_t;
+  dg->time = h.time;
   dg->time_step = s->time_step;
   dg->rk_step = s->rk_step;
+  dg->mg_solves = h.mg_solves;
+  dg->mg_cycles = h.mg_cycles;
+  dg->mg_max_cycles = h.mg_max_cycles;
+  dg->status = h.status;
   return EKM_OK;
 }
 
@@ -820,6 +969,7 @@ extern "C" int ekm_multigrid_mudpack(ekm_solver* s, int nsolve, const double* kx
   if (nsolve < 1 || nsolve > s->mg.nsolve_max) return EKM_ERR_ARG;
   s->have_prev_phi = 0;
   size_t pl = s->d.plane;
+  s->kx2_dirty = 1;
   EKM_CHECK(hipMemcpyAsync(s->mg.kx2, kx2, nsolve * sizeof(double), hipMemcpyHostToDevice, s->st));
   EKM_CHECK(hipMemcpyAsync(s->mg.rhs, rhs_tmp, nsolve * pl * sizeof(double), hipMemcpyHostToDevice, s->st));
   TRY(mg_solve_ex(s->mg, s->d, nsolve, 0, s->st));
@@ -911,23 +1061,27 @@ extern "C" int ekm_thomas_real_sp(double* a, double* b, double* c, double* g, do
 
 // ---- measurement hooks ----------------------------------------------------------
 extern "C" int ekm_prof_enable(ekm_solver* s, int on) {
-  if (on && !s->ev0) {
-    EKM_CHECK(hipEventCreate(&s->ev0));
-    EKM_CHECK(hipEventCreate(&s->ev1));
-  }
+  TRY(fold_events(s));
+  EKM_CHECK(hipStreamSynchronize(s->st));
+  const long long zero[3] = {0, 0, 0};
+  TRY(ss_put(s, offsetof(StepState, prof_launches), zero, sizeof(zero)));
   s->prof = on;
   s->prof_ms = 0.0;
-  s->prof_launches = s->prof_solves = s->prof_cycles = 0;
+  s->prof_timed = 0;
   return EKM_OK;
 }
 
-// total milliseconds of the multigrid launches since ekm_prof_enable, their
-// count, and the real solves and W cycles they performed
+// total milliseconds of the timed (eager) multigrid launches since
+// ekm_prof_enable and their count, and the real solves and cycles of every
+// projection since then
 extern "C" int ekm_prof_get(ekm_solver* s, double* ms, long* launches, long* solves, long* cycles) {
+  TRY(fold_events(s));
+  StepState h;
+  TRY(ss_get(s, &h));
   *ms = s->prof_ms;
-  *launches = s->prof_launches;
-  *solves = s->prof_solves;
-  *cycles = s->prof_cycles;
+  *launches = s->prof ? s->prof_timed : (long)h.prof_launches;
+  *solves = (long)h.prof_solves;
+  *cycles = (long)h.prof_cycles;
   return EKM_OK;
 }
 

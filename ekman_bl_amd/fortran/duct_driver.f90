@@ -4,7 +4,8 @@
 ! set the initial field, then DO TIME_STEP ... DO RK_STEP=1,3: CALL RK_DUCT.
 !
 ! Usage: duct_driver <run_dir>
-!   <run_dir>/duct.nml     namelist &duct: p (type ekm_params), n_steps
+!   <run_dir>/duct.nml     namelist &duct: p (type ekm_params), n_steps, and
+!                          use_graph (1: each substep replayed from a HIP graph)
 !   <run_dir>/GRID_YZ.dat  z y per line, K fastest (the reference's grid file,
 !                          read as at src/diablo.F90:1120-1128)
 !   <run_dir>/ic.bin       U1, U2, U3, TH physical fields, raw float64,
@@ -18,22 +19,26 @@ program duct_driver
   type(ekm_params) :: p
   type(ekm_diag) :: d
   character(len=1024) :: dir
-  integer :: n_steps, step, rk, u, ios, j, k
+  integer :: n_steps, step, rk, u, ios, j, k, use_graph
   integer(c_int) :: st
   integer(kind=8) :: nphys, nfour, t0, t1, rate
   real(c_double), allocatable :: xpoint(:,:), ypoint(:,:)
   real(c_double), allocatable :: u1(:), u2(:), u3(:), th(:)
   real(c_double), allocatable :: cu1(:), cu2(:), cu3(:), cp(:), cth(:)
-  namelist /duct/ p, n_steps
+  namelist /duct/ p, n_steps, use_graph
 
   call get_command_argument(1, dir)
   if (len_trim(dir) == 0) dir = '.'
   call ekm_params_default(p)
   n_steps = 1
+  use_graph = 0
   open(newunit=u, file=trim(dir)//'/duct.nml', status='old', action='read', iostat=ios)
   if (ios /= 0) stop 'duct_driver: cannot open duct.nml'
   read(u, nml=duct)
   close(u)
+  ! this driver has no TH_BAR file: a scalar needs the constant stratification
+  ! (src/diablo.F90:1152-1160)
+  if (p%n_th > 0 .and. p%cont_strat == 0) stop 'duct_driver: N_TH = 1 needs CONSTANT_STRATIFICATION'
 
   allocate(xpoint(0:p%nz+1, 0:p%ny+1), ypoint(0:p%nz+1, 0:p%ny+1))
   open(newunit=u, file=trim(dir)//'/GRID_YZ.dat', status='old', action='read', iostat=ios)
@@ -62,7 +67,11 @@ program duct_driver
   call system_clock(t0, rate)
   do step = 1, n_steps
      do rk = 1, 3
-        call RK_DUCT_GPU()
+        if (use_graph /= 0) then
+           call RK_DUCT_GRAPH_GPU()
+        else
+           call RK_DUCT_GPU()
+        end if
      end do
      call ekm_check(ekm_get_diag(ekm_handle, d), 'ekm_get_diag')
      write(6, '(i6,a9,f15.8,a7,f15.8,a7,es12.4,a9,i6)') d%time_step, '    time=', d%time, '    dt=', &

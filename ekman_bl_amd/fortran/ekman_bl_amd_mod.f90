@@ -3,7 +3,7 @@
 !
 ! A DIABLO build keeps its own driver and calls these in place of its CPU
 ! routines:
-!   RK_DUCT                          (src/duct.F90:237)      -> ekm_rk_duct
+!   RK_DUCT                          (src/duct.F90:237)      -> ekm_rk_duct, ekm_rk_duct_graph_launch
 !   REM_DIV_CURV                     (src/duct.F90:2472)     -> ekm_rem_div_curv
 !   FFT_X_TO_FOURIER_OP / _PHYSICAL_OP (calls src/mpi_duct.F90:290,308)
 !                                                            -> ekm_fft_x_to_fourier / _physical
@@ -182,6 +182,18 @@ module ekman_bl_amd
        integer(c_int), value :: nsteps
      end function ekm_time_step
 
+     ! the substep replayed from a captured HIP graph (same results)
+     integer(c_int) function ekm_rk_duct_graph_launch(s) bind(c, name="ekm_rk_duct_graph_launch")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: s
+     end function ekm_rk_duct_graph_launch
+
+     integer(c_int) function ekm_time_step_ex(s, nsteps, graph) bind(c, name="ekm_time_step_ex")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: s
+       integer(c_int), value :: nsteps, graph
+     end function ekm_time_step_ex
+
      integer(c_int) function ekm_rem_div_curv(s) bind(c, name="ekm_rem_div_curv")
        import :: c_ptr, c_int
        type(c_ptr), value :: s
@@ -355,6 +367,12 @@ contains
   subroutine RK_DUCT_GPU()
     call ekm_check(ekm_rk_duct(ekm_handle), 'RK_DUCT')
   end subroutine RK_DUCT_GPU
+
+  ! RK_DUCT as one HIP-graph launch; the projection's checks are read back by
+  ! ekm_get_diag / ekm_sync (sticky status)
+  subroutine RK_DUCT_GRAPH_GPU()
+    call ekm_check(ekm_rk_duct_graph_launch(ekm_handle), 'RK_DUCT (graph)')
+  end subroutine RK_DUCT_GRAPH_GPU
 
   ! REM_DIV_CURV (src/duct.F90:2472): projection of the device state.
   subroutine REM_DIV_CURV_GPU()

@@ -212,14 +212,16 @@ def read_grid_def(path: str) -> dict:
 
 
 def _fortran_tokens(line: str) -> list[str]:
-    return line.replace(",", " ").split()
+    """Values of one record of list-directed input: separators are blanks,
+    tabs and commas; a slash ends the record's values."""
+    return line.split("/")[0].replace(",", " ").split()
 
 
 def _fval(tok: str):
     t = tok.strip().lower()
-    if t.startswith(".t") or t == "true":
+    if t.startswith(".t") or t in ("t", "true"):
         return True
-    if t.startswith(".f") or t == "false":
+    if t.startswith(".f") or t in ("f", "false"):
         return False
     t = t.replace("d", "e")
     if re.fullmatch(r"[+-]?\d+", t):
@@ -227,51 +229,117 @@ def _fval(tok: str):
     return float(t)
 
 
+class _Records:
+    """The record cursor of a Fortran list-directed READ(11,*) sequence:
+    `skip()` is READ(11,*) with no items (one record, whatever it holds);
+    `read(n)` is READ(11,*) of n items, which takes values from as many
+    records as it needs (a blank record holds none) and drops the rest of
+    its last record."""
+
+    def __init__(self, lines: list[str], path: str):
+        self.lines, self.pos, self.path = lines, 0, path
+
+    def _next(self) -> str:
+        if self.pos >= len(self.lines):
+            raise ValueError(f"{self.path}: end of file after record {self.pos}")
+        self.pos += 1
+        return self.lines[self.pos - 1]
+
+    def skip(self) -> None:
+        self._next()
+
+    def read(self, n: int) -> list:
+        vals: list = []
+        while len(vals) < n:
+            rec = self.pos + 1
+            try:
+                vals += [_fval(t) for t in _fortran_tokens(self._next())]
+            except ValueError as e:
+                raise ValueError(f"{self.path}: record {rec}: {e}") from None
+        return vals[:n]
+
+
 def read_input_dat(path: str, p: EkmParams) -> dict:
-    """Fill `p` from an input.dat with the reference's line layout
-    (src/diablo.F90:242-337, version 1.21).  Returns every value read."""
-    lines = [ln for ln in open(path).read().splitlines()]
-    data = [ln for ln in lines if ln.strip() and not ln.lstrip().upper().startswith("C ")
-            and not ln.lstrip().startswith("#") and not ln.lstrip().upper() == "C"]
-    vals = [[_fval(t) for t in _fortran_tokens(ln)] for ln in data]
-    it = iter(vals)
-    out = {}
-    out["version"] = next(it)[0]
-    nu, lx, ly, lz = next(it)[:4]
-    p.nu, p.lx, p.ly, p.lz = nu, lx, ly, lz
-    nts, dt, _reset, vdt, cfl, _dfn, upd = next(it)[:7]
-    p.delta_t, p.variable_dt, p.cfl, p.update_dt = dt, int(vdt), cfl, upd
-    out["n_time_steps"] = nts
-    les, *_ = next(it)
-    out["les"] = les
-    mud_use, a, b, c, d = next(it)[:5]
+    """Fill `p` from an input.dat (version 1.21) by the reference's record
+    positions (src/diablo.F90:242-337): four header records, then each value
+    line preceded by one comment record, whatever that record holds, and the
+    10 records per scalar when N_TH = 1.  Returns every value read.
+
+    The LES line sets `p.les_model`: LES = .TRUE. with LES_MODEL_TYPE 2 is the
+    dynamic Smagorinsky model (and, with a scalar, LES_MODEL_TYPE_TH must be 2
+    as well); other model types raise ValueError, as does the combination the
+    reference stops on (src/diablo.F90:256-258)."""
+    r = _Records(open(path).read().splitlines(), path)
+    out: dict = {}
+    for _ in range(4):
+        r.skip()
+    out["version"] = r.read(1)[0]
+    if abs(float(out["version"]) - 1.21) > 1e-6:
+        raise ValueError(f"{path}: input file version {out['version']}, expected 1.21 (src/diablo.F90:248)")
+    r.skip()
+    p.nu, p.lx, p.ly, p.lz = r.read(4)
+    r.skip()
+    nts, dt, reset, vdt, cfl, dfn, upd = r.read(7)
+    p.delta_t, p.variable_dt, p.cfl, p.update_dt = dt, int(vdt), cfl, int(upd)
+    out.update(n_time_steps=nts, reset_time=reset, dfn=dfn)
+    r.skip()
+    les, les_type, les_type_th, wall_type = r.read(4)
+    out.update(les=les, les_model_type=les_type, les_model_type_th=les_type_th, wall_model_type=wall_type)
+    if les and les_type == 1 and les_type_th == 2:
+        raise ValueError(f"{path}: LES_MODEL_TYPE=1 with LES_MODEL_TYPE_TH=2 (the reference stops, "
+                         "src/diablo.F90:256-258)")
+    r.skip()
+    mud_use, a, b, c, d = r.read(5)
     out["mudpack_use"] = mud_use
     p.mudpack_bc[0], p.mudpack_bc[1], p.mudpack_bc[2], p.mudpack_bc[3] = a, b, c, d
-    io = next(it)  # VERBOSITY, SAVE_FLOW_INT, SAVE_STATS_INT, BACKUP_FLOW_INT, MOVIE
+    r.skip()
+    io = r.read(5)  # VERBOSITY, SAVE_FLOW_INT, SAVE_STATS_INT, BACKUP_FLOW_INT, MOVIE
     p.save_stats_int = int(io[2])
-    ics = next(it)
-    out["ic_type"] = ics[1]
-    ft, wb, px0, om, amp, _beta, f0, sponge = next(it)[:8]
+    out.update(save_flow_int=io[1], save_stats_int=io[2], backup_flow_int=io[3])
+    r.skip()
+    ics = r.read(5)  # CREATE_NEW_FLOW, IC_TYPE, KICK, INT_TREAT, WRITE_VEL_TECPLOT
+    out.update(create_new_flow=ics[0], ic_type=ics[1], kick=ics[2])
+    r.skip()
+    ft, wb, px0, om, amp, _beta, f0, sponge = r.read(8)
     p.f_type, p.wbulk0, p.px0, p.omega0, p.amp_omega0, p.f0 = ft, wb, px0, om, amp, f0
     out["sponge_type"] = sponge
     p.sponge_type = int(sponge)
-    # BCs: U,V,W lower; U,V,W upper; U,V,W left; U,V,W right (:284-306)
+    # BCs: U,V,W lower; U,V,W upper; U,V,W left; U,V,W right (:284-320)
     for slot in (YMIN, YMAX, ZMIN, ZMAX):
         for comp in range(3):
-            row = next(it)
+            r.skip()
+            row = r.read(4)
             p.u_bc[comp][slot] = row[0]
             p.u_bc_c1[comp][slot] = row[1]
-    next(it)  # stochastic forcing
-    next(it)  # filter
+    r.skip()
+    out["stochastic_forcing"] = r.read(1)[0]
+    r.skip()
+    out["filter_vel"] = r.read(2)
+    r.skip()
     if p.n_th:
-        next(it)  # CREATE_FLOW_TH
-        next(it)  # FILTER_TH
-        ri, pr, _bg, cs = next(it)[:4]
+        r.skip()
+        out["create_new_th"] = r.read(1)[0]
+        r.skip()
+        out["filter_th"] = r.read(2)
+        r.skip()
+        ri, pr, bg, cs = r.read(4)
         p.ri_tau, p.pr, p.cont_strat = ri, pr, int(cs)
+        out["background_grad"] = bg
         for slot in (YMIN, YMAX, ZMIN, ZMAX):
-            row = next(it)
+            r.skip()
+            row = r.read(4)
             p.th_bc[slot] = row[0]
             p.th_bc_c1[slot] = row[1]
+    if les:
+        if les_type != 2:
+            raise ValueError(f"{path}: LES_MODEL_TYPE {les_type} is not supported (dynamic Smagorinsky, "
+                             "type 2, only)")
+        if p.n_th and les_type_th != 2:
+            raise ValueError(f"{path}: LES_MODEL_TYPE_TH {les_type_th} is not supported with a scalar "
+                             "(dynamic model, type 2, only)")
+        p.les_model = 2
+    else:
+        p.les_model = 0
     return out
 
 

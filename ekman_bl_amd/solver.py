@@ -60,6 +60,8 @@ def lib():
         L.ekm_destroy.argtypes = [vp]
         L.ekm_sync.argtypes = [vp]
         L.ekm_rk_duct.argtypes = [vp]
+        L.ekm_rk_duct_graph_launch.argtypes = [vp]
+        L.ekm_time_step_ex.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.ekm_rem_div_curv.argtypes = [vp]
         L.ekm_time_step.argtypes = [vp, ctypes.c_int]
         L.ekm_set_state.argtypes = [vp, _D, _D, _D, _D, _D, ctypes.c_double, ctypes.c_int]
@@ -229,11 +231,16 @@ class DuctSolver:
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u1, u2, u3, th)]
         _check(lib().ekm_set_initial_physical(self.h, *[_p(a) for a in arrs]), "ekm_set_initial_physical")
 
-    def rk_duct(self) -> None:
-        _check(lib().ekm_rk_duct(self.h), "RK_DUCT")
+    def rk_duct(self, graph: bool = False) -> None:
+        """One substep, launched eagerly or replayed from a captured HIP graph.
+        Its projection checks are read back by diag(), sync() and time_step()."""
+        if graph:
+            _check(lib().ekm_rk_duct_graph_launch(self.h), "RK_DUCT (graph)")
+        else:
+            _check(lib().ekm_rk_duct(self.h), "RK_DUCT")
 
-    def time_step(self, n: int = 1) -> None:
-        _check(lib().ekm_time_step(self.h, int(n)), "time_step")
+    def time_step(self, n: int = 1, graph: bool = False) -> None:
+        _check(lib().ekm_time_step_ex(self.h, int(n), 1 if graph else 0), "time_step")
 
     def rem_div_curv(self) -> None:
         _check(lib().ekm_rem_div_curv(self.h), "REM_DIV_CURV")

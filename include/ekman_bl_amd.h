@@ -162,10 +162,23 @@ int ekm_set_initial_physical(ekm_solver* s, const double* u1, const double* u2,
                              const double* u3, const double* th);
 
 /* One RKW3+CN substep (RK_DUCT, src/duct.F90:237-2469); advances RK_STEP and,
- * after the third substep, TIME and TIME_STEP (src/diablo.F90:152-157). */
+ * after the third substep, TIME and TIME_STEP (src/diablo.F90:152-157).
+ * The substep is enqueued on the solver's stream without a host round trip:
+ * DELTA_T (CFL), the mean-pressure controller and the projection's checks run
+ * on the device, and the checks' outcome (EKM_ERR_DIV_NAN, EKM_ERR_DT_SMALL,
+ * the reference's STOPs at src/duct.F90:2811-2812) is kept as a sticky status
+ * that ekm_get_diag, ekm_sync and ekm_time_step read back.  Returns launch
+ * errors, or the sticky status as last read back. */
 int ekm_rk_duct(ekm_solver* s);
-/* n full time steps (three substeps each). */
+/* The same substep replayed from a HIP graph: each variant of the substep
+ * (RK stage, subgrid-model refresh, CFL update, ...) is captured on first use
+ * and then launched as one graph; results are bit-identical to ekm_rk_duct.
+ * Same return as ekm_rk_duct. */
+int ekm_rk_duct_graph_launch(ekm_solver* s);
+/* n full time steps (three substeps each); waits for the device and returns
+ * the run's status.  ekm_time_step_ex with graph = 1 replays captured substeps. */
 int ekm_time_step(ekm_solver* s, int nsteps);
+int ekm_time_step_ex(ekm_solver* s, int nsteps, int graph);
 /* Pressure projection alone (REM_DIV_CURV, src/duct.F90:2472-2818): the
  * correction phi is left in the solver for ekm_get_phi. */
 int ekm_rem_div_curv(ekm_solver* s);
@@ -205,7 +218,8 @@ int ekm_get_mean_grads(ekm_solver* s, double* out);
  * WWWV -- the x averages over the NX physical points of powers of the velocity
  * fluctuations about the mode-0 means. */
 int ekm_get_xmoments(ekm_solver* s, double* out);
-/* Blocks until queued work on the solver's stream is done. */
+/* Blocks until queued work on the solver's stream is done; returns the run's
+ * sticky status. */
 int ekm_sync(ekm_solver* s);
 /* Device pointer of state field f (0=U1,1=U2,2=U3,3=P,4=TH); the field's
  * representation is Fourier between substeps. */

@@ -113,6 +113,63 @@ def test_read_input_dat_fixture():
     assert p.th_bc[YMIN] == 5 and p.th_bc_c1[YMIN] == pytest.approx(1.56e-3)
     assert p.th_bc[YMAX] == 1 and p.th_bc[ZMIN] == 6 and p.th_bc[ZMAX] == 6
     assert p.pr == 1.0
+    assert vals["les"] is False and p.les_model == 0 and vals["wall_model_type"] == 3
+
+
+def _edited_input(tmp_path, edit) -> str:
+    lines = open(os.path.join(GOLDEN, "input_unst.dat")).read().splitlines()
+    path = tmp_path / "input.dat"
+    path.write_text("\n".join(edit(lines)) + "\n")
+    return str(path)
+
+
+def test_read_input_dat_les_line(tmp_path):
+    """LES = .TRUE. with LES_MODEL_TYPE 2 (and TH 2) selects the dynamic
+    Smagorinsky model; types the solver cannot honour raise, as does the
+    pairing the reference stops on (src/diablo.F90:254-258)."""
+    def les(model, model_th):
+        def e(lines):
+            i = next(n for n, ln in enumerate(lines) if ".FALSE." in ln and "2" in ln and n < 12)
+            lines[i] = f"              .TRUE.        {model}               {model_th}\t\t\t3"
+            return lines
+        return e
+    p = base_params(256, 127, 319, n_th=1)
+    vals = read_input_dat(_edited_input(tmp_path, les(2, 2)), p)
+    assert vals["les"] is True and p.les_model == 2
+    for model, model_th in ((2, 1), (1, 1), (1, 2), (3, 3)):
+        with pytest.raises(ValueError):
+            read_input_dat(_edited_input(tmp_path, les(model, model_th)), base_params(256, 127, 319, n_th=1))
+
+
+def test_read_input_dat_by_record_position(tmp_path):
+    """A comment record is skipped whatever it holds (tab-led, 'c:', blank),
+    as READ(11,*) with no items does; a blank record inside a data read holds
+    no values and the read continues to the next record."""
+    ref = base_params(256, 127, 319, n_th=1)
+    want = read_input_dat(os.path.join(GOLDEN, "input_unst.dat"), ref)
+
+    def retag(lines):
+        out = []
+        for n, ln in enumerate(lines):
+            if ln.lstrip().upper().startswith("C"):
+                ln = ("C\t" if n % 3 == 0 else "c:" if n % 3 == 1 else "") + ln.lstrip()[1:]
+                if n % 3 == 2:
+                    ln = ""           # an empty comment record
+            out.append(ln)
+        return out
+
+    p = base_params(256, 127, 319, n_th=1)
+    got = read_input_dat(_edited_input(tmp_path, retag), p)
+    assert got == want
+    assert bytes(p) == bytes(ref)
+
+    def blank_before_values(lines):
+        i = lines.index(next(ln for ln in lines if ln.strip().startswith("1.3E-4")))
+        return lines[:i] + [""] + lines[i:]
+
+    p2 = base_params(256, 127, 319, n_th=1)
+    assert read_input_dat(_edited_input(tmp_path, blank_before_values), p2) == want
+    assert bytes(p2) == bytes(ref)
 
 
 def test_params_struct_matches_header():
