@@ -20,6 +20,9 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
+#include <mutex>
+#include <vector>
 
 namespace {
 
@@ -463,11 +466,12 @@ int launch(bool fwd, const FftPlan& p, double* v, long npencil, hipStream_t st) 
 
 }  // namespace
 
-int fft_plan_init(FftPlan& p, int nx) {
-  if (nx < 8 || nx > 1024 || (nx & (nx - 1))) return EKM_ERR_UNSUPPORTED;
-  p.nx = nx;
-  int M = nx / 2;
-  std::vector<double> tw(2 * (M / 2 > 0 ? M / 2 : 1)), tw2(2 * (M + 1));
+// twiddle tables of an NX-point real transform: tw = exp(-2 pi i k / M),
+// k < M/2, and tw2 = exp(-2 pi i k / NX), k <= M, M = NX/2 (re, im pairs);
+// tw holds max(M/2, 1) entries
+void fft_twiddles(int nx, double* tw, double* tw2) {
+  const int M = nx / 2;
+  if (M / 2 == 0) tw[0] = tw[1] = 0.0;
   for (int k = 0; k < M / 2; ++k) {
     tw[2 * k] = std::cos(2.0 * M_PI * k / M);
     tw[2 * k + 1] = -std::sin(2.0 * M_PI * k / M);
@@ -476,6 +480,15 @@ int fft_plan_init(FftPlan& p, int nx) {
     tw2[2 * k] = std::cos(2.0 * M_PI * k / nx);
     tw2[2 * k + 1] = -std::sin(2.0 * M_PI * k / nx);
   }
+}
+size_t fft_tw_doubles(int nx) { return 2 * (size_t)std::max(nx / 4, 1); }
+size_t fft_tw2_doubles(int nx) { return 2 * (size_t)(nx / 2 + 1); }
+
+int fft_plan_init(FftPlan& p, int nx) {
+  if (nx < 8 || nx > 1024 || (nx & (nx - 1))) return EKM_ERR_UNSUPPORTED;
+  p.nx = nx;
+  std::vector<double> tw(fft_tw_doubles(nx)), tw2(fft_tw2_doubles(nx));
+  fft_twiddles(nx, tw.data(), tw2.data());
   EKM_CHECK(hipMalloc(&p.tw, tw.size() * sizeof(double)));
   EKM_CHECK(hipMalloc(&p.tw2, tw2.size() * sizeof(double)));
   EKM_CHECK(hipMemcpy(p.tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -502,4 +515,55 @@ int launch_flux_accum(const FftPlan& p, const Dims& d, const Metrics& M, const R
   if (M2 == 128) return launch_flux_accum_r<2>(p, d, M, r, U1, Q, U2b, U3b, nu, kx, CF, CR, temp5, nut, gl, st);
   if (M2 == 256) return launch_flux_accum_r<4>(p, d, M, r, U1, Q, U2b, U3b, nu, kx, CF, CR, temp5, nut, gl, st);
   return launch_flux_accum_r<8>(p, d, M, r, U1, Q, U2b, U3b, nu, kx, CF, CR, temp5, nut, gl, st);
+}
+
+// ---------------------------------------------------------------------------
+// The component drop-ins FFT_X_TO_FOURIER_OP / FFT_X_TO_PHYSICAL_OP on device
+// pointers: plans are built once per (device, NX) and kept for the process.
+// ---------------------------------------------------------------------------
+
+namespace {
+std::mutex g_plan_mu;
+std::vector<std::pair<std::pair<int, int>, FftPlan>> g_plans;
+
+int cached_plan(int nx, const FftPlan** out) {
+  int dev = 0;
+  EKM_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  for (auto& e : g_plans)
+    if (e.first.first == dev && e.first.second == nx) {
+      *out = &e.second;
+      return EKM_OK;
+    }
+  FftPlan p;
+  const int st = fft_plan_init(p, nx);
+  if (st != EKM_OK) {
+    fft_plan_free(p);
+    return st;
+  }
+  g_plans.reserve(64);
+  if (g_plans.size() == g_plans.capacity()) return EKM_ERR_ARG;  // pointers into the vector stay valid
+  g_plans.push_back({{dev, nx}, p});
+  *out = &g_plans.back().second;
+  return EKM_OK;
+}
+
+}  // namespace
+
+extern "C" int ekm_fft_x_to_fourier_dev(double* varp, int nx, int npencil, void* stream) {
+  const FftPlan* p = nullptr;
+  const int e = cached_plan(nx, &p);
+  if (e != EKM_OK) return e;
+  const int st = fft_forward(*p, varp, npencil, (hipStream_t)stream);
+  EKM_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  return st;
+}
+
+extern "C" int ekm_fft_x_to_physical_dev(double* varp, int nx, int npencil, void* stream) {
+  const FftPlan* p = nullptr;
+  const int e = cached_plan(nx, &p);
+  if (e != EKM_OK) return e;
+  const int st = fft_inverse(*p, varp, npencil, (hipStream_t)stream);
+  EKM_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  return st;
 }

@@ -23,6 +23,8 @@
   } while (0)
 
 void ekm_note_hip_error(hipError_t e, const char* what, const char* file, int line);
+// records a message for ekm_last_error (and prints it)
+void ekm_note_error(const char* msg);
 
 struct Dims {
   int NX, NY, NZ;       // grid_def sizes
@@ -107,6 +109,11 @@ struct FftPlan {
   double* tw2 = nullptr;  // exp(-2 pi i k / NX), k <= NX/2 (re,im)
 };
 int fft_plan_init(FftPlan& p, int nx);
+// the plan's twiddle tables on the host: fft_tw_doubles(nx) values of tw,
+// then fft_tw2_doubles(nx) of tw2
+void fft_twiddles(int nx, double* tw, double* tw2);
+size_t fft_tw_doubles(int nx);
+size_t fft_tw2_doubles(int nx);
 void fft_plan_free(FftPlan& p);
 // in-place transforms of npencil pencils of stride NX+2
 int fft_forward(const FftPlan& p, double* v, long npencil, hipStream_t st);

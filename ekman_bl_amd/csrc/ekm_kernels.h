@@ -120,7 +120,7 @@ struct StepConsts {
   double beta[3], zeta[3];   // BETA_BAR, ZETA_BAR (src/diablo.F90:568-576)
   int f_type;
   double px0, amp_omega0, omega0;
-  int guess_on;              // start the solve from H_BAR / H_BAR_prev times the previous phi
+  int guess_on;              // start the solve from mg.phi scaled by H_BAR / H_BAR_prev (1) or 1 (2)
 };
 int launch_step_begin(StepState* ss, int rk, const StepConsts& c, hipStream_t st);
 // F_TYPE 0 controller update from the flow rate in red[3] (src/duct.F90:414-436)

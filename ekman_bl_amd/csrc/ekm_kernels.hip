@@ -1471,7 +1471,7 @@ __global__ void k_step_begin(StepState* ss, int rk, StepConsts c) {
   else
     ss->px = c.px0 + c.amp_omega0 * cos(c.omega0 * ss->time);
   ss->amp = 0.1 / ss->delta_t;
-  ss->guess = c.guess_on ? hb / ss->hb_prev : 0.0;
+  ss->guess = c.guess_on == 1 ? hb / ss->hb_prev : c.guess_on == 2 ? 1.0 : 0.0;
 }
 
 // F_TYPE 0: PX0 <- PX0 - (WEIGHT/TEMP4)(Qn + Q0 - 2 Qn1) with the flow rate of

@@ -15,8 +15,14 @@
 #include "ekm_internal.h"
 #include "ekm_kernels.h"
 #include "ekm_les.h"
+#include "ekm_wsnet.h"
 
 static thread_local std::string g_last_error;
+
+void ekm_note_error(const char* msg) {
+  g_last_error = msg;
+  std::fprintf(stderr, "ekman_bl_amd: %s\n", msg);
+}
 
 void ekm_note_hip_error(hipError_t e, const char* what, const char* file, int line) {
   char buf[512];
@@ -78,6 +84,7 @@ struct ekm_solver {
   AdiField* adi_dl = nullptr;
   int create_new_flow = 1, first_time = 1;
   int have_prev_phi = 0;  // mg.phi holds the previous substep's correction
+  ekm_wsnet* net = nullptr;  // learned initial guess (proj_guess = 2), not owned
   // F_TYPE 0 flow-rate controller (src/duct.F90:414-436): its PX0 and Qn are
   // in the step state, started at Q0 = Qn = WBULK0 (src/diablo.F90:359-361);
   // y spacings of its integral
@@ -159,7 +166,7 @@ static int check_params(const ekm_params& p) {
     }
     if (!dir) return EKM_ERR_UNSUPPORTED;
     if (!(p.proj_tol > 0.0) || p.proj_maxit < 1) return EKM_ERR_ARG;
-    if (p.proj_guess < 0 || p.proj_guess > 1) return EKM_ERR_ARG;
+    if (p.proj_guess < 0 || p.proj_guess > 2) return EKM_ERR_ARG;
     if (p.proj_krylov < 0 || p.proj_krylov > 1) return EKM_ERR_ARG;
   }
   return EKM_OK;
@@ -638,13 +645,16 @@ extern "C" int ekm_get_state(ekm_solver* s, double* cu1, double* cu2, double* cu
 // the device into the step state; nothing is read back here.  guess: device
 // pointer to the scale of the consistent solve's start (null: from zero).
 // timed: HIP events around the solve launch (not inside a graph capture).
-static int rem_div(ekm_solver* s, const double* guess, bool timed) {
+static int rem_div(ekm_solver* s, const double* guess, bool timed, bool learned = false) {
   const Dims& d = s->d;
   hipStream_t st = s->st;
   int nmode = d.NKX + 1, nsolve = 2 * nmode;
   TRY(restore_kx2(s));
   TRY(launch_faces(d, s->M, s->rv, s->U[1], s->U[2], s->U2b, s->U3b, st));
   TRY(launch_mg_gather(d, s->M, s->rv, s->kx, s->U[0], s->U2b, s->U3b, s->mg.rhs, nmode, st));
+  // the learned start: the network reads this projection's right-hand sides
+  // and the previous solutions in mg.phi and writes its guess there
+  if (learned) TRY(wsnet_guess(s->net, s->mg.rhs, s->mg.phi, s->kx2s, nsolve, s->mg.phi, st));
   if (timed) {
     if (s->evs.size() < 2 * (s->ev_used + 1)) {
       hipEvent_t a, b;
@@ -719,10 +729,11 @@ extern "C" int ekm_set_initial_physical(ekm_solver* s, const double* u1, const d
 // is keyed by it.
 struct SubstepPlan {
   int rk, cur_step;
-  bool les_on, refresh, refresh_th, zero_cu2, cfl_now, guess_on;
+  bool les_on, refresh, refresh_th, zero_cu2, cfl_now;
+  int guess_on;  // 0 from zero, 1 the previous correction scaled, 2 the network's guess
   unsigned key() const {
     return (unsigned)rk | (les_on << 2) | (refresh << 3) | (refresh_th << 4) | (zero_cu2 << 5) | (cfl_now << 6) |
-           (guess_on << 7);
+           ((unsigned)guess_on << 7);
   }
 };
 
@@ -739,8 +750,12 @@ static SubstepPlan plan_substep(const ekm_solver* s) {
   // CFL time step at the third substep (:2417-2419)
   q.cfl_now = p.variable_dt && q.rk == 3 && (q.cur_step % p.update_dt) == 0;
   // proj_guess: the previous substep's correction, scaled by the ratio of the
-  // RK weights, starts the consistent solve
-  q.guess_on = p.proj_mode > 0 && p.proj_guess && s->have_prev_phi;
+  // RK weights, or the network's guess from it, starts the consistent solve
+  q.guess_on = 0;
+  if (p.proj_mode > 0 && p.proj_guess && s->have_prev_phi) {
+    if (p.proj_guess == 1) q.guess_on = 1;
+    else if (s->net) q.guess_on = 2;
+  }
   return q;
 }
 
@@ -870,7 +885,7 @@ static int enqueue_substep(ekm_solver* s, const SubstepPlan& q, bool timed) {
   // back to Fourier space and project (:2421-2455)
   for (int f = 0; f < 3; ++f) TRY(fwd(s, s->U[f]));
   if (th) TRY(fwd(s, s->TH));
-  TRY(rem_div(s, q.guess_on ? SS_PTR(s, guess) : nullptr, timed));
+  TRY(rem_div(s, q.guess_on ? SS_PTR(s, guess) : nullptr, timed, q.guess_on == 2));
   // pressure update (:2460-2466)
   TRY(launch_pupdate(d, s->rv, s->PHI, s->P, s->ss, st));
   TRY(launch_step_end(s->ss, rk, st));
@@ -939,6 +954,29 @@ extern "C" int ekm_time_step_ex(ekm_solver* s, int nsteps, int graph) {
 
 extern "C" int ekm_time_step(ekm_solver* s, int nsteps) { return ekm_time_step_ex(s, nsteps, 0); }
 
+extern "C" int ekm_set_warm_start_net(ekm_solver* s, ekm_wsnet* net) {
+  if (net) {
+    int nz2, ny2, bmax;
+    wsnet_dims(net, &nz2, &ny2, &bmax);
+    if (nz2 != s->d.NZ2 || ny2 != s->d.NY2 || bmax < 2 * (s->d.NKX + 1)) return EKM_ERR_ARG;
+  }
+  EKM_CHECK(hipStreamSynchronize(s->st));
+  // captured substeps with the learned start launch the previous network
+  for (auto& g : s->graphs) (void)hipGraphExecDestroy(g.second);
+  s->graphs.clear();
+  s->net = net;
+  return EKM_OK;
+}
+
+extern "C" int ekm_solve_planes(ekm_solver* s, const double** rhs, const double** phi, const double** kx2,
+                                int* nsolve) {
+  if (rhs) *rhs = s->mg.rhs;
+  if (phi) *phi = s->mg.phi;
+  if (kx2) *kx2 = s->kx2s;
+  if (nsolve) *nsolve = 2 * (s->d.NKX + 1);
+  return EKM_OK;
+}
+
 extern "C" int ekm_get_solve_cycles(ekm_solver* s, int* cycles, int n) {
   const int ns = (int)s->cyc_host.size();
   if (!cycles || n < ns) return EKM_ERR_ARG;
@@ -980,22 +1018,6 @@ extern "C" int ekm_multigrid_mudpack(ekm_solver* s, int nsolve, const double* kx
 }
 
 // ---- component drop-ins -----------------------------------------------------
-extern "C" int ekm_fft_x_to_fourier_dev(double* varp, int nx, int npencil, void* stream) {
-  FftPlan p;
-  TRY(fft_plan_init(p, nx));
-  int st = fft_forward(p, varp, npencil, (hipStream_t)stream);
-  EKM_CHECK(hipStreamSynchronize((hipStream_t)stream));
-  fft_plan_free(p);
-  return st;
-}
-extern "C" int ekm_fft_x_to_physical_dev(double* varp, int nx, int npencil, void* stream) {
-  FftPlan p;
-  TRY(fft_plan_init(p, nx));
-  int st = fft_inverse(p, varp, npencil, (hipStream_t)stream);
-  EKM_CHECK(hipStreamSynchronize((hipStream_t)stream));
-  fft_plan_free(p);
-  return st;
-}
 static int fft_host(double* varp, int nx, int npencil, int inverse) {
   if (nx < 8 || npencil < 0) return EKM_ERR_ARG;
   size_t n = (size_t)(nx + 2) * npencil * sizeof(double);
@@ -1108,5 +1130,23 @@ extern "C" int ekm_dev_alloc(void** p, size_t bytes) {
 }
 extern "C" int ekm_dev_free(void* p) {
   EKM_CHECK(hipFree(p));
+  return EKM_OK;
+}
+// dst row r = src row idx[r] (rows of row_bytes, device memory; idx on the host)
+extern "C" int ekm_gather_rows(void* dst, const void* src, const long* idx, int n, size_t row_bytes, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  for (int r = 0; r < n; ++r)
+    EKM_CHECK(hipMemcpyAsync((char*)dst + (size_t)r * row_bytes, (const char*)src + (size_t)idx[r] * row_bytes,
+                             row_bytes, hipMemcpyDeviceToDevice, st));
+  EKM_CHECK(hipStreamSynchronize(st));
+  return EKM_OK;
+}
+extern "C" int ekm_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  EKM_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return EKM_OK;
+}
+extern "C" int ekm_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  EKM_CHECK(hipDeviceSynchronize());
+  EKM_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
   return EKM_OK;
 }

@@ -272,6 +272,121 @@ module ekman_bl_amd
        type(c_ptr), value :: p
      end function ekm_dev_free
 
+     integer(c_int) function ekm_memcpy_h2d(dst, src, bytes) bind(c, name="ekm_memcpy_h2d")
+       import :: c_ptr, c_int, c_size_t
+       type(c_ptr), value :: dst, src
+       integer(c_size_t), value :: bytes
+     end function ekm_memcpy_h2d
+
+     integer(c_int) function ekm_memcpy_d2h(dst, src, bytes) bind(c, name="ekm_memcpy_d2h")
+       import :: c_ptr, c_int, c_size_t
+       type(c_ptr), value :: dst, src
+       integer(c_size_t), value :: bytes
+     end function ekm_memcpy_d2h
+
+     integer(c_int) function ekm_gather_rows(dst, src, idx, n, row_bytes, stream) bind(c, name="ekm_gather_rows")
+       import :: c_ptr, c_int, c_long, c_size_t
+       type(c_ptr), value :: dst, src, stream
+       integer(c_long), intent(in) :: idx(*)
+       integer(c_int), value :: n
+       integer(c_size_t), value :: row_bytes
+     end function ekm_gather_rows
+
+     ! ---- learned initial guess of the pressure solves (no counterpart in the
+     !      reference, whose MUDPACK solves start from zero) -------------------
+     integer(c_int) function ekm_wsnet_create(nz2, ny2, bmax, trainable, net) bind(c, name="ekm_wsnet_create")
+       import :: c_ptr, c_int
+       integer(c_int), value :: nz2, ny2, bmax, trainable
+       type(c_ptr), intent(out) :: net
+     end function ekm_wsnet_create
+
+     integer(c_int) function ekm_wsnet_destroy(net) bind(c, name="ekm_wsnet_destroy")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: net
+     end function ekm_wsnet_destroy
+
+     integer(c_long) function ekm_wsnet_nparams(net) bind(c, name="ekm_wsnet_nparams")
+       import :: c_ptr, c_long
+       type(c_ptr), value :: net
+     end function ekm_wsnet_nparams
+
+     integer(c_int) function ekm_wsnet_set_params(net, theta) bind(c, name="ekm_wsnet_set_params")
+       import :: c_ptr, c_int, c_float
+       type(c_ptr), value :: net
+       real(c_float), intent(in) :: theta(*)
+     end function ekm_wsnet_set_params
+
+     integer(c_int) function ekm_wsnet_get_params(net, theta) bind(c, name="ekm_wsnet_get_params")
+       import :: c_ptr, c_int, c_float
+       type(c_ptr), value :: net
+       real(c_float), intent(out) :: theta(*)
+     end function ekm_wsnet_get_params
+
+     integer(c_int) function ekm_wsnet_set_norm(net, sigma, kx_max) bind(c, name="ekm_wsnet_set_norm")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: net
+       real(c_double), value :: sigma, kx_max
+     end function ekm_wsnet_set_norm
+
+     integer(c_int) function ekm_wsnet_get_norm(net, sigma, kx_max) bind(c, name="ekm_wsnet_get_norm")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: net
+       real(c_double), intent(out) :: sigma, kx_max
+     end function ekm_wsnet_get_norm
+
+     integer(c_int) function ekm_wsnet_prepare(net, rhs, prev, kx2, nb, x1, scale, stream) &
+         bind(c, name="ekm_wsnet_prepare")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: net, rhs, prev, kx2, x1, scale, stream
+       integer(c_int), value :: nb
+     end function ekm_wsnet_prepare
+
+     integer(c_int) function ekm_wsnet_target(net, phi, scale, nb, t0, stream) bind(c, name="ekm_wsnet_target")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: net, phi, scale, t0, stream
+       integer(c_int), value :: nb
+     end function ekm_wsnet_target
+
+     integer(c_int) function ekm_wsnet_forward(net, x1, nb, y1, stream) bind(c, name="ekm_wsnet_forward")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: net, x1, y1, stream
+       integer(c_int), value :: nb
+     end function ekm_wsnet_forward
+
+     integer(c_int) function ekm_wsnet_gradient(net, x1, t0, nb, grad, loss) bind(c, name="ekm_wsnet_gradient")
+       import :: c_ptr, c_int, c_float
+       type(c_ptr), value :: net, x1, t0
+       integer(c_int), value :: nb
+       real(c_float), intent(out) :: grad(*), loss
+     end function ekm_wsnet_gradient
+
+     integer(c_int) function ekm_wsnet_train_step(net, x1, t0, nb, lr, loss, stream) &
+         bind(c, name="ekm_wsnet_train_step")
+       import :: c_ptr, c_int, c_float
+       type(c_ptr), value :: net, x1, t0, stream
+       integer(c_int), value :: nb
+       real(c_float), value :: lr
+       real(c_float), intent(out) :: loss
+     end function ekm_wsnet_train_step
+
+     integer(c_int) function ekm_wsnet_guess(net, rhs, prev, kx2, nb, out, stream) bind(c, name="ekm_wsnet_guess")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: net, rhs, prev, kx2, out, stream
+       integer(c_int), value :: nb
+     end function ekm_wsnet_guess
+
+     integer(c_int) function ekm_set_warm_start_net(s, net) bind(c, name="ekm_set_warm_start_net")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: s, net
+     end function ekm_set_warm_start_net
+
+     integer(c_int) function ekm_solve_planes(s, rhs, phi, kx2, nsolve) bind(c, name="ekm_solve_planes")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: s
+       type(c_ptr), intent(out) :: rhs, phi, kx2
+       integer(c_int), intent(out) :: nsolve
+     end function ekm_solve_planes
+
      integer(c_int) function ekm_copy_dev(dst, src, n, stream) bind(c, name="ekm_copy_dev")
        import :: c_ptr, c_int, c_size_t
        type(c_ptr), value :: dst, src, stream

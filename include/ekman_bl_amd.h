@@ -84,7 +84,8 @@ typedef struct ekm_params {
   int proj_maxit;          /* cycle cap of proj_mode 1, 2 */
   double proj_tol;         /* absolute residual tolerance of proj_mode 1, 2 */
   int proj_guess;          /* proj_mode 1, 2: 1 = start each solve from the previous substep's phi
-                            * scaled by the ratio of the RK weights h_bar (0 = from zero) */
+                            * scaled by the ratio of the RK weights h_bar; 2 = from the guess of the
+                            * network attached with ekm_set_warm_start_net (0 = from zero) */
   int proj_krylov;         /* proj_mode 1, 2: 1 = the outer iteration is GCR(1) (each cycle's
                             * correction made A-orthogonal to the previous one, then a minimal
                             * residual step); 0 = plain iterative refinement */
@@ -255,6 +256,57 @@ int ekm_fft_x_to_physical_dev(double* varp, int nx, int npencil, void* stream);
 int ekm_thomas_real_dev(const double* a, const double* b, const double* c, double* g,
                         int ny, int nx, void* stream);
 
+/* ---- learned initial guess of the pressure solves ------------------------
+ * No counterpart in the reference, whose MUDPACK solves always start from
+ * zero (iguess = 0, phi = 0: src/mg_solver2d.F90:157,190; the iguess branch
+ * of src/mud2cr.f:93,238).  A convolutional encoder-decoder (five levels,
+ * about 230k parameters, bf16 matrix-core products with fp32 accumulation)
+ * maps each real solve's right-hand side RHS_TMP, the previous projection's
+ * solution of the same solve and kx to a start; the consistent projection
+ * (proj_mode 1, 2 with proj_guess = 2) then iterates to the same tolerance.
+ * Planes are (NZ+2)*(NY+2) doubles, K fastest; NZ+2-1 and NY+2-1 must be
+ * multiples of 32.  Bulk arguments are device pointers (stream: hipStream_t
+ * or NULL); parameter vectors are host float arrays in the layer order of
+ * the network (per layer W[cout][cin][taps], then b[cout]). */
+typedef struct ekm_wsnet ekm_wsnet;
+/* bmax: the largest number of planes per call; trainable: also allocate the
+ * gradient, Adam and loss buffers.  Weights start He-initialised (seed 0). */
+int ekm_wsnet_create(int nz2, int ny2, int bmax, int trainable, ekm_wsnet** out);
+int ekm_wsnet_destroy(ekm_wsnet* n);
+long ekm_wsnet_nparams(const ekm_wsnet* n);
+int ekm_wsnet_set_params(ekm_wsnet* n, const float* host);
+int ekm_wsnet_get_params(ekm_wsnet* n, float* host);
+/* input normalisation: prev is divided by s*sigma (s = max|rhs| per plane),
+ * kx by kx_max */
+int ekm_wsnet_set_norm(ekm_wsnet* n, double sigma, double kx_max);
+int ekm_wsnet_get_norm(const ekm_wsnet* n, double* sigma, double* kx_max);
+/* network input x1 (nb x (NY+3)/2 x (NZ+3)/2 x 4 floats) and the scales s
+ * (nb doubles) of nb planes; prev may be NULL (zero) */
+int ekm_wsnet_prepare(ekm_wsnet* n, const double* rhs, const double* prev, const double* kx2, int nb, float* x1,
+                      double* scale, void* stream);
+/* training target phi / (s sigma) at the plane's resolution (nb planes of floats) */
+int ekm_wsnet_target(ekm_wsnet* n, const double* phi, const double* scale, int nb, float* t0, void* stream);
+/* prediction at level 1 (nb x (NY+3)/2 x (NZ+3)/2 floats; y1 may be NULL) */
+int ekm_wsnet_forward(ekm_wsnet* n, const float* x1, int nb, float* y1, void* stream);
+/* gradient of the mean relative loss  sum (up(y) - t)^2 / sum t^2  with
+ * respect to the parameters (host floats), and the loss; no update */
+int ekm_wsnet_gradient(ekm_wsnet* n, const float* x1, const float* t0, int nb, float* grad_host, float* loss);
+/* one Adam step (beta 0.9, 0.999) on that loss; returns the batch loss */
+int ekm_wsnet_train_step(ekm_wsnet* n, const float* x1, const float* t0, int nb, float lr, float* loss, void* stream);
+/* the guess of nb solves, s sigma up(y) as double planes; out may alias prev */
+int ekm_wsnet_guess(ekm_wsnet* n, const double* rhs, const double* prev, const double* kx2, int nb, double* out,
+                    void* stream);
+/* Attach a network to a solver (NULL detaches): with proj_mode 1 or 2 and
+ * proj_guess = 2, each real solve after the first projection starts from the
+ * network's guess.  The network must match the plane and hold bmax >= the
+ * solver's 2 (NX/3+1) solves; the solver does not own it. */
+int ekm_set_warm_start_net(ekm_solver* s, ekm_wsnet* net);
+/* Device pointers of the last projection's right-hand sides (RHS_TMP) and
+ * solutions (nsolve planes each, (NZ+2)*(NY+2) doubles, K fastest; solve 2m
+ * and 2m+1 belong to kx mode m) and of every solve's kx^2: what the
+ * warm-start training data is dumped from. */
+int ekm_solve_planes(ekm_solver* s, const double** rhs, const double** phi, const double** kx2, int* nsolve);
+
 /* ---- measurement and utility ------------------------------------------- */
 /* Time every multigrid launch of the solver with HIP events on its stream
  * (on = 1 resets the counters). */
@@ -264,6 +316,12 @@ int ekm_prof_get(ekm_solver* s, double* ms, long* launches, long* solves, long* 
 int ekm_copy_dev(double* dst, const double* src, size_t n, void* stream);
 int ekm_dev_alloc(void** p, size_t bytes);
 int ekm_dev_free(void* p);
+/* dst row r = src row idx[r] for r < n (device rows of row_bytes; idx is a
+ * host array); waits for the copies */
+int ekm_gather_rows(void* dst, const void* src, const long* idx, int n, size_t row_bytes, void* stream);
+/* host <-> device copies (synchronous) */
+int ekm_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int ekm_memcpy_d2h(void* dst, const void* src, size_t bytes);
 /* Host-side tables, no device needed: metrics (15*(NZ+3)*(NY+3): GMAT_11,12,22,
  * CJOB_11,12,21,22 with two faces each, then INT_JACOB) and the multigrid
  * coefficients cf(i,j,1..9) of level `lev` (1 = coarsest). */

@@ -4,7 +4,8 @@ eagerly (ekm_rk_duct_graph_launch vs ekm_rk_duct), through the C ABI.
 Both paths enqueue the same kernels with the same arguments (the substep's
 scalar decisions -- DELTA_T from the CFL minimum, the mean-pressure
 controller, the projection's checks -- run in one-thread device kernels), so
-the states must agree bit for bit.  The cases cover every variant the host
+the states must agree bit for bit (the divergence diagnostic, an atomic sum,
+to rounding).  The cases cover every variant the host
 decides for a substep: the three RK stages, the CFL update, the subgrid
 models with and without their coefficient refresh (every fifth and tenth
 step), the F_TYPE 0 controller, the sponge with its CU2 zeroing, and the
@@ -84,7 +85,11 @@ def test_graph_replay_is_bit_identical(name):
     for f, (a, b) in enumerate(zip(sa, sb)):
         assert np.array_equal(a, b), (name, f, np.abs(a - b).max())
     print(f"[graph {name}] diags {da[-1]}", flush=True)
-    assert da == db
+    # the divergence diagnostic is summed with atomics, whose order differs
+    # from run to run in either path; everything else is exact
+    for x, y in zip(da, db):
+        assert x[1:] == y[1:]
+        assert abs(x[0] - y[0]) <= 1e-12 * abs(y[0])
     assert fa == fb
     assert all(d[-1] == 0 for d in da)
 
