@@ -108,8 +108,10 @@ int launch_mean_grads(const Dims& d, const Metrics& M, const MeanGradArgs& a, hi
 // velocities U1p..U3p and the Fourier state's means
 int launch_xmoments(const Dims& d, const double* U1p, const double* U2p, const double* U3p, const double* U1,
                     const double* U2, const double* U3, double* out, hipStream_t st);
+// THOMAS_REAL on device arrays; ksp >= 0 adds THOMAS_REAL_SP's elimination of
+// row ksp+1 by row ksp with factor dd
 int launch_thomas_generic(const double* A, const double* B, const double* C, double* G, double* W, int ny,
-                          int nx, hipStream_t st);
+                          int nx, hipStream_t st, int ksp = -1, double dd = 0.0);
 
 // ---- step-state control kernels (one thread each) ---------------------------
 // constants of RK substep rk (1..3) from H_BAR: TEMP1, TEMP3, TEMP4, TEMP5

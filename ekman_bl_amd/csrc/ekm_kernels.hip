@@ -1260,20 +1260,37 @@ __global__ void k_div_reduce(Dims d, const double* __restrict__ S, double* __res
 
 // generic THOMAS_REAL with per-system coefficients (the drop-in entry point):
 // systems i = 0..nx, unknowns j = 0..ny, element (i,j) at i + (nx+1) j.
+// With ksp >= 0 it is THOMAS_REAL_SP (src/boundary.F90:4-47): row ksp+1 is
+// first eliminated by row ksp with the factor dd, on the fly (the inputs are
+// not modified).
 __global__ void k_thomas_generic(const double* __restrict__ A, const double* __restrict__ B,
                                  const double* __restrict__ C, double* __restrict__ G,
-                                 double* __restrict__ W, int ny, int nx) {
+                                 double* __restrict__ W, int ny, int nx, int ksp, double dd) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > nx) return;
   size_t ld = (size_t)nx + 1;
+  double ak = 0.0, bk = 0.0, ck = 0.0, gk = 0.0;
+  if (ksp >= 0) {
+    const size_t p = i + ld * ksp;
+    ak = A[p];
+    bk = B[p];
+    ck = C[p];
+    gk = G[p];
+  }
   double bp = B[i];
   W[i] = bp;
   double g = G[i];
   for (int j = 0; j < ny; ++j) {
     size_t qn = i + ld * (j + 1);
-    double mlt = -A[qn] / bp;
-    bp = B[qn] + mlt * C[qn - ld];
-    g = G[qn] + mlt * g;
+    double an = A[qn], bn = B[qn], gn = G[qn];
+    if (j == ksp) {
+      an = an - bk * dd / ak;
+      bn = bn - ck * dd / ak;
+      gn = gn - gk * dd / ak;
+    }
+    double mlt = -an / bp;
+    bp = bn + mlt * C[qn - ld];
+    g = gn + mlt * g;
     W[qn] = bp;
     G[qn] = g;
   }
@@ -1443,8 +1460,9 @@ int launch_div_reduce(const Dims& d, const double* S, double* out, hipStream_t s
   return EKM_OK;
 }
 int launch_thomas_generic(const double* A, const double* B, const double* C, double* G, double* W, int ny,
-                          int nx, hipStream_t st) {
-  hipLaunchKernelGGL(k_thomas_generic, dim3(nblk((size_t)nx + 1, 256)), dim3(256), 0, st, A, B, C, G, W, ny, nx);
+                          int nx, hipStream_t st, int ksp, double dd) {
+  hipLaunchKernelGGL(k_thomas_generic, dim3(nblk((size_t)nx + 1, 256)), dim3(256), 0, st, A, B, C, G, W, ny, nx,
+                     ksp, dd);
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }

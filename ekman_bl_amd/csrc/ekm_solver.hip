@@ -1002,19 +1002,26 @@ extern "C" int ekm_get_diag(ekm_solver* s, ekm_diag* dg) {
   return EKM_OK;
 }
 
-extern "C" int ekm_multigrid_mudpack(ekm_solver* s, int nsolve, const double* kx2, const double* rhs_tmp, double* phi,
-                                     int* cycles) {
-  if (nsolve < 1 || nsolve > s->mg.nsolve_max) return EKM_ERR_ARG;
+extern "C" int ekm_multigrid_mudpack_ex(ekm_solver* s, int nsolve, const double* kx2, const double* rhs_tmp,
+                                        double* phi, int* cycles, int iguess) {
+  if (nsolve < 1 || nsolve > s->mg.nsolve_max || iguess < 0 || iguess > 1) return EKM_ERR_ARG;
   s->have_prev_phi = 0;
   size_t pl = s->d.plane;
   s->kx2_dirty = 1;
   EKM_CHECK(hipMemcpyAsync(s->mg.kx2, kx2, nsolve * sizeof(double), hipMemcpyHostToDevice, s->st));
   EKM_CHECK(hipMemcpyAsync(s->mg.rhs, rhs_tmp, nsolve * pl * sizeof(double), hipMemcpyHostToDevice, s->st));
-  TRY(mg_solve_ex(s->mg, s->d, nsolve, 0, s->st));
+  // iguess = 1: phi holds the initial guesses (src/mud2cr.f:93,238)
+  if (iguess) EKM_CHECK(hipMemcpyAsync(s->mg.phi, phi, nsolve * pl * sizeof(double), hipMemcpyHostToDevice, s->st));
+  TRY(mg_solve_ex(s->mg, s->d, nsolve, iguess, s->st));
   EKM_CHECK(hipMemcpyAsync(phi, s->mg.phi, nsolve * pl * sizeof(double), hipMemcpyDeviceToHost, s->st));
   if (cycles) EKM_CHECK(hipMemcpyAsync(cycles, s->mg.cycles, nsolve * sizeof(int), hipMemcpyDeviceToHost, s->st));
   EKM_CHECK(hipStreamSynchronize(s->st));
   return EKM_OK;
+}
+
+extern "C" int ekm_multigrid_mudpack(ekm_solver* s, int nsolve, const double* kx2, const double* rhs_tmp, double* phi,
+                                     int* cycles) {
+  return ekm_multigrid_mudpack_ex(s, nsolve, kx2, rhs_tmp, phi, cycles, 0);
 }
 
 // ---- component drop-ins -----------------------------------------------------
@@ -1033,15 +1040,27 @@ static int fft_host(double* varp, int nx, int npencil, int inverse) {
 extern "C" int ekm_fft_x_to_fourier(double* varp, int nx, int npencil) { return fft_host(varp, nx, npencil, 0); }
 extern "C" int ekm_fft_x_to_physical(double* varp, int nx, int npencil) { return fft_host(varp, nx, npencil, 1); }
 
-extern "C" int ekm_thomas_real_dev(const double* a, const double* b, const double* c, double* g, int ny, int nx,
-                                   void* stream) {
+static int thomas_dev(const double* a, const double* b, const double* c, double* g, int ny, int nx, int ksp,
+                      double dd, void* stream) {
+  if (nx < 0 || ny < 1 || ksp >= ny) return EKM_ERR_ARG;
   double* w = nullptr;
   size_t n = (size_t)(nx + 1) * (ny + 1);
   EKM_CHECK(hipMalloc(&w, n * sizeof(double)));
-  int st = launch_thomas_generic(a, b, c, g, w, ny, nx, (hipStream_t)stream);
+  int st = launch_thomas_generic(a, b, c, g, w, ny, nx, (hipStream_t)stream, ksp, dd);
   if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) st = EKM_ERR_HIP;
   (void)hipFree(w);
   return st;
+}
+
+extern "C" int ekm_thomas_real_dev(const double* a, const double* b, const double* c, double* g, int ny, int nx,
+                                   void* stream) {
+  return thomas_dev(a, b, c, g, ny, nx, -1, 0.0, stream);
+}
+
+extern "C" int ekm_thomas_real_sp_dev(const double* a, const double* b, const double* c, double* g, double d, int k,
+                                      int ny, int nx, void* stream) {
+  if (k < 0) return EKM_ERR_ARG;
+  return thomas_dev(a, b, c, g, ny, nx, k, d, stream);
 }
 
 static int thomas_host(double* a, double* b, double* c, double* g, int ny, int nx) {

@@ -272,6 +272,24 @@ module ekman_bl_amd
        type(c_ptr), value :: p
      end function ekm_dev_free
 
+     integer(c_int) function ekm_thomas_real_sp_dev(a, b, c, g, d, k, ny, nx, stream) &
+         bind(c, name="ekm_thomas_real_sp_dev")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: a, b, c, g, stream
+       real(c_double), value :: d
+       integer(c_int), value :: k, ny, nx
+     end function ekm_thomas_real_sp_dev
+
+     integer(c_int) function ekm_multigrid_mudpack_ex(s, nsolve, kx2, rhs_tmp, phi, cycles, iguess) &
+         bind(c, name="ekm_multigrid_mudpack_ex")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: s
+       integer(c_int), value :: nsolve, iguess
+       real(c_double), intent(in) :: kx2(*), rhs_tmp(*)
+       real(c_double), intent(inout) :: phi(*)
+       integer(c_int), intent(out) :: cycles(*)
+     end function ekm_multigrid_mudpack_ex
+
      integer(c_int) function ekm_memcpy_h2d(dst, src, bytes) bind(c, name="ekm_memcpy_h2d")
        import :: c_ptr, c_int, c_size_t
        type(c_ptr), value :: dst, src

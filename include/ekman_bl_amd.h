@@ -249,12 +249,22 @@ int ekm_thomas_real_sp(double* a, double* b, double* c, double* g, double d, int
  * flags are the solver's. */
 int ekm_multigrid_mudpack(ekm_solver* s, int nsolve, const double* kx2,
                           const double* rhs_tmp, double* phi, int* cycles);
+/* The same with MUDPACK's iguess (iparm(4), src/mud2cr.f:93,238): 1 = phi
+ * holds the initial guesses on entry, 0 = start from zero as
+ * MULTIGRID_MUDPACK does (src/mg_solver2d.F90:157,190). */
+int ekm_multigrid_mudpack_ex(ekm_solver* s, int nsolve, const double* kx2, const double* rhs_tmp, double* phi,
+                             int* cycles, int iguess);
 
 /* ---- device-pointer variants (stream = hipStream_t or NULL) -------------- */
 int ekm_fft_x_to_fourier_dev(double* varp, int nx, int npencil, void* stream);
 int ekm_fft_x_to_physical_dev(double* varp, int nx, int npencil, void* stream);
 int ekm_thomas_real_dev(const double* a, const double* b, const double* c, double* g,
                         int ny, int nx, void* stream);
+/* THOMAS_REAL_SP(A,B,C,G,D,K,NY,NX) (src/boundary.F90:4-47) on device arrays;
+ * the elimination of row K+1 by row K is applied on the fly (A, B, C are not
+ * modified). */
+int ekm_thomas_real_sp_dev(const double* a, const double* b, const double* c, double* g, double d, int k,
+                           int ny, int nx, void* stream);
 
 /* ---- learned initial guess of the pressure solves ------------------------
  * No counterpart in the reference, whose MUDPACK solves always start from

@@ -61,6 +61,66 @@ def test_thomas_matches_oracle(oracle_lib):
     assert rel(g3, g4) < 1e-14
 
 
+def test_thomas_sp_device_drop_in_matches_host_path():
+    """ekm_thomas_real_sp_dev (device arrays, elimination on the fly) equals
+    the host-pointer THOMAS_REAL_SP bit for bit, and leaves A, B, C as given."""
+    import ctypes
+    from ekman_bl_amd import warmstart as WS
+    rng = np.random.default_rng(4)
+    ny, nx = 33, 70
+    a = rng.uniform(-1, 0, (ny + 1, nx + 1))
+    c = rng.uniform(-1, 0, (ny + 1, nx + 1))
+    b = 2.5 + rng.uniform(0, 1, (ny + 1, nx + 1))
+    g = rng.standard_normal((ny + 1, nx + 1))
+    gh = g.copy()
+    S.thomas_real_sp(a.copy(), b.copy(), c.copy(), gh, 0.7, ny - 2)
+    L = S.lib()
+    L.ekm_thomas_real_sp_dev.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                                                  ctypes.c_int, ctypes.c_void_p]
+    dev = [WS.DeviceArray(x.size, np.float64) for x in (a, b, c, g)]
+    for d, x in zip(dev, (a, b, c, g)):
+        d.upload(x)
+    assert L.ekm_thomas_real_sp_dev(*[d.ptr for d in dev], 0.7, ny - 2, ny, nx, None) == 0
+    assert np.array_equal(dev[3].download().reshape(g.shape), gh)
+    assert np.array_equal(dev[0].download().reshape(a.shape), a)
+
+
+def test_multigrid_iguess_starts_from_the_given_phi():
+    """MUDPACK's iguess = 1 (src/mud2cr.f:93,238) through
+    ekm_multigrid_mudpack_ex: the reference's 6 W cycles from zero stop short
+    of relmax 1e-6 on the stretched grid; another call started from that
+    result continues the iteration and lands closer to the converged
+    solution (600 cycles to relmax 1e-13)."""
+    import ctypes
+    cfg, p, xp, yp = _setup(1, (2, 2, 2, 1))
+    s = S.DuctSolver(p, xp, yp)
+    q = p.copy()
+    q.mg_maxcy, q.mg_tolmax = 600, 1e-13
+    s_ref = S.DuctSolver(q, xp, yp)
+    rng = np.random.default_rng(8)
+    kx2 = np.array([0.0, 0.25, 4.0])
+    rhs = rng.standard_normal((len(kx2), p.ny + 2, p.nz + 2))
+    phi0, cyc0 = s.multigrid_mudpack(kx2, rhs)
+    phi_star, _ = s_ref.multigrid_mudpack(kx2, rhs)
+    L = S.lib()
+    _D = ctypes.POINTER(ctypes.c_double)
+    L.ekm_multigrid_mudpack_ex.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, ctypes.POINTER(ctypes.c_int),
+                                           ctypes.c_int]
+    phi = np.ascontiguousarray(phi0.copy())
+    cyc = (ctypes.c_int * len(kx2))()
+    k = np.ascontiguousarray(kx2)
+    r = np.ascontiguousarray(rhs)
+    assert L.ekm_multigrid_mudpack_ex(s.h, len(kx2), k.ctypes.data_as(_D), r.ctypes.data_as(_D),
+                                      phi.ctypes.data_as(_D), cyc, 1) == 0
+    for m in range(len(kx2)):
+        e0, e1 = rel(phi0[m], phi_star[m]), rel(phi[m], phi_star[m])
+        print(f"[iguess] solve {m}: cycles {cyc0[m]} from zero, then {cyc[m]} more; error {e0:.2e} -> {e1:.2e}",
+              flush=True)
+        assert e1 < 0.5 * e0
+    s.close()
+    s_ref.close()
+
+
 def _setup(n, bcs=None):
     cfg = config(n)
     p = cfg.params
