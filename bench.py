@@ -72,11 +72,18 @@ def make_params(cfg, mode: int, tol: float, krylov: int, guess: int = 0):
     return p
 
 
+def tuning_of(cfg, mode: int):
+    """The autotuner's cached kernel tuning of this workload and form on this
+    device (ekman_bl_amd/data/tuning.json), or None (the defaults)."""
+    from ekman_bl_amd import autotune as AT
+    return AT.load(cfg.name, mode)
+
+
 def run_form(cfg, xp, yp, ic, mode, steps, warmup, tol, krylov, graph=False, net=None, prof=False,
              keep_state=False):
     """warmup + steps steps of one form; the timed part is the `steps` steps."""
     p = make_params(cfg, mode, tol, krylov, 2 if net is not None else 0)
-    s = S.DuctSolver(p, xp, yp)
+    s = S.DuctSolver(p, xp, yp, tuning=tuning_of(cfg, mode))
     if net is not None:
         net.attach(s)
     s.set_initial_physical(*ic)
@@ -324,6 +331,7 @@ def main() -> int:
                      "avg_launch_ms": round(avg_ms, 4), "cycles_per_launch": round(cyc_per_launch, 1),
                      "algorithmic_bytes_per_launch": int(launch_bytes)},
         "forms": forms,
+        "tuning": tuning_of(cfg, args.mode),
     }
     if not args.no_cpu:
         log("[bench] CPU oracle baseline")

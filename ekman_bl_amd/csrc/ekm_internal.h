@@ -164,6 +164,7 @@ struct TightPlan {
   int fp32 = 0, cm = 0, nsolve_max = 0;
   bool cross = false;                 // operator has cross-derivative terms (9-point)
   int lds_bytes = 0;                  // dynamic LDS of the solve kernel (coarse levels)
+  long lds_budget = 0;                // cap on it in bytes (0: what the device allows)
   long ws_elems = 0, ws64_elems = 0;  // per solve
   char* mem = nullptr;                // descriptor + coefficient tables
   void* fac = nullptr;                // line factors [live mode][...]
@@ -172,7 +173,7 @@ struct TightPlan {
 };
 // kx2_mode: kx^2 of modes 0..nmode-1 (solve s uses mode s/2)
 int tight_plan_build(TightPlan& tp, const ekm_params& p, const Dims& d, const HostMetrics& hm, const Ranges& rv,
-                     const double* kx2_mode, int nmode, int nsolve_max);
+                     const double* kx2_mode, int nmode, int nsolve_max, long lds_budget = 0);
 void tight_plan_free(TightPlan& tp);
 // solves nsolve planes of mg.rhs (RHS_TMP) into mg.phi; V-cycle counts in
 // mg.cycles.  guess (device pointer, may be null): when *guess != 0, start

@@ -53,7 +53,7 @@ struct AdiT1 {
 // descriptors in device memory, hf the same on the host (for the grid sizes).
 // th_corners applies the scalar corner rows to field th_field afterwards.
 int launch_adi_batch(const Dims& d, const Metrics& M, const AdiField* fs, const AdiField* hf, int nf,
-                     const AdiT1& t1, int th_field, int th_corners, hipStream_t st);
+                     const AdiT1& t1, int th_field, int th_corners, hipStream_t st, const ekm_tuning& tn);
 int launch_cfl(const Dims& d, const Metrics& M, const Ranges& r, const double* U1, const double* U2,
                const double* U3, double cfl, double dx, unsigned long long* out, hipStream_t st);
 int launch_faces(const Dims& d, const Metrics& M, const Ranges& r, const double* CU2, const double* CU3,

@@ -294,8 +294,9 @@ __device__ inline double bc_rhs(const Dims& d, const Metrics& M, const BcRow& e,
 // The sweeps below run each line's recurrence in blocks of kSw rows: the
 // right-hand sides and factors of a block are loaded before its recurrence
 // steps, so the memory latency is paid once per block instead of per row.
-constexpr int kSw = 8;
-
+// kSw and the workgroup size are tuning parameters (ekm_tuning); the
+// arithmetic does not depend on them.
+template <int kSw>
 __global__ void __launch_bounds__(256) k_ysweep(Dims d, Metrics M, const AdiField* __restrict__ fs, AdiT1 t1s) {
   const AdiField& F = fs[blockIdx.y];
   if (F.nut) return;
@@ -372,6 +373,7 @@ __global__ void __launch_bounds__(256) k_ysweep(Dims d, Metrics M, const AdiFiel
 }
 
 // z sweep for lines (i,J) (:1503-1596 and analogues)
+template <int kSw>
 __global__ void __launch_bounds__(256) k_zsweep(Dims d, Metrics M, const AdiField* __restrict__ fs) {
   const AdiField& F = fs[blockIdx.y];
   if (F.nut) return;
@@ -439,9 +441,8 @@ __global__ void __launch_bounds__(256) k_zsweep(Dims d, Metrics M, const AdiFiel
   }
 }
 
-// rows per batch of the sweeps with a subgrid viscosity, which keep more
-// operands per row in flight than the plain sweeps
-constexpr int kSwL = 6;
+// kSwL: rows per batch of the sweeps with a subgrid viscosity, which keep
+// more operands per row in flight than the plain sweeps (a tuning parameter)
 
 // ---------------------------------------------------------------------------
 // Sweeps with the subgrid viscosity (fields with nut): every (i, line) has its
@@ -452,6 +453,7 @@ constexpr int kSwL = 6;
 // the upper coefficients again.  Thomas arithmetic as THOMAS_REAL(_SP)
 // (src/boundary.F90:1-84).
 // ---------------------------------------------------------------------------
+template <int kSwL>
 __global__ void __launch_bounds__(256) k_ysweep_les(Dims d, Metrics M, const AdiField* __restrict__ fs, AdiT1 t1s) {
   const AdiField& F = fs[blockIdx.y];
   if (!F.nut) return;
@@ -568,6 +570,7 @@ __global__ void __launch_bounds__(256) k_ysweep_les(Dims d, Metrics M, const Adi
   }
 }
 
+template <int kSwL>
 __global__ void __launch_bounds__(256) k_zsweep_les(Dims d, Metrics M, const AdiField* __restrict__ fs, AdiT1 t1s) {
   const AdiField& F = fs[blockIdx.y];
   if (!F.nut) return;
@@ -1324,8 +1327,9 @@ int launch_contravariant(const Dims& d, const Metrics& M, const Ranges& r, const
   return EKM_OK;
 }
 int launch_adi_batch(const Dims& d, const Metrics& M, const AdiField* fs, const AdiField* hf, int nf,
-                     const AdiT1& t1, int th_field, int th_corners, hipStream_t st) {
+                     const AdiT1& t1, int th_field, int th_corners, hipStream_t st, const ekm_tuning& tn) {
   if (nf < 1 || nf > kAdiMaxFields) return EKM_ERR_ARG;
+  const int bs = tn.adi_block;
   if (d.NY + 2 > kLuMaxRows || d.NZ + 2 > kLuMaxRows) return EKM_ERR_UNSUPPORTED;
   size_t npre = 0, nly = 0, nlz = 0, nys = 0, nzs = 0;
   bool plain = false, les = false;
@@ -1351,19 +1355,39 @@ int launch_adi_batch(const Dims& d, const Metrics& M, const AdiField* fs, const 
     EKM_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_lu, dim3((unsigned)nlz, nf), dim3(kLuBlock), 0, st, d, M, fs, t1, 1);
     EKM_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_ysweep, dim3(nblk(nys, 256), nf), dim3(256), 0, st, d, M, fs, t1);
+    const dim3 g(nblk(nys, bs), nf), b(bs);
+    switch (tn.adi_rows) {
+      case 4: hipLaunchKernelGGL(k_ysweep<4>, g, b, 0, st, d, M, fs, t1); break;
+      case 12: hipLaunchKernelGGL(k_ysweep<12>, g, b, 0, st, d, M, fs, t1); break;
+      default: hipLaunchKernelGGL(k_ysweep<8>, g, b, 0, st, d, M, fs, t1); break;
+    }
     EKM_CHECK(hipGetLastError());
   }
   if (les) {
-    hipLaunchKernelGGL(k_ysweep_les, dim3(nblk(nys, 256), nf), dim3(256), 0, st, d, M, fs, t1);
+    const dim3 g(nblk(nys, 256), nf), b(256);
+    switch (tn.adi_les_rows) {
+      case 4: hipLaunchKernelGGL(k_ysweep_les<4>, g, b, 0, st, d, M, fs, t1); break;
+      case 8: hipLaunchKernelGGL(k_ysweep_les<8>, g, b, 0, st, d, M, fs, t1); break;
+      default: hipLaunchKernelGGL(k_ysweep_les<6>, g, b, 0, st, d, M, fs, t1); break;
+    }
     EKM_CHECK(hipGetLastError());
   }
   if (plain) {
-    hipLaunchKernelGGL(k_zsweep, dim3(nblk(nzs, 256), nf), dim3(256), 0, st, d, M, fs);
+    const dim3 g(nblk(nzs, bs), nf), b(bs);
+    switch (tn.adi_rows) {
+      case 4: hipLaunchKernelGGL(k_zsweep<4>, g, b, 0, st, d, M, fs); break;
+      case 12: hipLaunchKernelGGL(k_zsweep<12>, g, b, 0, st, d, M, fs); break;
+      default: hipLaunchKernelGGL(k_zsweep<8>, g, b, 0, st, d, M, fs); break;
+    }
     EKM_CHECK(hipGetLastError());
   }
   if (les) {
-    hipLaunchKernelGGL(k_zsweep_les, dim3(nblk(nzs, 256), nf), dim3(256), 0, st, d, M, fs, t1);
+    const dim3 g(nblk(nzs, 256), nf), b(256);
+    switch (tn.adi_les_rows) {
+      case 4: hipLaunchKernelGGL(k_zsweep_les<4>, g, b, 0, st, d, M, fs, t1); break;
+      case 8: hipLaunchKernelGGL(k_zsweep_les<8>, g, b, 0, st, d, M, fs, t1); break;
+      default: hipLaunchKernelGGL(k_zsweep_les<6>, g, b, 0, st, d, M, fs, t1); break;
+    }
     EKM_CHECK(hipGetLastError());
   }
   if (th_corners && th_field >= 0 && th_field < nf) {

@@ -967,7 +967,7 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
     EKM_CHECK(hipGetDevice(&dev));
     EKM_CHECK(hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     long budget = (long)maxsh - stat - 256;
-    if (const char* e = getenv("EKM_TIGHT_LDS_BYTES")) budget = std::min(budget, atol(e));
+    if (tp.lds_budget > 0) budget = std::min(budget, tp.lds_budget);
     long lds = 0;
     bool fits = true;
     for (int k = 1; k <= ng; ++k) {
@@ -1109,8 +1109,9 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
 }  // namespace
 
 int tight_plan_build(TightPlan& tp, const ekm_params& p, const Dims& d, const HostMetrics& hm, const Ranges& rv,
-                     const double* kx2_mode, int nmode, int nsolve_max) {
+                     const double* kx2_mode, int nmode, int nsolve_max, long lds_budget) {
   tp.fp32 = p.proj_mode == 2;
+  tp.lds_budget = lds_budget;
   const int nf = d.NZ2 > d.NY2 ? d.NZ2 : d.NY2;  // longest line of the finest level
   tp.cm = nf <= 320 ? 5 : nf <= 576 ? 9 : nf <= 832 ? 13 : 0;
   if (!tp.cm) return EKM_ERR_UNSUPPORTED;

@@ -33,6 +33,7 @@ void ekm_note_hip_error(hipError_t e, const char* what, const char* file, int li
 
 struct ekm_solver {
   ekm_params p;
+  ekm_tuning tn;
   Dims d;
   Ranges rv, rt;
   hipStream_t st = nullptr;
@@ -101,6 +102,23 @@ extern "C" {
 
 const char* ekm_version(void) { return "ekman_bl_amd 0.1.0 gfx950"; }
 const char* ekm_last_error(void) { return g_last_error.c_str(); }
+
+void ekm_tuning_default(ekm_tuning* t) {
+  t->adi_rows = 8;
+  t->adi_block = 256;
+  t->adi_les_rows = 6;
+  t->mg_block = 512;
+  t->tight_lds_kib = 0;
+}
+
+static int check_tuning(const ekm_tuning& t) {
+  if (t.adi_rows != 4 && t.adi_rows != 8 && t.adi_rows != 12) return EKM_ERR_ARG;
+  if (t.adi_block != 64 && t.adi_block != 128 && t.adi_block != 256) return EKM_ERR_ARG;
+  if (t.adi_les_rows != 4 && t.adi_les_rows != 6 && t.adi_les_rows != 8) return EKM_ERR_ARG;
+  if (t.mg_block != 256 && t.mg_block != 512 && t.mg_block != 1024) return EKM_ERR_ARG;
+  if (t.tight_lds_kib < 0) return EKM_ERR_ARG;
+  return EKM_OK;
+}
 
 void ekm_params_default(ekm_params* p) {
   std::memset(p, 0, sizeof(*p));
@@ -289,11 +307,15 @@ static int inv(ekm_solver* s, double* f) { return fft_inverse(s->fft, f, (long)s
     if (e__ != EKM_OK) return e__; \
   } while (0)
 
-extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const double* ypoint, const double* th_bar,
-                          ekm_solver** out) {
+extern "C" int ekm_create_tuned(const ekm_params* pp, const double* xpoint, const double* ypoint,
+                                const double* th_bar, const ekm_tuning* tune, ekm_solver** out) {
   if (!pp || !xpoint || !ypoint || !out) return EKM_ERR_ARG;
   *out = nullptr;
   TRY(check_params(*pp));
+  ekm_tuning tn;
+  ekm_tuning_default(&tn);
+  if (tune) tn = *tune;
+  TRY(check_tuning(tn));
   // a scalar without the constant stratification reads its background TH_BAR
   // (src/diablo.F90:1152-1160), which the caller must then pass
   if (pp->n_th && !pp->cont_strat && !th_bar) return EKM_ERR_ARG;
@@ -301,6 +323,7 @@ extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const doub
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return EKM_ERR_NODEV;
   ekm_solver* s = new ekm_solver();
   s->p = *pp;
+  s->tn = tn;
   const ekm_params& p = s->p;
   Dims& d = s->d;
   d.NX = p.nx; d.NY = p.ny; d.NZ = p.nz;
@@ -489,16 +512,13 @@ extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const doub
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&lds_bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
       return fail(EKM_ERR_HIP);
-    int block = 512;
-    size_t budget = (size_t)lds_bytes / sizeof(double);
-    if (const char* e = getenv("EKM_MG_BLOCK")) block = atoi(e);
-    if (const char* e = getenv("EKM_MG_LDS_DOUBLES")) budget = std::min(budget, (size_t)atoll(e));
-    if (block < 64 || block > 1024 || (block & (block - 1))) return fail(EKM_ERR_ARG);  // power of two: tree reduction
+    const int block = tn.mg_block;  // a power of two: tree reductions
+    const size_t budget = (size_t)lds_bytes / sizeof(double);
     st = mg_plan_build(s->mg, p, d, s->hm, nsolve, block, budget);
   }
   if (st != EKM_OK) return fail(st);
   if (p.proj_mode > 0) {
-    st = tight_plan_build(s->tp, p, d, s->hm, s->rv, kx2.data(), d.NKX + 1, nsolve);
+    st = tight_plan_build(s->tp, p, d, s->hm, s->rv, kx2.data(), d.NKX + 1, nsolve, (long)tn.tight_lds_kib * 1024);
     if (st != EKM_OK) return fail(st);
   }
   s->cyc_host.resize(nsolve);
@@ -506,6 +526,17 @@ extern "C" int ekm_create(const ekm_params* pp, const double* xpoint, const doub
   s->zeta_bar[0] = 0.0; s->zeta_bar[1] = -17.0 / 8.0; s->zeta_bar[2] = -5.0 / 4.0;
   if (hipStreamSynchronize(s->st) != hipSuccess) return fail(EKM_ERR_HIP);
   *out = s;
+  return EKM_OK;
+}
+
+extern "C" int ekm_create(const ekm_params* p, const double* xpoint, const double* ypoint, const double* th_bar,
+                          ekm_solver** out) {
+  return ekm_create_tuned(p, xpoint, ypoint, th_bar, nullptr, out);
+}
+
+extern "C" int ekm_get_tuning(ekm_solver* s, ekm_tuning* t) {
+  if (!t) return EKM_ERR_ARG;
+  *t = s->tn;
   return EKM_OK;
 }
 
@@ -874,7 +905,7 @@ static int enqueue_substep(ekm_solver* s, const SubstepPlan& q, bool timed) {
     const int corners = th ? ((p.th_bc[EKM_ZMAX] == 1 ? 1 : 0) | (p.th_bc[EKM_ZMIN] == 6 ? 2 : 0)) : 0;
     AdiT1 t1{s->ss, {1.0, 1.0, 1.0, p.pr}};
     TRY(launch_adi_batch(d, s->M, q.les_on ? s->adi_dl : s->adi_d, q.les_on ? s->adi_hl : s->adi_h, s->adi_nf, t1,
-                         3, corners, st));
+                         3, corners, st, s->tn));
   }
   // CFL time step at the third substep (:2417-2419)
   if (q.cfl_now) {

@@ -62,6 +62,11 @@ module ekman_bl_amd
      integer(c_int) :: time_step, rk_step, mg_solves, mg_cycles, mg_max_cycles, status
   end type ekm_diag
 
+  ! ekm_tuning: kernel launch and memory-placement parameters
+  type, bind(c) :: ekm_tuning
+     integer(c_int) :: adi_rows, adi_block, adi_les_rows, mg_block, tight_lds_kib
+  end type ekm_tuning
+
   type(c_ptr), save :: ekm_handle = c_null_ptr
 
   interface
@@ -170,6 +175,27 @@ module ekman_bl_amd
        type(c_ptr), value :: s
        real(c_double), intent(in) :: u1(*), u2(*), u3(*), th(*)
      end function ekm_set_initial_physical
+
+     ! kernel tuning (launch and memory placement; results do not depend on it)
+     subroutine ekm_tuning_default(t) bind(c, name="ekm_tuning_default")
+       import :: ekm_tuning
+       type(ekm_tuning), intent(out) :: t
+     end subroutine ekm_tuning_default
+
+     integer(c_int) function ekm_create_tuned(p, xpoint, ypoint, th_bar, t, s) bind(c, name="ekm_create_tuned")
+       import :: c_ptr, c_int, c_double, ekm_params, ekm_tuning
+       type(ekm_params), intent(in) :: p
+       real(c_double), intent(in) :: xpoint(*), ypoint(*)
+       type(c_ptr), value :: th_bar
+       type(ekm_tuning), intent(in) :: t
+       type(c_ptr), intent(out) :: s
+     end function ekm_create_tuned
+
+     integer(c_int) function ekm_get_tuning(s, t) bind(c, name="ekm_get_tuning")
+       import :: c_ptr, c_int, ekm_tuning
+       type(c_ptr), value :: s
+       type(ekm_tuning), intent(out) :: t
+     end function ekm_get_tuning
 
      integer(c_int) function ekm_rk_duct(s) bind(c, name="ekm_rk_duct")
        import :: c_ptr, c_int

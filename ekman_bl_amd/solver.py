@@ -45,6 +45,28 @@ class EkmDiag(ctypes.Structure):
                 ("mg_max_cycles", ctypes.c_int), ("status", ctypes.c_int)]
 
 
+class EkmTuning(ctypes.Structure):
+    """ekm_tuning: launch and memory-placement parameters (bit-identical results)."""
+    _fields_ = [("adi_rows", ctypes.c_int), ("adi_block", ctypes.c_int), ("adi_les_rows", ctypes.c_int),
+                ("mg_block", ctypes.c_int), ("tight_lds_kib", ctypes.c_int)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+    @classmethod
+    def from_dict(cls, d: dict) -> "EkmTuning":
+        t = default_tuning()
+        for k, v in d.items():
+            setattr(t, k, int(v))
+        return t
+
+
+def default_tuning() -> EkmTuning:
+    t = EkmTuning()
+    lib().ekm_tuning_default(ctypes.byref(t))
+    return t
+
+
 def lib():
     """Load libekman_bl_amd.so; raises when it is not built."""
     global _lib
@@ -57,6 +79,10 @@ def lib():
         L.ekm_last_error.restype = ctypes.c_char_p
         L.ekm_params_default.argtypes = [ctypes.POINTER(EkmParams)]
         L.ekm_create.argtypes = [ctypes.POINTER(EkmParams), _D, _D, _D, ctypes.POINTER(vp)]
+        L.ekm_create_tuned.argtypes = [ctypes.POINTER(EkmParams), _D, _D, _D, ctypes.POINTER(EkmTuning),
+                                       ctypes.POINTER(vp)]
+        L.ekm_tuning_default.argtypes = [ctypes.POINTER(EkmTuning)]
+        L.ekm_get_tuning.argtypes = [vp, ctypes.POINTER(EkmTuning)]
         L.ekm_destroy.argtypes = [vp]
         L.ekm_sync.argtypes = [vp]
         L.ekm_rk_duct.argtypes = [vp]
@@ -115,15 +141,23 @@ def _c(a: np.ndarray):
 class DuctSolver:
     """Device-resident duct state on the current HIP device."""
 
-    def __init__(self, p: EkmParams, xpoint: np.ndarray, ypoint: np.ndarray, th_bar=None):
+    def __init__(self, p: EkmParams, xpoint: np.ndarray, ypoint: np.ndarray, th_bar=None, tuning=None):
         self.p = p.copy()
         xp = np.ascontiguousarray(xpoint, dtype=np.float64)
         yp = np.ascontiguousarray(ypoint, dtype=np.float64)
         tb = None if th_bar is None else np.ascontiguousarray(th_bar, dtype=np.float64)
         h = ctypes.c_void_p()
-        _check(lib().ekm_create(ctypes.byref(self.p), _p(xp), _p(yp), _p(tb) if tb is not None else None,
-                                ctypes.byref(h)), "ekm_create")
+        if isinstance(tuning, dict):
+            tuning = EkmTuning.from_dict(tuning)
+        _check(lib().ekm_create_tuned(ctypes.byref(self.p), _p(xp), _p(yp), _p(tb) if tb is not None else None,
+                                      ctypes.byref(tuning) if tuning is not None else None, ctypes.byref(h)),
+               "ekm_create")
         self.h = h
+
+    def tuning(self) -> EkmTuning:
+        t = EkmTuning()
+        _check(lib().ekm_get_tuning(self.h, ctypes.byref(t)), "ekm_get_tuning")
+        return t
 
     def close(self):
         if getattr(self, "h", None):

@@ -123,6 +123,19 @@ typedef struct ekm_diag {
 
 typedef struct ekm_solver ekm_solver;  /* opaque: device-resident duct state */
 
+/* Launch and memory-placement parameters of the solver's kernels (no
+ * counterpart in the reference).  None changes the arithmetic: a solver gives
+ * bit-identical results for every allowed value; the project's autotuner
+ * (ekman_bl_amd/autotune.py) picks the fastest per configuration. */
+typedef struct ekm_tuning {
+  int adi_rows;       /* rows per batch of the plain ADI sweeps: 4, 8 or 12 (default 8) */
+  int adi_block;      /* threads per workgroup of the plain ADI sweeps: 64, 128 or 256 (default 256) */
+  int adi_les_rows;   /* rows per batch of the ADI sweeps with a subgrid viscosity: 4, 6 or 8 (default 6) */
+  int mg_block;       /* threads per workgroup of the MUDPACK replica: 256, 512 or 1024 (default 512) */
+  int tight_lds_kib;  /* LDS cap of the consistent projection's coarse levels in KiB (0: the device's maximum) */
+} ekm_tuning;
+void ekm_tuning_default(ekm_tuning* t);
+
 /* ---- whole-substep drop-in (replaces RK_DUCT, src/duct.F90:237) ---------- */
 
 /* Create a solver on the current HIP device. xpoint/ypoint are GRID_YZ.dat
@@ -131,6 +144,10 @@ typedef struct ekm_solver ekm_solver;  /* opaque: device-resident duct state */
  * cont_strat == 0, else NULL (src/diablo.F90:1152-1160). */
 int ekm_create(const ekm_params* p, const double* xpoint, const double* ypoint,
                const double* th_bar, ekm_solver** out);
+/* The same with the kernels' tuning (NULL: the defaults). */
+int ekm_create_tuned(const ekm_params* p, const double* xpoint, const double* ypoint, const double* th_bar,
+                     const ekm_tuning* t, ekm_solver** out);
+int ekm_get_tuning(ekm_solver* s, ekm_tuning* t);
 int ekm_destroy(ekm_solver* s);
 
 /* Upload / download the Fourier state CU1, CU2, CU3, CP, CTH (host arrays,

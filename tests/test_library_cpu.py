@@ -231,3 +231,17 @@ def test_fortran_module_binds_every_entry_point():
     bound = set(re.findall(r'bind\(c,\s*name="(ekm_\w+)"\)', f))
     assert decl - bound == set()
     assert bound - decl == set()
+
+
+def test_autotune_cache_round_trip(tmp_path):
+    """The autotuner's cache keeps one entry per (workload, form, device)."""
+    from ekman_bl_amd import autotune as AT
+    path = str(tmp_path / "tuning.json")
+    a = {"tuning": {"adi_rows": 12, "adi_block": 128}, "ms_per_step": 1.0, "trials": []}
+    b = {"tuning": {"adi_rows": 4, "adi_block": 256}, "ms_per_step": 2.0, "trials": []}
+    AT.save("w", 2, a, path, device="d1")
+    AT.save("w", 2, b, path, device="d2")
+    assert AT.load("w", 2, path, device="d1") == a["tuning"]
+    assert AT.load("w", 2, path, device="d2") == b["tuning"]
+    assert AT.load("w", 1, path, device="d1") is None
+    assert AT.load("w", 2, str(tmp_path / "none.json"), device="d1") is None
