@@ -40,11 +40,14 @@ SPACE = {
 
 
 def device_name() -> str:
-    try:
-        import torch
-        return torch.cuda.get_device_name(0)
-    except Exception:  # noqa: BLE001 - no torch / no device: the key still works
+    """The current HIP device's name and architecture, from the library."""
+    import ctypes
+    L = S.lib()
+    L.ekm_device_name.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    buf = ctypes.create_string_buffer(256)
+    if L.ekm_device_name(buf, 256) != 0:
         return "unknown"
+    return buf.value.decode(errors="replace")
 
 
 def cache_key(cfg_name: str, mode: int, device: str) -> str:

@@ -76,6 +76,7 @@ struct StepState {
   double red[4];
   double div_ref, div_max;
   int mg_solves, mg_cycles, mg_max_cycles, status;
+  int mg_unconverged;  // solves of the last projection that stopped at the cycle limit
   long long prof_launches, prof_solves, prof_cycles;  // totals since the last reset
 };
 
@@ -145,7 +146,7 @@ struct MgPlan {
   int nops_fmg = 0, nops_w = 0;
   double* ws = nullptr;      // global workspace [nsolve_max][ws_global]
   double* coefmem = nullptr; // all level coefficient arrays
-  int* cycles = nullptr;     // [nsolve_max]
+  int* cycles = nullptr;     // [2][nsolve_max]: cycle counts, then 1 where a consistent solve stopped unconverged
   void* desc = nullptr;      // device copy of the level descriptor, phase tables and cycle programs
   double* kx2 = nullptr;     // [nsolve_max]
   double* rhs = nullptr;     // [nsolve_max][plane] RHS_TMP

@@ -106,6 +106,11 @@ struct MeanGradArgs {
 int launch_mean_grads(const Dims& d, const Metrics& M, const MeanGradArgs& a, hipStream_t st);
 // the higher-order x moments (13 planes, see k_xmoments) from the physical
 // velocities U1p..U3p and the Fourier state's means
+// mean streamwise vorticity plane (NZ2*NY2) from the Fourier U2, U3
+int launch_omega_x(const Dims& d, const Metrics& M, const double* U2, const double* U3, double* out, hipStream_t st);
+// elements of a plane of axis 0 (x-y), 1 (z-y) or 2 (x-z); the gather of one from a physical field
+size_t plane_elems(const Dims& d, int axis);
+int launch_plane_gather(const Dims& d, int axis, int index, const double* U, double* out, hipStream_t st);
 int launch_xmoments(const Dims& d, const double* U1p, const double* U2p, const double* U3p, const double* U1,
                     const double* U2, const double* U3, double* out, hipStream_t st);
 // THOMAS_REAL on device arrays; ksp >= 0 adds THOMAS_REAL_SP's elimination of
@@ -132,6 +137,8 @@ int launch_dt_update(StepState* ss, hipStream_t st);
 // diagnostics of a projection: the reference's DIV and the max from red[0..1]
 // (src/duct.F90:2790-2800), cycle totals of the nsolve solves, the status
 // checks of src/duct.F90:2811-2812 (sticky)
-int launch_div_final(StepState* ss, const int* cycles, int nsolve, double denom, hipStream_t st);
+// unconv (may be null): per-solve flags of a consistent projection that stopped at its cycle limit
+int launch_div_final(StepState* ss, const int* cycles, const int* unconv, int nsolve, double denom,
+                     hipStream_t st);
 // end of a substep: H_BAR_prev, and TIME += DELTA_T after the third (src/diablo.F90:152-157)
 int launch_step_end(StepState* ss, int rk, hipStream_t st);

@@ -1197,6 +1197,53 @@ __global__ void __launch_bounds__(1024) k_mean_grads(Dims d, Metrics M, MeanGrad
   if (threadIdx.x == 0) o[(size_t)8 * pl] = a.th ? red[0] / (double)(float)(NY * NZ) : 0.0;
 }
 
+// Mean streamwise vorticity of SAVE_STATS_CURVI (src/flow_statistics.F90:
+// 301-327): omega_x = dW/dy - dV/dz of the x means (real parts of mode 0),
+// each derivative the cell-averaged face metrics times central index
+// differences over INT_JACOB, for K = 1..NZ, J = 1..NY; rows 0 and NY+1 copy
+// rows 1 and NY (:323-326); columns 0 and NZ+1 stay zero.  One thread per
+// point, each row fill recomputing its source row.
+__global__ void __launch_bounds__(256) k_omega_x(Dims d, Metrics M, const double2* __restrict__ u2,
+                                                 const double2* __restrict__ u3, double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int)d.plane) return;
+  const int k = t % d.NZ2, jo = t / d.NZ2;
+  if (k < 1 || k > d.NZ) {
+    out[t] = 0.0;
+    return;
+  }
+  const int j = jo < 1 ? 1 : jo > d.NY ? d.NY : jo;
+  auto m0 = [&](const double2* f, int kk, int jj) { return f[fo(d, 0, kk, jj)].x; };
+  const double Jc = JAC(k, j);
+  const double a11 = cell_avg(d, M.c11, k, j), a12 = cell_avg(d, M.c12, k, j);
+  const double a21 = cell_avg(d, M.c21, k, j), a22 = cell_avg(d, M.c22, k, j);
+  const double dwdy = (a12 * (m0(u3, k + 1, j) - m0(u3, k - 1, j)) + a22 * (m0(u3, k, j + 1) - m0(u3, k, j - 1))) / Jc;
+  const double dvdz = (a21 * (m0(u2, k, j + 1) - m0(u2, k, j - 1)) + a11 * (m0(u2, k + 1, j) - m0(u2, k - 1, j))) / Jc;
+  out[t] = dwdy - dvdz;
+}
+
+// One plane of a physical field (the gather under the reference's plane
+// writers, src/flow_statistics.F90:2091-2629): axis 0 the x-y plane at
+// K = index, out[j][i] (NY+2 rows of NX); axis 1 the z-y plane at I = index,
+// out[j][k]; axis 2 the x-z plane at J = index, out[k][i].
+__global__ void __launch_bounds__(256) k_plane_gather(Dims d, int axis, int index, const double* __restrict__ U,
+                                                      double* __restrict__ out) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (axis == 0) {
+    if (t >= (size_t)d.NY2 * d.NX) return;
+    const int i = (int)(t % d.NX), j = (int)(t / d.NX);
+    out[t] = U[ph(d, i, index, j)];
+  } else if (axis == 1) {
+    if (t >= d.plane) return;
+    const int k = (int)(t % d.NZ2), j = (int)(t / d.NZ2);
+    out[t] = U[ph(d, index, k, j)];
+  } else {
+    if (t >= (size_t)d.NZ2 * d.NX) return;
+    const int i = (int)(t % d.NX), k = (int)(t / d.NX);
+    out[t] = U[ph(d, i, k, index)];
+  }
+}
+
 // high_oder_momentums_curvi (src/flow_statistics.F90:1026-1095): one
 // wavefront per (K,J) pencil sums, over the NX physical points, the powers of
 // the velocity fluctuations about the mode-0 means (real parts of the Fourier
@@ -1471,6 +1518,20 @@ int launch_mean_grads(const Dims& d, const Metrics& M, const MeanGradArgs& a, hi
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }
+int launch_omega_x(const Dims& d, const Metrics& M, const double* U2, const double* U3, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_omega_x, dim3(nblk(d.plane, 256)), dim3(256), 0, st, d, M, (const double2*)U2,
+                     (const double2*)U3, out);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
+size_t plane_elems(const Dims& d, int axis) {
+  return axis == 0 ? (size_t)d.NY2 * d.NX : axis == 1 ? d.plane : (size_t)d.NZ2 * d.NX;
+}
+int launch_plane_gather(const Dims& d, int axis, int index, const double* U, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_plane_gather, dim3(nblk(plane_elems(d, axis), 256)), dim3(256), 0, st, d, axis, index, U, out);
+  EKM_CHECK(hipGetLastError());
+  return EKM_OK;
+}
 int launch_xmoments(const Dims& d, const double* U1p, const double* U2p, const double* U3p, const double* U1,
                     const double* U2, const double* U3, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_xmoments, dim3(nblk(d.plane, 4)), dim3(256), 0, st, d, U1p, U2p, U3p, (const double2*)U1,
@@ -1546,16 +1607,18 @@ __global__ void k_dt_update(StepState* ss) {
 // the projection's diagnostics: DIV (src/duct.F90:2790-2800), its max, the
 // cycle totals, and the STOP conditions of src/duct.F90:2811-2812 as a sticky
 // status (the first failure is kept)
-__global__ void __launch_bounds__(64) k_div_final(StepState* ss, const int* __restrict__ cycles, int nsolve,
-                                                  double denom) {
-  int sum = 0, mx = 0;
+__global__ void __launch_bounds__(64) k_div_final(StepState* ss, const int* __restrict__ cycles,
+                                                  const int* __restrict__ unconv, int nsolve, double denom) {
+  int sum = 0, mx = 0, nun = 0;
   for (int q = threadIdx.x; q < nsolve; q += 64) {
     const int c = cycles[q];
     sum += c;
     mx = c > mx ? c : mx;
+    if (unconv) nun += unconv[q];
   }
   for (int o = 32; o > 0; o >>= 1) {
     sum += __shfl_xor(sum, o);
+    nun += __shfl_xor(nun, o);
     const int m2 = __shfl_xor(mx, o);
     mx = m2 > mx ? m2 : mx;
   }
@@ -1568,6 +1631,7 @@ __global__ void __launch_bounds__(64) k_div_final(StepState* ss, const int* __re
   ss->mg_solves = nsolve;
   ss->mg_cycles = sum;
   ss->mg_max_cycles = mx;
+  ss->mg_unconverged = nun;
   ss->prof_launches += 1;
   ss->prof_solves += nsolve;
   ss->prof_cycles += sum;
@@ -1602,8 +1666,9 @@ int launch_dt_update(StepState* ss, hipStream_t st) {
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }
-int launch_div_final(StepState* ss, const int* cycles, int nsolve, double denom, hipStream_t st) {
-  hipLaunchKernelGGL(k_div_final, dim3(1), dim3(64), 0, st, ss, cycles, nsolve, denom);
+int launch_div_final(StepState* ss, const int* cycles, const int* unconv, int nsolve, double denom,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_div_final, dim3(1), dim3(64), 0, st, ss, cycles, unconv, nsolve, denom);
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }

@@ -885,7 +885,8 @@ int mg_plan_build(MgPlan& mg, const ekm_params& p, const Dims& d, const HostMetr
   std::memcpy(hbuf.data() + desc_bytes + tab_bytes, ops.data(), ops_bytes);
   EKM_CHECK(hipMemcpy(dmem, hbuf.data(), hbuf.size(), hipMemcpyHostToDevice));
   EKM_CHECK(hipMalloc(&mg.ws, (size_t)nsolve_max * mg.ws_global * sizeof(double)));
-  EKM_CHECK(hipMalloc(&mg.cycles, (size_t)nsolve_max * sizeof(int)));
+  EKM_CHECK(hipMalloc(&mg.cycles, (size_t)2 * nsolve_max * sizeof(int)));
+  EKM_CHECK(hipMemset(mg.cycles, 0, (size_t)2 * nsolve_max * sizeof(int)));
   EKM_CHECK(hipMalloc(&mg.kx2, (size_t)nsolve_max * sizeof(double)));
   EKM_CHECK(hipMalloc(&mg.rhs, (size_t)nsolve_max * d.plane * sizeof(double)));
   EKM_CHECK(hipMalloc(&mg.phi, (size_t)nsolve_max * d.plane * sizeof(double)));

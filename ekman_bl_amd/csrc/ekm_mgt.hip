@@ -662,8 +662,8 @@ template <typename T, int CM, int NT, bool X>
 __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__ gp, T* __restrict__ ws,
                                                      double* __restrict__ ws64, const double* __restrict__ rhs_tmp,
                                                      double* __restrict__ phi_out, const double* __restrict__ kx2v,
-                                                     int* __restrict__ cycles, size_t plane,
-                                                     const double* __restrict__ guessp, int nsolve) {
+                                                     int* __restrict__ cycles, int* __restrict__ unconv,
+                                                     size_t plane, const double* __restrict__ guessp, int nsolve) {
   __shared__ double red[NT];
   __shared__ T ylds[3 * kYMaxTasks * 64];
   extern __shared__ __align__(16) unsigned char tight_lds[];
@@ -724,6 +724,7 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
   __syncthreads();
   const double* __restrict__ c = g.cf64;
   int it = 0;
+  bool conv = false;  // the residual met the tolerance (uniform over the workgroup)
   for (;;) {
     double rmax = 0.0;
     if (it == 0 && zero_start) {
@@ -762,7 +763,11 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
     __syncthreads();
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) rmax = fmax(rmax, red[w]);
     __syncthreads();
-    if (rmax <= g.tol || it >= g.maxit) break;
+    if (rmax <= g.tol) {
+      conv = true;
+      break;
+    }
+    if (it >= g.maxit) break;
     // one cycle on the correction equation: the steps of the mode's program
     // in order, one call site of the smoother
     const T* ainv = fac + g.ainv_off;
@@ -801,7 +806,10 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
     const int j = p / nx, i = p - j * nx;
     po[p] = phi[(i + 1) + ld * (j + 1)];
   }
-  if (threadIdx.x == 0) cycles[s] = it;
+  if (threadIdx.x == 0) {
+    cycles[s] = it;
+    unconv[s] = conv ? 0 : 1;
+  }
 }
 
 // Gauss-Jordan inverse with partial pivoting of a dense n x n matrix
@@ -1147,7 +1155,7 @@ int tight_solve_ex(TightPlan& tp, MgPlan& mg, const Dims& d, int nsolve, const d
                                   hipFuncAttributeMaxDynamicSharedMemorySize, tp.lds_bytes));            \
     hipLaunchKernelGGL((k_tight_solve<T, CM, NT, X>), grid, dim3(NT), tp.lds_bytes, st, (const TDesc<T>*)tp.mem, \
                        (T*)tp.ws, tp.ws64, (const double*)mg.rhs, mg.phi, (const double*)mg.kx2, mg.cycles, \
-                       d.plane, guess, nsolve);                                                          \
+                       mg.cycles + mg.nsolve_max, d.plane, guess, nsolve);                                                          \
   } while (0)
 #define LAUNCH(T, CM)           \
   do {                          \

@@ -614,6 +614,66 @@ extern "C" int ekm_get_mean_grads(ekm_solver* s, double* out) {
   return st;
 }
 
+extern "C" int ekm_get_omega_x(ekm_solver* s, double* out) {
+  if (!out) return EKM_ERR_ARG;
+  const size_t n = s->d.plane * sizeof(double);
+  double* dev = nullptr;
+  EKM_CHECK(hipMalloc((void**)&dev, n));
+  int st = launch_omega_x(s->d, s->M, s->U[1], s->U[2], dev, s->st);
+  if (st == EKM_OK && hipMemcpyAsync(out, dev, n, hipMemcpyDeviceToHost, s->st) != hipSuccess) st = EKM_ERR_HIP;
+  if (hipStreamSynchronize(s->st) != hipSuccess) st = EKM_ERR_HIP;
+  (void)hipFree(dev);
+  return st;
+}
+
+extern "C" int ekm_get_physical(ekm_solver* s, int f, double* out) {
+  const Dims& d = s->d;
+  if (!out || f < 0 || f > 4) return EKM_ERR_ARG;
+  const double* src = f < 3 ? s->U[f] : f == 3 ? s->P : s->p.n_th ? s->TH : nullptr;
+  if (!src) {
+    std::memset(out, 0, d.npts * sizeof(double));
+    return EKM_OK;
+  }
+  double* tmp = nullptr;
+  EKM_CHECK(hipMalloc((void**)&tmp, d.npts * sizeof(double)));
+  int st = EKM_OK;
+  if (hipMemcpyAsync(tmp, src, d.npts * sizeof(double), hipMemcpyDeviceToDevice, s->st) != hipSuccess) st = EKM_ERR_HIP;
+  if (st == EKM_OK) st = inv(s, tmp);
+  if (st == EKM_OK && hipMemcpyAsync(out, tmp, d.npts * sizeof(double), hipMemcpyDeviceToHost, s->st) != hipSuccess)
+    st = EKM_ERR_HIP;
+  if (hipStreamSynchronize(s->st) != hipSuccess) st = EKM_ERR_HIP;
+  (void)hipFree(tmp);
+  return st;
+}
+
+extern "C" int ekm_get_plane(ekm_solver* s, int axis, int index, double* out) {
+  const Dims& d = s->d;
+  if (!out || axis < 0 || axis > 2) return EKM_ERR_ARG;
+  const int hi = axis == 0 ? d.NZ + 1 : axis == 1 ? d.NX - 1 : d.NY + 1;
+  if (index < 0 || index > hi) return EKM_ERR_ARG;
+  const size_t m = plane_elems(d, axis);
+  double* tmp = nullptr;  // one physical field, then the five planes
+  EKM_CHECK(hipMalloc((void**)&tmp, (d.npts + 5 * m) * sizeof(double)));
+  double* o = tmp + d.npts;
+  const double* src[5] = {s->U[0], s->U[1], s->U[2], s->P, s->p.n_th ? s->TH : nullptr};
+  int st = EKM_OK;
+  for (int f = 0; f < 5 && st == EKM_OK; ++f) {
+    if (!src[f]) {
+      if (hipMemsetAsync(o + f * m, 0, m * sizeof(double), s->st) != hipSuccess) st = EKM_ERR_HIP;
+      continue;
+    }
+    if (hipMemcpyAsync(tmp, src[f], d.npts * sizeof(double), hipMemcpyDeviceToDevice, s->st) != hipSuccess)
+      st = EKM_ERR_HIP;
+    if (st == EKM_OK) st = inv(s, tmp);
+    if (st == EKM_OK) st = launch_plane_gather(d, axis, index, tmp, o + f * m, s->st);
+  }
+  if (st == EKM_OK && hipMemcpyAsync(out, o, 5 * m * sizeof(double), hipMemcpyDeviceToHost, s->st) != hipSuccess)
+    st = EKM_ERR_HIP;
+  if (hipStreamSynchronize(s->st) != hipSuccess) st = EKM_ERR_HIP;
+  (void)hipFree(tmp);
+  return st;
+}
+
 extern "C" int ekm_get_xmoments(ekm_solver* s, double* out) {
   if (!out) return EKM_ERR_ARG;
   const Dims& d = s->d;
@@ -710,7 +770,9 @@ static int rem_div(ekm_solver* s, const double* guess, bool timed, bool learned 
   TRY(launch_project(d, s->M, s->rv, s->kx, s->PHI, s->U2b, s->U3b, s->U[0], s->U[1], s->U[2], s->S1, st));
   TRY(inv(s, s->S1));
   TRY(launch_div_reduce(d, s->S1, s->red, st));
-  TRY(launch_div_final(s->ss, s->mg.cycles, nsolve, (double)(d.NX + 1) * (double)d.NY * (double)d.NZ, st));
+  // the MUDPACK replica stops at MAXCY as the reference's solver does, with no flag
+  const int* unconv = s->p.proj_mode == 0 ? nullptr : s->mg.cycles + s->mg.nsolve_max;
+  TRY(launch_div_final(s->ss, s->mg.cycles, unconv, nsolve, (double)(d.NX + 1) * (double)d.NY * (double)d.NZ, st));
   return EKM_OK;
 }
 
@@ -1030,6 +1092,7 @@ extern "C" int ekm_get_diag(ekm_solver* s, ekm_diag* dg) {
   dg->mg_cycles = h.mg_cycles;
   dg->mg_max_cycles = h.mg_max_cycles;
   dg->status = h.status;
+  dg->mg_unconverged = h.mg_unconverged;
   return EKM_OK;
 }
 
@@ -1191,6 +1254,17 @@ extern "C" int ekm_gather_rows(void* dst, const void* src, const long* idx, int 
   EKM_CHECK(hipStreamSynchronize(st));
   return EKM_OK;
 }
+// name and architecture of the current device, e.g. "AMD Instinct MI355X (gfx950:sramecc+:xnack-)"
+extern "C" int ekm_device_name(char* buf, int n) {
+  if (!buf || n < 1) return EKM_ERR_ARG;
+  int dev = 0;
+  EKM_CHECK(hipGetDevice(&dev));
+  hipDeviceProp_t pr;
+  EKM_CHECK(hipGetDeviceProperties(&pr, dev));
+  std::snprintf(buf, (size_t)n, "%s (%s)", pr.name, pr.gcnArchName);
+  return EKM_OK;
+}
+
 extern "C" int ekm_memcpy_h2d(void* dst, const void* src, size_t bytes) {
   EKM_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
   return EKM_OK;

@@ -76,6 +76,8 @@ program duct_driver
      call ekm_check(ekm_get_diag(ekm_handle, d), 'ekm_get_diag')
      write(6, '(i6,a9,f15.8,a7,f15.8,a7,es12.4,a9,i6)') d%time_step, '    time=', d%time, '    dt=', &
           d%delta_t, '   div=', d%div_ref, '   cycles=', d%mg_cycles
+     if (d%mg_unconverged > 0) write(6, '(a,i6,a)') ' WARNING: ', d%mg_unconverged, &
+          ' pressure solves stopped at proj_maxit above proj_tol'
   end do
   call ekm_check(ekm_sync(ekm_handle), 'ekm_sync')
   call system_clock(t1)

@@ -60,6 +60,7 @@ module ekman_bl_amd
   type, bind(c) :: ekm_diag
      real(c_double) :: div_ref, div_max, delta_t, time
      integer(c_int) :: time_step, rk_step, mg_solves, mg_cycles, mg_max_cycles, status
+     integer(c_int) :: mg_unconverged
   end type ekm_diag
 
   ! ekm_tuning: kernel launch and memory-placement parameters
@@ -132,6 +133,26 @@ module ekman_bl_amd
 
      ! the higher-order x moments of high_oder_momentums_curvi: 13 planes
      ! (UU VV WW UUU VVV WWW UUUU VVVV WWWW UUV WWV UUUV WWWV)
+     integer(c_int) function ekm_get_omega_x(s, out) bind(c, name="ekm_get_omega_x")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: s
+       real(c_double), intent(out) :: out(*)
+     end function ekm_get_omega_x
+
+     integer(c_int) function ekm_get_physical(s, f, out) bind(c, name="ekm_get_physical")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: s
+       integer(c_int), value :: f
+       real(c_double), intent(out) :: out(*)
+     end function ekm_get_physical
+
+     integer(c_int) function ekm_get_plane(s, axis, index, out) bind(c, name="ekm_get_plane")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: s
+       integer(c_int), value :: axis, index
+       real(c_double), intent(out) :: out(*)
+     end function ekm_get_plane
+
      integer(c_int) function ekm_get_xmoments(s, out) bind(c, name="ekm_get_xmoments")
        import :: c_ptr, c_int, c_double
        type(c_ptr), value :: s
@@ -502,6 +523,12 @@ module ekman_bl_amd
      type(c_ptr) function ekm_last_error() bind(c, name="ekm_last_error")
        import :: c_ptr
      end function ekm_last_error
+
+     integer(c_int) function ekm_device_name(buf, n) bind(c, name="ekm_device_name")
+       import :: c_char, c_int
+       character(kind=c_char), intent(out) :: buf(*)
+       integer(c_int), value :: n
+     end function ekm_device_name
 
      type(c_ptr) function ekm_version() bind(c, name="ekm_version")
        import :: c_ptr

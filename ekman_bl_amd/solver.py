@@ -42,7 +42,8 @@ class EkmDiag(ctypes.Structure):
                 ("delta_t", ctypes.c_double), ("time", ctypes.c_double),
                 ("time_step", ctypes.c_int), ("rk_step", ctypes.c_int),
                 ("mg_solves", ctypes.c_int), ("mg_cycles", ctypes.c_int),
-                ("mg_max_cycles", ctypes.c_int), ("status", ctypes.c_int)]
+                ("mg_max_cycles", ctypes.c_int), ("status", ctypes.c_int),
+                ("mg_unconverged", ctypes.c_int)]
 
 
 class EkmTuning(ctypes.Structure):
@@ -99,6 +100,9 @@ def lib():
         L.ekm_get_xstats.argtypes = [vp, _D]
         L.ekm_get_mean_grads.argtypes = [vp, _D]
         L.ekm_get_xmoments.argtypes = [vp, _D]
+        L.ekm_get_omega_x.argtypes = [vp, _D]
+        L.ekm_get_plane.argtypes = [vp, ctypes.c_int, ctypes.c_int, _D]
+        L.ekm_get_physical.argtypes = [vp, ctypes.c_int, _D]
         L.ekm_set_initial_physical.argtypes = [vp, _D, _D, _D, _D]
         L.ekm_get_phi.argtypes = [vp, _D]
         L.ekm_get_diag.argtypes = [vp, ctypes.POINTER(EkmDiag)]
@@ -230,6 +234,34 @@ class DuctSolver:
         out = np.zeros((13, p.ny + 2, p.nz + 2))
         _check(lib().ekm_get_xmoments(self.h, _p(out)), "ekm_get_xmoments")
         return dict(zip(self.XMOMENTS, out))
+
+    def omega_x(self) -> np.ndarray:
+        """SAVE_STATS_CURVI's mean streamwise vorticity (ekm_get_omega_x): (NY+2, NZ+2)."""
+        p = self.p
+        out = np.zeros((p.ny + 2, p.nz + 2))
+        _check(lib().ekm_get_omega_x(self.h, _p(out)), "ekm_get_omega_x")
+        return out
+
+    PLANE_FIELDS = ("u1", "u2", "u3", "p", "th")
+
+    def plane(self, axis: int, index: int) -> np.ndarray:
+        """Physical U1, U2, U3, P, TH on one plane (ekm_get_plane): axis 0 the
+        x-y plane at K = index, (5, NY+2, NX); axis 1 the z-y plane at
+        I = index, (5, NY+2, NZ+2); axis 2 the x-z plane at J = index,
+        (5, NZ+2, NX)."""
+        p = self.p
+        shape = {0: (5, p.ny + 2, p.nx), 1: (5, p.ny + 2, p.nz + 2), 2: (5, p.nz + 2, p.nx)}[axis]
+        out = np.zeros(shape)
+        _check(lib().ekm_get_plane(self.h, axis, index, _p(out)), "ekm_get_plane")
+        return out
+
+    def physical(self, f: int) -> np.ndarray:
+        """Field f (0..4: U1, U2, U3, P, TH) in physical space (ekm_get_physical):
+        (NY+2, NZ+2, NX+2), the last two x entries undefined."""
+        p = self.p
+        out = np.zeros((p.ny + 2, p.nz + 2, p.nx + 2))
+        _check(lib().ekm_get_physical(self.h, f, _p(out)), "ekm_get_physical")
+        return out
 
     def mean_grads(self):
         """SAVE_STATS_CURVI's mean gradients of the current state

@@ -119,6 +119,9 @@ typedef struct ekm_diag {
   int mg_cycles;        /* W(2,1) cycles summed over those solves (reference iparm(17)) */
   int mg_max_cycles;    /* largest cycle count of a single solve */
   int status;           /* status of the last substep */
+  int mg_unconverged;   /* proj_mode 1/2: solves of the last projection that stopped at proj_maxit
+                         * with the residual above proj_tol (0 for the MUDPACK replica, which
+                         * stops at MAXCY without a flag as the reference's solver does) */
 } ekm_diag;
 
 typedef struct ekm_solver ekm_solver;  /* opaque: device-resident duct state */
@@ -236,6 +239,24 @@ int ekm_get_mean_grads(ekm_solver* s, double* out);
  * WWWV -- the x averages over the NX physical points of powers of the velocity
  * fluctuations about the mode-0 means. */
 int ekm_get_xmoments(ekm_solver* s, double* out);
+/* The mean streamwise vorticity of SAVE_STATS_CURVI (src/flow_statistics.F90:
+ * 301-327), omega_x = dW/dy - dV/dz of the x means: one plane of (NZ+2)*(NY+2)
+ * doubles (K fastest); rows 0 and NY+1 copy rows 1 and NY, columns 0 and NZ+1
+ * are zero. */
+int ekm_get_omega_x(ekm_solver* s, double* out);
+/* One plane of the physical U1, U2, U3, P and TH (five planes in that order,
+ * TH zero when N_TH = 0), the data the reference's plane writers take from
+ * U1X..THX (src/flow_statistics.F90:2004-2629):
+ *   axis 0: x-y plane at K = index (0..NZ+1), each (NY+2) x NX, I fastest
+ *   axis 1: z-y plane at I = index (0..NX-1), each (NY+2) x (NZ+2), K fastest
+ *   axis 2: x-z plane at J = index (0..NY+1), each (NZ+2) x NX, I fastest */
+int ekm_get_plane(ekm_solver* s, int axis, int index, double* out);
+/* Field f (0..4: U1, U2, U3, P, TH) of the current state in physical space,
+ * the physical layout above ((NX+2)*(NZ+2)*(NY+2) doubles; the two padding
+ * entries of each x pencil are undefined); zeros for TH when N_TH = 0.  The
+ * data of the reference's full 3D writer (plot_3D, src/flow_statistics.F90:
+ * 2004-2040). */
+int ekm_get_physical(ekm_solver* s, int f, double* out);
 /* Blocks until queued work on the solver's stream is done; returns the run's
  * sticky status. */
 int ekm_sync(ekm_solver* s);
@@ -366,6 +387,8 @@ int ekm_host_sponge_sigma_out(int type, int nz, int ny, int nz_s_r, int nz_s_l, 
                               double* sig);
 int ekm_host_mg_coefficients(const ekm_params* p, const double* xpoint, const double* ypoint,
                              int lev, double kx2, double* cf, int* nx, int* ny);
+/* Name and architecture of the current HIP device (NUL-terminated, at most n bytes). */
+int ekm_device_name(char* buf, int n);
 /* Message of the last HIP error seen by this thread. */
 const char* ekm_last_error(void);
 

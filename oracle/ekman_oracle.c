@@ -2571,6 +2571,47 @@ int ora_mean_grads(ora* o, double* out) {
   return EKM_OK;
 }
 
+/* omega_x of SAVE_STATS_CURVI (src/flow_statistics.F90:301-327): for
+ * K = 1..NZ, J = 1..NY, (the four-face average of CJOB_12 times the K
+ * difference of U3 plus that of CJOB_22 times the J difference) / INT_JACOB,
+ * minus (CJOB_21 times the J difference of U2 plus CJOB_11 times the K
+ * difference) / INT_JACOB, on the real parts of mode 0; then rows 0 and NY+1
+ * copied from rows 1 and NY (:323-326) */
+int ora_omega_x(ora* o, double* out) {
+  const int NZ = o->NZ, NY = o->NY;
+  memset(out, 0, (size_t)o->NZ2 * o->NY2 * sizeof(double));
+#define M0(f, k, j) ((f)[CRE(o, 0, k, j)])
+#define A4(c, k, j) (0.125 * (c(o, k, j, 1) + c(o, k, j + 1, 1) + c(o, k, j, 2) + c(o, k + 1, j, 2)))
+  const double *U2 = o->U[1], *U3 = o->U[2];
+  for (int k = 1; k <= NZ; ++k)
+    for (int j = 1; j <= NY; ++j)
+      out[PL(o, k, j)] = (A4(C12, k, j) * (M0(U3, k + 1, j) - M0(U3, k - 1, j)) +
+                          A4(C22, k, j) * (M0(U3, k, j + 1) - M0(U3, k, j - 1))) / JAC(o, k, j) -
+                         (A4(C21, k, j) * (M0(U2, k, j + 1) - M0(U2, k, j - 1)) +
+                          A4(C11, k, j) * (M0(U2, k + 1, j) - M0(U2, k - 1, j))) / JAC(o, k, j);
+  for (int k = 1; k <= NZ; ++k) {
+    out[PL(o, k, 0)] = out[PL(o, k, 1)];
+    out[PL(o, k, NY + 1)] = out[PL(o, k, NY)];
+  }
+#undef M0
+#undef A4
+  return EKM_OK;
+}
+
+/* field f (0..4: U1, U2, U3, P, TH) of the current state in physical space:
+ * (NX+2)*(NZ+2)*(NY+2) doubles, the inverse transform of a copy */
+int ora_get_physical(ora* o, int f, double* out) {
+  if (f < 0 || f > 4) return EKM_ERR_ARG;
+  const double* src = f < 3 ? o->U[f] : f == 3 ? o->P : o->p.n_th ? o->TH : NULL;
+  if (!src) {
+    memset(out, 0, o->npts * sizeof(double));
+    return EKM_OK;
+  }
+  memcpy(out, src, o->npts * sizeof(double));
+  inv(o, out);
+  return EKM_OK;
+}
+
 /* high_oder_momentums_curvi (src/flow_statistics.F90:1026-1095): per (K,J)
  * the x averages of powers of the velocity fluctuations about the mode-0
  * means, in the order UU, VV, WW, UUU, VVV, WWW, UUUU, VVVV, WWWW, UUV, WWV,

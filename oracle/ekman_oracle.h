@@ -43,6 +43,9 @@
  *                               checked against them)
  *   ora_xmoments                one x mode about non-zero means: the cos^n
  *                               averages in closed form
+ *   ora_omega_x                 sheared affine grid: linear U2, U3 give
+ *                               dW/dy - dV/dz exactly, and the row fills
+ *   ora_get_physical            the inverse transform pinned above
  *   ora_mean_grads              sheared affine grid: linear mean fields give
  *                               their exact derivatives, Rig, TKE, the fills
  *                               and THBULK in closed form
@@ -97,6 +100,10 @@ int ora_mean_grads(ora* o, double* out);
 /* the higher-order x moments of the velocity fluctuations (13 planes of
  * (NZ+2)*(NY+2): UU VV WW UUU VVV WWW UUUU VVVV WWWW UUV WWV UUUV WWWV) */
 int ora_xmoments(ora* o, double* out);
+/* omega_x of SAVE_STATS_CURVI: one plane of (NZ+2)*(NY+2) */
+int ora_omega_x(ora* o, double* out);
+/* field f (0..4: U1, U2, U3, P, TH) in physical space, (NX+2)*(NZ+2)*(NY+2) */
+int ora_get_physical(ora* o, int f, double* out);
 /* one field's ADI (comp 0..2 velocity, 3 scalar) on physical U, R in place,
  * with the subgrid viscosity nut (physical P field) unless NULL */
 int ora_adi_field(ora* o, int comp, double* U, double* R, double t1, const double* nut, double t2);

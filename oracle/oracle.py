@@ -26,7 +26,8 @@ class EkmDiag(ctypes.Structure):
                 ("delta_t", ctypes.c_double), ("time", ctypes.c_double),
                 ("time_step", ctypes.c_int), ("rk_step", ctypes.c_int),
                 ("mg_solves", ctypes.c_int), ("mg_cycles", ctypes.c_int),
-                ("mg_max_cycles", ctypes.c_int), ("status", ctypes.c_int)]
+                ("mg_max_cycles", ctypes.c_int), ("status", ctypes.c_int),
+                ("mg_unconverged", ctypes.c_int)]
 
 
 def build() -> None:
@@ -54,6 +55,8 @@ def lib():
         L.ora_get_sponge_out.argtypes = [ctypes.c_void_p, _D]
         L.ora_mean_grads.argtypes = [ctypes.c_void_p, _D]
         L.ora_xmoments.argtypes = [ctypes.c_void_p, _D]
+        L.ora_omega_x.argtypes = [ctypes.c_void_p, _D]
+        L.ora_get_physical.argtypes = [ctypes.c_void_p, ctypes.c_int, _D]
         L.ora_xstats.argtypes = [ctypes.c_void_p, _D]
         L.ora_adi_field.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.c_double, _D, ctypes.c_double]
         L.ora_set_initial_physical.argtypes = [ctypes.c_void_p, _D, _D, _D, _D]
@@ -164,6 +167,20 @@ class Oracle:
         out = np.zeros((13, p.ny + 2, p.nz + 2))
         lib().ora_xmoments(self.h, _p(out))
         return dict(zip(self.XMOMENTS, out))
+
+    def omega_x(self) -> np.ndarray:
+        """SAVE_STATS_CURVI's mean streamwise vorticity: (NY+2, NZ+2)."""
+        p = self.p
+        out = np.zeros((p.ny + 2, p.nz + 2))
+        lib().ora_omega_x(self.h, _p(out))
+        return out
+
+    def physical(self, f: int) -> np.ndarray:
+        """Field f (0..4: U1, U2, U3, P, TH) in physical space: (NY+2, NZ+2, NX+2)."""
+        p = self.p
+        out = np.zeros((p.ny + 2, p.nz + 2, p.nx + 2))
+        lib().ora_get_physical(self.h, f, _p(out))
+        return out
 
     def mean_grads(self):
         """SAVE_STATS_CURVI's mean gradients: (dict name -> (NY+2, NZ+2), THBULK)."""

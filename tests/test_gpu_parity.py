@@ -218,6 +218,32 @@ def test_consistent_projection_matches_oracle(mode, n, krylov, oracle_lib):
     s.close()
 
 
+def test_unconverged_projection_is_flagged():
+    """A consistent solve that stops at proj_maxit with its residual above
+    proj_tol is counted in ekm_diag.mg_unconverged; a converged projection
+    reports none, and the MUDPACK replica, which stops at MAXCY as the
+    reference's solver does, never flags."""
+    cfg, p, xp, yp = _setup(1)
+    ic = ekman_initial(p, yp, cfg.seed, cfg.noise, cfg.th_noise)
+    for mode, maxit, tol, want in ((2, 1, 1e-12, "all"), (1, 1, 1e-12, "all"), (2, 50, 1e-12, "none"),
+                                   (0, 1, 1e-12, "none")):
+        q = p.copy()
+        q.proj_mode, q.proj_maxit, q.proj_tol = mode, maxit, tol
+        s = S.DuctSolver(q, xp, yp)
+        s.set_initial_physical(*ic)
+        s.rk_duct()
+        d = s.diag()
+        print(f"[unconverged] mode {mode} maxit {maxit}: solves {d.mg_solves} unconverged {d.mg_unconverged}"
+              f" max cycles {d.mg_max_cycles} status {d.status}", flush=True)
+        assert d.status == 0
+        if want == "all":
+            assert d.mg_unconverged == d.mg_solves > 0
+            assert d.mg_max_cycles == maxit
+        else:
+            assert d.mg_unconverged == 0
+        s.close()
+
+
 def test_warm_starts_agree():
     """The warm start of the consistent solve (proj_guess 0: from zero, 1: the
     previous substep's correction scaled by the RK weights) only changes where
