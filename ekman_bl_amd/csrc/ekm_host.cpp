@@ -328,9 +328,30 @@ int host_bc_table(const ekm_params& p, int comp, int dir, BcTable& t) {
         else return EKM_ERR_UNSUPPORTED;
         break;
       case 6: put(t, 0, 0., 1., 0., BC_LAGGED, 0.0, NY); break;
+      case 3:
+        // wall-model rows of IC_TYPE 7 with WALL_MODEL_TYPE 3 (:532-578,
+        // :894-937, :1338-1380): U(1) - U(0) = the model's value, V's never
+        // set by the model (zero)
+        if (comp == 3 || p.wall_model_type != 3) return EKM_ERR_UNSUPPORTED;
+        if (comp == 1) put(t, 0, 0., -1., 1., BC_CONST, 0.0, 0);
+        else put(t, 0, 0., -1., 1., BC_WALL, 0.0, comp == 0 ? 0 : 1);
+        break;
       default: return EKM_ERR_UNSUPPORTED;
     }
     switch (th_) {
+      case 3:
+        // the upper type-3 rows (:635-650, :1016-1030, :1480-1493): their
+        // values come from U_BC_UPPER_NWM / W_BC_UPPER_NWM, which the
+        // reference allocates and never assigns; taken as zero here
+        if (comp == 3) return EKM_ERR_UNSUPPORTED;
+        if (comp == 1) {
+          put(t, top, 0., 1., 0., BC_CONST, 0.0, 0);
+          put(t, NY, 0., 1., 0., BC_CONST, 0.0, 0);
+        } else {
+          put(t, NY, -1., 1., 0., BC_CONST, 0.0, 0);
+          put(t, top, 0., 1., 0., BC_LAGGED, 0.0, NY);
+        }
+        break;
       case 0:
         put(t, top, 0., 1., 0., BC_CONST, ch, 0);
         if (comp == 1 || comp == 2) put(t, NY, 0., 1., 0., BC_CONST, ch, 0);  // :951-960, :1410-1419
@@ -376,6 +397,20 @@ int host_bc_table(const ekm_params& p, int comp, int dir, BcTable& t) {
         t.sp = 1;
         break;
       case 6: put(t, top, 0., 1., 0., BC_LAGGED, 0.0, 1); break;
+      case 3:
+        // V (:1249-1262): U2X(I,NZ,J) on row NZ+1; W (:1722-1734): zero rows,
+        // W_BC_UPPER_NWM never assigned.  U1's rows (:802-815) read U1X at
+        // the swapped index (I,J,NZ) and an unassigned DZ: not supported
+        if (comp == 1) {
+          put(t, NZ, -1., 1., 0., BC_CONST, 0.0, 0);
+          put(t, top, 0., 1., 0., BC_LAGGED, 0.0, NZ);
+        } else if (comp == 2) {
+          put(t, top, -1., 1., 0., BC_CONST, 0.0, 0);
+          put(t, NZ, 0., 1., 0., BC_CONST, 0.0, 0);
+        } else {
+          return EKM_ERR_UNSUPPORTED;
+        }
+        break;
       default: return EKM_ERR_UNSUPPORTED;
     }
   }

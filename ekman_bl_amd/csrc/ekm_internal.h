@@ -88,6 +88,7 @@ enum BcKind : int {
   BC_THFLUX_LO = 2,  // rhs = val / (0.5*(C22(K,1,2)+C22(K+1,1,1))) * J(K,1)
   BC_THFLUX_NY = 3,  // rhs = 0.5*val*(J(K,NY)+J(K,NY-1)) / C22(K,NY-1,1)
   BC_THFLUX_NY1 = 4, // rhs = 0.5*val*(J(K,NY+1)+J(K,NY)) / C22(K,NY,1)
+  BC_WALL = 5,       // rhs = the wall model's value at (i,K): array src (0 U1, 1 U3)
 };
 struct BcRow {
   int row;  // -1 = unused

@@ -40,6 +40,7 @@ struct AdiField {
   const double* gz;
   const double* gy;
   double* pv;
+  const double* wall;  // the wall model's row values, [2][NZ2][NX2] (U1, U3), or null
 };
 constexpr int kAdiMaxFields = 4;
 // Implicit weights of the ADI stage: TEMP1 / div[f] per field (div = 1, or
@@ -54,6 +55,20 @@ struct AdiT1 {
 // th_corners applies the scalar corner rows to field th_field afterwards.
 int launch_adi_batch(const Dims& d, const Metrics& M, const AdiField* fs, const AdiField* hf, int nf,
                      const AdiT1& t1, int th_field, int th_corners, hipStream_t st, const ekm_tuning& tn);
+// LES_WALL_MODEL with WALL_MODEL_TYPE 3 (src/les_chan.F90:1914-2144): the
+// row values of the lower type-3 rows from the physical U1, U3 at JSTART
+constexpr int kWallMaxNz2 = 2048;  // NZ+2 bound of the wall model's per-K means (LDS)
+struct WallArgs {
+  const double* u1;
+  const double* u3;
+  double* wall;          // [2][NZ2][NX2] row values (U1, U3)
+  Ranges r;
+  double z;              // DYY(2,2) = ypoint(2,2) - ypoint(2,1)
+  double nu;
+  double cos_a, sin_a;   // COS and SIN of the surface stress angle, as the reference forms them
+  int first;             // TIME_STEP = 1
+};
+int launch_wall_model(const Dims& d, const Metrics& M, const WallArgs& a, hipStream_t st);
 int launch_cfl(const Dims& d, const Metrics& M, const Ranges& r, const double* U1, const double* U2,
                const double* U3, double cfl, double dx, unsigned long long* out, hipStream_t st);
 int launch_faces(const Dims& d, const Metrics& M, const Ranges& r, const double* CU2, const double* CU3,

@@ -280,7 +280,8 @@ __global__ void __launch_bounds__(kLuBlock) k_lu(Dims d, Metrics M, const AdiFie
 
 __device__ inline double bc_rhs(const Dims& d, const Metrics& M, const BcRow& e, int K, double lag) {
   switch (e.kind) {
-    case BC_LAGGED: return lag;
+    case BC_LAGGED:
+    case BC_WALL: return lag;
     case BC_THFLUX_LO: return e.val / (0.5 * (C22(K, 1, 2) + C22(K + 1, 1, 1))) * JAC(K, 1);
     case BC_THFLUX_NY: return 0.5 * e.val * (JAC(K, d.NY) + JAC(K, d.NY - 1)) / C22(K, d.NY - 1, 1);
     case BC_THFLUX_NY1: return 0.5 * e.val * (JAC(K, d.NY + 1) + JAC(K, d.NY)) / C22(K, d.NY, 1);
@@ -319,7 +320,10 @@ __global__ void __launch_bounds__(256) k_ysweep(Dims d, Metrics M, const AdiFiel
   const size_t q0 = ph(d, i, K, 0);
   const size_t base = (size_t)K * (N + 1);
   double lag[4];
-  for (int e = 0; e < 4; ++e) lag[e] = (bc.r[e].kind == BC_LAGGED) ? U[q0 + dj * bc.r[e].src] : 0.0;
+  for (int e = 0; e < 4; ++e)
+    lag[e] = bc.r[e].kind == BC_LAGGED ? U[q0 + dj * bc.r[e].src]
+             : bc.r[e].kind == BC_WALL ? F.wall[(size_t)bc.r[e].src * dj + (size_t)K * d.NX2 + i]
+                                       : 0.0;
   double g = 0.0;
   for (int row0 = 0; row0 <= N; row0 += kSw) {
     double rr[kSw], mm[kSw];
@@ -474,7 +478,10 @@ __global__ void __launch_bounds__(256) k_ysweep_les(Dims d, Metrics M, const Adi
   const size_t dj = (size_t)d.NX2 * d.NZ2;
   const size_t q0 = ph(d, i, K, 0);
   double lag[4];
-  for (int e = 0; e < 4; ++e) lag[e] = (bc.r[e].kind == BC_LAGGED) ? U[q0 + dj * bc.r[e].src] : 0.0;
+  for (int e = 0; e < 4; ++e)
+    lag[e] = bc.r[e].kind == BC_LAGGED ? U[q0 + dj * bc.r[e].src]
+             : bc.r[e].kind == BC_WALL ? F.wall[(size_t)bc.r[e].src * dj + (size_t)K * d.NX2 + i]
+                                       : 0.0;
   double g = 0.0, pvp = 1.0, cp = 0.0;
   double nm = 0.0, nc = nut[q0];  // NU_T at row-1 and row
   for (int row0 = 0; row0 <= N; row0 += kSwL) {
@@ -1353,6 +1360,66 @@ __global__ void k_thomas_generic(const double* __restrict__ A, const double* __r
   }
 }
 
+// LES_WALL_MODEL, WALL_MODEL_TYPE 3 (src/les_chan.F90:1914-2144, one rank
+// holding every x point, MIN(NXP,NXP_L) = NX-1), one workgroup:
+// the x means of U1 and U3 at JSTART per K (sums over the NX points divided
+// by NX-1, :1996-1999), their z means U_1, W_1 over K = 1..NZ, M = |(U_1,W_1)|,
+// thirty fixed-point steps of U_STAR = M / (ln(Z U_STAR / NU)/KAPA + 5.2)
+// (:2004-2007; 5.2 is a default real), TAO_WALL = U_STAR^2 cos(alpha),
+// TAO_WALL_1 = U_STAR^2 sin(alpha) (:2012-2013); then per (I,K) the row
+// values W = U3/W_1 TAO/NU / (C22 face mean) J and U = max(U1/W_1 TAO/NU,
+// U1/U_1 TAO_1/NU) / (C22 face mean) J (:2101-2102).  On TIME_STEP 1 the U3
+// value is U3X(I,K,2) and the U1 value is kept (:2115-2118).
+__global__ void __launch_bounds__(1024) k_wall_model(Dims d, Metrics M, WallArgs a) {
+  __shared__ double sh[4];
+  __shared__ double tu[kWallMaxNz2], tw[kWallMaxNz2];
+  const size_t pl = (size_t)d.NX2 * d.NZ2;
+  double* __restrict__ wu = a.wall;
+  double* __restrict__ ww = a.wall + pl;
+  const int js = a.r.js, NX = d.NX, nk = a.r.ze - a.r.zs + 1;
+  if (!a.first) {
+    const double nxm = (double)(NX - 1);
+    for (int k = a.r.zs + (int)threadIdx.x; k <= a.r.ze; k += blockDim.x) {
+      double su = 0.0, sw = 0.0;
+      for (int i = 0; i < NX; ++i) {
+        su += a.u1[ph(d, i, k, js)];
+        sw += a.u3[ph(d, i, k, js)];
+      }
+      tu[k] = su / nxm;
+      tw[k] = sw / nxm;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double su = 0.0, sw = 0.0;
+      for (int k = 1; k <= d.NZ; ++k) {
+        su += tu[k];
+        sw += tw[k];
+      }
+      const double u1m = su / (double)d.NZ, w1m = sw / (double)d.NZ;
+      const double m = sqrt(u1m * u1m + w1m * w1m);
+      double us = m;
+      for (int it = 0; it < 30; ++it) us = m / ((1.0 / 0.41) * log(a.z / a.nu * us) + (double)5.2f);
+      sh[0] = u1m;
+      sh[1] = w1m;
+      sh[2] = us * us * a.cos_a;
+      sh[3] = us * us * a.sin_a;
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < NX * nk; t += blockDim.x) {
+    const int i = t % NX, k = a.r.zs + t / NX;
+    const size_t q = (size_t)k * d.NX2 + i;
+    if (a.first) {
+      ww[q] = a.u3[ph(d, i, k, 2)];
+      continue;
+    }
+    const double den = 0.5 * (C22(k, 1, 2) + C22(k + 1, 1, 1)), jk = JAC(k, 1);
+    const double u = a.u1[ph(d, i, k, js)], w = a.u3[ph(d, i, k, js)];
+    ww[q] = w / sh[1] * (sh[2] / a.nu) / den * jk;
+    wu[q] = fmax(u / sh[1] * (sh[2] / a.nu), u / sh[0] * (sh[3] / a.nu)) / den * jk;
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1441,6 +1508,11 @@ int launch_adi_batch(const Dims& d, const Metrics& M, const AdiField* fs, const 
     hipLaunchKernelGGL(k_th_corners, dim3(nblk(d.NX2, 256)), dim3(256), 0, st, d, hf[th_field].U, th_corners);
     EKM_CHECK(hipGetLastError());
   }
+  return EKM_OK;
+}
+int launch_wall_model(const Dims& d, const Metrics& M, const WallArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_wall_model, dim3(1), dim3(1024), 0, st, d, M, a);
+  EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }
 int launch_cfl(const Dims& d, const Metrics& M, const Ranges& r, const double* U1, const double* U2,

@@ -96,6 +96,8 @@ struct ekm_solver {
   double* sponge = nullptr;
   double sponge_jst = 0.0;
   double* sponge_out = nullptr;  // SPONGE_SIGMA_OUT(0:NZ+1), when the streamwise profile is on
+  double* wall = nullptr;        // the wall model's row values [2][NZ2][NX2] (W_BC_YMIN = 3)
+  double wall_z = 0.0;           // DYY(2,2)
 };
 
 extern "C" {
@@ -141,6 +143,7 @@ void ekm_params_default(ekm_params* p) {
   p->proj_krylov = 1;
   p->ny_s_t = p->nz_s_r = p->save_stats_int = 1 << 30;  // no sponge rows
   p->nz_s_l = -(1 << 30);
+  p->wall_model_type = 3;  // rundir_unst/input.dat:11
 }
 
 }  // extern "C"
@@ -162,6 +165,16 @@ static int check_params(const ekm_params& p) {
   if ((p.mudpack_bc[0] == 0) != (p.mudpack_bc[1] == 0)) return EKM_ERR_ARG;
   if ((p.mudpack_bc[2] == 0) != (p.mudpack_bc[3] == 0)) return EKM_ERR_ARG;
   if (p.proj_mode < 0 || p.proj_mode > 2) return EKM_ERR_UNSUPPORTED;
+  if (p.u_bc[2][EKM_YMIN] == 3) {
+    // LES_WALL_MODEL: type 3 only (type 1 reads a U_STAR it never sets,
+    // type 2 a CU1X_WALL_MODEL it never allocates, type 4 the scalar's new
+    // value, src/les_chan.F90:1914-2144); its z means read tmp_u(1:NZ),
+    // which it fills over ZSTART..ZEND only
+    if (p.wall_model_type != 3) return EKM_ERR_UNSUPPORTED;
+    const int uz0 = p.u_bc[0][EKM_ZMIN], uz1 = p.u_bc[0][EKM_ZMAX];
+    if (!(uz0 == 1 || uz0 == 6) || !(uz1 == 0 || uz1 == 1 || uz1 == 6)) return EKM_ERR_UNSUPPORTED;
+    if (p.nz + 2 > kWallMaxNz2) return EKM_ERR_UNSUPPORTED;
+  }
   if (p.les_model != 0 && p.les_model != 2) return EKM_ERR_UNSUPPORTED;  // dynamic Smagorinsky only
   // the scalar's subgrid model with a non-uniform background (cont_strat 0)
   // adds KAPPA_T indexed by Fourier mode to the background diffusion
@@ -262,7 +275,7 @@ static int fold_events(ekm_solver* s) {
 static int free_all(ekm_solver* s) {
   double* f[] = {s->U[0], s->U[1], s->U[2], s->P, s->TH, s->CF[0], s->CF[1], s->CF[2], s->CFTH,
                  s->R[0], s->R[1], s->R[2], s->RTH, s->S1, s->S2, s->U2b, s->U3b, s->PHI, s->lu,
-                 s->met, s->kx, s->kx2, s->kx2s, s->thbar, s->lesmet, s->fdy, s->sponge, s->sponge_out};
+                 s->met, s->kx, s->kx2, s->kx2s, s->thbar, s->lesmet, s->fdy, s->sponge, s->sponge_out, s->wall};
   for (double* q : f)
     if (q) (void)hipFree(q);
   if (s->adi_d) (void)hipFree(s->adi_d);
@@ -389,6 +402,14 @@ extern "C" int ekm_create_tuned(const ekm_params* pp, const double* xpoint, cons
   {
     double* Sx[kAdiMaxFields] = {s->S1, s->S2, s->U2b, s->U3b};
     const int nf = p.n_th ? 4 : 3;
+    bool wall_rows = false;
+    for (int c = 0; c < 3; ++c) wall_rows = wall_rows || p.u_bc[c][EKM_YMIN] == 3;
+    if (wall_rows) {
+      const size_t n = 2 * (size_t)d.NX2 * d.NZ2;
+      if (hipMalloc(&s->wall, n * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
+      if (hipMemset(s->wall, 0, n * sizeof(double)) != hipSuccess) return fail(EKM_ERR_HIP);
+      s->wall_z = ypoint[2 + (size_t)d.NZ2 * 2] - ypoint[2 + (size_t)d.NZ2 * 1];
+    }
     for (int f = 0; f < nf; ++f) {
       AdiField& a = s->adi_h[f];
       const int c = f < 3 ? f : 3;
@@ -402,6 +423,7 @@ extern "C" int ekm_create_tuned(const ekm_params* pp, const double* xpoint, cons
       a.nut = nullptr;
       a.gz = a.gy = nullptr;
       a.pv = nullptr;
+      a.wall = f < 3 ? s->wall : nullptr;
     }
     s->adi_nf = nf;
     if (hipMalloc(&s->adi_d, sizeof(s->adi_h)) != hipSuccess) return fail(EKM_ERR_HIP);
@@ -824,9 +846,10 @@ struct SubstepPlan {
   int rk, cur_step;
   bool les_on, refresh, refresh_th, zero_cu2, cfl_now;
   int guess_on;  // 0 from zero, 1 the previous correction scaled, 2 the network's guess
+  bool wall_on, wall_first;  // LES_WALL_MODEL (W_BC_YMIN = 3), on TIME_STEP 1
   unsigned key() const {
     return (unsigned)rk | (les_on << 2) | (refresh << 3) | (refresh_th << 4) | (zero_cu2 << 5) | (cfl_now << 6) |
-           ((unsigned)guess_on << 7);
+           ((unsigned)guess_on << 7) | ((unsigned)wall_on << 9) | ((unsigned)wall_first << 10);
   }
 };
 
@@ -842,6 +865,9 @@ static SubstepPlan plan_substep(const ekm_solver* s) {
   q.zero_cu2 = p.n_th && p.sponge_type && (2 * q.cur_step) % p.save_stats_int == 0;
   // CFL time step at the third substep (:2417-2419)
   q.cfl_now = p.variable_dt && q.rk == 3 && (q.cur_step % p.update_dt) == 0;
+  // the wall model before the velocities' implicit solves (:1687-1691)
+  q.wall_on = p.u_bc[2][EKM_YMIN] == 3;
+  q.wall_first = q.wall_on && q.cur_step == 1;
   // proj_guess: the previous substep's correction, scaled by the ratio of the
   // RK weights, or the network's guess from it, starts the consistent solve
   q.guess_on = 0;
@@ -960,6 +986,22 @@ static int enqueue_substep(ekm_solver* s, const SubstepPlan& q, bool timed) {
     TRY(launch_flux_accum(s->fft, d, s->M, s->rt, s->U[0], s->TH, s->U2b, s->U3b, p.nu / p.pr, s->kx, s->CFTH,
                           s->RTH, temp5, st, q.les_on ? s->les.kappa : nullptr, q.les_on ? s->M.g12 : nullptr));
     TRY(inv(s, s->RTH));
+  }
+  // LES_WALL_MODEL (:1687-1691) from the velocities entering the implicit
+  // solves, in physical space here as there
+  if (q.wall_on) {
+    WallArgs w;
+    w.u1 = s->U[0];
+    w.u3 = s->U[2];
+    w.wall = s->wall;
+    w.r = s->rv;
+    w.z = s->wall_z;
+    w.nu = p.nu;
+    // the stress angle alpha = 20.6 is an implicitly typed default real there
+    w.cos_a = (double)std::cos(20.6f * 3.1415f / 180.0f);
+    w.sin_a = (double)std::sin(20.6f * 3.1415f / 180.0f);
+    w.first = q.wall_first;
+    TRY(launch_wall_model(d, s->M, w, st));
   }
   // ADI of the scalar (:1334-1625) and the velocities (:1640-2391): the four
   // implicit solves are independent, so they share one set of launches

@@ -962,7 +962,8 @@ int ekm_wsnet_get_norm(const ekm_wsnet* n, double* sigma, double* kx_max) {
 
 int ekm_wsnet_prepare(ekm_wsnet* n, const double* rhs, const double* prev, const double* kx2, int nb, float* x1,
                       double* scale, void* stream) {
-  if (nb < 1 || nb > n->bmax) return EKM_ERR_ARG;
+  // caller buffers only: any number of planes
+  if (nb < 1) return EKM_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const long plane = (long)n->h[0] * n->w[0];
   hipLaunchKernelGGL(k_scale, dim3(nb), dim3(256), 0, st, rhs, plane, scale);

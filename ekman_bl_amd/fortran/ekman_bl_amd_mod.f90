@@ -55,6 +55,7 @@ module ekman_bl_amd
      integer(c_int) :: nz_s_r
      integer(c_int) :: save_stats_int
      integer(c_int) :: nz_s_l
+     integer(c_int) :: wall_model_type
   end type ekm_params
 
   type, bind(c) :: ekm_diag

@@ -59,6 +59,7 @@ class EkmParams(ctypes.Structure):
         ("nz_s_r", ctypes.c_int),
         ("save_stats_int", ctypes.c_int),
         ("nz_s_l", ctypes.c_int),
+        ("wall_model_type", ctypes.c_int),
     ]
 
     # -- convenience ---------------------------------------------------------
@@ -111,6 +112,7 @@ def base_params(nx: int, ny: int, nz: int, n_th: int = 0) -> EkmParams:
     p.proj_maxit, p.proj_tol, p.proj_krylov = 50, 1.0e-12, 1   # used by proj_mode 1, 2
     p.ny_s_t = p.nz_s_r = p.save_stats_int = 1 << 30             # no sponge rows
     p.nz_s_l = -(1 << 30)
+    p.wall_model_type = 3                                        # rundir_unst/input.dat:11
     return p
 
 
@@ -348,6 +350,7 @@ def read_input_dat(path: str, p: EkmParams) -> dict:
         p.les_model = 2
     else:
         p.les_model = 0
+    p.wall_model_type = int(wall_type)
     return out
 
 

@@ -102,6 +102,10 @@ typedef struct ekm_params {
                             * MOD(2*TIME_STEP, SAVE_STATS_INT) = 0 */
   int nz_s_l;              /* NZ_S_L (grid_def, NZ+1-NZ_S_R in the shipped file): last K of the
                             * left streamwise sponge */
+  int wall_model_type;     /* WALL_MODEL_TYPE (input.dat, src/diablo.F90:255): the model
+                            * LES_WALL_MODEL (src/les_chan.F90:1914-2144) evaluates in every
+                            * substep when W_BC_YMIN = 3; type 3 (the stress model the
+                            * reference attributes to Taylor) is supported */
 } ekm_params;
 
 /* Fill with the reference defaults that are not grid-size dependent. */

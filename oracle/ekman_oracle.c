@@ -74,6 +74,11 @@ struct ora {
   double px0, qn;
   /* SPONGE_SIGMA (0:NY+1) and SPONGE_SIGMA_OUT (0:NZ+1) of sponge_th */
   double *sigma, *sigma_out;
+  /* LES_WALL_MODEL (W_BC_YMIN = 3): U_BC_LOWER_WALLMODEL and
+   * W_BC_LOWER_WALLMODEL as [2][NZ2][NX2], DYY(2,2), and U_1, W_1, U_STAR,
+   * TAO_WALL, TAO_WALL_1 of the last evaluation */
+  double* wall;
+  double wall_z, wall_info[5];
 };
 
 #define IDX(o, i, k, j) ((size_t)(i) + (size_t)(o)->NX2 * ((size_t)(k) + (size_t)(o)->NZ2 * (size_t)(j)))
@@ -1466,8 +1471,17 @@ static int supported(const ekm_params* p) {
   for (int c = 0; c < 3; ++c)
     for (int s = 0; s < 4; ++s) {
       int t = p->u_bc[c][s];
-      if (t != 0 && t != 1 && t != 6) return 0;
+      /* type 3 (wall-model rows) on the y ends, and on the z-max end for V, W */
+      if (t != 0 && t != 1 && t != 6 && !(t == 3 && (s == EKM_YMIN || s == EKM_YMAX)) &&
+          !(t == 3 && s == EKM_ZMAX && c > 0))
+        return 0;
     }
+  if (p->u_bc[2][EKM_YMIN] == 3) {
+    /* WALL_MODEL_TYPE 3 only; its z means need ZSTART = 1 and ZEND = NZ */
+    int uz0 = p->u_bc[0][EKM_ZMIN], uz1 = p->u_bc[0][EKM_ZMAX];
+    if (p->wall_model_type != 3) return 0;
+    if (!(uz0 == 1 || uz0 == 6) || !(uz1 == 0 || uz1 == 1 || uz1 == 6)) return 0;
+  }
   if (p->n_th) {
     int t;
     t = p->th_bc[EKM_YMIN]; if (t != 0 && t != 1 && t != 5 && t != 6) return 0;
@@ -1578,6 +1592,10 @@ ora* ora_create(const ekm_params* p, const double* xpoint, const double* ypoint,
   o->px0 = p->px0;
   o->qn = p->wbulk0;
   o->sigma = dalloc((size_t)o->NY + 2);
+  if (p->u_bc[0][EKM_YMIN] == 3 || p->u_bc[1][EKM_YMIN] == 3 || p->u_bc[2][EKM_YMIN] == 3) {
+    o->wall = dalloc(2 * (size_t)o->NX2 * o->NZ2);
+    o->wall_z = o->yp[PL(o, 2, 2)] - o->yp[PL(o, 2, 1)]; /* DYY(2,2), src/diablo.F90:1140 */
+  }
   o->sigma_out = dalloc((size_t)o->NZ + 2);
   sponge_sigma(o);
   init_ranges(o);
@@ -1592,7 +1610,7 @@ ora* ora_create(const ekm_params* p, const double* xpoint, const double* ypoint,
 
 void ora_destroy(ora* o) {
   if (!o) return;
-  free(o->xp); free(o->yp); free(o->thbar); free(o->jac); free(o->kx); free(o->kx2); free(o->sigma); free(o->sigma_out);
+  free(o->xp); free(o->yp); free(o->thbar); free(o->jac); free(o->kx); free(o->kx2); free(o->sigma); free(o->sigma_out); free(o->wall);
   for (int q = 0; q < NMET; ++q) free(o->met[q]);
   double* f[] = {o->U[0], o->U[1], o->U[2], o->P, o->TH, o->CF[0], o->CF[1], o->CF[2], o->CFTH,
                  o->R[0], o->R[1], o->R[2], o->RTH, o->S1, o->S2, o->U2b, o->U3b, o->CS1, o->PHI};
@@ -1602,6 +1620,61 @@ void ora_destroy(ora* o) {
   free(o->SABS); free(o->NUT); free(o->DELTA); free(o->CDYN);
   free(o->S2B); free(o->KAPT); free(o->CDYNT);
   free(o);
+}
+
+/* ------------------------------------------------------------------------ */
+/* LES_WALL_MODEL, WALL_MODEL_TYPE 3 (src/les_chan.F90:1914-2144) for one   */
+/* rank holding every x point (MIN(NXP,NXP_L) = NX-1), on the physical U1,  */
+/* U3 entering the implicit solves; loop for loop                           */
+/* ------------------------------------------------------------------------ */
+static void wall_model(ora* o, const double* U1, const double* U3, int first) {
+  const int NX = o->NX, NZ = o->NZ, js = o->jstart;
+  const size_t pl = (size_t)o->NX2 * o->NZ2;
+  double *wu = o->wall, *ww = o->wall + pl;
+  if (first) { /* TIME_STEP = 1: W_BC = U3X(I,K,2), U_BC kept (:2112-2118) */
+    for (int k = o->zstart; k <= o->zend; ++k)
+      for (int i = 0; i < NX; ++i) ww[(size_t)k * o->NX2 + i] = U3[IDX(o, i, k, 2)];
+    return;
+  }
+  const double KAPA = 0.41, NU_tmp = o->p.nu, Z = o->wall_z;
+  double* tmp_w = dalloc((size_t)NZ + 1);
+  double* tmp_u = dalloc((size_t)NZ + 1);
+  for (int k = o->zstart; k <= o->zend; ++k) { /* :1996-1999 */
+    double sw = 0.0, su = 0.0;
+    for (int i = 0; i < NX; ++i) { sw += U3[IDX(o, i, k, js)]; su += U1[IDX(o, i, k, js)]; }
+    tmp_w[k] = sw / (double)(NX - 1);
+    tmp_u[k] = su / (double)(NX - 1);
+  }
+  double su = 0.0, sw = 0.0;
+  for (int k = 1; k <= NZ; ++k) { su += tmp_u[k]; sw += tmp_w[k]; }
+  const double U_1 = su / (double)NZ, W_1 = sw / (double)NZ;
+  const double M = sqrt(U_1 * U_1 + W_1 * W_1);
+  double U_STAR = M;
+  for (int it = 1; it <= 30; ++it) U_STAR = M / (1.0 / KAPA * log(Z / NU_tmp * U_STAR) + (double)5.2f);
+  const float alpha = 20.6f; /* implicitly typed default real */
+  const double TAO_WALL = U_STAR * U_STAR * (double)cosf(alpha * 3.1415f / 180.0f);
+  const double TAO_WALL_1 = U_STAR * U_STAR * (double)sinf(alpha * 3.1415f / 180.0f);
+  for (int k = o->zstart; k <= o->zend; ++k)
+    for (int i = 0; i < NX; ++i) { /* :2099-2102 */
+      const double den = 0.5 * (C22(o, k, 1, 2) + C22(o, k + 1, 1, 1)), jk = JAC(o, k, 1);
+      const double u = U1[IDX(o, i, k, js)], w = U3[IDX(o, i, k, js)];
+      ww[(size_t)k * o->NX2 + i] = w / W_1 * (TAO_WALL / NU_tmp) / den * jk;
+      wu[(size_t)k * o->NX2 + i] = fmax(u / W_1 * (TAO_WALL / NU_tmp), u / U_1 * (TAO_WALL_1 / NU_tmp)) / den * jk;
+    }
+  o->wall_info[0] = U_1; o->wall_info[1] = W_1; o->wall_info[2] = U_STAR;
+  o->wall_info[3] = TAO_WALL; o->wall_info[4] = TAO_WALL_1;
+  free(tmp_w); free(tmp_u);
+}
+
+/* the model on given physical U1, U3 ((NX+2)(NZ+2)(NY+2) each): the row
+ * values [2][NZ+2][NX+2] into out, U_1, W_1, U_STAR, TAO_WALL, TAO_WALL_1
+ * into info (5) */
+int ora_wall_model(ora* o, const double* u1, const double* u3, int first, double* out, double* info) {
+  if (!o->wall) return EKM_ERR_ARG;
+  wall_model(o, u1, u3, first);
+  memcpy(out, o->wall, 2 * (size_t)o->NX2 * o->NZ2 * sizeof(double));
+  memcpy(info, o->wall_info, sizeof(o->wall_info));
+  return EKM_OK;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1631,7 +1704,14 @@ static void bc_y(const ora* o, int comp, int K, sysb* s, const double* u, size_t
   double c0 = comp < 3 ? p->u_bc_c1[comp][EKM_YMIN] : p->th_bc_c1[EKM_YMIN];
   double c1 = comp < 3 ? p->u_bc_c1[comp][EKM_YMAX] : p->th_bc_c1[EKM_YMAX];
   /* LOWER */
-  if (lo == 0) { row(s, 0, 0., 1., 0.); rhsc(s, 0, c0); }
+  if (lo == 3) {
+    /* IC_TYPE 7, WALL_MODEL_TYPE 3 (:532-578, :894-937, :1338-1380):
+     * U(1) - U(0) = U_BC / W_BC_LOWER_WALLMODEL(I,K); V's array is never
+     * assigned by the model (zero) */
+    row(s, 0, 0., -1., 1.);
+    for (size_t i = 0; i < s->ld; ++i)
+      s->g[i] = comp == 1 ? 0.0 : o->wall[(comp == 0 ? 0 : (size_t)o->NX2 * o->NZ2) + (size_t)K * o->NX2 + i];
+  } else if (lo == 0) { row(s, 0, 0., 1., 0.); rhsc(s, 0, c0); }
   else if (lo == 1 || lo == 5) {
     if (comp == 3) { /* :163-178 */
       row(s, 0, 0., -1., 1.);
@@ -1640,7 +1720,17 @@ static void bc_y(const ora* o, int comp, int K, sysb* s, const double* u, size_t
     else { row(s, 1, 0., -1., 1.); rhsc(s, 1, c0); }                  /* :500-506, :868-874 */
   } else { row(s, 0, 0., 1., 0.); rhs_line(s, 0, u, off, stride, NY); }
   /* UPPER */
-  if (hi == 0) {
+  if (hi == 3) {
+    /* :635-650, :1016-1030, :1480-1493; U_BC_UPPER_NWM and W_BC_UPPER_NWM
+     * are allocated and never assigned by the reference: zero here */
+    if (comp == 1) {
+      row(s, NY + 1, 0., 1., 0.); rhsc(s, NY + 1, 0.0);
+      row(s, NY, 0., 1., 0.); rhsc(s, NY, 0.0);
+    } else {
+      row(s, NY, -1., 1., 0.); rhsc(s, NY, 0.0);
+      row(s, NY + 1, 0., 1., 0.); rhs_line(s, NY + 1, u, off, stride, NY);
+    }
+  } else if (hi == 0) {
     row(s, NY + 1, 0., 1., 0.); rhsc(s, NY + 1, c1);
     if (comp == 1 || comp == 2) { row(s, NY, 0., 1., 0.); rhsc(s, NY, c1); } /* :951-960, :1410-1419 */
   } else if (hi == 1) {
@@ -1674,7 +1764,15 @@ static void bc_z(const ora* o, int comp, int J, sysb* s, const double* u, size_t
   } else if (lo == 1) { row(s, 0, 0., -1., 1.); rhsc(s, 0, c0); }
   else { row(s, 0, 0., 1., 0.); rhs_line(s, 0, u, off, stride, NZ); }
   /* RIGHT */
-  if (hi == 0) {
+  if (hi == 3) { /* V :1249-1262, W :1722-1734 (W_BC_UPPER_NWM never assigned: zero) */
+    if (comp == 1) {
+      row(s, NZ, -1., 1., 0.); rhsc(s, NZ, 0.0);
+      row(s, NZ + 1, 0., 1., 0.); rhs_line(s, NZ + 1, u, off, stride, NZ);
+    } else {
+      row(s, NZ + 1, -1., 1., 0.); rhsc(s, NZ + 1, 0.0);
+      row(s, NZ, 0., 1., 0.); rhsc(s, NZ, 0.0);
+    }
+  } else if (hi == 0) {
     double vtop = comp == 2 ? c1 : 0.0; /* U3: C1 on NZ+1 and NZ (:1623-1632) */
     row(s, NZ + 1, 0., 1., 0.); rhsc(s, NZ + 1, vtop);
     row(s, NZ, 0., 1., 0.); rhsc(s, NZ, c1);
@@ -3001,6 +3099,8 @@ int ora_rk_duct(ora* o) {
     inv(o, o->RTH);
     adi(o, 3, CTH, o->RTH, TEMP1 / p->pr, les ? o->KAPT : NULL, les_gz(o, 3), les_gy(o, 3), hb / 2.0);
   }
+  /* :1687-1691 the wall model, before the velocities' implicit solves */
+  if (p->u_bc[2][EKM_YMIN] == 3) wall_model(o, o->U[0], o->U[2], cur_step == 1);
   /* :1640-2391 velocity ADI */
   for (int f = 0; f < 3; ++f)
     adi(o, f, o->U[f], o->R[f], TEMP1, les ? o->NUT : NULL, les_gz(o, f), les_gy(o, f), hb / 2.0);

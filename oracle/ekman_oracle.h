@@ -46,6 +46,11 @@
  *   ora_omega_x                 sheared affine grid: linear U2, U3 give
  *                               dW/dy - dV/dz exactly, and the row fills
  *   ora_get_physical            the inverse transform pinned above
+ *   ora_wall_model              uniform velocities at JSTART: U_1, W_1 in
+ *                               closed form, U_STAR a fixed point of the log
+ *                               law, the row values in closed form; the
+ *                               type-3 rows: U(1) - U(0) equals the model's
+ *                               value after the implicit solve
  *   ora_mean_grads              sheared affine grid: linear mean fields give
  *                               their exact derivatives, Rig, TKE, the fills
  *                               and THBULK in closed form
@@ -104,6 +109,10 @@ int ora_xmoments(ora* o, double* out);
 int ora_omega_x(ora* o, double* out);
 /* field f (0..4: U1, U2, U3, P, TH) in physical space, (NX+2)*(NZ+2)*(NY+2) */
 int ora_get_physical(ora* o, int f, double* out);
+/* LES_WALL_MODEL (type 3) on given physical U1, U3: the lower-row values
+ * [2][NZ+2][NX+2] (U1's, U3's) into out and U_1, W_1, U_STAR, TAO_WALL,
+ * TAO_WALL_1 into info[5]; EKM_ERR_ARG without type-3 rows */
+int ora_wall_model(ora* o, const double* u1, const double* u3, int first, double* out, double* info);
 /* one field's ADI (comp 0..2 velocity, 3 scalar) on physical U, R in place,
  * with the subgrid viscosity nut (physical P field) unless NULL */
 int ora_adi_field(ora* o, int comp, double* U, double* R, double t1, const double* nut, double t2);

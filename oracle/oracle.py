@@ -57,6 +57,7 @@ def lib():
         L.ora_xmoments.argtypes = [ctypes.c_void_p, _D]
         L.ora_omega_x.argtypes = [ctypes.c_void_p, _D]
         L.ora_get_physical.argtypes = [ctypes.c_void_p, ctypes.c_int, _D]
+        L.ora_wall_model.argtypes = [ctypes.c_void_p, _D, _D, ctypes.c_int, _D, _D]
         L.ora_xstats.argtypes = [ctypes.c_void_p, _D]
         L.ora_adi_field.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.c_double, _D, ctypes.c_double]
         L.ora_set_initial_physical.argtypes = [ctypes.c_void_p, _D, _D, _D, _D]
@@ -181,6 +182,19 @@ class Oracle:
         out = np.zeros((p.ny + 2, p.nz + 2, p.nx + 2))
         lib().ora_get_physical(self.h, f, _p(out))
         return out
+
+    def wall_model(self, u1: np.ndarray, u3: np.ndarray, first: bool = False):
+        """LES_WALL_MODEL (type 3) on physical U1, U3 ((NY+2, NZ+2, NX+2)):
+        (row values (2, NZ+2, NX+2) of U1 and U3, (U_1, W_1, U_STAR, TAO_WALL,
+        TAO_WALL_1))."""
+        p = self.p
+        out = np.zeros((2, p.nz + 2, p.nx + 2))
+        info = np.zeros(5)
+        a, b = np.ascontiguousarray(u1, dtype=np.float64), np.ascontiguousarray(u3, dtype=np.float64)
+        st = lib().ora_wall_model(self.h, _p(a), _p(b), int(first), _p(out), _p(info))
+        if st != 0:
+            raise ValueError("the oracle has no wall-model rows")
+        return out, info
 
     def mean_grads(self):
         """SAVE_STATS_CURVI's mean gradients: (dict name -> (NY+2, NZ+2), THBULK)."""

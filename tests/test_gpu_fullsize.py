@@ -3,7 +3,12 @@ stratified cases, which the oracle cannot run: properties that do not depend
 on the size.  Every substep of one step reports status 0, max |J div u|
 below 1e-10 and a positive cycle count; the x statistics (means, rms sums,
 fluxes) stay finite; the mean of U1 at the top rows stays near the
-geostrophic value the initial condition sets."""
+geostrophic value the initial condition sets.  The 1024x385x769 case then
+restarts from that state at step 3 (a restart's first step refreshes the
+subgrid coefficients in every substep) and runs a step with both subgrid
+models active: status 0, max |J div u| below 1e-10, C_DYN >= 0,
+C_DYN_TH >= 0, NU_T and KAPPA_T finite and non-negative, some NU_T
+positive."""
 import numpy as np
 import pytest
 
@@ -43,4 +48,22 @@ def test_full_size_headline_form_properties(n):
     ug = p.u_bc_c1[0][YMAX]
     top = st[S.DuctSolver.XSTATS[0]][p.ny - 2:p.ny, 2:p.nz]
     assert np.max(np.abs(top - ug)) < 0.1 * max(abs(ug), 1.0)
+    if p.les_model:
+        state = s.get_state()
+        s.set_state(*state[:4], state[4], time=0.0, time_step=3)
+        del state
+        for sub in range(3):
+            s.rk_duct()
+            d = s.diag()
+            print(f"[fullsize] config {n} LES substep {sub}: status {d.status} div_max {d.div_max:.2e} "
+                  f"cycles {d.mg_cycles}", flush=True)
+            assert d.status == 0 and d.div_max < 1e-10
+        nut, cd = s.get_les()
+        kap, cdt = s.get_les_th()
+        print(f"[fullsize] config {n} LES: max NU_T {nut.max():.3e}, max C_DYN {cd.max():.3e}, "
+              f"max KAPPA_T {kap.max():.3e}, max C_DYN_TH {cdt.max():.3e}", flush=True)
+        assert np.all(np.isfinite(nut)) and np.all(np.isfinite(kap))
+        assert cd.min() >= 0.0 and cdt.min() >= 0.0 and nut.min() >= 0.0 and kap.min() >= 0.0
+        assert nut.max() > 0.0
+        del nut, kap
     s.close()

@@ -10,9 +10,11 @@
   operator on the interior cells, so a discretisation-size difference);
 - a stratified LES run at 128x97x129 (the 256x193x257 case's physics, 5%
   velocity and 1% scalar noise) for 12 steps from a new flow: the subgrid
-  models start at step 2 and refresh C_DYN at steps 5 and 10 and C_DYN_TH at
-  step 10 (src/les_chan.F90:475-814, src/les_chan_th.F90:146-334); fields,
-  NU_T, KAPPA_T, C_DYN and C_DYN_TH held to 1e-10 relative L2.
+  models run from step 2 with zero coefficients (FIRST_TIME is cleared after
+  step 1, src/diablo.F90:157) until C_DYN's refresh at steps 5 and 10 and
+  C_DYN_TH's at step 10 (src/les_chan.F90:475-814, src/les_chan_th.F90:
+  146-334); fields, NU_T, KAPPA_T, C_DYN and C_DYN_TH held to 1e-10
+  relative L2.
 """
 import numpy as np
 import pytest

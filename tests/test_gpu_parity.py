@@ -244,6 +244,52 @@ def test_unconverged_projection_is_flagged():
         s.close()
 
 
+@pytest.mark.parametrize("les,upper", [(0, False), (2, False), (0, True)])
+def test_wall_model_rows_match_oracle(les, upper, oracle_lib):
+    """Type-3 lower rows of all three velocities with LES_WALL_MODEL type 3
+    (src/les_chan.F90:1914-2144, src/boundary.F90:532-578, :894-937,
+    :1338-1380) on the 64x65x65 neutral case, with and without the subgrid
+    model, against the oracle: without it, three steps from a new flow (step
+    1 takes the model's first-step branch, U3's row value from J = 2; steps 2
+    and 3 the stress model); with it, two steps of a restart at step 3 (the
+    subgrid coefficients refresh in every substep of a restart's first step).
+    With `upper`, the type-3 rows of the upper y end (all three velocities,
+    src/boundary.F90:635-650, :1016-1030, :1480-1493) and of the z-max end
+    (V and W, :1249-1262, :1722-1734) as well.  Fields to 1e-10 relative L2."""
+    O = oracle_lib
+    cfg, p, xp, yp = _setup(1)
+    for c in range(3):
+        p.u_bc[c][0] = 3
+        if upper:
+            p.u_bc[c][1] = 3
+    if upper:
+        p.u_bc[1][3] = p.u_bc[2][3] = 3
+    p.wall_model_type = 3
+    p.les_model = les
+    ic = ekman_initial(p, yp, cfg.seed, cfg.noise, cfg.th_noise)
+    o = O.Oracle(p, xp, yp)
+    s = S.DuctSolver(p, xp, yp)
+    o.set_initial_physical(*ic)
+    s.set_initial_physical(*ic)
+    if les:
+        st = o.get_state()
+        o.set_state(*st[:4], st[4], time=0.0, time_step=3)
+        s.set_state(*st[:4], st[4], time=0.0, time_step=3)
+    for step in range(1, 3 if les else 4):
+        s.time_step(1)
+        o.time_step(1)
+        r = _state_rel(s.get_state(), o.get_state())
+        d = s.diag()
+        print(f"[wall les={les} upper={upper}] step {step}: fields {max(r):.2e} status {d.status} div_max {d.div_max:.2e}",
+              flush=True)
+        assert d.status == 0
+        assert max(r) < 1e-10
+    if les:
+        nut, cd = o.get_les()
+        assert cd.max() > 0.0
+    s.close()
+
+
 def test_warm_starts_agree():
     """The warm start of the consistent solve (proj_guess 0: from zero, 1: the
     previous substep's correction scaled by the RK weights) only changes where

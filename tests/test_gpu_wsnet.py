@@ -5,7 +5,8 @@ the solver.
 - forward output and parameter gradients: the oracle rounds the matrix-core
   operands to bf16 as the GPU does, so the two differ only by fp32
   accumulation (relative L2 <= 2e-3 per layer);
-- input preparation (full weighting, per-solve scale, channels): <= 1e-6;
+- input preparation (full weighting, per-solve scale, channels): FP64
+  arithmetic rounded once to FP32, within 2.5e-7 relative of the oracle;
 - training with the library's own backward and Adam kernels lowers the loss
   on pairs dumped from the 64x65x65 case;
 - the consistent projection started from the trained network's guess
@@ -107,7 +108,12 @@ def test_prepare_matches_oracle():
     xo, so = OW.prepare(rhs, prev, np.sqrt(kx2), 2.0, 7.0)
     assert np.array_equal(sc.download(), so)
     xg = x1.download().reshape(n, 33, 33, 4)
-    assert np.abs(xg - xo).max() < 1e-6
+    # the kernel forms each input in FP64 and rounds it once to FP32: within
+    # two FP32 rounding units of the oracle's FP64 value (the second covers a
+    # different FP64 summation order in the restriction)
+    err = np.abs(xg - xo) / np.maximum(np.abs(xo), 1e-30)
+    print(f"[wsnet] prepare: max relative {err.max():.2e}", flush=True)
+    assert err.max() < 2.5e-7
     net.close()
 
 
