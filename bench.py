@@ -203,7 +203,9 @@ def main() -> int:
     ap.add_argument("--krylov", type=int, default=1, choices=(0, 1),
                     help="outer iteration of modes 1 and 2 (0: plain refinement, 1: GCR(1))")
     ap.add_argument("--side-steps", type=int, default=3, help="steps of the compared forms (0: skip them)")
-    ap.add_argument("--side-warmup", type=int, default=2, help="warm-up steps of the compared forms")
+    ap.add_argument("--side-warmup", type=int, default=4,
+                    help="warm-up steps of the compared forms (4: the subgrid models' first refresh, at step 5, "
+                         "falls in the timed steps)")
     ap.add_argument("--learned", default=None, help="trained warm-start weights (default: ekman_bl_amd/data)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", type=int, default=0,

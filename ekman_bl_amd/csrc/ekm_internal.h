@@ -70,6 +70,7 @@ struct StepState {
   double amp;        // Sponge_Amp = 0.1 / DELTA_T
   double guess;      // start of the consistent solve: 0 (from zero) or the scale of mg.phi
   double hb_prev;    // H_BAR of the previous substep
+  double prev_ratio; // H_BAR of the substep in flight over hb_prev (0 before any substep)
   double px0, qn;    // F_TYPE 0 controller
   // reduction accumulators; each is reset by the kernel that consumes it:
   // 0 sum |J div u|, 1 max |J div u| (bits), 2 CFL minimum (bits), 3 flow rate

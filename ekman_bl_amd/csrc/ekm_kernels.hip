@@ -1646,6 +1646,7 @@ __global__ void k_step_begin(StepState* ss, int rk, StepConsts c) {
   else
     ss->px = c.px0 + c.amp_omega0 * cos(c.omega0 * ss->time);
   ss->amp = 0.1 / ss->delta_t;
+  ss->prev_ratio = ss->hb_prev > 0.0 ? hb / ss->hb_prev : 0.0;
   ss->guess = c.guess_on == 1 ? hb / ss->hb_prev : c.guess_on == 2 ? 1.0 : 0.0;
 }
 

@@ -37,6 +37,9 @@
 namespace {
 
 constexpr int kMaxLevT = 16;
+// threads per workgroup of the solve kernel: 16 wavefronts where the line
+// registers fit (FP32, lines of at most 5 stripes, no cross terms), else 8
+constexpr int tight_block(bool fp32, int cm, bool cross) { return (fp32 && cm <= 5 && !cross) ? 1024 : 512; }
 constexpr int kDirectMax = 128;  // the coarse level solved directly has at most this many points
 constexpr int kMaxProg = 4 * kMaxLevT + 4;
 
@@ -46,7 +49,8 @@ struct TLev {
   int e_off, r_off, res_off;  // per-solve workspace offsets (elements of T)
   int t2_off;                 // second line-solve scratch (the residual array is the first)
   int lds;                    // 1: the four arrays above live in the workgroup's LDS
-  int f_off;                  // per-mode offset: z factors [3][n], y factors [3][n] (column-parity layout), c9 [n]
+  int f_off;                  // per-mode offset: z factors [3][n], y factors [3][n] (column-parity layout), c9 [n],
+                              // y-chunk products A [n] and Be [n] (column-parity layout)
   const T* cf;                // c1..c8 planes [8][n]; rows of specified points are 0 (c9 = 1)
   const T* cfy;               // the same planes in the column-parity layout of the y-line smoother
   const unsigned char* spec;  // specified (Dirichlet) points
@@ -170,6 +174,34 @@ __device__ inline Aff<T> wave_scan_down(Aff<T> v) {
 // 64-lane scan of affine maps, carried from stripe to stripe.  ZERO: the
 // correction is zero on the neighbouring rows (the first sweep of a cycle's
 // down leg), so they are not read.
+//
+// Lines of more than kZRegStripes stripes are streamed: each stripe's loads
+// are issued while the previous stripe is scanned and only the forward
+// results stay in registers, which keeps the kernel's register count (and so
+// its occupancy) independent of the line length.  The arithmetic is the same
+// in both forms.
+constexpr int kZRegStripes = 5;
+
+template <typename T, bool X, bool ZERO>
+__device__ inline void zline_point(const T* __restrict__ e, const T* __restrict__ r, const T* __restrict__ cf,
+                                   const T* fa, const T* f1, int nx, int n, int ld, int i, int j, T& a, T& b) {
+  a = (T)1;
+  b = (T)0;
+  if (i < nx) {
+    const int p = i + nx * j;
+    const int q = (i + 1) + ld * (j + 1);
+    T s = r[p];
+    if (!ZERO) {
+      s -= cf[2 * n + p] * e[q + ld] + cf[6 * n + p] * e[q - ld];
+      if (X)
+        s -= (cf[1 * n + p] * e[q + 1 + ld] + cf[3 * n + p] * e[q - 1 + ld]) +
+             (cf[5 * n + p] * e[q - 1 - ld] + cf[7 * n + p] * e[q + 1 - ld]);
+    }
+    a = fa[p];
+    b = s * f1[p];
+  }
+}
+
 template <typename T, int CM, bool X, bool ZERO = false>
 __device__ void zlines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ fac, int par) {
   T* W = wbase(L, Wp);
@@ -181,6 +213,42 @@ __device__ void zlines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
   const T* f1 = fac + L.f_off;
   const T* fa = f1 + n;
   const T* fc = fa + n;
+  if constexpr (CM > kZRegStripes) {
+    for (int j = par + 2 * wave; j < ny; j += 2 * nw) {
+      T dk[CM];
+      T an, bn;
+      zline_point<T, X, ZERO>(e, r, cf, fa, f1, nx, n, ld, lane, j, an, bn);
+      T carry = (T)0;
+#pragma unroll
+      for (int c = 0; c < CM; ++c) {
+        const T a = an, b = bn;
+        if (c + 1 < CM) zline_point<T, X, ZERO>(e, r, cf, fa, f1, nx, n, ld, 64 * (c + 1) + lane, j, an, bn);
+        const Aff<T> inc = wave_scan_up(Aff<T>{a, b});
+        dk[c] = inc.a * carry + inc.b;
+        carry = rdlane(dk[c], 63);
+      }
+      T cn = (T)0;
+      {
+        const int i = 64 * (CM - 1) + lane;
+        if (i < nx) cn = fc[i + nx * j];
+      }
+      T xo = (T)0;
+#pragma unroll
+      for (int c = CM - 1; c >= 0; --c) {
+        const T ck = cn;
+        if (c > 0) {
+          const int i = 64 * (c - 1) + lane;
+          cn = i < nx ? fc[i + nx * j] : (T)0;
+        }
+        const Aff<T> inc = wave_scan_down(Aff<T>{-ck, dk[c]});
+        const T x = inc.a * xo + inc.b;
+        const int i = 64 * c + lane;
+        if (i < nx) e[(i + 1) + ld * (j + 1)] = x;
+        xo = rdlane(x, 0);
+      }
+    }
+    return;
+  }
   for (int j = par + 2 * wave; j < ny; j += 2 * nw) {
     T ak[CM], bk[CM], ck[CM], dk[CM];
 #pragma unroll
@@ -242,6 +310,22 @@ __host__ __device__ inline int ysplit(int i, int j, int nx, int ny) {
   return (i & 1) ? ncol0 * ny + (i >> 1) + (nx >> 1) * j : (i >> 1) + ncol0 * j;
 }
 constexpr int kYMaxTasks = 32;
+// the rows of a y-line pass split into chunks: nch chunks of cl rows per
+// 64-column group (ngrp groups), one task per (group, chunk)
+struct YChunks {
+  int ncol, ngrp, nch, cl;
+};
+__host__ __device__ inline YChunks ychunks(int nx, int ny, int par, int nw) {
+  YChunks c;
+  c.ncol = (nx - par + 1) / 2;
+  c.ngrp = (c.ncol + 63) / 64;
+  c.nch = nw / c.ngrp;
+  if (c.nch < 1) c.nch = 1;
+  if (c.nch > ny) c.nch = ny;
+  if (c.ngrp * c.nch > kYMaxTasks) c.nch = kYMaxTasks / c.ngrp;
+  c.cl = (ny + c.nch - 1) / c.nch;
+  return c;
+}
 constexpr int kYUnroll = 3;
 constexpr int kYApply = 8;  // rows per batch of the last pass
 constexpr int kYBack = 6;   // rows per batch of the backward pass
@@ -259,21 +343,23 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
   const T* f1 = fac + L.f_off + 3 * n;
   const T* fa = f1 + n;
   const T* fc = fa + n;
-  const int ncol = (nx - par + 1) / 2;
+  // the data-independent parts of the chunk maps, precomputed per mode:
+  // yA = products of fa from the chunk's first row, yBe = products of -fc
+  // from its last row
+  const T* yA = fac + L.f_off + 7 * n;
+  const T* yBe = yA + n;
+  const YChunks yc = ychunks(nx, ny, par, nw);
+  const int ncol = yc.ncol, nch = yc.nch, cl = yc.cl;
   // column-parity layout: this parity's block starts at pb, rows ncol apart
   const int pb = par ? ((nx + 1) >> 1) * ny : 0;
-  const int ngrp = (ncol + 63) / 64;
-  int nch = nw / ngrp;
-  if (nch < 1) nch = 1;
-  if (nch > ny) nch = ny;
-  if (ngrp * nch > kYMaxTasks) nch = kYMaxTasks / ngrp;
-  const int cl = (ny + nch - 1) / nch;
-  const int ntask = ngrp * nch;
+  const int ntask = yc.ngrp * nch;
+  const int ngrp = yc.ngrp;
+  (void)tA;
   // LDS: per task and lane: chunk map (a, b), then the entering value
   T* sa = lds;
   T* sb = sa + kYMaxTasks * 64;
   T* sv = sb + kYMaxTasks * 64;
-  // forward: dp_j = A_j dp_in + B_j within the chunk
+  // forward: dp_j = A_j dp_in + B_j within the chunk; only B is formed here
   for (int task = wave; task < ntask; task += nw) {
     const int grp = task / nch, ch = task - grp * nch;
     const int col = grp * 64 + lane;
@@ -281,6 +367,7 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
     const int j0 = ch * cl, j1 = min(j0 + cl, ny);
     T A = (T)1, B = (T)0;
     if (i < nx) {
+      if (j1 > j0) A = yA[pb + col + ncol * (j1 - 1)];
       for (int jb = j0; jb < j1; jb += kYUnroll) {
         T sv_[kYUnroll], av[kYUnroll], fv[kYUnroll];
 #pragma unroll
@@ -305,11 +392,8 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
         for (int u = 0; u < kYUnroll; ++u) {
           const int j = jb + u;
           if (j < j1) {
-            A = av[u] * A;
             B = av[u] * B + sv_[u] * fv[u];
-            const int ps = pb + col + ncol * j;
-            tA[ps] = A;
-            tB[ps] = B;
+            tB[pb + col + ncol * j] = B;
           }
         }
       }
@@ -337,6 +421,7 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
     const T din = sv[task * 64 + lane];
     T al = (T)0, be = (T)1;
     if (i < nx) {
+      if (j1 > j0) be = yBe[pb + col + ncol * j0];
       for (int jb = j1 - 1; jb >= j0; jb -= kYBack) {
         T a_[kYBack], b_[kYBack], c_[kYBack];
 #pragma unroll
@@ -344,7 +429,7 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
           const int j = jb - u;
           if (j >= j0) {
             const int ps = pb + col + ncol * j;
-            a_[u] = tA[ps];
+            a_[u] = yA[ps];
             b_[u] = tB[ps];
             c_[u] = fc[ps];
           }
@@ -355,10 +440,7 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
           if (j >= j0) {
             const T dp = a_[u] * din + b_[u];
             al = dp - c_[u] * al;
-            be = -c_[u] * be;
-            const int ps = pb + col + ncol * j;
-            tA[ps] = al;
-            tB[ps] = be;
+            tB[pb + col + ncol * j] = al;
           }
         }
       }
@@ -389,8 +471,8 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
 #pragma unroll
       for (int u = 0; u < kYApply; ++u) {
         const int j = min(jb + u, j1 - 1);
-        a_[u] = tA[pb + col + ncol * j];
-        b_[u] = tB[pb + col + ncol * j];
+        a_[u] = tB[pb + col + ncol * j];
+        b_[u] = yBe[pb + col + ncol * j];
       }
 #pragma unroll
       for (int u = 0; u < kYApply; ++u)
@@ -884,7 +966,7 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
     L.ny = ny;
     L.n = nx * ny;
     L.f_off = (int)fstride;
-    fstride += 7L * L.n;  // z factors, y factors, c9
+    fstride += 9L * L.n;  // z factors, y factors, c9, y-chunk products
   }
   int lc = 1;
   for (int k = 1; k <= ng; ++k)
@@ -969,7 +1051,7 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
   // kernel's static arrays live there; the others (always the finest) in
   // global memory, one slice per solve
   {
-    const int nt = (sizeof(T) == 4 && tp.cm <= 5 && !cross) ? 1024 : 512;
+    const int nt = tight_block(sizeof(T) == 4, tp.cm, cross);
     const long stat = (long)nt * 8 + 3L * kYMaxTasks * 64 * (long)sizeof(T);
     int dev = 0, maxsh = 0;
     EKM_CHECK(hipGetDevice(&dev));
@@ -995,7 +1077,9 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
     tp.lds_bytes = (int)lds;
     g.ws = ws;
   }
-  // coefficient tables, and per live mode: line factors, c9, coarsest inverse
+  // coefficient tables, and per live mode: line factors, c9, y-chunk
+  // products (for the kernel's wavefront count), coarsest inverse
+  const int nw = tight_block(sizeof(T) == 4, tp.cm, cross) / 64;
   const size_t nfine = (size_t)g.lev[ng].n;
   std::vector<T> cfT(8 * tot_n), cfyT(8 * tot_n);
   std::vector<T> fac((size_t)nmode * fstride, (T)0);
@@ -1049,6 +1133,26 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
           fy[qs] = spec[off + q] ? (T)0 : (T)f1[j];
           fy[n + qs] = (T)fa[j];
           fy[2 * n + qs] = (T)fc[j];
+        }
+        // the chunk products of this column, in the cycle precision and the
+        // order the smoother would form them
+        const YChunks yc = ychunks(nx, ny, i & 1, nw);
+        T* yA = fz + 7 * n;
+        T* yBe = fz + 8 * n;
+        for (int ch = 0; ch < yc.nch; ++ch) {
+          const int j0 = ch * yc.cl, j1 = std::min(j0 + yc.cl, ny);
+          T A = (T)1;
+          for (int j = j0; j < j1; ++j) {
+            const size_t qs = (size_t)ysplit(i, j, nx, ny);
+            A = fy[n + qs] * A;
+            yA[qs] = A;
+          }
+          T be = (T)1;
+          for (int j = j1 - 1; j >= j0; --j) {
+            const size_t qs = (size_t)ysplit(i, j, nx, ny);
+            be = -fy[2 * n + qs] * be;
+            yBe[qs] = be;
+          }
         }
       }
       if (k == lc) {
@@ -1146,11 +1250,10 @@ int tight_solve_ex(TightPlan& tp, MgPlan& mg, const Dims& d, int nsolve, const d
   if (nsolve > tp.nsolve_max) return EKM_ERR_ARG;
   // a multiple of 16 workgroups for the XCD placement of the solve pairs
   const dim3 grid((unsigned)((nsolve + 15) & ~15));
-  // 16 wavefronts per workgroup where the line registers fit (FP32 up to 5
-  // points per lane), else 8
+  // workgroup size: tight_block
 #define LAUNCH2(T, CM, X)                                                                                \
   do {                                                                                                   \
-    constexpr int NT = (sizeof(T) == 4 && CM <= 5 && !X) ? 1024 : 512;                                   \
+    constexpr int NT = tight_block(sizeof(T) == 4, CM, X);                                                \
     EKM_CHECK(hipFuncSetAttribute((const void*)k_tight_solve<T, CM, NT, X>,                             \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, tp.lds_bytes));            \
     hipLaunchKernelGGL((k_tight_solve<T, CM, NT, X>), grid, dim3(NT), tp.lds_bytes, st, (const TDesc<T>*)tp.mem, \

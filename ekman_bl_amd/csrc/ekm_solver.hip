@@ -767,7 +767,7 @@ static int rem_div(ekm_solver* s, const double* guess, bool timed, bool learned 
   TRY(launch_mg_gather(d, s->M, s->rv, s->kx, s->U[0], s->U2b, s->U3b, s->mg.rhs, nmode, st));
   // the learned start: the network reads this projection's right-hand sides
   // and the previous solutions in mg.phi and writes its guess there
-  if (learned) TRY(wsnet_guess(s->net, s->mg.rhs, s->mg.phi, s->kx2s, nsolve, s->mg.phi, st));
+  if (learned) TRY(wsnet_guess(s->net, s->mg.rhs, s->mg.phi, SS_PTR(s, prev_ratio), s->kx2s, nsolve, s->mg.phi, st));
   if (timed) {
     if (s->evs.size() < 2 * (s->ev_used + 1)) {
       hipEvent_t a, b;
@@ -1104,7 +1104,8 @@ extern "C" int ekm_set_warm_start_net(ekm_solver* s, ekm_wsnet* net) {
 }
 
 extern "C" int ekm_solve_planes(ekm_solver* s, const double** rhs, const double** phi, const double** kx2,
-                                int* nsolve) {
+                                const double** ratio, int* nsolve) {
+  if (ratio) *ratio = SS_PTR(s, prev_ratio);
   if (rhs) *rhs = s->mg.rhs;
   if (phi) *phi = s->mg.phi;
   if (kx2) *kx2 = s->kx2s;

@@ -502,10 +502,12 @@ __device__ inline double fw(const double* v, int h0, int w0, int cy, int cx) {
   }
   return acc;
 }
-// network input at level 1 (4 channels) from the planes
+// network input at level 1 (4 channels) from the planes; the previous
+// solution enters scaled by ratio (the H_BAR ratio of the two substeps)
 __global__ void k_prep(const double* __restrict__ rhs, const double* __restrict__ prev,
-                       const double* __restrict__ kx2, const double* __restrict__ s, int n, int h0, int w0,
-                       double sigma, double kx_max, float* __restrict__ x) {
+                       const double* __restrict__ ratio, const double* __restrict__ kx2,
+                       const double* __restrict__ s, int n, int h0, int w0, double sigma, double kx_max,
+                       float* __restrict__ x) {
   const int h1 = (h0 + 1) / 2, w1 = (w0 + 1) / 2;
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (long)n * h1 * w1) return;
@@ -515,30 +517,36 @@ __global__ void k_prep(const double* __restrict__ rhs, const double* __restrict_
   if (sc > 0.0) {
     const long pl = (long)h0 * w0;
     o.x = (float)(fw(rhs + sm * pl, h0, w0, cy, cx) / sc);
-    o.y = prev ? (float)(fw(prev + sm * pl, h0, w0, cy, cx) / (sc * sigma)) : 0.f;
+    const double rt = ratio ? *ratio : 1.0;
+    o.y = prev ? (float)(rt * fw(prev + sm * pl, h0, w0, cy, cx) / (sc * sigma)) : 0.f;
     o.z = (float)(sqrt(kx2[sm]) / kx_max);
     o.w = (float)cy / (float)(h1 - 1);
   }
   reinterpret_cast<float4*>(x)[i] = o;
 }
-// training target t = phi / (s sigma) at level 0 (zero where s = 0)
-__global__ void k_target(const double* __restrict__ phi, const double* __restrict__ s, int n, long plane,
+// training target at level 0: what the network adds to the scaled previous
+// solution, t = (phi - ratio prev) / (s sigma) (zero where s = 0)
+__global__ void k_target(const double* __restrict__ phi, const double* __restrict__ prev,
+                         const double* __restrict__ ratio, const double* __restrict__ s, int n, long plane,
                          double sigma, float* __restrict__ t) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n * plane) return;
   const double sc = s[i / plane];
-  t[i] = sc > 0.0 ? (float)(phi[i] / (sc * sigma)) : 0.f;
+  const double base = prev ? (ratio ? *ratio : 1.0) * prev[i] : 0.0;
+  t[i] = sc > 0.0 ? (float)((phi[i] - base) / (sc * sigma)) : 0.f;
 }
-// the guess: s sigma up(y) at level 0, as doubles
-__global__ void k_guess_out(const float* __restrict__ y, const double* __restrict__ s, int n, int h0, int w0,
-                            double sigma, double* __restrict__ out) {
+// the guess at level 0, as doubles: ratio prev + s sigma up(y); out may
+// alias prev (each point reads its own previous value first)
+__global__ void k_guess_out(const float* __restrict__ y, const double* prev, const double* __restrict__ ratio,
+                            const double* __restrict__ s, int n, int h0, int w0, double sigma, double* out) {
   const int h1 = (h0 + 1) / 2, w1 = (w0 + 1) / 2;
   const long plane = (long)h0 * w0;
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n * plane) return;
   const int sm = (int)(i / plane), q = (int)(i - sm * plane), fy = q / w0, fx = q - fy * w0;
   const double sc = s[sm];
-  out[i] = sc > 0.0 ? sc * sigma * (double)up_at(y + (long)sm * h1 * w1, w1, fy, fx) : 0.0;
+  const double base = prev ? (ratio ? *ratio : 1.0) * prev[i] : 0.0;
+  out[i] = sc > 0.0 ? base + sc * sigma * (double)up_at(y + (long)sm * h1 * w1, w1, fy, fx) : base;
 }
 
 }  // namespace
@@ -814,21 +822,21 @@ void wsnet_dims(const ekm_wsnet* net, int* nz2, int* ny2, int* bmax) {
   *bmax = net->bmax;
 }
 
-int wsnet_guess(ekm_wsnet* n, const double* rhs, const double* prev, const double* kx2, int nb, double* out,
-                hipStream_t st) {
+int wsnet_guess(ekm_wsnet* n, const double* rhs, const double* prev, const double* ratio, const double* kx2,
+                int nb, double* out, hipStream_t st) {
   if (nb < 1 || nb > n->bmax) return EKM_ERR_ARG;
   if (ensure_packed(n, st) != EKM_OK) return EKM_ERR_HIP;
   const long plane = (long)n->h[0] * n->w[0];
   hipLaunchKernelGGL(k_scale, dim3(nb), dim3(256), 0, st, rhs, plane, n->scale);
   EKM_CHECK(hipGetLastError());
   const long N1 = (long)nb * n->h[1] * n->w[1];
-  hipLaunchKernelGGL(k_prep, dim3(nblk(N1, 256)), dim3(256), 0, st, rhs, prev, kx2, n->scale, nb, n->h[0], n->w[0],
-                     n->sigma, n->kx_max, n->T[0].a);
+  hipLaunchKernelGGL(k_prep, dim3(nblk(N1, 256)), dim3(256), 0, st, rhs, prev, ratio, kx2, n->scale, nb, n->h[0],
+                     n->w[0], n->sigma, n->kx_max, n->T[0].a);
   EKM_CHECK(hipGetLastError());
   const int e = run_forward(n, nb, st);
   if (e != EKM_OK) return e;
   hipLaunchKernelGGL(k_guess_out, dim3(nblk((size_t)nb * plane, 256)), dim3(256), 0, st, n->T[n->ops.back().out].a,
-                     n->scale, nb, n->h[0], n->w[0], n->sigma, out);
+                     prev, ratio, n->scale, nb, n->h[0], n->w[0], n->sigma, out);
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }
@@ -960,8 +968,8 @@ int ekm_wsnet_get_norm(const ekm_wsnet* n, double* sigma, double* kx_max) {
   return EKM_OK;
 }
 
-int ekm_wsnet_prepare(ekm_wsnet* n, const double* rhs, const double* prev, const double* kx2, int nb, float* x1,
-                      double* scale, void* stream) {
+int ekm_wsnet_prepare(ekm_wsnet* n, const double* rhs, const double* prev, const double* ratio, const double* kx2,
+                      int nb, float* x1, double* scale, void* stream) {
   // caller buffers only: any number of planes
   if (nb < 1) return EKM_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
@@ -969,16 +977,18 @@ int ekm_wsnet_prepare(ekm_wsnet* n, const double* rhs, const double* prev, const
   hipLaunchKernelGGL(k_scale, dim3(nb), dim3(256), 0, st, rhs, plane, scale);
   EKM_CHECK(hipGetLastError());
   const long N1 = (long)nb * n->h[1] * n->w[1];
-  hipLaunchKernelGGL(k_prep, dim3(nblk(N1, 256)), dim3(256), 0, st, rhs, prev, kx2, scale, nb, n->h[0], n->w[0],
-                     n->sigma, n->kx_max, x1);
+  hipLaunchKernelGGL(k_prep, dim3(nblk(N1, 256)), dim3(256), 0, st, rhs, prev, ratio, kx2, scale, nb, n->h[0],
+                     n->w[0], n->sigma, n->kx_max, x1);
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }
 
-int ekm_wsnet_target(ekm_wsnet* n, const double* phi, const double* scale, int nb, float* t0, void* stream) {
+int ekm_wsnet_target(ekm_wsnet* n, const double* phi, const double* prev, const double* ratio, const double* scale,
+                     int nb, float* t0, void* stream) {
+  if (nb < 1) return EKM_ERR_ARG;
   const long plane = (long)n->h[0] * n->w[0];
-  hipLaunchKernelGGL(k_target, dim3(nblk((size_t)nb * plane, 256)), dim3(256), 0, (hipStream_t)stream, phi, scale,
-                     nb, plane, n->sigma, t0);
+  hipLaunchKernelGGL(k_target, dim3(nblk((size_t)nb * plane, 256)), dim3(256), 0, (hipStream_t)stream, phi, prev,
+                     ratio, scale, nb, plane, n->sigma, t0);
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }
@@ -1041,9 +1051,10 @@ int ekm_wsnet_train_step(ekm_wsnet* n, const float* x1, const float* t0, int nb,
   return EKM_OK;
 }
 
-int ekm_wsnet_guess(ekm_wsnet* n, const double* rhs, const double* prev, const double* kx2, int nb, double* out,
+int ekm_wsnet_guess(ekm_wsnet* n, const double* rhs, const double* prev, const double* ratio, const double* kx2,
+                    int nb, double* out,
                     void* stream) {
-  return wsnet_guess(n, rhs, prev, kx2, nb, out, (hipStream_t)stream);
+  return wsnet_guess(n, rhs, prev, ratio, kx2, nb, out, (hipStream_t)stream);
 }
 
 }  // extern "C"

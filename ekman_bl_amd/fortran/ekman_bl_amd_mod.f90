@@ -400,16 +400,17 @@ module ekman_bl_amd
        real(c_double), intent(out) :: sigma, kx_max
      end function ekm_wsnet_get_norm
 
-     integer(c_int) function ekm_wsnet_prepare(net, rhs, prev, kx2, nb, x1, scale, stream) &
+     integer(c_int) function ekm_wsnet_prepare(net, rhs, prev, ratio, kx2, nb, x1, scale, stream) &
          bind(c, name="ekm_wsnet_prepare")
        import :: c_ptr, c_int
-       type(c_ptr), value :: net, rhs, prev, kx2, x1, scale, stream
+       type(c_ptr), value :: net, rhs, prev, ratio, kx2, x1, scale, stream
        integer(c_int), value :: nb
      end function ekm_wsnet_prepare
 
-     integer(c_int) function ekm_wsnet_target(net, phi, scale, nb, t0, stream) bind(c, name="ekm_wsnet_target")
+     integer(c_int) function ekm_wsnet_target(net, phi, prev, ratio, scale, nb, t0, stream) &
+         bind(c, name="ekm_wsnet_target")
        import :: c_ptr, c_int
-       type(c_ptr), value :: net, phi, scale, t0, stream
+       type(c_ptr), value :: net, phi, prev, ratio, scale, t0, stream
        integer(c_int), value :: nb
      end function ekm_wsnet_target
 
@@ -435,9 +436,10 @@ module ekman_bl_amd
        real(c_float), intent(out) :: loss
      end function ekm_wsnet_train_step
 
-     integer(c_int) function ekm_wsnet_guess(net, rhs, prev, kx2, nb, out, stream) bind(c, name="ekm_wsnet_guess")
+     integer(c_int) function ekm_wsnet_guess(net, rhs, prev, ratio, kx2, nb, out, stream) &
+         bind(c, name="ekm_wsnet_guess")
        import :: c_ptr, c_int
-       type(c_ptr), value :: net, rhs, prev, kx2, out, stream
+       type(c_ptr), value :: net, rhs, prev, ratio, kx2, out, stream
        integer(c_int), value :: nb
      end function ekm_wsnet_guess
 
@@ -446,10 +448,10 @@ module ekman_bl_amd
        type(c_ptr), value :: s, net
      end function ekm_set_warm_start_net
 
-     integer(c_int) function ekm_solve_planes(s, rhs, phi, kx2, nsolve) bind(c, name="ekm_solve_planes")
+     integer(c_int) function ekm_solve_planes(s, rhs, phi, kx2, ratio, nsolve) bind(c, name="ekm_solve_planes")
        import :: c_ptr, c_int
        type(c_ptr), value :: s
-       type(c_ptr), intent(out) :: rhs, phi, kx2
+       type(c_ptr), intent(out) :: rhs, phi, kx2, ratio
        integer(c_int), intent(out) :: nsolve
      end function ekm_solve_planes
 

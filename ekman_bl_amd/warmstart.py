@@ -8,7 +8,9 @@ The reference has no counterpart: MUDPACK starts every solve from zero
 proj_guess = 2 the consistent projection starts each real solve from the
 network's guess and iterates to the same tolerance as from zero, so the
 multigrid stays the judge of the guess; the V-cycles saved are measured by
-comparing cycle counts (`eval_cycles`).
+comparing cycle counts (`eval_cycles`).  The network predicts a correction to
+the previous substep's solution scaled by the ratio of the two substeps'
+H_BAR (the start of proj_guess 1): a network that outputs zero is that start.
 
 All bulk data stays in device memory (`DeviceArray`, allocated through the
 library); only parameters and scalars cross to the host.
@@ -40,14 +42,14 @@ def _lib():
         L.ekm_wsnet_get_params.argtypes = [_vp, _F]
         L.ekm_wsnet_set_norm.argtypes = [_vp, ctypes.c_double, ctypes.c_double]
         L.ekm_wsnet_get_norm.argtypes = [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
-        L.ekm_wsnet_prepare.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]
-        L.ekm_wsnet_target.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, _vp]
+        L.ekm_wsnet_prepare.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]
+        L.ekm_wsnet_target.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]
         L.ekm_wsnet_forward.argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp]
         L.ekm_wsnet_gradient.argtypes = [_vp, _vp, _vp, ctypes.c_int, _F, _F]
         L.ekm_wsnet_train_step.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_float, _F, _vp]
-        L.ekm_wsnet_guess.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]
+        L.ekm_wsnet_guess.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]
         L.ekm_set_warm_start_net.argtypes = [_vp, _vp]
-        L.ekm_solve_planes.argtypes = [_vp] + [ctypes.POINTER(_vp)] * 3 + [ctypes.POINTER(ctypes.c_int)]
+        L.ekm_solve_planes.argtypes = [_vp] + [ctypes.POINTER(_vp)] * 4 + [ctypes.POINTER(ctypes.c_int)]
         L.ekm_dev_alloc.argtypes = [ctypes.POINTER(_vp), ctypes.c_size_t]
         L.ekm_dev_free.argtypes = [_vp]
         L.ekm_copy_dev.argtypes = [_vp, _vp, ctypes.c_size_t, _vp]
@@ -132,12 +134,14 @@ class WarmStartNet:
         _lib().ekm_wsnet_get_norm(self.h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
 
-    def prepare(self, rhs, prev, kx2, n, x1: DeviceArray, x1_off: int, scale: DeviceArray, sc_off: int):
-        _check(_lib().ekm_wsnet_prepare(self.h, rhs, prev, kx2, n, x1.at(x1_off), scale.at(sc_off), None),
+    def prepare(self, rhs, prev, kx2, n, x1: DeviceArray, x1_off: int, scale: DeviceArray, sc_off: int,
+                ratio=None):
+        _check(_lib().ekm_wsnet_prepare(self.h, rhs, prev, ratio, kx2, n, x1.at(x1_off), scale.at(sc_off), None),
                "ekm_wsnet_prepare")
 
-    def target(self, phi, scale: DeviceArray, sc_off: int, n, t0: DeviceArray, t_off: int):
-        _check(_lib().ekm_wsnet_target(self.h, phi, scale.at(sc_off), n, t0.at(t_off), None), "ekm_wsnet_target")
+    def target(self, phi, scale: DeviceArray, sc_off: int, n, t0: DeviceArray, t_off: int, prev=None, ratio=None):
+        _check(_lib().ekm_wsnet_target(self.h, phi, prev, ratio, scale.at(sc_off), n, t0.at(t_off), None),
+               "ekm_wsnet_target")
 
     def forward(self, x1: DeviceArray, n: int, x1_off: int = 0) -> np.ndarray:
         y = DeviceArray(n * self.h1 * self.w1)
@@ -186,18 +190,20 @@ def detach(s: S.DuctSolver) -> None:
 
 
 def solve_planes(s: S.DuctSolver):
-    """(rhs, phi, kx2) device addresses and the solve count of the last projection."""
-    r, p, k = _vp(), _vp(), _vp()
+    """(rhs, phi, kx2, H_BAR ratio) device addresses and the solve count of
+    the last projection."""
+    r, p, k, q = _vp(), _vp(), _vp(), _vp()
     n = ctypes.c_int()
-    _check(_lib().ekm_solve_planes(s.h, ctypes.byref(r), ctypes.byref(p), ctypes.byref(k), ctypes.byref(n)),
-           "ekm_solve_planes")
-    return r, p, k, n.value
+    _check(_lib().ekm_solve_planes(s.h, ctypes.byref(r), ctypes.byref(p), ctypes.byref(k), ctypes.byref(q),
+                                   ctypes.byref(n)), "ekm_solve_planes")
+    return r, p, k, q, n.value
 
 
 class PairSet:
     """Training pairs on the device: network inputs x1 (from each projection's
-    RHS_TMP and the previous projection's solution of the same solve) and
-    targets t0 (the converged solution, scaled)."""
+    RHS_TMP and the previous projection's solution of the same solve, scaled
+    by the substeps' H_BAR ratio) and targets t0 (the converged solution minus
+    that scaled previous solution, normalised)."""
 
     def __init__(self, net: WarmStartNet, capacity: int):
         self.net, self.cap, self.n = net, capacity, 0
@@ -213,7 +219,7 @@ class PairSet:
         """Add the pairs of the projection the solver just did (after the
         first projection of a run, which has no previous solution)."""
         s.sync()
-        rhs, phi, kx2, ns = solve_planes(s)
+        rhs, phi, kx2, ratio, ns = solve_planes(s)
         plane = self.net.nz2 * self.net.ny2
         if self._prev is None:
             self._prev = DeviceArray(ns * plane, np.float64)
@@ -222,8 +228,8 @@ class PairSet:
         take = min(ns, self.cap - self.n)
         if take > 0:
             self.net.prepare(rhs, self._prev.ptr, kx2, take, self.x1, self.n * self.net.h1 * self.net.w1 * 4,
-                             self.scale, self.n)
-            self.net.target(phi, self.scale, self.n, take, self.t0, self.n * plane)
+                             self.scale, self.n, ratio=ratio)
+            self.net.target(phi, self.scale, self.n, take, self.t0, self.n * plane, prev=self._prev.ptr, ratio=ratio)
             self.n += take
         _check(_lib().ekm_copy_dev(self._prev.ptr, phi, ns * plane, None), "ekm_copy_dev")
         return take
@@ -252,8 +258,9 @@ def run_and_dump(p, xp, yp, ic, pairs: PairSet, steps: int, warmup: int = 1) -> 
 
 
 def estimate_sigma(pairs: PairSet, idx: np.ndarray, net: WarmStartNet) -> float:
-    """sigma such that the scaled targets are of order one: the median over
-    pairs of max|phi| / max|rhs| (the targets were formed with sigma = 1)."""
+    """sigma such that the normalised targets are of order one: the median
+    over pairs of max|phi - ratio prev| / max|rhs| (the targets were formed
+    with sigma = 1)."""
     plane = net.nz2 * net.ny2
     vals = []
     for i in idx[:: max(1, len(idx) // 256)]:

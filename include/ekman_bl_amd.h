@@ -328,16 +328,21 @@ int ekm_wsnet_destroy(ekm_wsnet* n);
 long ekm_wsnet_nparams(const ekm_wsnet* n);
 int ekm_wsnet_set_params(ekm_wsnet* n, const float* host);
 int ekm_wsnet_get_params(ekm_wsnet* n, float* host);
-/* input normalisation: prev is divided by s*sigma (s = max|rhs| per plane),
- * kx by kx_max */
+/* input normalisation: the scaled previous solution is divided by s*sigma
+ * (s = max|rhs| per plane), kx by kx_max.  The network predicts what it adds
+ * to ratio * prev, the previous substep's solution scaled by the ratio of the
+ * two substeps' H_BAR (the start proj_guess 1 uses). */
 int ekm_wsnet_set_norm(ekm_wsnet* n, double sigma, double kx_max);
 int ekm_wsnet_get_norm(const ekm_wsnet* n, double* sigma, double* kx_max);
 /* network input x1 (nb x (NY+3)/2 x (NZ+3)/2 x 4 floats) and the scales s
- * (nb doubles) of nb planes; prev may be NULL (zero) */
-int ekm_wsnet_prepare(ekm_wsnet* n, const double* rhs, const double* prev, const double* kx2, int nb, float* x1,
-                      double* scale, void* stream);
-/* training target phi / (s sigma) at the plane's resolution (nb planes of floats) */
-int ekm_wsnet_target(ekm_wsnet* n, const double* phi, const double* scale, int nb, float* t0, void* stream);
+ * (nb doubles) of nb planes; prev may be NULL (zero); ratio: device pointer
+ * to one double scaling prev, or NULL (1) */
+int ekm_wsnet_prepare(ekm_wsnet* n, const double* rhs, const double* prev, const double* ratio, const double* kx2,
+                      int nb, float* x1, double* scale, void* stream);
+/* training target (phi - ratio * prev) / (s sigma) at the plane's resolution
+ * (nb planes of floats) */
+int ekm_wsnet_target(ekm_wsnet* n, const double* phi, const double* prev, const double* ratio, const double* scale,
+                     int nb, float* t0, void* stream);
 /* prediction at level 1 (nb x (NY+3)/2 x (NZ+3)/2 floats; y1 may be NULL) */
 int ekm_wsnet_forward(ekm_wsnet* n, const float* x1, int nb, float* y1, void* stream);
 /* gradient of the mean relative loss  sum (up(y) - t)^2 / sum t^2  with
@@ -345,9 +350,10 @@ int ekm_wsnet_forward(ekm_wsnet* n, const float* x1, int nb, float* y1, void* st
 int ekm_wsnet_gradient(ekm_wsnet* n, const float* x1, const float* t0, int nb, float* grad_host, float* loss);
 /* one Adam step (beta 0.9, 0.999) on that loss; returns the batch loss */
 int ekm_wsnet_train_step(ekm_wsnet* n, const float* x1, const float* t0, int nb, float lr, float* loss, void* stream);
-/* the guess of nb solves, s sigma up(y) as double planes; out may alias prev */
-int ekm_wsnet_guess(ekm_wsnet* n, const double* rhs, const double* prev, const double* kx2, int nb, double* out,
-                    void* stream);
+/* the guess of nb solves, ratio * prev + s sigma up(y) as double planes; out
+ * may alias prev */
+int ekm_wsnet_guess(ekm_wsnet* n, const double* rhs, const double* prev, const double* ratio, const double* kx2,
+                    int nb, double* out, void* stream);
 /* Attach a network to a solver (NULL detaches): with proj_mode 1 or 2 and
  * proj_guess = 2, each real solve after the first projection starts from the
  * network's guess.  The network must match the plane and hold bmax >= the
@@ -355,9 +361,11 @@ int ekm_wsnet_guess(ekm_wsnet* n, const double* rhs, const double* prev, const d
 int ekm_set_warm_start_net(ekm_solver* s, ekm_wsnet* net);
 /* Device pointers of the last projection's right-hand sides (RHS_TMP) and
  * solutions (nsolve planes each, (NZ+2)*(NY+2) doubles, K fastest; solve 2m
- * and 2m+1 belong to kx mode m) and of every solve's kx^2: what the
+ * and 2m+1 belong to kx mode m), of every solve's kx^2 and of the H_BAR ratio
+ * of that projection's substep to the one before (one double): what the
  * warm-start training data is dumped from. */
-int ekm_solve_planes(ekm_solver* s, const double** rhs, const double** phi, const double** kx2, int* nsolve);
+int ekm_solve_planes(ekm_solver* s, const double** rhs, const double** phi, const double** kx2,
+                     const double** ratio, int* nsolve);
 
 /* ---- measurement and utility ------------------------------------------- */
 /* Time every multigrid launch of the solver with HIP events on its stream

@@ -64,10 +64,18 @@
  *                               stratified type-6 branch (whole-plane CF2
  *                               relaxation, CU2X(0,:,:) zeroing): parity
  *                               unpinned beyond the reference text
- *   ora_rk_duct / ora_time_step the composed operation order of RK_DUCT (lagged
- *                               rows, corner updates, dt timing) has no golden
- *                               vectors: parity unpinned beyond the pieces above
- *                               (the GPU path is checked against it)
+ *   ora_rk_duct / ora_time_step tests/test_oracle_timestep.py: one step of a
+ *                               uniform inviscid flow is the RK3 stability
+ *                               polynomial of i f dt (RK weights, Coriolis);
+ *                               Crank-Nicolson decay of discrete Dirichlet
+ *                               modes; the laminar Ekman spiral steady to
+ *                               second order; the CFL closed form; second-order
+ *                               scalar transport on a sheared grid; cofcr at
+ *                               second order; lagged periodic rows.  The ghost
+ *                               and corner updates and the dt-update timing:
+ *                               parity unpinned beyond the reference text (the
+ *                               GPU path is checked against them)
+ *   type-3 row assembly         parity unpinned beyond the reference text
  */
 #ifndef EKMAN_ORACLE_H
 #define EKMAN_ORACLE_H

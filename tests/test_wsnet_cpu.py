@@ -109,3 +109,22 @@ def test_prepare_inputs():
     assert abs(x[0, 1, 2, 0] - want) < 1e-14
     assert abs(x[1, 0, 0, 2] - 0.5) < 1e-15 and x[1, -1, 0, 3] == 1.0
     assert np.all(x[2] == 0.0)
+    # the previous solution enters scaled by the H_BAR ratio
+    x2, _ = W.prepare(rhs, prev, kx, 2.0, 10.0, ratio=0.25)
+    np.testing.assert_allclose(x2[..., 1], 0.25 * x[..., 1], rtol=1e-15)
+    assert np.array_equal(x2[..., 0], x[..., 0])
+
+
+def test_target_and_guess_are_the_scaled_previous_solution_plus_the_output():
+    """The target is what the network adds to ratio * prev (normalised), and
+    a network whose output is zero guesses ratio * prev exactly."""
+    rng = np.random.default_rng(9)
+    rhs = rng.standard_normal((2, 33, 33))
+    prev = rng.standard_normal((2, 33, 33))
+    phi = 0.4 * prev + 0.01 * rng.standard_normal((2, 33, 33))
+    s = np.abs(rhs).max(axis=(1, 2))
+    t = W.target(phi, prev, 0.4, s, 3.0)
+    np.testing.assert_allclose(t * (s * 3.0)[:, None, None] + 0.4 * prev, phi, rtol=1e-12)
+    theta = np.zeros(W.nparams())
+    g = W.guess(theta, rhs, prev, np.array([0.5, 1.0]), 2.0, 3.0, ratio=0.4)
+    np.testing.assert_array_equal(g, 0.4 * prev)
