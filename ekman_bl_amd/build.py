@@ -19,7 +19,7 @@ FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
 FORTRAN = os.path.join(HERE, "fortran")
 DRIVER = os.path.join(FORTRAN, "duct_driver")
 
-SOURCES = ["ekm_fft.hip", "ekm_kernels.hip", "ekm_les.hip", "ekm_mg.hip", "ekm_mgt.hip", "ekm_wsnet.hip", "ekm_solver.hip",
+SOURCES = ["ekm_fft.hip", "ekm_kernels.hip", "ekm_les.hip", "ekm_mg.hip", "ekm_mgt.hip", "ekm_wsnet.hip", "ekm_noalloc.hip", "ekm_solver.hip",
            "ekm_host.cpp"]
 
 # -ffp-contract=off keeps every product and sum separately rounded, as the

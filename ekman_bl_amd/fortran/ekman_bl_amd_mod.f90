@@ -468,6 +468,14 @@ module ekman_bl_amd
        integer(c_int), value :: nx, npencil
      end function ekm_fft_x_to_fourier_dev
 
+     ! the forward transform without device allocation (NX <= 1024)
+     integer(c_int) function ekm_fft_x_to_fourier_dev_noalloc(varp, nx, npencil, stream) &
+         bind(c, name="ekm_fft_x_to_fourier_dev_noalloc")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: varp, stream
+       integer(c_int), value :: nx, npencil
+     end function ekm_fft_x_to_fourier_dev_noalloc
+
      integer(c_int) function ekm_fft_x_to_physical_dev(varp, nx, npencil, stream) &
          bind(c, name="ekm_fft_x_to_physical_dev")
        import :: c_ptr, c_int

@@ -300,6 +300,15 @@ int ekm_multigrid_mudpack_ex(ekm_solver* s, int nsolve, const double* kx2, const
 /* ---- device-pointer variants (stream = hipStream_t or NULL) -------------- */
 int ekm_fft_x_to_fourier_dev(double* varp, int nx, int npencil, void* stream);
 int ekm_fft_x_to_physical_dev(double* varp, int nx, int npencil, void* stream);
+/* ekm_fft_x_to_fourier_dev without any device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, hosts that forbid hipMalloc).  The
+ * plan's twiddle tables live in a file-scope device array of 1538 doubles
+ * (12 KB on each device, held until the process ends), so NX <= 1024; a
+ * larger NX returns EKM_ERR_ARG with a message (ekm_last_error).  Every call
+ * on a device shares that array: calls that overlap on one device race.
+ * Each call rewrites the array and waits for its transform to end. */
+int ekm_fft_x_to_fourier_dev_noalloc(double* varp, int nx, int npencil, void* stream);
 int ekm_thomas_real_dev(const double* a, const double* b, const double* c, double* g,
                         int ny, int nx, void* stream);
 /* THOMAS_REAL_SP(A,B,C,G,D,K,NY,NX) (src/boundary.F90:4-47) on device arrays;
