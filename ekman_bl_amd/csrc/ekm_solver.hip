@@ -167,7 +167,7 @@ static int check_params(const ekm_params& p) {
   if (p.proj_mode < 0 || p.proj_mode > 2) return EKM_ERR_UNSUPPORTED;
   if (p.u_bc[2][EKM_YMIN] == 3) {
     // LES_WALL_MODEL: type 3 only (type 1 reads a U_STAR it never sets,
-    // type 2 a CU1X_WALL_MODEL it never allocates, type 4 the scalar's new
+    // type 2 a CU1X_WALL_MODEL it allocates and never assigns, type 4 the scalar's new
     // value, src/les_chan.F90:1914-2144); its z means read tmp_u(1:NZ),
     // which it fills over ZSTART..ZEND only
     if (p.wall_model_type != 3) return EKM_ERR_UNSUPPORTED;

@@ -1783,6 +1783,19 @@ static void bc_z(const ora* o, int comp, int J, sysb* s, const double* u, size_t
   else { row(s, NZ + 1, 0., 1., 0.); rhs_line(s, NZ + 1, u, off, stride, 1); }
 }
 
+/* the boundary rows of one line as the ADI sweeps assemble them, for tests:
+ * dir 0 the y line at K = line, dir 1 the z line at J = line; u the
+ * physical field the lagged rows read; a, b, c, g of len x NX2 entries
+ * (row-major by line row), prefilled by the caller, only the boundary rows
+ * written */
+int ora_bc_rows(ora* o, int comp, int dir, int line, const double* u, double* a, double* b, double* c, double* g) {
+  if (comp < 0 || comp > 3 || dir < 0 || dir > 1) return EKM_ERR_ARG;
+  sysb s = {a, b, c, g, (size_t)o->NX2};
+  if (dir == 0) bc_y(o, comp, line, &s, u, IDX(o, 0, line, 0), (size_t)o->NX2 * o->NZ2);
+  else bc_z(o, comp, line, &s, u, IDX(o, 0, 0, line), (size_t)o->NX2);
+  return EKM_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 /* ADI for one field (theta: :1334-1625; U1: :1640-1835; U2: :1877-2110;     */
 /* U3: :2137-2391)                                                           */

@@ -75,7 +75,11 @@
  *                               and corner updates and the dt-update timing:
  *                               parity unpinned beyond the reference text (the
  *                               GPU path is checked against them)
- *   type-3 row assembly         parity unpinned beyond the reference text
+ *   boundary rows (ora_bc_rows) the reference's own APPLY_BC_* (src/boundary.F90)
+ *                               compiled unchanged into oracle/_ref with the
+ *                               harness oracle/ref_bc_shim.f90: every supported
+ *                               type on every side, bit for bit
+ *                               (tests/test_ref_boundary_cpu.py)
  */
 #ifndef EKMAN_ORACLE_H
 #define EKMAN_ORACLE_H
@@ -121,6 +125,10 @@ int ora_get_physical(ora* o, int f, double* out);
  * [2][NZ+2][NX+2] (U1's, U3's) into out and U_1, W_1, U_STAR, TAO_WALL,
  * TAO_WALL_1 into info[5]; EKM_ERR_ARG without type-3 rows */
 int ora_wall_model(ora* o, const double* u1, const double* u3, int first, double* out, double* info);
+/* the boundary rows one ADI line gets (dir 0: y line at K = line, 1: z line
+ * at J = line) from the physical field u, into caller-filled a, b, c, g of
+ * (rows x (NX+2)) doubles; only the boundary rows are written */
+int ora_bc_rows(ora* o, int comp, int dir, int line, const double* u, double* a, double* b, double* c, double* g);
 /* one field's ADI (comp 0..2 velocity, 3 scalar) on physical U, R in place,
  * with the subgrid viscosity nut (physical P field) unless NULL */
 int ora_adi_field(ora* o, int comp, double* U, double* R, double t1, const double* nut, double t2);

@@ -58,6 +58,7 @@ def lib():
         L.ora_omega_x.argtypes = [ctypes.c_void_p, _D]
         L.ora_get_physical.argtypes = [ctypes.c_void_p, ctypes.c_int, _D]
         L.ora_wall_model.argtypes = [ctypes.c_void_p, _D, _D, ctypes.c_int, _D, _D]
+        L.ora_bc_rows.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, _D]
         L.ora_xstats.argtypes = [ctypes.c_void_p, _D]
         L.ora_adi_field.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.c_double, _D, ctypes.c_double]
         L.ora_set_initial_physical.argtypes = [ctypes.c_void_p, _D, _D, _D, _D]
@@ -195,6 +196,18 @@ class Oracle:
         if st != 0:
             raise ValueError("the oracle has no wall-model rows")
         return out, info
+
+    def bc_rows(self, comp: int, direction: int, line: int, u: np.ndarray, sentinel: float = np.nan):
+        """The boundary rows of one ADI line (ora_bc_rows): (a, b, c, g), each
+        (rows, NX+2), the entries that are not boundary rows at `sentinel`."""
+        p = self.p
+        n = (p.ny + 2) if direction == 0 else (p.nz + 2)
+        out = [np.full((n, p.nx + 2), sentinel) for _ in range(4)]
+        uu = np.ascontiguousarray(u, dtype=np.float64)
+        st = lib().ora_bc_rows(self.h, comp, direction, line, _p(uu), *(_p(a) for a in out))
+        if st != 0:
+            raise ValueError("ora_bc_rows failed")
+        return out
 
     def mean_grads(self):
         """SAVE_STATS_CURVI's mean gradients: (dict name -> (NY+2, NZ+2), THBULK)."""

@@ -69,7 +69,7 @@ def test_wall_model_closed_forms(oracle_lib):
 
 def test_wall_model_rejects_other_types(oracle_lib):
     """Types 1, 2 and 4 are not built (type 1 reads a U_STAR it never sets,
-    type 2 a CU1X_WALL_MODEL it never allocates, type 4 the scalar's new
+    type 2 a CU1X_WALL_MODEL it allocates and never assigns, type 4 the scalar's new
     value); the oracle refuses them."""
     O = oracle_lib
     p = _wall_params()
