@@ -166,6 +166,7 @@ int mg_solve_ex(MgPlan& mg, const Dims& d, int nsolve, int iguess, hipStream_t s
 struct TightPlan {
   int fp32 = 0, cm = 0, nsolve_max = 0;
   bool cross = false;                 // operator has cross-derivative terms (9-point)
+  bool per = false;                   // periodic z ends
   int lds_bytes = 0;                  // dynamic LDS of the solve kernel (coarse levels)
   long lds_budget = 0;                // cap on it in bytes (0: what the device allows)
   long ws_elems = 0, ws64_elems = 0;  // per solve

@@ -37,9 +37,16 @@
 namespace {
 
 constexpr int kMaxLevT = 16;
-// threads per workgroup of the solve kernel: 16 wavefronts where the line
-// registers fit (FP32, lines of at most 5 stripes, no cross terms), else 8
-constexpr int tight_block(bool fp32, int cm, bool cross) { return (fp32 && cm <= 5 && !cross) ? 1024 : 512; }
+// threads per workgroup of the solve kernel: 16 wavefronts for the FP32
+// cycles without cross terms (their registers fit the 128 a 1024-thread
+// workgroup allows; long z lines stream their loads), else 8.  On the
+// 1024x385x769 case the 16-wavefront form takes the solve from 110 to 91 ms
+// per launch (the workgroup is bound by its own memory round trips, which
+// twice the wavefronts overlap)
+constexpr int tight_block(bool fp32, int cm, bool cross) {
+  (void)cm;
+  return (fp32 && !cross) ? 1024 : 512;
+}
 constexpr int kDirectMax = 128;  // the coarse level solved directly has at most this many points
 constexpr int kMaxProg = 4 * kMaxLevT + 4;
 
@@ -50,7 +57,8 @@ struct TLev {
   int t2_off;                 // second line-solve scratch (the residual array is the first)
   int lds;                    // 1: the four arrays above live in the workgroup's LDS
   int f_off;                  // per-mode offset: z factors [3][n], y factors [3][n] (column-parity layout), c9 [n],
-                              // y-chunk products A [n] and Be [n] (column-parity layout)
+                              // y-chunk products A [n] and Be [n] (column-parity layout); periodic ends
+                              // add the z lines' wrap vectors W [2][n] and their 2x2 matrices [ny][4]
   const T* cf;                // c1..c8 planes [8][n]; rows of specified points are 0 (c9 = 1)
   const T* cfy;               // the same planes in the column-parity layout of the y-line smoother
   const unsigned char* spec;  // specified (Dirichlet) points
@@ -182,6 +190,56 @@ __device__ inline Aff<T> wave_scan_down(Aff<T> v) {
 // in both forms.
 constexpr int kZRegStripes = 5;
 
+// Periodic z ends (MUDPACK nxa = nxb = 0): every column of a level is an
+// unknown with its own row, and the virtual neighbours are e(-1) = e(nx-2)
+// and e(nx) = e(1) (MUDPACK's periodic points, src/mud2cr.f:1693-1795).
+// They are held in the ring columns of the correction, so every writer of
+// columns 1 and nx-2 also writes the ring copy; a value at index q of
+// column i goes to q + ring_off_hi(i) / q + ring_off_lo(i) as well.
+__device__ inline void ring_store_d(double* a, int q, int i, int nx, double v) {
+  if (i == 1) a[q + nx - 1] = v;
+  if (i == nx - 2) a[q - (nx - 1)] = v;
+}
+template <typename T>
+__device__ inline void ring_store(T* a, int q, int i, int nx, T v) {
+  if (i == 1) a[q + nx - 1] = v;
+  if (i == nx - 2) a[q - (nx - 1)] = v;
+}
+
+// A z line of a periodic level is the chain 0..nx-1 (the precomputed Thomas
+// factors) plus the couplings of point 0 to nx-2 and of point nx-1 to 1.
+// With T the chain, U = [c5(0) e_0, c1(nx-1) e_{nx-1}] and V = [e_{nx-2}, e_1],
+// Woodbury gives x = y - W Minv (y_{nx-2}, y_1) for y = T^-1 s, W = T^-1 U and
+// Minv = (I + V^T W)^-1, both W and Minv precomputed per row and mode.  y is
+// the chain solution the wavefront holds in x[] (stripe c, lane l: point
+// 64 c + l); this writes the line with its ring copies.
+template <typename T, int CM>
+__device__ inline void zline_periodic(const TLev<T>& L, T* __restrict__ e, const T* __restrict__ fz, const T* x,
+                                      int j, int lane) {
+  const int nx = L.nx, n = L.n, ld = nx + 2;
+  const T* __restrict__ w1 = fz + 9 * n;
+  const T* __restrict__ w2 = w1 + n;
+  const T* mi = w2 + n + 4 * j;
+  const int im = nx - 2, cs = im >> 6, ls = im & 63;
+  T xm = (T)0;
+#pragma unroll
+  for (int c = 0; c < CM; ++c)
+    if (c == cs) xm = x[c];
+  const T y1 = rdlane(x[0], 1), ym = rdlane(xm, ls);
+  const T g0 = mi[0] * ym + mi[1] * y1, g1 = mi[2] * ym + mi[3] * y1;
+#pragma unroll
+  for (int c = 0; c < CM; ++c) {
+    const int i = 64 * c + lane;
+    if (i < nx) {
+      const int p = i + nx * j;
+      const T v = x[c] - (g0 * w1[p] + g1 * w2[p]);
+      const int q = (i + 1) + ld * (j + 1);
+      e[q] = v;
+      ring_store(e, q, i, nx, v);
+    }
+  }
+}
+
 template <typename T, bool X, bool ZERO>
 __device__ inline void zline_point(const T* __restrict__ e, const T* __restrict__ r, const T* __restrict__ cf,
                                    const T* fa, const T* f1, int nx, int n, int ld, int i, int j, T& a, T& b) {
@@ -202,7 +260,7 @@ __device__ inline void zline_point(const T* __restrict__ e, const T* __restrict_
   }
 }
 
-template <typename T, int CM, bool X, bool ZERO = false>
+template <typename T, int CM, bool X, bool PER, bool ZERO = false>
 __device__ void zlines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ fac, int par) {
   T* W = wbase(L, Wp);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -243,9 +301,11 @@ __device__ void zlines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
         const Aff<T> inc = wave_scan_down(Aff<T>{-ck, dk[c]});
         const T x = inc.a * xo + inc.b;
         const int i = 64 * c + lane;
-        if (i < nx) e[(i + 1) + ld * (j + 1)] = x;
+        if (!PER && i < nx) e[(i + 1) + ld * (j + 1)] = x;
+        dk[c] = x;
         xo = rdlane(x, 0);
       }
+      if constexpr (PER) zline_periodic<T, CM>(L, e, f1, dk, j, lane);
     }
     return;
   }
@@ -285,9 +345,11 @@ __device__ void zlines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
       const Aff<T> inc = wave_scan_down(Aff<T>{-ck[c], dk[c]});
       const T x = inc.a * xo + inc.b;
       const int i = 64 * c + lane;
-      if (i < nx) e[(i + 1) + ld * (j + 1)] = x;
+      if (!PER && i < nx) e[(i + 1) + ld * (j + 1)] = x;
+      dk[c] = x;
       xo = rdlane(x, 0);
     }
+    if constexpr (PER) zline_periodic<T, CM>(L, e, f1, dk, j, lane);
   }
 }
 
@@ -330,7 +392,7 @@ constexpr int kYUnroll = 3;
 constexpr int kYApply = 8;  // rows per batch of the last pass
 constexpr int kYBack = 6;   // rows per batch of the backward pass
 
-template <typename T, bool X>
+template <typename T, bool X, bool PER>
 __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ fac, int par, T* lds) {
   T* W = wbase(L, Wp);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -476,7 +538,12 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
       }
 #pragma unroll
       for (int u = 0; u < kYApply; ++u)
-        if (jb + u < j1) e[(i + 1) + ld * (jb + u + 1)] = a_[u] + b_[u] * xo;
+        if (jb + u < j1) {
+          const int q = (i + 1) + ld * (jb + u + 1);
+          const T v = a_[u] + b_[u] * xo;
+          e[q] = v;
+          if constexpr (PER) ring_store(e, q, i, nx, v);
+        }
     }
   }
   __syncthreads();
@@ -487,18 +554,19 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
 // down leg: nothing has been written since the level was restricted to) is
 // not read at all, and the prolongation of the up leg need only have updated
 // the rows of parity 1.  The interior of the level's correction is fully
-// rewritten by the two z sweeps; its ring stays zero.
-template <typename T, int CM, bool X>
+// rewritten by the two z sweeps; its ring stays zero (with periodic ends its
+// ring columns are rewritten with columns 1 and nx-2).
+template <typename T, int CM, bool X, bool PER>
 __device__ void smooth(const TDesc<T>& g, const TLev<T>& L, const Ws<T>& W, const T* fac, T* lds, bool down) {
   if (down)
-    zlines<T, CM, X, true>(L, W, fac, 0);
+    zlines<T, CM, X, PER, true>(L, W, fac, 0);
   else
-    zlines<T, CM, X>(L, W, fac, 0);
+    zlines<T, CM, X, PER>(L, W, fac, 0);
   __syncthreads();
-  zlines<T, CM, X>(L, W, fac, 1);
+  zlines<T, CM, X, PER>(L, W, fac, 1);
   __syncthreads();
-  ylines<T, X>(L, W, fac, 0, lds);
-  ylines<T, X>(L, W, fac, 1, lds);
+  ylines<T, X, PER>(L, W, fac, 0, lds);
+  ylines<T, X, PER>(L, W, fac, 1, lds);
 }
 
 // res = r - A e on level L (specified points have the row e = r = 0); each
@@ -541,9 +609,10 @@ __device__ void residual(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict_
 }
 
 // full weighting of the residual of F onto the right-hand side of C (zero
-// outside the grid; a direction that is not coarsened keeps its index).  C's
+// outside the grid, or the periodic virtual columns; a direction that is not
+// coarsened keeps its index).  C's
 // correction is not cleared: the down leg's first sweep treats it as zero.
-template <typename T>
+template <typename T, bool PER>
 __device__ void restrict_fw(const TLev<T>& F, const TLev<T>& C, const Ws<T>& W) {
   const int sx = C.nx < F.nx ? 2 : 1, sy = C.ny < F.ny ? 2 : 1;
   const T* __restrict__ res = wbase(F, W) + F.res_off;
@@ -562,7 +631,10 @@ __device__ void restrict_fw(const TLev<T>& F, const TLev<T>& C, const Ws<T>& W) 
       for (int b = -1; b <= 1; ++b)
 #pragma unroll
         for (int a = -1; a <= 1; ++a) {
-          const int ii = i + a, jj = j + b;
+          int ii = i + a;
+          const int jj = j + b;
+          // periodic ends: the virtual columns -1 -> nx-2 and nx -> 1
+          if constexpr (PER) ii = ii < 0 ? F.nx - 2 : ii >= F.nx ? 1 : ii;
           const bool in = ii >= 0 && ii < F.nx && jj >= 0 && jj < F.ny;
           const T x = res[in ? ii + F.nx * jj : 0];
           acc += in ? (w[a + 1] * w[b + 1]) * x : (T)0;
@@ -580,7 +652,7 @@ __device__ void restrict_fw(const TLev<T>& F, const TLev<T>& C, const Ws<T>& W) 
 // rows of parity 1 only: the smoothing that follows solves the rows of
 // parity 0 first, reading only those of parity 1, and overwrites every row
 // before anything else reads the correction
-template <typename T>
+template <typename T, bool PER>
 __device__ void prolong_add(const TLev<T>& C, const TLev<T>& F, const Ws<T>& W) {
   const int sx = C.nx < F.nx ? 2 : 1, sy = C.ny < F.ny ? 2 : 1;
   const T* __restrict__ ec = wbase(C, W) + C.e_off;
@@ -607,13 +679,19 @@ __device__ void prolong_add(const TLev<T>& C, const TLev<T>& F, const Ws<T>& W) 
     }
 #pragma unroll
     for (int u = 0; u < kEw; ++u)
-      if (p0 + u * nt < nodd && !F.spec[pf[u]]) ef[qf[u]] = v[u];
+      if (p0 + u * nt < nodd && !F.spec[pf[u]]) {
+        ef[qf[u]] = v[u];
+        if constexpr (PER) {
+          const int i = pf[u] % F.nx;
+          ring_store(ef, qf[u], i, F.nx, v[u]);
+        }
+      }
   }
   __syncthreads();
 }
 
 // e = A^-1 r on the coarsest level of the cycle (A^-1 precomputed per mode)
-template <typename T>
+template <typename T, bool PER>
 __device__ void direct_solve(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ ainv) {
   T* W = wbase(L, Wp);
   const int n = L.n, nx = L.nx, ld = nx + 2;
@@ -624,7 +702,9 @@ __device__ void direct_solve(const TLev<T>& L, const Ws<T>& Wp, const T* __restr
     T acc = (T)0;
     for (int q = 0; q < n; ++q) acc += row[q] * r[q];
     const int j = p / nx, i = p - j * nx;
-    e[(i + 1) + ld * (j + 1)] = acc;
+    const int q = (i + 1) + ld * (j + 1);
+    e[q] = acc;
+    if constexpr (PER) ring_store(e, q, i, nx, acc);
   }
   __syncthreads();
 }
@@ -652,7 +732,7 @@ __device__ inline void block_sum4(double* v, double* red) {
 // updates phi, r and the kept direction, and r seeds the next cycle in the
 // cycle's precision.  Returns this thread's max |r|.  Kept out of line so its
 // pointers and scalars are not live across the cycle.
-template <typename T, int NT, bool X>
+template <typename T, int NT, bool X, bool PER>
 __device__ __noinline__ double gcr_step(const TDesc<T>& g, const TLev<T>& F, double* __restrict__ phi, T* W,
                                         const double* __restrict__ c964, double* red, int it) {
   __shared__ double sQQ;  // Qp . Qp of the kept direction
@@ -704,7 +784,9 @@ __device__ __noinline__ double gcr_step(const TDesc<T>& g, const TLev<T>& F, dou
   // one pass over the interior: phi and the kept direction (ring layout),
   // then A z and the residual.  The rings of phi and of the direction are
   // never visited: ef is zero there, so the update would keep them zero
-  // (phi's ring is cleared at the start; the direction's ring is never read).
+  // (phi's ring is cleared at the start; the direction's ring is never read);
+  // with periodic ends phi's ring holds the virtual columns, written with
+  // columns 1 and nx-2.
   for (int p0 = threadIdx.x; p0 < n; p0 += kEwG * NT) {
     double zv[kEwG], zk[kEwG], ph[kEwG], qn[kEwG], qk[kEwG], rk[kEwG];
     int qi[kEwG];
@@ -726,7 +808,9 @@ __device__ __noinline__ double gcr_step(const TDesc<T>& g, const TLev<T>& F, dou
       const int p = p0 + u * NT;
       if (p < n) {
         const double z = zv[u] - (hasq ? beta * zk[u] : 0.0);
-        phi[qi[u]] = ph[u] + alpha * z;
+        const double np = ph[u] + alpha * z;
+        phi[qi[u]] = np;
+        if constexpr (PER) ring_store_d(phi, qi[u], p % nx, nx, np);
         Zp[qi[u]] = z;
         const double q2 = qn[u] - (hasq ? beta * qk[u] : 0.0);
         const double r2 = rk[u] - alpha * q2;
@@ -740,7 +824,7 @@ __device__ __noinline__ double gcr_step(const TDesc<T>& g, const TLev<T>& F, dou
   return rmax;
 }
 
-template <typename T, int CM, int NT, bool X>
+template <typename T, int CM, int NT, bool X, bool PER>
 __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__ gp, T* __restrict__ ws,
                                                      double* __restrict__ ws64, const double* __restrict__ rhs_tmp,
                                                      double* __restrict__ phi_out, const double* __restrict__ kx2v,
@@ -775,7 +859,9 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
   for (int t = threadIdx.x; t < (nx + 2) * (ny + 2); t += blockDim.x) {
     const int jj = t / ld - 1, ii = t - (jj + 1) * ld - 1;
     double v = 0.0;
-    if (guess != 0.0 && ii >= 0 && ii < nx && jj >= 0 && jj < ny && !F.spec[ii + nx * jj]) v = guess * pg[ii + nx * jj];
+    // periodic ends: the ring columns start as the virtual columns
+    const int iv = PER ? (ii < 0 ? nx - 2 : ii >= nx ? 1 : ii) : ii;
+    if (guess != 0.0 && iv >= 0 && iv < nx && jj >= 0 && jj < ny && !F.spec[iv + nx * jj]) v = guess * pg[iv + nx * jj];
     phi[t] = v;
   }
   const int P = (nx + 2) * (ny + 2);
@@ -837,7 +923,7 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
           }
       }
     } else {
-      rmax = gcr_step<T, NT, X>(g, F, phi, W, c964, red, it);
+      rmax = gcr_step<T, NT, X, PER>(g, F, phi, W, c964, red, it);
     }
     // max over the workgroup: wavefront maxima, then one value per wavefront
     for (int o = 32; o > 0; o >>= 1) rmax = fmax(rmax, __shfl_xor(rmax, o, 64));
@@ -857,14 +943,14 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
       const int step = g.prog[k];
       const int l = step >> 2, op = step & 3;
       if (op == 2) {
-        direct_solve<T>(g.lev[l], Wp, ainv);
+        direct_solve<T, PER>(g.lev[l], Wp, ainv);
         continue;
       }
-      if (op == 1) prolong_add<T>(g.lev[l - 1], g.lev[l], Wp);
-      smooth<T, CM, X>(g, g.lev[l], Wp, fac, ylds, op == 0);
+      if (op == 1) prolong_add<T, PER>(g.lev[l - 1], g.lev[l], Wp);
+      smooth<T, CM, X, PER>(g, g.lev[l], Wp, fac, ylds, op == 0);
       if (op == 0) {
         residual<T, X>(g.lev[l], Wp, fac);
-        restrict_fw<T>(g.lev[l], g.lev[l - 1], Wp);
+        restrict_fw<T, PER>(g.lev[l], g.lev[l - 1], Wp);
       }
     }
     if (!g.krylov) {
@@ -953,6 +1039,15 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
   g.tol = p.proj_tol;
   std::vector<HostMgLevel> hl(ng + 1);
   long ws = 0, fstride = 0;
+  // periodic z ends; with no specified side the kx = 0 problem is singular
+  // (constants are in the null space of every row) and its solution is
+  // pinned at the point (K = 0, J = NY+1) of every level, a MUDPACK boundary
+  // row, by scaling that row's diagonal by kPin: phi there is 1e-20 of the
+  // neighbouring values, and every other row holds as it is
+  const bool per = p.mudpack_bc[0] == 0;
+  bool anyspec = false;
+  for (int s = 0; s < 4; ++s) anyspec = anyspec || p.mudpack_bc[s] == 1;
+  constexpr double kPin = 1e20;
   for (int k = 1; k <= ng; ++k) {
     const int ex = k + p.mg_iex - ng, ey = k + p.mg_jey - ng;
     const int nx = p.mg_ixp * (1 << ((ex > 1 ? ex : 1) - 1)) + 1;
@@ -967,6 +1062,7 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
     L.n = nx * ny;
     L.f_off = (int)fstride;
     fstride += 9L * L.n;  // z factors, y factors, c9, y-chunk products
+    if (per) fstride += 2L * L.n + 4L * ny;  // wrap vectors and matrices of the z lines
   }
   int lc = 1;
   for (int k = 1; k <= ng; ++k)
@@ -1046,6 +1142,7 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
       }
   }
   tp.cross = cross;
+  tp.per = per;
   // workspace: from the coarsest level up, the levels whose correction,
   // right-hand side and two scratch arrays fit in the LDS left beside the
   // kernel's static arrays live there; the others (always the finest) in
@@ -1099,7 +1196,11 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
       T* fz = fac.data() + (size_t)m * fstride + g.lev[k].f_off;
       T* fy = fz + 3 * n;
       T* fc9 = fz + 6 * n;
-      auto c9 = [&](size_t q) { return ((-(kx2 * H.jl[q])) - H.s9[q]) + H.corr9[q]; };
+      const size_t qpin = (!anyspec && kx2 == 0.0) ? (size_t)nx * (ny - 1) : (size_t)-1;
+      auto c9 = [&](size_t q) {
+        const double v = ((-(kx2 * H.jl[q])) - H.s9[q]) + H.corr9[q];
+        return q == qpin ? kPin * v : v;
+      };
       for (size_t q = 0; q < n; ++q) fc9[q] = (T)c9(q);
       if (k == ng)
         for (size_t q = 0; q < n; ++q) c964[(size_t)m * nfine + q] = c9(q);
@@ -1111,12 +1212,46 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
           up[i] = H.c[0 * n + q];
         }
         line_factors(nx, lo.data(), di.data(), up.data(), f1.data(), fa.data(), fc.data());
+        // the chain of a line ends at its last point: the scans carry the
+        // values of the lanes past it into the back substitution, so its
+        // upper factor must be zero (with periodic ends c1 there is the wrap
+        // coupling, which the Woodbury step handles)
+        fa[0] = 0.0;
+        fc[nx - 1] = 0.0;
         for (int i = 0; i < nx; ++i) {
           const size_t q = i + (size_t)nx * j;
           // a specified point's correction is zero: 1/piv is stored as 0
           fz[q] = spec[off + q] ? (T)0 : (T)f1[i];
           fz[n + q] = (T)fa[i];
           fz[2 * n + q] = (T)fc[i];
+        }
+        if (per) {
+          // Woodbury terms of the line's wrap couplings (see zline_periodic):
+          // W = T^-1 [c5(0) e_0, c1(nx-1) e_{nx-1}] with the chain's factors
+          std::vector<double> w(2 * (size_t)nx);
+          for (int v = 0; v < 2; ++v) {
+            double* wv = w.data() + (size_t)v * nx;
+            double dp = 0.0;
+            for (int i = 0; i < nx; ++i) {
+              const size_t q = i + (size_t)nx * j;
+              const double sv = (v == 0 && i == 0) ? lo[0] : (v == 1 && i == nx - 1) ? up[nx - 1] : 0.0;
+              dp = spec[off + q] ? 0.0 : fa[i] * dp + sv * f1[i];
+              wv[i] = dp;
+            }
+            for (int i = nx - 2; i >= 0; --i) wv[i] -= fc[i] * wv[i + 1];
+          }
+          const double m00 = 1.0 + w[nx - 2], m01 = w[nx + nx - 2], m10 = w[1], m11 = 1.0 + w[nx + 1];
+          const double det = m00 * m11 - m01 * m10;
+          T* fw = fz + 9 * n;
+          for (int i = 0; i < nx; ++i) {
+            fw[i + (size_t)nx * j] = (T)w[i];
+            fw[n + i + (size_t)nx * j] = (T)w[nx + i];
+          }
+          T* mi = fz + 11 * n + 4 * (size_t)j;
+          mi[0] = (T)(m11 / det);
+          mi[1] = (T)(-m01 / det);
+          mi[2] = (T)(-m10 / det);
+          mi[3] = (T)(m00 / det);
         }
       }
       for (int i = 0; i < nx; ++i) {
@@ -1164,7 +1299,9 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
             if (spec[off + q]) continue;
             static const int di8[8] = {1, 1, 0, -1, -1, -1, 0, 1}, dj8[8] = {0, 1, 1, 1, 0, -1, -1, -1};
             for (int c = 0; c < 8; ++c) {
-              const int ii = i + di8[c], jj = j + dj8[c];
+              int ii = i + di8[c];
+              const int jj = j + dj8[c];
+              if (per) ii = ii < 0 ? nx - 2 : ii >= nx ? 1 : ii;  // periodic virtual columns
               if (ii < 0 || ii >= nx || jj < 0 || jj >= ny) continue;
               const size_t qq = ii + (size_t)nx * jj;
               if (spec[off + qq]) continue;  // the correction is zero there
@@ -1251,14 +1388,21 @@ int tight_solve_ex(TightPlan& tp, MgPlan& mg, const Dims& d, int nsolve, const d
   // a multiple of 16 workgroups for the XCD placement of the solve pairs
   const dim3 grid((unsigned)((nsolve + 15) & ~15));
   // workgroup size: tight_block
-#define LAUNCH2(T, CM, X)                                                                                \
+#define LAUNCH3(T, CM, X, PER)                                                                           \
   do {                                                                                                   \
     constexpr int NT = tight_block(sizeof(T) == 4, CM, X);                                                \
-    EKM_CHECK(hipFuncSetAttribute((const void*)k_tight_solve<T, CM, NT, X>,                             \
+    EKM_CHECK(hipFuncSetAttribute((const void*)k_tight_solve<T, CM, NT, X, PER>,                        \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, tp.lds_bytes));            \
-    hipLaunchKernelGGL((k_tight_solve<T, CM, NT, X>), grid, dim3(NT), tp.lds_bytes, st, (const TDesc<T>*)tp.mem, \
+    hipLaunchKernelGGL((k_tight_solve<T, CM, NT, X, PER>), grid, dim3(NT), tp.lds_bytes, st, (const TDesc<T>*)tp.mem, \
                        (T*)tp.ws, tp.ws64, (const double*)mg.rhs, mg.phi, (const double*)mg.kx2, mg.cycles, \
                        mg.cycles + mg.nsolve_max, d.plane, guess, nsolve);                                                          \
+  } while (0)
+#define LAUNCH2(T, CM, X)          \
+  do {                             \
+    if (tp.per)                    \
+      LAUNCH3(T, CM, X, true);     \
+    else                           \
+      LAUNCH3(T, CM, X, false);    \
   } while (0)
 #define LAUNCH(T, CM)           \
   do {                          \
@@ -1278,6 +1422,7 @@ int tight_solve_ex(TightPlan& tp, MgPlan& mg, const Dims& d, int nsolve, const d
   }
 #undef LAUNCH
 #undef LAUNCH2
+#undef LAUNCH3
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }

@@ -188,14 +188,8 @@ static int check_params(const ekm_params& p) {
     if (p.nz_s_r < p.nz + 1 && (p.nz_s_r < 0 || p.nz_s_l < 0 || p.nz_s_l > p.nz + 1)) return EKM_ERR_ARG;
   }
   if (p.proj_mode > 0) {
-    // line smoothing is not cyclic, and the consistent system needs a
-    // specified side to be nonsingular at kx = 0
-    bool dir = false;
-    for (int k = 0; k < 4; ++k) {
-      if (p.mudpack_bc[k] == 0) return EKM_ERR_UNSUPPORTED;
-      if (p.mudpack_bc[k] == 1) dir = true;
-    }
-    if (!dir) return EKM_ERR_UNSUPPORTED;
+    // periodic z ends are wrapped z lines; periodic y ends are not handled
+    if (p.mudpack_bc[2] == 0) return EKM_ERR_UNSUPPORTED;
     if (!(p.proj_tol > 0.0) || p.proj_maxit < 1) return EKM_ERR_ARG;
     if (p.proj_guess < 0 || p.proj_guess > 2) return EKM_ERR_ARG;
     if (p.proj_krylov < 0 || p.proj_krylov > 1) return EKM_ERR_ARG;

@@ -1158,18 +1158,32 @@ static int mg_solve(mg_ctx* c, double kx2, const double* rhs_tmp, double* phiout
 /* rows keep MUDPACK's discretisation.  Solved by alternating zebra-line     */
 /* V(1,1) cycles (full weighting, bilinear prolongation) inside residual     */
 /* iterations until max|residual| <= proj_tol.                               */
+/*                                                                           */
+/* Periodic z ends (MUDPACK nxa = nxb = 0): as in MUDPACK, every column of a */
+/* level is an unknown with its own row, and the virtual neighbours are      */
+/* e(-1) = e(nx-2) and e(nx) = e(1) (src/mud2cr.f:1693-1795), held in the    */
+/* ring columns; a z line is then the chain 0..nx-1 plus the couplings of    */
+/* its first point to nx-2 and of its last to 1, and the restriction wraps   */
+/* the same way.  A problem with no specified side is singular at           */
+/* kx = 0 (constants are in the null space of every row); its solution is    */
+/* pinned to 0 at the point (K = 0, J = NY+1) of every level, a MUDPACK      */
+/* boundary row, so that the rows of the divergence cells all hold.          */
 /* ------------------------------------------------------------------------ */
 typedef struct tlev {
   int nx, ny;
-  double *cf, *e, *r, *res; /* cf: 9 planes [q][j][i]; e: (nx+2)(ny+2) with a zero ring */
+  double *cf, *e, *r, *res; /* cf: 9 planes [q][j][i]; e: (nx+2)(ny+2) with a ring */
   unsigned char* spec;
+  int pin;                  /* the pinned point (i + nx j) of a singular solve, or -1 */
 } tlev;
 
 typedef struct tctx {
   int ngrid, nxa, nxb, nyc, nyd;
+  int per;                  /* periodic z ends */
   tlev lev[17];
-  double *lo, *di, *up, *s, *cp, *dp, *phi;
+  double *lo, *di, *up, *s, *cp, *dp, *phi, *zz, *ys, *zs;
 } tctx;
+
+#define TSPEC(L, p) ((L)->spec[p] || (int)(p) == (L)->pin)
 
 /* Row of cell (K,J) of the consistent operator, coefficients in MUDPACK's
  * neighbour order c1 (K+1,J), c2 (K+1,J+1), c3 (K,J+1), c4 (K-1,J+1),
@@ -1198,6 +1212,7 @@ static void tight_setup(tctx* t, const mg_ctx* c) {
   memset(t, 0, sizeof(*t));
   t->ngrid = c->ngrid;
   t->nxa = c->nxa; t->nxb = c->nxb; t->nyc = c->nyc; t->nyd = c->nyd;
+  t->per = c->nxa == 0;
   size_t mx = 0;
   for (int k = 1; k <= c->ngrid; ++k) {
     tlev* L = &t->lev[k];
@@ -1208,6 +1223,7 @@ static void tight_setup(tctx* t, const mg_ctx* c) {
     L->r = dalloc(n);
     L->res = dalloc(n);
     L->spec = (unsigned char*)calloc(n, 1);
+    L->pin = -1;
     for (int j = 0; j < L->ny; ++j)
       for (int i = 0; i < L->nx; ++i)
         L->spec[i + (size_t)L->nx * j] = (t->nxa == 1 && i == 0) || (t->nxb == 1 && i == L->nx - 1) ||
@@ -1216,7 +1232,8 @@ static void tight_setup(tctx* t, const mg_ctx* c) {
     if ((size_t)L->ny > mx) mx = L->ny;
   }
   t->lo = dalloc(mx); t->di = dalloc(mx); t->up = dalloc(mx);
-  t->s = dalloc(mx); t->cp = dalloc(mx); t->dp = dalloc(mx);
+  t->s = dalloc(mx); t->cp = dalloc(mx); t->dp = dalloc(mx); t->zz = dalloc(mx);
+  t->ys = dalloc(mx); t->zs = dalloc(2 * mx);
   t->phi = dalloc((size_t)(c->nfx + 2) * (c->nfy + 2));
 }
 
@@ -1225,7 +1242,7 @@ static void tight_free(tctx* t) {
     tlev* L = &t->lev[k];
     free(L->cf); free(L->e); free(L->r); free(L->res); free(L->spec);
   }
-  free(t->lo); free(t->di); free(t->up); free(t->s); free(t->cp); free(t->dp); free(t->phi);
+  free(t->lo); free(t->di); free(t->up); free(t->s); free(t->cp); free(t->dp); free(t->zz); free(t->ys); free(t->zs); free(t->phi);
 }
 
 /* operator coefficients of every level for kx2: MUDPACK's discretisation,
@@ -1237,6 +1254,12 @@ static void tight_discretize(tctx* t, mg_ctx* c, const ora* o, double kx2) {
     tlev* L = &t->lev[k];
     size_t n = (size_t)L->nx * L->ny;
     for (int q = 0; q < 9; ++q) memcpy(L->cf + q * n, c->lev[k].cf + q * n, n * sizeof(double));
+  }
+  /* singular: kx = 0 with no specified side */
+  const int sing = kx2 == 0.0 && t->nxa != 1 && t->nxb != 1 && t->nyc != 1 && t->nyd != 1;
+  for (int k = 1; k <= t->ngrid; ++k) {
+    tlev* L = &t->lev[k];
+    L->pin = sing ? L->nx * (L->ny - 1) : -1;
   }
   tlev* F = &t->lev[t->ngrid];
   size_t n = (size_t)F->nx * F->ny;
@@ -1258,7 +1281,7 @@ static void tight_residual(const tlev* L, const double* e, const double* r, doub
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i) {
       size_t p = i + (size_t)nx * j;
-      if (L->spec[p]) { res[p] = 0.0; continue; }
+      if (TSPEC(L, p)) { res[p] = 0.0; continue; }
       double acc = 0.0;
       for (int q = 0; q < 8; ++q) acc += TC(L, q, i, j) * e[(i + di[q] + 1) + (size_t)(nx + 2) * (j + dj[q] + 1)];
       acc += TC(L, 8, i, j) * e[(i + 1) + (size_t)(nx + 2) * (j + 1)];
@@ -1279,6 +1302,40 @@ static void thomas_line(int n, const double* lo, const double* di, const double*
   for (int i = n - 2; i >= 0; --i) x[(size_t)i * stride] = dp[i] - cp[i] * x[(size_t)(i + 1) * stride];
 }
 
+/* periodic ends: the ring columns take the virtual values e(-1) = e(nx-2)
+ * and e(nx) = e(1) */
+static void tight_wrap(const tlev* L, double* e) {
+  const int nx = L->nx, ld = nx + 2;
+  for (int j = -1; j <= L->ny; ++j) {
+    double* row = e + (size_t)ld * (j + 1) + 1;
+    row[-1] = row[nx - 2];
+    row[nx] = row[1];
+  }
+}
+
+/* z line of a periodic level: the tridiagonal chain T (lo, di, up over
+ * 0..n-1) plus lo[0] coupling point 0 to n-2 and up[n-1] coupling point n-1
+ * to 1.  A = T + U V^T with U = [lo0 e_0, up_{n-1} e_{n-1}], V = [e_{n-2}, e_1];
+ * Woodbury: x = y - W (I + V^T W)^-1 V^T y with y = T^-1 s, W = T^-1 U. */
+static void thomas_periodic(tctx* t, int n, double* x, size_t stride) {
+  const double a0 = t->lo[0], bn = t->up[n - 1];
+  double* u = t->zz;
+  double* y = t->ys;
+  double* w = t->zs; /* [2][n] */
+  thomas_line(n, t->lo, t->di, t->up, t->s, t->cp, t->dp, y, 1);
+  for (int i = 0; i < n; ++i) u[i] = 0.0;
+  u[0] = a0;
+  thomas_line(n, t->lo, t->di, t->up, u, t->cp, t->dp, w, 1);
+  u[0] = 0.0;
+  u[n - 1] = bn;
+  thomas_line(n, t->lo, t->di, t->up, u, t->cp, t->dp, w + n, 1);
+  /* M = I + V^T W (2x2), rows: components n-2 and 1 */
+  const double m00 = 1.0 + w[n - 2], m01 = w[n + n - 2], m10 = w[1], m11 = 1.0 + w[n + 1];
+  const double det = m00 * m11 - m01 * m10;
+  const double g0 = (m11 * y[n - 2] - m01 * y[1]) / det, g1 = (m00 * y[1] - m10 * y[n - 2]) / det;
+  for (int i = 0; i < n; ++i) x[(size_t)i * stride] = y[i] - (w[i] * g0 + w[n + i] * g1);
+}
+
 /* zebra line Gauss-Seidel: dir 0 lines along i (z) for rows j of parity par,
  * dir 1 lines along j (y) for columns i of parity par */
 static void tight_lines(tctx* t, tlev* L, int dir, int par) {
@@ -1287,19 +1344,22 @@ static void tight_lines(tctx* t, tlev* L, int dir, int par) {
     for (int j = par; j < ny; j += 2) {
       for (int i = 0; i < nx; ++i) {
         size_t p = i + (size_t)nx * j;
-        if (L->spec[p]) { t->lo[i] = 0.0; t->di[i] = 1.0; t->up[i] = 0.0; t->s[i] = TE(L, i, j); continue; }
+        if (TSPEC(L, p)) { t->lo[i] = 0.0; t->di[i] = 1.0; t->up[i] = 0.0; t->s[i] = TE(L, i, j); continue; }
         t->s[i] = L->r[p] - (TC(L, 1, i, j) * TE(L, i + 1, j + 1) + TC(L, 2, i, j) * TE(L, i, j + 1) +
                              TC(L, 3, i, j) * TE(L, i - 1, j + 1) + TC(L, 5, i, j) * TE(L, i - 1, j - 1) +
                              TC(L, 6, i, j) * TE(L, i, j - 1) + TC(L, 7, i, j) * TE(L, i + 1, j - 1));
         t->lo[i] = TC(L, 4, i, j); t->di[i] = TC(L, 8, i, j); t->up[i] = TC(L, 0, i, j);
       }
-      thomas_line(nx, t->lo, t->di, t->up, t->s, t->cp, t->dp, &TE(L, 0, j), 1);
+      if (t->per)
+        thomas_periodic(t, nx, &TE(L, 0, j), 1);
+      else
+        thomas_line(nx, t->lo, t->di, t->up, t->s, t->cp, t->dp, &TE(L, 0, j), 1);
     }
   } else {
     for (int i = par; i < nx; i += 2) {
       for (int j = 0; j < ny; ++j) {
         size_t p = i + (size_t)nx * j;
-        if (L->spec[p]) { t->lo[j] = 0.0; t->di[j] = 1.0; t->up[j] = 0.0; t->s[j] = TE(L, i, j); continue; }
+        if (TSPEC(L, p)) { t->lo[j] = 0.0; t->di[j] = 1.0; t->up[j] = 0.0; t->s[j] = TE(L, i, j); continue; }
         t->s[j] = L->r[p] - (TC(L, 0, i, j) * TE(L, i + 1, j) + TC(L, 1, i, j) * TE(L, i + 1, j + 1) +
                              TC(L, 3, i, j) * TE(L, i - 1, j + 1) + TC(L, 4, i, j) * TE(L, i - 1, j) +
                              TC(L, 5, i, j) * TE(L, i - 1, j - 1) + TC(L, 7, i, j) * TE(L, i + 1, j - 1));
@@ -1308,6 +1368,7 @@ static void tight_lines(tctx* t, tlev* L, int dir, int par) {
       thomas_line(ny, t->lo, t->di, t->up, t->s, t->cp, t->dp, &TE(L, i, 0), (size_t)ld);
     }
   }
+  if (t->per) tight_wrap(L, L->e);
 }
 
 static void tight_smooth(tctx* t, tlev* L) {
@@ -1317,19 +1378,21 @@ static void tight_smooth(tctx* t, tlev* L) {
 
 /* full weighting (1/4,1/2,1/4 per direction, zero outside the grid) of the
  * fine residual onto the coarse points; a direction that is not coarsened
- * keeps its index */
-static void tight_restrict(const tlev* F, const double* res, tlev* C) {
+ * keeps its index; periodic ends take the virtual columns -1 -> nx-2 and
+ * nx -> 1 */
+static void tight_restrict(const tctx* t, const tlev* F, const double* res, tlev* C) {
   const int sx = C->nx < F->nx ? 2 : 1, sy = C->ny < F->ny ? 2 : 1;
   static const double w[3] = {0.25, 0.5, 0.25};
   for (int jc = 0; jc < C->ny; ++jc)
     for (int ic = 0; ic < C->nx; ++ic) {
       size_t pc = ic + (size_t)C->nx * jc;
-      if (C->spec[pc]) { C->r[pc] = 0.0; continue; }
+      if (TSPEC(C, pc)) { C->r[pc] = 0.0; continue; }
       int i = sx * ic, j = sy * jc;
       double acc = 0.0;
       for (int b = -1; b <= 1; ++b)
         for (int a = -1; a <= 1; ++a) {
           int ii = i + a, jj = j + b;
+          if (t->per) ii = ii < 0 ? F->nx - 2 : ii >= F->nx ? 1 : ii;
           if (ii < 0 || ii >= F->nx || jj < 0 || jj >= F->ny) continue;
           acc += (w[a + 1] * w[b + 1]) * res[ii + (size_t)F->nx * jj];
         }
@@ -1338,15 +1401,16 @@ static void tight_restrict(const tlev* F, const double* res, tlev* C) {
 }
 
 /* e_fine += bilinear interpolation of e_coarse (not at specified points) */
-static void tight_prolong(const tlev* C, tlev* F) {
+static void tight_prolong(const tctx* t, const tlev* C, tlev* F) {
   const int sx = C->nx < F->nx ? 2 : 1, sy = C->ny < F->ny ? 2 : 1;
   for (int j = 0; j < F->ny; ++j)
     for (int i = 0; i < F->nx; ++i) {
-      if (F->spec[i + (size_t)F->nx * j]) continue;
+      if (TSPEC(F, i + (size_t)F->nx * j)) continue;
       int i0 = i / sx, i1 = (i + sx - 1) / sx, j0 = j / sy, j1 = (j + sy - 1) / sy;
       double v = 0.25 * ((TE(C, i0, j0) + TE(C, i1, j0)) + (TE(C, i0, j1) + TE(C, i1, j1)));
       TE(F, i, j) += v;
     }
+  if (t->per) tight_wrap(F, F->e);
 }
 
 static void tight_vcycle(tctx* t, int l) {
@@ -1358,10 +1422,10 @@ static void tight_vcycle(tctx* t, int l) {
   tight_smooth(t, L);
   tight_residual(L, L->e, L->r, L->res);
   tlev* C = &t->lev[l - 1];
-  tight_restrict(L, L->res, C);
+  tight_restrict(t, L, L->res, C);
   memset(C->e, 0, (size_t)(C->nx + 2) * (C->ny + 2) * sizeof(double));
   tight_vcycle(t, l - 1);
-  tight_prolong(C, L);
+  tight_prolong(t, C, L);
   tight_smooth(t, L);
 }
 
@@ -1374,7 +1438,7 @@ static int tight_solve(tctx* t, mg_ctx* c, const ora* o, double kx2, const doubl
   const size_t n = (size_t)nx * ny;
   double* b = dalloc(n);
   double* res = dalloc(n);
-  for (size_t p = 0; p < n; ++p) b[p] = F->spec[p] ? 0.0 : -rhs_tmp[p];
+  for (size_t p = 0; p < n; ++p) b[p] = TSPEC(F, p) ? 0.0 : -rhs_tmp[p];
   memset(t->phi, 0, (size_t)(nx + 2) * (ny + 2) * sizeof(double));
   int it = 0;
   const int maxit = o->p.proj_maxit > 0 ? o->p.proj_maxit : 50;
@@ -1507,14 +1571,9 @@ static int supported(const ekm_params* p) {
   if (p->sponge_type < 0 || p->sponge_type > 2) return 0;
   if (p->sponge_type && (p->ny_s_t < 0 || p->ny_s_t == p->ny + 1 || p->save_stats_int < 1)) return 0;
   if (p->sponge_type && p->nz_s_r < p->nz + 1 && (p->nz_s_r < 0 || p->nz_s_l < 0 || p->nz_s_l > p->nz + 1)) return 0;
-  if (p->proj_mode == 1) {
-    int dir = 0;
-    for (int s = 0; s < 4; ++s) {
-      if (p->mudpack_bc[s] == 0) return 0;
-      if (p->mudpack_bc[s] == 1) dir = 1;
-    }
-    if (!dir || !(p->proj_tol > 0.0)) return 0;
-  }
+  /* the consistent projection: periodic z ends are cyclic lines, periodic
+   * y ends are not handled */
+  if (p->proj_mode == 1 && (p->mudpack_bc[2] == 0 || !(p->proj_tol > 0.0))) return 0;
   return 1;
 }
 

@@ -497,21 +497,43 @@ def test_tight_rows_are_divergence_of_projection_gradient(oracle_lib):
     assert worst < 1e-12, worst
 
 
-def test_tight_solve_matches_dense(oracle_lib):
+@pytest.mark.parametrize("bcs", [(2, 2, 2, 1), (0, 0, 2, 1), (0, 0, 2, 2), (2, 2, 2, 2)])
+def test_tight_solve_matches_dense(bcs, oracle_lib):
+    """The consistent solve against a sparse direct solve of the same
+    operator (its rows from tight_operator, assembled with MUDPACK's virtual
+    points for periodic z ends: phi(-1) = phi(nx-2), phi(nx) = phi(1)), for
+    the configurations' set, periodic z ends with a specified top, and the
+    two sets with no specified side, whose kx = 0 problem is singular: there
+    the direct solve replaces the row of the point (K = 0, J = NY+1) by
+    phi = 0, the pin the oracle applies, and every other row must hold."""
     O = oracle_lib
     cfg, p, xp, yp = _tight_oracle()
+    p.mudpack_bc[:] = bcs
+    xp = xp + 0.05 * np.sin(0.3 * yp)                   # every metric term active
     o = O.Oracle(p, xp, yp)
+    singular_seen = False
     for kx2 in (0.0, 0.25):
         rhs = np.random.default_rng(3).standard_normal((p.ny + 2, p.nz + 2)) * 1e-3
         phi, it = o.tight_solve(kx2, rhs)
         cf = o.tight_operator(kx2)
-        A, _, _ = _assemble(np.concatenate([cf, np.zeros((1,) + cf.shape[1:])]), p.nz + 2, p.ny + 2,
-                            tuple(p.mudpack_bc))
+        A, _, _ = _assemble(np.concatenate([cf, np.zeros((1,) + cf.shape[1:])]), p.nz + 2, p.ny + 2, bcs)
         b = -rhs.copy()
-        b[-1, :] = 0.0                                  # Dirichlet top (MUDPACK_BC(4) = 1)
+        if bcs[3] == 1:
+            b[-1, :] = 0.0                              # Dirichlet top (MUDPACK_BC(4) = 1)
+        if kx2 == 0.0 and 1 not in bcs:
+            singular_seen = True
+            # constants are in the null space of every row
+            assert np.abs(A @ np.ones(A.shape[0])).max() < 1e-9 * np.abs(A).max()
+            pin = (p.ny + 1) * (p.nz + 2)
+            A = A.tolil()
+            A[pin, :] = 0.0
+            A[pin, pin] = 1.0
+            b[-1, 0] = 0.0
+            assert phi[-1, 0] == 0.0
         x = spla.spsolve(A.tocsc(), b.ravel()).reshape(b.shape)
-        assert it < 30
-        assert rel(phi, x) < 1e-8
+        assert it < 40
+        assert rel(phi, x) < 1e-8, (bcs, kx2, rel(phi, x))
+    assert singular_seen == (1 not in bcs)
 
 
 def test_tight_projection_removes_divergence(oracle_lib):
