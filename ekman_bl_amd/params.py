@@ -200,24 +200,37 @@ def config(n: int) -> Config:
 # ---------------------------------------------------------------------------
 def read_grid_def(path: str) -> dict:
     """PARAMETER values of a grid_def file (NX, NY, NZ, N_TH, iixp, ...).
-    Integer expressions such as ``129-2`` are evaluated."""
+    Every ``name = expression`` of a PARAMETER statement whose expression,
+    after the names defined before it are replaced by their values, holds
+    only integers and + - * / ** ( ) is evaluated (``129-2``,
+    ``NZ + 1 - NZ_S_R``, ``iixp*2**(iiex-1)+1``); the others (intrinsic
+    calls such as ``ceiling(real(NZ+2)/real(NP))*NP``) are skipped."""
     out = {}
     for line in open(path):
         line = line.split("!")[0]
-        for name, expr in re.findall(r"PARAMETER\s*\(\s*(\w+)\s*=\s*([^,\)]+)", line, re.I):
-            expr = expr.strip()
-            # names defined on earlier lines (NZ_S_L = NZ + 1 - NZ_S_R)
-            expr = re.sub(r"[A-Za-z_]\w*", lambda m: str(out.get(m.group(0).lower(), m.group(0))), expr)
-            if re.fullmatch(r"[0-9+\-*/ ()]+", expr):
-                out[name.lower()] = int(eval(expr))  # noqa: S307 - digits and operators only
         m = re.search(r"PARAMETER\s*\((.*)\)", line, re.I)
-        if m and "," in m.group(1):
-            for part in m.group(1).split(","):
-                if "=" in part:
-                    k, v = part.split("=")
-                    v = v.strip()
-                    if re.fullmatch(r"[0-9+\-*/ ()]+", v):
-                        out[k.strip().lower()] = int(eval(v))  # noqa: S307
+        if not m:
+            continue
+        # split the statement at its top-level commas
+        parts, depth, cur = [], 0, ""
+        for ch in m.group(1):
+            depth += (ch == "(") - (ch == ")")
+            if ch == "," and depth == 0:
+                parts.append(cur)
+                cur = ""
+            else:
+                cur += ch
+        parts.append(cur)
+        for part in parts:
+            if "=" not in part:
+                continue
+            name, expr = (x.strip() for x in part.split("=", 1))
+            expr = re.sub(r"[A-Za-z_]\w*", lambda t: str(out.get(t.group(0).lower(), t.group(0))), expr)
+            if re.fullmatch(r"[0-9+\-*/ ()]+", expr):
+                try:
+                    out[name.lower()] = int(eval(expr))  # noqa: S307 - digits and operators only
+                except (SyntaxError, ZeroDivisionError):
+                    pass
     return out
 
 

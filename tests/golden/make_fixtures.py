@@ -43,7 +43,7 @@ def main(ref):
     for j, k, v in dev:
         rebuilt[j, k] = v
     assert np.array_equal(rebuilt, yp)
-    keys = ("nx", "ny", "nz", "n_th", "iixp", "jjyq", "iiex", "jjey", "np")
+    keys = ("nx", "ny", "nz", "n_th", "iixp", "jjyq", "iiex", "jjey", "np", "ny_s_t", "nz_s_r", "nz_s_l")
     fix = {"source": "rundir_unst/GRID_YZ.dat and rundir_unst/grid_def",
            "grid_def": {k: g[k] for k in keys if k in g},
            "y": [float(v) for v in y], "z": [float(v) for v in z], "y_exceptions": dev}

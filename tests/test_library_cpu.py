@@ -116,6 +116,40 @@ def test_read_input_dat_fixture():
     assert vals["les"] is False and p.les_model == 0 and vals["wall_model_type"] == 3
 
 
+def test_read_grid_def_expressions(tmp_path):
+    """PARAMETER statements in the forms a grid_def uses: one or several
+    names per statement, integer expressions, names defined on earlier lines,
+    nested parentheses inside an expression, intrinsic calls (skipped) and
+    trailing comments."""
+    from ekman_bl_amd.params import params_from_run_dir, read_grid_def
+    text = "\n".join([
+        "      INTEGER NX, NY, NZ",
+        "\tPARAMETER (NX = 64)",
+        "\tPARAMETER (NY = 33-2)   ! interior rows",
+        "\tPARAMETER (NZ = 65-2)",
+        "\tPARAMETER (N_TH = 1)",
+        "\tPARAMETER (NZ_S_R = 900)",
+        "\tPARAMETER (NZ_S_L = NZ + 1 - NZ_S_R)",
+        "\tPARAMETER (NY_S_T = 25)",
+        "\tPARAMETER (NP= 4)",
+        "\tPARAMETER (NZV=ceiling(real(NZ+2)/real(NP))*NP)",
+        "\tPARAMETER (iixp = 2 , jjyq = 2, iiex = 6, jjey = 5 )",
+        "\tPARAMETER (nnx=iixp*2**(iiex-1)+1, nny=jjyq*2**(jjey-1)+1)",
+    ]) + "\n"
+    path = tmp_path / "grid_def"
+    path.write_text(text)
+    g = read_grid_def(str(path))
+    assert (g["nx"], g["ny"], g["nz"], g["n_th"]) == (64, 31, 63, 1)
+    assert g["nz_s_l"] == 63 + 1 - 900 and g["ny_s_t"] == 25
+    assert (g["iixp"], g["jjyq"], g["iiex"], g["jjey"]) == (2, 2, 6, 5)
+    assert g["nnx"] == 65 and g["nny"] == 33
+    assert "nzv" not in g
+    p = params_from_run_dir(str(path), os.path.join(GOLDEN, "input_unst.dat"))
+    assert (p.nx, p.ny, p.nz, p.ny_s_t, p.nz_s_r) == (64, 31, 63, 25, 900)
+    assert (p.mg_ixp, p.mg_iex, p.mg_jyq, p.mg_jey) == (2, 6, 2, 5)
+    assert list(p.mudpack_bc) == [0, 0, 2, 2]
+
+
 def _edited_input(tmp_path, edit) -> str:
     lines = open(os.path.join(GOLDEN, "input_unst.dat")).read().splitlines()
     path = tmp_path / "input.dat"
