@@ -76,6 +76,10 @@ struct TDesc {
   const T* fac;     // [mode][fstride]
   const double* cf64;  // finest level c1..c8 in double [8][n]
   const double* c964;  // finest level c9 in double, [mode][n]
+  // singular kx = 0 problem (no specified side): the left null vector of the
+  // finest operator scaled to 1 at the pin point (finest index `pin`), or null
+  const double* wnull;
+  int pin;
   // the V(1,1) cycle as a program, one step per byte: (level << 2) | op,
   // op 0 = smooth, residual and restriction to the next coarser level,
   // 1 = prolongation from the next coarser level and smooth, 2 = direct solve
@@ -872,7 +876,11 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
   // so it is formed here instead of by the residual pass; b is kept only
   // where a later residual reads it.
   const bool zero_start = guess == 0.0;
+  // the singular kx = 0 solve: b_pin -= w.b makes the right-hand side
+  // compatible (w the left null vector, w_pin = 1)
+  const bool sing = g.wnull != nullptr && (s >> 1) == 0;
   double rmax0 = 0.0;
+  double wb[4] = {0.0, 0.0, 0.0, 0.0};
   for (int p = threadIdx.x; p < n; p += blockDim.x) {
     const double bv = F.spec[p] ? 0.0 : -rt[p];
     if (!zero_start || !g.krylov) b[p] = bv;
@@ -880,6 +888,20 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
       rmax0 = fmax(rmax0, fabs(bv));
       rf[p] = (T)bv;
       if (g.krylov) rr[p] = bv;
+    }
+    if (sing) wb[0] += g.wnull[p] * bv;
+  }
+  if (sing) {
+    block_sum4(wb, red);
+    if (threadIdx.x == 0) {
+      const int p = g.pin;
+      const double bv = (F.spec[p] ? 0.0 : -rt[p]) - wb[0];
+      if (!zero_start || !g.krylov) b[p] = bv;
+      if (zero_start) {
+        rmax0 = fmax(rmax0, fabs(bv));
+        rf[p] = (T)bv;
+        if (g.krylov) rr[p] = bv;
+      }
     }
   }
   // every level's correction starts at zero: the rings stay zero, the interiors
@@ -970,9 +992,11 @@ __global__ void __launch_bounds__(NT) k_tight_solve(const TDesc<T>* __restrict__
     ++it;
   }
   double* po = phi_out + (size_t)s * plane;
+  // the singular solve: the solution with phi = 0 at the pin
+  const double shift = sing ? phi[(g.pin % nx + 1) + ld * (g.pin / nx + 1)] : 0.0;
   for (int p = threadIdx.x; p < n; p += blockDim.x) {
     const int j = p / nx, i = p - j * nx;
-    po[p] = phi[(i + 1) + ld * (j + 1)];
+    po[p] = phi[(i + 1) + ld * (j + 1)] - shift;
   }
   if (threadIdx.x == 0) {
     cycles[s] = it;
@@ -1012,6 +1036,76 @@ bool invert_dense(int n, std::vector<double> A, std::vector<double>& inv) {
   return true;
 }
 
+// The left null vector w of a singular finest operator (kx = 0, no specified
+// side), scaled to w[pin] = 1: the equations of A^T w = 0 but the pin's,
+// which becomes w[pin] = 1 (the rows of A^T sum to zero, since A annihilates
+// constants, so any one of them follows from the others and the system is
+// nonsingular).  Points ordered i + nx j, so every coupling, the periodic
+// virtual columns included, lies within a band of half-width nx+1 (2nx-2
+// when the periodic wrap has cross terms); banded Gaussian elimination with
+// partial pivoting, O(n band^2) once per solver.
+bool left_null_vector(const HostMgLevel& H, bool per, int pin, std::vector<double>& w) {
+  const int nx = H.nx, ny = H.ny, n = nx * ny;
+  static const int di8[8] = {1, 1, 0, -1, -1, -1, 0, 1}, dj8[8] = {0, 1, 1, 1, 0, -1, -1, -1};
+  // the couplings: A(q, qq) = coefficient of point qq in row q, A^T(qq, q) = A(q, qq)
+  auto each = [&](auto&& fn) {
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const int q = i + nx * j;
+        for (int c = 0; c < 8; ++c) {
+          const double v = H.c[(size_t)c * n + q];
+          if (v == 0.0) continue;
+          int ii = i + di8[c];
+          const int jj = j + dj8[c];
+          if (per) ii = ii < 0 ? nx - 2 : ii >= nx ? 1 : ii;
+          if (ii < 0 || ii >= nx || jj < 0 || jj >= ny) continue;
+          fn(q, ii + nx * jj, v);
+        }
+      }
+  };
+  int kl = 1;
+  each([&](int q, int qq, double) { kl = std::max(kl, std::abs(qq - q)); });
+  const int ku = kl, wd = 2 * kl + ku + 1;
+  std::vector<double> band((size_t)n * wd, 0.0), rhs(n, 0.0);
+  auto at = [&](int r, int c) -> double& { return band[(size_t)r * wd + (c - r + kl)]; };
+  for (int q = 0; q < n; ++q)
+    if (q != pin) at(q, q) += (-H.s9[q]) + H.corr9[q];
+  each([&](int q, int qq, double v) {
+    if (qq != pin) at(qq, q) += v;
+  });
+  at(pin, pin) = 1.0;
+  rhs[pin] = 1.0;
+  for (int k = 0; k < n; ++k) {
+    const int last = std::min(n - 1, k + kl);
+    int pr = k;
+    for (int r = k + 1; r <= last; ++r)
+      if (std::fabs(at(r, k)) > std::fabs(at(pr, k))) pr = r;
+    if (at(pr, k) == 0.0) return false;
+    const int cend = std::min(n - 1, k + kl + ku);
+    if (pr != k) {
+      for (int c = k; c <= cend; ++c) std::swap(at(k, c), at(pr, c));
+      std::swap(rhs[k], rhs[pr]);
+    }
+    const double inv = 1.0 / at(k, k);
+    for (int r = k + 1; r <= last; ++r) {
+      const double f = at(r, k) * inv;
+      if (f == 0.0) continue;
+      double* rr = &at(r, k);
+      const double* rk = &at(k, k);
+      for (int c = 0; c <= cend - k; ++c) rr[c] -= f * rk[c];
+      rhs[r] -= f * rhs[k];
+    }
+  }
+  w.assign(n, 0.0);
+  for (int k = n - 1; k >= 0; --k) {
+    double acc = rhs[k];
+    const int cend = std::min(n - 1, k + kl + ku);
+    for (int c = k + 1; c <= cend; ++c) acc -= at(k, c) * w[c];
+    w[k] = acc / at(k, k);
+  }
+  return std::isfinite(w[0]);
+}
+
 // Thomas factors of one line: lower/diag/upper -> (1/piv, -lower/piv, upper/piv)
 void line_factors(int len, const double* lo, const double* di, const double* up, double* f1, double* fa,
                   double* fc) {
@@ -1040,10 +1134,16 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
   std::vector<HostMgLevel> hl(ng + 1);
   long ws = 0, fstride = 0;
   // periodic z ends; with no specified side the kx = 0 problem is singular
-  // (constants are in the null space of every row) and its solution is
-  // pinned at the point (K = 0, J = NY+1) of every level, a MUDPACK boundary
-  // row, by scaling that row's diagonal by kPin: phi there is 1e-20 of the
-  // neighbouring values, and every other row holds as it is
+  // (constants are in the null space of every row).  Its solution is the one
+  // with phi = 0 at the point p = (K = 0, J = NY+1), a MUDPACK boundary row,
+  // that satisfies every other row.  A pin on every level (p's diagonal
+  // scaled by kPin) gives that solution but slows the cycles several-fold
+  // (a point constraint is not a smooth error), so the finest level stays
+  // singular: its right-hand side is made compatible, b_p -= w.b with w the
+  // operator's left null vector scaled to w_p = 1 (which changes row p only),
+  // the cycles' coarse levels carry the pin, and the converged solution is
+  // shifted to phi_p = 0.  The coarse pins only move the correction by
+  // constants near p, which the finest residual sees and removes.
   const bool per = p.mudpack_bc[0] == 0;
   bool anyspec = false;
   for (int s = 0; s < 4; ++s) anyspec = anyspec || p.mudpack_bc[s] == 1;
@@ -1196,7 +1296,10 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
       T* fz = fac.data() + (size_t)m * fstride + g.lev[k].f_off;
       T* fy = fz + 3 * n;
       T* fc9 = fz + 6 * n;
-      const size_t qpin = (!anyspec && kx2 == 0.0) ? (size_t)nx * (ny - 1) : (size_t)-1;
+      // the pin: every level below the finest (the finest operator stays
+      // singular and its right-hand side is made compatible, see below), or
+      // the finest too when it is the directly solved level
+      const size_t qpin = (!anyspec && kx2 == 0.0 && (k < ng || lc == ng)) ? (size_t)nx * (ny - 1) : (size_t)-1;
       auto c9 = [&](size_t q) {
         const double v = ((-(kx2 * H.jl[q])) - H.s9[q]) + H.corr9[q];
         return q == qpin ? kPin * v : v;
@@ -1315,14 +1418,23 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
       }
     }
   }
-  // device copies: descriptor, spec, c1..c8 (T), finest c1..c8 (double), c9 per mode (double), factors
+  // the singular kx = 0 problem's left null vector (see the pin above)
+  std::vector<double> wnull;
+  g.pin = (int)((size_t)g.lev[ng].nx * (g.lev[ng].ny - 1));
+  if (!anyspec && lc < ng) {
+    if (nmode < 1 || kx2_mode[0] != 0.0) return EKM_ERR_UNSUPPORTED;
+    if (!left_null_vector(hl[ng], per, g.pin, wnull)) return EKM_ERR_UNSUPPORTED;
+  }
+  // device copies: descriptor, spec, c1..c8 (T), finest c1..c8 (double), c9 per mode (double),
+  // the left null vector, factors
   const size_t b_desc = (sizeof(TDesc<T>) + 255) & ~(size_t)255;
   const size_t b_spec = (tot_n + 255) & ~(size_t)255;
   const size_t b_cf = (8 * tot_n * sizeof(T) + 255) & ~(size_t)255;
   const size_t b_cf64 = (8 * nfine * sizeof(double) + 255) & ~(size_t)255;
   const size_t b_c964 = (c964.size() * sizeof(double) + 255) & ~(size_t)255;
   char* mem = nullptr;
-  EKM_CHECK(hipMalloc(&mem, b_desc + b_spec + 2 * b_cf + b_cf64 + b_c964));
+  const size_t b_w = (wnull.size() * sizeof(double) + 255) & ~(size_t)255;
+  EKM_CHECK(hipMalloc(&mem, b_desc + b_spec + 2 * b_cf + b_cf64 + b_c964 + b_w));
   tp.mem = mem;
   char* cur = mem + b_desc;
   unsigned char* d_spec = (unsigned char*)cur;
@@ -1341,6 +1453,9 @@ int tight_build_typed(TightPlan& tp, const ekm_params& p, const Dims& d, const H
   }
   g.cf64 = d_cf64;
   g.c964 = d_c964;
+  double* d_w = (double*)((char*)d_c964 + b_c964);
+  g.wnull = wnull.empty() ? nullptr : d_w;
+  if (!wnull.empty()) EKM_CHECK(hipMemcpy(d_w, wnull.data(), wnull.size() * sizeof(double), hipMemcpyHostToDevice));
   EKM_CHECK(hipMalloc(&tp.fac, fac.size() * sizeof(T)));
   g.fac = (const T*)tp.fac;
   EKM_CHECK(hipMemcpy(tp.fac, fac.data(), fac.size() * sizeof(T), hipMemcpyHostToDevice));

@@ -81,8 +81,9 @@ typedef struct ekm_params {
    *     the solve's residual; solved by FP32 alternating zebra-line V(1,1)
    *     cycles inside FP64 iterative refinement until max|residual| <= proj_tol.
    *     Periodic z ends (mudpack_bc 0 0) are supported, periodic y ends are
-   *     not (EKM_ERR_UNSUPPORTED); with no specified side the kx = 0 solve is
-   *     pinned to 0 at (K = 0, J = NY+1) */
+   *     not (EKM_ERR_UNSUPPORTED); with no specified side the kx = 0 solve
+   *     returns the solution with phi = 0 at (K = 0, J = NY+1) that satisfies
+   *     every other row (its left null vector is computed once at creation) */
   int proj_mode;
   int proj_maxit;          /* cycle cap of proj_mode 1, 2 */
   double proj_tol;         /* absolute residual tolerance of proj_mode 1, 2 */

@@ -224,8 +224,9 @@ def test_consistent_projection_periodic_ends_match_oracle(top, mode, krylov, ora
     (rundir_unst/input.dat: type-6 z rows, scalar bottom type 5, periodic
     multigrid z ends MUDPACK_BC 0 0), with the pressure top specified
     (top 1) or, as in the shipped file, a derivative condition (top 2, no
-    specified side: the kx = 0 solve is singular and pinned at
-    (K = 0, J = NY+1)).  A full step of the small stratified case, FP64 and
+    specified side: the kx = 0 solve is singular; its solution is the one
+    with phi = 0 at (K = 0, J = NY+1) that satisfies every other row, which
+    the oracle pins and the GPU reaches through a compatible right-hand side).  A full step of the small stratified case, FP64 and
     FP32 cycles, against the oracle's FP64 consistent solve: velocities and
     scalar to 1e-8 relative L2, the pressure (which sums phi / H_BAR over the
     substeps) to 1e-6, divergence below 1e-10, every solve converged."""
