@@ -28,6 +28,9 @@ FP64-cycle form after the same steps.
 `cpu_baseline`: the CPU oracle (the reference algorithm, OpenMP over the
 host's cores) on one substep of the same workload, restarted from the FP64
 form's state so that the subgrid models run.
+`reference_run`: the reference run's own configuration (rundir_unst,
+256x127x319 with periodic multigrid z ends and no specified side), the plain
+replica and the consistent projection side by side (not part of `value`).
 """
 from __future__ import annotations
 
@@ -48,7 +51,7 @@ from ekman_bl_amd import roofline  # noqa: E402
 from ekman_bl_amd import solver as S  # noqa: E402
 from ekman_bl_amd.grid import tanh_grid  # noqa: E402
 from ekman_bl_amd.ic import ekman_initial  # noqa: E402
-from ekman_bl_amd.params import config  # noqa: E402
+from ekman_bl_amd.params import config, params_from_fixture  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 DIV_BOUND = 1e-10
@@ -167,6 +170,47 @@ def learned_net(p, path: str | None):
     return net, os.path.relpath(path, ROOT)
 
 
+def reference_run(steps: int, tol: float, krylov: int) -> dict:
+    """The reference run's own configuration (rundir_unst: input.dat and
+    grid_def, kept as fixtures in tests/golden; 256x127x319, periodic
+    multigrid z ends with no specified side): the plain replica and the
+    consistent projection (FP32 cycles), one warm-up step and `steps` timed
+    steps each from the seeded Ekman spiral at the run's velocity scale."""
+    g = os.path.join(ROOT, "tests", "golden")
+    p, xp, yp = params_from_fixture(os.path.join(g, "grid_yz_unst.json"), os.path.join(g, "input_unst.dat"))
+    ic = ekman_initial(p, yp, 1, 0.05, 0.01, ug=p.wbulk0)
+    tb = 1e-3 * yp  # background scalar (the run reads TH_BAR from its restart)
+    out = {"config": f"rundir_unst {p.nx}x{p.ny + 2}x{p.nz + 2}, MUDPACK_BC {list(p.mudpack_bc)}, "
+                     f"{p.npts} points; seeded Ekman spiral at WBULK0 scale, linear background scalar"}
+    for mode, key in ((0, "plain_fp64_mudpack"), (2, "consistent_fp32_cycles")):
+        q = p.copy()
+        q.proj_mode = mode
+        if mode:
+            q.proj_tol, q.proj_maxit, q.proj_krylov = tol, 50, krylov
+        s = S.DuctSolver(q, xp, yp, th_bar=tb)
+        s.set_initial_physical(*ic)
+        s.time_step(1)
+        s.sync()
+        cyc, dmax, unconv = 0, 0.0, 0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for _r in range(3):
+                s.rk_duct()
+                d = s.diag()
+                cyc += d.mg_cycles
+                dmax = max(dmax, d.div_max)
+                unconv += d.mg_unconverged
+        s.sync()
+        el = time.perf_counter() - t0
+        d = s.diag()
+        out[key] = {"ms_per_step": round(el / steps * 1e3, 3), "mpt_steps_per_s": round(p.npts * steps / el / 1e6, 3),
+                    "mg_cycles_per_step": round(cyc / steps, 2), "div_max_max": dmax,
+                    "meets_div_bound": bool(dmax < DIV_BOUND), "unconverged_solves": unconv, "status": d.status}
+        s.close()
+    out["speedup"] = round(out["plain_fp64_mudpack"]["ms_per_step"] / out["consistent_fp32_cycles"]["ms_per_step"], 3)
+    return out
+
+
 def cpu_baseline(cfg, xp, yp, state):
     """CPU oracle (clean-room FP64 restatement of the reference algorithm,
     OpenMP over the host cores): one substep of the same workload, restarted
@@ -207,6 +251,8 @@ def main() -> int:
                     help="warm-up steps of the compared forms (4: the subgrid models' first refresh, at step 5, "
                          "falls in the timed steps)")
     ap.add_argument("--learned", default=None, help="trained warm-start weights (default: ekman_bl_amd/data)")
+    ap.add_argument("--ref-run-steps", type=int, default=3,
+                    help="timed steps of the reference run's own configuration, both forms (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="run the plain copy kernel on this many MiB once (HBM counter calibration)")
@@ -335,6 +381,9 @@ def main() -> int:
         "forms": forms,
         "tuning": tuning_of(cfg, args.mode),
     }
+    if args.ref_run_steps > 0:
+        log("[bench] the reference run's own configuration (rundir_unst), both forms")
+        out["reference_run"] = reference_run(args.ref_run_steps, args.tol, args.krylov)
     if not args.no_cpu:
         log("[bench] CPU oracle baseline")
         st = ref_state

@@ -367,6 +367,26 @@ def read_input_dat(path: str, p: EkmParams) -> dict:
     return out
 
 
+def params_from_fixture(grid_json: str, input_dat: str):
+    """(params, xpoint, ypoint) of a run directory kept as a fixture pair: a
+    JSON file with grid_def's PARAMETER values under "grid_def" and the
+    GRID_YZ.dat coordinates as the tanh family's end points ("y", "z";
+    gamma 2.3, the shipped grid's family), and the run's input.dat."""
+    import json
+
+    from .grid import tanh_grid
+    d = json.load(open(grid_json))
+    g = d["grid_def"]
+    p = base_params(g["nx"], g["ny"], g["nz"], n_th=g.get("n_th", 0))
+    read_input_dat(input_dat, p)
+    p.mg_ixp, p.mg_iex, p.mg_jyq, p.mg_jey = g["iixp"], g["iiex"], g["jjyq"], g["jjey"]
+    for key in ("ny_s_t", "nz_s_r", "nz_s_l"):
+        if key in g:
+            setattr(p, key, g[key])
+    xp, yp = tanh_grid(p.ny, p.nz, d["y"][-1], d["z"][-1], 2.3)
+    return p, xp, yp
+
+
 def params_from_run_dir(grid_def: str, input_dat: str) -> EkmParams:
     g = read_grid_def(grid_def)
     p = base_params(g["nx"], g["ny"], g["nz"], n_th=g.get("n_th", 0))

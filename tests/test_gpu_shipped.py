@@ -15,16 +15,14 @@ with every solve converged, the singular kx = 0 solve included (its
 right-hand side made compatible with the operator's left null vector); the
 plain MUDPACK replica, the reference's algorithm, runs beside it and the
 distance between the two forms' fields is reported."""
-import json
 import os
 
 import numpy as np
 import pytest
 
 from ekman_bl_amd import solver as S
-from ekman_bl_amd.grid import tanh_grid
 from ekman_bl_amd.ic import ekman_initial
-from ekman_bl_amd.params import base_params, read_input_dat
+from ekman_bl_amd.params import params_from_fixture
 
 pytestmark = pytest.mark.gpu
 
@@ -32,14 +30,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def shipped_params():
-    d = json.load(open(os.path.join(GOLDEN, "grid_yz_unst.json")))
-    g = d["grid_def"]
-    p = base_params(g["nx"], g["ny"], g["nz"], n_th=g["n_th"])
-    read_input_dat(os.path.join(GOLDEN, "input_unst.dat"), p)
-    p.mg_ixp, p.mg_iex, p.mg_jyq, p.mg_jey = g["iixp"], g["iiex"], g["jjyq"], g["jjey"]
-    p.ny_s_t, p.nz_s_r, p.nz_s_l = g["ny_s_t"], g["nz_s_r"], g["nz_s_l"]
-    xp, yp = tanh_grid(p.ny, p.nz, d["y"][-1], d["z"][-1], 2.3)
-    return p, xp, yp
+    return params_from_fixture(os.path.join(GOLDEN, "grid_yz_unst.json"), os.path.join(GOLDEN, "input_unst.dat"))
 
 
 def test_shipped_configuration_consistent_projection():
