@@ -6,7 +6,7 @@
 // the start of each real solve of the consistent projection from its
 // right-hand side, the previous projection's solution of the same solve and
 // kx; the multigrid then iterates to the same tolerance as from zero, so it
-// stays both the fallback and the judge of the guess.
+// stays both the fallback and what decides when a solve is done.
 //
 // Network (one plane per sample, activations (H, W, C) with C fastest; the
 // plane's K (z) index is W, J (y) is H; H0 x W0 = (NY+2) x (NZ+2)):

@@ -7,7 +7,7 @@ The reference has no counterpart: MUDPACK starts every solve from zero
 (iguess = 0, phi = 0; src/mg_solver2d.F90:157,190, src/mud2cr.f:93,238).  With
 proj_guess = 2 the consistent projection starts each real solve from the
 network's guess and iterates to the same tolerance as from zero, so the
-multigrid stays the judge of the guess; the V-cycles saved are measured by
+multigrid decides when the solve is done, whatever the guess; the V-cycles saved are measured by
 comparing cycle counts (`eval_cycles`).  The network predicts a correction to
 the previous substep's solution scaled by the ratio of the two substeps'
 H_BAR (the start of proj_guess 1): a network that outputs zero is that start.
