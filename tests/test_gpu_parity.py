@@ -218,15 +218,18 @@ def test_consistent_projection_matches_oracle(mode, n, krylov, oracle_lib):
     s.close()
 
 
-@pytest.mark.parametrize("top,mode,krylov", [(1, 1, 0), (1, 2, 1), (2, 1, 0), (2, 2, 1)])
-def test_consistent_projection_periodic_ends_match_oracle(top, mode, krylov, oracle_lib):
+@pytest.mark.parametrize("top,mode,krylov,skew", [(1, 1, 0, False), (1, 2, 1, False), (2, 1, 0, False),
+                                                  (2, 2, 1, False), (1, 2, 1, True), (2, 1, 1, True)])
+def test_consistent_projection_periodic_ends_match_oracle(top, mode, krylov, skew, oracle_lib):
     """The consistent projection on the reference run's boundary set
     (rundir_unst/input.dat: type-6 z rows, scalar bottom type 5, periodic
     multigrid z ends MUDPACK_BC 0 0), with the pressure top specified
     (top 1) or, as in the shipped file, a derivative condition (top 2, no
     specified side: the kx = 0 solve is singular; its solution is the one
     with phi = 0 at (K = 0, J = NY+1) that satisfies every other row, which
-    the oracle pins and the GPU reaches through a compatible right-hand side).  A full step of the small stratified case, FP64 and
+    the oracle pins and the GPU reaches through a compatible right-hand side);
+    `skew` shears the grid so that the cross terms, which read the periodic
+    virtual columns diagonally, are active.  A full step of the small stratified case, FP64 and
     FP32 cycles, against the oracle's FP64 consistent solve: velocities and
     scalar to 1e-8 relative L2, the pressure (which sums phi / H_BAR over the
     substeps) to 1e-6, divergence below 1e-10, every solve converged."""
@@ -241,12 +244,14 @@ def test_consistent_projection_periodic_ends_match_oracle(top, mode, krylov, ora
     p.mudpack_bc[3] = top
     p.proj_mode, p.proj_tol, p.proj_maxit, p.proj_krylov = mode, 1e-12, 50, krylov
     xp, yp = tanh_grid(p.ny, p.nz, p.ly, p.lz, cfg.gamma)
+    if skew:
+        xp = xp + 0.05 * np.sin(0.3 * yp)
     ic = ekman_initial(p, yp, cfg.seed, cfg.noise, cfg.th_noise)
     po = p.copy()
     po.proj_mode, po.proj_krylov = 1, 0
     o = O.Oracle(po, xp, yp)
     s = S.DuctSolver(p, xp, yp)
-    print(f"[periodic] top {top} mode {mode} krylov {krylov}", flush=True)
+    print(f"[periodic] top {top} mode {mode} krylov {krylov} skew {skew}", flush=True)
     o.set_initial_physical(*ic)
     s.set_initial_physical(*ic)
     for sub in range(3):
