@@ -33,7 +33,7 @@ CACHE = os.path.join(HERE, "data", "tuning.json")
 SPACE = {
     "adi_rows": (4, 8, 12),
     "adi_block": (64, 128, 256),
-    "adi_les_rows": (4, 6, 8),
+    "adi_les_rows": (4, 5, 6, 8),
     "mg_block": (256, 512, 1024),
     "tight_lds_kib": (0, 96, 48),
 }

@@ -484,6 +484,9 @@ __global__ void __launch_bounds__(256) k_ysweep_les(Dims d, Metrics M, const Adi
                                        : 0.0;
   double g = 0.0, pvp = 1.0, cp = 0.0;
   double nm = 0.0, nc = nut[q0];  // NU_T at row-1 and row
+  // G22 and the subgrid metric at row: the row+1 values of the row before
+  // (the rows js..je are consecutive)
+  double g0c = G22(K, r.js, 1), l0c = gy[mf(d, K, r.js, 1)];
   for (int row0 = 0; row0 <= N; row0 += kSwL) {
     double rr[kSwL], nn[kSwL];
 #pragma unroll
@@ -499,12 +502,15 @@ __global__ void __launch_bounds__(256) k_ysweep_les(Dims d, Metrics M, const Adi
       const double np = nn[u];
       double a = 0.0, b = 1.0, c = 0.0, rhs = rr[u];
       if (row >= r.js && row <= r.je) {
-        const double gp = G22(K, row + 1, 1), g0 = G22(K, row, 1);
+        const double gp = G22(K, row + 1, 1), g0 = g0c;
+        const double lp = gy[mf(d, K, row + 1, 1)];
         a = -(t1 * g0);
         b = JAC(K, row) + t1 * (gp + g0);
         c = -(t1 * gp);
-        const double wm = t2h * (nc + nm) * gy[mf(d, K, row, 1)];
-        const double wp = t2h * (nc + np) * gy[mf(d, K, row + 1, 1)];
+        const double wm = t2h * (nc + nm) * l0c;
+        const double wp = t2h * (nc + np) * lp;
+        g0c = gp;
+        l0c = lp;
         a = a - wm;
         b = b + wp + wm;
         c = c - wp;
@@ -535,8 +541,12 @@ __global__ void __launch_bounds__(256) k_ysweep_les(Dims d, Metrics M, const Adi
   double x2 = 0.0, x1 = g / pvp;  // x1 = x(row+1), x2 = x(row+2)
   U[q0 + dj * N] = x1;
   double n1 = nut[q0 + dj * N], n2 = 0.0;  // NU_T at row+1, row+2
+  // G22 and the subgrid metric at row+2 are the row+1 values of the row
+  // before; only the first row of the backward pass loads them
+  const bool in_n = N >= r.js && N <= r.je;
+  double gpc = in_n ? G22(K, N + 1, 1) : 0.0, lpc = in_n ? gy[mf(d, K, N + 1, 1)] : 0.0;
   for (int row0 = N - 1; row0 >= 0; row0 -= kSwL) {
-    double gg[kSwL], pp[kSwL], nl[kSwL], ss[kSwL], jc[kSwL], gp[kSwL], g0[kSwL], lp[kSwL], l0[kSwL];
+    double gg[kSwL], pp[kSwL], nl[kSwL], ss[kSwL], jc[kSwL], g0[kSwL], l0[kSwL];
 #pragma unroll
     for (int u = 0; u < kSwL; ++u) {
       const int row = row0 - u, jr = row + 1;
@@ -547,11 +557,19 @@ __global__ void __launch_bounds__(256) k_ysweep_les(Dims d, Metrics M, const Adi
       nl[u] = ok ? nut[q0 + dj * row] : 0.0;
       ss[u] = in ? S[q0 + dj * jr] : 0.0;
       jc[u] = in ? JAC(K, jr) : 0.0;
-      gp[u] = in ? G22(K, jr + 1, 1) : 0.0;
       g0[u] = ok ? G22(K, jr, 1) : 0.0;
-      lp[u] = in ? gy[mf(d, K, jr + 1, 1)] : 0.0;
       l0[u] = ok ? gy[mf(d, K, jr, 1)] : 0.0;
     }
+    double gp[kSwL], lp[kSwL];
+    gp[0] = gpc;
+    lp[0] = lpc;
+#pragma unroll
+    for (int u = 1; u < kSwL; ++u) {
+      gp[u] = g0[u - 1];
+      lp[u] = l0[u - 1];
+    }
+    gpc = g0[kSwL - 1];
+    lpc = l0[kSwL - 1];
 #pragma unroll
     for (int u = 0; u < kSwL; ++u) {
       const int row = row0 - u;
@@ -1481,6 +1499,7 @@ int launch_adi_batch(const Dims& d, const Metrics& M, const AdiField* fs, const 
     const dim3 g(nblk(nys, 256), nf), b(256);
     switch (tn.adi_les_rows) {
       case 4: hipLaunchKernelGGL(k_ysweep_les<4>, g, b, 0, st, d, M, fs, t1); break;
+      case 5: hipLaunchKernelGGL(k_ysweep_les<5>, g, b, 0, st, d, M, fs, t1); break;
       case 8: hipLaunchKernelGGL(k_ysweep_les<8>, g, b, 0, st, d, M, fs, t1); break;
       default: hipLaunchKernelGGL(k_ysweep_les<6>, g, b, 0, st, d, M, fs, t1); break;
     }
@@ -1499,6 +1518,7 @@ int launch_adi_batch(const Dims& d, const Metrics& M, const AdiField* fs, const 
     const dim3 g(nblk(nzs, 256), nf), b(256);
     switch (tn.adi_les_rows) {
       case 4: hipLaunchKernelGGL(k_zsweep_les<4>, g, b, 0, st, d, M, fs, t1); break;
+      case 5: hipLaunchKernelGGL(k_zsweep_les<5>, g, b, 0, st, d, M, fs, t1); break;
       case 8: hipLaunchKernelGGL(k_zsweep_les<8>, g, b, 0, st, d, M, fs, t1); break;
       default: hipLaunchKernelGGL(k_zsweep_les<6>, g, b, 0, st, d, M, fs, t1); break;
     }

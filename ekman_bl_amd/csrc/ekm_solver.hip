@@ -116,7 +116,7 @@ void ekm_tuning_default(ekm_tuning* t) {
 static int check_tuning(const ekm_tuning& t) {
   if (t.adi_rows != 4 && t.adi_rows != 8 && t.adi_rows != 12) return EKM_ERR_ARG;
   if (t.adi_block != 64 && t.adi_block != 128 && t.adi_block != 256) return EKM_ERR_ARG;
-  if (t.adi_les_rows != 4 && t.adi_les_rows != 6 && t.adi_les_rows != 8) return EKM_ERR_ARG;
+  if (t.adi_les_rows != 4 && t.adi_les_rows != 5 && t.adi_les_rows != 6 && t.adi_les_rows != 8) return EKM_ERR_ARG;
   if (t.mg_block != 256 && t.mg_block != 512 && t.mg_block != 1024) return EKM_ERR_ARG;
   if (t.tight_lds_kib < 0) return EKM_ERR_ARG;
   return EKM_OK;

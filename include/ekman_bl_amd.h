@@ -141,7 +141,7 @@ typedef struct ekm_solver ekm_solver;  /* opaque: device-resident duct state */
 typedef struct ekm_tuning {
   int adi_rows;       /* rows per batch of the plain ADI sweeps: 4, 8 or 12 (default 8) */
   int adi_block;      /* threads per workgroup of the plain ADI sweeps: 64, 128 or 256 (default 256) */
-  int adi_les_rows;   /* rows per batch of the ADI sweeps with a subgrid viscosity: 4, 6 or 8 (default 6) */
+  int adi_les_rows;   /* rows per batch of the ADI sweeps with a subgrid viscosity: 4, 5, 6 or 8 (default 6) */
   int mg_block;       /* threads per workgroup of the MUDPACK replica: 256, 512 or 1024 (default 512) */
   int tight_lds_kib;  /* LDS cap of the consistent projection's coarse levels in KiB (0: the device's maximum) */
 } ekm_tuning;
