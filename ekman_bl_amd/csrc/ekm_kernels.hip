@@ -620,6 +620,9 @@ __global__ void __launch_bounds__(256) k_zsweep_les(Dims d, Metrics M, const Adi
   double g = 0.0, pvp = 1.0, cp = 0.0;
   double ap = 0.0, bp = 1.0, rp = 0.0;  // previous row before elimination (THOMAS_REAL_SP)
   double nm = 0.0, nc = nut[q0];
+  // G11 and the subgrid metric at row: the row+1 values of the row before
+  // (the rows zs..ze are consecutive)
+  double g0c = G11(r.zs, J, 2), l0c = gz[mf(d, r.zs, J, 2)];
   for (int row0 = 0; row0 <= N; row0 += kSwL) {
     double rr[kSwL], nn[kSwL];
 #pragma unroll
@@ -635,12 +638,15 @@ __global__ void __launch_bounds__(256) k_zsweep_les(Dims d, Metrics M, const Adi
       const double np = nn[u];
       double a = 0.0, b = 1.0, c = 0.0, rhs = rr[u];
       if (row >= r.zs && row <= r.ze) {
-        const double gp = G11(row + 1, J, 2), g0 = G11(row, J, 2);
+        const double gp = G11(row + 1, J, 2), g0 = g0c;
+        const double lp = gz[mf(d, row + 1, J, 2)];
         a = -(t1 * g0);
         b = JAC(row, J) + t1 * (gp + g0);
         c = -(t1 * gp);
-        const double wm = t2h * (nc + nm) * gz[mf(d, row, J, 2)];
-        const double wp = t2h * (np + nc) * gz[mf(d, row + 1, J, 2)];
+        const double wm = t2h * (nc + nm) * l0c;
+        const double wp = t2h * (np + nc) * lp;
+        g0c = gp;
+        l0c = lp;
         a = a - wm;
         b = b + wp + wm;
         c = c - wp;

@@ -478,12 +478,16 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
     }
   }
   __syncthreads();
-  // backward: x_j = alpha_j + beta_j x_out within the chunk
+  // backward: x_j = alpha_j + beta_j x_out within the chunk.  The first
+  // chunk enters with dp = 0 (its dp_j are the B_j), and the last chunk
+  // leaves with x_out = 0, so its alpha_j are the solution and go straight
+  // to e (the last pass skips that chunk).
   for (int task = wave; task < ntask; task += nw) {
     const int grp = task / nch, ch = task - grp * nch;
     const int col = grp * 64 + lane;
     const int i = par + 2 * col;
     const int j0 = ch * cl, j1 = min(j0 + cl, ny);
+    const bool first = ch == 0, last = ch == nch - 1;
     const T din = sv[task * 64 + lane];
     T al = (T)0, be = (T)1;
     if (i < nx) {
@@ -493,9 +497,10 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
 #pragma unroll
         for (int u = 0; u < kYBack; ++u) {
           const int j = jb - u;
+          a_[u] = (T)0;
           if (j >= j0) {
             const int ps = pb + col + ncol * j;
-            a_[u] = yA[ps];
+            if (!first) a_[u] = yA[ps];
             b_[u] = tB[ps];
             c_[u] = fc[ps];
           }
@@ -506,7 +511,13 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
           if (j >= j0) {
             const T dp = a_[u] * din + b_[u];
             al = dp - c_[u] * al;
-            tB[pb + col + ncol * j] = al;
+            if (last) {
+              const int q = (i + 1) + ld * (j + 1);
+              e[q] = al;
+              if constexpr (PER) ring_store(e, q, i, nx, al);
+            } else {
+              tB[pb + col + ncol * j] = al;
+            }
           }
         }
       }
@@ -529,7 +540,7 @@ __device__ void ylines(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict__ 
     const int grp = task / nch, ch = task - grp * nch;
     const int col = grp * 64 + lane;
     const int i = par + 2 * col;
-    if (i >= nx) continue;
+    if (i >= nx || ch == nch - 1) continue;
     const int j0 = ch * cl, j1 = min(j0 + cl, ny);
     const T xo = sv[task * 64 + lane];
     for (int jb = j0; jb < j1; jb += kYApply) {
