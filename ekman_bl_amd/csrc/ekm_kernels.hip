@@ -12,6 +12,7 @@
 // i, so they are computed once per line by k_lu and shared; with it every
 // point's system is formed inside its sweep.
 #include "ekm_kernels.h"
+#include <algorithm>
 
 namespace {
 
@@ -1309,24 +1310,38 @@ __global__ void __launch_bounds__(256) k_xmoments(Dims d, const double* __restri
 }
 
 // divergence diagnostic (:2790-2800): sum and max of |div| over
-// J = 2..NY-1, K = 2..NZ-1, I = 1..NX+1 of the physical field
-__global__ void k_div_reduce(Dims d, const double* __restrict__ S, double* __restrict__ out) {
-  int nk = d.NZ - 2, nj = d.NY - 2, ni = d.NX + 1;
-  size_t n = (size_t)ni * nk * nj;
+// J = 2..NY-1, K = 2..NZ-1, I = 1..NX+1 of the physical field.  A block takes
+// one (K, J) line at a time (x contiguous, so the loads are coalesced), each
+// thread issues the loads of kDivLd points of the line before using any, and
+// the line index is 32-bit (no 64-bit division per point).
+constexpr int kDivBlock = 256, kDivLd = 4;
+__global__ void __launch_bounds__(kDivBlock) k_div_reduce(Dims d, const double* __restrict__ S,
+                                                          double* __restrict__ out) {
+  const int nk = d.NZ - 2, nj = d.NY - 2, ni = d.NX + 1;
+  const int nl = nk * nj;
   double s = 0.0, mx = 0.0;
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-    int i = 1 + (int)(t % ni);
-    size_t rr = t / ni;
-    int k = 2 + (int)(rr % nk), j = 2 + (int)(rr / nk);
-    double v = fabs(S[ph(d, i, k, j)]);
-    s += v;
-    mx = fmax(mx, v);
+  for (int l = blockIdx.x; l < nl; l += gridDim.x) {
+    const int k = 2 + l % nk, j = 2 + l / nk;
+    const double* __restrict__ row = S + ph(d, 0, k, j);
+    for (int i0 = 1; i0 <= ni; i0 += kDivBlock * kDivLd) {
+      double v[kDivLd];
+#pragma unroll
+      for (int u = 0; u < kDivLd; ++u) {
+        const int i = i0 + (int)threadIdx.x + kDivBlock * u;
+        v[u] = i <= ni ? fabs(row[i]) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kDivLd; ++u) {
+        s += v[u];
+        mx = fmax(mx, v[u]);
+      }
+    }
   }
-  __shared__ double rs[256], rm[256];
+  __shared__ double rs[kDivBlock], rm[kDivBlock];
   rs[threadIdx.x] = s;
   rm[threadIdx.x] = mx;
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+  for (int w = kDivBlock / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       rs[threadIdx.x] += rs[threadIdx.x + w];
       rm[threadIdx.x] = fmax(rm[threadIdx.x], rm[threadIdx.x + w]);
@@ -1638,7 +1653,9 @@ int launch_xmoments(const Dims& d, const double* U1p, const double* U2p, const d
   return EKM_OK;
 }
 int launch_div_reduce(const Dims& d, const double* S, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_div_reduce, dim3(512), dim3(256), 0, st, d, S, out);
+  if (d.NZ < 3 || d.NY < 3) return EKM_OK;
+  const int nl = (d.NZ - 2) * (d.NY - 2);
+  hipLaunchKernelGGL(k_div_reduce, dim3((unsigned)std::min(nl, 4096)), dim3(kDivBlock), 0, st, d, S, out);
   EKM_CHECK(hipGetLastError());
   return EKM_OK;
 }
