@@ -326,13 +326,29 @@ __global__ void __launch_bounds__(256) k_ysweep(Dims d, Metrics M, const AdiFiel
              : bc.r[e].kind == BC_WALL ? F.wall[(size_t)bc.r[e].src * dj + (size_t)K * d.NX2 + i]
                                        : 0.0;
   double g = 0.0;
+  // the loads of the next batch of rows are issued before this batch is
+  // eliminated
+  double rn[kSw], mn[kSw];
+#pragma unroll
+  for (int u = 0; u < kSw; ++u) {
+    const int row = u;
+    rn[u] = (row <= N && row >= r.js && row <= r.je) ? R[q0 + dj * row] : 0.0;
+    mn[u] = row <= N ? mult[base + row] : 0.0;
+  }
   for (int row0 = 0; row0 <= N; row0 += kSw) {
     double rr[kSw], mm[kSw];
 #pragma unroll
     for (int u = 0; u < kSw; ++u) {
-      const int row = row0 + u;
-      rr[u] = (row <= N && row >= r.js && row <= r.je) ? R[q0 + dj * row] : 0.0;
-      mm[u] = row <= N ? mult[base + row] : 0.0;
+      rr[u] = rn[u];
+      mm[u] = mn[u];
+    }
+    if (row0 + kSw <= N) {
+#pragma unroll
+      for (int u = 0; u < kSw; ++u) {
+        const int row = row0 + kSw + u;
+        rn[u] = (row <= N && row >= r.js && row <= r.je) ? R[q0 + dj * row] : 0.0;
+        mn[u] = row <= N ? mult[base + row] : 0.0;
+      }
     }
 #pragma unroll
     for (int u = 0; u < kSw; ++u) {
@@ -402,13 +418,29 @@ __global__ void __launch_bounds__(256) k_zsweep(Dims d, Metrics M, const AdiFiel
   double lag[4];
   for (int e = 0; e < 4; ++e) lag[e] = (bc.r[e].kind == BC_LAGGED) ? U[q0 + dk * bc.r[e].src] : 0.0;
   double g = 0.0, rprev = 0.0;
+  // the loads of the next batch of rows are issued before this batch is
+  // eliminated
+  double rn[kSw], mn[kSw];
+#pragma unroll
+  for (int u = 0; u < kSw; ++u) {
+    const int row = u;
+    rn[u] = (row <= N && row >= r.zs && row <= r.ze) ? R[q0 + dk * row] : 0.0;
+    mn[u] = row <= N ? mult[base + row] : 0.0;
+  }
   for (int row0 = 0; row0 <= N; row0 += kSw) {
     double rr[kSw], mm[kSw];
 #pragma unroll
     for (int u = 0; u < kSw; ++u) {
-      const int row = row0 + u;
-      rr[u] = (row <= N && row >= r.zs && row <= r.ze) ? R[q0 + dk * row] : 0.0;
-      mm[u] = row <= N ? mult[base + row] : 0.0;
+      rr[u] = rn[u];
+      mm[u] = mn[u];
+    }
+    if (row0 + kSw <= N) {
+#pragma unroll
+      for (int u = 0; u < kSw; ++u) {
+        const int row = row0 + kSw + u;
+        rn[u] = (row <= N && row >= r.zs && row <= r.ze) ? R[q0 + dk * row] : 0.0;
+        mn[u] = row <= N ? mult[base + row] : 0.0;
+      }
     }
 #pragma unroll
     for (int u = 0; u < kSw; ++u) {
@@ -624,13 +656,29 @@ __global__ void __launch_bounds__(256) k_zsweep_les(Dims d, Metrics M, const Adi
   // G11 and the subgrid metric at row: the row+1 values of the row before
   // (the rows zs..ze are consecutive)
   double g0c = G11(r.zs, J, 2), l0c = gz[mf(d, r.zs, J, 2)];
+  // the loads of the next batch of rows are issued before this batch is
+  // eliminated
+  double rn[kSwL], nx_[kSwL];
+#pragma unroll
+  for (int u = 0; u < kSwL; ++u) {
+    const int row = u;
+    rn[u] = (row <= N && row >= r.zs && row <= r.ze) ? R[q0 + dk * row] : 0.0;
+    nx_[u] = row + 1 <= N ? nut[q0 + dk * (row + 1)] : 0.0;
+  }
   for (int row0 = 0; row0 <= N; row0 += kSwL) {
     double rr[kSwL], nn[kSwL];
 #pragma unroll
     for (int u = 0; u < kSwL; ++u) {
-      const int row = row0 + u;
-      rr[u] = (row <= N && row >= r.zs && row <= r.ze) ? R[q0 + dk * row] : 0.0;
-      nn[u] = row + 1 <= N ? nut[q0 + dk * (row + 1)] : 0.0;
+      rr[u] = rn[u];
+      nn[u] = nx_[u];
+    }
+    if (row0 + kSwL <= N) {
+#pragma unroll
+      for (int u = 0; u < kSwL; ++u) {
+        const int row = row0 + kSwL + u;
+        rn[u] = (row <= N && row >= r.zs && row <= r.ze) ? R[q0 + dk * row] : 0.0;
+        nx_[u] = row + 1 <= N ? nut[q0 + dk * (row + 1)] : 0.0;
+      }
     }
 #pragma unroll
     for (int u = 0; u < kSwL; ++u) {
