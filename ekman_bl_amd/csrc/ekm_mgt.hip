@@ -392,7 +392,7 @@ __host__ __device__ inline YChunks ychunks(int nx, int ny, int par, int nw) {
   c.cl = (ny + c.nch - 1) / c.nch;
   return c;
 }
-constexpr int kYUnroll = 3;
+constexpr int kYUnroll = 4;
 constexpr int kYApply = 8;  // rows per batch of the last pass
 constexpr int kYBack = 6;   // rows per batch of the backward pass
 
