@@ -252,15 +252,33 @@ __device__ inline void zline_point(const T* __restrict__ e, const T* __restrict_
   if (i < nx) {
     const int p = i + nx * j;
     const int q = (i + 1) + ld * (j + 1);
-    T s = r[p];
+    // every load of the point is issued before any is used
+    const T rv = r[p], av = fa[p], fv = f1[p];
+    T c2 = (T)0, c6 = (T)0, eu = (T)0, ed = (T)0;
+    T c1 = (T)0, c3 = (T)0, c5 = (T)0, c7 = (T)0, e1 = (T)0, e3 = (T)0, e5 = (T)0, e7 = (T)0;
     if (!ZERO) {
-      s -= cf[2 * n + p] * e[q + ld] + cf[6 * n + p] * e[q - ld];
-      if (X)
-        s -= (cf[1 * n + p] * e[q + 1 + ld] + cf[3 * n + p] * e[q - 1 + ld]) +
-             (cf[5 * n + p] * e[q - 1 - ld] + cf[7 * n + p] * e[q + 1 - ld]);
+      c2 = cf[2 * n + p];
+      c6 = cf[6 * n + p];
+      eu = e[q + ld];
+      ed = e[q - ld];
+      if (X) {
+        c1 = cf[1 * n + p];
+        c3 = cf[3 * n + p];
+        c5 = cf[5 * n + p];
+        c7 = cf[7 * n + p];
+        e1 = e[q + 1 + ld];
+        e3 = e[q - 1 + ld];
+        e5 = e[q - 1 - ld];
+        e7 = e[q + 1 - ld];
+      }
     }
-    a = fa[p];
-    b = s * f1[p];
+    T s = rv;
+    if (!ZERO) {
+      s -= c2 * eu + c6 * ed;
+      if (X) s -= (c1 * e1 + c3 * e3) + (c5 * e5 + c7 * e7);
+    }
+    a = av;
+    b = s * fv;
   }
 }
 
