@@ -627,12 +627,14 @@ __device__ void residual(const TLev<T>& L, const Ws<T>& Wp, const T* __restrict_
       const int p = min(p0 + u * nt, n - 1);
       const int j = p / nx, i = p - j * nx;
       const int q = (i + 1) + ld * (j + 1);
-      T acc = (cf[0 * n + p] * e[q + 1] + cf[2 * n + p] * e[q + ld]) + (cf[4 * n + p] * e[q - 1] + cf[6 * n + p] * e[q - ld]);
+      const T k0 = cf[0 * n + p], k2 = cf[2 * n + p], k4 = cf[4 * n + p], k6 = cf[6 * n + p], k9 = c9[p];
+      const T e0 = e[q + 1], e2 = e[q + ld], e4 = e[q - 1], e6 = e[q - ld], e9 = e[q], rv = r[p];
+      T acc = (k0 * e0 + k2 * e2) + (k4 * e4 + k6 * e6);
       if (X)
         acc += (cf[1 * n + p] * e[q + 1 + ld] + cf[3 * n + p] * e[q - 1 + ld]) +
                (cf[5 * n + p] * e[q - 1 - ld] + cf[7 * n + p] * e[q + 1 - ld]);
-      acc += c9[p] * e[q];
-      v[u] = r[p] - acc;
+      acc += k9 * e9;
+      v[u] = rv - acc;
     }
 #pragma unroll
     for (int u = 0; u < kEwR; ++u)
