@@ -661,7 +661,9 @@ __device__ void restrict_fw(const TLev<T>& F, const TLev<T>& C, const Ws<T>& W) 
       const int pc = min(p0 + u * nt, C.n - 1);
       const int jc = pc / C.nx, ic = pc - jc * C.nx;
       const int i = sx * ic, j = sy * jc;
-      T acc = (T)0;
+      // the nine loads are issued before the weighted sum uses any
+      T x[9];
+      bool in[9];
 #pragma unroll
       for (int b = -1; b <= 1; ++b)
 #pragma unroll
@@ -670,11 +672,20 @@ __device__ void restrict_fw(const TLev<T>& F, const TLev<T>& C, const Ws<T>& W) 
           const int jj = j + b;
           // periodic ends: the virtual columns -1 -> nx-2 and nx -> 1
           if constexpr (PER) ii = ii < 0 ? F.nx - 2 : ii >= F.nx ? 1 : ii;
-          const bool in = ii >= 0 && ii < F.nx && jj >= 0 && jj < F.ny;
-          const T x = res[in ? ii + F.nx * jj : 0];
-          acc += in ? (w[a + 1] * w[b + 1]) * x : (T)0;
+          const int k = 3 * (b + 1) + (a + 1);
+          in[k] = ii >= 0 && ii < F.nx && jj >= 0 && jj < F.ny;
+          x[k] = res[in[k] ? ii + F.nx * jj : 0];
         }
-      v[u] = C.spec[pc] ? (T)0 : acc;
+      const bool sp = C.spec[pc];
+      T acc = (T)0;
+#pragma unroll
+      for (int b = -1; b <= 1; ++b)
+#pragma unroll
+        for (int a = -1; a <= 1; ++a) {
+          const int k = 3 * (b + 1) + (a + 1);
+          acc += in[k] ? (w[a + 1] * w[b + 1]) * x[k] : (T)0;
+        }
+      v[u] = sp ? (T)0 : acc;
     }
 #pragma unroll
     for (int u = 0; u < kEwR; ++u)
@@ -708,9 +719,13 @@ __device__ void prolong_add(const TLev<T>& C, const TLev<T>& F, const Ws<T>& W) 
       pf[u] = p;
       const int i0 = i / sx, i1 = (i + sx - 1) / sx, j0 = j / sy, j1 = (j + sy - 1) / sy;
       qf[u] = (i + 1) + lf * (j + 1);
-      const T x = (T)0.25 * ((ec[(i0 + 1) + lc * (j0 + 1)] + ec[(i1 + 1) + lc * (j0 + 1)]) +
-                             (ec[(i0 + 1) + lc * (j1 + 1)] + ec[(i1 + 1) + lc * (j1 + 1)]));
-      v[u] = F.spec[p] ? (T)0 : x + ef[qf[u]];
+      // the six loads are issued before any is used
+      const T c00 = ec[(i0 + 1) + lc * (j0 + 1)], c10 = ec[(i1 + 1) + lc * (j0 + 1)];
+      const T c01 = ec[(i0 + 1) + lc * (j1 + 1)], c11 = ec[(i1 + 1) + lc * (j1 + 1)];
+      const T fo = ef[qf[u]];
+      const bool sp = F.spec[p];
+      const T x = (T)0.25 * ((c00 + c10) + (c01 + c11));
+      v[u] = sp ? (T)0 : x + fo;
     }
 #pragma unroll
     for (int u = 0; u < kEw; ++u)
